@@ -1,0 +1,305 @@
+"""CRTM binary coefficient files: Fortran sequential-unformatted records.
+
+Writers and readers for the on-disk formats the native loader
+(crtmv3_amd/csrc/coeff_load.cpp) consumes, so synthetic coefficient sets can
+be produced offline in exactly the byte layout the reference reads.
+
+Record framing follows gfortran/flang sequential unformatted I/O: every
+record is ``int32 nbytes | payload | int32 nbytes`` little-endian.  The first
+record of every file is the magic number 123456789
+(reference: src/Utility/Binary_File_Utility.f90:56, 241).
+
+Formats:
+  * SpcCoeff, release 8 / version 3
+    (src/Coefficients/SpcCoeff/SpcCoeff_Binary_IO.f90:374-470,
+     type at src/Coefficients/SpcCoeff/SpcCoeff_Define.f90:163-189)
+  * ODPS TauCoeff, release 2
+    (src/Coefficients/TauCoeff/ODPS/ODPS_Binary_IO.f90:617-800,
+     type at src/Coefficients/TauCoeff/ODPS/ODPS_Define.f90:109-205)
+"""
+from __future__ import annotations
+
+import dataclasses
+import struct
+from typing import List, Optional
+
+import numpy as np
+
+MAGIC_NUMBER = 123456789
+SENSOR_ID_LEN = 20           # CHARACTER(SL), SL = 20
+SPCCOEFF_RELEASE = 8
+SPCCOEFF_VERSION = 3
+ODPS_RELEASE = 2
+ODPS_VERSION = 1
+ODPS_ALGORITHM = 2           # src/CRTM_Parameters.f90:128
+N_PREDICTOR_USED_OPTRAN = 6  # ODPS_Define.f90:103
+
+MICROWAVE_SENSOR = 1
+INFRARED_SENSOR = 2
+VISIBLE_SENSOR = 3
+ULTRAVIOLET_SENSOR = 4
+
+
+class RecordWriter:
+    def __init__(self, path: str):
+        self._f = open(path, "wb")
+        self.record(np.int32(MAGIC_NUMBER))
+
+    def record(self, *items) -> None:
+        payload = b"".join(_to_bytes(x) for x in items)
+        n = struct.pack("<i", len(payload))
+        self._f.write(n + payload + n)
+
+    def close(self) -> None:
+        self._f.close()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+def _to_bytes(x) -> bytes:
+    if isinstance(x, bytes):
+        return x
+    if isinstance(x, str):
+        return x.encode("ascii").ljust(SENSOR_ID_LEN)[:SENSOR_ID_LEN]
+    a = np.asarray(x)
+    if a.dtype.kind == "i":
+        a = a.astype("<i4")
+    elif a.dtype == np.float32:
+        a = a.astype("<f4")
+    elif a.dtype.kind == "f":
+        a = a.astype("<f8")
+    # Fortran arrays are column-major
+    return np.asfortranarray(a).tobytes(order="F")
+
+
+class RecordReader:
+    def __init__(self, path: str):
+        with open(path, "rb") as f:
+            self._buf = f.read()
+        self._pos = 0
+        magic = self.read_ints(1)[0]
+        if magic != MAGIC_NUMBER:
+            raise ValueError(f"{path}: bad magic number {magic}")
+
+    def _next(self) -> bytes:
+        (n,) = struct.unpack_from("<i", self._buf, self._pos)
+        p = self._pos + 4
+        payload = self._buf[p:p + n]
+        (n2,) = struct.unpack_from("<i", self._buf, p + n)
+        if n2 != n:
+            raise ValueError("corrupt record framing")
+        self._pos = p + n + 4
+        return payload
+
+    def read(self, fmt: List[tuple]):
+        """fmt: list of (kind, shape) with kind in 'i4','f4','f8','s'."""
+        payload = self._next()
+        out, off = [], 0
+        for kind, shape in fmt:
+            if kind == "s":
+                out.append(payload[off:off + SENSOR_ID_LEN].decode("ascii").rstrip())
+                off += SENSOR_ID_LEN
+                continue
+            dt = np.dtype("<" + kind)
+            cnt = int(np.prod(shape)) if shape else 1
+            a = np.frombuffer(payload, dt, cnt, off)
+            off += cnt * dt.itemsize
+            out.append(a.reshape(shape, order="F") if shape else a[0])
+        if off != len(payload):
+            raise ValueError(f"record length mismatch {off} != {len(payload)}")
+        return out
+
+    def read_ints(self, n):
+        payload = self._next()
+        return np.frombuffer(payload, "<i4", n)
+
+
+# --------------------------------------------------------------------------
+# SpcCoeff
+# --------------------------------------------------------------------------
+@dataclasses.dataclass
+class SpcCoeff:
+    sensor_id: str
+    sensor_type: int
+    wmo_satellite_id: int
+    wmo_sensor_id: int
+    sensor_channel: np.ndarray          # int32 [L]
+    polarization: np.ndarray            # int32 [L]
+    pol_angle: np.ndarray               # f8 [L]
+    channel_flag: np.ndarray            # int32 [L]
+    frequency: np.ndarray               # GHz [L]
+    wavenumber: np.ndarray              # cm^-1 [L]
+    planck_c1: np.ndarray
+    planck_c2: np.ndarray
+    band_c1: np.ndarray
+    band_c2: np.ndarray
+    cosmic_background_radiance: np.ndarray
+    solar_irradiance: np.ndarray
+
+    @property
+    def n_channels(self) -> int:
+        return int(self.sensor_channel.size)
+
+
+def write_spccoeff(path: str, sc: SpcCoeff) -> None:
+    """Record order of SpcCoeff_Binary_IO.f90:374-470 (Version > 2)."""
+    with RecordWriter(path) as w:
+        w.record(np.int32(SPCCOEFF_RELEASE), np.int32(SPCCOEFF_VERSION))
+        w.record(np.int32(sc.n_channels))
+        w.record(sc.sensor_id, np.int32(sc.sensor_type),
+                 np.int32(sc.wmo_satellite_id), np.int32(sc.wmo_sensor_id))
+        w.record(np.asarray(sc.sensor_channel, np.int32),
+                 np.asarray(sc.polarization, np.int32),
+                 np.asarray(sc.pol_angle, np.float64),
+                 np.asarray(sc.channel_flag, np.int32),
+                 np.asarray(sc.frequency, np.float64),
+                 np.asarray(sc.wavenumber, np.float64),
+                 np.asarray(sc.planck_c1, np.float64),
+                 np.asarray(sc.planck_c2, np.float64),
+                 np.asarray(sc.band_c1, np.float64),
+                 np.asarray(sc.band_c2, np.float64),
+                 np.asarray(sc.cosmic_background_radiance, np.float64),
+                 np.asarray(sc.solar_irradiance, np.float64))
+        w.record(np.int32(0))   # antenna-correction data not present
+        w.record(np.int32(0))   # NLTE-correction data not present
+
+
+def read_spccoeff(path: str) -> SpcCoeff:
+    r = RecordReader(path)
+    rel, ver = r.read([("i4", ()), ("i4", ())])
+    if rel != SPCCOEFF_RELEASE:
+        raise ValueError(f"SpcCoeff release {rel} unsupported")
+    (n,) = r.read([("i4", ())])
+    sid, stype, wsat, wsen = r.read([("s", None), ("i4", ()), ("i4", ()), ("i4", ())])
+    if ver > 2:
+        a = r.read([("i4", (n,)), ("i4", (n,)), ("f8", (n,)), ("i4", (n,))] + [("f8", (n,))] * 8)
+        ch, pol, pang, flag = a[:4]
+        rest = a[4:]
+    else:
+        a = r.read([("i4", (n,)), ("i4", (n,)), ("i4", (n,))] + [("f8", (n,))] * 8)
+        ch, pol, flag = a[:3]
+        pang = np.zeros(n)
+        rest = a[3:]
+    return SpcCoeff(sid, int(stype), int(wsat), int(wsen), ch.copy(), pol.copy(), pang.copy(),
+                    flag.copy(), *[x.copy() for x in rest])
+
+
+# --------------------------------------------------------------------------
+# ODPS TauCoeff
+# --------------------------------------------------------------------------
+@dataclasses.dataclass
+class ODPS:
+    group_index: int
+    sensor_id: str
+    wmo_satellite_id: int
+    wmo_sensor_id: int
+    sensor_type: int
+    sensor_channel: np.ndarray      # int32 [L]
+    component_id: np.ndarray        # int32 [J]
+    absorber_id: np.ndarray         # int32 [Jm]
+    ref_level_pressure: np.ndarray  # f8 [Ko+1]
+    ref_pressure: np.ndarray        # f8 [Ko]
+    ref_temperature: np.ndarray     # f8 [Ko]
+    ref_absorber: np.ndarray        # f8 [Ko, Jm]
+    min_absorber: np.ndarray        # f8 [Ko, Jm]
+    max_absorber: np.ndarray        # f8 [Ko, Jm]
+    n_predictors: np.ndarray        # int32 [J, L]
+    pos_index: np.ndarray           # int32 [J, L]  (1-based start into C)
+    c: np.ndarray                   # f4 [n_coeffs]
+    # OPTRAN water-vapour-line block (optional)
+    osignificance: Optional[np.ndarray] = None   # int32 [L]
+    order: Optional[np.ndarray] = None           # int32 [L]
+    op_index: Optional[np.ndarray] = None        # int32 [7, L]  (row 0 = count)
+    opos_index: Optional[np.ndarray] = None      # int32 [L]     (1-based start into OC)
+    oc: Optional[np.ndarray] = None              # f8 [n_ocoeffs]
+    alpha: float = 0.0
+    alpha_c1: float = 0.0
+    alpha_c2: float = 0.0
+    ocomponent_index: int = -1
+
+    @property
+    def n_layers(self) -> int:
+        return int(self.ref_temperature.size)
+
+    @property
+    def n_components(self) -> int:
+        return int(self.component_id.size)
+
+    @property
+    def n_absorbers(self) -> int:
+        return int(self.absorber_id.size)
+
+    @property
+    def n_channels(self) -> int:
+        return int(self.sensor_channel.size)
+
+    @property
+    def n_ocoeffs(self) -> int:
+        return 0 if self.oc is None else int(self.oc.size)
+
+
+def write_odps(path: str, t: ODPS) -> None:
+    """Record order of Read_ODPS_Data, ODPS_Binary_IO.f90:617-800."""
+    L, J, Jm, Ko = t.n_channels, t.n_components, t.n_absorbers, t.n_layers
+    with RecordWriter(path) as w:
+        w.record(np.int32(ODPS_RELEASE), np.int32(ODPS_VERSION))
+        w.record(np.int32(ODPS_ALGORITHM))
+        w.record(*[np.int32(x) for x in (Ko, J, Jm, L, t.c.size, N_PREDICTOR_USED_OPTRAN, t.n_ocoeffs)])
+        w.record(np.int32(t.group_index))
+        w.record(t.sensor_id, np.int32(t.wmo_satellite_id), np.int32(t.wmo_sensor_id),
+                 np.int32(t.sensor_type))
+        w.record(np.asarray(t.sensor_channel, np.int32))
+        w.record(np.asarray(t.component_id, np.int32))
+        w.record(np.asarray(t.absorber_id, np.int32))
+        w.record(np.asarray(t.ref_level_pressure, np.float64),
+                 np.asarray(t.ref_pressure, np.float64),
+                 np.asarray(t.ref_temperature, np.float64),
+                 np.asarray(t.ref_absorber, np.float64).reshape(Ko, Jm),
+                 np.asarray(t.min_absorber, np.float64).reshape(Ko, Jm),
+                 np.asarray(t.max_absorber, np.float64).reshape(Ko, Jm))
+        w.record(np.asarray(t.n_predictors, np.int32).reshape(J, L),
+                 np.asarray(t.pos_index, np.int32).reshape(J, L))
+        if t.c.size > 0:
+            w.record(np.asarray(t.c, np.float32))
+        if t.n_ocoeffs > 0:
+            w.record(np.asarray(t.osignificance, np.int32),
+                     np.asarray(t.order, np.int32),
+                     np.asarray(t.op_index, np.int32).reshape(N_PREDICTOR_USED_OPTRAN + 1, L),
+                     np.asarray(t.opos_index, np.int32),
+                     np.asarray(t.oc, np.float64),
+                     np.float64(t.alpha), np.float64(t.alpha_c1), np.float64(t.alpha_c2),
+                     np.int32(t.ocomponent_index))
+
+
+def read_odps(path: str) -> ODPS:
+    r = RecordReader(path)
+    rel, _ver = r.read([("i4", ()), ("i4", ())])
+    if rel != ODPS_RELEASE:
+        raise ValueError(f"ODPS release {rel} unsupported")
+    (alg,) = r.read([("i4", ())])
+    if alg != ODPS_ALGORITHM:
+        raise ValueError(f"not an ODPS file (algorithm {alg})")
+    Ko, J, Jm, L, nc, noi, noc = [int(x) for x in r.read([("i4", ())] * 7)]
+    (grp,) = r.read([("i4", ())])
+    sid, wsat, wsen, stype = r.read([("s", None), ("i4", ()), ("i4", ()), ("i4", ())])
+    (ch,) = r.read([("i4", (L,))])
+    (cid,) = r.read([("i4", (J,))])
+    (aid,) = r.read([("i4", (Jm,))])
+    rlp, rp, rt, ra, mn, mx = r.read([("f8", (Ko + 1,)), ("f8", (Ko,)), ("f8", (Ko,)),
+                                      ("f8", (Ko, Jm)), ("f8", (Ko, Jm)), ("f8", (Ko, Jm))])
+    npred, pos = r.read([("i4", (J, L)), ("i4", (J, L))])
+    c = r.read([("f4", (nc,))])[0] if nc > 0 else np.zeros(0, np.float32)
+    t = ODPS(int(grp), sid, int(wsat), int(wsen), int(stype), ch.copy(), cid.copy(), aid.copy(),
+             rlp.copy(), rp.copy(), rt.copy(), ra.copy(), mn.copy(), mx.copy(),
+             npred.copy(), pos.copy(), c.copy())
+    if noc > 0:
+        o = r.read([("i4", (L,)), ("i4", (L,)), ("i4", (noi + 1, L)), ("i4", (L,)),
+                    ("f8", (noc,)), ("f8", ()), ("f8", ()), ("f8", ()), ("i4", ())])
+        t.osignificance, t.order, t.op_index, t.opos_index, t.oc = [x.copy() for x in o[:5]]
+        t.alpha, t.alpha_c1, t.alpha_c2 = float(o[5]), float(o[6]), float(o[7])
+        t.ocomponent_index = int(o[8])
+    return t

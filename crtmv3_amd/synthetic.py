@@ -1,0 +1,493 @@
+"""Seeded synthetic sensors and atmospheric profiles.
+
+No CRTM coefficient data can be fetched offline, so every run (tests, bench,
+smoke) writes synthetic SpcCoeff / ODPS TauCoeff files in the reference binary
+formats (crtmv3_amd/coeffio.py) and reads them back through the native loader,
+exactly as a user would load real fix/ files.
+
+Sensor layouts ("like" = same channel count, spectral range and ODPS group):
+  amsua_syn   15 ch  MW  ODPS group 3   (AMSU-A-like)
+  atms_syn    22 ch  MW  ODPS group 3   (ATMS-like)
+  cris_syn  2211 ch  IR  ODPS group 1   (CrIS full-spectral-resolution-like, OPTRAN WLO)
+  iasi_syn  8461 ch  IR  ODPS group 1   (IASI-like, OPTRAN WLO)
+  hirs_syn    19 ch  IR  ODPS group 2   (small group-2 test sensor)
+
+ODPS reference grid: the 101-level model pressure grid of
+src/Atmosphere/CRTM_Model_Profiles.f90:80-97 with the US-standard climatology
+as the reference profile.  Regression coefficients are built so that, at the
+reference state, each component's layer optical depth equals a smooth target
+profile; perturbations then move OD through every predictor the component
+uses.
+
+Profiles follow the construction of the benchmark configurations: 100 layers
+on pressure levels log-spaced 0.005-1100 hPa, US-standard temperature plus a
+vertically correlated N(0, 5 K) field (1 km correlation length in
+hypsometric height), log-normal water vapour (sigma 0.3 in ln), ozone x
+U(0.8, 1.2), ocean surface at U(270, 305) K, zenith U(0, 60 deg).
+"""
+from __future__ import annotations
+
+import dataclasses
+import json
+import os
+from typing import Dict, List, Optional, Sequence
+
+import numpy as np
+
+from . import coeffio
+
+_DATA = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "model_profiles.json")
+
+# Planck radiation constants for radiance in mW/(m^2 sr cm^-1), wavenumber in cm^-1
+PLANCK_C1 = 1.191042972e-05
+PLANCK_C2 = 1.4387752
+COSMIC_T = 2.7253
+GHZ_TO_INV_CM = 1.0 / 29.9792458
+
+# CRTM absorber ids (src/Atmosphere/CRTM_Atmosphere_Define.f90)
+H2O_ID, CO2_ID, O3_ID, N2O_ID, CO_ID, CH4_ID = 1, 2, 3, 4, 5, 6
+
+# ODPS group layouts (src/AtmAbsorption/ODPS/ODPS_Predictor.f90:85-207)
+GROUPS = {
+    1: dict(n_pred=[7, 18, 7, 11, 11, 14, 10, 11], comp_id=[7, 101, 15, 114, 121, 120, 119, 118],
+            abs_id=[H2O_ID, O3_ID, CO2_ID, N2O_ID, CO_ID, CH4_ID]),
+    2: dict(n_pred=[7, 15, 7, 11, 10], comp_id=[20, 101, 15, 114, 121],
+            abs_id=[H2O_ID, O3_ID, CO2_ID]),
+    3: dict(n_pred=[7, 14], comp_id=[113, 12], abs_id=[H2O_ID]),
+}
+
+DT_SCALE = 5.0   # typical |T - Tref| used to scale coefficients of DT-type predictors
+
+
+def climatology() -> dict:
+    with open(_DATA) as f:
+        return json.load(f)
+
+
+def _layer_mean(levels_p, level_vals):
+    """Layer value as the ln(p)-weighted mean of the bounding level values."""
+    return 0.5 * (np.asarray(level_vals)[1:] + np.asarray(level_vals)[:-1])
+
+
+def reference_profile():
+    """ODPS reference grid: 101 model levels, US-standard layer profile."""
+    clim = climatology()
+    p = np.array(clim["level_pressure"])
+    us = clim["models"].index("US_STANDARD_ATMOSPHERE")
+    t = _layer_mean(p, clim["temperature"][us])
+    ab = np.stack([_layer_mean(p, clim["absorber"][us][i]) for i in range(6)], axis=1)
+    lay_p = (p[1:] - p[:-1]) / np.log(p[1:] / p[:-1])
+    # absorber columns in CRTM absorber-id order 1..6 -> clim order H2O,O3,CO2,N2O,CO,CH4
+    by_id = {H2O_ID: ab[:, 0], O3_ID: ab[:, 1], CO2_ID: ab[:, 2], N2O_ID: ab[:, 3],
+             CO_ID: ab[:, 4], CH4_ID: ab[:, 5]}
+    return p, lay_p, t, by_id
+
+
+# ----------------------------------------------------------------------------
+# Nominal predictor magnitudes at the reference state (T = Tref, all absorber
+# ratios and integrated ratios = 1), as functions of the secant s.  A value of
+# the form ("dt", v) marks a predictor that vanishes at the reference state
+# because it carries a factor DT or DT*|DT|; those get small signed weights.
+# ----------------------------------------------------------------------------
+def _nominal(group: int, comp: int, s: float):
+    r = np.sqrt(s)
+    q = s ** 0.25
+    if group in (1, 2):
+        tab = {
+            0: [s, s, s, 1, s * s, 1, 1],
+            1: [s, ("dt", s), s * s, ("dt2", s), q, s ** 3, r, ("dt", r), s ** 4, 1, r, s * s, s, s, s,
+                s, ("dt", s * s), s],
+            2: [s] * 7,
+            3: [s, ("dt", s), s, s * s, s, s * r, ("dt", r), r, r, s, s * s],
+            4: [s, s, 1, 1, s, s, s, s * s, 1, s, s],
+            5: [("dt", s), r, ("dt", 1), q, s, s, s, s * s, 1, r, s, s, s, s * s],
+            6: [s, ("dt", s), q, ("dt", r), s * s, r, ("dt2", s), s, r, s],
+            7: [("dt", s), r, s * s, s, ("dt", 1), s, s * s, q, 1, s, r],
+        }
+    else:
+        tab = {
+            0: [s, s, s, 1, s * s, 1, 1],
+            1: [s, s, s, s, s, s, s, ("dt", s), s * s, s, s, s ** 3, s ** 4, 1],
+        }
+    return tab[comp]
+
+
+@dataclasses.dataclass
+class SensorDef:
+    sensor_id: str
+    sensor_type: int
+    group: int
+    frequency: np.ndarray       # GHz (MW) or 0
+    wavenumber: np.ndarray      # cm^-1
+    strength: np.ndarray        # [J, L] column OD of each component at the reference state
+    optran_fraction: float = 0.0
+
+
+def _smooth_noise(rng, n, corr, sigma):
+    x = rng.standard_normal(n + 4 * corr)
+    k = np.exp(-0.5 * (np.arange(-2 * corr, 2 * corr + 1) / max(corr, 1)) ** 2)
+    y = np.convolve(x, k / np.sqrt((k * k).sum()), mode="same")[2 * corr:2 * corr + n]
+    return sigma * y
+
+
+def _bump(nu, c, w):
+    return np.exp(-0.5 * ((nu - c) / w) ** 2)
+
+
+def _ir_strengths(nu, group, rng):
+    L = nu.size
+    J = len(GROUPS[group]["n_pred"])
+    lines = np.exp(rng.standard_normal((J, L)) * 0.7)
+    A = np.zeros((J, L))
+    A[0] = (0.02 + 0.06 * _bump(nu, 1600, 150) + 0.05 * _bump(nu, 2350, 60)) * lines[0]
+    A[1] = (0.02 + 25.0 * _bump(nu, 1560, 130) + 0.3 * _bump(nu, 1300, 60)) * lines[1]
+    A[2] = (0.03 + 0.25 * _bump(nu, 1000, 250) + 0.1 * _bump(nu, 1500, 200)) * lines[2]
+    A[3] = (0.002 + 4.0 * _bump(nu, 1040, 25) + 0.3 * _bump(nu, 2110, 20)) * lines[3]
+    A[4] = (0.01 + 60.0 * _bump(nu, 667, 25) + 8.0 * _bump(nu, 720, 35) + 40.0 * _bump(nu, 2330, 40)
+            + 0.5 * _bump(nu, 960, 15)) * lines[4]
+    if J > 5:
+        A[5] = (0.002 + 1.2 * _bump(nu, 1280, 25) + 2.0 * _bump(nu, 2220, 25)) * lines[5]
+        A[6] = (0.001 + 1.5 * _bump(nu, 2150, 30)) * lines[6]
+        A[7] = (0.002 + 1.5 * _bump(nu, 1305, 35) + 0.3 * _bump(nu, 3000, 50)) * lines[7]
+    return A
+
+
+def sensor_def(name: str, seed: int = 20261203) -> SensorDef:
+    rng = np.random.default_rng(seed + sum(map(ord, name)))
+    if name == "amsua_syn":
+        f = np.array([23.8, 31.4, 50.3, 52.8, 53.596, 54.4, 54.94, 55.5,
+                      57.29, 57.29, 57.29, 57.29, 57.29, 57.29, 89.0])
+        peak = np.array([0, 0, 1000, 900, 600, 400, 250, 150, 90, 50, 25, 10, 5, 2.5, 0.0])
+        dry = np.where(peak > 0, 1100.0 / np.maximum(peak, 1e-9), [0.05, 0.03] + [0] * 12 + [0.1])
+        wet = np.array([0.35, 0.12, 0.06, 0.04, 0.02, 0.01, 0.005, 0.003, 0.002, 0.002, 0.002, 0.002,
+                        0.002, 0.002, 0.5])
+        return SensorDef(name, coeffio.MICROWAVE_SENSOR, 3, f, f * GHZ_TO_INV_CM, np.stack([dry, wet]))
+    if name == "atms_syn":
+        f = np.array([23.8, 31.4, 50.3, 51.76, 52.8, 53.596, 54.4, 54.94, 55.5,
+                      57.29, 57.29, 57.29, 57.29, 57.29, 57.29, 88.2, 165.5,
+                      183.31 + 7.0, 183.31 + 4.5, 183.31 + 3.0, 183.31 + 1.8, 183.31 + 1.0])
+        peak = np.array([0, 0, 1000, 950, 900, 600, 400, 250, 150, 90, 50, 25, 10, 5, 2.5,
+                         0, 0, 0, 0, 0, 0, 0], float)
+        dry = np.where(peak > 0, 1100.0 / np.maximum(peak, 1e-9), 0.05)
+        wet = np.array([0.35, 0.12, 0.06, 0.05, 0.04, 0.02, 0.01, 0.005, 0.003, 0.002, 0.002, 0.002,
+                        0.002, 0.002, 0.002, 0.5, 1.5, 4.0, 8.0, 16.0, 30.0, 60.0])
+        return SensorDef(name, coeffio.MICROWAVE_SENSOR, 3, f, f * GHZ_TO_INV_CM, np.stack([dry, wet]))
+    if name == "cris_syn":
+        nu = np.concatenate([650 + 0.625 * np.arange(713), 1210 + 0.625 * np.arange(865),
+                             2155 + 0.625 * np.arange(633)])
+        return SensorDef(name, coeffio.INFRARED_SENSOR, 1, np.zeros_like(nu), nu,
+                         _ir_strengths(nu, 1, rng), optran_fraction=0.35)
+    if name == "iasi_syn":
+        nu = 645 + 0.25 * np.arange(8461)
+        return SensorDef(name, coeffio.INFRARED_SENSOR, 1, np.zeros_like(nu), nu,
+                         _ir_strengths(nu, 1, rng), optran_fraction=0.35)
+    if name == "hirs_syn":
+        nu = np.array([669.0, 680, 690, 703, 716, 733, 749, 900, 1030, 1345, 1365, 1533,
+                       2188, 2210, 2235, 2245, 2420, 2515, 2660])
+        return SensorDef(name, coeffio.INFRARED_SENSOR, 2, np.zeros_like(nu), nu,
+                         _ir_strengths(nu, 2, rng), optran_fraction=0.0)
+    if name == "cris_small_syn":
+        # 96-channel subset spanning all three CrIS bands; fast for CPU oracle tests
+        full = np.concatenate([650 + 0.625 * np.arange(713), 1210 + 0.625 * np.arange(865),
+                               2155 + 0.625 * np.arange(633)])
+        nu = full[np.linspace(0, full.size - 1, 96).astype(int)]
+        return SensorDef(name, coeffio.INFRARED_SENSOR, 1, np.zeros_like(nu), nu,
+                         _ir_strengths(nu, 1, rng), optran_fraction=0.5)
+    raise KeyError(name)
+
+
+SENSORS = ("amsua_syn", "atms_syn", "cris_syn", "iasi_syn", "hirs_syn", "cris_small_syn")
+
+
+def _layer_shapes(group, lvl_p, lay_p, ref_abs):
+    """Normalised vertical distribution of each component's reference OD."""
+    dp = np.diff(lvl_p)
+    x = lay_p / 1100.0
+    q = ref_abs[H2O_ID]
+    if group == 3:
+        shapes = [dp * (0.5 + 0.5 * x), q * dp * x ** 0.3]
+    else:
+        shapes = [dp * x ** 0.5, q * dp * x ** 0.3, q * q * dp * x, ref_abs[O3_ID] * dp,
+                  dp * x ** 0.7]
+        if group == 1:
+            shapes += [ref_abs[N2O_ID] * dp, ref_abs[CO_ID] * dp, ref_abs[CH4_ID] * dp]
+    return [s / s.sum() for s in shapes]
+
+
+MAX_LAYER_OD = 12.0   # stays below the ODPS clamp MAX_OD = 20
+
+
+def make_tau_coeff(sd: SensorDef, seed: int = 20261203) -> coeffio.ODPS:
+    rng = np.random.default_rng(seed + 7 + sum(map(ord, sd.sensor_id)))
+    grp = GROUPS[sd.group]
+    lvl_p, lay_p, ref_t, ref_by_id = reference_profile()
+    Ko = lay_p.size
+    J = len(grp["n_pred"])
+    L = sd.wavenumber.size
+    abs_ids = grp["abs_id"]
+    ref_abs = np.stack([ref_by_id[a] for a in abs_ids], axis=1)
+    shapes = _layer_shapes(sd.group, lvl_p, lay_p, ref_by_id)
+    s_nom = 1.25
+    n_pred = np.zeros((J, L), np.int32)
+    pos = np.zeros((J, L), np.int32)
+    chunks = []
+    nc = 0
+    for l in range(L):
+        for j in range(J):
+            A = sd.strength[j, l]
+            if j == 0:
+                npj = grp["n_pred"][0]
+            elif A < 0.004:
+                npj = 0
+            else:
+                npj = int(rng.integers((grp["n_pred"][j] + 1) // 2, grp["n_pred"][j] + 1))
+            n_pred[j, l] = npj
+            if npj == 0:
+                pos[j, l] = nc + 1
+                continue
+            A = min(A, MAX_LAYER_OD / shapes[j].max())
+            tau = A * shapes[j]                      # [Ko]
+            nom = _nominal(sd.group, j, s_nom)[:npj]
+            w = rng.dirichlet(np.ones(npj))
+            coef = np.zeros((npj, Ko))
+            for i, v in enumerate(nom):
+                if isinstance(v, tuple):
+                    scale = DT_SCALE if v[0] == "dt" else DT_SCALE * DT_SCALE
+                    coef[i] = 0.05 * rng.standard_normal() * tau / (scale * v[1])
+                else:
+                    coef[i] = w[i] * tau / v
+            pos[j, l] = nc + 1
+            chunks.append(coef.astype(np.float32).ravel())   # (i-1)*Ko + k
+            nc += npj * Ko
+    c = np.concatenate(chunks) if chunks else np.zeros(0, np.float32)
+    t = coeffio.ODPS(
+        group_index=sd.group, sensor_id=sd.sensor_id, wmo_satellite_id=1023, wmo_sensor_id=2047,
+        sensor_type=sd.sensor_type, sensor_channel=np.arange(1, L + 1, dtype=np.int32),
+        component_id=np.array(grp["comp_id"], np.int32), absorber_id=np.array(abs_ids, np.int32),
+        ref_level_pressure=lvl_p, ref_pressure=lay_p, ref_temperature=ref_t, ref_absorber=ref_abs,
+        min_absorber=ref_abs * 1e-3, max_absorber=ref_abs * 1e3, n_predictors=n_pred,
+        pos_index=pos, c=c)
+    if sd.optran_fraction > 0:
+        _add_optran(t, sd, rng, shapes[1], lvl_p, lay_p, ref_t, ref_by_id[H2O_ID])
+    return t
+
+
+def _add_optran(t, sd, rng, wlo_shape, lvl_p, lay_p, ref_t, ref_q):
+    """OPTRAN water-vapour-line block for a subset of strong WLO channels.
+
+    For OPTRAN channels the WLO layer OD is chi*dA with
+    ln(chi) = b0(Ap) + sum_i b_i(Ap) OX_i   (ODPS_AtmAbsorption.f90:664-727).
+    b0 is a least-squares polynomial in Ap1 fitted so that chi*dA reproduces
+    the channel's WLO target OD at the reference state.
+    """
+    L = t.n_channels
+    order_max = 4
+    g_inv = 1.0 / 980.665
+    s_nom = 1.25
+    dA = g_inv * np.diff(lvl_p) * ref_q * s_nom
+    intv = np.cumsum(dA)
+    aveA = 0.5 * (intv + np.concatenate([[0.0], intv[:-1]]))
+    alpha, c1, c2 = 9.0, 1.0e-9, 0.0
+    ap1 = np.log((aveA - c2) / c1) / alpha
+    ox_ref = {1: ref_t, 2: lay_p, 10: ref_q, 14: np.full_like(lay_p, s_nom)}
+    osig = np.zeros(L, np.int32)
+    order = np.zeros(L, np.int32)
+    op_index = np.zeros((coeffio.N_PREDICTOR_USED_OPTRAN + 1, L), np.int32)
+    opos = np.ones(L, np.int32)
+    oc = []
+    n = 0
+    for l in range(L):
+        A = sd.strength[1, l]
+        if t.n_predictors[1, l] == 0 or A < 0.5 or rng.random() > sd.optran_fraction:
+            continue
+        A = min(A, MAX_LAYER_OD / wlo_shape.max())
+        tau = A * wlo_shape
+        preds = [1, 2, 10, 14][: int(rng.integers(1, 5))]
+        b = {1: 2.0e-3 * rng.standard_normal(), 2: 2.0e-4 * rng.standard_normal(),
+             10: 0.02 * rng.standard_normal(), 14: 0.1 * rng.standard_normal()}
+        resid = np.log(tau / dA) - sum(b[i] * ox_ref[i] for i in preds)
+        coef0 = np.polynomial.polynomial.polyfit(ap1, resid, order_max)
+        osig[l] = 1
+        order[l] = order_max
+        op_index[0, l] = len(preds)
+        op_index[1:1 + len(preds), l] = preds
+        opos[l] = n + 1
+        block = [coef0] + [np.r_[b[i], 1.0e-4 * rng.standard_normal(order_max)] for i in preds]
+        for row in block:
+            oc.append(row)
+            n += order_max + 1
+    if n == 0:
+        return
+    t.osignificance, t.order, t.op_index, t.opos_index = osig, order, op_index, opos
+    t.oc = np.concatenate(oc).astype(np.float64)
+    t.alpha, t.alpha_c1, t.alpha_c2 = alpha, c1, c2
+    t.ocomponent_index = 2
+
+
+def make_spc_coeff(sd: SensorDef, seed: int = 20261203) -> coeffio.SpcCoeff:
+    rng = np.random.default_rng(seed + 11 + sum(map(ord, sd.sensor_id)))
+    L = sd.wavenumber.size
+    nu = sd.wavenumber
+    c1 = PLANCK_C1 * nu ** 3
+    c2 = PLANCK_C2 * nu
+    if sd.sensor_type == coeffio.MICROWAVE_SENSOR:
+        bc1 = np.zeros(L)
+        bc2 = np.ones(L)
+    else:
+        bc1 = rng.uniform(-0.05, 0.05, L)
+        bc2 = 1.0 + rng.uniform(-2e-4, 2e-4, L)
+    if sd.sensor_type == coeffio.MICROWAVE_SENSOR:
+        cosmic = c1 / (np.exp(c2 / (bc1 + bc2 * COSMIC_T)) - 1.0)
+    else:
+        cosmic = np.zeros(L)
+    return coeffio.SpcCoeff(
+        sensor_id=sd.sensor_id, sensor_type=sd.sensor_type, wmo_satellite_id=1023,
+        wmo_sensor_id=2047, sensor_channel=np.arange(1, L + 1, dtype=np.int32),
+        polarization=np.ones(L, np.int32), pol_angle=np.zeros(L), channel_flag=np.zeros(L, np.int32),
+        frequency=sd.frequency, wavenumber=nu, planck_c1=c1, planck_c2=c2, band_c1=bc1,
+        band_c2=bc2, cosmic_background_radiance=cosmic, solar_irradiance=np.zeros(L))
+
+
+def write_sensor(dirpath: str, name: str, seed: int = 20261203) -> str:
+    """Write <name>.SpcCoeff.bin and <name>.TauCoeff.bin; returns dirpath."""
+    os.makedirs(dirpath, exist_ok=True)
+    spc = os.path.join(dirpath, f"{name}.SpcCoeff.bin")
+    tau = os.path.join(dirpath, f"{name}.TauCoeff.bin")
+    stamp = os.path.join(dirpath, f"{name}.seed")
+    if os.path.exists(spc) and os.path.exists(tau) and os.path.exists(stamp) \
+            and open(stamp).read().strip() == str(seed):
+        return dirpath
+    sd = sensor_def(name, seed)
+    coeffio.write_spccoeff(spc + ".tmp", make_spc_coeff(sd, seed))
+    coeffio.write_odps(tau + ".tmp", make_tau_coeff(sd, seed))
+    os.replace(spc + ".tmp", spc)
+    os.replace(tau + ".tmp", tau)
+    with open(stamp, "w") as f:
+        f.write(str(seed))
+    return dirpath
+
+
+def default_coeff_dir() -> str:
+    d = os.environ.get("CRTM_SYNTH_DIR")
+    if d:
+        return d
+    return os.path.join(os.environ.get("TMPDIR", "/tmp"), "crtmv3_amd_coeffs")
+
+
+# ----------------------------------------------------------------------------
+# Profiles
+# ----------------------------------------------------------------------------
+@dataclasses.dataclass
+class ProfileSet:
+    """Flat, layer-major profile batch (top-down layers, CRTM ordering).
+
+    Per-profile layer counts may differ; layer_offset[m] indexes the first
+    layer of profile m in the per-layer arrays, level arrays use
+    layer_offset[m] + m.
+    """
+    n_layers: np.ndarray                 # int32 [M]
+    layer_offset: np.ndarray             # int64 [M+1]
+    absorber_id: np.ndarray              # int32 [nabs]
+    absorber_units: np.ndarray           # int32 [nabs]
+    level_pressure: np.ndarray           # [sum(K)+M]
+    pressure: np.ndarray                 # [sum K]
+    temperature: np.ndarray              # [sum K]
+    absorber: np.ndarray                 # [nabs, sum K]
+    # surface
+    land_coverage: np.ndarray
+    water_coverage: np.ndarray
+    snow_coverage: np.ndarray
+    ice_coverage: np.ndarray
+    land_temperature: np.ndarray
+    water_temperature: np.ndarray
+    snow_temperature: np.ndarray
+    ice_temperature: np.ndarray
+    # geometry
+    sensor_zenith: np.ndarray
+    sensor_scan: np.ndarray
+    source_zenith: np.ndarray
+    # clouds: per profile up to n_clouds slots; water_content[n_clouds, sum K]
+    cloud_type: Optional[np.ndarray] = None          # int32 [M, nc]
+    cloud_water_content: Optional[np.ndarray] = None # kg/m^2 [nc, sum K]
+    cloud_effective_radius: Optional[np.ndarray] = None
+    cloud_fraction: Optional[np.ndarray] = None      # [sum K]
+    aerosol_type: Optional[np.ndarray] = None        # int32 [M, na]
+    aerosol_concentration: Optional[np.ndarray] = None
+    aerosol_effective_radius: Optional[np.ndarray] = None
+
+    @property
+    def n_profiles(self) -> int:
+        return int(self.n_layers.size)
+
+    def profile(self, m: int) -> Dict[str, np.ndarray]:
+        a, b = int(self.layer_offset[m]), int(self.layer_offset[m + 1])
+        return dict(level_pressure=self.level_pressure[a + m:b + m + 1], pressure=self.pressure[a:b],
+                    temperature=self.temperature[a:b], absorber=self.absorber[:, a:b])
+
+
+def _log_levels(K):
+    return np.exp(np.log(0.005) + np.arange(K + 1) / K * (np.log(1100.0) - np.log(0.005)))
+
+
+def _gp_factor(lay_p, t_clim, corr_km=1.0):
+    """Cholesky factor of the squared-exponential covariance in height."""
+    H = 0.001 * 287.05 * t_clim / 9.80665         # scale height, km
+    z = np.concatenate([[0.0], np.cumsum(-H[:-1] * np.diff(np.log(lay_p)))])
+    z = z[-1] - z                                   # increasing upwards from the bottom layer
+    d = z[:, None] - z[None, :]
+    cov = np.exp(-0.5 * (d / corr_km) ** 2) + 1e-8 * np.eye(z.size)
+    return np.linalg.cholesky(cov)
+
+
+def make_profiles(n_profiles: int, n_layers: int | Sequence[int] = 100, seed: int = 20261203,
+                  ozone: bool = True, sensor_zenith_max: float = 60.0) -> ProfileSet:
+    rng = np.random.default_rng(seed)
+    clim = climatology()
+    us = clim["models"].index("US_STANDARD_ATMOSPHERE")
+    cp = np.log(np.array(clim["level_pressure"]))
+    if np.isscalar(n_layers):
+        ks = np.full(n_profiles, int(n_layers), np.int32)
+    else:
+        ks = np.asarray(n_layers, np.int32)
+        assert ks.size == n_profiles
+    off = np.zeros(n_profiles + 1, np.int64)
+    off[1:] = np.cumsum(ks)
+    ntot = int(off[-1])
+    lvl = np.zeros(ntot + n_profiles)
+    lay = np.zeros(ntot)
+    tt = np.zeros(ntot)
+    nabs = 2 if ozone else 1
+    ab = np.zeros((nabs, ntot))
+    cache = {}
+    for m in range(n_profiles):
+        K = int(ks[m])
+        if K not in cache:
+            lp = _log_levels(K)
+            p = (lp[1:] - lp[:-1]) / np.log(lp[1:] / lp[:-1])
+            t0 = np.interp(np.log(p), cp, clim["temperature"][us])
+            q0 = np.exp(np.interp(np.log(p), cp, np.log(clim["absorber"][us][0])))
+            o0 = np.exp(np.interp(np.log(p), cp, np.log(clim["absorber"][us][1])))
+            cache[K] = (lp, p, t0, q0, o0, _gp_factor(p, t0))
+        lp, p, t0, q0, o0, Lc = cache[K]
+        a, b = int(off[m]), int(off[m + 1])
+        lvl[a + m:b + m + 1] = lp
+        lay[a:b] = p
+        tt[a:b] = t0 + 5.0 * (Lc @ rng.standard_normal(K))
+        ab[0, a:b] = q0 * np.exp(0.3 * (Lc @ rng.standard_normal(K)))
+        if ozone:
+            ab[1, a:b] = o0 * rng.uniform(0.8, 1.2)
+    M = n_profiles
+    return ProfileSet(
+        n_layers=ks, layer_offset=off,
+        absorber_id=np.array([H2O_ID, O3_ID][:nabs], np.int32),
+        absorber_units=np.array([3, 1][:nabs], np.int32),   # MASS_MIXING_RATIO (g/kg), VOLUME_MIXING_RATIO (ppmv)
+        level_pressure=lvl, pressure=lay, temperature=tt, absorber=ab,
+        land_coverage=np.zeros(M), water_coverage=np.ones(M), snow_coverage=np.zeros(M),
+        ice_coverage=np.zeros(M), land_temperature=np.full(M, 280.0),
+        water_temperature=rng.uniform(270.0, 305.0, M), snow_temperature=np.full(M, 260.0),
+        ice_temperature=np.full(M, 260.0), sensor_zenith=rng.uniform(0.0, sensor_zenith_max, M),
+        sensor_scan=np.zeros(M), source_zenith=np.full(M, 100.0))
+
+
+def default_emissivity(sensor_type: int, n_channels: int) -> np.ndarray:
+    return np.full(n_channels, 0.55 if sensor_type == coeffio.MICROWAVE_SENSOR else 0.98)
