@@ -1,0 +1,101 @@
+/* crtm_oracle.h -- FP64 CPU restatement of CRTM's per-profile x per-channel
+ * hot path.  TEST INFRASTRUCTURE ONLY: the oracle is the checker for the
+ * HIP implementation in crtmv3_amd/csrc; it is never linked into the
+ * product.  Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline
+ * leg load it.
+ *
+ * Array conventions: Fortran column-major arrays from the coefficient files
+ * are passed unchanged (e.g. n_predictors(J,L) -> index j + J*l).  Profile
+ * layers are top-down, levels 0..K.  absorber[j*K + k].
+ */
+#ifndef CRTM_ORACLE_H
+#define CRTM_ORACLE_H
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+  /* ODPS TauCoeff (src/Coefficients/TauCoeff/ODPS/ODPS_Define.f90:109-205) */
+  int group, n_layers, n_components, n_absorbers, n_channels;
+  const int* absorber_id;            /* [Jm] */
+  const double* ref_level_pressure;  /* [Ko+1] */
+  const double* ref_pressure;        /* [Ko] */
+  const double* ref_temperature;     /* [Ko] */
+  const double* ref_absorber;        /* (Ko,Jm) column-major */
+  const double* min_absorber;
+  const double* max_absorber;
+  const int* n_predictors;           /* (J,L) */
+  const int* pos_index;              /* (J,L), 1-based */
+  const float* c;
+  int n_ocoeffs;
+  const int* osignificance;          /* [L] */
+  const int* order;                  /* [L] */
+  const int* op_index;               /* (0:6,L) */
+  const int* opos_index;             /* [L], 1-based */
+  const double* oc;
+  double alpha, alpha_c1, alpha_c2;
+  int ocomponent_index;
+  /* SpcCoeff (src/Coefficients/SpcCoeff/SpcCoeff_Define.f90:163-189) */
+  int sensor_type;
+  const double* planck_c1;
+  const double* planck_c2;
+  const double* band_c1;
+  const double* band_c2;
+  const double* cosmic_background;
+} or_sensor;
+
+typedef struct {
+  int n_layers, n_absorbers;
+  const int* absorber_id;       /* [nabs] */
+  const double* level_pressure; /* [K+1] */
+  const double* pressure;       /* [K] */
+  const double* temperature;    /* [K] */
+  const double* absorber;       /* [nabs*K] */
+  double coverage[4];           /* land, water, snow, ice */
+  double sfc_temperature[4];
+  double sensor_zenith, sensor_scan, surface_altitude;
+} or_profile;
+
+typedef struct {
+  double* radiance;       /* [L] */
+  double* bt;             /* [L] */
+  double* tau;            /* [L*K] nadir layer OD, optional */
+  double* up_radiance;    /* [L] optional */
+  double* down_radiance;  /* [L] optional */
+  /* Jacobians (K-matrix), optional */
+  double* t_k;            /* [L*K] */
+  double* abs_k;          /* [L*nabs*K] */
+  double* sfc_t_k;        /* [L*4] land, water, snow, ice */
+  double* emis_k;         /* [L] */
+  double* radiance_k;     /* [L] radiance adjoint after the BT pre-process */
+} or_out;
+
+/* Forward model for one profile, all channels. */
+int oracle_forward(const or_sensor* s, const or_profile* p, const double* emissivity, or_out* o);
+
+/* K-matrix by the hand-written reverse-mode adjoint (CRTM_K_Matrix analogue).
+ * bt_k, rad_k: RTSolution_K Brightness_Temperature / Radiance inputs [L]. */
+int oracle_k_matrix(const or_sensor* s, const or_profile* p, const double* emissivity,
+                    const double* bt_k, const double* rad_k, or_out* o);
+
+/* K-matrix by complex-step forward-mode differentiation of the forward
+ * restatement: exact derivatives, independent of the hand-written adjoint. */
+int oracle_k_matrix_cstep(const or_sensor* s, const or_profile* p, const double* emissivity,
+                          const double* bt_k, const double* rad_k, or_out* o);
+
+/* Batched K-matrix over M profiles with OpenMP (CPU baseline); outputs are
+ * concatenated per profile with the same per-profile layout as above. */
+int oracle_k_matrix_batch(const or_sensor* s, int n_profiles, const or_profile* p,
+                          const double* emissivity, const double* bt_k, const double* rad_k,
+                          or_out* o, int n_threads);
+
+/* Intermediate quantities for white-box tests. */
+int oracle_layer_avg(int n_odps, const double* ref_lnp, int n_user, const double* user_lnp,
+                     double* weights /* [n_odps*n_user] (user fastest) */, int* index /* [2*n_odps] */);
+int oracle_predictors(const or_sensor* s, const or_profile* p, double* x /* [Ko*J*18] */,
+                      double* secant /* [Ko] */, double* t_odps, double* a_odps);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,190 @@
+"""ctypes bindings to the FP64 oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's CPU-baseline leg, never by the product package.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle.so")
+REF_PATH = os.path.join(HERE, "_ref", "libcrtm_ref.so")
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_fp = ctypes.POINTER(ctypes.c_float)
+
+
+class OrSensor(ctypes.Structure):
+    _fields_ = [("group", ctypes.c_int), ("n_layers", ctypes.c_int), ("n_components", ctypes.c_int),
+                ("n_absorbers", ctypes.c_int), ("n_channels", ctypes.c_int),
+                ("absorber_id", _ip), ("ref_level_pressure", _dp), ("ref_pressure", _dp),
+                ("ref_temperature", _dp), ("ref_absorber", _dp), ("min_absorber", _dp),
+                ("max_absorber", _dp), ("n_predictors", _ip), ("pos_index", _ip), ("c", _fp),
+                ("n_ocoeffs", ctypes.c_int), ("osignificance", _ip), ("order", _ip),
+                ("op_index", _ip), ("opos_index", _ip), ("oc", _dp),
+                ("alpha", ctypes.c_double), ("alpha_c1", ctypes.c_double),
+                ("alpha_c2", ctypes.c_double), ("ocomponent_index", ctypes.c_int),
+                ("sensor_type", ctypes.c_int), ("planck_c1", _dp), ("planck_c2", _dp),
+                ("band_c1", _dp), ("band_c2", _dp), ("cosmic_background", _dp)]
+
+
+class OrProfile(ctypes.Structure):
+    _fields_ = [("n_layers", ctypes.c_int), ("n_absorbers", ctypes.c_int), ("absorber_id", _ip),
+                ("level_pressure", _dp), ("pressure", _dp), ("temperature", _dp), ("absorber", _dp),
+                ("coverage", ctypes.c_double * 4), ("sfc_temperature", ctypes.c_double * 4),
+                ("sensor_zenith", ctypes.c_double), ("sensor_scan", ctypes.c_double),
+                ("surface_altitude", ctypes.c_double)]
+
+
+class OrOut(ctypes.Structure):
+    _fields_ = [(n, _dp) for n in ("radiance", "bt", "tau", "up_radiance", "down_radiance", "t_k",
+                                   "abs_k", "sfc_t_k", "emis_k", "radiance_k")]
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-f", os.path.join(HERE, "Makefile"), "all"], check=True)
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        _lib = ctypes.CDLL(LIB_PATH)
+        for fn in ("oracle_forward",):
+            getattr(_lib, fn).argtypes = [ctypes.POINTER(OrSensor), ctypes.POINTER(OrProfile), _dp,
+                                          ctypes.POINTER(OrOut)]
+        for fn in ("oracle_k_matrix", "oracle_k_matrix_cstep"):
+            getattr(_lib, fn).argtypes = [ctypes.POINTER(OrSensor), ctypes.POINTER(OrProfile), _dp,
+                                          _dp, _dp, ctypes.POINTER(OrOut)]
+        _lib.oracle_k_matrix_batch.argtypes = [ctypes.POINTER(OrSensor), ctypes.c_int,
+                                               ctypes.POINTER(OrProfile), _dp, _dp, _dp,
+                                               ctypes.POINTER(OrOut), ctypes.c_int]
+    return _lib
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t) if a is not None else ctypes.cast(None, t)
+
+
+class Sensor:
+    """Oracle view of one sensor's ODPS + SpcCoeff data (keeps arrays alive)."""
+
+    def __init__(self, tau, spc):
+        self._keep = []
+
+        def keep(a, dt):
+            a = np.ascontiguousarray(np.asarray(a).ravel(order="F"), dt)
+            self._keep.append(a)
+            return a
+        s = OrSensor()
+        s.group, s.n_layers, s.n_components = tau.group_index, tau.n_layers, tau.n_components
+        s.n_absorbers, s.n_channels = tau.n_absorbers, tau.n_channels
+        s.absorber_id = _p(keep(tau.absorber_id, np.int32), _ip)
+        s.ref_level_pressure = _p(keep(tau.ref_level_pressure, np.float64), _dp)
+        s.ref_pressure = _p(keep(tau.ref_pressure, np.float64), _dp)
+        s.ref_temperature = _p(keep(tau.ref_temperature, np.float64), _dp)
+        s.ref_absorber = _p(keep(tau.ref_absorber, np.float64), _dp)
+        s.min_absorber = _p(keep(tau.min_absorber, np.float64), _dp)
+        s.max_absorber = _p(keep(tau.max_absorber, np.float64), _dp)
+        s.n_predictors = _p(keep(tau.n_predictors, np.int32), _ip)
+        s.pos_index = _p(keep(tau.pos_index, np.int32), _ip)
+        s.c = _p(keep(tau.c, np.float32), _fp)
+        s.n_ocoeffs = tau.n_ocoeffs
+        if tau.n_ocoeffs > 0:
+            s.osignificance = _p(keep(tau.osignificance, np.int32), _ip)
+            s.order = _p(keep(tau.order, np.int32), _ip)
+            s.op_index = _p(keep(tau.op_index, np.int32), _ip)
+            s.opos_index = _p(keep(tau.opos_index, np.int32), _ip)
+            s.oc = _p(keep(tau.oc, np.float64), _dp)
+            s.alpha, s.alpha_c1, s.alpha_c2 = tau.alpha, tau.alpha_c1, tau.alpha_c2
+            s.ocomponent_index = tau.ocomponent_index
+        s.sensor_type = spc.sensor_type
+        s.planck_c1 = _p(keep(spc.planck_c1, np.float64), _dp)
+        s.planck_c2 = _p(keep(spc.planck_c2, np.float64), _dp)
+        s.band_c1 = _p(keep(spc.band_c1, np.float64), _dp)
+        s.band_c2 = _p(keep(spc.band_c2, np.float64), _dp)
+        s.cosmic_background = _p(keep(spc.cosmic_background_radiance, np.float64), _dp)
+        self.s = s
+        self.n_channels = tau.n_channels
+
+
+def make_profile(ps, m, keep):
+    pr = ps.profile(m)
+    p = OrProfile()
+    K = pr["temperature"].size
+    na = pr["absorber"].shape[0]
+    arrs = [np.ascontiguousarray(pr["level_pressure"], np.float64),
+            np.ascontiguousarray(pr["pressure"], np.float64),
+            np.ascontiguousarray(pr["temperature"], np.float64),
+            np.ascontiguousarray(pr["absorber"], np.float64).ravel(),
+            np.ascontiguousarray(ps.absorber_id, np.int32)]
+    keep.extend(arrs)
+    p.n_layers, p.n_absorbers = K, na
+    p.level_pressure, p.pressure, p.temperature, p.absorber = [_p(a, _dp) for a in arrs[:4]]
+    p.absorber_id = _p(arrs[4], _ip)
+    p.coverage[:] = [ps.land_coverage[m], ps.water_coverage[m], ps.snow_coverage[m], ps.ice_coverage[m]]
+    p.sfc_temperature[:] = [ps.land_temperature[m], ps.water_temperature[m], ps.snow_temperature[m],
+                            ps.ice_temperature[m]]
+    p.sensor_zenith = ps.sensor_zenith[m]
+    p.sensor_scan = ps.sensor_scan[m]
+    p.surface_altitude = 0.0
+    return p
+
+
+def run_profile(sensor: Sensor, ps, m, emissivity, mode="forward", bt_k=None, rad_k=None):
+    """mode: 'forward', 'k' (reverse adjoint) or 'cstep' (complex-step)."""
+    keep = []
+    p = make_profile(ps, m, keep)
+    L = sensor.n_channels
+    K = p.n_layers
+    na = p.n_absorbers
+    out = {"radiance": np.zeros(L), "bt": np.zeros(L), "tau": np.zeros((L, K)),
+           "up_radiance": np.zeros(L), "down_radiance": np.zeros(L)}
+    if mode != "forward":
+        out.update(t_k=np.zeros((L, K)), abs_k=np.zeros((L, na, K)), sfc_t_k=np.zeros((L, 4)),
+                   emis_k=np.zeros(L), radiance_k=np.zeros(L))
+    o = OrOut()
+    for k, v in out.items():
+        setattr(o, k, _p(v, _dp))
+    eps = np.ascontiguousarray(emissivity, np.float64)
+    if mode == "forward":
+        lib().oracle_forward(ctypes.byref(sensor.s), ctypes.byref(p), _p(eps, _dp), ctypes.byref(o))
+    else:
+        btk = np.ascontiguousarray(np.ones(L) if bt_k is None else bt_k, np.float64)
+        rk = np.ascontiguousarray(np.zeros(L) if rad_k is None else rad_k, np.float64)
+        fn = lib().oracle_k_matrix if mode == "k" else lib().oracle_k_matrix_cstep
+        fn(ctypes.byref(sensor.s), ctypes.byref(p), _p(eps, _dp), _p(btk, _dp), _p(rk, _dp),
+           ctypes.byref(o))
+    return out
+
+
+def k_matrix_batch(sensor: Sensor, ps, emissivity, n_threads=0, bt_k=None, rad_k=None):
+    """All profiles of a ProfileSet; returns dict of outputs (t_k layer-concatenated)."""
+    M = ps.n_profiles
+    L = sensor.n_channels
+    keep = []
+    profs = (OrProfile * M)(*[make_profile(ps, m, keep) for m in range(M)])
+    na = ps.absorber.shape[0]
+    ntot = int(ps.layer_offset[-1])
+    out = {"radiance": np.zeros((M, L)), "bt": np.zeros((M, L)), "t_k": np.zeros(ntot * L),
+           "abs_k": np.zeros(ntot * L * na), "sfc_t_k": np.zeros((M, L, 4)), "emis_k": np.zeros((M, L)),
+           "radiance_k": np.zeros((M, L))}
+    o = OrOut()
+    for k, v in out.items():
+        setattr(o, k, _p(v, _dp))
+    eps = np.ascontiguousarray(np.broadcast_to(emissivity, (M, L)), np.float64)
+    btk = np.ascontiguousarray(np.ones((M, L)) if bt_k is None else bt_k, np.float64)
+    rk = np.ascontiguousarray(np.zeros((M, L)) if rad_k is None else rad_k, np.float64)
+    lib().oracle_k_matrix_batch(ctypes.byref(sensor.s), M, profs, _p(eps, _dp), _p(btk, _dp),
+                                _p(rk, _dp), ctypes.byref(o), n_threads)
+    return out
