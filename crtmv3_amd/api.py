@@ -1,0 +1,284 @@
+"""Python mirror of the CRTM API over the C-ABI (include/crtm_mi355x.h).
+
+Names follow the reference interface (src/CRTM_Module.F90:30-38):
+  crtm_init(sensor_ids, file_path)            -> Context     (CRTM_Init)
+  Context.forward(profiles, emissivity, ...)  -> RTSolution  (CRTM_Forward)
+  Context.k_matrix(profiles, emissivity, ...) -> (RTSolution, KMatrix)  (CRTM_K_Matrix)
+  Context.destroy()                                          (CRTM_Destroy)
+
+Errors surface as CRTMError carrying the reference status code
+(SUCCESS=0, FAILURE=3, src/Utility/Message_Handler.f90:27-31).
+
+Host arrays (numpy) go through CRTM_MEM_HOST; torch tensors already on the
+GPU go through CRTM_MEM_DEVICE with no host round trip.  The library must be
+built (python -m crtmv3_amd.build); there is no CPU fallback.
+"""
+from __future__ import annotations
+
+import ctypes
+import dataclasses
+import os
+from typing import Dict, Optional, Sequence
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libcrtm_mi355x.so")
+
+SUCCESS, FAILURE = 0, 3
+PREC_FP64, PREC_FP32_SPLIT = 0, 1
+MEM_HOST, MEM_DEVICE = 0, 1
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_ip = ctypes.POINTER(ctypes.c_int)
+_i64p = ctypes.POINTER(ctypes.c_int64)
+_vp = ctypes.c_void_p
+
+
+class CRTMError(RuntimeError):
+    def __init__(self, msg, status=FAILURE):
+        super().__init__(msg)
+        self.status = status
+
+
+class CAtmosphere(ctypes.Structure):
+    _fields_ = [("n_profiles", ctypes.c_int), ("n_layers", _vp), ("layer_offset", _vp),
+                ("n_absorbers", ctypes.c_int), ("absorber_id", _vp), ("absorber_units", _vp),
+                ("level_pressure", _vp), ("pressure", _vp), ("temperature", _vp), ("absorber", _vp)]
+
+
+class CSurface(ctypes.Structure):
+    _fields_ = [(n, _vp) for n in ("land_coverage", "water_coverage", "snow_coverage", "ice_coverage",
+                                   "land_temperature", "water_temperature", "snow_temperature",
+                                   "ice_temperature")]
+
+
+class CGeometry(ctypes.Structure):
+    _fields_ = [(n, _vp) for n in ("sensor_zenith_angle", "sensor_scan_angle", "source_zenith_angle",
+                                   "surface_altitude")]
+
+
+class COptions(ctypes.Structure):
+    _fields_ = [("use_emissivity", ctypes.c_int), ("emissivity", _vp), ("precision", ctypes.c_int),
+                ("memory", ctypes.c_int), ("jacobian_f32", ctypes.c_int), ("check_input", ctypes.c_int)]
+
+
+class CRTSolution(ctypes.Structure):
+    _fields_ = [(n, _vp) for n in ("radiance", "brightness_temperature", "up_radiance", "down_radiance",
+                                   "surface_planck_radiance", "surface_emissivity", "status")]
+
+
+class CKOutput(ctypes.Structure):
+    _fields_ = [(n, _vp) for n in ("brightness_temperature_k", "radiance_k", "surface_emissivity_k",
+                                   "temperature_k", "absorber_k", "land_temperature_k",
+                                   "water_temperature_k", "snow_temperature_k", "ice_temperature_k")]
+
+
+_lib = None
+
+
+def lib():
+    """Load the native library; fails loudly if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise CRTMError(f"{LIB_PATH} missing: run `python -m crtmv3_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        L.crtm_init.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_char_p), ctypes.c_char_p,
+                                ctypes.POINTER(_vp)]
+        L.crtm_destroy.argtypes = [_vp]
+        L.crtm_n_sensors.argtypes = [_vp]
+        L.crtm_n_channels_total.argtypes = [_vp]
+        L.crtm_channel_info.argtypes = [_vp, ctypes.c_int, _ip, _ip, _ip, ctypes.c_char_p]
+        L.crtm_forward.argtypes = [_vp, ctypes.POINTER(CAtmosphere), ctypes.POINTER(CSurface),
+                                   ctypes.POINTER(CGeometry), ctypes.POINTER(COptions),
+                                   ctypes.POINTER(CRTSolution)]
+        L.crtm_k_matrix.argtypes = L.crtm_forward.argtypes + [ctypes.POINTER(CKOutput)]
+        L.crtm_synchronize.argtypes = [_vp]
+        L.crtm_last_error.argtypes = [_vp]
+        L.crtm_last_error.restype = ctypes.c_char_p
+        _lib = L
+    return _lib
+
+
+def _is_torch(x) -> bool:
+    return type(x).__module__.startswith("torch")
+
+
+def _ptr(x):
+    if x is None:
+        return None
+    if _is_torch(x):
+        return x.data_ptr()
+    return x.ctypes.data
+
+
+@dataclasses.dataclass
+class ChannelInfo:
+    """CRTM_ChannelInfo_type (src/ChannelInfo/CRTM_ChannelInfo_Define.f90:70-85)."""
+    sensor_id: str
+    sensor_type: int
+    n_channels: int
+    sensor_channel: np.ndarray
+    channel_offset: int
+
+
+class Context:
+    def __init__(self, handle, channel_info):
+        self._h = handle
+        self.channel_info = channel_info
+        self.n_channels = sum(c.n_channels for c in channel_info)
+
+    def destroy(self):
+        if self._h:
+            lib().crtm_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.destroy()
+        except Exception:
+            pass
+
+    def _err(self, what):
+        msg = lib().crtm_last_error(self._h).decode()
+        raise CRTMError(f"{what}: {msg}")
+
+    # -- argument marshalling ------------------------------------------------
+    def _structs(self, prof, emissivity, precision, memory, jacobian_f32, check_input, keep):
+        def arr(x, dt):
+            if x is None:
+                return None
+            if memory == MEM_DEVICE:
+                return x
+            a = np.ascontiguousarray(x, dt)
+            keep.append(a)
+            return a
+        atm = CAtmosphere()
+        atm.n_profiles = prof.n_profiles
+        atm.n_layers = _ptr(arr(prof.n_layers, np.int32))
+        atm.layer_offset = _ptr(arr(prof.layer_offset, np.int64))
+        atm.n_absorbers = int(prof.absorber.shape[0])
+        atm.absorber_id = _ptr(arr(prof.absorber_id, np.int32))
+        atm.absorber_units = _ptr(arr(prof.absorber_units, np.int32))
+        atm.level_pressure = _ptr(arr(prof.level_pressure, np.float64))
+        atm.pressure = _ptr(arr(prof.pressure, np.float64))
+        atm.temperature = _ptr(arr(prof.temperature, np.float64))
+        atm.absorber = _ptr(arr(prof.absorber, np.float64))
+        sfc = CSurface()
+        for f in ("land_coverage", "water_coverage", "snow_coverage", "ice_coverage", "land_temperature",
+                  "water_temperature", "snow_temperature", "ice_temperature"):
+            setattr(sfc, f, _ptr(arr(getattr(prof, f), np.float64)))
+        geo = CGeometry()
+        geo.sensor_zenith_angle = _ptr(arr(prof.sensor_zenith, np.float64))
+        geo.sensor_scan_angle = _ptr(arr(prof.sensor_scan, np.float64))
+        geo.source_zenith_angle = _ptr(arr(prof.source_zenith, np.float64))
+        opt = COptions()
+        opt.use_emissivity = 1
+        opt.emissivity = _ptr(arr(emissivity, np.float64))
+        opt.precision = precision
+        opt.memory = memory
+        opt.jacobian_f32 = int(jacobian_f32)
+        opt.check_input = int(check_input)
+        return atm, sfc, geo, opt
+
+    def _alloc(self, shape, dtype, memory, like=None):
+        if memory == MEM_DEVICE:
+            import torch
+            tdt = {np.float64: torch.float64, np.float32: torch.float32, np.int32: torch.int32}[dtype]
+            return torch.zeros(shape, dtype=tdt, device=like.device)
+        return np.zeros(shape, dtype)
+
+    def forward(self, prof, emissivity, precision=PREC_FP64, extras=False, check_input=True):
+        """CRTM_Forward over a ProfileSet (host numpy) or a device batch (torch)."""
+        memory = MEM_DEVICE if _is_torch(prof.temperature) else MEM_HOST
+        keep = []
+        atm, sfc, geo, opt = self._structs(prof, emissivity, precision, memory, False, check_input, keep)
+        M, L = prof.n_profiles, self.n_channels
+        out = {k: self._alloc((M, L), np.float64, memory, prof.temperature)
+               for k in ("radiance", "brightness_temperature")}
+        if extras:
+            for k in ("up_radiance", "down_radiance", "surface_planck_radiance", "surface_emissivity"):
+                out[k] = self._alloc((M, L), np.float64, memory, prof.temperature)
+        out["status"] = self._alloc((M,), np.int32, memory, prof.temperature)
+        rts = CRTSolution(**{k: _ptr(v) for k, v in out.items()})
+        st = lib().crtm_forward(self._h, atm, sfc, geo, opt, rts)
+        if st != SUCCESS:
+            self._err("crtm_forward")
+        return out
+
+    def k_matrix(self, prof, emissivity, bt_k=None, rad_k=None, precision=PREC_FP64,
+                 jacobian_f32=False, check_input=True):
+        """CRTM_K_Matrix: returns (RTSolution dict, K dict).
+
+        K layout: temperature_k [NT, L] (profile m rows layer_offset[m]:...),
+        absorber_k [n_absorbers, NT, L]; surface temperature K per type [M, L]."""
+        memory = MEM_DEVICE if _is_torch(prof.temperature) else MEM_HOST
+        keep = []
+        atm, sfc, geo, opt = self._structs(prof, emissivity, precision, memory, jacobian_f32,
+                                           check_input, keep)
+        M, L = prof.n_profiles, self.n_channels
+        NT = int(prof.layer_offset[-1])
+        na = int(prof.absorber.shape[0])
+        like = prof.temperature
+        rts_d = {k: self._alloc((M, L), np.float64, memory, like) for k in ("radiance", "brightness_temperature")}
+        rts_d["status"] = self._alloc((M,), np.int32, memory, like)
+        kdt = np.float32 if jacobian_f32 else np.float64
+        if bt_k is None:
+            bt_k = self._alloc((M, L), np.float64, memory, like) + 1.0
+        if rad_k is None:
+            rad_k = self._alloc((M, L), np.float64, memory, like)
+        else:
+            rad_k = rad_k.clone() if _is_torch(rad_k) else np.array(rad_k, np.float64)
+        if memory == MEM_HOST:
+            bt_k = np.ascontiguousarray(bt_k, np.float64)
+        kd = {"radiance_k": rad_k,
+              "surface_emissivity_k": self._alloc((M, L), np.float64, memory, like),
+              "temperature_k": self._alloc((NT, L), kdt, memory, like),
+              "absorber_k": self._alloc((na, NT, L), kdt, memory, like)}
+        for f in ("land_temperature_k", "water_temperature_k", "snow_temperature_k", "ice_temperature_k"):
+            kd[f] = self._alloc((M, L), np.float64, memory, like)
+        rts = CRTSolution(**{k: _ptr(v) for k, v in rts_d.items()})
+        ko = CKOutput(brightness_temperature_k=_ptr(bt_k), **{k: _ptr(v) for k, v in kd.items()})
+        st = lib().crtm_k_matrix(self._h, atm, sfc, geo, opt, rts, ko)
+        if st != SUCCESS:
+            self._err("crtm_k_matrix")
+        return rts_d, kd
+
+    def synchronize(self):
+        if lib().crtm_synchronize(self._h) != SUCCESS:
+            self._err("crtm_synchronize")
+
+
+def crtm_init(sensor_ids: Sequence[str], file_path: str) -> Context:
+    """CRTM_Init (src/CRTM_LifeCycle.f90:520)."""
+    L = lib()
+    ids = (ctypes.c_char_p * len(sensor_ids))(*[s.encode() for s in sensor_ids])
+    h = _vp()
+    st = L.crtm_init(len(sensor_ids), ids, file_path.encode(), ctypes.byref(h))
+    if st != SUCCESS:
+        raise CRTMError(f"crtm_init failed for {list(sensor_ids)} in {file_path}", st)
+    infos = []
+    off = 0
+    for i in range(L.crtm_n_sensors(h)):
+        n, t = ctypes.c_int(), ctypes.c_int()
+        L.crtm_channel_info(h, i, ctypes.byref(n), ctypes.byref(t), None, None)
+        ch = np.zeros(n.value, np.int32)
+        sid = ctypes.create_string_buffer(32)
+        L.crtm_channel_info(h, i, None, None, ch.ctypes.data_as(_ip), sid)
+        infos.append(ChannelInfo(sid.value.decode(), t.value, n.value, ch, off))
+        off += n.value
+    return Context(h, infos)
+
+
+def device_profiles(prof, device="cuda"):
+    """Copy a synthetic.ProfileSet to torch tensors on the GPU (device-memory calls)."""
+    import torch
+    d = {}
+    for f in dataclasses.fields(prof):
+        v = getattr(prof, f.name)
+        d[f.name] = None if v is None else torch.as_tensor(np.ascontiguousarray(v)).to(device)
+    out = dataclasses.replace(prof, **d)
+    # host copies kept for bookkeeping the wrapper needs
+    object.__setattr__(out, "layer_offset_host", np.asarray(prof.layer_offset))
+    return out

@@ -1,0 +1,849 @@
+// crtm_abi.cpp -- host side of the MI355X CRTM hot path: coefficient file
+// loaders (CRTM binary formats), the device coefficient layout, the device
+// memory pool, and the C-ABI declared in include/crtm_mi355x.h.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <memory>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/crtm_mi355x.h"
+#include "crtm_device.h"
+
+namespace crtm {
+template <int G, class RT>
+void launch_prep(const DevSensor&, const DevBatch&, const ProfWork<RT>&, hipStream_t);
+template <int G, class RT, class KT, bool DOK>
+void launch_fwdk(const DevSensor&, const DevBatch&, const ProfWork<RT>&, const DevOut&, hipStream_t);
+}  // namespace crtm
+
+using namespace crtm;
+
+namespace {
+
+struct CrtmError : std::runtime_error {
+  using std::runtime_error::runtime_error;
+};
+
+std::string fmt(const char* f, ...) {
+  char buf[1024];
+  va_list ap;
+  va_start(ap, f);
+  vsnprintf(buf, sizeof buf, f, ap);
+  va_end(ap);
+  return buf;
+}
+
+#define HIPCHECK(x)                                                                   \
+  do {                                                                                \
+    hipError_t e_ = (x);                                                              \
+    if (e_ != hipSuccess) throw CrtmError(fmt("%s: %s", #x, hipGetErrorString(e_)));  \
+  } while (0)
+
+// ---------------------------------------------------------------------------
+// Fortran sequential-unformatted record reader (magic number first record,
+// src/Utility/Binary_File_Utility.f90:56, 235)
+// ---------------------------------------------------------------------------
+class Records {
+ public:
+  explicit Records(const std::string& path) : path_(path) {
+    std::ifstream f(path, std::ios::binary);
+    if (!f) throw CrtmError("File " + path + " not found.");
+    buf_.assign(std::istreambuf_iterator<char>(f), {});
+    auto r = next();
+    int32_t magic;
+    if (r.size() != 4) throw CrtmError("Error reading magic number from " + path);
+    std::memcpy(&magic, r.data(), 4);
+    if (magic != 123456789) throw CrtmError("Invalid magic number in " + path);
+  }
+  std::vector<char> next() {
+    if (pos_ + 4 > buf_.size()) throw CrtmError("Unexpected end of file " + path_);
+    int32_t n;
+    std::memcpy(&n, &buf_[pos_], 4);
+    if (n < 0 || pos_ + 8 + (size_t)n > buf_.size()) throw CrtmError("Corrupt record in " + path_);
+    std::vector<char> r(buf_.begin() + pos_ + 4, buf_.begin() + pos_ + 4 + n);
+    int32_t n2;
+    std::memcpy(&n2, &buf_[pos_ + 4 + n], 4);
+    if (n2 != n) throw CrtmError("Corrupt record framing in " + path_);
+    pos_ += 8 + n;
+    return r;
+  }
+
+ private:
+  std::string path_;
+  std::vector<char> buf_;
+  size_t pos_ = 0;
+};
+
+class Cursor {
+ public:
+  explicit Cursor(std::vector<char> r) : r_(std::move(r)) {}
+  template <class T> T get() {
+    need(sizeof(T));
+    T v;
+    std::memcpy(&v, &r_[p_], sizeof(T));
+    p_ += sizeof(T);
+    return v;
+  }
+  template <class T> std::vector<T> arr(size_t n) {
+    need(n * sizeof(T));
+    std::vector<T> v(n);
+    if (n) std::memcpy(v.data(), &r_[p_], n * sizeof(T));
+    p_ += n * sizeof(T);
+    return v;
+  }
+  std::string str(size_t n) {
+    need(n);
+    std::string s(&r_[p_], n);
+    p_ += n;
+    while (!s.empty() && (s.back() == ' ' || s.back() == '\0')) s.pop_back();
+    return s;
+  }
+  void done(const char* what) {
+    if (p_ != r_.size()) throw CrtmError(fmt("record length mismatch reading %s", what));
+  }
+
+ private:
+  void need(size_t n) {
+    if (p_ + n > r_.size()) throw CrtmError("record too short");
+  }
+  std::vector<char> r_;
+  size_t p_ = 0;
+};
+
+struct SpcCoeffH {
+  std::string sensor_id;
+  int sensor_type = 0, n_channels = 0;
+  std::vector<int> sensor_channel;
+  std::vector<double> frequency, wavenumber, c1, c2, bc1, bc2, cosmic, solar;
+};
+
+// SpcCoeff_Binary_IO.f90:374-470
+SpcCoeffH read_spccoeff(const std::string& path) {
+  Records rec(path);
+  SpcCoeffH sc;
+  Cursor r0(rec.next());
+  int rel = r0.get<int32_t>(), ver = r0.get<int32_t>();
+  r0.done("SpcCoeff release");
+  if (rel != 8) throw CrtmError(fmt("SpcCoeff Release check failed for %s (release %d)", path.c_str(), rel));
+  Cursor r1(rec.next());
+  const int n = r1.get<int32_t>();
+  r1.done("SpcCoeff dimensions");
+  sc.n_channels = n;
+  Cursor r2(rec.next());
+  sc.sensor_id = r2.str(20);
+  sc.sensor_type = r2.get<int32_t>();
+  r2.get<int32_t>();
+  r2.get<int32_t>();
+  r2.done("SpcCoeff sensor ids");
+  if (sc.sensor_type > 100) sc.sensor_type -= 100;   // ACTIVE_SENSOR offset
+  Cursor r3(rec.next());
+  sc.sensor_channel = r3.arr<int32_t>(n);
+  r3.arr<int32_t>(n);                    // polarization
+  if (ver > 2) r3.arr<double>(n);        // PolAngle
+  r3.arr<int32_t>(n);                    // channel flag
+  sc.frequency = r3.arr<double>(n);
+  sc.wavenumber = r3.arr<double>(n);
+  sc.c1 = r3.arr<double>(n);
+  sc.c2 = r3.arr<double>(n);
+  sc.bc1 = r3.arr<double>(n);
+  sc.bc2 = r3.arr<double>(n);
+  sc.cosmic = r3.arr<double>(n);
+  sc.solar = r3.arr<double>(n);
+  r3.done("SpcCoeff channel data");
+  return sc;
+}
+
+struct OdpsH {
+  int Ko = 0, J = 0, Jm = 0, L = 0, n_coeffs = 0, n_opindex = 0, n_ocoeffs = 0, group = 0, sensor_type = 0;
+  std::string sensor_id;
+  std::vector<int> sensor_channel, component_id, absorber_id, n_pred, pos;
+  std::vector<double> ref_lp, ref_p, ref_t, ref_a, min_a, max_a;
+  std::vector<float> c;
+  std::vector<int> osig, order, op_index, opos;
+  std::vector<double> oc;
+  double alpha = 0, alpha_c1 = 0, alpha_c2 = 0;
+  int ocomp = -1;
+};
+
+// Read_ODPS_Data, ODPS_Binary_IO.f90:617-800
+OdpsH read_odps(const std::string& path) {
+  Records rec(path);
+  OdpsH t;
+  {
+    Cursor r(rec.next());
+    int rel = r.get<int32_t>();
+    r.get<int32_t>();
+    r.done("ODPS release");
+    if (rel != 2) throw CrtmError(fmt("ODPS Release check failed for %s (release %d)", path.c_str(), rel));
+  }
+  {
+    Cursor r(rec.next());
+    int alg = r.get<int32_t>();
+    r.done("ODPS algorithm");
+    if (alg != 2) throw CrtmError(fmt("%s is not an ODPS TauCoeff file (algorithm %d)", path.c_str(), alg));
+  }
+  {
+    Cursor r(rec.next());
+    t.Ko = r.get<int32_t>();
+    t.J = r.get<int32_t>();
+    t.Jm = r.get<int32_t>();
+    t.L = r.get<int32_t>();
+    t.n_coeffs = r.get<int32_t>();
+    t.n_opindex = r.get<int32_t>();
+    t.n_ocoeffs = r.get<int32_t>();
+    r.done("ODPS dimensions");
+  }
+  {
+    Cursor r(rec.next());
+    t.group = r.get<int32_t>();
+    r.done("ODPS group");
+  }
+  {
+    Cursor r(rec.next());
+    t.sensor_id = r.str(20);
+    r.get<int32_t>();
+    r.get<int32_t>();
+    t.sensor_type = r.get<int32_t>();
+    r.done("ODPS sensor info");
+  }
+  {
+    Cursor r(rec.next());
+    t.sensor_channel = r.arr<int32_t>(t.L);
+    r.done("ODPS channels");
+  }
+  {
+    Cursor r(rec.next());
+    t.component_id = r.arr<int32_t>(t.J);
+    r.done("ODPS component ids");
+  }
+  {
+    Cursor r(rec.next());
+    t.absorber_id = r.arr<int32_t>(t.Jm);
+    r.done("ODPS absorber ids");
+  }
+  {
+    Cursor r(rec.next());
+    t.ref_lp = r.arr<double>(t.Ko + 1);
+    t.ref_p = r.arr<double>(t.Ko);
+    t.ref_t = r.arr<double>(t.Ko);
+    t.ref_a = r.arr<double>((size_t)t.Ko * t.Jm);
+    t.min_a = r.arr<double>((size_t)t.Ko * t.Jm);
+    t.max_a = r.arr<double>((size_t)t.Ko * t.Jm);
+    r.done("ODPS reference profiles");
+  }
+  {
+    Cursor r(rec.next());
+    t.n_pred = r.arr<int32_t>((size_t)t.J * t.L);
+    t.pos = r.arr<int32_t>((size_t)t.J * t.L);
+    r.done("ODPS n_Predictors/Pos_Index");
+  }
+  if (t.n_coeffs > 0) {
+    Cursor r(rec.next());
+    t.c = r.arr<float>(t.n_coeffs);
+    r.done("ODPS coefficients");
+  }
+  if (t.n_ocoeffs > 0) {
+    Cursor r(rec.next());
+    t.osig = r.arr<int32_t>(t.L);
+    t.order = r.arr<int32_t>(t.L);
+    t.op_index = r.arr<int32_t>((size_t)(t.n_opindex + 1) * t.L);
+    t.opos = r.arr<int32_t>(t.L);
+    t.oc = r.arr<double>(t.n_ocoeffs);
+    t.alpha = r.get<double>();
+    t.alpha_c1 = r.get<double>();
+    t.alpha_c2 = r.get<double>();
+    t.ocomp = r.get<int32_t>();
+    r.done("ODPS OPTRAN block");
+  }
+  return t;
+}
+
+// ---------------------------------------------------------------------------
+// Device memory
+// ---------------------------------------------------------------------------
+struct DevBuf {
+  void* p = nullptr;
+  size_t n = 0;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  void reserve(size_t bytes) {
+    if (bytes <= n) return;
+    if (p) HIPCHECK(hipFree(p));
+    p = nullptr;
+    n = 0;
+    HIPCHECK(hipMalloc(&p, bytes));
+    n = bytes;
+  }
+  template <class T> T* as() const { return reinterpret_cast<T*>(p); }
+};
+
+template <class T> void upload(DevBuf& b, const std::vector<T>& v) {
+  b.reserve(std::max<size_t>(v.size() * sizeof(T), 16));
+  if (!v.empty()) HIPCHECK(hipMemcpy(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+}
+
+struct Sensor {
+  OdpsH tc;
+  SpcCoeffH sc;
+  DevSensor d{};
+  int n_blocks = 0;
+  std::vector<std::unique_ptr<DevBuf>> bufs;
+  template <class T> const T* put(const std::vector<T>& v) {
+    bufs.emplace_back(new DevBuf);
+    upload(*bufs.back(), v);
+    return bufs.back()->as<T>();
+  }
+};
+
+const double R_DRYAIR_H = 8.314472 / (28.9648 * 1.0e-3);
+const double CCH_H = 0.001 * R_DRYAIR_H / 9.80665;
+const double CV_H = (1.0 / (18.01528 / 28.9648) - 1.0) / 1000.0;
+
+void build_sensor(Sensor& S, int chan_offset) {
+  const OdpsH& t = S.tc;
+  const SpcCoeffH& sc = S.sc;
+  if (t.L != sc.n_channels)
+    throw CrtmError(fmt("%s: TauCoeff has %d channels, SpcCoeff %d", t.sensor_id.c_str(), t.L, sc.n_channels));
+  if (t.group < 1 || t.group > 3) throw CrtmError(fmt("unsupported ODPS group %d", t.group));
+  static const int NCOMP[4] = {0, 8, 5, 2}, NABS[4] = {0, 6, 3, 1};
+  static const int NPMAX[4][8] = {{0}, {7, 18, 7, 11, 11, 14, 10, 11}, {7, 15, 7, 11, 10}, {7, 14}};
+  if (t.J != NCOMP[t.group] || t.Jm != NABS[t.group])
+    throw CrtmError(fmt("ODPS group %d expects %d components / %d absorbers, file has %d / %d", t.group,
+                        NCOMP[t.group], NABS[t.group], t.J, t.Jm));
+  if (t.Ko > KO_MAX) throw CrtmError(fmt("ODPS layers %d exceed %d", t.Ko, KO_MAX));
+  const int Ko = t.Ko, J = t.J, Jm = t.Jm, L = t.L;
+  DevSensor& d = S.d;
+  d.group = t.group;
+  d.Ko = Ko;
+  d.J = J;
+  d.Jm = Jm;
+  d.L = L;
+  d.chan_offset = chan_offset;
+  // Map_Input resets the H2O index on every absorber pass (ODPS_CoordinateMapping.f90:253-258)
+  int h2o_idx = 1;
+  for (int j = 0; j < Jm; ++j) {
+    h2o_idx = 1;
+    if (t.absorber_id[j] == 1) h2o_idx = j + 1;
+  }
+  d.h2o_idx = h2o_idx;
+  // Geopotential_Height of the reference profile (ODPS_CoordinateMapping.f90:607-634)
+  std::vector<double> z(Ko + 1);
+  z[Ko] = 0.0;
+  for (int k = Ko; k >= 1; --k) {
+    double tv = t.ref_t[k - 1] * (1.0 + CV_H * t.ref_a[(size_t)(h2o_idx - 1) * Ko + k - 1]);
+    z[k - 1] = z[k] + CCH_H * tv * std::log(t.ref_lp[k] / t.ref_lp[k - 1]);
+  }
+  // integrated reference sums (ODPS_Predictor.f90:707-759)
+  std::vector<double> pdp(Ko), tzr(Ko), tzpr(Ko), gazpr((size_t)Jm * Ko), gatzpr((size_t)Jm * Ko);
+  {
+    double a = 0, b = 0;
+    std::vector<double> g1(Jm, 0.0), g2(Jm, 0.0);
+    for (int k = 1; k <= Ko; ++k) {
+      double p = (k == 1) ? t.ref_lp[0] * (t.ref_lp[1] - t.ref_lp[0]) : t.ref_lp[k] * (t.ref_lp[k] - t.ref_lp[k - 1]);
+      pdp[k - 1] = p;
+      a += t.ref_t[k - 1];
+      b += p * t.ref_t[k - 1];
+      tzr[k - 1] = a;
+      tzpr[k - 1] = b;
+      for (int j = 0; j < Jm; ++j) {
+        double ra = t.ref_a[(size_t)j * Ko + k - 1];
+        g1[j] += p * ra;
+        g2[j] += p * ra * t.ref_t[k - 1];
+        gazpr[(size_t)j * Ko + k - 1] = g1[j];
+        gatzpr[(size_t)j * Ko + k - 1] = g2[j];
+      }
+    }
+  }
+  // coefficient block layout: [block][k][slot][lane]
+  const int nb = (L + WAVE - 1) / WAVE;
+  S.n_blocks = nb;
+  d.n_blocks = nb;
+  const bool optran = t.n_ocoeffs > 0;
+  const int ocj = t.ocomp - 1;
+  std::vector<int> lane_opt((size_t)nb * WAVE, 0);
+  if (optran)
+    for (int l = 0; l < L; ++l)
+      lane_opt[l] = (t.osig[l] == 1 && ocj >= 0 && ocj < J && t.n_pred[ocj + (size_t)J * l] > 0 &&
+                     t.op_index[(size_t)(t.n_opindex + 1) * l] > 0);
+  std::vector<int> meta((size_t)nb * BM_N, 0);
+  std::vector<int64_t> base(nb), obase(nb, -1);
+  int64_t total = 0, ototal = 0;
+  for (int b = 0; b < nb; ++b) {
+    int* mt = &meta[(size_t)b * BM_N];
+    int nsl = 0;
+    for (int j = 0; j < J; ++j) {
+      int mx = 0;
+      for (int l = b * WAVE; l < std::min(L, (b + 1) * WAVE); ++l) {
+        int np = t.n_pred[j + (size_t)J * l];
+        if (lane_opt[l] && j == ocj) np = 0;
+        if (np > NPMAX[t.group][j])
+          throw CrtmError(fmt("channel %d component %d has %d predictors (max %d)", l + 1, j + 1, np,
+                              NPMAX[t.group][j]));
+        mx = std::max(mx, np);
+      }
+      mt[BM_NP0 + j] = mx;
+      mt[BM_OFF0 + j] = nsl;
+      nsl += mx;
+    }
+    mt[BM_NSL] = nsl;
+    base[b] = total;
+    total += (int64_t)Ko * nsl * WAVE;
+    bool any = false;
+    int omax = 0, pmask = 0;
+    for (int l = b * WAVE; l < std::min(L, (b + 1) * WAVE); ++l)
+      if (lane_opt[l]) {
+        any = true;
+        omax = std::max(omax, t.order[l]);
+        pmask |= 1;
+        const int np = t.op_index[(size_t)(t.n_opindex + 1) * l];
+        for (int i = 1; i <= np; ++i) pmask |= 1 << t.op_index[(size_t)(t.n_opindex + 1) * l + i];
+      }
+    if (any) {
+      if (omax > MAX_OPTRAN_ORDER) throw CrtmError("OPTRAN order exceeds 10");
+      obase[b] = ototal;
+      ototal += (int64_t)OPZ * OPO * WAVE;
+      mt[BM_OPMASK] = pmask;
+      mt[BM_OMAXO] = omax;
+    }
+  }
+  std::vector<float> cblk(std::max<int64_t>(total, 1), 0.0f);
+  for (int b = 0; b < nb; ++b) {
+    const int* mt = &meta[(size_t)b * BM_N];
+    const int nsl = mt[BM_NSL];
+    for (int l = b * WAVE; l < std::min(L, (b + 1) * WAVE); ++l) {
+      const int lane = l - b * WAVE;
+      for (int j = 0; j < J; ++j) {
+        if (lane_opt[l] && j == ocj) continue;
+        const int np = t.n_pred[j + (size_t)J * l];
+        const int64_t p0 = t.pos[j + (size_t)J * l] - 1;   // 1-based Pos_Index
+        for (int i = 0; i < np; ++i)
+          for (int k = 0; k < Ko; ++k) {
+            const int64_t src = p0 + (int64_t)i * Ko + k;
+            if (src < 0 || src >= t.n_coeffs) throw CrtmError("Pos_Index out of range");
+            cblk[base[b] + ((int64_t)k * nsl + mt[BM_OFF0 + j] + i) * WAVE + lane] = t.c[src];
+          }
+      }
+    }
+  }
+  std::vector<double> ocd(std::max<int64_t>(ototal, 1), 0.0);
+  if (optran)
+    for (int l = 0; l < L; ++l) {
+      if (!lane_opt[l]) continue;
+      const int b = l / WAVE, lane = l % WAVE;
+      const int np = t.op_index[(size_t)(t.n_opindex + 1) * l];
+      const int no = t.order[l];
+      for (int i = 0; i <= np; ++i) {
+        const int p = i == 0 ? 0 : t.op_index[(size_t)(t.n_opindex + 1) * l + i];
+        const int64_t js = t.opos[l] - 1 + (int64_t)i * (no + 1);
+        for (int q = 0; q <= no; ++q) {
+          if (js + q < 0 || js + q >= t.n_ocoeffs) throw CrtmError("OPos_Index out of range");
+          ocd[obase[b] + ((int64_t)p * OPO + q) * WAVE + lane] += t.oc[js + q];
+        }
+      }
+    }
+  d.optran = optran ? 1 : 0;
+  d.ocomp = t.ocomp;
+  d.alpha = t.alpha;
+  d.alpha_c1 = t.alpha_c1;
+  d.alpha_c2 = t.alpha_c2;
+  d.ref_lp = S.put(t.ref_lp);
+  d.ref_p = S.put(t.ref_p);
+  d.ref_t = S.put(t.ref_t);
+  d.ref_a = S.put(t.ref_a);
+  d.min_a = S.put(t.min_a);
+  d.max_a = S.put(t.max_a);
+  d.z_ref = S.put(z);
+  d.pdp = S.put(pdp);
+  d.tz_ref = S.put(tzr);
+  d.tzp_ref = S.put(tzpr);
+  d.gazp_ref = S.put(gazpr);
+  d.gatzp_ref = S.put(gatzpr);
+  d.abs_id = S.put(t.absorber_id);
+  d.blk_base = S.put(base);
+  d.blk_meta = S.put(meta);
+  d.cblk = S.put(cblk);
+  d.oblk_base = S.put(obase);
+  d.ocd = S.put(ocd);
+  d.lane_optran = S.put(lane_opt);
+  d.c1 = S.put(sc.c1);
+  d.c2 = S.put(sc.c2);
+  d.bc1 = S.put(sc.bc1);
+  d.bc2 = S.put(sc.bc2);
+  d.cosmic = S.put(sc.cosmic);
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Context
+// ---------------------------------------------------------------------------
+struct crtm_context {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::vector<std::unique_ptr<Sensor>> sensors;
+  int n_channels = 0;
+  std::string last_error;
+  // work pool (grown on demand, reused across calls)
+  DevBuf wx, wu, wo, wacci, waccw, wuidx, wufrac, wgeo, wstatus;
+  DevBuf stage;  // host-mode staging of inputs/outputs
+};
+
+namespace {
+
+int fail(crtm_context* ctx, const std::string& msg) {
+  if (ctx) ctx->last_error = msg;
+  std::fprintf(stderr, "CRTM_MI355X: %s\n", msg.c_str());
+  return CRTM_FAILURE;
+}
+
+template <class RT>
+ProfWork<RT> make_work(crtm_context* c, int M, int Ko, int nx, int max_k) {
+  ProfWork<RT> w{};
+  c->wx.reserve((size_t)M * Ko * nx * sizeof(RT));
+  c->wu.reserve((size_t)M * Ko * NU * sizeof(RT));
+  c->wo.reserve((size_t)M * Ko * NO * sizeof(RT));
+  c->wacci.reserve((size_t)M * Ko * 2 * sizeof(int));
+  c->waccw.reserve((size_t)M * Ko * MAXW * sizeof(RT));
+  c->wuidx.reserve((size_t)M * (max_k + 1) * sizeof(int));
+  c->wufrac.reserve((size_t)M * (max_k + 1) * sizeof(RT));
+  c->wgeo.reserve((size_t)M * 4 * sizeof(double));
+  c->wstatus.reserve((size_t)M * sizeof(int));
+  w.x = c->wx.as<RT>();
+  w.u = c->wu.as<RT>();
+  w.o = c->wo.as<RT>();
+  w.acc_i = c->wacci.as<int>();
+  w.acc_w = c->waccw.as<RT>();
+  w.uidx = c->wuidx.as<int>();
+  w.ufrac = c->wufrac.as<RT>();
+  w.geo = c->wgeo.as<double>();
+  w.status = c->wstatus.as<int>();
+  return w;
+}
+
+template <class RT, class KT, bool DOK>
+void run_sensor(crtm_context* c, Sensor& S, const DevBatch& b, const DevOut& o) {
+  const int nx = S.d.group == 1 ? 89 : S.d.group == 2 ? 50 : 21;
+  ProfWork<RT> w = make_work<RT>(c, b.M, S.d.Ko, nx, b.max_k);
+  DevBatch bb = b;
+  for (int j = 0; j < NA_MAX; ++j) bb.jmap[j] = -1;
+  for (int j = 0; j < S.d.Jm; ++j)
+    for (int ju = 0; ju < b.na; ++ju)
+      if (S.tc.absorber_id[j] == b.jmap[ju]) {   // b.jmap carries the user absorber ids here
+        bb.jmap[j] = ju;
+        break;
+      }
+  switch (S.d.group) {
+    case 1: launch_prep<1, RT>(S.d, bb, w, c->stream); launch_fwdk<1, RT, KT, DOK>(S.d, bb, w, o, c->stream); break;
+    case 2: launch_prep<2, RT>(S.d, bb, w, c->stream); launch_fwdk<2, RT, KT, DOK>(S.d, bb, w, o, c->stream); break;
+    case 3: launch_prep<3, RT>(S.d, bb, w, c->stream); launch_fwdk<3, RT, KT, DOK>(S.d, bb, w, o, c->stream); break;
+  }
+  HIPCHECK(hipGetLastError());
+}
+
+// Host-mode staging: copy every input array into one device block.
+struct Stager {
+  std::vector<std::pair<const void*, size_t>> ins;
+  std::vector<std::pair<void*, size_t>> outs;
+  std::vector<size_t> in_off, out_off;
+  size_t total = 0;
+  size_t add(size_t bytes) {
+    size_t o = total;
+    total += (bytes + 255) & ~size_t(255);
+    return o;
+  }
+};
+
+int do_call(crtm_context* ctx, const crtm_atmosphere* atm, const crtm_surface* sfc, const crtm_geometry* geo,
+            const crtm_options* opt, crtm_rtsolution* rts, crtm_k_output* k) {
+  if (!ctx) return fail(nullptr, "null context");
+  if (!atm || !sfc || !geo || !opt || !rts) return fail(ctx, "null argument");
+  if (!opt->use_emissivity || !opt->emissivity)
+    return fail(ctx, "surface optics models are not part of this library: set Options%Use_Emissivity");
+  const int M = atm->n_profiles;
+  if (M <= 0) return CRTM_SUCCESS;
+  if (atm->n_absorbers < 1 || atm->n_absorbers > NA_MAX)
+    return fail(ctx, fmt("n_absorbers %d outside 1..%d", atm->n_absorbers, NA_MAX));
+  if (!rts->radiance || !rts->brightness_temperature) return fail(ctx, "RTSolution radiance/BT arrays required");
+  const bool host = opt->memory == CRTM_MEM_HOST;
+  const bool split = opt->precision == CRTM_PREC_FP32_SPLIT;
+  const bool kf32 = opt->jacobian_f32 != 0;
+  const int L = ctx->n_channels;
+  HIPCHECK(hipSetDevice(ctx->device));
+  // layer bookkeeping needs host copies of n_layers / offsets
+  std::vector<int> nl(M);
+  std::vector<int64_t> off(M + 1);
+  if (host) {
+    std::memcpy(nl.data(), atm->n_layers, M * sizeof(int));
+    std::memcpy(off.data(), atm->layer_offset, (M + 1) * sizeof(int64_t));
+  } else {
+    HIPCHECK(hipMemcpyAsync(nl.data(), atm->n_layers, M * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHECK(hipMemcpyAsync(off.data(), atm->layer_offset, (M + 1) * sizeof(int64_t), hipMemcpyDeviceToHost,
+                            ctx->stream));
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+  }
+  int max_k = 0;
+  for (int m = 0; m < M; ++m) {
+    if (nl[m] < 1 || nl[m] > CRTM_MAX_N_LAYERS) return fail(ctx, fmt("profile %d: invalid n_layers %d", m + 1, nl[m]));
+    if (off[m + 1] - off[m] != nl[m]) return fail(ctx, fmt("profile %d: layer_offset inconsistent", m + 1));
+    max_k = std::max(max_k, nl[m]);
+  }
+  const int64_t NT = off[M];
+  const int na = atm->n_absorbers;
+  std::vector<int> abs_id(na);
+  if (host) std::memcpy(abs_id.data(), atm->absorber_id, na * sizeof(int));
+  else HIPCHECK(hipMemcpy(abs_id.data(), atm->absorber_id, na * sizeof(int), hipMemcpyDeviceToHost));
+  if (opt->check_input && host) {
+    for (int m = 0; m < M; ++m) {
+      const double* lp = atm->level_pressure + off[m] + m;
+      for (int kk = 0; kk < nl[m]; ++kk)
+        if (!(lp[kk + 1] > lp[kk]) || !(atm->temperature[off[m] + kk] > 0.0))
+          return fail(ctx, fmt("Input data check failed for profile #%d", m + 1));
+    }
+  }
+  const size_t szK = kf32 ? 4 : 8;
+  if (rts->status && host)
+    for (int m = 0; m < M; ++m) rts->status[m] = CRTM_SUCCESS;
+  DevBatch b{};
+  b.M = M;
+  b.na = na;
+  b.max_k = max_k;
+  b.NT = NT;
+  b.Ltot = L;
+  for (int j = 0; j < NA_MAX; ++j) b.jmap[j] = j < na ? abs_id[j] : -1;
+  DevOut o{};
+  o.k_f32 = kf32;
+  const bool dok = k != nullptr;
+  if (host) {
+    // stage inputs
+    struct In { const void* h; size_t n; const void** d; };
+    std::vector<In> ins;
+    auto in = [&](const void* h, size_t n, const void** d) { if (h) ins.push_back({h, n, d}); else *d = nullptr; };
+    in(atm->n_layers, M * sizeof(int), (const void**)&b.n_layers);
+    in(atm->layer_offset, (M + 1) * sizeof(int64_t), (const void**)&b.loff);
+    in(atm->level_pressure, (NT + M) * sizeof(double), (const void**)&b.level_p);
+    in(atm->pressure, NT * sizeof(double), (const void**)&b.p);
+    in(atm->temperature, NT * sizeof(double), (const void**)&b.t);
+    in(atm->absorber, na * NT * sizeof(double), (const void**)&b.a);
+    const double* cov[4] = {sfc->land_coverage, sfc->water_coverage, sfc->snow_coverage, sfc->ice_coverage};
+    const double* st[4] = {sfc->land_temperature, sfc->water_temperature, sfc->snow_temperature, sfc->ice_temperature};
+    for (int c = 0; c < 4; ++c) {
+      in(cov[c], M * sizeof(double), (const void**)&b.cov[c]);
+      in(st[c], M * sizeof(double), (const void**)&b.sfc_t[c]);
+    }
+    in(geo->sensor_zenith_angle, M * sizeof(double), (const void**)&b.zenith);
+    in(geo->surface_altitude, M * sizeof(double), (const void**)&b.altitude);
+    in(opt->emissivity, (size_t)M * L * sizeof(double), (const void**)&b.emissivity);
+    if (dok) {
+      in(k->brightness_temperature_k, (size_t)M * L * sizeof(double), (const void**)&o.bt_k);
+      in(k->radiance_k, (size_t)M * L * sizeof(double), (const void**)&o.rad_k);
+    }
+    struct Out { void* h; size_t n; void** d; };
+    std::vector<Out> outs;
+    auto out = [&](void* h, size_t n, void** d) { if (h) outs.push_back({h, n, d}); else *d = nullptr; };
+    const size_t ML = (size_t)M * L * sizeof(double);
+    out(rts->radiance, ML, (void**)&o.radiance);
+    out(rts->brightness_temperature, ML, (void**)&o.bt);
+    out(rts->up_radiance, ML, (void**)&o.up_radiance);
+    out(rts->down_radiance, ML, (void**)&o.down_radiance);
+    out(rts->surface_planck_radiance, ML, (void**)&o.sfc_planck);
+    out(rts->surface_emissivity, ML, (void**)&o.sfc_emis);
+    if (dok) {
+      out(k->radiance_k, ML, (void**)&o.rad_k);
+      out(k->surface_emissivity_k, ML, (void**)&o.emis_k);
+      double* stk[4] = {k->land_temperature_k, k->water_temperature_k, k->snow_temperature_k, k->ice_temperature_k};
+      for (int c = 0; c < 4; ++c) out(stk[c], ML, (void**)&o.sfc_t_k[c]);
+      out(k->temperature_k, (size_t)NT * L * szK, (void**)&o.t_k);
+      out(k->absorber_k, (size_t)na * NT * L * szK, (void**)&o.a_k);
+    }
+    // rad_k is both input and output: give it one device slot
+    size_t total = 0;
+    std::vector<size_t> ioff, ooff;
+    for (auto& x : ins) { ioff.push_back(total); total += (x.n + 255) & ~size_t(255); }
+    for (auto& x : outs) {
+      if (dok && x.h == (void*)k->radiance_k) { ooff.push_back(SIZE_MAX); continue; }
+      ooff.push_back(total);
+      total += (x.n + 255) & ~size_t(255);
+    }
+    ctx->stage.reserve(std::max<size_t>(total, 256));
+    char* base = ctx->stage.as<char>();
+    for (size_t i = 0; i < ins.size(); ++i) {
+      HIPCHECK(hipMemcpyAsync(base + ioff[i], ins[i].h, ins[i].n, hipMemcpyHostToDevice, ctx->stream));
+      *ins[i].d = base + ioff[i];
+    }
+    for (size_t i = 0; i < outs.size(); ++i) {
+      if (ooff[i] == SIZE_MAX) { ooff[i] = (char*)o.rad_k - base; continue; }
+      *outs[i].d = base + ooff[i];
+    }
+    if (dok && !k->radiance_k) o.rad_k = nullptr;
+    for (auto& S : ctx->sensors) {
+      if (dok) {
+        if (split) { if (kf32) run_sensor<float, float, true>(ctx, *S, b, o); else run_sensor<float, double, true>(ctx, *S, b, o); }
+        else { if (kf32) run_sensor<double, float, true>(ctx, *S, b, o); else run_sensor<double, double, true>(ctx, *S, b, o); }
+      } else {
+        if (split) run_sensor<float, float, false>(ctx, *S, b, o); else run_sensor<double, double, false>(ctx, *S, b, o);
+      }
+      if (rts->status) {
+        std::vector<int> st(M);
+        HIPCHECK(hipMemcpyAsync(st.data(), ctx->wstatus.p, M * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHECK(hipStreamSynchronize(ctx->stream));
+        for (int m = 0; m < M; ++m) rts->status[m] = std::max(rts->status[m], st[m] ? CRTM_FAILURE : CRTM_SUCCESS);
+      }
+    }
+    for (size_t i = 0; i < outs.size(); ++i)
+      HIPCHECK(hipMemcpyAsync(outs[i].h, base + ooff[i], outs[i].n, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHECK(hipStreamSynchronize(ctx->stream));
+    if (rts->status)
+      for (int m = 0; m < M; ++m)
+        if (rts->status[m] != CRTM_SUCCESS) return fail(ctx, fmt("profile #%d failed", m + 1));
+    return CRTM_SUCCESS;
+  }
+  // device mode: pointers are used as given, asynchronously on the context stream
+  b.n_layers = atm->n_layers;
+  b.loff = atm->layer_offset;
+  b.level_p = atm->level_pressure;
+  b.p = atm->pressure;
+  b.t = atm->temperature;
+  b.a = atm->absorber;
+  b.cov[0] = sfc->land_coverage; b.cov[1] = sfc->water_coverage; b.cov[2] = sfc->snow_coverage; b.cov[3] = sfc->ice_coverage;
+  b.sfc_t[0] = sfc->land_temperature; b.sfc_t[1] = sfc->water_temperature;
+  b.sfc_t[2] = sfc->snow_temperature; b.sfc_t[3] = sfc->ice_temperature;
+  for (int c = 0; c < 4; ++c)
+    if (!b.cov[c] || !b.sfc_t[c]) return fail(ctx, "surface coverage/temperature arrays required");
+  b.zenith = geo->sensor_zenith_angle;
+  b.altitude = geo->surface_altitude;
+  b.emissivity = opt->emissivity;
+  o.radiance = rts->radiance;
+  o.bt = rts->brightness_temperature;
+  o.up_radiance = rts->up_radiance;
+  o.down_radiance = rts->down_radiance;
+  o.sfc_planck = rts->surface_planck_radiance;
+  o.sfc_emis = rts->surface_emissivity;
+  if (dok) {
+    o.bt_k = k->brightness_temperature_k;
+    o.rad_k = k->radiance_k;
+    o.emis_k = k->surface_emissivity_k;
+    o.sfc_t_k[0] = k->land_temperature_k;
+    o.sfc_t_k[1] = k->water_temperature_k;
+    o.sfc_t_k[2] = k->snow_temperature_k;
+    o.sfc_t_k[3] = k->ice_temperature_k;
+    o.t_k = k->temperature_k;
+    o.a_k = k->absorber_k;
+    if (!o.t_k || !o.a_k) return fail(ctx, "K-matrix temperature/absorber arrays required");
+  }
+  for (auto& S : ctx->sensors) {
+    if (dok) {
+      if (split) { if (kf32) run_sensor<float, float, true>(ctx, *S, b, o); else run_sensor<float, double, true>(ctx, *S, b, o); }
+      else { if (kf32) run_sensor<double, float, true>(ctx, *S, b, o); else run_sensor<double, double, true>(ctx, *S, b, o); }
+    } else {
+      if (split) run_sensor<float, float, false>(ctx, *S, b, o); else run_sensor<double, double, false>(ctx, *S, b, o);
+    }
+    if (rts->status)
+      HIPCHECK(hipMemcpyAsync(rts->status, ctx->wstatus.p, M * sizeof(int), hipMemcpyDeviceToDevice, ctx->stream));
+  }
+  return CRTM_SUCCESS;
+}
+
+}  // namespace
+
+// ===========================================================================
+// C-ABI
+// ===========================================================================
+extern "C" {
+
+int crtm_init(int n_sensors, const char* const* sensor_ids, const char* file_path, crtm_context** out) {
+  if (!out) return CRTM_FAILURE;
+  *out = nullptr;
+  std::unique_ptr<crtm_context> ctx(new crtm_context);
+  try {
+    HIPCHECK(hipGetDevice(&ctx->device));
+    HIPCHECK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    std::string dir = file_path ? file_path : ".";
+    if (!dir.empty() && dir.back() != '/') dir += '/';
+    int off = 0;
+    for (int i = 0; i < n_sensors; ++i) {
+      std::unique_ptr<Sensor> S(new Sensor);
+      S->sc = read_spccoeff(dir + sensor_ids[i] + ".SpcCoeff.bin");
+      S->tc = read_odps(dir + sensor_ids[i] + ".TauCoeff.bin");
+      build_sensor(*S, off);
+      off += S->tc.L;
+      ctx->sensors.push_back(std::move(S));
+    }
+    ctx->n_channels = off;
+    HIPCHECK(hipDeviceSynchronize());
+  } catch (const std::exception& e) {
+    std::fprintf(stderr, "CRTM_MI355X crtm_init: %s\n", e.what());
+    return CRTM_FAILURE;
+  }
+  *out = ctx.release();
+  return CRTM_SUCCESS;
+}
+
+int crtm_destroy(crtm_context* ctx) {
+  if (!ctx) return CRTM_FAILURE;
+  (void)hipSetDevice(ctx->device);
+  if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+  ctx->sensors.clear();
+  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return CRTM_SUCCESS;
+}
+
+int crtm_n_sensors(const crtm_context* ctx) { return ctx ? (int)ctx->sensors.size() : 0; }
+int crtm_n_channels_total(const crtm_context* ctx) { return ctx ? ctx->n_channels : 0; }
+
+int crtm_channel_info(const crtm_context* ctx, int s, int* n_channels, int* sensor_type, int* sensor_channel,
+                      char* sensor_id) {
+  if (!ctx || s < 0 || s >= (int)ctx->sensors.size()) return CRTM_FAILURE;
+  const Sensor& S = *ctx->sensors[s];
+  if (n_channels) *n_channels = S.tc.L;
+  if (sensor_type) *sensor_type = S.sc.sensor_type;
+  if (sensor_channel) std::copy(S.sc.sensor_channel.begin(), S.sc.sensor_channel.end(), sensor_channel);
+  if (sensor_id) {
+    std::strncpy(sensor_id, S.sc.sensor_id.c_str(), 20);
+    sensor_id[20] = '\0';
+  }
+  return CRTM_SUCCESS;
+}
+
+int crtm_forward(crtm_context* ctx, const crtm_atmosphere* atm, const crtm_surface* sfc, const crtm_geometry* geo,
+                 const crtm_options* opt, crtm_rtsolution* rts) {
+  try {
+    return do_call(ctx, atm, sfc, geo, opt, rts, nullptr);
+  } catch (const std::exception& e) {
+    return fail(ctx, e.what());
+  }
+}
+
+int crtm_k_matrix(crtm_context* ctx, const crtm_atmosphere* atm, const crtm_surface* sfc, const crtm_geometry* geo,
+                  const crtm_options* opt, crtm_rtsolution* rts, crtm_k_output* k) {
+  if (!k) return fail(ctx, "null K-matrix output");
+  try {
+    return do_call(ctx, atm, sfc, geo, opt, rts, k);
+  } catch (const std::exception& e) {
+    return fail(ctx, e.what());
+  }
+}
+
+int crtm_synchronize(crtm_context* ctx) {
+  if (!ctx) return CRTM_FAILURE;
+  hipError_t e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) return fail(ctx, hipGetErrorString(e));
+  return CRTM_SUCCESS;
+}
+
+const char* crtm_last_error(const crtm_context* ctx) { return ctx ? ctx->last_error.c_str() : "null context"; }
+
+}  // extern "C"

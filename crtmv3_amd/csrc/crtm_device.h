@@ -1,0 +1,158 @@
+// crtm_device.h -- device-side data structures and ODPS group tables shared by
+// the profile-preparation and the fused forward/K-matrix kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace crtm {
+
+constexpr int WAVE = 64;           // CDNA wavefront width: one channel per lane
+constexpr int KO_MAX = 128;        // ODPS internal layers supported
+constexpr int K_MAX = 200;         // MAX_N_LAYERS (src/CRTM_Parameters.f90:64)
+constexpr int MAXW = 16;           // max user layers averaged into one ODPS layer
+constexpr int NA_MAX = 6;          // user absorbers carried through the adjoint
+constexpr int MAX_OPTRAN_ORDER = 10;
+constexpr int N_OPTRAN_PRED = 14;
+constexpr int OPZ = 15;            // OPTRAN predictor rows incl. constant row 0
+constexpr int OPO = 11;            // OPTRAN polynomial orders 0..10
+
+// ODPS predictor groups (src/AtmAbsorption/ODPS/ODPS_Predictor.f90:85-117)
+template <int G> struct GT;
+template <> struct GT<1> {
+  static constexpr int NCOMP = 8, NABS = 6, NX = 89;
+  static constexpr int NP[8] = {7, 18, 7, 11, 11, 14, 10, 11};
+  static constexpr int XOFF[8] = {0, 7, 25, 32, 43, 54, 68, 78};
+};
+template <> struct GT<2> {
+  static constexpr int NCOMP = 5, NABS = 3, NX = 50;
+  static constexpr int NP[5] = {7, 15, 7, 11, 10};
+  static constexpr int XOFF[5] = {0, 7, 22, 29, 40};
+};
+template <> struct GT<3> {
+  static constexpr int NCOMP = 2, NABS = 1, NX = 21;
+  static constexpr int NP[2] = {7, 14};
+  static constexpr int XOFF[2] = {0, 7};
+};
+constexpr int NX_MAX = 89;
+constexpr int NCOMP_MAX = 8;
+
+// Per-layer uniform table used by the hand-written predictor adjoint.
+// Values are channel-independent and read as wave-uniform scalars.
+enum UIdx {
+  U_SEC = 0, U_T, U_IT, U_IT2, U_IT4, U_DT, U_ADT, U_DT2,
+  U_H2O, U_H2OA, U_H2OR, U_H2OS, U_H2OR4, U_GT0, U_H2ODTZP, U_H2OP,
+  U_O3, U_O3A, U_O3R, U_O3P, U_SP, U_SQSO3P, U_TZP, U_ST, U_CO2P,
+  U_CO, U_N2O, U_CH4, U_N2OA, U_N2OR, U_COA, U_COR, U_COZP, U_COQ, U_CH4A, U_CH4R,
+  U_CH4ZP, U_CH4Q, U_N2OZP, U_SQCOR, U_SQCH4R, U_N2OPOW, U_SQCOZP,
+  U_IRT, U_PDP, U_ITZREF, U_ITZPREF, U_TODPS,
+  U_IRA0,                 // 6 entries: 1/ref_absorber(j) (0 where the AD is killed)
+  U_IGAZP0 = U_IRA0 + 6,  // 6 entries: 1/GAzp_ref(j)
+  U_IGATZP0 = U_IGAZP0 + 6,
+  U_AODPS0 = U_IGATZP0 + 6,
+  U_MASK0 = U_AODPS0 + 6, // 6 entries: 1 if the absorber adjoint passes Map_Input_AD
+  NU = U_MASK0 + 6
+};
+
+// Per-layer uniform OPTRAN table (ODPS_Compute_Predictor_ODAS and its AD)
+enum OIdx {
+  O_X0 = 0,               // OX(0..14), OX(0) = 1
+  O_AP0 = O_X0 + OPZ,     // Ap(0..10), Ap(0) = 1
+  O_DA = O_AP0 + OPO,
+  O_AP1, O_IDEN, O_INV, O_ST, O_SP, O_IIV2, O_DPSEC, O_PR, O_T, O_VAP, O_VT4,
+  NO = O_VT4 + 1
+};
+
+// Sensor resident in HBM (built by crtm_init from the coefficient files)
+struct DevSensor {
+  int group, Ko, J, Jm, L;
+  int n_blocks;
+  int chan_offset;          // first channel of this sensor in the context channel list
+  int h2o_idx;              // 1-based ODPS absorber index used for heights (reference quirk)
+  int optran;               // OPTRAN block present
+  int ocomp;                // OComponent_Index (1-based)
+  double alpha, alpha_c1, alpha_c2;
+  const double* ref_lp;     // [Ko+1]
+  const double* ref_p;      // [Ko]
+  const double* ref_t;      // [Ko]
+  const double* ref_a;      // [Jm][Ko]
+  const double* min_a;      // [Jm][Ko]
+  const double* max_a;      // [Jm][Ko]
+  const double* z_ref;      // [Ko+1] geopotential heights of the reference profile (Z(Ko)=0)
+  const double* pdp;        // [Ko]
+  const double* tz_ref;     // [Ko]
+  const double* tzp_ref;    // [Ko]
+  const double* gazp_ref;   // [Jm][Ko]
+  const double* gatzp_ref;  // [Jm][Ko]
+  const int* abs_id;        // [Jm]
+  // coefficients: per 64-channel block, [k][slot][lane]
+  const int64_t* blk_base;  // [n_blocks] float offset of the block
+  const int* blk_meta;      // [n_blocks][BM_N]
+  const float* cblk;
+  // OPTRAN per block: [p][o][lane] doubles, dense 15 x 11
+  const int64_t* oblk_base; // [n_blocks], -1 when the block has no OPTRAN lane
+  const double* ocd;
+  const int* lane_optran;   // [n_blocks*64] 1 if the lane's WLO OD comes from OPTRAN
+  // per channel (SpcCoeff)
+  const double* c1;
+  const double* c2;
+  const double* bc1;
+  const double* bc2;
+  const double* cosmic;
+};
+
+// blk_meta layout
+enum BMeta { BM_NSL = 0, BM_NP0 = 1, BM_OFF0 = BM_NP0 + NCOMP_MAX, BM_OPMASK = BM_OFF0 + NCOMP_MAX,
+             BM_OMAXO, BM_N };
+
+// Batch of profiles (device pointers)
+struct DevBatch {
+  int M;
+  int na;                   // user absorbers
+  int max_k;
+  const int* n_layers;      // [M]
+  const int64_t* loff;      // [M+1]
+  const double* level_p;
+  const double* p;
+  const double* t;
+  const double* a;          // [na][NT]
+  int64_t NT;               // total layers
+  const double* cov[4];     // [M] each: land, water, snow, ice coverage
+  const double* sfc_t[4];   // [M] each: land, water, snow, ice temperature
+  const double* zenith;     // [M]
+  const double* altitude;   // [M] or null
+  const double* emissivity; // [M][Ltot]
+  int Ltot;
+  int jmap[NA_MAX];         // ODPS absorber j -> user absorber index (or -1)
+};
+
+// Per-profile work produced by the preparation kernel, precision RT
+template <class RT> struct ProfWork {
+  RT* x;          // [M][KO_MAX? Ko][NX]
+  RT* u;          // [M][Ko][NU]
+  RT* o;          // [M][Ko][NO]
+  int* acc_i;     // [M][Ko][2]: first user layer (0-based), count
+  RT* acc_w;      // [M][Ko][MAXW]
+  int* uidx;      // [M][max_k+1]: lower ODPS level k1 (0-based); k2 = k1+1 unless flagged
+  RT* ufrac;      // [M][max_k+1]: interpolation fraction, or -1 when k1 == k2
+  double* geo;    // [M][4]: sec_sfc, cos_sensor, sfc_T, (unused)
+  int* status;    // [M]
+};
+
+struct DevOut {
+  double* radiance;        // [M][Ltot]
+  double* bt;
+  double* up_radiance;
+  double* down_radiance;
+  double* sfc_planck;
+  double* sfc_emis;
+  // K-matrix
+  const double* bt_k;      // [M][Ltot]
+  double* rad_k;           // [M][Ltot] in/out
+  double* emis_k;
+  double* sfc_t_k[4];
+  void* t_k;               // [NT][Ltot]
+  void* a_k;               // [na][NT][Ltot]
+  int k_f32;
+};
+
+}  // namespace crtm

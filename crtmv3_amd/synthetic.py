@@ -417,7 +417,7 @@ class ProfileSet:
 
     @property
     def n_profiles(self) -> int:
-        return int(self.n_layers.size)
+        return int(len(self.n_layers))
 
     def profile(self, m: int) -> Dict[str, np.ndarray]:
         a, b = int(self.layer_offset[m]), int(self.layer_offset[m + 1])

@@ -1,0 +1,168 @@
+/*
+ * crtm_mi355x.h -- C-ABI of the MI355X implementation of CRTM's forward and
+ * K-matrix hot path.  Plain pointers and sizes only; every entry point
+ * returns a CRTM status code.  This is the boundary a Fortran ISO_C_BINDING
+ * shim (crtmv3_amd/fortran/crtm_module.f90) or a Python ctypes binding
+ * (crtmv3_amd/api.py) binds.
+ *
+ * Layout conventions
+ *   - M profiles, each with K_m layers (top-down, index 0 = top of atmosphere)
+ *     and K_m+1 levels.  layer_offset[m] = sum_{i<m} K_i (length M+1).
+ *     Per-layer arrays have layer_offset[M] elements, per-level arrays
+ *     layer_offset[M] + M (level block of profile m starts at
+ *     layer_offset[m] + m).  absorber is [n_absorbers][layer_offset[M]].
+ *   - L = total channels over all sensors of the context (ChannelInfo order).
+ *     Per-channel outputs are [M][L].
+ *   - Jacobians (K-matrix) for profile m are stored channel-fastest,
+ *     [K_m][L] per variable; the block of profile m starts at
+ *     layer_offset[m] * L.  absorber_k is [n_absorbers] such blocks.
+ *   - memory = CRTM_MEM_HOST: all batch pointers are host pointers
+ *     (the library stages them through its device pool);
+ *     memory = CRTM_MEM_DEVICE: all batch pointers are device pointers on
+ *     the context's GPU and the call is asynchronous on the context stream
+ *     (crtm_synchronize waits).
+ */
+#ifndef CRTM_MI355X_H
+#define CRTM_MI355X_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Status codes: src/Utility/Message_Handler.f90:27-31 */
+#define CRTM_SUCCESS     0
+#define CRTM_INFORMATION 1
+#define CRTM_WARNING     2
+#define CRTM_FAILURE     3
+
+/* Sensor types: src/InstrumentInfo/SensorInfo/SensorInfo_Parameters.f90:74-78 */
+#define CRTM_INVALID_SENSOR     0
+#define CRTM_MICROWAVE_SENSOR   1
+#define CRTM_INFRARED_SENSOR    2
+#define CRTM_VISIBLE_SENSOR     3
+#define CRTM_ULTRAVIOLET_SENSOR 4
+
+/* Arithmetic of the call */
+#define CRTM_PREC_FP64       0  /* everything in FP64 */
+#define CRTM_PREC_FP32_SPLIT 1  /* FP32 absorption + RT, FP64 Planck / BT inversion */
+
+#define CRTM_MEM_HOST   0
+#define CRTM_MEM_DEVICE 1
+
+#define CRTM_MAX_N_LAYERS 200   /* src/CRTM_Parameters.f90:64 */
+
+typedef struct crtm_context crtm_context;
+
+/* ---- Atmosphere (src/Atmosphere/CRTM_Atmosphere_Define.f90:433-464) ---- */
+typedef struct {
+  int n_profiles;
+  const int* n_layers;             /* [M] */
+  const int64_t* layer_offset;     /* [M+1] */
+  int n_absorbers;
+  const int* absorber_id;          /* [n_absorbers], CRTM absorber ids (H2O=1, O3=3, ...) */
+  const int* absorber_units;       /* [n_absorbers] */
+  const double* level_pressure;    /* hPa, [layer_offset[M]+M] */
+  const double* pressure;          /* hPa, [layer_offset[M]] */
+  const double* temperature;       /* K */
+  const double* absorber;          /* [n_absorbers][layer_offset[M]] */
+} crtm_atmosphere;
+
+/* ---- Surface (src/Surface/CRTM_Surface_Define.f90:204-244): [M] each ---- */
+typedef struct {
+  const double* land_coverage;
+  const double* water_coverage;
+  const double* snow_coverage;
+  const double* ice_coverage;
+  const double* land_temperature;
+  const double* water_temperature;
+  const double* snow_temperature;
+  const double* ice_temperature;
+} crtm_surface;
+
+/* ---- Geometry (src/GeometryInfo/CRTM_Geometry_Define.f90:99-121): [M] ---- */
+typedef struct {
+  const double* sensor_zenith_angle;   /* degrees */
+  const double* sensor_scan_angle;     /* degrees, may be NULL */
+  const double* source_zenith_angle;   /* degrees, may be NULL (100 = no source) */
+  const double* surface_altitude;      /* metres, may be NULL */
+} crtm_geometry;
+
+/* ---- Options (src/Options/CRTM_Options_Define.f90:122-179) ---- */
+typedef struct {
+  int use_emissivity;            /* Options%Use_Emissivity */
+  const double* emissivity;      /* [M][L] when use_emissivity */
+  int precision;                 /* CRTM_PREC_* */
+  int memory;                    /* CRTM_MEM_* */
+  int jacobian_f32;              /* K outputs are float (1) or double (0) */
+  int check_input;               /* Options%Check_Input */
+} crtm_options;
+
+/* ---- RTSolution (src/RTSolution/CRTM_RTSolution_Define.f90:201-245) ---- */
+typedef struct {
+  double* radiance;                /* [M][L] */
+  double* brightness_temperature;  /* [M][L] */
+  double* up_radiance;             /* [M][L], optional */
+  double* down_radiance;           /* [M][L], optional */
+  double* surface_planck_radiance; /* [M][L], optional */
+  double* surface_emissivity;      /* [M][L], optional */
+  int* status;                     /* [M] per-profile status, optional */
+} crtm_rtsolution;
+
+/* ---- K-matrix adjoint inputs/outputs ---- */
+typedef struct {
+  /* RTSolution_K inputs ([M][L]); radiance_k is overwritten with the total
+   * radiance adjoint (Pre_Process_RTSolution_K, src/CRTM_K_Matrix_Module.f90:1725) */
+  const double* brightness_temperature_k;
+  double* radiance_k;
+  /* RTSolution_K%Surface_Emissivity output [M][L], optional */
+  double* surface_emissivity_k;
+  /* Atmosphere_K: layout above; element type per options.jacobian_f32.
+   * Overwritten (the reference accumulates into caller-zeroed arrays; the
+   * Fortran/Python layers zero-then-add to keep that contract). */
+  void* temperature_k;
+  void* absorber_k;
+  /* Surface_K temperatures [M][L] each, optional */
+  double* land_temperature_k;
+  double* water_temperature_k;
+  double* snow_temperature_k;
+  double* ice_temperature_k;
+} crtm_k_output;
+
+/* CRTM_Init (src/CRTM_LifeCycle.f90:520): loads <id>.SpcCoeff.bin and
+ * <id>.TauCoeff.bin (ODPS) for each sensor from file_path, uploads the
+ * coefficients to the current GPU and creates the context stream. */
+int crtm_init(int n_sensors, const char* const* sensor_ids, const char* file_path,
+              crtm_context** ctx);
+
+/* CRTM_Destroy (src/CRTM_LifeCycle.f90:1131) */
+int crtm_destroy(crtm_context* ctx);
+
+/* ChannelInfo (src/ChannelInfo/CRTM_ChannelInfo_Define.f90:70-85) */
+int crtm_n_sensors(const crtm_context* ctx);
+int crtm_n_channels_total(const crtm_context* ctx);
+int crtm_channel_info(const crtm_context* ctx, int sensor_index, int* n_channels, int* sensor_type,
+                      int* sensor_channel /* [n_channels] or NULL */,
+                      char* sensor_id /* >= 21 bytes or NULL */);
+
+/* CRTM_Forward (src/CRTM_Forward_Module.f90:229-245) */
+int crtm_forward(crtm_context* ctx, const crtm_atmosphere* atm, const crtm_surface* sfc,
+                 const crtm_geometry* geo, const crtm_options* opt, crtm_rtsolution* rts);
+
+/* CRTM_K_Matrix (src/CRTM_K_Matrix_Module.f90:300-325) */
+int crtm_k_matrix(crtm_context* ctx, const crtm_atmosphere* atm, const crtm_surface* sfc,
+                  const crtm_geometry* geo, const crtm_options* opt, crtm_rtsolution* rts,
+                  crtm_k_output* k);
+
+/* Wait for the context stream (device-memory calls are asynchronous). */
+int crtm_synchronize(crtm_context* ctx);
+
+/* Last error message of the context (static storage, NUL-terminated). */
+const char* crtm_last_error(const crtm_context* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CRTM_MI355X_H */
