@@ -1,0 +1,223 @@
+#!/usr/bin/env python3
+"""Forward + K-matrix throughput of the MI355X CRTM hot path.
+
+Workload (one step): clear-sky forward + K-matrix for a 2211-channel CrIS-like
+sensor (ODPS group 1 with OPTRAN water-vapour lines) over 4096 synthetic
+100-layer profiles, Jacobians dBT/dT, dBT/dq, dBT/dO3 per layer.  Inputs and
+outputs are resident in HBM for the timed region (device-memory C-ABI calls).
+
+Reported: FP32-split profile-channels/s (value), the plain FP64 GPU path
+measured on the same inputs, the BT / Jacobian differences between them, the
+roofline of the fused forward/K kernel measured with HIP events on the
+context stream, and the CPU baseline (the FP64 restatement in oracle/, timed
+on this host's cores on a bounded sample of the same workload).
+
+    python bench.py [--gpus 1] [--steps 5] [--warmup 2]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "forward+K-matrix profile-channels/s, FP32-split vs FP64 GPU, at 0.01 K BT err"
+PEAK_FP32_TFLOPS = 157.3     # MI355X FP32 (vector incl. dual issue / MFMA)
+PEAK_HBM_GBS = 8000.0
+C_PAD = 250                  # adjoint flops per ODPS layer of the group-1 predictor chain
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def algorithmic_flops_per_pc(pbar, ko, k):
+    """F_fwd = 2 P Ko + 10 Ko + 25 K;  F_K = 2 P Ko + c_pAD Ko + 70 K."""
+    return 4.0 * pbar * ko + (10 + C_PAD) * ko + 95.0 * k
+
+
+def algorithmic_bytes_per_pc(k, n_abs, kbytes):
+    # Jacobians (T + absorbers) + BT, radiance, emissivity-K, water-temperature-K
+    return (1 + n_abs) * k * kbytes + 4 * 8
+
+
+def cpu_baseline(sensor_name, coeff_dir, ps, emissivity, bt_gpu, seconds, threads):
+    """Time the oracle (FP64 CPU restatement, OpenMP over profiles) on a sample."""
+    from crtmv3_amd import coeffio, synthetic
+    from oracle import oracle as orc
+    tau = coeffio.read_odps(os.path.join(coeff_dir, f"{sensor_name}.TauCoeff.bin"))
+    spc = coeffio.read_spccoeff(os.path.join(coeff_dir, f"{sensor_name}.SpcCoeff.bin"))
+    s = orc.Sensor(tau, spc)
+    L = tau.n_channels
+    # calibrate on a small chunk, then size the sample to ~`seconds`
+    chunk = max(threads, 16)
+    sub = _subset(ps, 0, chunk)
+    t0 = time.perf_counter()
+    out = orc.k_matrix_batch(s, sub, emissivity[:chunk], n_threads=threads)
+    dt = time.perf_counter() - t0
+    n = min(ps.n_profiles, max(chunk, int(chunk * seconds / max(dt, 1e-6))))
+    sub = _subset(ps, 0, n)
+    t0 = time.perf_counter()
+    out = orc.k_matrix_batch(s, sub, emissivity[:n], n_threads=threads)
+    dt = time.perf_counter() - t0
+    bt_err = float(np.max(np.abs(out["bt"] - bt_gpu[:n])))
+    return {"value": n * L / dt, "unit": "profile-channels/s", "cores": threads, "kind": "port",
+            "sample": f"{n} of the {ps.n_profiles} profiles x {L} channels, forward+K-matrix, "
+                      f"FP64 oracle (oracle_k_matrix_batch), {dt:.1f} s"}, bt_err
+
+
+def _subset(ps, a, b):
+    import dataclasses
+    off = ps.layer_offset
+    la, lb = int(off[a]), int(off[b])
+    d = {}
+    for f in dataclasses.fields(ps):
+        v = getattr(ps, f.name)
+        if v is None:
+            d[f.name] = None
+        elif f.name == "layer_offset":
+            d[f.name] = off[a:b + 1] - off[a]
+        elif f.name == "level_pressure":
+            d[f.name] = v[la + a:lb + b]
+        elif f.name in ("pressure", "temperature"):
+            d[f.name] = v[la:lb]
+        elif f.name == "absorber":
+            d[f.name] = v[:, la:lb]
+        elif f.name in ("absorber_id", "absorber_units"):
+            d[f.name] = v
+        else:
+            d[f.name] = v[a:b]
+    return type(ps)(**d)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--profiles", type=int, default=4096)
+    ap.add_argument("--sensor", default="cris_syn")
+    ap.add_argument("--fp64-steps", type=int, default=2)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--split-only", action="store_true", help="skip the FP64 comparison leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: the MI355X path runs on a single device (profiles shard "
+              "across GPUs with no exchange; see DESIGN.md). Use --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from crtmv3_amd import api, synthetic
+
+    dev = torch.device("cuda:0")
+    coeff_dir = synthetic.default_coeff_dir()
+    synthetic.write_sensor(coeff_dir, args.sensor)
+    ctx = api.crtm_init([args.sensor], coeff_dir)
+    L = ctx.n_channels
+    M = args.profiles
+    K = 100
+    log(f"sensor {args.sensor}: {L} channels; generating {M} profiles")
+    ps = synthetic.make_profiles(M, K, seed=20261203 + 2)
+    dps = api.device_profiles(ps, dev)
+    st = ctx.channel_info[0].sensor_type
+    emis_h = np.tile(synthetic.default_emissivity(st, L), (M, 1))
+    emis = torch.as_tensor(emis_h, device=dev)
+    bt_k = torch.ones((M, L), dtype=torch.float64, device=dev)
+    torch.cuda.synchronize()
+
+    def run(prec, f32, out):
+        return ctx.k_matrix(dps, emis, bt_k=bt_k, precision=prec, jacobian_f32=f32, out=out)
+
+    # ---- FP32-split (the headline) ----
+    out = run(api.PREC_FP32_SPLIT, True, None)
+    for _ in range(args.warmup):
+        out = run(api.PREC_FP32_SPLIT, True, out)
+    ctx.synchronize()
+    torch.cuda.synchronize()
+    ctx.profile_enable(True)
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        out = run(api.PREC_FP32_SPLIT, True, out)
+    ctx.synchronize()
+    torch.cuda.synchronize()
+    t_split = (time.perf_counter() - t0) / args.steps
+    prep_ms, main_ms, nl = ctx.profile_query()
+    ctx.profile_enable(False)
+    main_ms /= max(nl, 1)
+    prep_ms /= max(nl, 1)
+    bt_split = out[0]["brightness_temperature"].cpu().numpy()
+    log(f"split: {t_split * 1e3:.2f} ms/step (fwd/K kernel {main_ms:.2f} ms, prep {prep_ms:.2f} ms)")
+
+    result = {}
+    if not args.split_only:
+        out64 = run(api.PREC_FP64, False, None)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.fp64_steps):
+            out64 = run(api.PREC_FP64, False, out64)
+        ctx.synchronize()
+        torch.cuda.synchronize()
+        t64 = (time.perf_counter() - t0) / args.fp64_steps
+        bt64 = out64[0]["brightness_temperature"].cpu().numpy()
+        bt_err = float(np.max(np.abs(bt_split - bt64)))
+        jac = {}
+        for key, name in (("temperature_k", "dT"), ("absorber_k", "dabs")):
+            a = out[1][key].double()
+            b = out64[1][key]
+            jac[name] = float((a - b).abs().max().item() / b.abs().max().item())
+        result["fp64"] = {"ms_per_step": t64 * 1e3, "profile_channels_per_s": M * L / t64,
+                          "speedup_split_over_fp64": t64 / t_split}
+        result["bt_err_split_vs_fp64_K"] = bt_err
+        result["jacobian_relerr_split_vs_fp64"] = jac
+        log(f"fp64: {t64 * 1e3:.2f} ms/step; BT err {bt_err:.2e} K; jac {jac}")
+        del out64
+        torch.cuda.empty_cache()
+
+    # ---- roofline of the fused forward/K kernel ----
+    from crtmv3_amd import coeffio
+    tau = coeffio.read_odps(os.path.join(coeff_dir, f"{args.sensor}.TauCoeff.bin"))
+    pbar = float(tau.n_predictors.sum(0).mean())
+    fpc = algorithmic_flops_per_pc(pbar, tau.n_layers, K)
+    bpc = algorithmic_bytes_per_pc(K, ps.absorber.shape[0], 4)
+    achieved = fpc * M * L / (main_ms * 1e-3) / 1e12
+    roof = {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+            "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+            "kernel": "clear_fwdk_kernel<1,float,float,true>", "kernel_ms": main_ms,
+            "flops_per_pc": fpc, "bytes_per_pc": bpc,
+            "hbm_achieved_GBs": bpc * M * L / (main_ms * 1e-3) / 1e9}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        try:
+            threads = min(16, os.cpu_count() or 1)
+            cpu, bt_oracle_err = cpu_baseline(args.sensor, coeff_dir, ps, emis_h, bt_split,
+                                              args.cpu_seconds, threads)
+            result["bt_err_split_vs_oracle_K"] = bt_oracle_err
+            log(f"cpu baseline: {cpu['value']:.3e} pc/s on {threads} threads; split-vs-oracle BT err "
+                f"{bt_oracle_err:.2e} K")
+        except Exception as e:  # the baseline is reported, never required
+            log(f"cpu baseline unavailable: {e}")
+
+    line = {"metric": METRIC, "value": M * L / t_split, "unit": "profile-channels/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": t_split * 1e3,
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32+fp64-split",
+            "data": "synthetic (seeded profiles; synthetic ODPS/SpcCoeff files in CRTM binary format)",
+            "config": {"workload": "clear-sky forward+K-matrix, CrIS-like 2211-ch ODPS group 1, "
+                                   f"{M} profiles x {K} layers, dBT/dT,dq,dO3",
+                       "n_profiles": M, "n_channels": L, "n_layers": K, "precision": "fp32-split",
+                       "parallelism": "single GPU"},
+            "roofline": roof, "cpu_baseline": cpu}
+    line.update(result)
+    print(json.dumps(line), flush=True)
+    ctx.destroy()
+
+
+if __name__ == "__main__":
+    main()

@@ -42,7 +42,8 @@ class CRTMError(RuntimeError):
 
 
 class CAtmosphere(ctypes.Structure):
-    _fields_ = [("n_profiles", ctypes.c_int), ("n_layers", _vp), ("layer_offset", _vp),
+    _fields_ = [("n_profiles", ctypes.c_int), ("max_layers", ctypes.c_int), ("total_layers", ctypes.c_int64),
+                ("n_layers", _vp), ("layer_offset", _vp),
                 ("n_absorbers", ctypes.c_int), ("absorber_id", _vp), ("absorber_units", _vp),
                 ("level_pressure", _vp), ("pressure", _vp), ("temperature", _vp), ("absorber", _vp)]
 
@@ -97,6 +98,8 @@ def lib():
         L.crtm_synchronize.argtypes = [_vp]
         L.crtm_last_error.argtypes = [_vp]
         L.crtm_last_error.restype = ctypes.c_char_p
+        L.crtm_profile_enable.argtypes = [_vp, ctypes.c_int]
+        L.crtm_profile_query.argtypes = [_vp, _dp, _dp, _ip]
         _lib = L
     return _lib
 
@@ -156,11 +159,22 @@ class Context:
             return a
         atm = CAtmosphere()
         atm.n_profiles = prof.n_profiles
+        host_nl = getattr(prof, "n_layers_host", None)
+        if host_nl is None and memory == MEM_HOST:
+            host_nl = np.asarray(prof.n_layers)
+        if host_nl is not None:
+            atm.max_layers = int(np.max(host_nl))
+            atm.total_layers = int(np.sum(host_nl))
         atm.n_layers = _ptr(arr(prof.n_layers, np.int32))
         atm.layer_offset = _ptr(arr(prof.layer_offset, np.int64))
         atm.n_absorbers = int(prof.absorber.shape[0])
-        atm.absorber_id = _ptr(arr(prof.absorber_id, np.int32))
-        atm.absorber_units = _ptr(arr(prof.absorber_units, np.int32))
+        ids = np.ascontiguousarray(np.asarray(prof.absorber_id if not _is_torch(prof.absorber_id)
+                                              else prof.absorber_id.cpu()), np.int32)
+        units = np.ascontiguousarray(np.asarray(prof.absorber_units if not _is_torch(prof.absorber_units)
+                                                else prof.absorber_units.cpu()), np.int32)
+        keep.extend([ids, units])
+        atm.absorber_id = _ptr(ids)
+        atm.absorber_units = _ptr(units)
         atm.level_pressure = _ptr(arr(prof.level_pressure, np.float64))
         atm.pressure = _ptr(arr(prof.pressure, np.float64))
         atm.temperature = _ptr(arr(prof.temperature, np.float64))
@@ -208,42 +222,59 @@ class Context:
         return out
 
     def k_matrix(self, prof, emissivity, bt_k=None, rad_k=None, precision=PREC_FP64,
-                 jacobian_f32=False, check_input=True):
+                 jacobian_f32=False, check_input=True, out=None):
         """CRTM_K_Matrix: returns (RTSolution dict, K dict).
 
         K layout: temperature_k [NT, L] (profile m rows layer_offset[m]:...),
-        absorber_k [n_absorbers, NT, L]; surface temperature K per type [M, L]."""
+        absorber_k [n_absorbers, NT, L]; surface temperature K per type [M, L].
+        bt_k / rad_k are RTSolution_K inputs (BT_K defaults to 1, Radiance_K
+        to 0).  `out` = (rts, k) dicts from a previous call are reused as the
+        output buffers (every element is overwritten)."""
         memory = MEM_DEVICE if _is_torch(prof.temperature) else MEM_HOST
         keep = []
         atm, sfc, geo, opt = self._structs(prof, emissivity, precision, memory, jacobian_f32,
                                            check_input, keep)
         M, L = prof.n_profiles, self.n_channels
-        NT = int(prof.layer_offset[-1])
+        host_nl = getattr(prof, "n_layers_host", None)
+        NT = int(np.sum(host_nl if host_nl is not None else np.asarray(prof.n_layers)))
         na = int(prof.absorber.shape[0])
         like = prof.temperature
-        rts_d = {k: self._alloc((M, L), np.float64, memory, like) for k in ("radiance", "brightness_temperature")}
-        rts_d["status"] = self._alloc((M,), np.int32, memory, like)
         kdt = np.float32 if jacobian_f32 else np.float64
+        if out is not None:
+            rts_d, kd = out
+        else:
+            rts_d = {k: self._alloc((M, L), np.float64, memory, like)
+                     for k in ("radiance", "brightness_temperature")}
+            rts_d["status"] = self._alloc((M,), np.int32, memory, like)
+            kd = {"surface_emissivity_k": self._alloc((M, L), np.float64, memory, like),
+                  "temperature_k": self._alloc((NT, L), kdt, memory, like),
+                  "absorber_k": self._alloc((na, NT, L), kdt, memory, like)}
+            for f in ("land_temperature_k", "water_temperature_k", "snow_temperature_k", "ice_temperature_k"):
+                kd[f] = self._alloc((M, L), np.float64, memory, like)
         if bt_k is None:
             bt_k = self._alloc((M, L), np.float64, memory, like) + 1.0
-        if rad_k is None:
-            rad_k = self._alloc((M, L), np.float64, memory, like)
-        else:
-            rad_k = rad_k.clone() if _is_torch(rad_k) else np.array(rad_k, np.float64)
         if memory == MEM_HOST:
             bt_k = np.ascontiguousarray(bt_k, np.float64)
-        kd = {"radiance_k": rad_k,
-              "surface_emissivity_k": self._alloc((M, L), np.float64, memory, like),
-              "temperature_k": self._alloc((NT, L), kdt, memory, like),
-              "absorber_k": self._alloc((na, NT, L), kdt, memory, like)}
-        for f in ("land_temperature_k", "water_temperature_k", "snow_temperature_k", "ice_temperature_k"):
-            kd[f] = self._alloc((M, L), np.float64, memory, like)
+            keep.append(bt_k)
+        if rad_k is not None:
+            kd["radiance_k"] = rad_k.clone() if _is_torch(rad_k) else np.array(rad_k, np.float64)
         rts = CRTSolution(**{k: _ptr(v) for k, v in rts_d.items()})
         ko = CKOutput(brightness_temperature_k=_ptr(bt_k), **{k: _ptr(v) for k, v in kd.items()})
         st = lib().crtm_k_matrix(self._h, atm, sfc, geo, opt, rts, ko)
         if st != SUCCESS:
             self._err("crtm_k_matrix")
         return rts_d, kd
+
+    def profile_enable(self, on=True):
+        if lib().crtm_profile_enable(self._h, int(on)) != SUCCESS:
+            self._err("crtm_profile_enable")
+
+    def profile_query(self):
+        """(prep_ms, main_ms, n_launches) summed since the last enable/query."""
+        a, b, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_int()
+        if lib().crtm_profile_query(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(n)) != SUCCESS:
+            self._err("crtm_profile_query")
+        return a.value, b.value, n.value
 
     def synchronize(self):
         if lib().crtm_synchronize(self._h) != SUCCESS:
@@ -279,6 +310,6 @@ def device_profiles(prof, device="cuda"):
         v = getattr(prof, f.name)
         d[f.name] = None if v is None else torch.as_tensor(np.ascontiguousarray(v)).to(device)
     out = dataclasses.replace(prof, **d)
-    # host copies kept for bookkeeping the wrapper needs
-    object.__setattr__(out, "layer_offset_host", np.asarray(prof.layer_offset))
+    # host copy of the layer counts: lets device calls size their work without a sync
+    out.n_layers_host = np.asarray(prof.n_layers)
     return out

@@ -499,6 +499,21 @@ struct crtm_context {
   // work pool (grown on demand, reused across calls)
   DevBuf wx, wu, wo, wacci, waccw, wuidx, wufrac, wgeo, wstatus;
   DevBuf stage;  // host-mode staging of inputs/outputs
+  // kernel timing (crtm_profile_enable): HIP events on the context stream
+  bool profiling = false;
+  std::vector<hipEvent_t> ev;      // pairs (start, stop) per recorded launch
+  std::vector<int> ev_kind;        // 0 = preparation kernel, 1 = forward/K kernel
+  size_t ev_used = 0;
+  hipEvent_t mark() {
+    if (ev_used == ev.size()) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess) throw std::runtime_error("hipEventCreate failed");
+      ev.push_back(e);
+    }
+    hipEvent_t e = ev[ev_used++];
+    if (hipEventRecord(e, stream) != hipSuccess) throw std::runtime_error("hipEventRecord failed");
+    return e;
+  }
 };
 
 namespace {
@@ -545,10 +560,27 @@ void run_sensor(crtm_context* c, Sensor& S, const DevBatch& b, const DevOut& o) 
         bb.jmap[j] = ju;
         break;
       }
+  auto timed = [&](int kind, auto&& fn) {
+    if (c->profiling) {
+      c->mark();
+      c->ev_kind.push_back(kind);
+    }
+    fn();
+    if (c->profiling) c->mark();
+  };
   switch (S.d.group) {
-    case 1: launch_prep<1, RT>(S.d, bb, w, c->stream); launch_fwdk<1, RT, KT, DOK>(S.d, bb, w, o, c->stream); break;
-    case 2: launch_prep<2, RT>(S.d, bb, w, c->stream); launch_fwdk<2, RT, KT, DOK>(S.d, bb, w, o, c->stream); break;
-    case 3: launch_prep<3, RT>(S.d, bb, w, c->stream); launch_fwdk<3, RT, KT, DOK>(S.d, bb, w, o, c->stream); break;
+    case 1:
+      timed(0, [&] { launch_prep<1, RT>(S.d, bb, w, c->stream); });
+      timed(1, [&] { launch_fwdk<1, RT, KT, DOK>(S.d, bb, w, o, c->stream); });
+      break;
+    case 2:
+      timed(0, [&] { launch_prep<2, RT>(S.d, bb, w, c->stream); });
+      timed(1, [&] { launch_fwdk<2, RT, KT, DOK>(S.d, bb, w, o, c->stream); });
+      break;
+    case 3:
+      timed(0, [&] { launch_prep<3, RT>(S.d, bb, w, c->stream); });
+      timed(1, [&] { launch_fwdk<3, RT, KT, DOK>(S.d, bb, w, o, c->stream); });
+      break;
   }
   HIPCHECK(hipGetLastError());
 }
@@ -582,29 +614,36 @@ int do_call(crtm_context* ctx, const crtm_atmosphere* atm, const crtm_surface* s
   const bool kf32 = opt->jacobian_f32 != 0;
   const int L = ctx->n_channels;
   HIPCHECK(hipSetDevice(ctx->device));
-  // layer bookkeeping needs host copies of n_layers / offsets
-  std::vector<int> nl(M);
-  std::vector<int64_t> off(M + 1);
-  if (host) {
-    std::memcpy(nl.data(), atm->n_layers, M * sizeof(int));
-    std::memcpy(off.data(), atm->layer_offset, (M + 1) * sizeof(int64_t));
-  } else {
-    HIPCHECK(hipMemcpyAsync(nl.data(), atm->n_layers, M * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
-    HIPCHECK(hipMemcpyAsync(off.data(), atm->layer_offset, (M + 1) * sizeof(int64_t), hipMemcpyDeviceToHost,
-                            ctx->stream));
-    HIPCHECK(hipStreamSynchronize(ctx->stream));
+  // layer bookkeeping: host copies of n_layers / offsets (host mode), or the
+  // caller's scalars (device mode, no synchronisation)
+  std::vector<int> nl;
+  std::vector<int64_t> off;
+  int max_k = atm->max_layers;
+  int64_t NT = atm->total_layers;
+  if (host || max_k <= 0 || NT <= 0) {
+    nl.resize(M);
+    off.resize(M + 1);
+    if (host) {
+      std::memcpy(nl.data(), atm->n_layers, M * sizeof(int));
+      std::memcpy(off.data(), atm->layer_offset, (M + 1) * sizeof(int64_t));
+    } else {
+      HIPCHECK(hipMemcpyAsync(nl.data(), atm->n_layers, M * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+      HIPCHECK(hipMemcpyAsync(off.data(), atm->layer_offset, (M + 1) * sizeof(int64_t), hipMemcpyDeviceToHost,
+                              ctx->stream));
+      HIPCHECK(hipStreamSynchronize(ctx->stream));
+    }
+    max_k = 0;
+    for (int m = 0; m < M; ++m) {
+      if (nl[m] < 1 || nl[m] > CRTM_MAX_N_LAYERS)
+        return fail(ctx, fmt("profile %d: invalid n_layers %d", m + 1, nl[m]));
+      if (off[m + 1] - off[m] != nl[m]) return fail(ctx, fmt("profile %d: layer_offset inconsistent", m + 1));
+      max_k = std::max(max_k, nl[m]);
+    }
+    NT = off[M];
   }
-  int max_k = 0;
-  for (int m = 0; m < M; ++m) {
-    if (nl[m] < 1 || nl[m] > CRTM_MAX_N_LAYERS) return fail(ctx, fmt("profile %d: invalid n_layers %d", m + 1, nl[m]));
-    if (off[m + 1] - off[m] != nl[m]) return fail(ctx, fmt("profile %d: layer_offset inconsistent", m + 1));
-    max_k = std::max(max_k, nl[m]);
-  }
-  const int64_t NT = off[M];
+  if (max_k < 1 || max_k > CRTM_MAX_N_LAYERS) return fail(ctx, fmt("invalid max_layers %d", max_k));
   const int na = atm->n_absorbers;
-  std::vector<int> abs_id(na);
-  if (host) std::memcpy(abs_id.data(), atm->absorber_id, na * sizeof(int));
-  else HIPCHECK(hipMemcpy(abs_id.data(), atm->absorber_id, na * sizeof(int), hipMemcpyDeviceToHost));
+  std::vector<int> abs_id(atm->absorber_id, atm->absorber_id + na);
   if (opt->check_input && host) {
     for (int m = 0; m < M; ++m) {
       const double* lp = atm->level_pressure + off[m] + m;
@@ -796,6 +835,7 @@ int crtm_destroy(crtm_context* ctx) {
   (void)hipSetDevice(ctx->device);
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   ctx->sensors.clear();
+  for (hipEvent_t e : ctx->ev) (void)hipEventDestroy(e);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return CRTM_SUCCESS;
@@ -845,5 +885,34 @@ int crtm_synchronize(crtm_context* ctx) {
 }
 
 const char* crtm_last_error(const crtm_context* ctx) { return ctx ? ctx->last_error.c_str() : "null context"; }
+
+int crtm_profile_enable(crtm_context* ctx, int on) {
+  if (!ctx) return CRTM_FAILURE;
+  if (hipStreamSynchronize(ctx->stream) != hipSuccess) return fail(ctx, "stream synchronize failed");
+  ctx->profiling = on != 0;
+  ctx->ev_used = 0;
+  ctx->ev_kind.clear();
+  return CRTM_SUCCESS;
+}
+
+int crtm_profile_query(crtm_context* ctx, double* prep_ms, double* main_ms, int* n_launches) {
+  if (!ctx) return CRTM_FAILURE;
+  if (hipStreamSynchronize(ctx->stream) != hipSuccess) return fail(ctx, "stream synchronize failed");
+  double t[2] = {0.0, 0.0};
+  int n[2] = {0, 0};
+  for (size_t i = 0; i < ctx->ev_kind.size(); ++i) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, ctx->ev[2 * i], ctx->ev[2 * i + 1]) != hipSuccess)
+      return fail(ctx, "hipEventElapsedTime failed");
+    t[ctx->ev_kind[i]] += ms;
+    n[ctx->ev_kind[i]] += 1;
+  }
+  if (prep_ms) *prep_ms = t[0];
+  if (main_ms) *main_ms = t[1];
+  if (n_launches) *n_launches = n[1];
+  ctx->ev_used = 0;
+  ctx->ev_kind.clear();
+  return CRTM_SUCCESS;
+}
 
 }  // extern "C"

@@ -18,9 +18,10 @@
  *     layer_offset[m] * L.  absorber_k is [n_absorbers] such blocks.
  *   - memory = CRTM_MEM_HOST: all batch pointers are host pointers
  *     (the library stages them through its device pool);
- *     memory = CRTM_MEM_DEVICE: all batch pointers are device pointers on
- *     the context's GPU and the call is asynchronous on the context stream
- *     (crtm_synchronize waits).
+ *     memory = CRTM_MEM_DEVICE: all batch pointers (except the absorber id
+ *     and unit lists) are device pointers on the context's GPU; with
+ *     max_layers and total_layers given the call is asynchronous on the
+ *     context stream (crtm_synchronize waits) and issues no host sync.
  */
 #ifndef CRTM_MI355X_H
 #define CRTM_MI355X_H
@@ -59,11 +60,13 @@ typedef struct crtm_context crtm_context;
 /* ---- Atmosphere (src/Atmosphere/CRTM_Atmosphere_Define.f90:433-464) ---- */
 typedef struct {
   int n_profiles;
+  int max_layers;                  /* host scalar: max K_m (0 = derive from n_layers) */
+  int64_t total_layers;            /* host scalar: layer_offset[M] (0 = derive) */
   const int* n_layers;             /* [M] */
   const int64_t* layer_offset;     /* [M+1] */
   int n_absorbers;
-  const int* absorber_id;          /* [n_absorbers], CRTM absorber ids (H2O=1, O3=3, ...) */
-  const int* absorber_units;       /* [n_absorbers] */
+  const int* absorber_id;          /* [n_absorbers], host array in both memory modes */
+  const int* absorber_units;       /* [n_absorbers], host array in both memory modes */
   const double* level_pressure;    /* hPa, [layer_offset[M]+M] */
   const double* pressure;          /* hPa, [layer_offset[M]] */
   const double* temperature;       /* K */
@@ -161,6 +164,14 @@ int crtm_synchronize(crtm_context* ctx);
 
 /* Last error message of the context (static storage, NUL-terminated). */
 const char* crtm_last_error(const crtm_context* ctx);
+
+/* Kernel timing, the analogue of the forward driver's compile-time timing
+ * hook (src/CRTM_Forward_Module.f90:262-282): when enabled, every kernel
+ * launch is bracketed by HIP events on the context stream.  query returns
+ * the summed milliseconds of the preparation and forward/K kernels since the
+ * last enable/query and resets them. */
+int crtm_profile_enable(crtm_context* ctx, int on);
+int crtm_profile_query(crtm_context* ctx, double* prep_ms, double* main_ms, int* n_launches);
 
 #ifdef __cplusplus
 }
