@@ -30,8 +30,20 @@ constexpr double MAX_OD = 20.0;
 constexpr double LIMIT_EXP = 20.0, LIMIT_LOG = 4.8e+08;
 
 template <class RT> __device__ inline RT rexp(RT x);
-template <> __device__ inline float rexp<float>(float x) { return expf(x); }
+template <> __device__ inline float rexp<float>(float x) { return __expf(x); }
 template <> __device__ inline double rexp<double>(double x) { return exp(x); }
+
+// Planck radiance B = c1 / (exp(c2/Te) - 1), Te = bc1 + bc2*T
+// (CRTM_Planck_Functions.f90:116-136); expm1 keeps the microwave case
+// (c2/Te ~ 0.01) accurate in FP32.
+__device__ inline float planck(float c1, float c2, float bc1, float bc2, double t, float& te) {
+  te = bc1 + bc2 * (float)t;
+  return __fdividef(c1, expm1f(__fdividef(c2, te)));
+}
+__device__ inline double planck(double c1, double c2, double bc1, double bc2, double t, double& te) {
+  te = bc1 + bc2 * t;
+  return c1 / (exp(c2 / te) - 1.0);
+}
 
 template <class RT, class KT>
 __device__ inline void store_k(void* base, int64_t idx, RT v) {
@@ -94,13 +106,14 @@ __device__ inline void comp_adjoint(const RT* xa, const RT* u, LayerAdj<RT>& a) 
     const RT o3 = u[U_O3], o3_a = u[U_O3A], o3_r = u[U_O3R], o3p = u[U_O3P], dt = u[U_DT];
     a.o3_a += xa[0] + xa[1] * dt + xa[2] * o3 * o3p + RT(2) * xa[3] * o3_a + xa[4] * o3p + xa[5] * u[U_SP];
     a.dt += xa[1] * o3_a + xa[6] * o3_r;
-    a.o3_r += xa[6] * dt + xa[7] + xa[8] * o3 / o3p;
-    a.o3 += xa[2] * o3_a * o3p + xa[8] * o3_r / o3p;
+    const RT io3p = u[U_IO3P];
+    a.o3_r += xa[6] * dt + xa[7] + xa[8] * o3 * io3p;
+    a.o3 += xa[2] * o3_a * o3p + xa[8] * o3_r * io3p;
     a.gz[1] += xa[2] * o3_a * o3 + xa[4] * o3_a + xa[5] * RT(0.5) * o3_a * u[U_SQSO3P] -
-               xa[8] * o3_r * o3 / (o3p * o3p) + xa[9] * sec + xa[10] * RT(2) * sec * sec * o3p;
+               xa[8] * o3_r * o3 * io3p * io3p + xa[9] * sec + xa[10] * RT(2) * sec * sec * o3p;
   } else if constexpr (J == 4) {  // CO2
     const RT tzp = u[U_TZP], sT = u[U_ST];
-    a.T += xa[0] * sec + xa[2] + xa[9] * sec * tzp * RT(0.5) / sT;
+    a.T += xa[0] * sec + xa[2] + xa[9] * sec * tzp * RT(0.5) * u[U_IST];
     a.T2 += xa[1] * sec + xa[3];
     a.ra[2] += xa[5] * sec;
     a.tzp += xa[6] * sec + xa[8] * RT(3) * tzp * tzp + xa[9] * sec * sT;
@@ -110,9 +123,10 @@ __device__ inline void comp_adjoint(const RT* xa, const RT* u, LayerAdj<RT>& a) 
     const RT dt = u[U_DT], n2o_a = u[U_N2OA], n2o = u[U_N2O], n2o_r = u[U_N2OR], n2ozp = u[U_N2OZP];
     a.n2o_a += xa[0] * dt + xa[3] * RT(0.25) * u[U_N2OPOW] + xa[4];
     a.dt += xa[0] * n2o_a + xa[2] * n2o;
-    a.n2o_r += xa[1] + xa[9] * n2o / n2ozp;
-    a.n2o += xa[2] * dt + xa[9] * n2o_r / n2ozp;
-    a.gz[3] += xa[5] * sec - xa[9] * n2o_r * n2o / (n2ozp * n2ozp);
+    const RT in2ozp = u[U_IN2OZP];
+    a.n2o_r += xa[1] + xa[9] * n2o * in2ozp;
+    a.n2o += xa[2] * dt + xa[9] * n2o_r * in2ozp;
+    a.gz[3] += xa[5] * sec - xa[9] * n2o_r * n2o * in2ozp * in2ozp;
     a.gt[3] += xa[6] * sec + xa[8];
     a.n2o_s += xa[7];
     a.ch4_a += xa[10] + xa[11] * u[U_CH4ZP];
@@ -123,20 +137,22 @@ __device__ inline void comp_adjoint(const RT* xa, const RT* u, LayerAdj<RT>& a) 
     const RT dt = u[U_DT], co_a = u[U_COA], co_r = u[U_COR], co_q = u[U_COQ];
     a.co_a += xa[0] + xa[1] * dt + xa[6] * u[U_DT2];
     a.dt += xa[1] * co_a + xa[3] * co_r;
-    a.co_r += xa[2] * RT(0.5) / u[U_SQCOR] + xa[3] * dt + xa[5] - xa[8] * co_q / (co_r * co_r);
+    const RT icor = u[U_ICOR];
+    a.co_r += xa[2] * RT(0.5) * u[U_ISQCOR] + xa[3] * dt + xa[5] - xa[8] * co_q * icor * icor;
     a.co_s += xa[4];
     a.dt2 += xa[6] * co_a;
-    a.co_q += xa[7] + xa[8] / co_r + xa[9] * u[U_SQCOZP];
-    a.gz[4] += xa[9] * RT(0.5) * co_q / u[U_SQCOZP];
+    a.co_q += xa[7] + xa[8] * icor + xa[9] * u[U_SQCOZP];
+    a.gz[4] += xa[9] * RT(0.5) * co_q * u[U_ISQCOZP];
   } else if constexpr (J == 7) {  // CH4
     const RT dt = u[U_DT], ch4_a = u[U_CH4A], ch4 = u[U_CH4], ch4_r = u[U_CH4R], ch4zp = u[U_CH4ZP];
     a.ch4_a += xa[0] * dt + xa[2] * RT(2) * ch4_a + xa[3];
     a.dt += xa[0] * ch4_a + xa[4] * ch4;
-    a.ch4_r += xa[1] + xa[7] * RT(0.5) / u[U_SQCH4R] + xa[10] * ch4 / ch4zp;
-    a.ch4 += xa[4] * dt + xa[10] * ch4_r / ch4zp;
+    const RT ich4zp = u[U_ICH4ZP];
+    a.ch4_r += xa[1] + xa[7] * RT(0.5) * u[U_ISQCH4R] + xa[10] * ch4 * ich4zp;
+    a.ch4 += xa[4] * dt + xa[10] * ch4_r * ich4zp;
     a.ch4_q += xa[5] + xa[6] * RT(2) * u[U_CH4Q];
     a.gt[5] += xa[8] + xa[9] * sec;
-    a.gz[5] -= xa[10] * ch4_r * ch4 / (ch4zp * ch4zp);
+    a.gz[5] -= xa[10] * ch4_r * ch4 * ich4zp * ich4zp;
   }
 }
 
@@ -147,28 +163,29 @@ __device__ inline void finish_layer(const RT* u, LayerAdj<RT>& a) {
   const RT sec = u[U_SEC];
   if constexpr (G == 1) {
     a.gz[5] += sec * a.ch4_q;
-    a.ch4_a += RT(0.5) / u[U_CH4R] * a.ch4_r;
+    a.ch4_a += RT(0.5) * u[U_ICH4R] * a.ch4_r;
     a.ch4 += sec * a.ch4_a;
     const RT co_a = u[U_COA], co = u[U_CO], cozp = u[U_COZP];
-    a.gz[4] -= a.co_q * co_a * co / (cozp * cozp);
-    a.co_a += a.co_q * co / cozp + a.co_s * RT(2) * co_a + a.co_r * RT(0.5) / u[U_COR];
-    a.co += a.co_q * co_a / cozp + a.co_a * sec;
-    a.n2o_a += a.n2o_r * RT(0.5) / u[U_N2OR] + a.n2o_s * RT(2) * u[U_N2OA];
+    const RT icozp = u[U_ICOZP];
+    a.gz[4] -= a.co_q * co_a * co * icozp * icozp;
+    a.co_a += a.co_q * co * icozp + a.co_s * RT(2) * co_a + a.co_r * RT(0.5) * u[U_ICOR];
+    a.co += a.co_q * co_a * icozp + a.co_a * sec;
+    a.n2o_a += a.n2o_r * RT(0.5) * u[U_IN2OR] + a.n2o_s * RT(2) * u[U_N2OA];
     a.n2o += a.n2o_a * sec;
     a.ra[5] += a.ch4;
     a.ra[3] += a.n2o;
     a.ra[4] += a.co;
   }
   if constexpr (G == 1 || G == 2) {
-    a.o3_a += a.o3_r * RT(0.5) / u[U_O3R];
+    a.o3_a += a.o3_r * RT(0.5) * u[U_IO3R];
     a.o3 += a.o3_a * sec;
     a.ra[1] += a.o3;
   }
-  const RT gt0 = u[U_GT0];
-  a.gt[0] -= a.h2o_q * u[U_H2O] / (gt0 * gt0);
-  a.h2o_r += a.h2o_r4 * RT(0.5) / u[U_H2OR4];
-  a.h2o_a += a.h2o_s * RT(2) * u[U_H2OA] + a.h2o_r * RT(0.5) / u[U_H2OR];
-  a.h2o += a.h2o_a * sec + a.h2o_q / gt0;
+  const RT igt0 = u[U_IGT0];
+  a.gt[0] -= a.h2o_q * u[U_H2O] * igt0 * igt0;
+  a.h2o_r += a.h2o_r4 * RT(0.5) * u[U_IH2OR4];
+  a.h2o_a += a.h2o_s * RT(2) * u[U_H2OA] + a.h2o_r * RT(0.5) * u[U_IH2OR];
+  a.h2o += a.h2o_a * sec + a.h2o_q * igt0;
   a.ra[0] += a.h2o;
   a.dt += a.dt2 * RT(2) * u[U_ADT];
   a.T += a.T2 * RT(2) * u[U_T];
@@ -183,8 +200,13 @@ __device__ inline void comp_layer_ad(const float* ck, const int* npb, const int*
     if (np > 0) {
       RT xa[Tg::NP[J]];
       const float* cj = ck + coff[J] * WAVE;
+      // all NP[J] slots are read (the coefficient array is padded) and the
+      // ones beyond this block's predictor count masked, so the loads batch
 #pragma unroll
-      for (int i = 0; i < Tg::NP[J]; ++i) xa[i] = (i < np) ? (RT)cj[i * WAVE] * od_ad : RT(0);
+      for (int i = 0; i < Tg::NP[J]; ++i) {
+        const RT c = (RT)cj[i * WAVE];
+        xa[i] = (i < np) ? c * od_ad : RT(0);
+      }
       comp_adjoint<G, J, RT>(xa, uk, a);
     }
     comp_layer_ad<G, RT, J + 1>(ck, npb, coff, od_ad, uk, a);
@@ -201,8 +223,10 @@ __device__ inline RT comp_layer_od(const float* ck, const int* npb, const int* c
       const float* cj = ck + coff[J] * WAVE;
       const RT* xj = xk + Tg::XOFF[J];
 #pragma unroll
-      for (int i = 0; i < Tg::NP[J]; ++i)
-        if (i < np) od += (RT)cj[i * WAVE] * xj[i];
+      for (int i = 0; i < Tg::NP[J]; ++i) {
+        const RT c = (RT)cj[i * WAVE];
+        od += (i < np ? c : RT(0)) * xj[i];
+      }
     }
     return od + comp_layer_od<G, RT, J + 1>(ck, npb, coff, xk);
   } else {
@@ -210,17 +234,53 @@ __device__ inline RT comp_layer_od(const float* ck, const int* npb, const int* c
   }
 }
 
+// OPTRAN layer OD pieces of one block at layer k: ln(chi) and, for the
+// adjoint, the per-row contributions (Add_OPTRAN_wloOD, ODPS_AtmAbsorption.f90:664-727)
+template <class RT>
+__device__ inline RT optran_lnchi(const double* ocb, const int* meta, const RT* ok) {
+  const int nrow = meta[BM_ONROW], nq = meta[BM_OMAXO] + 1;
+  RT lnchi = RT(0);
+  for (int r = 0; r < nrow; ++r) {
+    const int p = meta[BM_OROW0 + r];
+    const double* cr = ocb + (size_t)r * nq * WAVE;
+    RT bp = RT(0);
+    for (int q = 0; q < nq; ++q) bp += (RT)cr[q * WAVE] * ok[O_AP0 + q];
+    lnchi += bp * ok[O_X0 + p];
+  }
+  return lnchi;
+}
+
+constexpr int BPW = 4;                 // channel blocks (waves) per workgroup
+constexpr int TPB = BPW * WAVE;
+// LDS row: predictors X or adjoint table U, followed by the OPTRAN table O
+constexpr int TABW = (NX_MAX > NU ? NX_MAX : NU) + NO;
+static_assert(TABW <= TPB, "one table element per thread");
+
 template <int G, class RT, class KT, bool DOK>
-__global__ void __launch_bounds__(WAVE) clear_fwdk_kernel(DevSensor s, DevBatch bt, ProfWork<RT> w, DevOut o) {
+__global__ void __attribute__((amdgpu_flat_work_group_size(TPB, TPB), amdgpu_waves_per_eu(4)))
+clear_fwdk_kernel(DevSensor s, DevBatch bt, ProfWork<RT> w, DevOut o) {
   using Tg = GT<G>;
+  // Workgroup = BPW consecutive channel blocks of one profile: the profile's
+  // per-layer uniform rows are staged once into LDS and shared by its waves.
+  // XCD-aware order: workgroups are dispatched round-robin over the 8 XCDs,
+  // so XCD x receives ids x, x+8, ...; it gets the contiguous range
+  // [x*chunk, (x+1)*chunk) of the (block group, profile) order, block-group
+  // major, so waves resident on one XCD share coefficient blocks in its L2.
   const int nb = s.n_blocks;
-  const int wg = blockIdx.x;
-  const int b = wg % nb;
-  const int m = wg / nb;
-  const int lane = threadIdx.x;
+  const int nbg = (nb + BPW - 1) / BPW;
+  const int64_t total = (int64_t)nbg * bt.M;
+  const int64_t chunk = (total + 7) / 8;
+  const int64_t tix = (int64_t)(blockIdx.x & 7) * chunk + (blockIdx.x >> 3);
+  if (tix >= total) return;
+  const int tid = threadIdx.x;
+  const int m = (int)(tix % bt.M);
+  const int bq = (int)(tix / bt.M) * BPW + (tid >> 6);
+  const bool bvalid = bq < nb;
+  const int b = bvalid ? bq : nb - 1;
+  const int lane = tid & (WAVE - 1);
   const int l = b * WAVE + lane;
-  const bool valid = l < s.L;
-  const int lc = valid ? l : s.L - 1;
+  const bool valid = bvalid && l < s.L;
+  const int lc = l < s.L ? l : s.L - 1;
   const int lg = s.chan_offset + lc;
   const int64_t ol = (int64_t)m * bt.Ltot + lg;
   if (w.status[m] != 0) {
@@ -246,32 +306,32 @@ __global__ void __launch_bounds__(WAVE) clear_fwdk_kernel(DevSensor s, DevBatch 
   const bool blk_opt = ob >= 0;
   const bool lane_opt = blk_opt && s.lane_optran[b * WAVE + lane] != 0;
   const double* ocb = blk_opt ? s.ocd + ob + lane : nullptr;
-  const int opmask = meta[BM_OPMASK], omaxo = meta[BM_OMAXO];
   const RT* X = w.x + (size_t)m * Ko * Tg::NX;
   const RT* Ut = w.u + (size_t)m * Ko * NU;
   const RT* Ot = w.o + (size_t)m * Ko * NO;
+  const int optw = s.optran ? NO : 0;
+  __shared__ RT tab[2][TABW];
+  // element t of layer k's staged row: [X or U row | O row]
+  auto row = [&](const RT* base, int nbase, int k, int t) -> RT {
+    if (t < nbase) return base[(size_t)k * nbase + t];
+    if (t < nbase + optw) return Ot[(size_t)k * NO + (t - nbase)];
+    return RT(0);
+  };
 
-  // ---------------- forward: ODPS absorption ----------------
+  // ------------- forward (1): ODPS layer OD and level-to-space path -------------
   RT path[KO_MAX + 1];
-  RT chi[KO_MAX];
   uint64_t odclamp[2] = {0, 0}, lcclamp[2] = {0, 0};
   path[0] = RT(0);
+  tab[0][tid < TABW ? tid : 0] = tid < TABW ? row(X, Tg::NX, 0, tid) : RT(0);
+  __syncthreads();
   for (int k = 0; k < Ko; ++k) {
+    const RT nxt = (k + 1 < Ko && tid < TABW) ? row(X, Tg::NX, k + 1, tid) : RT(0);
+    const RT* xk = tab[k & 1];
     const float* ck = cb + (int64_t)k * nsl * WAVE;
-    RT od = comp_layer_od<G, RT, 0>(ck, npb, coff, X + (size_t)k * Tg::NX);
+    RT od = comp_layer_od<G, RT, 0>(ck, npb, coff, xk);
     if (blk_opt) {
-      const RT* ok = Ot + (size_t)k * NO;
-      RT lnchi = RT(0);
-#pragma unroll
-      for (int p = 0; p < OPZ; ++p) {
-        if ((opmask >> p) & 1) {
-          RT bp = RT(0);
-#pragma unroll
-          for (int q = 0; q < OPO; ++q)
-            if (q <= omaxo) bp += (RT)ocb[(p * OPO + q) * WAVE] * ok[O_AP0 + q];
-          lnchi += bp * ok[O_X0 + p];
-        }
-      }
+      const RT* ok = xk + Tg::NX;
+      const RT lnchi = optran_lnchi<RT>(ocb, meta, ok);
       if (lane_opt) {
         RT c;
         if (lnchi > RT(LIMIT_EXP)) {
@@ -283,7 +343,6 @@ __global__ void __launch_bounds__(WAVE) clear_fwdk_kernel(DevSensor s, DevBatch 
         } else {
           c = rexp<RT>(lnchi);
         }
-        chi[k] = c;
         od += c * ok[O_DA];
       }
     }
@@ -296,13 +355,17 @@ __global__ void __launch_bounds__(WAVE) clear_fwdk_kernel(DevSensor s, DevBatch 
       odclamp[k >> 6] |= 1ull << (k & 63);
     }
     path[k + 1] = path[k] + odc;
+    if (k + 1 < Ko && tid < TABW) tab[(k + 1) & 1][tid] = nxt;
+    __syncthreads();
   }
 
-  // ---------------- forward: user layers, Planck, Emission ----------------
+  // ------------- forward (2): user layers, Planck, Emission (top-down) -------------
   const double* geo = w.geo + (size_t)m * 4;
   const double sec_sfc = geo[0], cosu = geo[1], tsfc = geo[2];
   const RT rsec = (RT)sec_sfc;
+  const RT isec = (RT)(1.0 / sec_sfc);
   const RT u = (RT)cosu;
+  const RT iu = (RT)(1.0 / cosu);
   const int* ui = w.uidx + (size_t)m * (bt.max_k + 1);
   const RT* uf = w.ufrac + (size_t)m * (bt.max_k + 1);
   auto upath = [&](int i) -> RT {
@@ -310,76 +373,105 @@ __global__ void __launch_bounds__(WAVE) clear_fwdk_kernel(DevSensor s, DevBatch 
     const RT f = uf[i];
     return f < RT(0) ? path[k1] : path[k1] + (path[k1 + 1] - path[k1]) * f;
   };
-  const double c1 = s.c1[lc], c2 = s.c2[lc], bc1 = s.bc1[lc], bc2 = s.bc2[lc];
+  const RT c1 = (RT)s.c1[lc], c2 = (RT)s.c2[lc], bc1 = (RT)s.bc1[lc], bc2 = (RT)s.bc2[lc];
+  const RT ic1 = (RT)(1.0 / s.c1[lc]);
   const double* tuser = bt.t + loff;
-  RT tr[K_MAX], Bk[K_MAX], rd[K_MAX + 1], ru[K_MAX + 1];
-  rd[0] = (RT)s.cosmic[lc];
-  RT upp = upath(0);
+  RT rd = (RT)s.cosmic[lc];
+  RT uacc = RT(0), tc = RT(1);
+  const RT up0 = upath(0);
+  RT upp = up0;
   for (int k = 0; k < K; ++k) {
     const RT up = upath(k + 1);
-    const RT tau = (up - upp) / rsec;
+    const RT tau = (up - upp) * isec;
     upp = up;
-    const RT t = rexp<RT>(-tau / u);
-    const double te = bc1 + bc2 * tuser[k];
-    const double B = c1 / (exp(c2 / te) - 1.0);
-    tr[k] = t;
-    Bk[k] = (RT)B;
-    rd[k + 1] = rd[k] * t + (RT)B * (RT(1) - t);
+    const RT t = rexp<RT>(-tau * iu);
+    RT te;
+    const RT B = planck(c1, c2, bc1, bc2, tuser[k], te);
+    rd = rd * t + B * (RT(1) - t);
+    uacc += B * (RT(1) - t) * tc;
+    tc *= t;
   }
-  const double tes = bc1 + bc2 * tsfc;
-  const double Bs_d = c1 / (exp(c2 / tes) - 1.0);
-  const RT Bs = (RT)Bs_d;
+  RT tes;
+  const RT Bs = planck(c1, c2, bc1, bc2, tsfc, tes);
   const double eps_d = bt.emissivity[(size_t)m * bt.Ltot + lg];
-  const RT eps = (RT)eps_d;
-  ru[K] = eps * Bs + (RT(1) - eps) * rd[K];
-  RT upr = RT(0);
-  for (int k = K - 1; k >= 0; --k) {
-    const RT src = Bk[k] * (RT(1) - tr[k]);
-    ru[k] = ru[k + 1] * tr[k] + src;
-    upr = upr * tr[k] + src;
-  }
-  const double R = (double)ru[0];
-  const double bt_v = (c2 / log(c1 / R + 1.0) - bc1) / bc2;
+  const RT eps = (RT)eps_d, rho = RT(1) - eps;
+  const RT sfc_src = eps * Bs + rho * rd;
+  const double R = (double)(uacc + tc * sfc_src);
+  // CRTM_Planck_Temperature in FP64 (CRTM_Planck_Functions.f90:470-492)
+  const double c1d = s.c1[lc], c2d = s.c2[lc], bc1d = s.bc1[lc], bc2d = s.bc2[lc];
+  const double bt_v = (c2d / log(c1d / R + 1.0) - bc1d) / bc2d;
   if (valid) {
     o.radiance[ol] = R;
     o.bt[ol] = bt_v;
-    if (o.up_radiance) o.up_radiance[ol] = (double)upr;
-    if (o.down_radiance) o.down_radiance[ol] = (double)rd[K];
-    if (o.sfc_planck) o.sfc_planck[ol] = Bs_d;
+    if (o.up_radiance) o.up_radiance[ol] = (double)uacc;
+    if (o.down_radiance) o.down_radiance[ol] = (double)rd;
+    if (o.sfc_planck) o.sfc_planck[ol] = (double)Bs;
     if (o.sfc_emis) o.sfc_emis[ol] = eps_d;
   }
   if constexpr (!DOK) return;
 
-  // ---------------- adjoint: BT -> radiance, Emission_AD ----------------
-  const double arg = c1 / R + 1.0;
+  // ------------- adjoint seeds: Pre_Process_RTSolution_K and the surface terms -------------
+  const double arg = c1d / R + 1.0;
   const double lga = log(arg);
-  const double F = c1 * c2 / (bc2 * arg * (R * lga) * (R * lga));
+  const double F = c1d * c2d / (bc2d * arg * (R * lga) * (R * lga));
   const double rad_ad_d = (o.rad_k ? o.rad_k[ol] : 0.0) + F * (o.bt_k ? o.bt_k[ol] : 0.0);
   if (valid && o.rad_k) o.rad_k[ol] = rad_ad_d;
   const RT rad_ad = (RT)rad_ad_d;
-  RT tau_ad[K_MAX], b_ad[K_MAX];
-  RT upad = rad_ad;
-  for (int k = 0; k < K; ++k) {
-    const RT t = tr[k];
-    const RT a = t * (ru[k + 1] - Bk[k]) * upad;
-    b_ad[k] = upad * (RT(1) - t);
-    upad *= t;
-    tau_ad[k] = -a / u;
-  }
-  const RT eps_ad = upad * Bs, bs_ad = eps * upad, r_ad = upad * rd[K];
-  RT dnad = (RT(1) - eps) * upad;
+  const RT upad_sfc = rad_ad * tc;           // adjoint of the surface-leaving radiance
   if (valid) {
-    if (o.emis_k) o.emis_k[ol] = (double)(eps_ad - r_ad);
-    const double dbsdt = c2 * bc2 * Bs_d * (Bs_d + c1) / (c1 * tes * tes);
-    const double sad = dbsdt * (double)bs_ad;
+    if (o.emis_k) o.emis_k[ol] = (double)(upad_sfc * Bs - upad_sfc * rd);
+    const RT dbsdt = c2 * bc2 * Bs * (Bs + c1) * ic1 / (tes * tes);
+    const double sad = (double)(dbsdt * eps * upad_sfc);
     for (int c = 0; c < 4; ++c)
       if (o.sfc_t_k[c]) o.sfc_t_k[c][ol] = bt.cov[c][m] * sad;
   }
+
+  // ------------- adjoint sweep, bottom-up: user levels interleaved with ODPS layers -------------
+  // Emission_AD in carried form: ru = upwelling radiance at the bottom of the
+  // current layer, dn = adjoint of the downwelling radiance there, gd = rd*dn
+  // (so the downwelling profile never has to be stored or inverted).
+  RT ru = sfc_src;
+  RT dn = rho * upad_sfc;
+  RT gd = rd * dn;
   RT path_ad[KO_MAX + 1];
   for (int k = 0; k <= Ko; ++k) path_ad[k] = RT(0);
-  auto scatter = [&](int i, RT v) {
-    const int k1 = ui[i];
-    const RT f = uf[i];
+  const int64_t Lt = bt.Ltot;
+  const int na = bt.na;
+  KT* tko = reinterpret_cast<KT*>(o.t_k) + loff * Lt + lg;
+  KT* ako = reinterpret_cast<KT*>(o.a_k) + loff * Lt + lg;
+  const int64_t astr = bt.NT * Lt;
+  RT carry = RT(0);
+  int i = K;
+  auto ustep = [&](int iu_) {
+    RT v;
+    if (iu_ >= 1) {
+      const int k = iu_ - 1;   // user layer above level iu_
+      const RT upk = upath(k), upk1 = upath(k + 1);
+      const RT t = rexp<RT>(-((upk1 - upk) * isec) * iu);
+      const RT tabove = rexp<RT>(-((upk - up0) * isec) * iu);
+      RT te;
+      const RT B = planck(c1, c2, bc1, bc2, tuser[k], te);
+      const RT upad = rad_ad * tabove;
+      const RT a_up = t * (ru - B) * upad;
+      const RT a_dn = gd - B * dn;
+      const RT b_ad = (upad + dn) * (RT(1) - t);
+      const RT tau_ad = -(a_up + a_dn) * iu;
+      gd -= B * (RT(1) - t) * dn;
+      dn *= t;
+      ru = ru * t + B * (RT(1) - t);
+      const RT dbdt = c2 * bc2 * B * (B + c1) * ic1 / (te * te);
+      if (valid) {
+        tko[(int64_t)k * Lt] = (KT)(dbdt * b_ad);
+        for (int ja = 0; ja < na; ++ja) ako[ja * astr + (int64_t)k * Lt] = KT(0);
+      }
+      const RT gq = tau_ad * isec;
+      v = carry + gq;
+      carry = -gq;
+    } else {
+      v = carry;
+    }
+    const int k1 = ui[iu_];
+    const RT f = uf[iu_];
     if (f < RT(0)) {
       path_ad[k1] += v;
     } else {
@@ -387,28 +479,7 @@ __global__ void __launch_bounds__(WAVE) clear_fwdk_kernel(DevSensor s, DevBatch 
       path_ad[k1 + 1] += f * v;
     }
   };
-  RT tk_acc[K_MAX];
-  RT carry = RT(0);
-  for (int k = K - 1; k >= 0; --k) {
-    const RT t = tr[k];
-    const RT a = t * (rd[k] - Bk[k]) * dnad;
-    const RT bad = b_ad[k] + dnad * (RT(1) - t);
-    dnad *= t;
-    const RT tad = tau_ad[k] - a / u;
-    const double te = bc1 + bc2 * tuser[k];
-    const double Bd = (double)Bk[k];
-    tk_acc[k] = (RT)(c2 * bc2 * Bd * (Bd + c1) / (c1 * te * te)) * bad;
-    const RT gq = tad / rsec;
-    scatter(k + 1, carry + gq);
-    carry = -gq;
-  }
-  scatter(0, carry);
 
-  // ---------------- adjoint: ODPS regression, predictors, Map_Input ----------------
-  const int na = bt.na;
-  RT ak_acc[NA_MAX][K_MAX];
-  for (int j = 0; j < na; ++j)
-    for (int k = 0; k < K; ++k) ak_acc[j][k] = RT(0);
   RT S = RT(0), tzs = RT(0), tzps = RT(0);
   RT gzs[Tg::NABS], gts[Tg::NABS];
 #pragma unroll
@@ -417,62 +488,63 @@ __global__ void __launch_bounds__(WAVE) clear_fwdk_kernel(DevSensor s, DevBatch 
   const int h2o_j = s.h2o_idx - 1;
   const int* acc_i = w.acc_i + (size_t)m * Ko * 2;
   const RT* acc_w = w.acc_w + (size_t)m * Ko * MAXW;
+  if (tid < TABW) tab[(Ko - 1) & 1][tid] = row(Ut, NU, Ko - 1, tid);
+  __syncthreads();
   for (int k = Ko - 1; k >= 0; --k) {
+    const RT nxt = (k > 0 && tid < TABW) ? row(Ut, NU, k - 1, tid) : RT(0);
+    const int i1 = acc_i[2 * k], n = acc_i[2 * k + 1];
+    // user levels whose path adjoint feeds ODPS level k+1, and every user
+    // layer this ODPS layer's averaging kernel touches
+    while (i >= 0 && (ui[i] >= k || i > i1)) {
+      ustep(i);
+      --i;
+    }
     S += path_ad[k + 1];
     const RT od_ad = ((odclamp[k >> 6] >> (k & 63)) & 1) ? RT(0) : S;
-    const RT* uk = Ut + (size_t)k * NU;
+    const RT* uk = tab[k & 1];
     LayerAdj<RT> a{};
     const float* ck = cb + (int64_t)k * nsl * WAVE;
     comp_layer_ad<G, RT, 0>(ck, npb, coff, od_ad, uk, a);
     RT t_ad = RT(0), vap_ad = RT(0);
     if (blk_opt) {
-      const RT* ok = Ot + (size_t)k * NO;
-      const RT chi_ad = od_ad * ok[O_DA];
-      const RT da_ad = lane_opt ? od_ad * chi[k] : RT(0);
+      // Add_OPTRAN_wloOD_AD (ODPS_AtmAbsorption.f90:911-984) with chi recomputed
+      const RT* ok = uk + NU;
+      const RT lnchi = optran_lnchi<RT>(ocb, meta, ok);
       const bool lcc = (lcclamp[k >> 6] >> (k & 63)) & 1;
-      const RT lc_ad = (lane_opt && !lcc) ? chi[k] * chi_ad : RT(0);
-      RT ox_ad[OPZ], ap_ad[OPO];
-#pragma unroll
-      for (int p = 0; p < OPZ; ++p) ox_ad[p] = RT(0);
-#pragma unroll
-      for (int q = 0; q < OPO; ++q) ap_ad[q] = RT(0);
-#pragma unroll
-      for (int p = 0; p < OPZ; ++p) {
-        if ((opmask >> p) & 1) {
-          RT bp = RT(0);
-#pragma unroll
-          for (int q = 0; q < OPO; ++q)
-            if (q <= omaxo) bp += (RT)ocb[(p * OPO + q) * WAVE] * ok[O_AP0 + q];
-          ox_ad[p] += lc_ad * bp;
-          const RT bad = lc_ad * ok[O_X0 + p];
-#pragma unroll
-          for (int q = 0; q < OPO; ++q)
-            if (q <= omaxo) ap_ad[q] += (RT)ocb[(p * OPO + q) * WAVE] * bad;
+      const RT chi = lane_opt ? (lcc ? (lnchi > RT(0) ? RT(LIMIT_LOG) : RT(0)) : rexp<RT>(lnchi)) : RT(0);
+      const RT da_ad = lane_opt ? od_ad * chi : RT(0);
+      const RT lc_ad = (lane_opt && !lcc) ? chi * od_ad * ok[O_DA] : RT(0);
+      const int nrow = meta[BM_ONROW], nq = meta[BM_OMAXO] + 1;
+      RT ap1_ad = RT(0), e12 = RT(0), e13 = RT(0);
+      for (int r = 0; r < nrow; ++r) {
+        const int p = meta[BM_OROW0 + r];
+        const double* cr = ocb + (size_t)r * nq * WAVE;
+        RT bp = RT(0), dbp = RT(0);
+        for (int q = 0; q < nq; ++q) {
+          const RT cq = (RT)cr[q * WAVE];
+          bp += cq * ok[O_AP0 + q];
+          if (q > 0) dbp += RT(q) * cq * ok[O_AP0 + q - 1];
         }
+        const RT xa = lc_ad * bp;          // adjoint of OX(p)
+        ap1_ad += lc_ad * ok[O_X0 + p] * dbp;
+        t_ad += xa * ok[O_DX_DT0 + p];
+        vap_ad += xa * ok[O_DX_DV0 + p];
+        if (p == 12) e12 += xa;
+        if (p == 13) e13 += xa;
       }
-      // ODPS_Compute_Predictor_ODAS_AD at layer k (reverse running state)
-      RT iv_ad = ivn;
-      RT ap1_ad = RT(0);
-#pragma unroll
-      for (int q = 1; q < OPO; ++q) ap1_ad += RT(q) * ok[O_AP0 + q - 1] * ap_ad[q];
+      // ODPS_Compute_Predictor_ODAS_AD at layer k (ODPS_Predictor.f90:3092-3231)
       const RT ave_ad = ap1_ad * ok[O_IDEN];
-      const RT xlt_ad = xltc + ox_ad[12], xlp_ad = xlpc + ox_ad[13];
-      xltc = ox_ad[12];
-      xlpc = ox_ad[13];
+      const RT xlt_ad = xltc + e12, xlp_ad = xlpc + e13;
+      xltc = e12;
+      xlpc = e13;
       sta += RT(0.5) * ok[O_INV] * xlt_ad;
       spa += RT(0.5) * ok[O_INV] * xlp_ad;
-      const RT inv_ad = RT(0.5) * ok[O_ST] * xlt_ad + RT(0.5) * ok[O_SP] * xlp_ad;
-      iv_ad -= inv_ad * ok[O_IIV2];
-      RT dab_ad = da_ad + ok[O_PR] * spa + ok[O_T] * sta;
-      t_ad += ok[O_DA] * sta;
-      iv_ad += RT(0.5) * ave_ad;
+      const RT inv_ad = RT(0.5) * (ok[O_ST] * xlt_ad + ok[O_SP] * xlp_ad);
+      const RT iv_ad = ivn - inv_ad * ok[O_IIV2] + RT(0.5) * ave_ad;
       ivn = RT(0.5) * ave_ad + iv_ad;
-      dab_ad += iv_ad;
+      const RT dab_ad = da_ad + ok[O_PR] * spa + ok[O_T] * sta + iv_ad;
+      t_ad += ok[O_DA] * sta;
       vap_ad += ok[O_DPSEC] * dab_ad;
-      const RT tk = ok[O_T], pk = ok[O_PR];
-      t_ad += ox_ad[1] + pk * ox_ad[5] + pk * pk * ox_ad[7] +
-              RT(2) * tk * (ox_ad[3] + pk * ox_ad[6] + pk * pk * ox_ad[8] - ok[O_VT4] * ox_ad[11]);
-      vap_ad += ox_ad[10] + ox_ad[11] / (tk * tk);
     }
     finish_layer<G, RT>(uk, a);
     t_ad += a.T * uk[U_IRT] + a.dt;
@@ -491,31 +563,39 @@ __global__ void __launch_bounds__(WAVE) clear_fwdk_kernel(DevSensor s, DevBatch 
     tzps += a.tzp * uk[U_ITZPREF];
     tzs += a.tz * uk[U_ITZREF];
     t_ad += tzs + pdp * tzps;
-    // Map_Input_AD: scatter onto the user layers averaged into this ODPS layer
-    const int i1 = acc_i[2 * k], n = acc_i[2 * k + 1];
-    const RT* aw = acc_w + (size_t)k * MAXW;
-    for (int q = 0; q < n; ++q) tk_acc[i1 + q] += aw[q] * t_ad;
+    // Map_Input_AD (ODPS_CoordinateMapping.f90:480-547): add onto the user
+    // layers averaged into this ODPS layer (their Planck part is already stored)
+    if (valid) {
+      const RT* aw = acc_w + (size_t)k * MAXW;
+      for (int q = 0; q < n; ++q) {
+        KT* d = tko + (int64_t)(i1 + q) * Lt;
+        *d = (KT)((RT)*d + aw[q] * t_ad);
+      }
 #pragma unroll
-    for (int j = 0; j < Tg::NABS; ++j) {
-      const int ju = bt.jmap[j];
-      if (ju >= 0 && uk[U_MASK0 + j] != RT(0))
-        for (int q = 0; q < n; ++q) ak_acc[ju][i1 + q] += aw[q] * a_ad[j];
+      for (int j = 0; j < Tg::NABS; ++j) {
+        const int ju = bt.jmap[j];
+        if (ju >= 0 && uk[U_MASK0 + j] != RT(0))
+          for (int q = 0; q < n; ++q) {
+            KT* d = ako + ju * astr + (int64_t)(i1 + q) * Lt;
+            *d = (KT)((RT)*d + aw[q] * a_ad[j]);
+          }
+      }
     }
+    if (k > 0 && tid < TABW) tab[(k - 1) & 1][tid] = nxt;
+    __syncthreads();
   }
-  if (valid) {
-    const int64_t Lt = bt.Ltot;
-    for (int k = 0; k < K; ++k) store_k<RT, KT>(o.t_k, (loff + k) * Lt + lg, tk_acc[k]);
-    for (int j = 0; j < na; ++j)
-      for (int k = 0; k < K; ++k)
-        store_k<RT, KT>(o.a_k, ((int64_t)j * bt.NT + loff + k) * Lt + lg, ak_acc[j][k]);
+  while (i >= 0) {   // levels above the first ODPS level (none on a common top pressure)
+    ustep(i);
+    --i;
   }
 }
 
 template <int G, class RT, class KT, bool DOK>
 void launch_fwdk(const DevSensor& s, const DevBatch& b, const ProfWork<RT>& w, const DevOut& o,
                  hipStream_t stream) {
-  const int64_t n = (int64_t)s.n_blocks * b.M;
-  hipLaunchKernelGGL((clear_fwdk_kernel<G, RT, KT, DOK>), dim3((unsigned)n), dim3(WAVE), 0, stream, s, b, w, o);
+  const int64_t nbg = (s.n_blocks + BPW - 1) / BPW;
+  const int64_t n = 8 * ((nbg * b.M + 7) / 8);
+  hipLaunchKernelGGL((clear_fwdk_kernel<G, RT, KT, DOK>), dim3((unsigned)n), dim3(TPB), 0, stream, s, b, w, o);
 }
 
 #define INST(G, RT, KT, DOK) \

@@ -409,17 +409,26 @@ void build_sensor(Sensor& S, int chan_offset) {
         omax = std::max(omax, t.order[l]);
         pmask |= 1;
         const int np = t.op_index[(size_t)(t.n_opindex + 1) * l];
-        for (int i = 1; i <= np; ++i) pmask |= 1 << t.op_index[(size_t)(t.n_opindex + 1) * l + i];
+        for (int i = 1; i <= np; ++i) {
+          const int p = t.op_index[(size_t)(t.n_opindex + 1) * l + i];
+          if (p < 1 || p > N_OPTRAN_PRED) throw CrtmError(fmt("channel %d: OPTRAN predictor index %d", l + 1, p));
+          pmask |= 1 << p;
+        }
       }
     if (any) {
       if (omax > MAX_OPTRAN_ORDER) throw CrtmError("OPTRAN order exceeds 10");
-      obase[b] = ototal;
-      ototal += (int64_t)OPZ * OPO * WAVE;
-      mt[BM_OPMASK] = pmask;
+      int nrow = 0;
+      for (int p = 0; p < OPZ; ++p)
+        if (pmask >> p & 1) mt[BM_OROW0 + nrow++] = p;
+      mt[BM_ONROW] = nrow;
       mt[BM_OMAXO] = omax;
+      obase[b] = ototal;
+      ototal += (int64_t)nrow * (omax + 1) * WAVE;
     }
   }
-  std::vector<float> cblk(std::max<int64_t>(total, 1), 0.0f);
+  // padded so a wave may read a component's full predictor range past the
+  // last slot of the last block
+  std::vector<float> cblk(total + (int64_t)NX_MAX * WAVE, 0.0f);
   for (int b = 0; b < nb; ++b) {
     const int* mt = &meta[(size_t)b * BM_N];
     const int nsl = mt[BM_NSL];
@@ -443,14 +452,18 @@ void build_sensor(Sensor& S, int chan_offset) {
     for (int l = 0; l < L; ++l) {
       if (!lane_opt[l]) continue;
       const int b = l / WAVE, lane = l % WAVE;
+      const int* mt = &meta[(size_t)b * BM_N];
+      const int nq = mt[BM_OMAXO] + 1;
       const int np = t.op_index[(size_t)(t.n_opindex + 1) * l];
       const int no = t.order[l];
       for (int i = 0; i <= np; ++i) {
         const int p = i == 0 ? 0 : t.op_index[(size_t)(t.n_opindex + 1) * l + i];
+        int r = 0;
+        while (mt[BM_OROW0 + r] != p) ++r;
         const int64_t js = t.opos[l] - 1 + (int64_t)i * (no + 1);
         for (int q = 0; q <= no; ++q) {
           if (js + q < 0 || js + q >= t.n_ocoeffs) throw CrtmError("OPos_Index out of range");
-          ocd[obase[b] + ((int64_t)p * OPO + q) * WAVE + lane] += t.oc[js + q];
+          ocd[obase[b] + ((int64_t)r * nq + q) * WAVE + lane] += t.oc[js + q];
         }
       }
     }

@@ -45,6 +45,9 @@ enum UIdx {
   U_CO, U_N2O, U_CH4, U_N2OA, U_N2OR, U_COA, U_COR, U_COZP, U_COQ, U_CH4A, U_CH4R,
   U_CH4ZP, U_CH4Q, U_N2OZP, U_SQCOR, U_SQCH4R, U_N2OPOW, U_SQCOZP,
   U_IRT, U_PDP, U_ITZREF, U_ITZPREF, U_TODPS,
+  // reciprocals so the adjoint chain needs no divisions
+  U_IO3P, U_IN2OZP, U_ICOR, U_ISQCOR, U_ISQCOZP, U_ISQCH4R, U_ICH4ZP, U_IST, U_ICH4R, U_ICOZP,
+  U_IN2OR, U_IO3R, U_IGT0, U_IH2OR4, U_IH2OR,
   U_IRA0,                 // 6 entries: 1/ref_absorber(j) (0 where the AD is killed)
   U_IGAZP0 = U_IRA0 + 6,  // 6 entries: 1/GAzp_ref(j)
   U_IGATZP0 = U_IGAZP0 + 6,
@@ -57,9 +60,11 @@ enum UIdx {
 enum OIdx {
   O_X0 = 0,               // OX(0..14), OX(0) = 1
   O_AP0 = O_X0 + OPZ,     // Ap(0..10), Ap(0) = 1
-  O_DA = O_AP0 + OPO,
-  O_AP1, O_IDEN, O_INV, O_ST, O_SP, O_IIV2, O_DPSEC, O_PR, O_T, O_VAP, O_VT4,
-  NO = O_VT4 + 1
+  O_DX_DT0 = O_AP0 + OPO, // dOX(p)/dT
+  O_DX_DV0 = O_DX_DT0 + OPZ, // dOX(p)/dVapour
+  O_DA = O_DX_DV0 + OPZ,
+  O_IDEN, O_INV, O_ST, O_SP, O_IIV2, O_DPSEC, O_PR, O_T,
+  NO = O_T + 1
 };
 
 // Sensor resident in HBM (built by crtm_init from the coefficient files)
@@ -88,7 +93,8 @@ struct DevSensor {
   const int64_t* blk_base;  // [n_blocks] float offset of the block
   const int* blk_meta;      // [n_blocks][BM_N]
   const float* cblk;
-  // OPTRAN per block: [p][o][lane] doubles, dense 15 x 11
+  // OPTRAN per block: [row][order][lane] doubles over the block's active
+  // predictor rows (blk_meta BM_OROW0..) and orders 0..BM_OMAXO
   const int64_t* oblk_base; // [n_blocks], -1 when the block has no OPTRAN lane
   const double* ocd;
   const int* lane_optran;   // [n_blocks*64] 1 if the lane's WLO OD comes from OPTRAN
@@ -101,8 +107,8 @@ struct DevSensor {
 };
 
 // blk_meta layout
-enum BMeta { BM_NSL = 0, BM_NP0 = 1, BM_OFF0 = BM_NP0 + NCOMP_MAX, BM_OPMASK = BM_OFF0 + NCOMP_MAX,
-             BM_OMAXO, BM_N };
+enum BMeta { BM_NSL = 0, BM_NP0 = 1, BM_OFF0 = BM_NP0 + NCOMP_MAX, BM_ONROW = BM_OFF0 + NCOMP_MAX,
+             BM_OMAXO, BM_OROW0, BM_N = BM_OROW0 + OPZ };
 
 // Batch of profiles (device pointers)
 struct DevBatch {

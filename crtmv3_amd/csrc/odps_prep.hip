@@ -337,6 +337,22 @@ __global__ void __launch_bounds__(PREP_THREADS) odps_prep_kernel(DevSensor s, De
       XS(2, 8, h2o_a * dt); XS(2, 9, (sec * h2op) * (sec * h2op)); XS(2, 10, sec * h2op);
       XS(2, 11, sec); XS(2, 12, h2o_s * h2o_a); XS(2, 13, h2o_s * h2o_s); XS(2, 14, h2o_dtzp);
     }
+    auto inv = [](double v) { return v != 0.0 ? 1.0 / v : 0.0; };
+    U[U_IO3P] = inv(U[U_O3P]);
+    U[U_IN2OZP] = inv(U[U_N2OZP]);
+    U[U_ICOR] = inv(U[U_COR]);
+    U[U_ISQCOR] = inv(U[U_SQCOR]);
+    U[U_ISQCOZP] = inv(U[U_SQCOZP]);
+    U[U_ISQCH4R] = inv(U[U_SQCH4R]);
+    U[U_ICH4ZP] = inv(U[U_CH4ZP]);
+    U[U_IST] = inv(U[U_ST]);
+    U[U_ICH4R] = inv(U[U_CH4R]);
+    U[U_ICOZP] = inv(U[U_COZP]);
+    U[U_IN2OR] = inv(U[U_N2OR]);
+    U[U_IO3R] = inv(U[U_O3R]);
+    U[U_IGT0] = inv(gt0);
+    U[U_IH2OR4] = inv(h2o_r4);
+    U[U_IH2OR] = inv(h2o_r);
     U[U_IRT] = 1.0 / rt;
     U[U_PDP] = s.pdp[k];
     U[U_ITZREF] = 1.0 / s.tz_ref[k];
@@ -368,9 +384,18 @@ __global__ void __launch_bounds__(PREP_THREADS) odps_prep_kernel(DevSensor s, De
       const double ap1 = log((ave_s[k] - s.alpha_c2) / s.alpha_c1) / s.alpha;
       O[O_AP0] = 1.0;
       for (int i = 1; i <= MAX_OPTRAN_ORDER; ++i) O[O_AP0 + i] = O[O_AP0 + i - 1] * ap1;
+      for (int p = 0; p < OPZ; ++p) O[O_DX_DT0 + p] = O[O_DX_DV0 + p] = 0.0;
+      O[O_DX_DT0 + 1] = 1.0;
+      O[O_DX_DT0 + 3] = 2.0 * tk;
+      O[O_DX_DT0 + 5] = pk;
+      O[O_DX_DT0 + 6] = 2.0 * tk * pk;
+      O[O_DX_DT0 + 7] = p2;
+      O[O_DX_DT0 + 8] = 2.0 * tk * p2;
+      O[O_DX_DT0 + 11] = -2.0 * vap / (t2 * tk);
+      O[O_DX_DV0 + 10] = 1.0;
+      O[O_DX_DV0 + 11] = 1.0 / t2;
       const double dpong = RECIP_GRAV * (s.ref_lp[k + 1] - s.ref_lp[k]);
       O[O_DA] = dpong * vap * sec;
-      O[O_AP1] = ap1;
       O[O_IDEN] = 1.0 / (s.alpha * (ave_s[k] - s.alpha_c2));
       O[O_INV] = iv_s[k] > MIN_ABSORBER ? 1.0 / iv_s[k] : 0.0;
       O[O_ST] = st_s[k];
@@ -379,8 +404,6 @@ __global__ void __launch_bounds__(PREP_THREADS) odps_prep_kernel(DevSensor s, De
       O[O_DPSEC] = dpong * sec;
       O[O_PR] = pk;
       O[O_T] = tk;
-      O[O_VAP] = vap;
-      O[O_VT4] = vap / (t2 * t2);
       RT* oo = w.o + ((size_t)m * Ko + k) * NO;
       for (int i = 0; i < NO; ++i) oo[i] = (RT)O[i];
     }
