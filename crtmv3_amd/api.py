@@ -96,6 +96,7 @@ def lib():
                                    ctypes.POINTER(CRTSolution)]
         L.crtm_k_matrix.argtypes = L.crtm_forward.argtypes + [ctypes.POINTER(CKOutput)]
         L.crtm_synchronize.argtypes = [_vp]
+        L.crtm_set_stream.argtypes = [_vp, _vp, ctypes.c_int]
         L.crtm_last_error.argtypes = [_vp]
         L.crtm_last_error.restype = ctypes.c_char_p
         L.crtm_profile_enable.argtypes = [_vp, ctypes.c_int]
@@ -143,6 +144,17 @@ class Context:
         except Exception:
             pass
 
+    def _use_torch_stream(self):
+        """Device-memory calls run on torch's current stream, so they are
+        ordered after the kernels that produced their inputs and before the
+        ones that consume their outputs."""
+        import torch
+        h = torch.cuda.current_stream().cuda_stream
+        if h != getattr(self, "_stream", None):
+            if lib().crtm_set_stream(self._h, h, 0) != SUCCESS:
+                self._err("crtm_set_stream")
+            self._stream = h
+
     def _err(self, what):
         msg = lib().crtm_last_error(self._h).decode()
         raise CRTMError(f"{what}: {msg}")
@@ -162,7 +174,7 @@ class Context:
         host_nl = getattr(prof, "n_layers_host", None)
         if host_nl is None and memory == MEM_HOST:
             host_nl = np.asarray(prof.n_layers)
-        if host_nl is not None:
+        if host_nl is not None and len(host_nl):
             atm.max_layers = int(np.max(host_nl))
             atm.total_layers = int(np.sum(host_nl))
         atm.n_layers = _ptr(arr(prof.n_layers, np.int32))
@@ -206,6 +218,8 @@ class Context:
     def forward(self, prof, emissivity, precision=PREC_FP64, extras=False, check_input=True):
         """CRTM_Forward over a ProfileSet (host numpy) or a device batch (torch)."""
         memory = MEM_DEVICE if _is_torch(prof.temperature) else MEM_HOST
+        if memory == MEM_DEVICE:
+            self._use_torch_stream()
         keep = []
         atm, sfc, geo, opt = self._structs(prof, emissivity, precision, memory, False, check_input, keep)
         M, L = prof.n_profiles, self.n_channels
@@ -231,6 +245,8 @@ class Context:
         to 0).  `out` = (rts, k) dicts from a previous call are reused as the
         output buffers (every element is overwritten)."""
         memory = MEM_DEVICE if _is_torch(prof.temperature) else MEM_HOST
+        if memory == MEM_DEVICE:
+            self._use_torch_stream()
         keep = []
         atm, sfc, geo, opt = self._structs(prof, emissivity, precision, memory, jacobian_f32,
                                            check_input, keep)

@@ -322,7 +322,7 @@ clear_fwdk_kernel(DevSensor s, DevBatch bt, ProfWork<RT> w, DevOut o) {
   RT path[KO_MAX + 1];
   uint64_t odclamp[2] = {0, 0}, lcclamp[2] = {0, 0};
   path[0] = RT(0);
-  tab[0][tid < TABW ? tid : 0] = tid < TABW ? row(X, Tg::NX, 0, tid) : RT(0);
+  if (tid < TABW) tab[0][tid] = row(X, Tg::NX, 0, tid);
   __syncthreads();
   for (int k = 0; k < Ko; ++k) {
     const RT nxt = (k + 1 < Ko && tid < TABW) ? row(X, Tg::NX, k + 1, tid) : RT(0);
@@ -486,13 +486,13 @@ clear_fwdk_kernel(DevSensor s, DevBatch bt, ProfWork<RT> w, DevOut o) {
   for (int j = 0; j < Tg::NABS; ++j) gzs[j] = gts[j] = RT(0);
   RT ivn = RT(0), sta = RT(0), spa = RT(0), xltc = RT(0), xlpc = RT(0);
   const int h2o_j = s.h2o_idx - 1;
-  const int* acc_i = w.acc_i + (size_t)m * Ko * 2;
-  const RT* acc_w = w.acc_w + (size_t)m * Ko * MAXW;
+  const int* acc_i = w.acc_i + (size_t)m * Ko * 3;
+  const RT* acc_w = w.acc_w + (size_t)m * NZ_MAX;
   if (tid < TABW) tab[(Ko - 1) & 1][tid] = row(Ut, NU, Ko - 1, tid);
   __syncthreads();
   for (int k = Ko - 1; k >= 0; --k) {
     const RT nxt = (k > 0 && tid < TABW) ? row(Ut, NU, k - 1, tid) : RT(0);
-    const int i1 = acc_i[2 * k], n = acc_i[2 * k + 1];
+    const int i1 = acc_i[3 * k], n = acc_i[3 * k + 1];
     // user levels whose path adjoint feeds ODPS level k+1, and every user
     // layer this ODPS layer's averaging kernel touches
     while (i >= 0 && (ui[i] >= k || i > i1)) {
@@ -566,7 +566,7 @@ clear_fwdk_kernel(DevSensor s, DevBatch bt, ProfWork<RT> w, DevOut o) {
     // Map_Input_AD (ODPS_CoordinateMapping.f90:480-547): add onto the user
     // layers averaged into this ODPS layer (their Planck part is already stored)
     if (valid) {
-      const RT* aw = acc_w + (size_t)k * MAXW;
+      const RT* aw = acc_w + acc_i[3 * k + 2];
       for (int q = 0; q < n; ++q) {
         KT* d = tko + (int64_t)(i1 + q) * Lt;
         *d = (KT)((RT)*d + aw[q] * t_ad);

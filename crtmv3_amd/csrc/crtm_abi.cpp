@@ -505,7 +505,8 @@ void build_sensor(Sensor& S, int chan_offset) {
 // ---------------------------------------------------------------------------
 struct crtm_context {
   int device = 0;
-  hipStream_t stream = nullptr;
+  hipStream_t own_stream = nullptr;  // created by crtm_init
+  hipStream_t stream = nullptr;      // where work is issued: own_stream or the caller's (crtm_set_stream)
   std::vector<std::unique_ptr<Sensor>> sensors;
   int n_channels = 0;
   std::string last_error;
@@ -543,8 +544,8 @@ ProfWork<RT> make_work(crtm_context* c, int M, int Ko, int nx, int max_k) {
   c->wx.reserve((size_t)M * Ko * nx * sizeof(RT));
   c->wu.reserve((size_t)M * Ko * NU * sizeof(RT));
   c->wo.reserve((size_t)M * Ko * NO * sizeof(RT));
-  c->wacci.reserve((size_t)M * Ko * 2 * sizeof(int));
-  c->waccw.reserve((size_t)M * Ko * MAXW * sizeof(RT));
+  c->wacci.reserve((size_t)M * Ko * 3 * sizeof(int));
+  c->waccw.reserve((size_t)M * NZ_MAX * sizeof(RT));
   c->wuidx.reserve((size_t)M * (max_k + 1) * sizeof(int));
   c->wufrac.reserve((size_t)M * (max_k + 1) * sizeof(RT));
   c->wgeo.reserve((size_t)M * 4 * sizeof(double));
@@ -821,7 +822,8 @@ int crtm_init(int n_sensors, const char* const* sensor_ids, const char* file_pat
   std::unique_ptr<crtm_context> ctx(new crtm_context);
   try {
     HIPCHECK(hipGetDevice(&ctx->device));
-    HIPCHECK(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
+    HIPCHECK(hipStreamCreateWithFlags(&ctx->own_stream, hipStreamNonBlocking));
+    ctx->stream = ctx->own_stream;
     std::string dir = file_path ? file_path : ".";
     if (!dir.empty() && dir.back() != '/') dir += '/';
     int off = 0;
@@ -849,7 +851,7 @@ int crtm_destroy(crtm_context* ctx) {
   if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
   ctx->sensors.clear();
   for (hipEvent_t e : ctx->ev) (void)hipEventDestroy(e);
-  if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+  if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
   delete ctx;
   return CRTM_SUCCESS;
 }
@@ -888,6 +890,19 @@ int crtm_k_matrix(crtm_context* ctx, const crtm_atmosphere* atm, const crtm_surf
   } catch (const std::exception& e) {
     return fail(ctx, e.what());
   }
+}
+
+int crtm_set_stream(crtm_context* ctx, void* stream, int own_stream) {
+  if (!ctx) return CRTM_FAILURE;
+  hipStream_t next = own_stream ? ctx->own_stream : (hipStream_t)stream;
+  if (next == ctx->stream) return CRTM_SUCCESS;
+  // work already issued on the previous stream completes before the switch
+  hipError_t e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) return fail(ctx, hipGetErrorString(e));
+  ctx->stream = next;
+  ctx->ev_used = 0;
+  ctx->ev_kind.clear();
+  return CRTM_SUCCESS;
 }
 
 int crtm_synchronize(crtm_context* ctx) {

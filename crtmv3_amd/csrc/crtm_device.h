@@ -9,7 +9,7 @@ namespace crtm {
 constexpr int WAVE = 64;           // CDNA wavefront width: one channel per lane
 constexpr int KO_MAX = 128;        // ODPS internal layers supported
 constexpr int K_MAX = 200;         // MAX_N_LAYERS (src/CRTM_Parameters.f90:64)
-constexpr int MAXW = 16;           // max user layers averaged into one ODPS layer
+constexpr int NZ_MAX = 2048;       // LayerAvg weights stored per profile (all ODPS layers)
 constexpr int NA_MAX = 6;          // user absorbers carried through the adjoint
 constexpr int MAX_OPTRAN_ORDER = 10;
 constexpr int N_OPTRAN_PRED = 14;
@@ -136,8 +136,8 @@ template <class RT> struct ProfWork {
   RT* x;          // [M][KO_MAX? Ko][NX]
   RT* u;          // [M][Ko][NU]
   RT* o;          // [M][Ko][NO]
-  int* acc_i;     // [M][Ko][2]: first user layer (0-based), count
-  RT* acc_w;      // [M][Ko][MAXW]
+  int* acc_i;     // [M][Ko][3]: first user layer (0-based), count, offset into acc_w
+  RT* acc_w;      // [M][NZ_MAX]: weights of ODPS layer k at [offset, offset + count)
   int* uidx;      // [M][max_k+1]: lower ODPS level k1 (0-based); k2 = k1+1 unless flagged
   RT* ufrac;      // [M][max_k+1]: interpolation fraction, or -1 when k1 == k2
   double* geo;    // [M][4]: sec_sfc, cos_sensor, sfc_T, (unused)

@@ -34,7 +34,7 @@ constexpr int PREP_THREADS = 128;
 }  // namespace
 
 // Status codes written per profile
-enum { ST_OK = 0, ST_LAYERAVG = 1, ST_MAXW = 2, ST_LAYERS = 3, ST_BRACKET = 4 };
+enum { ST_OK = 0, ST_LAYERAVG = 1, ST_NZ = 2, ST_LAYERS = 3, ST_BRACKET = 4 };
 
 template <int G, class RT>
 __global__ void __launch_bounds__(PREP_THREADS) odps_prep_kernel(DevSensor s, DevBatch bt, ProfWork<RT> w) {
@@ -53,8 +53,8 @@ __global__ void __launch_bounds__(PREP_THREADS) odps_prep_kernel(DevSensor s, De
   __shared__ double rlnp[KO_MAX];
   __shared__ double rlnp_lev[KO_MAX + 1];
   __shared__ double pz[K_MAX];            // LayerAvg scratch column
-  __shared__ double accw[KO_MAX][MAXW];
-  __shared__ int acc_i1[KO_MAX], acc_n[KO_MAX];
+  __shared__ double accw[NZ_MAX];          // compact LayerAvg weights, ODPS layer k at acc_off[k]
+  __shared__ int acc_i1[KO_MAX], acc_n[KO_MAX], acc_off[KO_MAX];
   __shared__ double t_o[KO_MAX];
   __shared__ double a_o[6][KO_MAX];
   __shared__ double sec_o[KO_MAX];
@@ -86,6 +86,7 @@ __global__ void __launch_bounds__(PREP_THREADS) odps_prep_kernel(DevSensor s, De
     auto P2 = [&](int i) { return ulnp[i - 1]; };
     int istart = 1;
     int st = ST_OK;
+    int nz = 0;
     for (int ki = 1; ki <= kn1 && st == ST_OK; ++ki) {
       double z2 = P1(ki);
       double z1 = (ki == 1) ? 2.0 * z2 - P1(ki + 1) : P1(ki - 1);
@@ -147,10 +148,12 @@ __global__ void __launch_bounds__(PREP_THREADS) odps_prep_kernel(DevSensor s, De
       for (int ii = i1 + 1; ii <= kn2; ++ii)
         if (ii > hi || pz[ii - 1] == 0.0) { i2 = ii - 1; break; }
       int n = i2 - i1 + 1;
-      if (n > MAXW) { st = ST_MAXW; break; }
+      if (nz + n > NZ_MAX) { st = ST_NZ; break; }
       double zsum = 0.0;
       for (int ii = i1; ii <= i2; ++ii) zsum += pz[ii - 1];
-      for (int ii = i1; ii <= i2; ++ii) accw[ki - 1][ii - i1] = pz[ii - 1] / zsum;
+      for (int ii = i1; ii <= i2; ++ii) accw[nz + ii - i1] = pz[ii - 1] / zsum;
+      acc_off[ki - 1] = nz;
+      nz += n;
       acc_i1[ki - 1] = i1 - 1;
       acc_n[ki - 1] = n;
       for (int ii = lo; ii <= hi; ++ii) pz[ii - 1] = 0.0;
@@ -200,7 +203,7 @@ __global__ void __launch_bounds__(PREP_THREADS) odps_prep_kernel(DevSensor s, De
   for (int k = tid; k < Ko; k += PREP_THREADS) {
     const int i1 = acc_i1[k], n = acc_n[k];
     double sum = 0.0;
-    for (int q = 0; q < n; ++q) sum += accw[k][q] * t_u[i1 + q];
+    for (int q = 0; q < n; ++q) sum += accw[acc_off[k] + q] * t_u[i1 + q];
     t_o[k] = sum;
     for (int j = 0; j < T::NABS; ++j) {
       const int ju = bt.jmap[j];
@@ -208,7 +211,7 @@ __global__ void __launch_bounds__(PREP_THREADS) odps_prep_kernel(DevSensor s, De
       if (ju >= 0) {
         const double* au = bt.a + (size_t)ju * bt.NT + loff;
         double sm = 0.0;
-        for (int q = 0; q < n; ++q) sm += accw[k][q] * au[i1 + q];
+        for (int q = 0; q < n; ++q) sm += accw[acc_off[k] + q] * au[i1 + q];
         v = sm;
         if (v <= TOLERANCE) v = TOLERANCE;
         const double mn = s.min_a[(size_t)j * Ko + k], mx = s.max_a[(size_t)j * Ko + k];
@@ -407,11 +410,12 @@ __global__ void __launch_bounds__(PREP_THREADS) odps_prep_kernel(DevSensor s, De
       RT* oo = w.o + ((size_t)m * Ko + k) * NO;
       for (int i = 0; i < NO; ++i) oo[i] = (RT)O[i];
     }
-    int* ai = w.acc_i + ((size_t)m * Ko + k) * 2;
+    int* ai = w.acc_i + ((size_t)m * Ko + k) * 3;
     ai[0] = acc_i1[k];
     ai[1] = acc_n[k];
-    RT* aw = w.acc_w + ((size_t)m * Ko + k) * MAXW;
-    for (int q = 0; q < MAXW; ++q) aw[q] = q < acc_n[k] ? (RT)accw[k][q] : RT(0);
+    ai[2] = acc_off[k];
+    RT* aw = w.acc_w + (size_t)m * NZ_MAX + acc_off[k];
+    for (int q = 0; q < acc_n[k]; ++q) aw[q] = (RT)accw[acc_off[k] + q];
   }
 
   // ---- ODPS level brackets of the user levels (Compute_Interp_Index) ----

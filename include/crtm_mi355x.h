@@ -159,6 +159,15 @@ int crtm_k_matrix(crtm_context* ctx, const crtm_atmosphere* atm, const crtm_surf
                   const crtm_geometry* geo, const crtm_options* opt, crtm_rtsolution* rts,
                   crtm_k_output* k);
 
+/* Issue the context's work on `stream` (a hipStream_t of the context's
+ * device, e.g. the caller's framework stream; NULL is the null stream)
+ * instead of the context's own stream; own_stream != 0 restores the own
+ * stream and ignores `stream`.  Device-memory calls are then
+ * ordered with the caller's other work on that stream.  The reference has
+ * no streams (its calls are synchronous); this is the MI355X analogue of
+ * the OpenMP execution-context choice in src/CRTM_Forward_Module.f90:337-357. */
+int crtm_set_stream(crtm_context* ctx, void* stream, int own_stream);
+
 /* Wait for the context stream (device-memory calls are asynchronous). */
 int crtm_synchronize(crtm_context* ctx);
 

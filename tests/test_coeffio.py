@@ -1,0 +1,107 @@
+"""Coefficient files in CRTM binary format: the package writer/reader round
+trip, and the reference's own reader (Read_ODPS_Binary,
+src/Coefficients/TauCoeff/ODPS/ODPS_Binary_IO.f90:481, built under
+oracle/_ref) loading the synthetic files to the same arrays."""
+from __future__ import annotations
+
+import ctypes
+import os
+import struct
+
+import numpy as np
+import pytest
+
+from conftest import SMALL_SENSORS, load_sensor
+
+
+@pytest.mark.parametrize("name", SMALL_SENSORS)
+def test_odps_roundtrip(coeff_dir, tmp_path, name):
+    from crtmv3_amd import coeffio
+    tau, spc = load_sensor(coeff_dir, name)
+    p = str(tmp_path / "t.bin")
+    coeffio.write_odps(p, tau)
+    back = coeffio.read_odps(p)
+    for f in ("ref_level_pressure", "ref_pressure", "ref_temperature", "ref_absorber", "min_absorber",
+              "max_absorber", "n_predictors", "pos_index", "c", "absorber_id", "component_id"):
+        np.testing.assert_array_equal(getattr(back, f), getattr(tau, f), err_msg=f)
+    assert (back.group_index, back.sensor_id, back.sensor_type) == (tau.group_index, tau.sensor_id,
+                                                                    tau.sensor_type)
+    # identical bytes on a second write
+    p2 = str(tmp_path / "t2.bin")
+    coeffio.write_odps(p2, back)
+    assert open(p, "rb").read() == open(p2, "rb").read()
+
+
+@pytest.mark.parametrize("name", SMALL_SENSORS)
+def test_spccoeff_roundtrip(coeff_dir, tmp_path, name):
+    from crtmv3_amd import coeffio
+    _, spc = load_sensor(coeff_dir, name)
+    p = str(tmp_path / "s.bin")
+    coeffio.write_spccoeff(p, spc)
+    back = coeffio.read_spccoeff(p)
+    for f in ("sensor_channel", "frequency", "wavenumber", "planck_c1", "planck_c2", "band_c1", "band_c2",
+              "cosmic_background_radiance"):
+        np.testing.assert_array_equal(getattr(back, f), getattr(spc, f), err_msg=f)
+
+
+def test_fortran_record_framing(coeff_dir):
+    """Sequential unformatted records: 4-byte length markers around each
+    record, the first record holding the endian magic number 123456789
+    (src/Utility/Binary_File_Utility.f90)."""
+    path = os.path.join(coeff_dir, "amsua_syn.TauCoeff.bin")
+    raw = open(path, "rb").read()
+    n = struct.unpack("<i", raw[:4])[0]
+    assert n == 4 and struct.unpack("<i", raw[4:8])[0] == 123456789
+    assert struct.unpack("<i", raw[8:12])[0] == 4
+    pos, count = 0, 0
+    while pos < len(raw):
+        n = struct.unpack("<i", raw[pos:pos + 4])[0]
+        assert struct.unpack("<i", raw[pos + 4 + n:pos + 8 + n])[0] == n
+        pos += n + 8
+        count += 1
+    assert pos == len(raw) and count > 5
+
+
+def _ref_lib():
+    from oracle import oracle as orc
+    if not os.path.exists(orc.REF_PATH):
+        pytest.skip("oracle/_ref not built (needs the reference tree)")
+    L = ctypes.CDLL(orc.REF_PATH)
+    L.ref_odps_load.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_void_p]
+    return L
+
+
+@pytest.mark.parametrize("name", SMALL_SENSORS)
+def test_reference_reader_agrees(coeff_dir, name):
+    L = _ref_lib()
+    tau, _ = load_sensor(coeff_dir, name)
+    path = os.path.join(coeff_dir, f"{name}.TauCoeff.bin").encode()
+    dims = np.zeros(9, np.int32)
+    assert L.ref_odps_load(path, len(path), dims.ctypes.data) == 0
+    Ko, J, Jm, Lc, nC, nO, grp, stype, ocomp = [int(x) for x in dims]
+    assert (Ko, J, Jm, Lc, nC, grp, stype) == (tau.n_layers, tau.n_components, tau.n_absorbers,
+                                              tau.n_channels, tau.c.size, tau.group_index, tau.sensor_type)
+    assert nO == tau.n_ocoeffs
+    a = {k: np.zeros(n) for k, n in (("lp", Ko + 1), ("p", Ko), ("t", Ko), ("ra", Ko * Jm),
+                                     ("mn", Ko * Jm), ("mx", Ko * Jm), ("oc", max(nO, 1)), ("al", 3))}
+    npred = np.zeros(J * Lc, np.int32)
+    pos = np.zeros(J * Lc, np.int32)
+    c = np.zeros(nC, np.float32)
+    osig, order, opos = (np.zeros(Lc, np.int32) for _ in range(3))
+    opidx = np.zeros(7 * Lc, np.int32)
+    L.ref_odps_fetch(*[x.ctypes.data_as(ctypes.c_void_p) for x in (
+        a["lp"], a["p"], a["t"], a["ra"], a["mn"], a["mx"], npred, pos, c, osig, order, opidx, opos,
+        a["oc"], a["al"])])
+    np.testing.assert_array_equal(a["lp"], tau.ref_level_pressure)
+    np.testing.assert_array_equal(a["t"], tau.ref_temperature)
+    np.testing.assert_array_equal(a["ra"], np.ravel(tau.ref_absorber, order="F"))
+    np.testing.assert_array_equal(a["mx"], np.ravel(tau.max_absorber, order="F"))
+    np.testing.assert_array_equal(npred, np.ravel(tau.n_predictors, order="F"))
+    np.testing.assert_array_equal(pos, np.ravel(tau.pos_index, order="F"))
+    np.testing.assert_array_equal(c, tau.c)
+    if nO:
+        assert ocomp == tau.ocomponent_index
+        np.testing.assert_array_equal(osig, tau.osignificance)
+        np.testing.assert_array_equal(opidx, np.ravel(tau.op_index, order="F"))
+        np.testing.assert_array_equal(a["oc"], tau.oc)
+        np.testing.assert_array_equal(a["al"], [tau.alpha, tau.alpha_c1, tau.alpha_c2])

@@ -1,0 +1,142 @@
+"""GPU parity: the HIP forward and K-matrix kernels, called through the
+C-ABI (libcrtm_mi355x.so), against the FP64 oracle on the same seeded
+inputs.
+
+Tolerances (brightness temperature in K, Jacobians relative to the largest
+element of the same Jacobian block):
+  FP64 path        BT 1e-8 K,  Jacobians 1e-9
+  FP32-split path  BT 1e-2 K,  Jacobians 1e-3
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from conftest import SMALL_SENSORS, load_sensor
+
+pytestmark = pytest.mark.gpu
+
+TOL = {0: dict(bt=1e-8, jac=1e-9), 1: dict(bt=1e-2, jac=1e-3)}
+
+
+@pytest.fixture(scope="module")
+def gpu():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch.device("cuda:0")
+
+
+@pytest.fixture(scope="module")
+def contexts(coeff_dir, gpu):
+    from crtmv3_amd import api
+    ctxs = {name: api.crtm_init([name], coeff_dir) for name in SMALL_SENSORS}
+    yield ctxs
+    for c in ctxs.values():
+        c.destroy()
+
+
+def _oracle_batch(orc, coeff_dir, name, ps, eps, bt_k=None):
+    tau, spc = load_sensor(coeff_dir, name)
+    s = orc.Sensor(tau, spc)
+    return [orc.run_profile(s, ps, m, eps[m], "k", bt_k=None if bt_k is None else bt_k[m])
+            for m in range(ps.n_profiles)]
+
+
+def _check(ps, rts, kd, ref, prec, L, check_k=True):
+    tol = TOL[prec]
+    bt_ref = np.stack([r["bt"] for r in ref])
+    assert np.abs(rts["brightness_temperature"] - bt_ref).max() <= tol["bt"]
+    rad_ref = np.stack([r["radiance"] for r in ref])
+    np.testing.assert_allclose(rts["radiance"], rad_ref, rtol=1e-3 if prec else 1e-10)
+    if not check_k:
+        return
+    for m, r in enumerate(ref):
+        a, b = int(ps.layer_offset[m]), int(ps.layer_offset[m + 1])
+        tk = np.asarray(kd["temperature_k"][a:b], np.float64).T
+        ak = np.asarray(kd["absorber_k"][:, a:b, :], np.float64).transpose(2, 0, 1)
+        for got, want, what in ((tk, r["t_k"], "T_K"), (ak, r["abs_k"], "absorber_K")):
+            err = np.abs(got - want).max() / np.abs(want).max()
+            assert err <= tol["jac"], f"profile {m} {what} rel err {err:.3e}"
+        for got, want, what in ((kd["water_temperature_k"][m], r["sfc_t_k"][:, 1], "Ts_K"),
+                                (kd["surface_emissivity_k"][m], r["emis_k"], "emis_K")):
+            err = np.abs(got - want).max() / max(np.abs(want).max(), 1e-30)
+            assert err <= tol["jac"], f"profile {m} {what} rel err {err:.3e}"
+
+
+@pytest.mark.parametrize("prec", [0, 1], ids=["fp64", "split"])
+@pytest.mark.parametrize("name", SMALL_SENSORS)
+def test_k_matrix_matches_oracle_uneven_layers(contexts, coeff_dir, oracle_mod, name, prec):
+    from crtmv3_amd import synthetic
+    ctx = contexts[name]
+    L = ctx.n_channels
+    ks = [100, 60, 137, 83, 100, 71]
+    ps = synthetic.make_profiles(len(ks), ks, seed=101)
+    rng = np.random.default_rng(5)
+    eps = rng.uniform(0.5, 0.99, (len(ks), L))
+    btk = rng.uniform(0.5, 1.5, (len(ks), L))
+    print(f"case {name} prec={prec} M={len(ks)} layers={ks}", flush=True)
+    ref = _oracle_batch(oracle_mod, coeff_dir, name, ps, eps, btk)
+    rts, kd = ctx.k_matrix(ps, eps, bt_k=btk, precision=prec, jacobian_f32=bool(prec))
+    _check(ps, rts, kd, ref, prec, L)
+    fwd = ctx.forward(ps, eps, precision=prec, extras=True)
+    _check(ps, fwd, None, ref, prec, L, check_k=False)
+    # forward and K-matrix share the forward half: identical BT
+    np.testing.assert_array_equal(fwd["brightness_temperature"], rts["brightness_temperature"])
+
+
+@pytest.mark.parametrize("prec", [0, 1], ids=["fp64", "split"])
+def test_device_memory_equals_host_memory(contexts, gpu, prec):
+    import torch
+    from crtmv3_amd import api, synthetic
+    name = "cris_small_syn"
+    ctx = contexts[name]
+    L = ctx.n_channels
+    ps = synthetic.make_profiles(5, [90, 100, 64, 120, 100], seed=7)
+    eps = np.full((5, L), 0.97)
+    print(f"case device-vs-host {name} prec={prec}", flush=True)
+    h_rts, h_k = ctx.k_matrix(ps, eps, precision=prec, jacobian_f32=False)
+    dps = api.device_profiles(ps, gpu)
+    d_rts, d_k = ctx.k_matrix(dps, torch.as_tensor(eps, device=gpu), precision=prec, jacobian_f32=False)
+    ctx.synchronize()
+    np.testing.assert_array_equal(d_rts["brightness_temperature"].cpu().numpy(), h_rts["brightness_temperature"])
+    np.testing.assert_array_equal(d_k["temperature_k"].cpu().numpy(), h_k["temperature_k"])
+    np.testing.assert_array_equal(d_k["absorber_k"].cpu().numpy(), h_k["absorber_k"])
+
+
+def test_high_zenith_and_scan(contexts, coeff_dir, oracle_mod):
+    """Zenith beyond the 63.6122 deg transmittance cap."""
+    from crtmv3_amd import synthetic
+    name = "hirs_syn"
+    ctx = contexts[name]
+    ps = synthetic.make_profiles(3, 100, seed=77)
+    ps.sensor_zenith[:] = [63.0, 64.0, 72.0]
+    eps = np.full((3, ctx.n_channels), 0.95)
+    print("case high zenith", flush=True)
+    ref = _oracle_batch(oracle_mod, coeff_dir, name, ps, eps)
+    rts, kd = ctx.k_matrix(ps, eps, precision=0)
+    _check(ps, rts, kd, ref, 0, ctx.n_channels)
+
+
+def test_empty_batch_is_a_noop(contexts):
+    from crtmv3_amd import synthetic
+    ctx = contexts["amsua_syn"]
+    ps = synthetic.make_profiles(0, 100, seed=1)
+    out = ctx.forward(ps, np.zeros((0, ctx.n_channels)))
+    assert out["brightness_temperature"].shape == (0, ctx.n_channels)
+
+
+def test_invalid_inputs_are_reported(contexts):
+    from crtmv3_amd import api, synthetic
+    ctx = contexts["amsua_syn"]
+    eps = np.full((2, ctx.n_channels), 0.6)
+    ps = synthetic.make_profiles(2, [100, 201], seed=1)      # CRTM_MAX_N_LAYERS = 200
+    with pytest.raises(api.CRTMError, match="n_layers"):
+        ctx.forward(ps, eps)
+    ps = synthetic.make_profiles(2, 100, seed=1)
+    ps.temperature[150] = -1.0
+    with pytest.raises(api.CRTMError, match="profile #2"):
+        ctx.forward(ps, eps, check_input=True)
+    # the context stays usable after an error
+    ok = ctx.forward(synthetic.make_profiles(2, 100, seed=1), eps)
+    assert np.all(np.isfinite(ok["brightness_temperature"]))
