@@ -104,7 +104,7 @@ def main():
     ap.add_argument("--profiles", type=int, default=4096)
     ap.add_argument("--sensor", default="cris_syn")
     ap.add_argument("--fp64-steps", type=int, default=2)
-    ap.add_argument("--cpu-seconds", type=float, default=10.0)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--split-only", action="store_true", help="skip the FP64 comparison leg")
     args = ap.parse_args()
