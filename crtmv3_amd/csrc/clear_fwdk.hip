@@ -234,63 +234,74 @@ __device__ inline RT comp_layer_od(const float* ck, const int* npb, const int* c
   }
 }
 
-// OPTRAN layer OD pieces of one block at layer k: ln(chi) and, for the
-// adjoint, the per-row contributions (Add_OPTRAN_wloOD, ODPS_AtmAbsorption.f90:664-727)
-template <class RT>
-__device__ inline RT optran_lnchi(const double* ocb, const int* meta, const RT* ok) {
+// Workgroup shape per arithmetic type, from measurements at the benchmark
+// shape (2211 channels x 1024 profiles): FP32 runs best with 16 profiles per
+// workgroup at 4 waves per SIMD (<= 128 VGPRs); FP64 needs more registers and
+// runs best with 8 profiles, a single LDS buffer and no wave-count target.
+//   PW   profiles (waves) per workgroup, all on one 64-channel block
+//   NBUF LDS buffers per staged layer (2: the next layer's stores overlap
+//        the current layer's reads; 1: half the LDS, one more barrier)
+//   WPE  waves per SIMD the register allocation targets (0: compiler's choice)
+template <class RT> struct KCfg { static constexpr int PW = 16, NBUF = 2, WPE = 4; };
+template <> struct KCfg<double> { static constexpr int PW = 8, NBUF = 1, WPE = 0; };
+// LDS row of one wave: predictors X or adjoint table U, followed by the OPTRAN table O
+constexpr int TABW = (NX_MAX > NU ? NX_MAX : NU) + NO;
+constexpr int RQ = (TABW + WAVE - 1) / WAVE;   // row elements loaded per lane
+constexpr int OC_LDS_BYTES = 16384;            // OPTRAN coefficients staged when they fit
+
+// OPTRAN ln(chi) of one lane at one layer (Add_OPTRAN_wloOD,
+// ODPS_AtmAbsorption.f90:664-727): b_p = sum_q coef(p,q) Ap(q), lnchi = sum_p b_p OX(p)
+template <class RT, class OC>
+__device__ inline RT optran_lnchi(const OC& oc, const int* meta, const RT* ok) {
   const int nrow = meta[BM_ONROW], nq = meta[BM_OMAXO] + 1;
   RT lnchi = RT(0);
   for (int r = 0; r < nrow; ++r) {
     const int p = meta[BM_OROW0 + r];
-    const double* cr = ocb + (size_t)r * nq * WAVE;
     RT bp = RT(0);
-    for (int q = 0; q < nq; ++q) bp += (RT)cr[q * WAVE] * ok[O_AP0 + q];
+    for (int q = 0; q < nq; ++q) bp += oc(r * nq + q) * ok[O_AP0 + q];
     lnchi += bp * ok[O_X0 + p];
   }
   return lnchi;
 }
 
-constexpr int BPW = 4;                 // channel blocks (waves) per workgroup
-constexpr int TPB = BPW * WAVE;
-// LDS row: predictors X or adjoint table U, followed by the OPTRAN table O
-constexpr int TABW = (NX_MAX > NU ? NX_MAX : NU) + NO;
-static_assert(TABW <= TPB, "one table element per thread");
-
 template <int G, class RT, class KT, bool DOK>
-__global__ void __attribute__((amdgpu_flat_work_group_size(TPB, TPB), amdgpu_waves_per_eu(4)))
+__global__ void __attribute__((amdgpu_flat_work_group_size(KCfg<RT>::PW * WAVE, KCfg<RT>::PW * WAVE),
+                               amdgpu_waves_per_eu(KCfg<RT>::WPE > 0 ? KCfg<RT>::WPE : 1)))
 clear_fwdk_kernel(DevSensor s, DevBatch bt, ProfWork<RT> w, DevOut o) {
   using Tg = GT<G>;
-  // Workgroup = BPW consecutive channel blocks of one profile: the profile's
-  // per-layer uniform rows are staged once into LDS and shared by its waves.
+  constexpr int PW = KCfg<RT>::PW, NBUF = KCfg<RT>::NBUF, TPB = PW * WAVE;
+  // Workgroup = one 64-channel block x PW profiles (one wave each).  The
+  // block's coefficients of the current layer are staged once in LDS and
+  // read by all PW waves, so each coefficient leaves L2 once per PW
+  // profiles; each wave stages its own profile's uniform row.
   // XCD-aware order: workgroups are dispatched round-robin over the 8 XCDs,
   // so XCD x receives ids x, x+8, ...; it gets the contiguous range
-  // [x*chunk, (x+1)*chunk) of the (block group, profile) order, block-group
-  // major, so waves resident on one XCD share coefficient blocks in its L2.
-  const int nb = s.n_blocks;
-  const int nbg = (nb + BPW - 1) / BPW;
-  const int64_t total = (int64_t)nbg * bt.M;
+  // [x*chunk, (x+1)*chunk) of the block-major (block, profile group) order,
+  // so the workgroups resident on one XCD share coefficient blocks in its L2.
+  const int ng = (bt.M + PW - 1) / PW;
+  const int64_t total = (int64_t)s.n_blocks * ng;
   const int64_t chunk = (total + 7) / 8;
   const int64_t tix = (int64_t)(blockIdx.x & 7) * chunk + (blockIdx.x >> 3);
-  if (tix >= total) return;
+  if (tix >= total) return;                 // uniform over the workgroup
+  const int b = (int)(tix / ng);
   const int tid = threadIdx.x;
-  const int m = (int)(tix % bt.M);
-  const int bq = (int)(tix / bt.M) * BPW + (tid >> 6);
-  const bool bvalid = bq < nb;
-  const int b = bvalid ? bq : nb - 1;
+  const int wv = tid >> 6;
   const int lane = tid & (WAVE - 1);
+  const int m_raw = (int)(tix % ng) * PW + wv;
+  const bool in_batch = m_raw < bt.M;
+  const int m = in_batch ? m_raw : bt.M - 1;
+  const bool ok_prof = in_batch && w.status[m] == 0;   // wave-uniform
   const int l = b * WAVE + lane;
-  const bool valid = bvalid && l < s.L;
-  const int lc = l < s.L ? l : s.L - 1;
+  const bool lane_in = l < s.L;
+  const bool valid = ok_prof && lane_in;
+  const int lc = lane_in ? l : s.L - 1;
   const int lg = s.chan_offset + lc;
   const int64_t ol = (int64_t)m * bt.Ltot + lg;
-  if (w.status[m] != 0) {
-    if (valid) {
-      o.radiance[ol] = __builtin_nan("");
-      o.bt[ol] = __builtin_nan("");
-    }
-    return;
+  if (in_batch && !ok_prof && lane_in) {
+    o.radiance[ol] = __builtin_nan("");
+    o.bt[ol] = __builtin_nan("");
   }
-  const int K = bt.n_layers[m];
+  const int K = ok_prof ? bt.n_layers[m] : 0;
   const int64_t loff = bt.loff[m];
   const int Ko = s.Ko;
   const int* meta = s.blk_meta + b * BM_N;
@@ -301,37 +312,87 @@ clear_fwdk_kernel(DevSensor s, DevBatch bt, ProfWork<RT> w, DevOut o) {
     npb[j] = meta[BM_NP0 + j];
     coff[j] = meta[BM_OFF0 + j];
   }
-  const float* cb = s.cblk + s.blk_base[b] + lane;
   const int64_t ob = s.optran ? s.oblk_base[b] : -1;
   const bool blk_opt = ob >= 0;
   const bool lane_opt = blk_opt && s.lane_optran[b * WAVE + lane] != 0;
-  const double* ocb = blk_opt ? s.ocd + ob + lane : nullptr;
   const RT* X = w.x + (size_t)m * Ko * Tg::NX;
   const RT* Ut = w.u + (size_t)m * Ko * NU;
   const RT* Ot = w.o + (size_t)m * Ko * NO;
   const int optw = s.optran ? NO : 0;
-  __shared__ RT tab[2][TABW];
-  // element t of layer k's staged row: [X or U row | O row]
-  auto row = [&](const RT* base, int nbase, int k, int t) -> RT {
-    if (t < nbase) return base[(size_t)k * nbase + t];
-    if (t < nbase + optw) return Ot[(size_t)k * NO + (t - nbase)];
-    return RT(0);
+
+  __shared__ float cs[NBUF][Tg::NX * WAVE];       // coefficients [slot][lane] of the staged layer(s)
+  __shared__ RT tab[PW][NBUF][TABW];              // per-wave uniform rows
+  constexpr int OCAP = OC_LDS_BYTES / (WAVE * (int)sizeof(RT));
+  __shared__ RT ocs[OCAP * WAVE];                 // OPTRAN coefficients [row*order][lane]
+
+  // ---- coefficient staging: layer k's [nsl][64] floats, float4 per thread ----
+  constexpr int CQ = (Tg::NX * WAVE / 4 + TPB - 1) / TPB;
+  const float4* cg = reinterpret_cast<const float4*>(s.cblk + s.blk_base[b]);
+  const int n4 = nsl * (WAVE / 4);
+  float4 creg[CQ];
+  auto cload = [&](int k) {
+#pragma unroll
+    for (int q = 0; q < CQ; ++q) {
+      const int e = tid + q * TPB;
+      if (e < n4) creg[q] = cg[(int64_t)k * n4 + e];
+    }
   };
+  auto cstore = [&](int buf) {
+#pragma unroll
+    for (int q = 0; q < CQ; ++q) {
+      const int e = tid + q * TPB;
+      if (e < n4) reinterpret_cast<float4*>(cs[buf])[e] = creg[q];
+    }
+  };
+  // ---- per-wave row staging: element t of layer k is [X or U row | O row] ----
+  RT rreg[RQ];
+  auto rload = [&](const RT* base, int nbase, int k) {
+#pragma unroll
+    for (int q = 0; q < RQ; ++q) {
+      const int t = lane + q * WAVE;
+      RT v = RT(0);
+      if (ok_prof) {
+        if (t < nbase) v = base[(size_t)k * nbase + t];
+        else if (t < nbase + optw) v = Ot[(size_t)k * NO + (t - nbase)];
+      }
+      rreg[q] = v;
+    }
+  };
+  auto rstore = [&](int buf) {
+#pragma unroll
+    for (int q = 0; q < RQ; ++q) {
+      const int t = lane + q * WAVE;
+      if (t < TABW) tab[wv][buf][t] = rreg[q];
+    }
+  };
+  // ---- OPTRAN coefficients: LDS copy when the block's table fits ----
+  const int onq = blk_opt ? meta[BM_ONROW] * (meta[BM_OMAXO] + 1) : 0;
+  const bool oc_lds = onq <= OCAP;
+  const double* ocg = blk_opt ? s.ocd + ob : nullptr;
+  if (blk_opt && oc_lds)
+    for (int e = tid; e < onq * WAVE; e += TPB) ocs[e] = (RT)ocg[e];
+  auto ocoef = [&](int i) -> RT { return oc_lds ? ocs[i * WAVE + lane] : (RT)ocg[i * WAVE + lane]; };
 
   // ------------- forward (1): ODPS layer OD and level-to-space path -------------
   RT path[KO_MAX + 1];
   uint64_t odclamp[2] = {0, 0}, lcclamp[2] = {0, 0};
   path[0] = RT(0);
-  if (tid < TABW) tab[0][tid] = row(X, Tg::NX, 0, tid);
+  cload(0);
+  rload(X, Tg::NX, 0);
+  cstore(0);
+  rstore(0);
   __syncthreads();
   for (int k = 0; k < Ko; ++k) {
-    const RT nxt = (k + 1 < Ko && tid < TABW) ? row(X, Tg::NX, k + 1, tid) : RT(0);
-    const RT* xk = tab[k & 1];
-    const float* ck = cb + (int64_t)k * nsl * WAVE;
+    if (k + 1 < Ko) {
+      cload(k + 1);
+      rload(X, Tg::NX, k + 1);
+    }
+    const RT* xk = tab[wv][k % NBUF];
+    const float* ck = cs[k % NBUF] + lane;
     RT od = comp_layer_od<G, RT, 0>(ck, npb, coff, xk);
     if (blk_opt) {
-      const RT* ok = xk + Tg::NX;
-      const RT lnchi = optran_lnchi<RT>(ocb, meta, ok);
+      const RT* okr = xk + Tg::NX;
+      const RT lnchi = optran_lnchi<RT>(ocoef, meta, okr);
       if (lane_opt) {
         RT c;
         if (lnchi > RT(LIMIT_EXP)) {
@@ -343,7 +404,7 @@ clear_fwdk_kernel(DevSensor s, DevBatch bt, ProfWork<RT> w, DevOut o) {
         } else {
           c = rexp<RT>(lnchi);
         }
-        od += c * ok[O_DA];
+        od += c * okr[O_DA];
       }
     }
     RT odc = od;
@@ -355,16 +416,18 @@ clear_fwdk_kernel(DevSensor s, DevBatch bt, ProfWork<RT> w, DevOut o) {
       odclamp[k >> 6] |= 1ull << (k & 63);
     }
     path[k + 1] = path[k] + odc;
-    if (k + 1 < Ko && tid < TABW) tab[(k + 1) & 1][tid] = nxt;
+    if constexpr (NBUF == 1) __syncthreads();   // every wave is done with the buffer
+    if (k + 1 < Ko) {
+      cstore((k + 1) % NBUF);
+      rstore((k + 1) % NBUF);
+    }
     __syncthreads();
   }
 
   // ------------- forward (2): user layers, Planck, Emission (top-down) -------------
   const double* geo = w.geo + (size_t)m * 4;
   const double sec_sfc = geo[0], cosu = geo[1], tsfc = geo[2];
-  const RT rsec = (RT)sec_sfc;
   const RT isec = (RT)(1.0 / sec_sfc);
-  const RT u = (RT)cosu;
   const RT iu = (RT)(1.0 / cosu);
   const int* ui = w.uidx + (size_t)m * (bt.max_k + 1);
   const RT* uf = w.ufrac + (size_t)m * (bt.max_k + 1);
@@ -378,7 +441,7 @@ clear_fwdk_kernel(DevSensor s, DevBatch bt, ProfWork<RT> w, DevOut o) {
   const double* tuser = bt.t + loff;
   RT rd = (RT)s.cosmic[lc];
   RT uacc = RT(0), tc = RT(1);
-  const RT up0 = upath(0);
+  const RT up0 = ok_prof ? upath(0) : RT(0);
   RT upp = up0;
   for (int k = 0; k < K; ++k) {
     const RT up = upath(k + 1);
@@ -414,7 +477,7 @@ clear_fwdk_kernel(DevSensor s, DevBatch bt, ProfWork<RT> w, DevOut o) {
   const double arg = c1d / R + 1.0;
   const double lga = log(arg);
   const double F = c1d * c2d / (bc2d * arg * (R * lga) * (R * lga));
-  const double rad_ad_d = (o.rad_k ? o.rad_k[ol] : 0.0) + F * (o.bt_k ? o.bt_k[ol] : 0.0);
+  const double rad_ad_d = valid ? (o.rad_k ? o.rad_k[ol] : 0.0) + F * (o.bt_k ? o.bt_k[ol] : 0.0) : 0.0;
   if (valid && o.rad_k) o.rad_k[ol] = rad_ad_d;
   const RT rad_ad = (RT)rad_ad_d;
   const RT upad_sfc = rad_ad * tc;           // adjoint of the surface-leaving radiance
@@ -433,20 +496,32 @@ clear_fwdk_kernel(DevSensor s, DevBatch bt, ProfWork<RT> w, DevOut o) {
   RT ru = sfc_src;
   RT dn = rho * upad_sfc;
   RT gd = rd * dn;
-  RT path_ad[KO_MAX + 1];
-  for (int k = 0; k <= Ko; ++k) path_ad[k] = RT(0);
+  // Path adjoint at ODPS levels kc+1 and kc while layer kc is processed: a
+  // user level is stepped at the layer kc = its bracket ui (the last layer
+  // whose path it reads), so it only adds to levels kc and kc+1.
+  RT pa_hi = RT(0), pa_lo = RT(0);
   const int64_t Lt = bt.Ltot;
   const int na = bt.na;
   KT* tko = reinterpret_cast<KT*>(o.t_k) + loff * Lt + lg;
   KT* ako = reinterpret_cast<KT*>(o.a_k) + loff * Lt + lg;
   const int64_t astr = bt.NT * Lt;
+  // Jacobian pieces kept in private memory until the sweep ends, so each
+  // output element is written exactly once: Planck part per user layer, and
+  // the ODPS-layer adjoints of T and of the user absorbers that Map_Input_AD
+  // spreads over the user layers.
+  RT plk[K_MAX];
+  RT tadk[KO_MAX];
+  RT aadk[NA_MAX][KO_MAX];
   RT carry = RT(0);
   int i = K;
-  auto ustep = [&](int iu_) {
+  RT up_next = ok_prof ? upath(K) : RT(0);   // path at user level iu_ of the next step
+  auto ustep = [&](int iu_, int kc) {
     RT v;
     if (iu_ >= 1) {
       const int k = iu_ - 1;   // user layer above level iu_
-      const RT upk = upath(k), upk1 = upath(k + 1);
+      const RT upk1 = up_next;
+      const RT upk = upath(k);
+      up_next = upk;
       const RT t = rexp<RT>(-((upk1 - upk) * isec) * iu);
       const RT tabove = rexp<RT>(-((upk - up0) * isec) * iu);
       RT te;
@@ -460,10 +535,7 @@ clear_fwdk_kernel(DevSensor s, DevBatch bt, ProfWork<RT> w, DevOut o) {
       dn *= t;
       ru = ru * t + B * (RT(1) - t);
       const RT dbdt = c2 * bc2 * B * (B + c1) * ic1 / (te * te);
-      if (valid) {
-        tko[(int64_t)k * Lt] = (KT)(dbdt * b_ad);
-        for (int ja = 0; ja < na; ++ja) ako[ja * astr + (int64_t)k * Lt] = KT(0);
-      }
+      plk[k] = dbdt * b_ad;
       const RT gq = tau_ad * isec;
       v = carry + gq;
       carry = -gq;
@@ -472,11 +544,13 @@ clear_fwdk_kernel(DevSensor s, DevBatch bt, ProfWork<RT> w, DevOut o) {
     }
     const int k1 = ui[iu_];
     const RT f = uf[iu_];
-    if (f < RT(0)) {
-      path_ad[k1] += v;
-    } else {
-      path_ad[k1] += (RT(1) - f) * v;
-      path_ad[k1 + 1] += f * v;
+    const RT v0 = f < RT(0) ? v : (RT(1) - f) * v;   // to level k1
+    const RT v1 = f < RT(0) ? RT(0) : f * v;          // to level k1+1
+    if (k1 == kc) {
+      pa_lo += v0;
+      pa_hi += v1;
+    } else {                                          // k1 == kc+1: the bottom level, f < 0
+      pa_hi += v0;
     }
   };
 
@@ -488,114 +562,153 @@ clear_fwdk_kernel(DevSensor s, DevBatch bt, ProfWork<RT> w, DevOut o) {
   const int h2o_j = s.h2o_idx - 1;
   const int* acc_i = w.acc_i + (size_t)m * Ko * 3;
   const RT* acc_w = w.acc_w + (size_t)m * NZ_MAX;
-  if (tid < TABW) tab[(Ko - 1) & 1][tid] = row(Ut, NU, Ko - 1, tid);
+  // the forward loop ended on a barrier, so both LDS buffers are free
+  cload(Ko - 1);
+  rload(Ut, NU, Ko - 1);
+  cstore((Ko - 1) % NBUF);
+  rstore((Ko - 1) % NBUF);
   __syncthreads();
   for (int k = Ko - 1; k >= 0; --k) {
-    const RT nxt = (k > 0 && tid < TABW) ? row(Ut, NU, k - 1, tid) : RT(0);
-    const int i1 = acc_i[3 * k], n = acc_i[3 * k + 1];
-    // user levels whose path adjoint feeds ODPS level k+1, and every user
-    // layer this ODPS layer's averaging kernel touches
-    while (i >= 0 && (ui[i] >= k || i > i1)) {
-      ustep(i);
-      --i;
+    if (k > 0) {
+      cload(k - 1);
+      rload(Ut, NU, k - 1);
     }
-    S += path_ad[k + 1];
-    const RT od_ad = ((odclamp[k >> 6] >> (k & 63)) & 1) ? RT(0) : S;
-    const RT* uk = tab[k & 1];
-    LayerAdj<RT> a{};
-    const float* ck = cb + (int64_t)k * nsl * WAVE;
-    comp_layer_ad<G, RT, 0>(ck, npb, coff, od_ad, uk, a);
-    RT t_ad = RT(0), vap_ad = RT(0);
-    if (blk_opt) {
-      // Add_OPTRAN_wloOD_AD (ODPS_AtmAbsorption.f90:911-984) with chi recomputed
-      const RT* ok = uk + NU;
-      const RT lnchi = optran_lnchi<RT>(ocb, meta, ok);
-      const bool lcc = (lcclamp[k >> 6] >> (k & 63)) & 1;
-      const RT chi = lane_opt ? (lcc ? (lnchi > RT(0) ? RT(LIMIT_LOG) : RT(0)) : rexp<RT>(lnchi)) : RT(0);
-      const RT da_ad = lane_opt ? od_ad * chi : RT(0);
-      const RT lc_ad = (lane_opt && !lcc) ? chi * od_ad * ok[O_DA] : RT(0);
-      const int nrow = meta[BM_ONROW], nq = meta[BM_OMAXO] + 1;
-      RT ap1_ad = RT(0), e12 = RT(0), e13 = RT(0);
-      for (int r = 0; r < nrow; ++r) {
-        const int p = meta[BM_OROW0 + r];
-        const double* cr = ocb + (size_t)r * nq * WAVE;
-        RT bp = RT(0), dbp = RT(0);
-        for (int q = 0; q < nq; ++q) {
-          const RT cq = (RT)cr[q * WAVE];
-          bp += cq * ok[O_AP0 + q];
-          if (q > 0) dbp += RT(q) * cq * ok[O_AP0 + q - 1];
+    if (ok_prof) {
+      // user levels whose path interpolation reads ODPS level k or k+1
+      while (i >= 0 && ui[i] >= k) {
+        ustep(i, k);
+        --i;
+      }
+      S += pa_hi;
+      pa_hi = pa_lo;
+      pa_lo = RT(0);
+      const RT od_ad = ((odclamp[k >> 6] >> (k & 63)) & 1) ? RT(0) : S;
+      const RT* uk = tab[wv][k % NBUF];
+      LayerAdj<RT> a{};
+      const float* ck = cs[k % NBUF] + lane;
+      comp_layer_ad<G, RT, 0>(ck, npb, coff, od_ad, uk, a);
+      RT t_ad = RT(0), vap_ad = RT(0);
+      if (blk_opt) {
+        // Add_OPTRAN_wloOD_AD (ODPS_AtmAbsorption.f90:911-984) with chi recomputed
+        const RT* okr = uk + NU;
+        const RT lnchi = optran_lnchi<RT>(ocoef, meta, okr);
+        const bool lcc = (lcclamp[k >> 6] >> (k & 63)) & 1;
+        const RT chi = lane_opt ? (lcc ? (lnchi > RT(0) ? RT(LIMIT_LOG) : RT(0)) : rexp<RT>(lnchi)) : RT(0);
+        const RT da_ad = lane_opt ? od_ad * chi : RT(0);
+        const RT lc_ad = (lane_opt && !lcc) ? chi * od_ad * okr[O_DA] : RT(0);
+        const int nrow = meta[BM_ONROW], nq = meta[BM_OMAXO] + 1;
+        RT ap1_ad = RT(0), e12 = RT(0), e13 = RT(0);
+        for (int r = 0; r < nrow; ++r) {
+          const int p = meta[BM_OROW0 + r];
+          RT bp = RT(0), dbp = RT(0);
+          for (int q = 0; q < nq; ++q) {
+            const RT cq = ocoef(r * nq + q);
+            bp += cq * okr[O_AP0 + q];
+            if (q > 0) dbp += RT(q) * cq * okr[O_AP0 + q - 1];
+          }
+          const RT xa = lc_ad * bp;          // adjoint of OX(p)
+          ap1_ad += lc_ad * okr[O_X0 + p] * dbp;
+          t_ad += xa * okr[O_DX_DT0 + p];
+          vap_ad += xa * okr[O_DX_DV0 + p];
+          if (p == 12) e12 += xa;
+          if (p == 13) e13 += xa;
         }
-        const RT xa = lc_ad * bp;          // adjoint of OX(p)
-        ap1_ad += lc_ad * ok[O_X0 + p] * dbp;
-        t_ad += xa * ok[O_DX_DT0 + p];
-        vap_ad += xa * ok[O_DX_DV0 + p];
-        if (p == 12) e12 += xa;
-        if (p == 13) e13 += xa;
+        // ODPS_Compute_Predictor_ODAS_AD at layer k (ODPS_Predictor.f90:3092-3231)
+        const RT ave_ad = ap1_ad * okr[O_IDEN];
+        const RT xlt_ad = xltc + e12, xlp_ad = xlpc + e13;
+        xltc = e12;
+        xlpc = e13;
+        sta += RT(0.5) * okr[O_INV] * xlt_ad;
+        spa += RT(0.5) * okr[O_INV] * xlp_ad;
+        const RT inv_ad = RT(0.5) * (okr[O_ST] * xlt_ad + okr[O_SP] * xlp_ad);
+        const RT iv_ad = ivn - inv_ad * okr[O_IIV2] + RT(0.5) * ave_ad;
+        ivn = RT(0.5) * ave_ad + iv_ad;
+        const RT dab_ad = da_ad + okr[O_PR] * spa + okr[O_T] * sta + iv_ad;
+        t_ad += okr[O_DA] * sta;
+        vap_ad += okr[O_DPSEC] * dab_ad;
       }
-      // ODPS_Compute_Predictor_ODAS_AD at layer k (ODPS_Predictor.f90:3092-3231)
-      const RT ave_ad = ap1_ad * ok[O_IDEN];
-      const RT xlt_ad = xltc + e12, xlp_ad = xlpc + e13;
-      xltc = e12;
-      xlpc = e13;
-      sta += RT(0.5) * ok[O_INV] * xlt_ad;
-      spa += RT(0.5) * ok[O_INV] * xlp_ad;
-      const RT inv_ad = RT(0.5) * (ok[O_ST] * xlt_ad + ok[O_SP] * xlp_ad);
-      const RT iv_ad = ivn - inv_ad * ok[O_IIV2] + RT(0.5) * ave_ad;
-      ivn = RT(0.5) * ave_ad + iv_ad;
-      const RT dab_ad = da_ad + ok[O_PR] * spa + ok[O_T] * sta + iv_ad;
-      t_ad += ok[O_DA] * sta;
-      vap_ad += ok[O_DPSEC] * dab_ad;
-    }
-    finish_layer<G, RT>(uk, a);
-    t_ad += a.T * uk[U_IRT] + a.dt;
-    RT a_ad[Tg::NABS];
+      finish_layer<G, RT>(uk, a);
+      t_ad += a.T * uk[U_IRT] + a.dt;
+      RT a_ad[Tg::NABS];
 #pragma unroll
-    for (int j = 0; j < Tg::NABS; ++j) a_ad[j] = a.ra[j] * uk[U_IRA0 + j];
-    if (s.optran) a_ad[h2o_j] += vap_ad;
-    const RT pdp = uk[U_PDP];
+      for (int j = 0; j < Tg::NABS; ++j) a_ad[j] = a.ra[j] * uk[U_IRA0 + j];
+      if (s.optran) a_ad[h2o_j] += vap_ad;
+      const RT pdp = uk[U_PDP];
 #pragma unroll
-    for (int j = 0; j < Tg::NABS; ++j) {
-      gts[j] += a.gt[j] * uk[U_IGATZP0 + j];
-      gzs[j] += a.gz[j] * uk[U_IGAZP0 + j];
-      t_ad += gts[j] * pdp * uk[U_AODPS0 + j];
-      a_ad[j] += gzs[j] * pdp + gts[j] * pdp * uk[U_TODPS];
-    }
-    tzps += a.tzp * uk[U_ITZPREF];
-    tzs += a.tz * uk[U_ITZREF];
-    t_ad += tzs + pdp * tzps;
-    // Map_Input_AD (ODPS_CoordinateMapping.f90:480-547): add onto the user
-    // layers averaged into this ODPS layer (their Planck part is already stored)
-    if (valid) {
-      const RT* aw = acc_w + acc_i[3 * k + 2];
-      for (int q = 0; q < n; ++q) {
-        KT* d = tko + (int64_t)(i1 + q) * Lt;
-        *d = (KT)((RT)*d + aw[q] * t_ad);
+      for (int j = 0; j < Tg::NABS; ++j) {
+        gts[j] += a.gt[j] * uk[U_IGATZP0 + j];
+        gzs[j] += a.gz[j] * uk[U_IGAZP0 + j];
+        t_ad += gts[j] * pdp * uk[U_AODPS0 + j];
+        a_ad[j] += gzs[j] * pdp + gts[j] * pdp * uk[U_TODPS];
       }
+      tzps += a.tzp * uk[U_ITZPREF];
+      tzs += a.tz * uk[U_ITZREF];
+      t_ad += tzs + pdp * tzps;
+      // Map_Input_AD (ODPS_CoordinateMapping.f90:480-547): add onto the user
+      // layers averaged into this ODPS layer (their Planck part is already stored)
+      tadk[k] = t_ad;
 #pragma unroll
       for (int j = 0; j < Tg::NABS; ++j) {
         const int ju = bt.jmap[j];
-        if (ju >= 0 && uk[U_MASK0 + j] != RT(0))
-          for (int q = 0; q < n; ++q) {
-            KT* d = ako + ju * astr + (int64_t)(i1 + q) * Lt;
-            *d = (KT)((RT)*d + aw[q] * a_ad[j]);
-          }
+        if (ju >= 0) aadk[ju][k] = uk[U_MASK0 + j] != RT(0) ? a_ad[j] : RT(0);
       }
     }
-    if (k > 0 && tid < TABW) tab[(k - 1) & 1][tid] = nxt;
+    if constexpr (NBUF == 1) __syncthreads();
+    if (k > 0) {
+      cstore((k - 1) % NBUF);
+      rstore((k - 1) % NBUF);
+    }
     __syncthreads();
   }
-  while (i >= 0) {   // levels above the first ODPS level (none on a common top pressure)
-    ustep(i);
+  while (i >= 0) {   // no level remains: every bracket ui is >= 0
+    ustep(i, -1);
     --i;
+  }
+  if (!valid) return;
+
+  // ------------- Map_Input_AD (ODPS_CoordinateMapping.f90:480-547) as a gather -------------
+  // User layer i collects w(k,i) * adjoint(k) over the ODPS layers k whose
+  // averaging support [i1(k), i1(k)+n(k)) contains it; the supports move
+  // monotonically with k, so [kmin, kmax] slides down the profile.
+  bool amap[NA_MAX];
+#pragma unroll
+  for (int ju = 0; ju < NA_MAX; ++ju) {
+    amap[ju] = false;
+#pragma unroll
+    for (int j = 0; j < Tg::NABS; ++j) amap[ju] = amap[ju] || bt.jmap[j] == ju;
+  }
+  int kmin = 0, kmax = -1;
+  for (int iu = 0; iu < K; ++iu) {
+    while (kmax + 1 < Ko && acc_i[3 * (kmax + 1)] <= iu) ++kmax;
+    while (kmin < Ko && acc_i[3 * kmin] + acc_i[3 * kmin + 1] - 1 < iu) ++kmin;
+    RT st = plk[iu];
+    RT sa[NA_MAX];
+#pragma unroll
+    for (int ju = 0; ju < NA_MAX; ++ju) sa[ju] = RT(0);
+    for (int k = kmin; k <= kmax; ++k) {
+      const int i1 = acc_i[3 * k], n = acc_i[3 * k + 1];
+      if (iu < i1 || iu >= i1 + n) continue;
+      const RT wk = acc_w[acc_i[3 * k + 2] + iu - i1];
+      st += wk * tadk[k];
+#pragma unroll
+      for (int ju = 0; ju < NA_MAX; ++ju)
+        if (amap[ju]) sa[ju] += wk * aadk[ju][k];
+    }
+    tko[(int64_t)iu * Lt] = (KT)st;
+#pragma unroll
+    for (int ju = 0; ju < NA_MAX; ++ju)
+      if (ju < na) ako[ju * astr + (int64_t)iu * Lt] = (KT)sa[ju];
   }
 }
 
 template <int G, class RT, class KT, bool DOK>
 void launch_fwdk(const DevSensor& s, const DevBatch& b, const ProfWork<RT>& w, const DevOut& o,
                  hipStream_t stream) {
-  const int64_t nbg = (s.n_blocks + BPW - 1) / BPW;
-  const int64_t n = 8 * ((nbg * b.M + 7) / 8);
-  hipLaunchKernelGGL((clear_fwdk_kernel<G, RT, KT, DOK>), dim3((unsigned)n), dim3(TPB), 0, stream, s, b, w, o);
+  constexpr int PW = KCfg<RT>::PW;
+  const int64_t ng = (b.M + PW - 1) / PW;
+  const int64_t n = 8 * ((s.n_blocks * ng + 7) / 8);
+  hipLaunchKernelGGL((clear_fwdk_kernel<G, RT, KT, DOK>), dim3((unsigned)n), dim3(PW * WAVE), 0, stream, s, b, w,
+                     o);
 }
 
 #define INST(G, RT, KT, DOK) \
