@@ -192,43 +192,36 @@ __device__ inline void finish_layer(const RT* u, LayerAdj<RT>& a) {
 }
 
 template <int G, class RT, int J>
-__device__ inline void comp_layer_ad(const float* ck, const int* npb, const int* coff, RT od_ad, const RT* uk,
+__device__ inline void comp_layer_ad(const float* ck, const int* npb, RT od_ad, const RT* uk,
                                      LayerAdj<RT>& a) {
   using Tg = GT<G>;
   if constexpr (J < Tg::NCOMP) {
     const int np = npb[J];
     if (np > 0) {
       RT xa[Tg::NP[J]];
-      const float* cj = ck + coff[J] * WAVE;
-      // all NP[J] slots are read (the coefficient array is padded) and the
-      // ones beyond this block's predictor count masked, so the loads batch
+      const float* cj = ck + Tg::XOFF[J] * WAVE;
+      // slots beyond a channel's predictor count hold zeros
 #pragma unroll
-      for (int i = 0; i < Tg::NP[J]; ++i) {
-        const RT c = (RT)cj[i * WAVE];
-        xa[i] = (i < np) ? c * od_ad : RT(0);
-      }
+      for (int i = 0; i < Tg::NP[J]; ++i) xa[i] = (RT)cj[i * WAVE] * od_ad;
       comp_adjoint<G, J, RT>(xa, uk, a);
     }
-    comp_layer_ad<G, RT, J + 1>(ck, npb, coff, od_ad, uk, a);
+    comp_layer_ad<G, RT, J + 1>(ck, npb, od_ad, uk, a);
   }
 }
 
 template <int G, class RT, int J>
-__device__ inline RT comp_layer_od(const float* ck, const int* npb, const int* coff, const RT* xk) {
+__device__ inline RT comp_layer_od(const float* ck, const int* npb, const RT* xk) {
   using Tg = GT<G>;
   if constexpr (J < Tg::NCOMP) {
     RT od = RT(0);
     const int np = npb[J];
     if (np > 0) {
-      const float* cj = ck + coff[J] * WAVE;
+      const float* cj = ck + Tg::XOFF[J] * WAVE;
       const RT* xj = xk + Tg::XOFF[J];
 #pragma unroll
-      for (int i = 0; i < Tg::NP[J]; ++i) {
-        const RT c = (RT)cj[i * WAVE];
-        od += (i < np ? c : RT(0)) * xj[i];
-      }
+      for (int i = 0; i < Tg::NP[J]; ++i) od += (RT)cj[i * WAVE] * xj[i];
     }
-    return od + comp_layer_od<G, RT, J + 1>(ck, npb, coff, xk);
+    return od + comp_layer_od<G, RT, J + 1>(ck, npb, xk);
   } else {
     return RT(0);
   }
@@ -305,13 +298,9 @@ clear_fwdk_kernel(DevSensor s, DevBatch bt, ProfWork<RT> w, DevOut o) {
   const int64_t loff = bt.loff[m];
   const int Ko = s.Ko;
   const int* meta = s.blk_meta + b * BM_N;
-  const int nsl = meta[BM_NSL];
-  int npb[Tg::NCOMP], coff[Tg::NCOMP];
+  int npb[Tg::NCOMP];
 #pragma unroll
-  for (int j = 0; j < Tg::NCOMP; ++j) {
-    npb[j] = meta[BM_NP0 + j];
-    coff[j] = meta[BM_OFF0 + j];
-  }
+  for (int j = 0; j < Tg::NCOMP; ++j) npb[j] = meta[BM_NP0 + j];
   const int64_t ob = s.optran ? s.oblk_base[b] : -1;
   const bool blk_opt = ob >= 0;
   const bool lane_opt = blk_opt && s.lane_optran[b * WAVE + lane] != 0;
@@ -325,10 +314,10 @@ clear_fwdk_kernel(DevSensor s, DevBatch bt, ProfWork<RT> w, DevOut o) {
   constexpr int OCAP = OC_LDS_BYTES / (WAVE * (int)sizeof(RT));
   __shared__ RT ocs[OCAP * WAVE];                 // OPTRAN coefficients [row*order][lane]
 
-  // ---- coefficient staging: layer k's [nsl][64] floats, float4 per thread ----
+  // ---- coefficient staging: layer k's [NX][64] floats, float4 per thread ----
   constexpr int CQ = (Tg::NX * WAVE / 4 + TPB - 1) / TPB;
   const float4* cg = reinterpret_cast<const float4*>(s.cblk + s.blk_base[b]);
-  const int n4 = nsl * (WAVE / 4);
+  constexpr int n4 = Tg::NX * (WAVE / 4);
   float4 creg[CQ];
   auto cload = [&](int k) {
 #pragma unroll
@@ -371,7 +360,9 @@ clear_fwdk_kernel(DevSensor s, DevBatch bt, ProfWork<RT> w, DevOut o) {
   const double* ocg = blk_opt ? s.ocd + ob : nullptr;
   if (blk_opt && oc_lds)
     for (int e = tid; e < onq * WAVE; e += TPB) ocs[e] = (RT)ocg[e];
-  auto ocoef = [&](int i) -> RT { return oc_lds ? ocs[i * WAVE + lane] : (RT)ocg[i * WAVE + lane]; };
+  // two accessors, chosen once per block, so the inner loops carry no select
+  auto oc_l = [&](int i) -> RT { return ocs[i * WAVE + lane]; };
+  auto oc_g = [&](int i) -> RT { return (RT)ocg[i * WAVE + lane]; };
 
   // ------------- forward (1): ODPS layer OD and level-to-space path -------------
   RT path[KO_MAX + 1];
@@ -389,10 +380,10 @@ clear_fwdk_kernel(DevSensor s, DevBatch bt, ProfWork<RT> w, DevOut o) {
     }
     const RT* xk = tab[wv][k % NBUF];
     const float* ck = cs[k % NBUF] + lane;
-    RT od = comp_layer_od<G, RT, 0>(ck, npb, coff, xk);
+    RT od = comp_layer_od<G, RT, 0>(ck, npb, xk);
     if (blk_opt) {
       const RT* okr = xk + Tg::NX;
-      const RT lnchi = optran_lnchi<RT>(ocoef, meta, okr);
+      const RT lnchi = oc_lds ? optran_lnchi<RT>(oc_l, meta, okr) : optran_lnchi<RT>(oc_g, meta, okr);
       if (lane_opt) {
         RT c;
         if (lnchi > RT(LIMIT_EXP)) {
@@ -496,10 +487,13 @@ clear_fwdk_kernel(DevSensor s, DevBatch bt, ProfWork<RT> w, DevOut o) {
   RT ru = sfc_src;
   RT dn = rho * upad_sfc;
   RT gd = rd * dn;
-  // Path adjoint at ODPS levels kc+1 and kc while layer kc is processed: a
-  // user level is stepped at the layer kc = its bracket ui (the last layer
-  // whose path it reads), so it only adds to levels kc and kc+1.
+  // Path adjoint per ODPS level, written once per level: user levels are
+  // stepped bottom-up with non-increasing brackets ui, so their additions
+  // go to a two-level window (lo = level cur, hi = level cur+1) that is
+  // flushed as it slides up.
+  RT pad[KO_MAX + 2];
   RT pa_hi = RT(0), pa_lo = RT(0);
+  int pcur = Ko;
   const int64_t Lt = bt.Ltot;
   const int na = bt.na;
   KT* tko = reinterpret_cast<KT*>(o.t_k) + loff * Lt + lg;
@@ -513,9 +507,8 @@ clear_fwdk_kernel(DevSensor s, DevBatch bt, ProfWork<RT> w, DevOut o) {
   RT tadk[KO_MAX];
   RT aadk[NA_MAX][KO_MAX];
   RT carry = RT(0);
-  int i = K;
   RT up_next = ok_prof ? upath(K) : RT(0);   // path at user level iu_ of the next step
-  auto ustep = [&](int iu_, int kc) {
+  auto ustep = [&](int iu_) {
     RT v;
     if (iu_ >= 1) {
       const int k = iu_ - 1;   // user layer above level iu_
@@ -544,15 +537,27 @@ clear_fwdk_kernel(DevSensor s, DevBatch bt, ProfWork<RT> w, DevOut o) {
     }
     const int k1 = ui[iu_];
     const RT f = uf[iu_];
-    const RT v0 = f < RT(0) ? v : (RT(1) - f) * v;   // to level k1
-    const RT v1 = f < RT(0) ? RT(0) : f * v;          // to level k1+1
-    if (k1 == kc) {
-      pa_lo += v0;
-      pa_hi += v1;
-    } else {                                          // k1 == kc+1: the bottom level, f < 0
-      pa_hi += v0;
+    while (pcur > k1) {
+      pad[pcur + 1] = pa_hi;
+      pa_hi = pa_lo;
+      pa_lo = RT(0);
+      --pcur;
     }
+    pa_lo += f < RT(0) ? v : (RT(1) - f) * v;   // level k1
+    pa_hi += f < RT(0) ? RT(0) : f * v;          // level k1+1
   };
+  // CRTM_Emission_AD over all user levels first (no barrier inside, so the
+  // waves' serial chains overlap), then the ODPS sweep reads pad[].
+  if (ok_prof) {
+    for (int i = K; i >= 0; --i) ustep(i);
+    while (pcur >= 0) {
+      pad[pcur + 1] = pa_hi;
+      pa_hi = pa_lo;
+      pa_lo = RT(0);
+      --pcur;
+    }
+  }
+  RT pad_next = ok_prof ? pad[Ko] : RT(0);
 
   RT S = RT(0), tzs = RT(0), tzps = RT(0);
   RT gzs[Tg::NABS], gts[Tg::NABS];
@@ -574,45 +579,43 @@ clear_fwdk_kernel(DevSensor s, DevBatch bt, ProfWork<RT> w, DevOut o) {
       rload(Ut, NU, k - 1);
     }
     if (ok_prof) {
-      // user levels whose path interpolation reads ODPS level k or k+1
-      while (i >= 0 && ui[i] >= k) {
-        ustep(i, k);
-        --i;
-      }
-      S += pa_hi;
-      pa_hi = pa_lo;
-      pa_lo = RT(0);
+      S += pad_next;
+      pad_next = k > 0 ? pad[k] : RT(0);
       const RT od_ad = ((odclamp[k >> 6] >> (k & 63)) & 1) ? RT(0) : S;
       const RT* uk = tab[wv][k % NBUF];
       LayerAdj<RT> a{};
       const float* ck = cs[k % NBUF] + lane;
-      comp_layer_ad<G, RT, 0>(ck, npb, coff, od_ad, uk, a);
+      comp_layer_ad<G, RT, 0>(ck, npb, od_ad, uk, a);
       RT t_ad = RT(0), vap_ad = RT(0);
       if (blk_opt) {
         // Add_OPTRAN_wloOD_AD (ODPS_AtmAbsorption.f90:911-984) with chi recomputed
         const RT* okr = uk + NU;
-        const RT lnchi = optran_lnchi<RT>(ocoef, meta, okr);
+        const RT lnchi = oc_lds ? optran_lnchi<RT>(oc_l, meta, okr) : optran_lnchi<RT>(oc_g, meta, okr);
         const bool lcc = (lcclamp[k >> 6] >> (k & 63)) & 1;
         const RT chi = lane_opt ? (lcc ? (lnchi > RT(0) ? RT(LIMIT_LOG) : RT(0)) : rexp<RT>(lnchi)) : RT(0);
         const RT da_ad = lane_opt ? od_ad * chi : RT(0);
         const RT lc_ad = (lane_opt && !lcc) ? chi * od_ad * okr[O_DA] : RT(0);
         const int nrow = meta[BM_ONROW], nq = meta[BM_OMAXO] + 1;
         RT ap1_ad = RT(0), e12 = RT(0), e13 = RT(0);
-        for (int r = 0; r < nrow; ++r) {
-          const int p = meta[BM_OROW0 + r];
-          RT bp = RT(0), dbp = RT(0);
-          for (int q = 0; q < nq; ++q) {
-            const RT cq = ocoef(r * nq + q);
-            bp += cq * okr[O_AP0 + q];
-            if (q > 0) dbp += RT(q) * cq * okr[O_AP0 + q - 1];
+        auto rows_ad = [&](const auto& oc) {
+          for (int r = 0; r < nrow; ++r) {
+            const int p = meta[BM_OROW0 + r];
+            RT bp = RT(0), dbp = RT(0);
+            for (int q = 0; q < nq; ++q) {
+              const RT cq = oc(r * nq + q);
+              bp += cq * okr[O_AP0 + q];
+              if (q > 0) dbp += RT(q) * cq * okr[O_AP0 + q - 1];
+            }
+            const RT xa = lc_ad * bp;          // adjoint of OX(p)
+            ap1_ad += lc_ad * okr[O_X0 + p] * dbp;
+            t_ad += xa * okr[O_DX_DT0 + p];
+            vap_ad += xa * okr[O_DX_DV0 + p];
+            if (p == 12) e12 += xa;
+            if (p == 13) e13 += xa;
           }
-          const RT xa = lc_ad * bp;          // adjoint of OX(p)
-          ap1_ad += lc_ad * okr[O_X0 + p] * dbp;
-          t_ad += xa * okr[O_DX_DT0 + p];
-          vap_ad += xa * okr[O_DX_DV0 + p];
-          if (p == 12) e12 += xa;
-          if (p == 13) e13 += xa;
-        }
+        };
+        if (oc_lds) rows_ad(oc_l);
+        else rows_ad(oc_g);
         // ODPS_Compute_Predictor_ODAS_AD at layer k (ODPS_Predictor.f90:3092-3231)
         const RT ave_ad = ap1_ad * okr[O_IDEN];
         const RT xlt_ad = xltc + e12, xlp_ad = xlpc + e13;
@@ -659,10 +662,6 @@ clear_fwdk_kernel(DevSensor s, DevBatch bt, ProfWork<RT> w, DevOut o) {
       rstore((k - 1) % NBUF);
     }
     __syncthreads();
-  }
-  while (i >= 0) {   // no level remains: every bracket ui is >= 0
-    ustep(i, -1);
-    --i;
   }
   if (!valid) return;
 

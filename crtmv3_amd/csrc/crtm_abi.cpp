@@ -367,7 +367,12 @@ void build_sensor(Sensor& S, int chan_offset) {
       }
     }
   }
-  // coefficient block layout: [block][k][slot][lane]
+  // coefficient block layout: [block][k][slot][lane] with the group's fixed
+  // slot positions (component j at XOFF[j]); slots a channel does not use
+  // hold zeros, so the kernel multiplies them without masks
+  int xoff[NCOMP_MAX] = {0};
+  for (int j = 1; j < J; ++j) xoff[j] = xoff[j - 1] + NPMAX[t.group][j - 1];
+  const int nslots = xoff[J - 1] + NPMAX[t.group][J - 1];
   const int nb = (L + WAVE - 1) / WAVE;
   S.n_blocks = nb;
   d.n_blocks = nb;
@@ -395,9 +400,9 @@ void build_sensor(Sensor& S, int chan_offset) {
         mx = std::max(mx, np);
       }
       mt[BM_NP0 + j] = mx;
-      mt[BM_OFF0 + j] = nsl;
-      nsl += mx;
+      mt[BM_OFF0 + j] = xoff[j];
     }
+    nsl = nslots;
     mt[BM_NSL] = nsl;
     base[b] = total;
     total += (int64_t)Ko * nsl * WAVE;
