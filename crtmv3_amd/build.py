@@ -48,6 +48,31 @@ def _compile(src: str, force: bool) -> str:
     return obj
 
 
+FORTRAN = os.environ.get("FC_AMD", "/opt/rocm/bin/amdflang")
+F_SOURCES = ["crtm_mi355x_module.f90", "crtm_mi355x_driver.f90"]
+DRIVER = os.path.join(HERE, "crtm_mi355x_driver")
+
+
+def build_fortran() -> str:
+    """Fortran host interface (CRTM_MI355X_Module) and its example driver,
+    linked against libcrtm_mi355x.so."""
+    fdir = os.path.join(OBJ, "fortran")
+    os.makedirs(fdir, exist_ok=True)
+    objs = []
+    for src in F_SOURCES:
+        obj = os.path.join(fdir, src.replace(".f90", ".o"))
+        cmd = [FORTRAN, "-O2", "-c", os.path.join(HERE, "fortran", src), "-o", obj, "-module-dir", fdir]
+        r = subprocess.run(cmd, capture_output=True, text=True, cwd=fdir)
+        if r.returncode != 0:
+            raise RuntimeError(f"{FORTRAN} failed for {src}:\n{r.stderr[-4000:]}")
+        objs.append(obj)
+    cmd = [FORTRAN, "-o", DRIVER] + objs[::-1] + ["-L" + HERE, "-lcrtm_mi355x", "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"Fortran link failed:\n{r.stderr[-4000:]}")
+    return DRIVER
+
+
 def build(force: bool = False) -> str:
     os.makedirs(OBJ, exist_ok=True)
     with ThreadPoolExecutor(max_workers=len(SOURCES)) as ex:
@@ -58,6 +83,8 @@ def build(force: bool = False) -> str:
         if r.returncode != 0:
             raise RuntimeError(f"link failed:\n{r.stderr[-4000:]}")
         os.replace(LIB + ".tmp", LIB)
+    if os.path.exists(FORTRAN):
+        build_fortran()
     return LIB
 
 

@@ -1,0 +1,84 @@
+! crtm_mi355x_driver.f90 -- Fortran caller of CRTM_MI355X_Module written the
+! way a CRTM application calls CRTM_Module: CRTM_Init, CRTM_Forward,
+! CRTM_K_Matrix with zeroed Atmosphere_K / Surface_K, CRTM_Destroy.
+! Reads a batch from a text file and writes BT, T_K, absorber_K, water
+! temperature K and emissivity K as text (used by the Fortran tests).
+!
+!   crtm_mi355x_driver <input.txt> <output.txt>
+PROGRAM crtm_mi355x_driver
+  USE CRTM_MI355X_Module
+  IMPLICIT NONE
+  CHARACTER(LEN=1024) :: fin, fout, cdir
+  CHARACTER(LEN=20) :: sid(1)
+  TYPE(CRTM_ChannelInfo_type) :: chinfo(1)
+  TYPE(CRTM_Atmosphere_type), ALLOCATABLE :: atm(:), atm_k(:, :)
+  TYPE(CRTM_Surface_type), ALLOCATABLE :: sfc(:), sfc_k(:, :)
+  TYPE(CRTM_Geometry_type), ALLOCATABLE :: geo(:)
+  TYPE(CRTM_RTSolution_type), ALLOCATABLE :: rts(:, :), rts_k(:, :), rts_f(:, :)
+  TYPE(CRTM_Options_type), ALLOCATABLE :: opt(:)
+  INTEGER :: M, na, prec, m1, K, L, j, l1, k1, err, u
+  REAL(fp) :: zen, ts
+
+  CALL GET_COMMAND_ARGUMENT(1, fin)
+  CALL GET_COMMAND_ARGUMENT(2, fout)
+  OPEN(NEWUNIT=u, FILE=TRIM(fin), STATUS='OLD', ACTION='READ')
+  READ(u, '(A)') sid(1)
+  READ(u, '(A)') cdir
+  READ(u, *) M, na, prec
+  err = CRTM_Init(sid, chinfo, File_Path=TRIM(cdir))
+  IF (err /= SUCCESS) STOP 'CRTM_Init failed'
+  L = chinfo(1)%n_Channels
+  ALLOCATE(atm(M), sfc(M), geo(M), opt(M), rts(L, M), rts_k(L, M), rts_f(L, M), atm_k(L, M), sfc_k(L, M))
+  DO m1 = 1, M
+    READ(u, *) K, zen, ts
+    CALL CRTM_Atmosphere_Create(atm(m1:m1), K, na, 0, 0)
+    atm(m1)%Absorber_ID = (/ (MERGE(H2O_ID, O3_ID, j == 1), j = 1, na) /)
+    atm(m1)%Absorber_Units = (/ (MERGE(MASS_MIXING_RATIO_UNITS, VOLUME_MIXING_RATIO_UNITS, j == 1), j = 1, na) /)
+    READ(u, *) atm(m1)%Level_Pressure(0:K)
+    READ(u, *) atm(m1)%Pressure(1:K)
+    READ(u, *) atm(m1)%Temperature(1:K)
+    DO j = 1, na
+      READ(u, *) atm(m1)%Absorber(1:K, j)
+    END DO
+    sfc(m1)%Water_Coverage = 1.0_fp
+    sfc(m1)%Water_Temperature = ts
+    geo(m1)%Sensor_Zenith_Angle = zen
+    opt(m1)%Use_Emissivity = .TRUE.
+    opt(m1)%Precision = prec
+    ALLOCATE(opt(m1)%Emissivity(L))
+    READ(u, *) opt(m1)%Emissivity(1:L)
+    DO l1 = 1, L
+      CALL CRTM_Atmosphere_Create(atm_k(l1:l1, m1), K, na, 0, 0)
+    END DO
+  END DO
+  CLOSE(u)
+
+  err = CRTM_Forward(atm, sfc, geo, chinfo, rts_f, opt)
+  IF (err /= SUCCESS) STOP 'CRTM_Forward failed'
+  CALL CRTM_Atmosphere_Zero(atm_k)
+  CALL CRTM_Surface_Zero(sfc_k)
+  rts_k(:, :)%Brightness_Temperature = 1.0_fp
+  rts_k(:, :)%Radiance = 0.0_fp
+  err = CRTM_K_Matrix(atm, sfc, rts_k, geo, chinfo, atm_k, sfc_k, rts, opt)
+  IF (err /= SUCCESS) STOP 'CRTM_K_Matrix failed'
+
+  OPEN(NEWUNIT=u, FILE=TRIM(fout), STATUS='REPLACE', ACTION='WRITE')
+  WRITE(u, '(I8, I8)') M, L
+  DO m1 = 1, M
+    K = atm(m1)%n_Layers
+    WRITE(u, '(*(ES25.16E3))') (rts_f(l1, m1)%Brightness_Temperature, l1 = 1, L)
+    WRITE(u, '(*(ES25.16E3))') (rts(l1, m1)%Brightness_Temperature, l1 = 1, L)
+    WRITE(u, '(*(ES25.16E3))') (sfc_k(l1, m1)%Water_Temperature, l1 = 1, L)
+    WRITE(u, '(*(ES25.16E3))') (rts_k(l1, m1)%Surface_Emissivity, l1 = 1, L)
+    DO k1 = 1, K
+      WRITE(u, '(*(ES25.16E3))') (atm_k(l1, m1)%Temperature(k1), l1 = 1, L)
+    END DO
+    DO j = 1, na
+      DO k1 = 1, K
+        WRITE(u, '(*(ES25.16E3))') (atm_k(l1, m1)%Absorber(k1, j), l1 = 1, L)
+      END DO
+    END DO
+  END DO
+  CLOSE(u)
+  err = CRTM_Destroy(chinfo)
+END PROGRAM crtm_mi355x_driver

@@ -1,0 +1,473 @@
+! crtm_mi355x_module.f90 -- Fortran host interface of the MI355X CRTM hot path.
+!
+! A thin ISO_C_BINDING layer over libcrtm_mi355x.so (include/crtm_mi355x.h)
+! that keeps the names, argument order and result codes of CRTM's public
+! API for the clear-sky forward / K-matrix path, so a caller written against
+! CRTM_Module (src/CRTM_Module.F90:7-38) compiles unchanged for that path:
+!
+!   CRTM_Init(Sensor_ID, ChannelInfo, File_Path)          src/CRTM_LifeCycle.f90:520
+!   CRTM_Forward(Atm, Sfc, Geo, ChannelInfo, RTS, Opt)    src/CRTM_Forward_Module.f90:229-245
+!   CRTM_K_Matrix(Atm, Sfc, RTS_K, Geo, ChannelInfo, Atm_K, Sfc_K, RTS, Opt)
+!                                                         src/CRTM_K_Matrix_Module.f90:300-325
+!   CRTM_Destroy(ChannelInfo)                             src/CRTM_LifeCycle.f90:1131
+!   CRTM_Atmosphere_Create(Atm, n_Layers, n_Absorbers, n_Clouds, n_Aerosols)
+!                                                         src/Atmosphere/CRTM_Atmosphere_Define.f90:612
+!
+! Derived types carry the components of the reference types this path
+! reads or writes (src/Atmosphere/CRTM_Atmosphere_Define.f90:433-464,
+! src/Surface/CRTM_Surface_Define.f90:204-244,
+! src/GeometryInfo/CRTM_Geometry_Define.f90:99-121,
+! src/RTSolution/CRTM_RTSolution_Define.f90:201-245,
+! src/Options/CRTM_Options_Define.f90:122-179).  As in the reference,
+! Atmosphere_K / Surface_K are accumulated into (callers zero them first).
+MODULE CRTM_MI355X_Module
+  USE ISO_C_BINDING
+  IMPLICIT NONE
+  PRIVATE
+
+  INTEGER, PARAMETER, PUBLIC :: fp = C_DOUBLE
+  INTEGER, PARAMETER, PUBLIC :: SUCCESS = 0, FAILURE = 3
+  INTEGER, PARAMETER, PUBLIC :: H2O_ID = 1, CO2_ID = 2, O3_ID = 3
+  INTEGER, PARAMETER, PUBLIC :: VOLUME_MIXING_RATIO_UNITS = 1, MASS_MIXING_RATIO_UNITS = 3
+  INTEGER, PARAMETER, PUBLIC :: CRTM_PREC_FP64 = 0, CRTM_PREC_FP32_SPLIT = 1
+
+  TYPE, PUBLIC :: CRTM_ChannelInfo_type
+    CHARACTER(LEN=20) :: Sensor_ID = ''
+    INTEGER :: Sensor_Type = 0
+    INTEGER :: n_Channels = 0
+    INTEGER, ALLOCATABLE :: Sensor_Channel(:)
+    INTEGER :: Channel_Offset = 0
+    TYPE(C_PTR) :: ctx = C_NULL_PTR      ! shared by all sensors of one CRTM_Init call
+  END TYPE
+
+  TYPE, PUBLIC :: CRTM_Atmosphere_type
+    INTEGER :: n_Layers = 0, n_Absorbers = 0, n_Clouds = 0, n_Aerosols = 0
+    INTEGER, ALLOCATABLE :: Absorber_ID(:), Absorber_Units(:)
+    REAL(fp), ALLOCATABLE :: Level_Pressure(:)     ! (0:n_Layers)
+    REAL(fp), ALLOCATABLE :: Pressure(:), Temperature(:)
+    REAL(fp), ALLOCATABLE :: Absorber(:, :)        ! (n_Layers, n_Absorbers)
+  END TYPE
+
+  TYPE, PUBLIC :: CRTM_Surface_type
+    REAL(fp) :: Land_Coverage = 0, Water_Coverage = 0, Snow_Coverage = 0, Ice_Coverage = 0
+    REAL(fp) :: Land_Temperature = 283, Water_Temperature = 283, Snow_Temperature = 263, Ice_Temperature = 263
+  END TYPE
+
+  TYPE, PUBLIC :: CRTM_Geometry_type
+    REAL(fp) :: Sensor_Zenith_Angle = 0, Sensor_Scan_Angle = 0, Source_Zenith_Angle = 100
+  END TYPE
+
+  TYPE, PUBLIC :: CRTM_RTSolution_type
+    REAL(fp) :: Radiance = 0, Brightness_Temperature = 0
+    REAL(fp) :: Up_Radiance = 0, Down_Radiance = 0, Surface_Planck_Radiance = 0, Surface_Emissivity = 0
+  END TYPE
+
+  TYPE, PUBLIC :: CRTM_Options_type
+    LOGICAL :: Use_Emissivity = .FALSE.
+    REAL(fp), ALLOCATABLE :: Emissivity(:)         ! (n_Channels)
+    LOGICAL :: Check_Input = .TRUE.
+    INTEGER :: Precision = CRTM_PREC_FP64          ! extension: arithmetic of the GPU path
+  END TYPE
+
+  PUBLIC :: CRTM_Init, CRTM_Destroy, CRTM_Forward, CRTM_K_Matrix
+  PUBLIC :: CRTM_Atmosphere_Create, CRTM_Atmosphere_Zero, CRTM_Surface_Zero
+
+  ! ---- C-ABI mirrors (include/crtm_mi355x.h) ----
+  TYPE, BIND(C) :: c_atmosphere
+    INTEGER(C_INT) :: n_profiles, max_layers
+    INTEGER(C_INT64_T) :: total_layers
+    TYPE(C_PTR) :: n_layers, layer_offset
+    INTEGER(C_INT) :: n_absorbers
+    TYPE(C_PTR) :: absorber_id, absorber_units, level_pressure, pressure, temperature, absorber
+  END TYPE
+  TYPE, BIND(C) :: c_surface
+    TYPE(C_PTR) :: land_coverage, water_coverage, snow_coverage, ice_coverage, &
+                   land_temperature, water_temperature, snow_temperature, ice_temperature
+  END TYPE
+  TYPE, BIND(C) :: c_geometry
+    TYPE(C_PTR) :: sensor_zenith_angle, sensor_scan_angle, source_zenith_angle, surface_altitude
+  END TYPE
+  TYPE, BIND(C) :: c_options
+    INTEGER(C_INT) :: use_emissivity
+    TYPE(C_PTR) :: emissivity
+    INTEGER(C_INT) :: precision, memory, jacobian_f32, check_input
+  END TYPE
+  TYPE, BIND(C) :: c_rtsolution
+    TYPE(C_PTR) :: radiance, brightness_temperature, up_radiance, down_radiance, &
+                   surface_planck_radiance, surface_emissivity, status
+  END TYPE
+  TYPE, BIND(C) :: c_k_output
+    TYPE(C_PTR) :: brightness_temperature_k, radiance_k, surface_emissivity_k, temperature_k, &
+                   absorber_k, land_temperature_k, water_temperature_k, snow_temperature_k, ice_temperature_k
+  END TYPE
+
+  INTERFACE
+    INTEGER(C_INT) FUNCTION c_crtm_init(n, ids, path, ctx) BIND(C, NAME='crtm_init')
+      IMPORT
+      INTEGER(C_INT), VALUE :: n
+      TYPE(C_PTR), INTENT(IN) :: ids(*)
+      CHARACTER(KIND=C_CHAR), INTENT(IN) :: path(*)
+      TYPE(C_PTR), INTENT(OUT) :: ctx
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION c_crtm_destroy(ctx) BIND(C, NAME='crtm_destroy')
+      IMPORT
+      TYPE(C_PTR), VALUE :: ctx
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION c_crtm_n_sensors(ctx) BIND(C, NAME='crtm_n_sensors')
+      IMPORT
+      TYPE(C_PTR), VALUE :: ctx
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION c_crtm_channel_info(ctx, s, n, stype, chans, sid) BIND(C, NAME='crtm_channel_info')
+      IMPORT
+      TYPE(C_PTR), VALUE :: ctx
+      INTEGER(C_INT), VALUE :: s
+      TYPE(C_PTR), VALUE :: n, stype, chans, sid
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION c_crtm_forward(ctx, atm, sfc, geo, opt, rts) BIND(C, NAME='crtm_forward')
+      IMPORT
+      TYPE(C_PTR), VALUE :: ctx
+      TYPE(c_atmosphere), INTENT(IN) :: atm
+      TYPE(c_surface), INTENT(IN) :: sfc
+      TYPE(c_geometry), INTENT(IN) :: geo
+      TYPE(c_options), INTENT(IN) :: opt
+      TYPE(c_rtsolution), INTENT(INOUT) :: rts
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION c_crtm_k_matrix(ctx, atm, sfc, geo, opt, rts, k) BIND(C, NAME='crtm_k_matrix')
+      IMPORT
+      TYPE(C_PTR), VALUE :: ctx
+      TYPE(c_atmosphere), INTENT(IN) :: atm
+      TYPE(c_surface), INTENT(IN) :: sfc
+      TYPE(c_geometry), INTENT(IN) :: geo
+      TYPE(c_options), INTENT(IN) :: opt
+      TYPE(c_rtsolution), INTENT(INOUT) :: rts
+      TYPE(c_k_output), INTENT(INOUT) :: k
+    END FUNCTION
+    FUNCTION c_crtm_last_error(ctx) RESULT(p) BIND(C, NAME='crtm_last_error')
+      IMPORT
+      TYPE(C_PTR), VALUE :: ctx
+      TYPE(C_PTR) :: p
+    END FUNCTION
+  END INTERFACE
+
+  ! Flat host arrays of one call (kept alive while the C-ABI reads them)
+  TYPE :: batch_type
+    INTEGER(C_INT), ALLOCATABLE :: nl(:), ids(:), units(:)
+    INTEGER(C_INT64_T), ALLOCATABLE :: off(:)
+    REAL(C_DOUBLE), ALLOCATABLE :: lp(:), p(:), t(:), a(:), cov(:, :), st(:, :), zen(:), scan(:), src(:), emis(:)
+    REAL(C_DOUBLE), ALLOCATABLE :: rad(:), bt(:), up(:), dn(:), sp(:), se(:)
+    INTEGER(C_INT), ALLOCATABLE :: status(:)
+    TYPE(c_atmosphere) :: atm
+    TYPE(c_surface) :: sfc
+    TYPE(c_geometry) :: geo
+    TYPE(c_options) :: opt
+    TYPE(c_rtsolution) :: rts
+  END TYPE
+
+CONTAINS
+
+  FUNCTION CRTM_Init(Sensor_ID, ChannelInfo, File_Path) RESULT(Error_Status)
+    CHARACTER(*), INTENT(IN) :: Sensor_ID(:)
+    TYPE(CRTM_ChannelInfo_type), INTENT(INOUT) :: ChannelInfo(:)
+    CHARACTER(*), OPTIONAL, INTENT(IN) :: File_Path
+    INTEGER :: Error_Status
+    CHARACTER(KIND=C_CHAR, LEN=64), TARGET :: cid(SIZE(Sensor_ID))
+    TYPE(C_PTR) :: pid(SIZE(Sensor_ID)), ctx
+    CHARACTER(KIND=C_CHAR, LEN=1024) :: cpath
+    INTEGER(C_INT), TARGET :: n, st
+    CHARACTER(KIND=C_CHAR, LEN=32), TARGET :: sid
+    INTEGER(C_INT), ALLOCATABLE, TARGET :: chans(:)
+    INTEGER :: i, off
+    IF (SIZE(ChannelInfo) < SIZE(Sensor_ID)) THEN
+      Error_Status = FAILURE
+      RETURN
+    END IF
+    DO i = 1, SIZE(Sensor_ID)
+      cid(i) = TRIM(Sensor_ID(i)) // C_NULL_CHAR
+      pid(i) = C_LOC(cid(i))
+    END DO
+    cpath = '.' // C_NULL_CHAR
+    IF (PRESENT(File_Path)) cpath = TRIM(File_Path) // C_NULL_CHAR
+    Error_Status = c_crtm_init(INT(SIZE(Sensor_ID), C_INT), pid, cpath, ctx)
+    IF (Error_Status /= SUCCESS) RETURN
+    off = 0
+    DO i = 1, SIZE(Sensor_ID)
+      Error_Status = c_crtm_channel_info(ctx, INT(i - 1, C_INT), C_LOC(n), C_LOC(st), C_NULL_PTR, C_NULL_PTR)
+      IF (Error_Status /= SUCCESS) RETURN
+      ChannelInfo(i)%n_Channels = n
+      ChannelInfo(i)%Sensor_Type = st
+      IF (ALLOCATED(ChannelInfo(i)%Sensor_Channel)) DEALLOCATE(ChannelInfo(i)%Sensor_Channel)
+      ALLOCATE(ChannelInfo(i)%Sensor_Channel(n))
+      IF (ALLOCATED(chans)) DEALLOCATE(chans)
+      ALLOCATE(chans(n))
+      Error_Status = c_crtm_channel_info(ctx, INT(i - 1, C_INT), C_NULL_PTR, C_NULL_PTR, C_LOC(chans), C_LOC(sid))
+      ChannelInfo(i)%Sensor_Channel = chans
+      ChannelInfo(i)%Sensor_ID = c_to_f(sid)
+      ChannelInfo(i)%Channel_Offset = off
+      ChannelInfo(i)%ctx = ctx
+      off = off + n
+    END DO
+  END FUNCTION CRTM_Init
+
+  FUNCTION CRTM_Destroy(ChannelInfo) RESULT(Error_Status)
+    TYPE(CRTM_ChannelInfo_type), INTENT(INOUT) :: ChannelInfo(:)
+    INTEGER :: Error_Status
+    Error_Status = SUCCESS
+    IF (SIZE(ChannelInfo) < 1) RETURN
+    IF (C_ASSOCIATED(ChannelInfo(1)%ctx)) Error_Status = c_crtm_destroy(ChannelInfo(1)%ctx)
+    ChannelInfo(:)%ctx = C_NULL_PTR
+  END FUNCTION CRTM_Destroy
+
+  SUBROUTINE CRTM_Atmosphere_Create(Atm, n_Layers, n_Absorbers, n_Clouds, n_Aerosols)
+    TYPE(CRTM_Atmosphere_type), INTENT(INOUT) :: Atm(:)
+    INTEGER, INTENT(IN) :: n_Layers, n_Absorbers, n_Clouds, n_Aerosols
+    INTEGER :: m
+    DO m = 1, SIZE(Atm)
+      Atm(m)%n_Layers = n_Layers
+      Atm(m)%n_Absorbers = n_Absorbers
+      Atm(m)%n_Clouds = n_Clouds
+      Atm(m)%n_Aerosols = n_Aerosols
+      IF (ALLOCATED(Atm(m)%Level_Pressure)) DEALLOCATE(Atm(m)%Level_Pressure, Atm(m)%Pressure, &
+          Atm(m)%Temperature, Atm(m)%Absorber, Atm(m)%Absorber_ID, Atm(m)%Absorber_Units)
+      ALLOCATE(Atm(m)%Level_Pressure(0:n_Layers), Atm(m)%Pressure(n_Layers), Atm(m)%Temperature(n_Layers), &
+               Atm(m)%Absorber(n_Layers, n_Absorbers), Atm(m)%Absorber_ID(n_Absorbers), &
+               Atm(m)%Absorber_Units(n_Absorbers))
+      Atm(m)%Level_Pressure = 0; Atm(m)%Pressure = 0; Atm(m)%Temperature = 0; Atm(m)%Absorber = 0
+      Atm(m)%Absorber_ID = 0; Atm(m)%Absorber_Units = 0
+    END DO
+  END SUBROUTINE CRTM_Atmosphere_Create
+
+  SUBROUTINE CRTM_Atmosphere_Zero(Atm)
+    TYPE(CRTM_Atmosphere_type), INTENT(INOUT) :: Atm(:, :)
+    INTEGER :: i, j
+    DO j = 1, SIZE(Atm, 2)
+      DO i = 1, SIZE(Atm, 1)
+        IF (ALLOCATED(Atm(i, j)%Temperature)) Atm(i, j)%Temperature = 0
+        IF (ALLOCATED(Atm(i, j)%Absorber)) Atm(i, j)%Absorber = 0
+      END DO
+    END DO
+  END SUBROUTINE CRTM_Atmosphere_Zero
+
+  ! CRTM_Surface_Zero (src/Surface/CRTM_Surface_Define.f90): zero the
+  ! coverages and temperatures of a Surface_K array before CRTM_K_Matrix
+  SUBROUTINE CRTM_Surface_Zero(Sfc)
+    TYPE(CRTM_Surface_type), INTENT(INOUT) :: Sfc(:, :)
+    Sfc(:, :)%Land_Coverage = 0; Sfc(:, :)%Water_Coverage = 0
+    Sfc(:, :)%Snow_Coverage = 0; Sfc(:, :)%Ice_Coverage = 0
+    Sfc(:, :)%Land_Temperature = 0; Sfc(:, :)%Water_Temperature = 0
+    Sfc(:, :)%Snow_Temperature = 0; Sfc(:, :)%Ice_Temperature = 0
+  END SUBROUTINE CRTM_Surface_Zero
+
+  FUNCTION CRTM_Forward(Atmosphere, Surface, Geometry, ChannelInfo, RTSolution, Options) RESULT(Error_Status)
+    TYPE(CRTM_Atmosphere_type), INTENT(IN) :: Atmosphere(:)
+    TYPE(CRTM_Surface_type), INTENT(IN) :: Surface(:)
+    TYPE(CRTM_Geometry_type), INTENT(IN) :: Geometry(:)
+    TYPE(CRTM_ChannelInfo_type), INTENT(IN) :: ChannelInfo(:)
+    TYPE(CRTM_RTSolution_type), INTENT(INOUT) :: RTSolution(:, :)
+    TYPE(CRTM_Options_type), OPTIONAL, INTENT(IN) :: Options(:)
+    INTEGER :: Error_Status
+    TYPE(batch_type), TARGET :: b
+    INTEGER :: L, M, mm, ic
+    Error_Status = pack_batch(Atmosphere, Surface, Geometry, ChannelInfo, RTSolution, Options, b, L, M)
+    IF (Error_Status /= SUCCESS) RETURN
+    Error_Status = c_crtm_forward(ChannelInfo(1)%ctx, b%atm, b%sfc, b%geo, b%opt, b%rts)
+    IF (Error_Status /= SUCCESS) THEN
+      CALL report(ChannelInfo(1)%ctx, 'CRTM_Forward')
+      RETURN
+    END IF
+    DO mm = 1, M
+      DO ic = 1, L
+        CALL unpack_rts(b, (mm - 1) * L + ic, RTSolution(ic, mm))
+      END DO
+    END DO
+  END FUNCTION CRTM_Forward
+
+  FUNCTION CRTM_K_Matrix(Atmosphere, Surface, RTSolution_K, Geometry, ChannelInfo, Atmosphere_K, Surface_K, &
+                         RTSolution, Options) RESULT(Error_Status)
+    TYPE(CRTM_Atmosphere_type), INTENT(IN) :: Atmosphere(:)
+    TYPE(CRTM_Surface_type), INTENT(IN) :: Surface(:)
+    TYPE(CRTM_RTSolution_type), INTENT(INOUT) :: RTSolution_K(:, :)
+    TYPE(CRTM_Geometry_type), INTENT(IN) :: Geometry(:)
+    TYPE(CRTM_ChannelInfo_type), INTENT(IN) :: ChannelInfo(:)
+    TYPE(CRTM_Atmosphere_type), INTENT(INOUT) :: Atmosphere_K(:, :)
+    TYPE(CRTM_Surface_type), INTENT(INOUT) :: Surface_K(:, :)
+    TYPE(CRTM_RTSolution_type), INTENT(INOUT) :: RTSolution(:, :)
+    TYPE(CRTM_Options_type), OPTIONAL, INTENT(IN) :: Options(:)
+    INTEGER :: Error_Status
+    TYPE(batch_type), TARGET :: b
+    TYPE(c_k_output) :: k
+    REAL(C_DOUBLE), ALLOCATABLE, TARGET :: btk(:), radk(:), emk(:), tk(:), ak(:), stk(:, :)
+    INTEGER :: L, M, mm, ic, j, kk, NT, na, K0
+    Error_Status = pack_batch(Atmosphere, Surface, Geometry, ChannelInfo, RTSolution, Options, b, L, M)
+    IF (Error_Status /= SUCCESS) RETURN
+    NT = INT(b%off(M + 1))
+    na = Atmosphere(1)%n_Absorbers
+    ALLOCATE(btk(L * M), radk(L * M), emk(L * M), tk(NT * L), ak(na * NT * L), stk(L * M, 4))
+    DO mm = 1, M
+      DO ic = 1, L
+        btk((mm - 1) * L + ic) = RTSolution_K(ic, mm)%Brightness_Temperature
+        radk((mm - 1) * L + ic) = RTSolution_K(ic, mm)%Radiance
+      END DO
+    END DO
+    k%brightness_temperature_k = C_LOC(btk)
+    k%radiance_k = C_LOC(radk)
+    k%surface_emissivity_k = C_LOC(emk)
+    k%temperature_k = C_LOC(tk)
+    k%absorber_k = C_LOC(ak)
+    k%land_temperature_k = C_LOC(stk(1, 1))
+    k%water_temperature_k = C_LOC(stk(1, 2))
+    k%snow_temperature_k = C_LOC(stk(1, 3))
+    k%ice_temperature_k = C_LOC(stk(1, 4))
+    Error_Status = c_crtm_k_matrix(ChannelInfo(1)%ctx, b%atm, b%sfc, b%geo, b%opt, b%rts, k)
+    IF (Error_Status /= SUCCESS) THEN
+      CALL report(ChannelInfo(1)%ctx, 'CRTM_K_Matrix')
+      RETURN
+    END IF
+    DO mm = 1, M
+      K0 = INT(b%off(mm))
+      DO ic = 1, L
+        CALL unpack_rts(b, (mm - 1) * L + ic, RTSolution(ic, mm))
+        RTSolution_K(ic, mm)%Radiance = radk((mm - 1) * L + ic)
+        RTSolution_K(ic, mm)%Surface_Emissivity = RTSolution_K(ic, mm)%Surface_Emissivity + emk((mm - 1) * L + ic)
+        DO kk = 1, Atmosphere(mm)%n_Layers
+          Atmosphere_K(ic, mm)%Temperature(kk) = Atmosphere_K(ic, mm)%Temperature(kk) + tk((K0 + kk - 1) * L + ic)
+          DO j = 1, na
+            Atmosphere_K(ic, mm)%Absorber(kk, j) = Atmosphere_K(ic, mm)%Absorber(kk, j) + &
+                ak(((j - 1) * NT + K0 + kk - 1) * L + ic)
+          END DO
+        END DO
+        Surface_K(ic, mm)%Land_Temperature = Surface_K(ic, mm)%Land_Temperature + stk((mm - 1) * L + ic, 1)
+        Surface_K(ic, mm)%Water_Temperature = Surface_K(ic, mm)%Water_Temperature + stk((mm - 1) * L + ic, 2)
+        Surface_K(ic, mm)%Snow_Temperature = Surface_K(ic, mm)%Snow_Temperature + stk((mm - 1) * L + ic, 3)
+        Surface_K(ic, mm)%Ice_Temperature = Surface_K(ic, mm)%Ice_Temperature + stk((mm - 1) * L + ic, 4)
+      END DO
+    END DO
+  END FUNCTION CRTM_K_Matrix
+
+  ! ---- helpers ----
+  FUNCTION pack_batch(Atm, Sfc, Geo, ChannelInfo, RTS, Opt, b, L, M) RESULT(Error_Status)
+    TYPE(CRTM_Atmosphere_type), INTENT(IN) :: Atm(:)
+    TYPE(CRTM_Surface_type), INTENT(IN) :: Sfc(:)
+    TYPE(CRTM_Geometry_type), INTENT(IN) :: Geo(:)
+    TYPE(CRTM_ChannelInfo_type), INTENT(IN) :: ChannelInfo(:)
+    TYPE(CRTM_RTSolution_type), INTENT(IN) :: RTS(:, :)
+    TYPE(CRTM_Options_type), OPTIONAL, INTENT(IN) :: Opt(:)
+    TYPE(batch_type), TARGET, INTENT(INOUT) :: b
+    INTEGER, INTENT(OUT) :: L, M
+    INTEGER :: Error_Status, mm, NT, na, K0, K, j
+    Error_Status = FAILURE
+    M = SIZE(Atm)
+    L = SUM(ChannelInfo(:)%n_Channels)
+    IF (SIZE(Sfc) /= M .OR. SIZE(Geo) /= M .OR. SIZE(RTS, 2) < M .OR. SIZE(RTS, 1) < L) RETURN
+    IF (.NOT. PRESENT(Opt)) RETURN                  ! emissivity is required by this path
+    IF (SIZE(Opt) /= M) RETURN
+    na = Atm(1)%n_Absorbers
+    ALLOCATE(b%nl(M), b%off(M + 1))
+    b%off(1) = 0
+    DO mm = 1, M
+      IF (Atm(mm)%n_Absorbers /= na) RETURN
+      b%nl(mm) = Atm(mm)%n_Layers
+      b%off(mm + 1) = b%off(mm) + Atm(mm)%n_Layers
+    END DO
+    NT = INT(b%off(M + 1))
+    ALLOCATE(b%ids(na), b%units(na), b%lp(NT + M), b%p(NT), b%t(NT), b%a(na * NT), b%cov(M, 4), b%st(M, 4), &
+             b%zen(M), b%scan(M), b%src(M), b%emis(L * M), b%rad(L * M), b%bt(L * M), b%up(L * M), &
+             b%dn(L * M), b%sp(L * M), b%se(L * M), b%status(M))
+    b%ids = Atm(1)%Absorber_ID
+    b%units = Atm(1)%Absorber_Units
+    DO mm = 1, M
+      K0 = INT(b%off(mm))
+      K = Atm(mm)%n_Layers
+      b%lp(K0 + mm:K0 + mm + K) = Atm(mm)%Level_Pressure(0:K)
+      b%p(K0 + 1:K0 + K) = Atm(mm)%Pressure(1:K)
+      b%t(K0 + 1:K0 + K) = Atm(mm)%Temperature(1:K)
+      DO j = 1, na
+        b%a((j - 1) * NT + K0 + 1:(j - 1) * NT + K0 + K) = Atm(mm)%Absorber(1:K, j)
+      END DO
+      b%cov(mm, :) = (/ Sfc(mm)%Land_Coverage, Sfc(mm)%Water_Coverage, Sfc(mm)%Snow_Coverage, Sfc(mm)%Ice_Coverage /)
+      b%st(mm, :) = (/ Sfc(mm)%Land_Temperature, Sfc(mm)%Water_Temperature, Sfc(mm)%Snow_Temperature, &
+                       Sfc(mm)%Ice_Temperature /)
+      b%zen(mm) = Geo(mm)%Sensor_Zenith_Angle
+      b%scan(mm) = Geo(mm)%Sensor_Scan_Angle
+      b%src(mm) = Geo(mm)%Source_Zenith_Angle
+      IF (.NOT. Opt(mm)%Use_Emissivity) RETURN
+      IF (SIZE(Opt(mm)%Emissivity) < L) RETURN
+      b%emis((mm - 1) * L + 1:mm * L) = Opt(mm)%Emissivity(1:L)
+    END DO
+    b%atm%n_profiles = M
+    b%atm%max_layers = MAXVAL(b%nl)
+    b%atm%total_layers = NT
+    b%atm%n_layers = C_LOC(b%nl)
+    b%atm%layer_offset = C_LOC(b%off)
+    b%atm%n_absorbers = na
+    b%atm%absorber_id = C_LOC(b%ids)
+    b%atm%absorber_units = C_LOC(b%units)
+    b%atm%level_pressure = C_LOC(b%lp)
+    b%atm%pressure = C_LOC(b%p)
+    b%atm%temperature = C_LOC(b%t)
+    b%atm%absorber = C_LOC(b%a)
+    b%sfc%land_coverage = C_LOC(b%cov(1, 1))
+    b%sfc%water_coverage = C_LOC(b%cov(1, 2))
+    b%sfc%snow_coverage = C_LOC(b%cov(1, 3))
+    b%sfc%ice_coverage = C_LOC(b%cov(1, 4))
+    b%sfc%land_temperature = C_LOC(b%st(1, 1))
+    b%sfc%water_temperature = C_LOC(b%st(1, 2))
+    b%sfc%snow_temperature = C_LOC(b%st(1, 3))
+    b%sfc%ice_temperature = C_LOC(b%st(1, 4))
+    b%geo%sensor_zenith_angle = C_LOC(b%zen)
+    b%geo%sensor_scan_angle = C_LOC(b%scan)
+    b%geo%source_zenith_angle = C_LOC(b%src)
+    b%geo%surface_altitude = C_NULL_PTR
+    b%opt%use_emissivity = 1
+    b%opt%emissivity = C_LOC(b%emis)
+    b%opt%precision = Opt(1)%Precision
+    b%opt%memory = 0
+    b%opt%jacobian_f32 = 0
+    b%opt%check_input = MERGE(1, 0, Opt(1)%Check_Input)
+    b%rts%radiance = C_LOC(b%rad)
+    b%rts%brightness_temperature = C_LOC(b%bt)
+    b%rts%up_radiance = C_LOC(b%up)
+    b%rts%down_radiance = C_LOC(b%dn)
+    b%rts%surface_planck_radiance = C_LOC(b%sp)
+    b%rts%surface_emissivity = C_LOC(b%se)
+    b%rts%status = C_LOC(b%status)
+    Error_Status = SUCCESS
+  END FUNCTION pack_batch
+
+  SUBROUTINE unpack_rts(b, i, r)
+    TYPE(batch_type), INTENT(IN) :: b
+    INTEGER, INTENT(IN) :: i
+    TYPE(CRTM_RTSolution_type), INTENT(INOUT) :: r
+    r%Radiance = b%rad(i)
+    r%Brightness_Temperature = b%bt(i)
+    r%Up_Radiance = b%up(i)
+    r%Down_Radiance = b%dn(i)
+    r%Surface_Planck_Radiance = b%sp(i)
+    r%Surface_Emissivity = b%se(i)
+  END SUBROUTINE unpack_rts
+
+  SUBROUTINE report(ctx, who)
+    TYPE(C_PTR), INTENT(IN) :: ctx
+    CHARACTER(*), INTENT(IN) :: who
+    CHARACTER(KIND=C_CHAR), POINTER :: msg(:)
+    INTEGER :: n
+    CALL C_F_POINTER(c_crtm_last_error(ctx), msg, (/ 512 /))
+    n = 0
+    DO WHILE (n < 512)
+      IF (msg(n + 1) == C_NULL_CHAR) EXIT
+      n = n + 1
+    END DO
+    WRITE(*, '(A, ": ", 512A)') who, msg(1:n)
+  END SUBROUTINE report
+
+  FUNCTION c_to_f(s) RESULT(f)
+    CHARACTER(KIND=C_CHAR, LEN=*), INTENT(IN) :: s
+    CHARACTER(LEN=20) :: f
+    INTEGER :: i
+    f = ''
+    DO i = 1, MIN(20, LEN(s))
+      IF (s(i:i) == C_NULL_CHAR) EXIT
+      f(i:i) = s(i:i)
+    END DO
+  END FUNCTION c_to_f
+
+END MODULE CRTM_MI355X_Module

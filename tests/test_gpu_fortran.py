@@ -1,0 +1,72 @@
+"""The Fortran host interface (CRTM_MI355X_Module): a Fortran program calls
+CRTM_Init / CRTM_Forward / CRTM_K_Matrix / CRTM_Destroy the way a CRTM
+application does, through the ISO_C_BINDING layer, and its results match
+the FP64 oracle."""
+from __future__ import annotations
+
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from conftest import load_sensor
+
+pytestmark = pytest.mark.gpu
+
+
+def _write_input(path, name, coeff_dir, ps, eps, prec):
+    with open(path, "w") as f:
+        f.write(f"{name}\n{coeff_dir}\n{ps.n_profiles} {ps.absorber.shape[0]} {prec}\n")
+        for m in range(ps.n_profiles):
+            pr = ps.profile(m)
+            K = pr["temperature"].size
+            f.write(f"{K} {float(ps.sensor_zenith[m])!r} {float(ps.water_temperature[m])!r}\n")
+            for arr in (pr["level_pressure"], pr["pressure"], pr["temperature"], *pr["absorber"], eps[m]):
+                f.write(" ".join(repr(float(x)) for x in arr) + "\n")
+
+
+def _read_output(path, ps, na):
+    vals = open(path).read().split()
+    M, L = int(vals[0]), int(vals[1])
+    it = iter(float(v) for v in vals[2:])
+    out = []
+    for m in range(M):
+        K = int(ps.n_layers[m])
+        take = lambda n: np.array([next(it) for _ in range(n)])
+        r = {"bt_fwd": take(L), "bt": take(L), "ts_k": take(L), "emis_k": take(L),
+             "t_k": take(K * L).reshape(K, L).T, "abs_k": take(na * K * L).reshape(na, K, L).transpose(2, 0, 1)}
+        out.append(r)
+    return out
+
+
+@pytest.mark.parametrize("prec", [0, 1], ids=["fp64", "split"])
+def test_fortran_driver_matches_oracle(coeff_dir, oracle_mod, tmp_path, prec):
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from crtmv3_amd import build, synthetic
+    if not os.path.exists(build.DRIVER):
+        raise RuntimeError(f"{build.DRIVER} not built (python -m crtmv3_amd.build)")
+    name = "cris_small_syn"
+    ps = synthetic.make_profiles(3, [100, 77, 120], seed=55)
+    tau, spc = load_sensor(coeff_dir, name)
+    L = tau.n_channels
+    eps = np.random.default_rng(1).uniform(0.9, 0.99, (3, L))
+    fin, fout = str(tmp_path / "in.txt"), str(tmp_path / "out.txt")
+    _write_input(fin, name, coeff_dir, ps, eps, prec)
+    print(f"case fortran driver prec={prec}", flush=True)
+    r = subprocess.run([build.DRIVER, fin, fout], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+    got = _read_output(fout, ps, 2)
+    s = oracle_mod.Sensor(tau, spc)
+    bt_tol, jac_tol = (1e-8, 1e-9) if prec == 0 else (1e-2, 1e-3)
+    for m in range(3):
+        ref = oracle_mod.run_profile(s, ps, m, eps[m], "k")
+        assert np.abs(got[m]["bt_fwd"] - ref["bt"]).max() <= bt_tol
+        assert np.abs(got[m]["bt"] - ref["bt"]).max() <= bt_tol
+        for key, rkey in (("t_k", "t_k"), ("abs_k", "abs_k")):
+            err = np.abs(got[m][key] - ref[rkey]).max() / np.abs(ref[rkey]).max()
+            assert err <= jac_tol, (m, key, err)
+        assert np.abs(got[m]["ts_k"] - ref["sfc_t_k"][:, 1]).max() <= jac_tol * np.abs(ref["sfc_t_k"][:, 1]).max()
+        assert np.abs(got[m]["emis_k"] - ref["emis_k"]).max() <= jac_tol * np.abs(ref["emis_k"]).max()
