@@ -45,7 +45,8 @@ class CAtmosphere(ctypes.Structure):
     _fields_ = [("n_profiles", ctypes.c_int), ("max_layers", ctypes.c_int), ("total_layers", ctypes.c_int64),
                 ("n_layers", _vp), ("layer_offset", _vp),
                 ("n_absorbers", ctypes.c_int), ("absorber_id", _vp), ("absorber_units", _vp),
-                ("level_pressure", _vp), ("pressure", _vp), ("temperature", _vp), ("absorber", _vp)]
+                ("level_pressure", _vp), ("pressure", _vp), ("temperature", _vp), ("absorber", _vp),
+                ("climatology", _vp)]
 
 
 class CSurface(ctypes.Structure):
@@ -191,6 +192,8 @@ class Context:
         atm.pressure = _ptr(arr(prof.pressure, np.float64))
         atm.temperature = _ptr(arr(prof.temperature, np.float64))
         atm.absorber = _ptr(arr(prof.absorber, np.float64))
+        clim = getattr(prof, "climatology", None)
+        atm.climatology = _ptr(arr(clim, np.int32)) if clim is not None else None
         sfc = CSurface()
         for f in ("land_coverage", "water_coverage", "snow_coverage", "ice_coverage", "land_temperature",
                   "water_temperature", "snow_temperature", "ice_temperature"):

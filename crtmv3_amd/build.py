@@ -28,11 +28,37 @@ FLAGS = ["-O3", "-std=c++17", "-fPIC", f"--offload-arch={ARCH}",
          "-Wno-unused-command-line-argument"]
 
 
+CLIM_JSON = os.path.join(HERE, "data", "model_profiles.json")
+CLIM_INC = os.path.join(OBJ, "model_profiles.inc")
+
+
+def _write_climatology_table() -> None:
+    """C table of the six model climatologies (from data/model_profiles.json,
+    extracted from src/Atmosphere/CRTM_Model_Profiles.f90) for AddLayers:
+    level pressure, temperature per model, absorbers per model in absorber-id
+    order H2O, CO2, O3, N2O, CO, CH4."""
+    import json
+    d = json.load(open(CLIM_JSON))
+    order = [d["absorbers"].index(n) for n in ("H2O", "CO2", "O3", "N2O", "CO", "CH4")]
+    vals = list(d["level_pressure"])
+    for mdl in range(6):
+        vals += d["temperature"][mdl]
+    for mdl in range(6):
+        for j in order:
+            vals += d["absorber"][mdl][j]
+    body = ",\n".join(", ".join(repr(float(v)) for v in vals[i:i + 6]) for i in range(0, len(vals), 6))
+    txt = ("// generated by crtmv3_amd/build.py from data/model_profiles.json\n"
+           f"static const double CLIM_TABLE[{len(vals)}] = {{\n{body}}};\n")
+    if not os.path.exists(CLIM_INC) or open(CLIM_INC).read() != txt:
+        with open(CLIM_INC, "w") as f:
+            f.write(txt)
+
+
 def _stale(obj: str, src: str) -> bool:
     if not os.path.exists(obj):
         return True
     t = os.path.getmtime(obj)
-    deps = [src] + [os.path.join(CSRC, h) for h in HEADERS]
+    deps = [src, CLIM_INC] + [os.path.join(CSRC, h) for h in HEADERS]
     return any(os.path.getmtime(d) > t for d in deps)
 
 
@@ -41,7 +67,7 @@ def _compile(src: str, force: bool) -> str:
     obj = os.path.join(OBJ, src + ".o")
     if force or _stale(obj, path):
         lang = [] if src.endswith(".hip") else ["-x", "hip"]
-        cmd = [HIPCC] + FLAGS + lang + ["-c", path, "-o", obj]
+        cmd = [HIPCC] + FLAGS + lang + ["-I" + OBJ, "-c", path, "-o", obj]
         r = subprocess.run(cmd, capture_output=True, text=True)
         if r.returncode != 0:
             raise RuntimeError(f"hipcc failed for {src}:\n{r.stderr[-6000:]}")
@@ -75,6 +101,7 @@ def build_fortran() -> str:
 
 def build(force: bool = False) -> str:
     os.makedirs(OBJ, exist_ok=True)
+    _write_climatology_table()
     with ThreadPoolExecutor(max_workers=len(SOURCES)) as ex:
         objs = list(ex.map(lambda s: _compile(s, force), SOURCES))
     if force or not os.path.exists(LIB) or any(os.path.getmtime(o) > os.path.getmtime(LIB) for o in objs):

@@ -496,9 +496,13 @@ clear_fwdk_kernel(DevSensor s, DevBatch bt, ProfWork<RT> w, DevOut o) {
   int pcur = Ko;
   const int64_t Lt = bt.Ltot;
   const int na = bt.na;
-  KT* tko = reinterpret_cast<KT*>(o.t_k) + loff * Lt + lg;
-  KT* ako = reinterpret_cast<KT*>(o.a_k) + loff * Lt + lg;
-  const int64_t astr = bt.NT * Lt;
+  // Jacobian rows of the caller's layers; the nx layers AddLayers put on
+  // top carry no output (CRTM_Atmosphere_AddLayers_AD keeps only the
+  // original layers, src/Atmosphere/CRTM_Atmosphere.f90:569-659)
+  const int nx = ok_prof ? bt.n_extra[m] : 0;
+  KT* tko = reinterpret_cast<KT*>(o.t_k) + (bt.out_loff[m] - nx) * Lt + lg;
+  KT* ako = reinterpret_cast<KT*>(o.a_k) + (bt.out_loff[m] - nx) * Lt + lg;
+  const int64_t astr = bt.out_NT * Lt;
   // Jacobian pieces kept in private memory until the sweep ends, so each
   // output element is written exactly once: Planck part per user layer, and
   // the ODPS-layer adjoints of T and of the user absorbers that Map_Input_AD
@@ -678,6 +682,7 @@ clear_fwdk_kernel(DevSensor s, DevBatch bt, ProfWork<RT> w, DevOut o) {
   }
   int kmin = 0, kmax = -1;
   for (int iu = 0; iu < K; ++iu) {
+    if (iu < nx) continue;
     while (kmax + 1 < Ko && acc_i[3 * (kmax + 1)] <= iu) ++kmax;
     while (kmin < Ko && acc_i[3 * kmin] + acc_i[3 * kmin + 1] - 1 < iu) ++kmin;
     RT st = plk[iu];

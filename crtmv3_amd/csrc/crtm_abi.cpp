@@ -17,8 +17,10 @@
 
 #include "../../include/crtm_mi355x.h"
 #include "crtm_device.h"
+#include "model_profiles.inc"   // CLIM_TABLE[CLIM_SIZE], generated from the model climatologies
 
 namespace crtm {
+void launch_add_layers(const DevBatch&, const ExtBatch&, const double* clim, const int* ids, hipStream_t);
 template <int G, class RT>
 void launch_prep(const DevSensor&, const DevBatch&, const ProfWork<RT>&, hipStream_t);
 template <int G, class RT, class KT, bool DOK>
@@ -518,6 +520,8 @@ struct crtm_context {
   // work pool (grown on demand, reused across calls)
   DevBuf wx, wu, wo, wacci, waccw, wuidx, wufrac, wgeo, wstatus;
   DevBuf stage;  // host-mode staging of inputs/outputs
+  DevBuf clim;   // model climatology table (AddLayers)
+  DevBuf xnl, xloff, xnx, xlp, xp, xt, xa;   // profiles extended by AddLayers
   // kernel timing (crtm_profile_enable): HIP events on the context stream
   bool profiling = false;
   std::vector<hipEvent_t> ev;      // pairs (start, stop) per recorded launch
@@ -617,6 +621,37 @@ struct Stager {
   }
 };
 
+// CRTM_Atmosphere_AddLayers for the whole batch: the returned batch
+// describes the extended profiles; Jacobian outputs keep the caller's layout.
+DevBatch add_layers(crtm_context* c, const DevBatch& b, const int* user_ids) {
+  const int M = b.M;
+  const int64_t NTe = b.NT + (int64_t)M * N_EXTRA_MAX;
+  c->xnl.reserve(M * sizeof(int));
+  c->xloff.reserve((M + 1) * sizeof(int64_t));
+  c->xnx.reserve(M * sizeof(int));
+  c->xlp.reserve((NTe + M) * sizeof(double));
+  c->xp.reserve(NTe * sizeof(double));
+  c->xt.reserve(NTe * sizeof(double));
+  c->xa.reserve(std::max(b.na, 1) * NTe * sizeof(double));
+  ExtBatch e{c->xnl.as<int>(), c->xloff.as<int64_t>(), c->xnx.as<int>(), c->xlp.as<double>(), c->xp.as<double>(),
+             c->xt.as<double>(), c->xa.as<double>(), NTe};
+  launch_add_layers(b, e, c->clim.as<double>(), user_ids, c->stream);
+  HIPCHECK(hipGetLastError());
+  DevBatch be = b;
+  be.n_layers = e.n_layers;
+  be.loff = e.loff;
+  be.level_p = e.level_p;
+  be.p = e.p;
+  be.t = e.t;
+  be.a = e.a;
+  be.NT = NTe;
+  be.max_k = b.max_k + N_EXTRA_MAX;
+  be.n_extra = e.n_extra;
+  be.out_loff = b.loff;
+  be.out_NT = b.NT;
+  return be;
+}
+
 int do_call(crtm_context* ctx, const crtm_atmosphere* atm, const crtm_surface* sfc, const crtm_geometry* geo,
             const crtm_options* opt, crtm_rtsolution* rts, crtm_k_output* k) {
   if (!ctx) return fail(nullptr, "null context");
@@ -663,6 +698,7 @@ int do_call(crtm_context* ctx, const crtm_atmosphere* atm, const crtm_surface* s
   if (max_k < 1 || max_k > CRTM_MAX_N_LAYERS) return fail(ctx, fmt("invalid max_layers %d", max_k));
   const int na = atm->n_absorbers;
   std::vector<int> abs_id(atm->absorber_id, atm->absorber_id + na);
+  abs_id.resize(NA_MAX, 0);
   if (opt->check_input && host) {
     for (int m = 0; m < M; ++m) {
       const double* lp = atm->level_pressure + off[m] + m;
@@ -695,6 +731,7 @@ int do_call(crtm_context* ctx, const crtm_atmosphere* atm, const crtm_surface* s
     in(atm->pressure, NT * sizeof(double), (const void**)&b.p);
     in(atm->temperature, NT * sizeof(double), (const void**)&b.t);
     in(atm->absorber, na * NT * sizeof(double), (const void**)&b.a);
+    in(atm->climatology, M * sizeof(int), (const void**)&b.climatology);
     const double* cov[4] = {sfc->land_coverage, sfc->water_coverage, sfc->snow_coverage, sfc->ice_coverage};
     const double* st[4] = {sfc->land_temperature, sfc->water_temperature, sfc->snow_temperature, sfc->ice_temperature};
     for (int c = 0; c < 4; ++c) {
@@ -746,12 +783,13 @@ int do_call(crtm_context* ctx, const crtm_atmosphere* atm, const crtm_surface* s
       *outs[i].d = base + ooff[i];
     }
     if (dok && !k->radiance_k) o.rad_k = nullptr;
+    const DevBatch be = add_layers(ctx, b, abs_id.data());
     for (auto& S : ctx->sensors) {
       if (dok) {
-        if (split) { if (kf32) run_sensor<float, float, true>(ctx, *S, b, o); else run_sensor<float, double, true>(ctx, *S, b, o); }
-        else { if (kf32) run_sensor<double, float, true>(ctx, *S, b, o); else run_sensor<double, double, true>(ctx, *S, b, o); }
+        if (split) { if (kf32) run_sensor<float, float, true>(ctx, *S, be, o); else run_sensor<float, double, true>(ctx, *S, be, o); }
+        else { if (kf32) run_sensor<double, float, true>(ctx, *S, be, o); else run_sensor<double, double, true>(ctx, *S, be, o); }
       } else {
-        if (split) run_sensor<float, float, false>(ctx, *S, b, o); else run_sensor<double, double, false>(ctx, *S, b, o);
+        if (split) run_sensor<float, float, false>(ctx, *S, be, o); else run_sensor<double, double, false>(ctx, *S, be, o);
       }
       if (rts->status) {
         std::vector<int> st(M);
@@ -775,6 +813,7 @@ int do_call(crtm_context* ctx, const crtm_atmosphere* atm, const crtm_surface* s
   b.p = atm->pressure;
   b.t = atm->temperature;
   b.a = atm->absorber;
+  b.climatology = atm->climatology;
   b.cov[0] = sfc->land_coverage; b.cov[1] = sfc->water_coverage; b.cov[2] = sfc->snow_coverage; b.cov[3] = sfc->ice_coverage;
   b.sfc_t[0] = sfc->land_temperature; b.sfc_t[1] = sfc->water_temperature;
   b.sfc_t[2] = sfc->snow_temperature; b.sfc_t[3] = sfc->ice_temperature;
@@ -801,12 +840,13 @@ int do_call(crtm_context* ctx, const crtm_atmosphere* atm, const crtm_surface* s
     o.a_k = k->absorber_k;
     if (!o.t_k || !o.a_k) return fail(ctx, "K-matrix temperature/absorber arrays required");
   }
+  const DevBatch be = add_layers(ctx, b, abs_id.data());
   for (auto& S : ctx->sensors) {
     if (dok) {
-      if (split) { if (kf32) run_sensor<float, float, true>(ctx, *S, b, o); else run_sensor<float, double, true>(ctx, *S, b, o); }
-      else { if (kf32) run_sensor<double, float, true>(ctx, *S, b, o); else run_sensor<double, double, true>(ctx, *S, b, o); }
+      if (split) { if (kf32) run_sensor<float, float, true>(ctx, *S, be, o); else run_sensor<float, double, true>(ctx, *S, be, o); }
+      else { if (kf32) run_sensor<double, float, true>(ctx, *S, be, o); else run_sensor<double, double, true>(ctx, *S, be, o); }
     } else {
-      if (split) run_sensor<float, float, false>(ctx, *S, b, o); else run_sensor<double, double, false>(ctx, *S, b, o);
+      if (split) run_sensor<float, float, false>(ctx, *S, be, o); else run_sensor<double, double, false>(ctx, *S, be, o);
     }
     if (rts->status)
       HIPCHECK(hipMemcpyAsync(rts->status, ctx->wstatus.p, M * sizeof(int), hipMemcpyDeviceToDevice, ctx->stream));
@@ -841,6 +881,7 @@ int crtm_init(int n_sensors, const char* const* sensor_ids, const char* file_pat
       ctx->sensors.push_back(std::move(S));
     }
     ctx->n_channels = off;
+    upload(ctx->clim, std::vector<double>(CLIM_TABLE, CLIM_TABLE + CLIM_SIZE));
     HIPCHECK(hipDeviceSynchronize());
   } catch (const std::exception& e) {
     std::fprintf(stderr, "CRTM_MI355X crtm_init: %s\n", e.what());

@@ -8,7 +8,10 @@ namespace crtm {
 
 constexpr int WAVE = 64;           // CDNA wavefront width: one channel per lane
 constexpr int KO_MAX = 128;        // ODPS internal layers supported
-constexpr int K_MAX = 200;         // MAX_N_LAYERS (src/CRTM_Parameters.f90:64)
+constexpr int MAX_N_LAYERS = 200;  // user layers (src/CRTM_Parameters.f90:64)
+constexpr int N_MODEL_LEVELS = 101; // climatology levels (src/Atmosphere/CRTM_Model_Profiles.f90)
+constexpr int N_EXTRA_MAX = N_MODEL_LEVELS - 1;   // layers AddLayers can put on top
+constexpr int K_MAX = MAX_N_LAYERS + N_EXTRA_MAX; // layers of an extended profile
 constexpr int NZ_MAX = 2048;       // LayerAvg weights stored per profile (all ODPS layers)
 constexpr int NA_MAX = 6;          // user absorbers carried through the adjoint
 constexpr int MAX_OPTRAN_ORDER = 10;
@@ -129,7 +132,33 @@ struct DevBatch {
   const double* emissivity; // [M][Ltot]
   int Ltot;
   int jmap[NA_MAX];         // ODPS absorber j -> user absorber index (or -1)
+  // AddLayers: profile arrays above describe the extended profiles; the
+  // Jacobian outputs keep the caller's layout
+  const int* climatology;   // [M] or null (input of the extension)
+  const int* n_extra;       // [M] layers added on top of profile m
+  const int64_t* out_loff;  // [M+1] caller's layer offsets (Jacobian rows)
+  int64_t out_NT;           // caller's total layers (absorber Jacobian stride)
 };
+
+// Extended profiles written by the AddLayers kernel: profile m occupies
+// layers [loff[m], loff[m] + n_layers[m]) with loff[m] = caller's offset +
+// m * N_EXTRA_MAX (levels: loff[m] + m), the added layers first.
+struct ExtBatch {
+  int* n_layers;      // [M]
+  int64_t* loff;      // [M+1]
+  int* n_extra;       // [M]
+  double* level_p;    // [NT + M]
+  double* p;          // [NT]
+  double* t;          // [NT]
+  double* a;          // [na][NT]
+  int64_t NT;         // caller's total layers + M * N_EXTRA_MAX
+};
+
+// Model climatology table on the device: level pressure [101], temperature
+// [6][101], absorbers [6 models][6 ids: H2O, CO2, O3, N2O, CO, CH4][101]
+constexpr int CLIM_T0 = N_MODEL_LEVELS;
+constexpr int CLIM_A0 = CLIM_T0 + 6 * N_MODEL_LEVELS;
+constexpr int CLIM_SIZE = CLIM_A0 + 36 * N_MODEL_LEVELS;
 
 // Per-profile work produced by the preparation kernel, precision RT
 template <class RT> struct ProfWork {

@@ -36,6 +36,95 @@ constexpr int PREP_THREADS = 128;
 // Status codes written per profile
 enum { ST_OK = 0, ST_LAYERAVG = 1, ST_NZ = 2, ST_LAYERS = 3, ST_BRACKET = 4 };
 
+// ---------------------------------------------------------------------------
+// CRTM_Atmosphere_AddLayers (src/Atmosphere/CRTM_Atmosphere.f90:248-400):
+// a profile whose top level is below TOA_PRESSURE gets the model
+// climatology's layers above it (Extra_Layers :1043-1050, Interp_LPoly /
+// Interp_Linear :1070-1084, Layer_P / Layer_X :1054-1066), shifted so the
+// lowest added layer continues the profile's top two layers (Shift_Profile
+// :1088-1098), and negative absorber amounts raised to
+// MINIMUM_ABSORBER_AMOUNT.  One 64-thread workgroup per profile: lane 0
+// builds the added layers, all lanes copy the extended profile.
+// ---------------------------------------------------------------------------
+constexpr double TOA_PRESSURE = 0.005;   // src/CRTM_Parameters.f90:157
+
+__global__ void __launch_bounds__(64) add_layers_kernel(DevBatch in, ExtBatch e, const double* clim,
+                                                        int ids0, int ids1, int ids2, int ids3, int ids4, int ids5) {
+  const int m = blockIdx.x, tid = threadIdx.x;
+  const int ids[NA_MAX] = {ids0, ids1, ids2, ids3, ids4, ids5};
+  const int K = in.n_layers[m];
+  const int64_t lo = in.loff[m];
+  const double* lp = in.level_p + lo + m;
+  const int na = in.na;
+  __shared__ double xlp[N_EXTRA_MAX + 1], xp[N_EXTRA_MAX], xt[N_EXTRA_MAX], xa[NA_MAX][N_EXTRA_MAX];
+  __shared__ int sn;
+  if (tid == 0) {
+    int n = 0;
+    const double ptop = lp[0];
+    if (ptop > TOA_PRESSURE) {
+      n = -1;                                   // no model level below the top: reference FAILURE
+      for (int q = 1; q < N_MODEL_LEVELS; ++q)
+        if (clim[q] >= ptop) {
+          n = q;
+          break;
+        }
+      for (int j = 0; j < na; ++j)
+        if (ids[j] < 1 || ids[j] > 6) n = -1;   // no climatology for this absorber
+      if (K < 2) n = -1;                        // Shift_Profile needs two layers
+    }
+    if (n > 0) {
+      const int c = in.climatology ? in.climatology[m] : 6;
+      const int model = (c >= 1 && c <= 6) ? c - 1 : 5;
+      const double* P = clim;
+      const double* T = clim + CLIM_T0 + model * N_MODEL_LEVELS;
+      const double lpoly = (log(ptop) - log(P[n - 1])) / (log(P[n]) - log(P[n - 1]));
+      for (int q = 0; q < n; ++q) xlp[q] = P[q];
+      xlp[n] = ptop;
+      auto lev = [&](const double* X, int q) { return q < n ? X[q] : (X[n] - X[n - 1]) * lpoly + X[n - 1]; };
+      for (int k = 0; k < n; ++k) {
+        xp[k] = (xlp[k + 1] - xlp[k]) / log(xlp[k + 1] / xlp[k]);
+        xt[k] = 0.5 * (lev(T, k) + lev(T, k + 1));
+      }
+      const double elpoly = (log(xp[n - 1]) - log(in.p[lo])) / (log(in.p[lo + 1]) - log(in.p[lo]));
+      auto shift = [&](double* x, const double* xu) {
+        const double dx = (xu[1] - xu[0]) * elpoly + xu[0] - x[n - 1];
+        for (int k = 0; k < n; ++k) x[k] += dx;
+      };
+      shift(xt, in.t + lo);
+      for (int j = 0; j < na; ++j) {
+        const double* A = clim + CLIM_A0 + (model * 6 + ids[j] - 1) * N_MODEL_LEVELS;
+        for (int k = 0; k < n; ++k) xa[j][k] = 0.5 * (lev(A, k) + lev(A, k + 1));
+        shift(xa[j], in.a + (size_t)j * in.NT + lo);
+        for (int k = 0; k < n; ++k)
+          if (xa[j][k] < 0.0) xa[j][k] = MIN_ABSORBER;
+      }
+    }
+    sn = n;
+  }
+  __syncthreads();
+  const int n = sn < 0 ? 0 : sn;
+  const int64_t eo = lo + (int64_t)m * N_EXTRA_MAX;
+  if (tid == 0) {
+    e.n_layers[m] = sn < 0 ? 0 : K + n;         // 0 layers: the preparation kernel fails the profile
+    e.n_extra[m] = n;
+    e.loff[m] = eo;
+    if (m == in.M - 1) e.loff[in.M] = in.loff[in.M] + (int64_t)in.M * N_EXTRA_MAX;
+  }
+  const int Kt = K + n;
+  for (int q = tid; q <= Kt; q += 64) e.level_p[eo + m + q] = q <= n ? (q < n ? xlp[q] : lp[0]) : lp[q - n];
+  for (int q = tid; q < Kt; q += 64) {
+    const bool add = q < n;
+    e.p[eo + q] = add ? xp[q] : in.p[lo + q - n];
+    e.t[eo + q] = add ? xt[q] : in.t[lo + q - n];
+    for (int j = 0; j < na; ++j) e.a[(size_t)j * e.NT + eo + q] = add ? xa[j][q] : in.a[(size_t)j * in.NT + lo + q - n];
+  }
+}
+
+void launch_add_layers(const DevBatch& b, const ExtBatch& e, const double* clim, const int* ids, hipStream_t stream) {
+  hipLaunchKernelGGL(add_layers_kernel, dim3(b.M), dim3(64), 0, stream, b, e, clim, ids[0], ids[1], ids[2], ids[3],
+                     ids[4], ids[5]);
+}
+
 template <int G, class RT>
 __global__ void __launch_bounds__(PREP_THREADS) odps_prep_kernel(DevSensor s, DevBatch bt, ProfWork<RT> w) {
   using T = GT<G>;

@@ -42,6 +42,7 @@ MODULE CRTM_MI355X_Module
 
   TYPE, PUBLIC :: CRTM_Atmosphere_type
     INTEGER :: n_Layers = 0, n_Absorbers = 0, n_Clouds = 0, n_Aerosols = 0
+    INTEGER :: Climatology = 6                     ! US_STANDARD_ATMOSPHERE
     INTEGER, ALLOCATABLE :: Absorber_ID(:), Absorber_Units(:)
     REAL(fp), ALLOCATABLE :: Level_Pressure(:)     ! (0:n_Layers)
     REAL(fp), ALLOCATABLE :: Pressure(:), Temperature(:)
@@ -79,6 +80,7 @@ MODULE CRTM_MI355X_Module
     TYPE(C_PTR) :: n_layers, layer_offset
     INTEGER(C_INT) :: n_absorbers
     TYPE(C_PTR) :: absorber_id, absorber_units, level_pressure, pressure, temperature, absorber
+    TYPE(C_PTR) :: climatology
   END TYPE
   TYPE, BIND(C) :: c_surface
     TYPE(C_PTR) :: land_coverage, water_coverage, snow_coverage, ice_coverage, &
@@ -151,7 +153,7 @@ MODULE CRTM_MI355X_Module
 
   ! Flat host arrays of one call (kept alive while the C-ABI reads them)
   TYPE :: batch_type
-    INTEGER(C_INT), ALLOCATABLE :: nl(:), ids(:), units(:)
+    INTEGER(C_INT), ALLOCATABLE :: nl(:), ids(:), units(:), clim(:)
     INTEGER(C_INT64_T), ALLOCATABLE :: off(:)
     REAL(C_DOUBLE), ALLOCATABLE :: lp(:), p(:), t(:), a(:), cov(:, :), st(:, :), zen(:), scan(:), src(:), emis(:)
     REAL(C_DOUBLE), ALLOCATABLE :: rad(:), bt(:), up(:), dn(:), sp(:), se(:)
@@ -361,11 +363,12 @@ CONTAINS
     IF (.NOT. PRESENT(Opt)) RETURN                  ! emissivity is required by this path
     IF (SIZE(Opt) /= M) RETURN
     na = Atm(1)%n_Absorbers
-    ALLOCATE(b%nl(M), b%off(M + 1))
+    ALLOCATE(b%nl(M), b%off(M + 1), b%clim(M))
     b%off(1) = 0
     DO mm = 1, M
       IF (Atm(mm)%n_Absorbers /= na) RETURN
       b%nl(mm) = Atm(mm)%n_Layers
+      b%clim(mm) = Atm(mm)%Climatology
       b%off(mm + 1) = b%off(mm) + Atm(mm)%n_Layers
     END DO
     NT = INT(b%off(M + 1))
@@ -405,6 +408,7 @@ CONTAINS
     b%atm%pressure = C_LOC(b%p)
     b%atm%temperature = C_LOC(b%t)
     b%atm%absorber = C_LOC(b%a)
+    b%atm%climatology = C_LOC(b%clim)
     b%sfc%land_coverage = C_LOC(b%cov(1, 1))
     b%sfc%water_coverage = C_LOC(b%cov(1, 2))
     b%sfc%snow_coverage = C_LOC(b%cov(1, 3))

@@ -414,6 +414,7 @@ class ProfileSet:
     aerosol_type: Optional[np.ndarray] = None        # int32 [M, na]
     aerosol_concentration: Optional[np.ndarray] = None
     aerosol_effective_radius: Optional[np.ndarray] = None
+    climatology: Optional[np.ndarray] = None         # int32 [M], 1..6 (None: US standard)
 
     @property
     def n_profiles(self) -> int:
@@ -426,7 +427,11 @@ class ProfileSet:
 
 
 def _log_levels(K):
-    return np.exp(np.log(0.005) + np.arange(K + 1) / K * (np.log(1100.0) - np.log(0.005)))
+    lp = np.exp(np.log(0.005) + np.arange(K + 1) / K * (np.log(1100.0) - np.log(0.005)))
+    # exact end points: the top must be TOA_PRESSURE itself, not a rounding
+    # above it (which would make CRTM add a vanishing climatology layer)
+    lp[0], lp[-1] = 0.005, 1100.0
+    return lp
 
 
 def _gp_factor(lay_p, t_clim, corr_km=1.0):

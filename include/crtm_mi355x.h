@@ -71,6 +71,9 @@ typedef struct {
   const double* pressure;          /* hPa, [layer_offset[M]] */
   const double* temperature;       /* K */
   const double* absorber;          /* [n_absorbers][layer_offset[M]] */
+  const int* climatology;          /* [M] model climatology 1..6 used to extend a profile
+                                      whose top is below TOA_PRESSURE (CRTM_Atmosphere_AddLayers,
+                                      src/Atmosphere/CRTM_Atmosphere.f90:248-400); NULL = US standard */
 } crtm_atmosphere;
 
 /* ---- Surface (src/Surface/CRTM_Surface_Define.f90:204-244): [M] each ---- */

@@ -21,6 +21,7 @@
 #include <cmath>
 #include <complex>
 #include <cstring>
+#include <stdexcept>
 #include <vector>
 #ifdef _OPENMP
 #include <omp.h>
@@ -630,10 +631,135 @@ double planck_r_ad_factor(const or_sensor* s, int l, double t) {
 }  // namespace
 
 // ===========================================================================
+// CRTM_Atmosphere_AddLayers (src/Atmosphere/CRTM_Atmosphere.f90:248-400)
+// ===========================================================================
+namespace {
+const double TOA_PRESSURE = 0.005;             // src/CRTM_Parameters.f90:157
+const double MINIMUM_ABSORBER_AMOUNT = 1.0e-76; // src/CRTM_Parameters.f90:140
+const int N_CLIM_LEVELS = 101;
+struct Climatology {
+  bool set = false;
+  double p[N_CLIM_LEVELS];
+  double t[6][N_CLIM_LEVELS];
+  double a[6][6][N_CLIM_LEVELS];   // [model][absorber id - 1][level]
+} g_clim;
+
+// Profile extended to TOA_PRESSURE with climatology layers on top; owns its arrays.
+struct ExtProfile {
+  or_profile p;
+  int n = 0;                       // layers added on top
+  std::vector<double> lp, pr, t, a;
+};
+
+// Extra_Layers: first model level (counting from index 1 of the 0-based
+// level array, as the reference's loop does) at or below the profile top.
+int extra_layers(double ptop) {
+  for (int n = 1; n < N_CLIM_LEVELS; ++n)
+    if (g_clim.p[n] >= ptop) return n;
+  return 0;
+}
+
+void extend_profile(const or_profile* in, ExtProfile& e) {
+  e.p = *in;
+  e.n = 0;
+  const int K = in->n_layers, na = in->n_absorbers;
+  if (in->level_pressure[0] <= TOA_PRESSURE) return;
+  if (!g_clim.set) throw std::runtime_error("oracle_set_climatology not called");
+  const int n = extra_layers(in->level_pressure[0]);
+  if (n < 1) throw std::runtime_error("Error determining extra layer count");
+  const int model = (in->climatology >= 1 && in->climatology <= 6) ? in->climatology - 1 : 5;
+  // model levels 0..n, level n replaced by the profile top (Interp_LPoly / Interp_Linear)
+  std::vector<double> pl(n + 1), tl(n + 1), al((size_t)na * (n + 1));
+  for (int k = 0; k <= n; ++k) {
+    pl[k] = g_clim.p[k];
+    tl[k] = g_clim.t[model][k];
+    for (int j = 0; j < na; ++j) al[(size_t)j * (n + 1) + k] = g_clim.a[model][in->absorber_id[j] - 1][k];
+  }
+  const double ptop = in->level_pressure[0];
+  const double lpoly = (std::log(ptop) - std::log(pl[n - 1])) / (std::log(pl[n]) - std::log(pl[n - 1]));
+  pl[n] = ptop;
+  tl[n] = (tl[n] - tl[n - 1]) * lpoly + tl[n - 1];
+  for (int j = 0; j < na; ++j) {
+    double* x = &al[(size_t)j * (n + 1)];
+    x[n] = (x[n] - x[n - 1]) * lpoly + x[n - 1];
+  }
+  // layer means (Layer_P: log-mean pressure; Layer_X: arithmetic mean)
+  std::vector<double> pn(n), tn(n), an((size_t)na * n);
+  for (int k = 1; k <= n; ++k) {
+    pn[k - 1] = (pl[k] - pl[k - 1]) / std::log(pl[k] / pl[k - 1]);
+    tn[k - 1] = 0.5 * (tl[k - 1] + tl[k]);
+    for (int j = 0; j < na; ++j) an[(size_t)j * n + k - 1] = 0.5 * (al[(size_t)j * (n + 1) + k - 1] + al[(size_t)j * (n + 1) + k]);
+  }
+  // Shift_Profile: move the added layers so the lowest one continues the
+  // profile's top two layers (extrapolated in log pressure)
+  const double elpoly = (std::log(pn[n - 1]) - std::log(in->pressure[0])) / (std::log(in->pressure[1]) - std::log(in->pressure[0]));
+  auto shift = [&](double* x, const double* xtoa) {
+    const double xint = (xtoa[1] - xtoa[0]) * elpoly + xtoa[0];
+    const double dx = xint - x[n - 1];
+    for (int k = 0; k < n; ++k) x[k] += dx;
+  };
+  shift(tn.data(), in->temperature);
+  for (int j = 0; j < na; ++j) {
+    shift(&an[(size_t)j * n], in->absorber + (size_t)j * K);
+    for (int k = 0; k < n; ++k)
+      if (an[(size_t)j * n + k] < 0.0) an[(size_t)j * n + k] = MINIMUM_ABSORBER_AMOUNT;
+  }
+  const int Kt = K + n;
+  e.n = n;
+  e.lp.resize(Kt + 1);
+  e.pr.resize(Kt);
+  e.t.resize(Kt);
+  e.a.resize((size_t)na * Kt);
+  for (int k = 0; k <= n; ++k) e.lp[k] = pl[k];
+  for (int k = 1; k <= K; ++k) e.lp[n + k] = in->level_pressure[k];
+  for (int k = 0; k < n; ++k) {
+    e.pr[k] = pn[k];
+    e.t[k] = tn[k];
+  }
+  for (int k = 0; k < K; ++k) {
+    e.pr[n + k] = in->pressure[k];
+    e.t[n + k] = in->temperature[k];
+  }
+  for (int j = 0; j < na; ++j) {
+    for (int k = 0; k < n; ++k) e.a[(size_t)j * Kt + k] = an[(size_t)j * n + k];
+    for (int k = 0; k < K; ++k) e.a[(size_t)j * Kt + n + k] = in->absorber[(size_t)j * K + k];
+  }
+  e.p.n_layers = Kt;
+  e.p.level_pressure = e.lp.data();
+  e.p.pressure = e.pr.data();
+  e.p.temperature = e.t.data();
+  e.p.absorber = e.a.data();
+}
+
+// Outputs on the extended profile, then the user layers' part copied out
+// (CRTM_Atmosphere_AddLayers_AD, CRTM_Atmosphere.f90:569-659, keeps only the
+// sensitivities of the original layers).
+struct ExtOut {
+  or_out o;
+  std::vector<double> tau, t_k, abs_k;
+  void init(const or_out* user, int L, int Kt, int na) {
+    o = *user;
+    if (user->tau) { tau.assign((size_t)L * Kt, 0.0); o.tau = tau.data(); }
+    if (user->t_k) { t_k.assign((size_t)L * Kt, 0.0); o.t_k = t_k.data(); }
+    if (user->abs_k) { abs_k.assign((size_t)L * na * Kt, 0.0); o.abs_k = abs_k.data(); }
+  }
+  void copy_back(const or_out* user, int L, int K, int n, int na) const {
+    const int Kt = K + n;
+    for (int l = 0; l < L; ++l) {
+      if (user->tau) std::memcpy(user->tau + (size_t)l * K, &tau[(size_t)l * Kt + n], sizeof(double) * K);
+      if (user->t_k) std::memcpy(user->t_k + (size_t)l * K, &t_k[(size_t)l * Kt + n], sizeof(double) * K);
+      if (user->abs_k)
+        for (int j = 0; j < na; ++j)
+          std::memcpy(user->abs_k + ((size_t)l * na + j) * K, &abs_k[((size_t)l * na + j) * Kt + n], sizeof(double) * K);
+    }
+  }
+};
+}  // namespace
+
+// ===========================================================================
 // C API
 // ===========================================================================
-extern "C" int oracle_forward(const or_sensor* s, const or_profile* p, const double* emissivity,
-                              or_out* o) {
+static int forward_ext(const or_sensor* s, const or_profile* p, const double* emissivity, or_out* o) {
   const int K = p->n_layers, L = s->n_channels;
   ProfileState<double> st;
   map_and_predict<double>(s, p, p->temperature, p->absorber, st);
@@ -650,11 +776,10 @@ extern "C" int oracle_forward(const or_sensor* s, const or_profile* p, const dou
   return 0;
 }
 
-extern "C" int oracle_k_matrix_cstep(const or_sensor* s, const or_profile* p,
-                                     const double* emissivity, const double* bt_k,
-                                     const double* rad_k, or_out* o) {
+static int cstep_ext(const or_sensor* s, const or_profile* p, const double* emissivity, const double* bt_k,
+                     const double* rad_k, or_out* o) {
   const int K = p->n_layers, L = s->n_channels, na = p->n_absorbers;
-  oracle_forward(s, p, emissivity, o);
+  forward_ext(s, p, emissivity, o);
   const double h = 1.0e-30;
   const int ndir = K * (1 + na) + 2;
   std::vector<cplx> tu(K), au((size_t)na * K), eps(L);
@@ -1078,10 +1203,58 @@ void k_matrix_profile(const or_sensor* s, const or_profile* p, const double* emi
 
 }  // namespace
 
+namespace {
+// Run `fn` on the profile extended to TOA_PRESSURE and copy the user layers' part out.
+template <class F>
+int with_added_layers(const or_sensor* s, const or_profile* p, or_out* o, F fn) {
+  ExtProfile e;
+  extend_profile(p, e);
+  if (e.n == 0) return fn(p, o);
+  ExtOut eo;
+  eo.init(o, s->n_channels, e.p.n_layers, p->n_absorbers);
+  int r = fn(&e.p, &eo.o);
+  eo.copy_back(o, s->n_channels, p->n_layers, e.n, p->n_absorbers);
+  return r;
+}
+}  // namespace
+
+extern "C" void oracle_set_climatology(const double* p, const double* t, const double* a) {
+  std::memcpy(g_clim.p, p, sizeof(g_clim.p));
+  std::memcpy(g_clim.t, t, sizeof(g_clim.t));
+  std::memcpy(g_clim.a, a, sizeof(g_clim.a));
+  g_clim.set = true;
+}
+
+extern "C" int oracle_add_layers(const or_profile* p, double* level_pressure, double* pressure,
+                                 double* temperature, double* absorber) {
+  ExtProfile e;
+  extend_profile(p, e);
+  if (level_pressure) {
+    const int Kt = e.p.n_layers;
+    std::memcpy(level_pressure, e.p.level_pressure, sizeof(double) * (Kt + 1));
+    std::memcpy(pressure, e.p.pressure, sizeof(double) * Kt);
+    std::memcpy(temperature, e.p.temperature, sizeof(double) * Kt);
+    std::memcpy(absorber, e.p.absorber, sizeof(double) * Kt * p->n_absorbers);
+  }
+  return e.n;
+}
+
+extern "C" int oracle_forward(const or_sensor* s, const or_profile* p, const double* emissivity, or_out* o) {
+  return with_added_layers(s, p, o, [&](const or_profile* q, or_out* oo) { return forward_ext(s, q, emissivity, oo); });
+}
+
+extern "C" int oracle_k_matrix_cstep(const or_sensor* s, const or_profile* p, const double* emissivity,
+                                     const double* bt_k, const double* rad_k, or_out* o) {
+  return with_added_layers(s, p, o,
+                           [&](const or_profile* q, or_out* oo) { return cstep_ext(s, q, emissivity, bt_k, rad_k, oo); });
+}
+
 extern "C" int oracle_k_matrix(const or_sensor* s, const or_profile* p, const double* emissivity,
                                const double* bt_k, const double* rad_k, or_out* o) {
-  k_matrix_profile(s, p, emissivity, bt_k, rad_k, o);
-  return 0;
+  return with_added_layers(s, p, o, [&](const or_profile* q, or_out* oo) {
+    k_matrix_profile(s, q, emissivity, bt_k, rad_k, oo);
+    return 0;
+  });
 }
 
 extern "C" int oracle_k_matrix_batch(const or_sensor* s, int n_profiles, const or_profile* p,
@@ -1105,7 +1278,7 @@ extern "C" int oracle_k_matrix_batch(const or_sensor* s, int n_profiles, const o
     if (o->sfc_t_k) om.sfc_t_k = o->sfc_t_k + (size_t)m * L * 4;
     if (o->emis_k) om.emis_k = o->emis_k + (size_t)m * L;
     if (o->radiance_k) om.radiance_k = o->radiance_k + (size_t)m * L;
-    k_matrix_profile(s, &p[m], emissivity + (size_t)m * L, bt_k + (size_t)m * L, rad_k + (size_t)m * L, &om);
+    oracle_k_matrix(s, &p[m], emissivity + (size_t)m * L, bt_k + (size_t)m * L, rad_k + (size_t)m * L, &om);
   }
   return 0;
 }

@@ -54,6 +54,7 @@ typedef struct {
   double coverage[4];           /* land, water, snow, ice */
   double sfc_temperature[4];
   double sensor_zenith, sensor_scan, surface_altitude;
+  int climatology;              /* 1..6 (src/Atmosphere/CRTM_Atmosphere_Define.f90:406-411), 0 = US standard */
 } or_profile;
 
 typedef struct {
@@ -88,6 +89,19 @@ int oracle_k_matrix_cstep(const or_sensor* s, const or_profile* p, const double*
 int oracle_k_matrix_batch(const or_sensor* s, int n_profiles, const or_profile* p,
                           const double* emissivity, const double* bt_k, const double* rad_k,
                           or_out* o, int n_threads);
+
+/* The six model climatologies on the 101 model levels
+ * (src/Atmosphere/CRTM_Model_Profiles.f90): p[101] hPa, t[6][101] K,
+ * a[6 models][6 absorbers in id order H2O,CO2,O3,N2O,CO,CH4][101].  Must be
+ * set before profiles whose top is below TOA_PRESSURE are computed. */
+void oracle_set_climatology(const double* p, const double* t, const double* a);
+
+/* CRTM_Atmosphere_AddLayers (src/Atmosphere/CRTM_Atmosphere.f90:248-400):
+ * number of layers added above the profile (0 when its top is at or above
+ * TOA_PRESSURE); when the out arrays are non-NULL (sized K+n+1 / K+n /
+ * nabs*(K+n)) the extended profile is written to them. */
+int oracle_add_layers(const or_profile* p, double* level_pressure, double* pressure, double* temperature,
+                      double* absorber);
 
 /* Intermediate quantities for white-box tests. */
 int oracle_layer_avg(int n_odps, const double* ref_lnp, int n_user, const double* user_lnp,

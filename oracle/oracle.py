@@ -39,7 +39,7 @@ class OrProfile(ctypes.Structure):
                 ("level_pressure", _dp), ("pressure", _dp), ("temperature", _dp), ("absorber", _dp),
                 ("coverage", ctypes.c_double * 4), ("sfc_temperature", ctypes.c_double * 4),
                 ("sensor_zenith", ctypes.c_double), ("sensor_scan", ctypes.c_double),
-                ("surface_altitude", ctypes.c_double)]
+                ("surface_altitude", ctypes.c_double), ("climatology", ctypes.c_int)]
 
 
 class OrOut(ctypes.Structure):
@@ -69,7 +69,23 @@ def lib():
         _lib.oracle_k_matrix_batch.argtypes = [ctypes.POINTER(OrSensor), ctypes.c_int,
                                                ctypes.POINTER(OrProfile), _dp, _dp, _dp,
                                                ctypes.POINTER(OrOut), ctypes.c_int]
+        _lib.oracle_set_climatology.argtypes = [_dp, _dp, _dp]
+        _lib.oracle_add_layers.argtypes = [ctypes.POINTER(OrProfile), _dp, _dp, _dp, _dp]
+        _set_climatology(_lib)
     return _lib
+
+
+def _set_climatology(L):
+    """Model climatologies (extracted from src/Atmosphere/CRTM_Model_Profiles.f90)
+    in absorber-id order H2O, CO2, O3, N2O, CO, CH4."""
+    import json
+    d = json.load(open(os.path.join(os.path.dirname(HERE), "crtmv3_amd", "data", "model_profiles.json")))
+    order = [d["absorbers"].index(n) for n in ("H2O", "CO2", "O3", "N2O", "CO", "CH4")]
+    p = np.ascontiguousarray(d["level_pressure"], np.float64)
+    t = np.ascontiguousarray(d["temperature"], np.float64)
+    a = np.ascontiguousarray(np.asarray(d["absorber"], np.float64)[:, order, :])
+    _set_climatology.keep = (p, t, a)
+    L.oracle_set_climatology(_p(p, _dp), _p(t, _dp), _p(a, _dp))
 
 
 def _p(a, t):
@@ -138,6 +154,8 @@ def make_profile(ps, m, keep):
     p.sensor_zenith = ps.sensor_zenith[m]
     p.sensor_scan = ps.sensor_scan[m]
     p.surface_altitude = 0.0
+    clim = getattr(ps, "climatology", None)
+    p.climatology = int(clim[m]) if clim is not None else 0
     return p
 
 

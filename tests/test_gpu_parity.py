@@ -140,3 +140,44 @@ def test_invalid_inputs_are_reported(contexts):
     # the context stays usable after an error
     ok = ctx.forward(synthetic.make_profiles(2, 100, seed=1), eps)
     assert np.all(np.isfinite(ok["brightness_temperature"]))
+
+
+@pytest.mark.parametrize("prec", [0, 1], ids=["fp64", "split"])
+def test_reference_regression_profiles_with_added_layers(gpu, prec):
+    """The reference regression test's two 92-layer profiles (top at
+    0.714 hPa, so AddLayers extends them with climatology; mixed surface
+    types) against the golden fixture."""
+    import json
+    import os
+    import sys
+    from conftest import ROOT
+    from crtmv3_amd import api, synthetic
+    gold = os.path.join(ROOT, "tests", "golden")
+    sys.path.insert(0, gold)
+    import make_golden
+    doc = json.load(open(os.path.join(gold, "regression_inputs.json")))
+    exp = np.load(os.path.join(gold, "regression_expected.npz"), allow_pickle=False)
+    ps = make_golden.profile_set(doc["profiles"])
+    d = synthetic.default_coeff_dir()
+    tol = TOL[prec]
+    for name in make_golden.SENSORS:
+        synthetic.write_sensor(d, name)
+        assert make_golden.fingerprint(d, name) == doc["coefficients"][name], "regenerate the golden fixture"
+        ctx = api.crtm_init([name], d)
+        L = ctx.n_channels
+        eps = np.tile(make_golden.emissivity(ctx.channel_info[0].sensor_type, L), (2, 1))
+        print(f"case golden {name} prec={prec}", flush=True)
+        rts, kd = ctx.k_matrix(ps, eps, precision=prec, jacobian_f32=bool(prec))
+        ctx.destroy()
+        for m in range(2):
+            bt_err = np.abs(rts["brightness_temperature"][m] - exp[f"{name}/{m}/bt"]).max()
+            assert bt_err <= tol["bt"], (name, m, bt_err)
+            a, b = 92 * m, 92 * (m + 1)
+            for got, key in ((np.asarray(kd["temperature_k"][a:b], np.float64).T, "t_k"),
+                             (np.asarray(kd["absorber_k"][:, a:b, :], np.float64).transpose(2, 0, 1), "abs_k")):
+                want = exp[f"{name}/{m}/{key}"]
+                err = np.abs(got - want).max() / np.abs(want).max()
+                assert err <= tol["jac"], (name, m, key, err)
+            sfc = np.stack([kd[f"{c}_temperature_k"][m] for c in ("land", "water", "snow", "ice")], axis=1)
+            want = exp[f"{name}/{m}/sfc_t_k"]
+            assert np.abs(sfc - want).max() <= tol["jac"] * np.abs(want).max()
