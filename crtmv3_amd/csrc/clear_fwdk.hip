@@ -241,6 +241,8 @@ template <> struct KCfg<double> { static constexpr int PW = 8, NBUF = 1, WPE = 0
 constexpr int TABW = (NX_MAX > NU ? NX_MAX : NU) + NO;
 constexpr int RQ = (TABW + WAVE - 1) / WAVE;   // row elements loaded per lane
 constexpr int OC_LDS_BYTES = 16384;            // OPTRAN coefficients staged when they fit
+constexpr int MW = 4;                          // Map_Input_AD register window (ODPS layers)
+constexpr int NAW = 2;                         // user absorbers the window carries
 
 // OPTRAN ln(chi) of one lane at one layer (Add_OPTRAN_wloOD,
 // ODPS_AtmAbsorption.f90:664-727): b_p = sum_q coef(p,q) Ap(q), lnchi = sum_p b_p OX(p)
@@ -276,7 +278,7 @@ clear_fwdk_kernel(DevSensor s, DevBatch bt, ProfWork<RT> w, DevOut o) {
   const int64_t chunk = (total + 7) / 8;
   const int64_t tix = (int64_t)(blockIdx.x & 7) * chunk + (blockIdx.x >> 3);
   if (tix >= total) return;                 // uniform over the workgroup
-  const int b = (int)(tix / ng);
+  const int b = s.blk_order[(int)(tix / ng)];
   const int tid = threadIdx.x;
   const int wv = tid >> 6;
   const int lane = tid & (WAVE - 1);
@@ -503,13 +505,8 @@ clear_fwdk_kernel(DevSensor s, DevBatch bt, ProfWork<RT> w, DevOut o) {
   KT* tko = reinterpret_cast<KT*>(o.t_k) + (bt.out_loff[m] - nx) * Lt + lg;
   KT* ako = reinterpret_cast<KT*>(o.a_k) + (bt.out_loff[m] - nx) * Lt + lg;
   const int64_t astr = bt.out_NT * Lt;
-  // Jacobian pieces kept in private memory until the sweep ends, so each
-  // output element is written exactly once: Planck part per user layer, and
-  // the ODPS-layer adjoints of T and of the user absorbers that Map_Input_AD
-  // spreads over the user layers.
-  RT plk[K_MAX];
-  RT tadk[KO_MAX];
-  RT aadk[NA_MAX][KO_MAX];
+  // The Emission_AD pass stores each user layer's Planck part of T_K (and
+  // zero absorber K) in the output rows; Map_Input_AD then completes them.
   RT carry = RT(0);
   RT up_next = ok_prof ? upath(K) : RT(0);   // path at user level iu_ of the next step
   auto ustep = [&](int iu_) {
@@ -532,7 +529,10 @@ clear_fwdk_kernel(DevSensor s, DevBatch bt, ProfWork<RT> w, DevOut o) {
       dn *= t;
       ru = ru * t + B * (RT(1) - t);
       const RT dbdt = c2 * bc2 * B * (B + c1) * ic1 / (te * te);
-      plk[k] = dbdt * b_ad;
+      if (valid && k >= nx) {
+        tko[(int64_t)k * Lt] = (KT)(dbdt * b_ad);
+        for (int ja = 0; ja < na; ++ja) ako[ja * astr + (int64_t)k * Lt] = KT(0);
+      }
       const RT gq = tau_ad * isec;
       v = carry + gq;
       carry = -gq;
@@ -562,6 +562,46 @@ clear_fwdk_kernel(DevSensor s, DevBatch bt, ProfWork<RT> w, DevOut o) {
     }
   }
   RT pad_next = ok_prof ? pad[Ko] : RT(0);
+  const int* acc_i = w.acc_i + (size_t)m * Ko * 3;
+  const RT* acc_w = w.acc_w + (size_t)m * NZ_MAX;
+  // Map_Input_AD in the sweep: user layer i is complete once the lowest ODPS
+  // layer whose support holds it has been processed; with at most MW such
+  // layers per user layer, their adjoints sit in a register window.  Wider
+  // supports (very fine user grids) or more absorbers take the gather path.
+  const bool winmap = ok_prof && geo[3] <= (double)MW && na <= NAW;
+  RT tw[MW], aw[MW][NAW];
+#pragma unroll
+  for (int q = 0; q < MW; ++q) {
+    tw[q] = RT(0);
+#pragma unroll
+    for (int ju = 0; ju < NAW; ++ju) aw[q][ju] = RT(0);
+  }
+  auto finish_user_layer = [&](int iu, int k) {   // window slot q holds ODPS layer k + q
+    if (!valid || iu < nx) return;
+    RT st = (RT)tko[(int64_t)iu * Lt];             // Planck part from the Emission_AD pass
+    RT sa[NAW];
+#pragma unroll
+    for (int ju = 0; ju < NAW; ++ju) sa[ju] = RT(0);
+#pragma unroll
+    for (int q = 0; q < MW; ++q) {
+      const int kk = k + q;
+      if (kk >= Ko) break;
+      const int i1 = acc_i[3 * kk], n = acc_i[3 * kk + 1];
+      if (iu < i1 || iu >= i1 + n) continue;
+      const RT wk = acc_w[acc_i[3 * kk + 2] + iu - i1];
+      st += wk * tw[q];
+#pragma unroll
+      for (int ju = 0; ju < NAW; ++ju) sa[ju] += wk * aw[q][ju];
+    }
+    tko[(int64_t)iu * Lt] = (KT)st;
+#pragma unroll
+    for (int ju = 0; ju < NAW; ++ju)
+      if (ju < na) ako[ju * astr + (int64_t)iu * Lt] = (KT)sa[ju];
+  };
+  if (winmap) {   // user layers below every ODPS support take only their Planck part
+    const int ilast = acc_i[3 * (Ko - 1)] + acc_i[3 * (Ko - 1) + 1] - 1;
+    for (int iu = ilast + 1; iu < K; ++iu) finish_user_layer(iu, Ko);
+  }
 
   RT S = RT(0), tzs = RT(0), tzps = RT(0);
   RT gzs[Tg::NABS], gts[Tg::NABS];
@@ -569,8 +609,6 @@ clear_fwdk_kernel(DevSensor s, DevBatch bt, ProfWork<RT> w, DevOut o) {
   for (int j = 0; j < Tg::NABS; ++j) gzs[j] = gts[j] = RT(0);
   RT ivn = RT(0), sta = RT(0), spa = RT(0), xltc = RT(0), xlpc = RT(0);
   const int h2o_j = s.h2o_idx - 1;
-  const int* acc_i = w.acc_i + (size_t)m * Ko * 3;
-  const RT* acc_w = w.acc_w + (size_t)m * NZ_MAX;
   // the forward loop ended on a barrier, so both LDS buffers are free
   cload(Ko - 1);
   rload(Ut, NU, Ko - 1);
@@ -653,11 +691,43 @@ clear_fwdk_kernel(DevSensor s, DevBatch bt, ProfWork<RT> w, DevOut o) {
       t_ad += tzs + pdp * tzps;
       // Map_Input_AD (ODPS_CoordinateMapping.f90:480-547): add onto the user
       // layers averaged into this ODPS layer (their Planck part is already stored)
-      tadk[k] = t_ad;
+      if (winmap) {
 #pragma unroll
-      for (int j = 0; j < Tg::NABS; ++j) {
-        const int ju = bt.jmap[j];
-        if (ju >= 0) aadk[ju][k] = uk[U_MASK0 + j] != RT(0) ? a_ad[j] : RT(0);
+        for (int q = MW - 1; q > 0; --q) {
+          tw[q] = tw[q - 1];
+#pragma unroll
+          for (int ju = 0; ju < NAW; ++ju) aw[q][ju] = aw[q - 1][ju];
+        }
+        tw[0] = t_ad;
+#pragma unroll
+        for (int ju = 0; ju < NAW; ++ju) aw[0][ju] = RT(0);
+#pragma unroll
+        for (int j = 0; j < Tg::NABS; ++j) {
+          const int ju = bt.jmap[j];
+#pragma unroll
+          for (int q = 0; q < NAW; ++q)
+            if (ju == q && uk[U_MASK0 + j] != RT(0)) aw[0][q] = a_ad[j];
+        }
+        // user layers (i2(k-1), i2(k)] are complete
+        const int ihi = acc_i[3 * k] + acc_i[3 * k + 1] - 1;
+        const int ilo = k > 0 ? acc_i[3 * (k - 1)] + acc_i[3 * (k - 1) + 1] : 0;
+        for (int iu = ilo; iu <= ihi; ++iu) finish_user_layer(iu, k);
+      } else if (valid) {
+        // wide supports or more absorbers: add onto the output rows.  Each
+        // lane owns its channel's column, so the adds need no atomicity; the
+        // returnless atomic add retires without a load round trip (same-lane,
+        // same-address operations stay in program order).
+        const int i1 = acc_i[3 * k], n = acc_i[3 * k + 1];
+        const RT* awk = acc_w + acc_i[3 * k + 2];
+        for (int q = 0; q < n; ++q)
+          if (i1 + q >= nx) unsafeAtomicAdd(tko + (int64_t)(i1 + q) * Lt, (KT)(awk[q] * t_ad));
+#pragma unroll
+        for (int j = 0; j < Tg::NABS; ++j) {
+          const int ju = bt.jmap[j];
+          if (ju >= 0 && uk[U_MASK0 + j] != RT(0))
+            for (int q = 0; q < n; ++q)
+              if (i1 + q >= nx) unsafeAtomicAdd(ako + ju * astr + (int64_t)(i1 + q) * Lt, (KT)(awk[q] * a_ad[j]));
+        }
       }
     }
     if constexpr (NBUF == 1) __syncthreads();
@@ -666,42 +736,6 @@ clear_fwdk_kernel(DevSensor s, DevBatch bt, ProfWork<RT> w, DevOut o) {
       rstore((k - 1) % NBUF);
     }
     __syncthreads();
-  }
-  if (!valid) return;
-
-  // ------------- Map_Input_AD (ODPS_CoordinateMapping.f90:480-547) as a gather -------------
-  // User layer i collects w(k,i) * adjoint(k) over the ODPS layers k whose
-  // averaging support [i1(k), i1(k)+n(k)) contains it; the supports move
-  // monotonically with k, so [kmin, kmax] slides down the profile.
-  bool amap[NA_MAX];
-#pragma unroll
-  for (int ju = 0; ju < NA_MAX; ++ju) {
-    amap[ju] = false;
-#pragma unroll
-    for (int j = 0; j < Tg::NABS; ++j) amap[ju] = amap[ju] || bt.jmap[j] == ju;
-  }
-  int kmin = 0, kmax = -1;
-  for (int iu = 0; iu < K; ++iu) {
-    if (iu < nx) continue;
-    while (kmax + 1 < Ko && acc_i[3 * (kmax + 1)] <= iu) ++kmax;
-    while (kmin < Ko && acc_i[3 * kmin] + acc_i[3 * kmin + 1] - 1 < iu) ++kmin;
-    RT st = plk[iu];
-    RT sa[NA_MAX];
-#pragma unroll
-    for (int ju = 0; ju < NA_MAX; ++ju) sa[ju] = RT(0);
-    for (int k = kmin; k <= kmax; ++k) {
-      const int i1 = acc_i[3 * k], n = acc_i[3 * k + 1];
-      if (iu < i1 || iu >= i1 + n) continue;
-      const RT wk = acc_w[acc_i[3 * k + 2] + iu - i1];
-      st += wk * tadk[k];
-#pragma unroll
-      for (int ju = 0; ju < NA_MAX; ++ju)
-        if (amap[ju]) sa[ju] += wk * aadk[ju][k];
-    }
-    tko[(int64_t)iu * Lt] = (KT)st;
-#pragma unroll
-    for (int ju = 0; ju < NA_MAX; ++ju)
-      if (ju < na) ako[ju * astr + (int64_t)iu * Lt] = (KT)sa[ju];
   }
 }
 

@@ -498,6 +498,27 @@ void build_sensor(Sensor& S, int chan_offset) {
   d.oblk_base = S.put(obase);
   d.ocd = S.put(ocd);
   d.lane_optran = S.put(lane_opt);
+  // Workgroup order over blocks: heavy and light blocks alternate, so every
+  // XCD's contiguous range of the order (the kernel's XCD mapping) carries
+  // a similar share of the work.  Cost ~ predictor slots in use, with the
+  // OPTRAN blocks' extra regression and adjoint work.
+  {
+    std::vector<std::pair<double, int>> cost(nb);
+    for (int b = 0; b < nb; ++b) {
+      const int* mt = &meta[(size_t)b * BM_N];
+      double c = 20.0;
+      for (int j = 0; j < J; ++j) c += mt[BM_NP0 + j];
+      if (obase[b] >= 0) c *= 1.5;
+      cost[b] = {c, b};
+    }
+    std::sort(cost.begin(), cost.end(), [](const auto& a, const auto& b) { return a.first > b.first; });
+    std::vector<int> order;
+    for (int lo = 0, hi = nb - 1; lo <= hi; ++lo, --hi) {
+      order.push_back(cost[lo].second);
+      if (lo != hi) order.push_back(cost[hi].second);
+    }
+    d.blk_order = S.put(order);
+  }
   d.c1 = S.put(sc.c1);
   d.c2 = S.put(sc.c2);
   d.bc1 = S.put(sc.bc1);

@@ -101,6 +101,7 @@ struct DevSensor {
   const int64_t* oblk_base; // [n_blocks], -1 when the block has no OPTRAN lane
   const double* ocd;
   const int* lane_optran;   // [n_blocks*64] 1 if the lane's WLO OD comes from OPTRAN
+  const int* blk_order;     // [n_blocks] block processed at each position of the workgroup order
   // per channel (SpcCoeff)
   const double* c1;
   const double* c2;
@@ -169,7 +170,7 @@ template <class RT> struct ProfWork {
   RT* acc_w;      // [M][NZ_MAX]: weights of ODPS layer k at [offset, offset + count)
   int* uidx;      // [M][max_k+1]: lower ODPS level k1 (0-based); k2 = k1+1 unless flagged
   RT* ufrac;      // [M][max_k+1]: interpolation fraction, or -1 when k1 == k2
-  double* geo;    // [M][4]: sec_sfc, cos_sensor, sfc_T, (unused)
+  double* geo;    // [M][4]: sec_sfc, cos_sensor, sfc_T, widest Map_Input span (ODPS layers per user layer)
   int* status;    // [M]
 };
 

@@ -538,11 +538,23 @@ __global__ void __launch_bounds__(PREP_THREADS) odps_prep_kernel(DevSensor s, De
       }
       if (!found) st = ST_BRACKET;
     }
+    // widest run of ODPS layers whose averaging supports share one user
+    // layer: the K-matrix kernel folds Map_Input_AD into its sweep with a
+    // register window when this fits the window
+    int span = 0;
+    {
+      int kmin = 0, kmax = -1;
+      for (int iu = 0; iu < K; ++iu) {
+        while (kmax + 1 < Ko && acc_i1[kmax + 1] <= iu) ++kmax;
+        while (kmin < Ko && acc_i1[kmin] + acc_n[kmin] - 1 < iu) ++kmin;
+        span = max(span, kmax - kmin + 1);
+      }
+    }
     double* g = w.geo + (size_t)m * 4;
     g[0] = geo[0];
     g[1] = geo[1];
     g[2] = geo[2];
-    g[3] = geo[3];
+    g[3] = (double)span;
     w.status[m] = st;
   }
 }

@@ -70,7 +70,7 @@ def test_k_matrix_matches_oracle_uneven_layers(contexts, coeff_dir, oracle_mod, 
     from crtmv3_amd import synthetic
     ctx = contexts[name]
     L = ctx.n_channels
-    ks = [100, 60, 137, 83, 100, 71]
+    ks = [100, 60, 137, 83, 100, 71, 30]     # 30 layers: wide averaging supports
     ps = synthetic.make_profiles(len(ks), ks, seed=101)
     rng = np.random.default_rng(5)
     eps = rng.uniform(0.5, 0.99, (len(ks), L))
@@ -181,3 +181,22 @@ def test_reference_regression_profiles_with_added_layers(gpu, prec):
             sfc = np.stack([kd[f"{c}_temperature_k"][m] for c in ("land", "water", "snow", "ice")], axis=1)
             want = exp[f"{name}/{m}/sfc_t_k"]
             assert np.abs(sfc - want).max() <= tol["jac"] * np.abs(want).max()
+
+
+def test_three_absorbers(contexts, coeff_dir, oracle_mod):
+    """H2O, CO2 and O3 as user absorbers (CO2 Jacobian through the ODPS CO2
+    component) on the group-1 sensor."""
+    from crtmv3_amd import synthetic
+    name = "cris_small_syn"
+    ctx = contexts[name]
+    ps = synthetic.make_profiles(3, [100, 64, 120], seed=41)
+    co2 = np.full((1, ps.absorber.shape[1]), 395.0) * np.random.default_rng(2).uniform(0.97, 1.03, (1, ps.absorber.shape[1]))
+    ps.absorber = np.concatenate([ps.absorber[:1], co2, ps.absorber[1:]])
+    ps.absorber_id = np.array([1, 2, 3], np.int32)
+    ps.absorber_units = np.array([3, 1, 1], np.int32)
+    eps = np.full((3, ctx.n_channels), 0.97)
+    print("case three absorbers", flush=True)
+    ref = _oracle_batch(oracle_mod, coeff_dir, name, ps, eps)
+    for prec in (0, 1):
+        rts, kd = ctx.k_matrix(ps, eps, precision=prec, jacobian_f32=bool(prec))
+        _check(ps, rts, kd, ref, prec, ctx.n_channels)
