@@ -241,8 +241,9 @@ template <> struct KCfg<double> { static constexpr int PW = 8, NBUF = 1, WPE = 0
 constexpr int TABW = (NX_MAX > NU ? NX_MAX : NU) + NO;
 constexpr int RQ = (TABW + WAVE - 1) / WAVE;   // row elements loaded per lane
 constexpr int OC_LDS_BYTES = 16384;            // OPTRAN coefficients staged when they fit
-constexpr int MW = 4;                          // Map_Input_AD register window (ODPS layers)
+constexpr int MW = MAP_WIN;                    // Map_Input_AD register window (ODPS layers)
 constexpr int NAW = 2;                         // user absorbers the window carries
+constexpr int PF = 4;                          // user-layer rows prefetched one ODPS layer ahead
 
 // OPTRAN ln(chi) of one lane at one layer (Add_OPTRAN_wloOD,
 // ODPS_AtmAbsorption.f90:664-727): b_p = sum_q coef(p,q) Ap(q), lnchi = sum_p b_p OX(p)
@@ -576,19 +577,40 @@ clear_fwdk_kernel(DevSensor s, DevBatch bt, ProfWork<RT> w, DevOut o) {
 #pragma unroll
     for (int ju = 0; ju < NAW; ++ju) aw[q][ju] = RT(0);
   }
-  auto finish_user_layer = [&](int iu, int k) {   // window slot q holds ODPS layer k + q
+  // The Planck part of the rows finalized at ODPS layer k is loaded while
+  // layer k+1 is processed (pfc / pfn: current and next layer's rows from
+  // pf_lo / pf_lo_n), so the read does not stall the synchronized waves.
+  RT pfc[PF], pfn[PF];
+  int pf_lo = 0, pf_lo_n = 0;
+  const int* frange = w.frange + (size_t)m * Ko * 2;
+  const RT* uwin = w.uwin + (size_t)m * bt.max_k * MW;
+  auto final_lo = [&](int k) { return frange[2 * k]; };
+  auto prefetch = [&](int k) {   // rows finalized at ODPS layer k, into pfn
+    pf_lo_n = final_lo(k);
+#pragma unroll
+    for (int q = 0; q < PF; ++q) {
+      const int iu = pf_lo_n + q;
+      pfn[q] = (valid && iu >= nx && iu < K) ? (RT)tko[(int64_t)iu * Lt] : RT(0);
+    }
+  };
+  auto finish_user_layer = [&](int iu) {   // completed at the ODPS layer in window slot 0
     if (!valid || iu < nx) return;
-    RT st = (RT)tko[(int64_t)iu * Lt];             // Planck part from the Emission_AD pass
+    RT st;                                         // Planck part from the Emission_AD pass
+    if ((unsigned)(iu - pf_lo) < (unsigned)PF) {
+      st = pfc[0];
+#pragma unroll
+      for (int q = 1; q < PF; ++q)
+        if (iu - pf_lo == q) st = pfc[q];
+    } else {
+      st = (RT)tko[(int64_t)iu * Lt];
+    }
+    const RT* wr = uwin + (size_t)iu * MW;         // weights from ODPS layers in slots 0..MW-1
     RT sa[NAW];
 #pragma unroll
     for (int ju = 0; ju < NAW; ++ju) sa[ju] = RT(0);
 #pragma unroll
     for (int q = 0; q < MW; ++q) {
-      const int kk = k + q;
-      if (kk >= Ko) break;
-      const int i1 = acc_i[3 * kk], n = acc_i[3 * kk + 1];
-      if (iu < i1 || iu >= i1 + n) continue;
-      const RT wk = acc_w[acc_i[3 * kk + 2] + iu - i1];
+      const RT wk = wr[q];
       st += wk * tw[q];
 #pragma unroll
       for (int ju = 0; ju < NAW; ++ju) sa[ju] += wk * aw[q][ju];
@@ -600,7 +622,9 @@ clear_fwdk_kernel(DevSensor s, DevBatch bt, ProfWork<RT> w, DevOut o) {
   };
   if (winmap) {   // user layers below every ODPS support take only their Planck part
     const int ilast = acc_i[3 * (Ko - 1)] + acc_i[3 * (Ko - 1) + 1] - 1;
-    for (int iu = ilast + 1; iu < K; ++iu) finish_user_layer(iu, Ko);
+    pf_lo = K;
+    for (int iu = ilast + 1; iu < K; ++iu) finish_user_layer(iu);   // window empty: Planck part only
+    prefetch(Ko - 1);
   }
 
   RT S = RT(0), tzs = RT(0), tzps = RT(0);
@@ -619,6 +643,12 @@ clear_fwdk_kernel(DevSensor s, DevBatch bt, ProfWork<RT> w, DevOut o) {
     if (k > 0) {
       cload(k - 1);
       rload(Ut, NU, k - 1);
+    }
+    if (winmap) {
+#pragma unroll
+      for (int q = 0; q < PF; ++q) pfc[q] = pfn[q];
+      pf_lo = pf_lo_n;
+      if (k > 0) prefetch(k - 1);
     }
     if (ok_prof) {
       S += pad_next;
@@ -709,9 +739,8 @@ clear_fwdk_kernel(DevSensor s, DevBatch bt, ProfWork<RT> w, DevOut o) {
             if (ju == q && uk[U_MASK0 + j] != RT(0)) aw[0][q] = a_ad[j];
         }
         // user layers (i2(k-1), i2(k)] are complete
-        const int ihi = acc_i[3 * k] + acc_i[3 * k + 1] - 1;
-        const int ilo = k > 0 ? acc_i[3 * (k - 1)] + acc_i[3 * (k - 1) + 1] : 0;
-        for (int iu = ilo; iu <= ihi; ++iu) finish_user_layer(iu, k);
+        const int ihi = frange[2 * k + 1];
+        for (int iu = pf_lo; iu <= ihi; ++iu) finish_user_layer(iu);
       } else if (valid) {
         // wide supports or more absorbers: add onto the output rows.  Each
         // lane owns its channel's column, so the adds need no atomicity; the

@@ -539,7 +539,7 @@ struct crtm_context {
   int n_channels = 0;
   std::string last_error;
   // work pool (grown on demand, reused across calls)
-  DevBuf wx, wu, wo, wacci, waccw, wuidx, wufrac, wgeo, wstatus;
+  DevBuf wx, wu, wo, wacci, waccw, wuidx, wufrac, wgeo, wstatus, wuwin, wfrange;
   DevBuf stage;  // host-mode staging of inputs/outputs
   DevBuf clim;   // model climatology table (AddLayers)
   DevBuf xnl, xloff, xnx, xlp, xp, xt, xa;   // profiles extended by AddLayers
@@ -579,6 +579,8 @@ ProfWork<RT> make_work(crtm_context* c, int M, int Ko, int nx, int max_k) {
   c->wuidx.reserve((size_t)M * (max_k + 1) * sizeof(int));
   c->wufrac.reserve((size_t)M * (max_k + 1) * sizeof(RT));
   c->wgeo.reserve((size_t)M * 4 * sizeof(double));
+  c->wuwin.reserve((size_t)M * max_k * MAP_WIN * sizeof(RT));
+  c->wfrange.reserve((size_t)M * Ko * 2 * sizeof(int));
   c->wstatus.reserve((size_t)M * sizeof(int));
   w.x = c->wx.as<RT>();
   w.u = c->wu.as<RT>();
@@ -588,6 +590,8 @@ ProfWork<RT> make_work(crtm_context* c, int M, int Ko, int nx, int max_k) {
   w.uidx = c->wuidx.as<int>();
   w.ufrac = c->wufrac.as<RT>();
   w.geo = c->wgeo.as<double>();
+  w.uwin = c->wuwin.as<RT>();
+  w.frange = c->wfrange.as<int>();
   w.status = c->wstatus.as<int>();
   return w;
 }

@@ -13,6 +13,7 @@ constexpr int N_MODEL_LEVELS = 101; // climatology levels (src/Atmosphere/CRTM_M
 constexpr int N_EXTRA_MAX = N_MODEL_LEVELS - 1;   // layers AddLayers can put on top
 constexpr int K_MAX = MAX_N_LAYERS + N_EXTRA_MAX; // layers of an extended profile
 constexpr int NZ_MAX = 2048;       // LayerAvg weights stored per profile (all ODPS layers)
+constexpr int MAP_WIN = 4;         // ODPS layers per user layer the K-matrix kernel's window holds
 constexpr int NA_MAX = 6;          // user absorbers carried through the adjoint
 constexpr int MAX_OPTRAN_ORDER = 10;
 constexpr int N_OPTRAN_PRED = 14;
@@ -171,6 +172,8 @@ template <class RT> struct ProfWork {
   int* uidx;      // [M][max_k+1]: lower ODPS level k1 (0-based); k2 = k1+1 unless flagged
   RT* ufrac;      // [M][max_k+1]: interpolation fraction, or -1 when k1 == k2
   double* geo;    // [M][4]: sec_sfc, cos_sensor, sfc_T, widest Map_Input span (ODPS layers per user layer)
+  RT* uwin;       // [M][max_k][MAP_WIN]: Map_Input weights of user layer i from ODPS layers kmin(i)+c
+  int* frange;    // [M][Ko][2]: user layers whose last Map_Input contribution comes from ODPS layer k
   int* status;    // [M]
 };
 

@@ -543,11 +543,30 @@ __global__ void __launch_bounds__(PREP_THREADS) odps_prep_kernel(DevSensor s, De
     // register window when this fits the window
     int span = 0;
     {
+      // user layer iu is complete after ODPS layer kmin(iu), the first whose
+      // support reaches it; its weights from layers kmin(iu)+c, c < MAP_WIN
+      RT* uw = w.uwin + (size_t)m * bt.max_k * MAP_WIN;
       int kmin = 0, kmax = -1;
       for (int iu = 0; iu < K; ++iu) {
         while (kmax + 1 < Ko && acc_i1[kmax + 1] <= iu) ++kmax;
         while (kmin < Ko && acc_i1[kmin] + acc_n[kmin] - 1 < iu) ++kmin;
         span = max(span, kmax - kmin + 1);
+        for (int c = 0; c < MAP_WIN; ++c) {
+          const int kk = kmin + c;
+          double wv = 0.0;
+          if (kk < Ko && kk <= kmax && iu >= acc_i1[kk] && iu < acc_i1[kk] + acc_n[kk])
+            wv = accw[acc_off[kk] + iu - acc_i1[kk]];
+          uw[(size_t)iu * MAP_WIN + c] = (RT)wv;
+        }
+      }
+      // rows completed at ODPS layer k: (i2(k-1), i2(k)], i2 = last user layer of the support
+      int* fr = w.frange + (size_t)m * Ko * 2;
+      for (int k = 0; k < Ko; ++k) {
+        const int i2 = acc_i1[k] + acc_n[k] - 1;
+        const int lo = k > 0 ? acc_i1[k - 1] + acc_n[k - 1] : 0;
+        if (i2 < lo - 1) span = 1 << 20;   // supports not ordered: the kernel takes the general path
+        fr[2 * k] = lo;
+        fr[2 * k + 1] = i2;
       }
     }
     double* g = w.geo + (size_t)m * 4;
