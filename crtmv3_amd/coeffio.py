@@ -16,6 +16,9 @@ Formats:
   * ODPS TauCoeff, release 2
     (src/Coefficients/TauCoeff/ODPS/ODPS_Binary_IO.f90:617-800,
      type at src/Coefficients/TauCoeff/ODPS/ODPS_Define.f90:109-205)
+  * CloudCoeff, release 3
+    (src/Coefficients/CloudCoeff/CloudCoeff_Binary_IO.f90:340-460,
+     type at src/Coefficients/CloudCoeff/CloudCoeff_Define.f90:95-154)
 """
 from __future__ import annotations
 
@@ -33,6 +36,9 @@ ODPS_RELEASE = 2
 ODPS_VERSION = 1
 ODPS_ALGORITHM = 2           # src/CRTM_Parameters.f90:128
 N_PREDICTOR_USED_OPTRAN = 6  # ODPS_Define.f90:103
+
+CLOUDCOEFF_RELEASE = 3
+CLOUDCOEFF_VERSION = 1
 
 MICROWAVE_SENSOR = 1
 INFRARED_SENSOR = 2
@@ -303,3 +309,82 @@ def read_odps(path: str) -> ODPS:
         t.alpha, t.alpha_c1, t.alpha_c2 = float(o[5]), float(o[6]), float(o[7])
         t.ocomponent_index = int(o[8])
     return t
+
+
+# --------------------------------------------------------------------------
+# CloudCoeff
+# --------------------------------------------------------------------------
+@dataclasses.dataclass
+class CloudCoeff:
+    """Mie/TAMU cloud optical property tables.
+
+    Shapes (Fortran order kept, numpy arrays indexed the same way):
+      ke/w/g_l_mw (nF_mw, nR_mw, nT), pcoeff_l_mw (nF_mw, nR_mw, nT, nL+1, nP)
+      ke/w/g_s_mw (nF_mw, nR_mw, nD), pcoeff_s_mw (nF_mw, nR_mw, nD, nL+1, nP)
+      ke/w/g_ir  (nF_ir, nR_ir, nD+1), pcoeff_ir  (nF_ir, nR_ir, nD+1, nL, nP)
+    The backscatter (kb) tables are not part of the file.
+    """
+    frequency_mw: np.ndarray
+    frequency_ir: np.ndarray
+    reff_mw: np.ndarray
+    reff_ir: np.ndarray
+    temperature: np.ndarray
+    density_mw: np.ndarray
+    ke_l_mw: np.ndarray
+    w_l_mw: np.ndarray
+    g_l_mw: np.ndarray
+    pcoeff_l_mw: np.ndarray
+    ke_s_mw: np.ndarray
+    w_s_mw: np.ndarray
+    g_s_mw: np.ndarray
+    pcoeff_s_mw: np.ndarray
+    ke_ir: np.ndarray
+    w_ir: np.ndarray
+    g_ir: np.ndarray
+    pcoeff_ir: np.ndarray
+    release: int = CLOUDCOEFF_RELEASE
+    version: int = CLOUDCOEFF_VERSION
+
+    @property
+    def n_legendre_terms(self) -> int:
+        return int(self.pcoeff_l_mw.shape[3] - 1)
+
+    @property
+    def n_phase_elements(self) -> int:
+        return int(self.pcoeff_l_mw.shape[4])
+
+    @property
+    def n_ir_densities(self) -> int:
+        return int(self.density_mw.size + 1)
+
+
+def write_cloudcoeff(path: str, cc: CloudCoeff) -> None:
+    """Record order of CloudCoeff_Binary_ReadFile (CloudCoeff_Binary_IO.f90:367-446)."""
+    f8 = lambda a: np.asarray(a, np.float64)
+    with RecordWriter(path) as w:
+        w.record(np.int32(cc.release), np.int32(cc.version))
+        w.record(*[np.int32(x) for x in (cc.frequency_mw.size, cc.reff_mw.size, cc.frequency_ir.size,
+                                         cc.reff_ir.size, cc.temperature.size, cc.density_mw.size,
+                                         cc.n_legendre_terms, cc.n_phase_elements)])
+        w.record(f8(cc.frequency_mw), f8(cc.frequency_ir), f8(cc.reff_mw), f8(cc.reff_ir),
+                 f8(cc.temperature), f8(cc.density_mw))
+        w.record(f8(cc.ke_l_mw), f8(cc.w_l_mw), f8(cc.g_l_mw), f8(cc.pcoeff_l_mw))
+        w.record(f8(cc.ke_s_mw), f8(cc.w_s_mw), f8(cc.g_s_mw), f8(cc.pcoeff_s_mw))
+        w.record(f8(cc.ke_ir), f8(cc.w_ir), f8(cc.g_ir), f8(cc.pcoeff_ir))
+
+
+def read_cloudcoeff(path: str) -> CloudCoeff:
+    r = RecordReader(path)
+    rel, ver = r.read([("i4", ()), ("i4", ())])
+    if rel < CLOUDCOEFF_RELEASE:
+        raise ValueError(f"CloudCoeff release {rel} unsupported (need {CLOUDCOEFF_RELEASE})")
+    nfm, nrm, nfi, nri, nt, ndm, nl, npe = [int(x) for x in r.read([("i4", ())] * 8)]
+    ndi = ndm + 1
+    fm, fi, rm, ri, t, dm = r.read([("f8", (nfm,)), ("f8", (nfi,)), ("f8", (nrm,)), ("f8", (nri,)),
+                                    ("f8", (nt,)), ("f8", (ndm,))])
+    lm = r.read([("f8", (nfm, nrm, nt))] * 3 + [("f8", (nfm, nrm, nt, nl + 1, npe))])
+    sm = r.read([("f8", (nfm, nrm, ndm))] * 3 + [("f8", (nfm, nrm, ndm, nl + 1, npe))])
+    ir = r.read([("f8", (nfi, nri, ndi))] * 3 + [("f8", (nfi, nri, ndi, nl, npe))])
+    return CloudCoeff(fm.copy(), fi.copy(), rm.copy(), ri.copy(), t.copy(), dm.copy(),
+                      *[a.copy() for a in lm], *[a.copy() for a in sm], *[a.copy() for a in ir],
+                      release=int(rel), version=int(ver))

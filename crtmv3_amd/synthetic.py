@@ -24,6 +24,12 @@ on pressure levels log-spaced 0.005-1100 hPa, US-standard temperature plus a
 vertically correlated N(0, 5 K) field (1 km correlation length in
 hypsometric height), log-normal water vapour (sigma 0.3 in ln), ozone x
 U(0.8, 1.2), ocean surface at U(270, 305) K, zenith U(0, 60 deg).
+
+Clouds (add_clouds): liquid, ice, rain and snow water contents log-normal
+around 0.05 g/m^3 (sigma 1 in ln) in 0-10 contiguous layers per type, liquid
+radii U(5, 20) um, ice radii U(20, 80) um.  The synthetic CloudCoeff tables
+(make_cloud_coeff) are smooth Rayleigh-to-Mie-like functions of frequency,
+radius and temperature with Henyey-Greenstein Legendre coefficients.
 """
 from __future__ import annotations
 
@@ -496,3 +502,131 @@ def make_profiles(n_profiles: int, n_layers: int | Sequence[int] = 100, seed: in
 
 def default_emissivity(sensor_type: int, n_channels: int) -> np.ndarray:
     return np.full(n_channels, 0.55 if sensor_type == coeffio.MICROWAVE_SENSOR else 0.98)
+
+
+# ----------------------------------------------------------------------------
+# CloudCoeff (Mie/TAMU-layout tables) and cloudy profiles
+# ----------------------------------------------------------------------------
+WATER_CLOUD, ICE_CLOUD, RAIN_CLOUD, SNOW_CLOUD, GRAUPEL_CLOUD, HAIL_CLOUD = 1, 2, 3, 4, 5, 6
+CLOUDCOEFF_FILE = "CloudCoeff.bin"
+# stream sets and their Legendre offsets (src/Coefficients/CloudCoeff/CloudCoeff_Define.f90:83-87)
+STREAM_SETS = ((2, 0), (4, 0), (6, 5), (8, 12), (16, 21))
+N_LEGENDRE_FILE = 38
+
+
+def _hg_slots(g, n_slots, one_based=False):
+    """Legendre coefficient slots for every stream set: slot offset+l holds
+    (2l+1)/2 g^l, l = 1..n (g <= 0.5 keeps the delta truncation 1 - f w > 0)."""
+    g = np.asarray(g)
+    out = np.zeros(g.shape + (n_slots,))
+    if not one_based:
+        out[..., 0] = 0.5
+    for n, off in STREAM_SETS:
+        for l in range(1, n + 1):
+            out[..., off + l - (1 if one_based else 0)] = (2 * l + 1) / 2.0 * g ** l
+    return out
+
+
+def make_cloud_coeff() -> coeffio.CloudCoeff:
+    f_mw = np.array([10.65, 18.7, 23.8, 31.4, 36.5, 50.3, 52.8, 55.5, 57.29, 60.0, 70.0, 80.0, 89.0, 100.0,
+                     118.75, 130.0, 150.0, 165.5, 175.0, 183.31, 190.0, 200.0, 220.0])
+    r_mw = np.array([5.0, 10.0, 20.0, 40.0, 80.0, 150.0, 300.0, 500.0, 800.0, 1200.0, 2000.0])
+    temp = np.array([243.15, 253.15, 263.15, 273.15, 283.15, 293.15, 303.15])
+    dens = np.array([100.0, 400.0, 917.0])
+    f_ir = np.geomspace(100.0, 3500.0, 48)
+    r_ir = np.array([2.0, 5.0, 10.0, 20.0, 40.0, 60.0, 80.0, 100.0, 150.0, 200.0])
+    nL1 = N_LEGENDRE_FILE + 1
+    # microwave size parameter x = 2 pi r / lambda
+    lam_mm = 299.792458 / f_mw
+    x = 2.0 * np.pi * (r_mw[None, :] * 1e-3) / lam_mm[:, None]            # (f, r)
+    x3 = x ** 3
+    g_fr = 0.5 * x ** 2 / (1.0 + x ** 2)
+    # liquid (f, r, T)
+    kab_l = (0.3 * (f_mw / 23.8) ** 1.7 / (1.0 + (f_mw / 120.0) ** 1.2))[:, None, None] * \
+        np.exp(0.012 * (283.15 - temp))[None, None, :]
+    ksc_l = (8.0 * x3 / (1.0 + x3) * np.sqrt(f_mw / 23.8)[:, None])[:, :, None] * np.ones(temp.size)
+    ke_l = kab_l + ksc_l
+    w_l = ksc_l / ke_l
+    g_l = np.broadcast_to(g_fr[:, :, None], ke_l.shape).copy()
+    pc_l = _hg_slots(g_l, nL1)[..., None]                                   # (f, r, T, 0:38, 1)
+    # solid (f, r, D)
+    rho = dens / 917.0
+    kab_s = (0.004 * (f_mw / 23.8))[:, None, None] * rho[None, None, :] * np.ones((1, r_mw.size, 1))
+    ksc_s = (6.0 * x3 / (1.0 + x3) * np.sqrt(f_mw / 23.8)[:, None])[:, :, None] * rho[None, None, :] ** 0.7
+    ke_s = kab_s + ksc_s
+    w_s = ksc_s / ke_s
+    g_s = np.broadcast_to(g_fr[:, :, None], ke_s.shape).copy()
+    pc_s = _hg_slots(g_s, nL1)[..., None]
+    # infrared (f, r, D+1): index 0 liquid, 1..3 frozen
+    lam_um = 1.0e4 / f_ir
+    xi = 2.0 * np.pi * r_ir[None, :] / lam_um[:, None]
+    q = 2.0 * xi ** 2 / (1.0 + xi ** 2)
+    rho_ir = np.concatenate([[1000.0], dens])
+    ke_ir = (1500.0 * q / r_ir[None, :])[:, :, None] * (1000.0 / rho_ir)[None, None, :]
+    w_ir = (0.55 + 0.35 * np.sin(f_ir / 700.0)[:, None] * xi / (1.0 + xi))[:, :, None] * \
+        np.ones(rho_ir.size)
+    g_ir = np.broadcast_to((0.5 * xi ** 2 / (1.0 + xi ** 2))[:, :, None], ke_ir.shape).copy()
+    pc_ir = _hg_slots(g_ir, N_LEGENDRE_FILE, one_based=True)[..., None]
+    return coeffio.CloudCoeff(
+        frequency_mw=f_mw, frequency_ir=f_ir, reff_mw=r_mw, reff_ir=r_ir, temperature=temp, density_mw=dens,
+        ke_l_mw=ke_l, w_l_mw=w_l, g_l_mw=g_l, pcoeff_l_mw=pc_l,
+        ke_s_mw=ke_s, w_s_mw=w_s, g_s_mw=g_s, pcoeff_s_mw=pc_s,
+        ke_ir=ke_ir, w_ir=w_ir, g_ir=g_ir, pcoeff_ir=pc_ir)
+
+
+def write_cloud_coeff(dirpath: str) -> str:
+    """Write CloudCoeff.bin (idempotent); returns its path."""
+    os.makedirs(dirpath, exist_ok=True)
+    path = os.path.join(dirpath, CLOUDCOEFF_FILE)
+    if not os.path.exists(path):
+        coeffio.write_cloudcoeff(path + ".tmp", make_cloud_coeff())
+        os.replace(path + ".tmp", path)
+    return path
+
+
+# pressure band (hPa) of each cloud type's block and its radius range (um)
+_CLOUD_BANDS = {WATER_CLOUD: ((600.0, 950.0), (5.0, 20.0)), ICE_CLOUD: ((150.0, 450.0), (20.0, 80.0)),
+                RAIN_CLOUD: ((700.0, 1000.0), (5.0, 20.0)), SNOW_CLOUD: ((350.0, 750.0), (20.0, 80.0))}
+
+
+def add_clouds(ps: ProfileSet, seed: int = 20261204,
+               types: Sequence[int] = (WATER_CLOUD, ICE_CLOUD, RAIN_CLOUD, SNOW_CLOUD),
+               max_layers: int = 10, median_gm3: float = 0.05, sigma: float = 1.0,
+               cloudy_fraction: float = 1.0, cover: str = "overcast",
+               reff_scale: float = 1.0) -> ProfileSet:
+    """Cloudy copy of ps: per profile and type a block of U{0..max_layers}
+    contiguous layers in the type's pressure band, water content
+    median_gm3 * exp(N(0, sigma)) g/m^3 times the hypsometric layer thickness
+    (kg/m^2).  cloudy_fraction of the profiles get clouds.  cover: 'overcast'
+    (cloud fraction 1 in cloudy layers) or 'random' (U(0.2, 1))."""
+    rng = np.random.default_rng(seed)
+    M, nc = ps.n_profiles, len(types)
+    ntot = int(ps.layer_offset[-1])
+    wc = np.zeros((nc, ntot))
+    re_ = np.zeros((nc, ntot))
+    cf = np.zeros(ntot)
+    for m in range(M):
+        a, b = int(ps.layer_offset[m]), int(ps.layer_offset[m + 1])
+        if rng.uniform() >= cloudy_fraction:
+            continue
+        lp = ps.level_pressure[a + m:b + m + 1]
+        p = ps.pressure[a:b]
+        dz = 29.2712 * ps.temperature[a:b] * np.log(lp[1:] / lp[:-1])      # m (R_d / g0)
+        for n, t in enumerate(types):
+            (plo, phi), (rlo, rhi) = _CLOUD_BANDS[t]
+            n_lay = int(rng.integers(0, max_layers + 1))
+            band = np.nonzero((p >= plo) & (p <= phi))[0]
+            if n_lay == 0 or band.size == 0:
+                continue
+            n_lay = min(n_lay, band.size)
+            k0 = band[0] + int(rng.integers(0, band.size - n_lay + 1))
+            ks = np.arange(k0, k0 + n_lay)
+            wc[n, a + ks] = median_gm3 * 1e-3 * np.exp(sigma * rng.standard_normal(n_lay)) * dz[ks]
+            re_[n, a + ks] = reff_scale * rng.uniform(rlo, rhi, n_lay)
+            cf[a + ks] = 1.0 if cover == "overcast" else rng.uniform(0.2, 1.0, n_lay)
+    out = dataclasses.replace(ps)
+    out.cloud_type = np.tile(np.asarray(types, np.int32), (M, 1))
+    out.cloud_water_content = wc
+    out.cloud_effective_radius = re_
+    out.cloud_fraction = cf
+    return out

@@ -18,6 +18,8 @@
 //     (src/AtmAbsorption/ODPS/ODPS_CoordinateMapping.f90:264-270 vs :516-518)
 #include "crtm_oracle.h"
 
+#include <algorithm>
+#include <array>
 #include <cmath>
 #include <complex>
 #include <cstring>
@@ -1282,3 +1284,5 @@ extern "C" int oracle_k_matrix_batch(const or_sensor* s, int n_profiles, const o
   }
   return 0;
 }
+
+#include "crtm_oracle_allsky.inc"

@@ -42,6 +42,8 @@ typedef struct {
   const double* band_c1;
   const double* band_c2;
   const double* cosmic_background;
+  const double* frequency;           /* GHz [L] (microwave cloud tables) */
+  const double* wavenumber;          /* cm^-1 [L] (infrared tables, stream choice) */
 } or_sensor;
 
 typedef struct {
@@ -102,6 +104,62 @@ void oracle_set_climatology(const double* p, const double* t, const double* a);
  * nabs*(K+n)) the extended profile is written to them. */
 int oracle_add_layers(const or_profile* p, double* level_pressure, double* pressure, double* temperature,
                       double* absorber);
+
+/* ---------------- all-sky (clouds, ADA) ---------------- */
+
+/* CloudCoeff release 3 (src/Coefficients/CloudCoeff/CloudCoeff_Define.f90:95-154);
+ * arrays column-major as read from the file. */
+typedef struct {
+  int n_mw_frequencies, n_mw_radii, n_ir_frequencies, n_ir_radii, n_temperatures;
+  int n_mw_densities, n_ir_densities, n_legendre_terms, n_phase_elements;
+  const double *frequency_mw, *frequency_ir, *reff_mw, *reff_ir, *temperature, *density_mw;
+  const double *ke_l_mw, *w_l_mw, *pcoeff_l_mw;  /* (f,r,T), (f,r,T,0:nL,nP) */
+  const double *ke_s_mw, *w_s_mw, *pcoeff_s_mw;  /* (f,r,D), (f,r,D,0:nL,nP) */
+  const double *ke_ir, *w_ir, *pcoeff_ir;        /* (f,r,Dir), (f,r,Dir,1:nL,nP) */
+} or_cloudcoeff;
+
+typedef struct {
+  int n_clouds;
+  const int* type;        /* [n_clouds], CRTM cloud type 1..6 */
+  const double* wc;       /* [n_clouds*K] water content, kg/m^2 */
+  const double* reff;     /* [n_clouds*K] effective radius, microns */
+  const double* fraction; /* [K] layer cloud fraction (NULL: 0) */
+} or_clouds;
+
+typedef struct {
+  int n_streams;  /* Options%n_Streams (0: chosen from the Mie parameter) */
+  int overlap;    /* Options%Overlap_Id 1..5 (0: default, average) */
+} or_scatter_opts;
+
+typedef struct {
+  double* radiance;          /* [L] */
+  double* bt;                /* [L] */
+  double* r_clear;           /* [L] optional */
+  double* tb_clear;          /* [L] optional */
+  double* total_cloud_cover; /* [L] optional */
+  double* sod;               /* [L] optional, scattering optical depth */
+  int* n_streams;            /* [L] optional, Legendre terms of the scattering RT (0: Emission) */
+} or_allsky_out;
+
+/* CRTM_Forward with clouds for one profile, all channels. */
+int oracle_forward_allsky(const or_sensor* s, const or_cloudcoeff* cc, const or_profile* p, const or_clouds* c,
+                          const double* emissivity, const or_scatter_opts* opt, or_allsky_out* o);
+
+/* White-box pieces of the all-sky path. */
+int oracle_cloud_optics(const or_cloudcoeff* cc, int microwave, double freq, int type, double reff,
+                        double temperature, int n_legendre, double* ke, double* w, double* pcoeff);
+void oracle_lpoly_interp(const double* x, int n, double xi, const double* z, double* out);
+void oracle_gauss(int n, double* mu, double* w);
+int oracle_phase_matrix(int n_streams, double cos_sensor, int n_legendre, const double* pl, double* mu,
+                        double* wt, double* pff, double* pbb);
+int oracle_matrix_exp(int n, double tau, const double* A, const double* B, double* refl, double* trans);
+double oracle_cloud_cover(int K, int n_clouds, double* wc, const double* cf, int overlap);
+double oracle_combine(int n_legendre, int K, double* od, double* ssa, double* pc);
+int oracle_ada(int n_streams, double cos_sensor, int n_legendre, int K, const double* tau, const double* ssa,
+               const double* pl, const double* planck, double planck_sfc, double emissivity, double cosmic,
+               double* radiance);
+int oracle_amom_layer(int n_streams, double cos_sensor, int n_legendre, const double* pl, double tau, double ssa,
+                      double planck, double* refl, double* trans, double* src);
 
 /* Intermediate quantities for white-box tests. */
 int oracle_layer_avg(int n_odps, const double* ref_lnp, int n_user, const double* user_lnp,

@@ -31,7 +31,30 @@ class OrSensor(ctypes.Structure):
                 ("alpha", ctypes.c_double), ("alpha_c1", ctypes.c_double),
                 ("alpha_c2", ctypes.c_double), ("ocomponent_index", ctypes.c_int),
                 ("sensor_type", ctypes.c_int), ("planck_c1", _dp), ("planck_c2", _dp),
-                ("band_c1", _dp), ("band_c2", _dp), ("cosmic_background", _dp)]
+                ("band_c1", _dp), ("band_c2", _dp), ("cosmic_background", _dp),
+                ("frequency", _dp), ("wavenumber", _dp)]
+
+
+class OrCloudCoeff(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int) for n in ("n_mw_frequencies", "n_mw_radii", "n_ir_frequencies", "n_ir_radii",
+                                            "n_temperatures", "n_mw_densities", "n_ir_densities",
+                                            "n_legendre_terms", "n_phase_elements")] + \
+               [(n, _dp) for n in ("frequency_mw", "frequency_ir", "reff_mw", "reff_ir", "temperature",
+                                   "density_mw", "ke_l_mw", "w_l_mw", "pcoeff_l_mw", "ke_s_mw", "w_s_mw",
+                                   "pcoeff_s_mw", "ke_ir", "w_ir", "pcoeff_ir")]
+
+
+class OrClouds(ctypes.Structure):
+    _fields_ = [("n_clouds", ctypes.c_int), ("type", _ip), ("wc", _dp), ("reff", _dp), ("fraction", _dp)]
+
+
+class OrScatterOpts(ctypes.Structure):
+    _fields_ = [("n_streams", ctypes.c_int), ("overlap", ctypes.c_int)]
+
+
+class OrAllskyOut(ctypes.Structure):
+    _fields_ = [(n, _dp) for n in ("radiance", "bt", "r_clear", "tb_clear", "total_cloud_cover", "sod")] + \
+               [("n_streams", _ip)]
 
 
 class OrProfile(ctypes.Structure):
@@ -71,6 +94,25 @@ def lib():
                                                ctypes.POINTER(OrOut), ctypes.c_int]
         _lib.oracle_set_climatology.argtypes = [_dp, _dp, _dp]
         _lib.oracle_add_layers.argtypes = [ctypes.POINTER(OrProfile), _dp, _dp, _dp, _dp]
+        _lib.oracle_forward_allsky.argtypes = [ctypes.POINTER(OrSensor), ctypes.POINTER(OrCloudCoeff),
+                                               ctypes.POINTER(OrProfile), ctypes.POINTER(OrClouds), _dp,
+                                               ctypes.POINTER(OrScatterOpts), ctypes.POINTER(OrAllskyOut)]
+        _lib.oracle_cloud_optics.argtypes = [ctypes.POINTER(OrCloudCoeff), ctypes.c_int, ctypes.c_double,
+                                             ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                             _dp, _dp, _dp]
+        _lib.oracle_lpoly_interp.argtypes = [_dp, ctypes.c_int, ctypes.c_double, _dp, _dp]
+        _lib.oracle_gauss.argtypes = [ctypes.c_int, _dp, _dp]
+        _lib.oracle_phase_matrix.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_int, _dp, _dp, _dp,
+                                             _dp, _dp]
+        _lib.oracle_matrix_exp.argtypes = [ctypes.c_int, ctypes.c_double, _dp, _dp, _dp, _dp]
+        _lib.oracle_cloud_cover.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, ctypes.c_int]
+        _lib.oracle_cloud_cover.restype = ctypes.c_double
+        _lib.oracle_combine.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp]
+        _lib.oracle_combine.restype = ctypes.c_double
+        _lib.oracle_ada.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp,
+                                    _dp, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp]
+        _lib.oracle_amom_layer.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_int, _dp, ctypes.c_double,
+                                           ctypes.c_double, ctypes.c_double, _dp, _dp, _dp]
         _set_climatology(_lib)
     return _lib
 
@@ -130,6 +172,8 @@ class Sensor:
         s.band_c1 = _p(keep(spc.band_c1, np.float64), _dp)
         s.band_c2 = _p(keep(spc.band_c2, np.float64), _dp)
         s.cosmic_background = _p(keep(spc.cosmic_background_radiance, np.float64), _dp)
+        s.frequency = _p(keep(spc.frequency, np.float64), _dp)
+        s.wavenumber = _p(keep(spc.wavenumber, np.float64), _dp)
         self.s = s
         self.n_channels = tau.n_channels
 
@@ -205,4 +249,65 @@ def k_matrix_batch(sensor: Sensor, ps, emissivity, n_threads=0, bt_k=None, rad_k
     rk = np.ascontiguousarray(np.zeros((M, L)) if rad_k is None else rad_k, np.float64)
     lib().oracle_k_matrix_batch(ctypes.byref(sensor.s), M, profs, _p(eps, _dp), _p(btk, _dp),
                                 _p(rk, _dp), ctypes.byref(o), n_threads)
+    return out
+
+
+class CloudCoeff:
+    """Oracle view of a coeffio.CloudCoeff (keeps arrays alive)."""
+
+    def __init__(self, cc):
+        self._keep = []
+
+        def keep(a):
+            a = np.ascontiguousarray(np.asarray(a, np.float64).ravel(order="F"))
+            self._keep.append(a)
+            return _p(a, _dp)
+        c = OrCloudCoeff()
+        c.n_mw_frequencies, c.n_mw_radii = cc.frequency_mw.size, cc.reff_mw.size
+        c.n_ir_frequencies, c.n_ir_radii = cc.frequency_ir.size, cc.reff_ir.size
+        c.n_temperatures, c.n_mw_densities = cc.temperature.size, cc.density_mw.size
+        c.n_ir_densities = cc.n_ir_densities
+        c.n_legendre_terms, c.n_phase_elements = cc.n_legendre_terms, cc.n_phase_elements
+        for n in ("frequency_mw", "frequency_ir", "reff_mw", "reff_ir", "temperature", "density_mw",
+                  "ke_l_mw", "w_l_mw", "pcoeff_l_mw", "ke_s_mw", "w_s_mw", "pcoeff_s_mw", "ke_ir", "w_ir",
+                  "pcoeff_ir"):
+            setattr(c, n, keep(getattr(cc, n)))
+        self.c = c
+
+
+def make_clouds(ps, m, keep):
+    """OrClouds of profile m (None when the set carries no clouds)."""
+    if ps.cloud_type is None:
+        return None
+    a, b = int(ps.layer_offset[m]), int(ps.layer_offset[m + 1])
+    types = np.ascontiguousarray(ps.cloud_type[m], np.int32)
+    wc = np.ascontiguousarray(ps.cloud_water_content[:, a:b], np.float64)
+    re_ = np.ascontiguousarray(ps.cloud_effective_radius[:, a:b], np.float64)
+    cf = None if ps.cloud_fraction is None else np.ascontiguousarray(ps.cloud_fraction[a:b], np.float64)
+    keep.extend([types, wc, re_, cf])
+    c = OrClouds()
+    c.n_clouds = types.size
+    c.type, c.wc, c.reff, c.fraction = _p(types, _ip), _p(wc, _dp), _p(re_, _dp), _p(cf, _dp)
+    return c
+
+
+def run_allsky(sensor: Sensor, cloudcoeff: CloudCoeff, ps, m, emissivity, n_streams=0, overlap=0):
+    """CRTM_Forward with clouds for profile m; dict of per-channel outputs."""
+    keep = []
+    p = make_profile(ps, m, keep)
+    c = make_clouds(ps, m, keep)
+    L = sensor.n_channels
+    out = {k: np.zeros(L) for k in ("radiance", "bt", "r_clear", "tb_clear", "total_cloud_cover", "sod")}
+    out["n_streams"] = np.zeros(L, np.int32)
+    o = OrAllskyOut()
+    for k, v in out.items():
+        setattr(o, k, _p(v, _ip if k == "n_streams" else _dp))
+    opt = OrScatterOpts(int(n_streams), int(overlap))
+    eps = np.ascontiguousarray(emissivity, np.float64)
+    rc = lib().oracle_forward_allsky(ctypes.byref(sensor.s),
+                                     ctypes.byref(cloudcoeff.c) if cloudcoeff is not None else None,
+                                     ctypes.byref(p), ctypes.byref(c) if c is not None else None,
+                                     _p(eps, _dp), ctypes.byref(opt), ctypes.byref(o))
+    if rc != 0:
+        raise RuntimeError(f"oracle_forward_allsky failed ({rc})")
     return out

@@ -6,8 +6,12 @@
 ! Wrapped reference routines:
 !   Read_ODPS_Binary         src/Coefficients/TauCoeff/ODPS/ODPS_Binary_IO.f90:481
 !   CRTM_GeometryInfo_Compute src/GeometryInfo/CRTM_GeometryInfo.f90:100
-!   matinv, ASYMTX, Double_Gauss_Quadrature, Legendre
+!   matinv, ASYMTX, Double_Gauss_Quadrature, Legendre, Legendre_M
 !                            src/CRTM_Utility/CRTM_Utility.f90:343-1094
+!   CloudCoeff_Binary_ReadFile src/Coefficients/CloudCoeff/CloudCoeff_Binary_IO.f90:340
+!   Find_Index, LPoly, Interp_1D/2D/3D
+!                            src/Interpolation/CRTM_Interpolation.f90:225-860
+!   CRTM_AtmOptics_Combine   src/AtmOptics/CRTM_AtmOptics.f90:369
 MODULE ref_harness
   USE ISO_C_BINDING
   USE Type_Kinds, ONLY: fp
@@ -17,9 +21,16 @@ MODULE ref_harness
                                       CRTM_GeometryInfo_GetValue
   USE CRTM_Geometry_Define, ONLY: CRTM_Geometry_type, CRTM_Geometry_SetValue
   USE CRTM_GeometryInfo, ONLY: CRTM_GeometryInfo_Compute
-  USE CRTM_Utility, ONLY: matinv, ASYMTX, Double_Gauss_Quadrature, Legendre
+  USE CRTM_Utility, ONLY: matinv, ASYMTX, Double_Gauss_Quadrature, Legendre, Legendre_M
+  USE CloudCoeff_Define, ONLY: CloudCoeff_type, CloudCoeff_Destroy
+  USE CloudCoeff_Binary_IO, ONLY: CloudCoeff_Binary_ReadFile
+  USE CRTM_Interpolation, ONLY: Find_Index, LPoly, LPoly_type, Interp_1D, Interp_2D, Interp_3D
+  USE CRTM_AtmOptics_Define, ONLY: CRTM_AtmOptics_type, CRTM_AtmOptics_Create
+  USE AOvar_Define, ONLY: AOvar_type, AOvar_Create
+  USE CRTM_AtmOptics, ONLY: CRTM_AtmOptics_Combine
   IMPLICIT NONE
   TYPE(ODPS_type), SAVE :: tc
+  TYPE(CloudCoeff_type), SAVE :: ccf
 CONTAINS
 
   FUNCTION c2f(cstr, n) RESULT(s)
@@ -124,5 +135,113 @@ CONTAINS
     CALL ASYMTX(aa, n, n, n, evec, eval, err)
     ref_asymtx = err
   END FUNCTION ref_asymtx
+
+  SUBROUTINE ref_legendre_m(nl, x, p) BIND(C, NAME='ref_legendre_m')
+    INTEGER(C_INT), VALUE :: nl
+    REAL(C_DOUBLE), VALUE :: x
+    REAL(C_DOUBLE), INTENT(OUT) :: p(0:nl)
+    CALL Legendre_M(0, nl, x, p)
+  END SUBROUTINE ref_legendre_m
+
+  ! Load a CloudCoeff file with the reference reader; dims(1:9) =
+  ! n_MW_Frequencies, n_MW_Radii, n_IR_Frequencies, n_IR_Radii, n_Temperatures,
+  ! n_MW_Densities, n_IR_Densities, n_Legendre_Terms, n_Phase_Elements
+  INTEGER(C_INT) FUNCTION ref_cloudcoeff_load(path, n, dims) BIND(C, NAME='ref_cloudcoeff_load')
+    CHARACTER(KIND=C_CHAR), INTENT(IN) :: path(*)
+    INTEGER(C_INT), VALUE :: n
+    INTEGER(C_INT), INTENT(OUT) :: dims(9)
+    CALL CloudCoeff_Destroy(ccf)
+    ref_cloudcoeff_load = CloudCoeff_Binary_ReadFile(c2f(path, n), ccf, Quiet=.TRUE.)
+    IF (ref_cloudcoeff_load /= 0) RETURN
+    dims = (/ ccf%n_MW_Frequencies, ccf%n_MW_Radii, ccf%n_IR_Frequencies, ccf%n_IR_Radii, &
+              ccf%n_Temperatures, ccf%n_MW_Densities, ccf%n_IR_Densities, ccf%n_Legendre_Terms, &
+              ccf%n_Phase_Elements /)
+  END FUNCTION ref_cloudcoeff_load
+
+  ! Copy the loaded CloudCoeff arrays out (column-major, as stored)
+  SUBROUTINE ref_cloudcoeff_fetch(fmw, fir, rmw, rir, t, dmw, dir, kel, wl, pl, kes, ws, ps, kei, wi, pi) &
+      BIND(C, NAME='ref_cloudcoeff_fetch')
+    REAL(C_DOUBLE), INTENT(OUT) :: fmw(*), fir(*), rmw(*), rir(*), t(*), dmw(*), dir(*)
+    REAL(C_DOUBLE), INTENT(OUT) :: kel(*), wl(*), pl(*), kes(*), ws(*), ps(*), kei(*), wi(*), pi(*)
+    fmw(1:SIZE(ccf%Frequency_MW)) = ccf%Frequency_MW
+    fir(1:SIZE(ccf%Frequency_IR)) = ccf%Frequency_IR
+    rmw(1:SIZE(ccf%Reff_MW)) = ccf%Reff_MW
+    rir(1:SIZE(ccf%Reff_IR)) = ccf%Reff_IR
+    t(1:SIZE(ccf%Temperature)) = ccf%Temperature
+    dmw(1:SIZE(ccf%Density_MW)) = ccf%Density_MW
+    dir(1:SIZE(ccf%Density_IR)) = ccf%Density_IR
+    kel(1:SIZE(ccf%ke_L_MW)) = RESHAPE(ccf%ke_L_MW, (/SIZE(ccf%ke_L_MW)/))
+    wl(1:SIZE(ccf%w_L_MW)) = RESHAPE(ccf%w_L_MW, (/SIZE(ccf%w_L_MW)/))
+    pl(1:SIZE(ccf%pcoeff_L_MW)) = RESHAPE(ccf%pcoeff_L_MW, (/SIZE(ccf%pcoeff_L_MW)/))
+    kes(1:SIZE(ccf%ke_S_MW)) = RESHAPE(ccf%ke_S_MW, (/SIZE(ccf%ke_S_MW)/))
+    ws(1:SIZE(ccf%w_S_MW)) = RESHAPE(ccf%w_S_MW, (/SIZE(ccf%w_S_MW)/))
+    ps(1:SIZE(ccf%pcoeff_S_MW)) = RESHAPE(ccf%pcoeff_S_MW, (/SIZE(ccf%pcoeff_S_MW)/))
+    kei(1:SIZE(ccf%ke_IR)) = RESHAPE(ccf%ke_IR, (/SIZE(ccf%ke_IR)/))
+    wi(1:SIZE(ccf%w_IR)) = RESHAPE(ccf%w_IR, (/SIZE(ccf%w_IR)/))
+    pi(1:SIZE(ccf%pcoeff_IR)) = RESHAPE(ccf%pcoeff_IR, (/SIZE(ccf%pcoeff_IR)/))
+  END SUBROUTINE ref_cloudcoeff_fetch
+
+  ! Table interpolation as CRTM_CloudScatter does it: clamp, Find_Index,
+  ! LPoly per axis, Interp_nD over the 4^nd sub-table.  nd = 1..3; x packs
+  ! the axes (n(1), n(2), n(3) points), z the full column-major table.
+  SUBROUTINE ref_interp(nd, n, x, xi, z, out) BIND(C, NAME='ref_interp')
+    INTEGER(C_INT), VALUE :: nd
+    INTEGER(C_INT), INTENT(IN) :: n(3)
+    REAL(C_DOUBLE), INTENT(IN) :: x(*), xi(3), z(*)
+    REAL(C_DOUBLE), INTENT(OUT) :: out
+    TYPE(LPoly_type) :: lp(3)
+    INTEGER :: i1(3), i2(3), d, o, a, b, c
+    REAL(fp) :: xv, sub1(4), sub2(4,4), sub3(4,4,4)
+    LOGICAL :: oob
+    o = 0
+    DO d = 1, nd
+      xv = MAX(MIN(x(o+n(d)), xi(d)), x(o+1))
+      CALL Find_Index(x(o+1:o+n(d)), xv, i1(d), i2(d), oob)
+      CALL LPoly(x(o+i1(d):o+i2(d)), xv, lp(d))
+      o = o + n(d)
+    END DO
+    SELECT CASE (nd)
+      CASE (1)
+        sub1 = z(i1(1):i2(1))
+        CALL Interp_1D(sub1, lp(1), out)
+      CASE (2)
+        DO b = 1, 4
+          DO a = 1, 4
+            sub2(a,b) = z((i1(1)+a-1) + n(1)*(i1(2)+b-2))
+          END DO
+        END DO
+        CALL Interp_2D(sub2, lp(1), lp(2), out)
+      CASE (3)
+        DO c = 1, 4
+          DO b = 1, 4
+            DO a = 1, 4
+              sub3(a,b,c) = z((i1(1)+a-1) + n(1)*((i1(2)+b-2) + n(2)*(i1(3)+c-2)))
+            END DO
+          END DO
+        END DO
+        CALL Interp_3D(sub3, lp(1), lp(2), lp(3), out)
+    END SELECT
+  END SUBROUTINE ref_interp
+
+  ! CRTM_AtmOptics_Combine on K layers: od, ssa in/out, pc(0:nl, K) in/out, sod out
+  SUBROUTINE ref_combine(nl, k, od, ssa, pc, sod) BIND(C, NAME='ref_combine')
+    INTEGER(C_INT), VALUE :: nl, k
+    REAL(C_DOUBLE), INTENT(INOUT) :: od(k), ssa(k), pc(0:nl, k)
+    REAL(C_DOUBLE), INTENT(OUT) :: sod
+    TYPE(CRTM_AtmOptics_type) :: ao
+    TYPE(AOvar_type) :: aov
+    CALL CRTM_AtmOptics_Create(ao, k, nl, 1)
+    CALL AOvar_Create(aov, k)
+    ao%Optical_Depth = od
+    ao%Single_Scatter_Albedo = ssa
+    ao%Phase_Coefficient = 0.0_fp
+    ao%Phase_Coefficient(0:nl, 1, 1:k) = pc
+    ao%Backscat_Coefficient = 0.0_fp
+    CALL CRTM_AtmOptics_Combine(ao, aov)
+    od = ao%Optical_Depth
+    ssa = ao%Single_Scatter_Albedo
+    pc = ao%Phase_Coefficient(0:nl, 1, 1:k)
+    sod = ao%Scattering_Optical_Depth
+  END SUBROUTINE ref_combine
 
 END MODULE ref_harness
