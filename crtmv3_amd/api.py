@@ -46,7 +46,8 @@ class CAtmosphere(ctypes.Structure):
                 ("n_layers", _vp), ("layer_offset", _vp),
                 ("n_absorbers", ctypes.c_int), ("absorber_id", _vp), ("absorber_units", _vp),
                 ("level_pressure", _vp), ("pressure", _vp), ("temperature", _vp), ("absorber", _vp),
-                ("climatology", _vp)]
+                ("climatology", _vp), ("n_clouds", ctypes.c_int), ("cloud_type", _vp),
+                ("cloud_water_content", _vp), ("cloud_effective_radius", _vp), ("cloud_fraction", _vp)]
 
 
 class CSurface(ctypes.Structure):
@@ -62,12 +63,14 @@ class CGeometry(ctypes.Structure):
 
 class COptions(ctypes.Structure):
     _fields_ = [("use_emissivity", ctypes.c_int), ("emissivity", _vp), ("precision", ctypes.c_int),
-                ("memory", ctypes.c_int), ("jacobian_f32", ctypes.c_int), ("check_input", ctypes.c_int)]
+                ("memory", ctypes.c_int), ("jacobian_f32", ctypes.c_int), ("check_input", ctypes.c_int),
+                ("n_streams", ctypes.c_int), ("overlap_id", ctypes.c_int)]
 
 
 class CRTSolution(ctypes.Structure):
     _fields_ = [(n, _vp) for n in ("radiance", "brightness_temperature", "up_radiance", "down_radiance",
-                                   "surface_planck_radiance", "surface_emissivity", "status")]
+                                   "surface_planck_radiance", "surface_emissivity", "status", "r_clear",
+                                   "tb_clear", "total_cloud_cover")]
 
 
 class CKOutput(ctypes.Structure):
@@ -85,6 +88,7 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB_PATH):
             raise CRTMError(f"{LIB_PATH} missing: run `python -m crtmv3_amd.build`")
+        _torch_device_first()
         L = ctypes.CDLL(LIB_PATH)
         L.crtm_init.argtypes = [ctypes.c_int, ctypes.POINTER(ctypes.c_char_p), ctypes.c_char_p,
                                 ctypes.POINTER(_vp)]
@@ -102,8 +106,23 @@ def lib():
         L.crtm_last_error.restype = ctypes.c_char_p
         L.crtm_profile_enable.argtypes = [_vp, ctypes.c_int]
         L.crtm_profile_query.argtypes = [_vp, _dp, _dp, _ip]
+        L.crtm_profile_query_kinds.argtypes = [_vp, _dp, _ip, ctypes.c_int]
         _lib = L
     return _lib
+
+
+def _torch_device_first():
+    """PyTorch-ROCm ships its own HIP runtime next to the system one the
+    library links.  Both can drive the GPU of one process when torch's runtime
+    opens the device first; the other order leaves torch without a device.
+    So, when torch is installed and sees a GPU, it is initialised before the
+    library is loaded."""
+    try:
+        import torch
+    except ImportError:
+        return
+    if torch.cuda.is_available():
+        torch.cuda.init()
 
 
 def _is_torch(x) -> bool:
@@ -161,7 +180,8 @@ class Context:
         raise CRTMError(f"{what}: {msg}")
 
     # -- argument marshalling ------------------------------------------------
-    def _structs(self, prof, emissivity, precision, memory, jacobian_f32, check_input, keep):
+    def _structs(self, prof, emissivity, precision, memory, jacobian_f32, check_input, keep, n_streams=0,
+                 overlap=0):
         def arr(x, dt):
             if x is None:
                 return None
@@ -194,6 +214,14 @@ class Context:
         atm.absorber = _ptr(arr(prof.absorber, np.float64))
         clim = getattr(prof, "climatology", None)
         atm.climatology = _ptr(arr(clim, np.int32)) if clim is not None else None
+        ctype = getattr(prof, "cloud_type", None)
+        if ctype is not None and int(ctype.shape[1]) > 0:
+            atm.n_clouds = int(ctype.shape[1])
+            atm.cloud_type = _ptr(arr(ctype, np.int32))
+            atm.cloud_water_content = _ptr(arr(prof.cloud_water_content, np.float64))
+            atm.cloud_effective_radius = _ptr(arr(prof.cloud_effective_radius, np.float64))
+            cf = getattr(prof, "cloud_fraction", None)
+            atm.cloud_fraction = _ptr(arr(cf, np.float64)) if cf is not None else None
         sfc = CSurface()
         for f in ("land_coverage", "water_coverage", "snow_coverage", "ice_coverage", "land_temperature",
                   "water_temperature", "snow_temperature", "ice_temperature"):
@@ -209,6 +237,8 @@ class Context:
         opt.memory = memory
         opt.jacobian_f32 = int(jacobian_f32)
         opt.check_input = int(check_input)
+        opt.n_streams = int(n_streams)
+        opt.overlap_id = int(overlap)
         return atm, sfc, geo, opt
 
     def _alloc(self, shape, dtype, memory, like=None):
@@ -218,19 +248,28 @@ class Context:
             return torch.zeros(shape, dtype=tdt, device=like.device)
         return np.zeros(shape, dtype)
 
-    def forward(self, prof, emissivity, precision=PREC_FP64, extras=False, check_input=True):
-        """CRTM_Forward over a ProfileSet (host numpy) or a device batch (torch)."""
+    def forward(self, prof, emissivity, precision=PREC_FP64, extras=False, check_input=True, n_streams=0,
+                overlap=0):
+        """CRTM_Forward over a ProfileSet (host numpy) or a device batch (torch).
+
+        Profiles carrying clouds take the all-sky path; n_streams is
+        Options%n_Streams (0: chosen from the Mie parameter) and overlap
+        Options%Overlap_Id (0: average)."""
         memory = MEM_DEVICE if _is_torch(prof.temperature) else MEM_HOST
         if memory == MEM_DEVICE:
             self._use_torch_stream()
         keep = []
-        atm, sfc, geo, opt = self._structs(prof, emissivity, precision, memory, False, check_input, keep)
+        atm, sfc, geo, opt = self._structs(prof, emissivity, precision, memory, False, check_input, keep,
+                                           n_streams, overlap)
         M, L = prof.n_profiles, self.n_channels
         out = {k: self._alloc((M, L), np.float64, memory, prof.temperature)
                for k in ("radiance", "brightness_temperature")}
         if extras:
             for k in ("up_radiance", "down_radiance", "surface_planck_radiance", "surface_emissivity"):
                 out[k] = self._alloc((M, L), np.float64, memory, prof.temperature)
+            if atm.n_clouds > 0:
+                for k in ("r_clear", "tb_clear", "total_cloud_cover"):
+                    out[k] = self._alloc((M, L), np.float64, memory, prof.temperature)
         out["status"] = self._alloc((M,), np.int32, memory, prof.temperature)
         rts = CRTSolution(**{k: _ptr(v) for k, v in out.items()})
         st = lib().crtm_forward(self._h, atm, sfc, geo, opt, rts)
@@ -287,6 +326,15 @@ class Context:
     def profile_enable(self, on=True):
         if lib().crtm_profile_enable(self._h, int(on)) != SUCCESS:
             self._err("crtm_profile_enable")
+
+    def profile_query_kinds(self):
+        """Milliseconds and launches per kernel kind since the last enable/query:
+        ([prep, forward/K, scattering], [n_prep, n_main, n_scattering])."""
+        ms = (ctypes.c_double * 3)()
+        n = (ctypes.c_int * 3)()
+        if lib().crtm_profile_query_kinds(self._h, ms, n, 3) != SUCCESS:
+            self._err("crtm_profile_query_kinds")
+        return list(ms), list(n)
 
     def profile_query(self):
         """(prep_ms, main_ms, n_launches) summed since the last enable/query."""

@@ -29,22 +29,6 @@ namespace {
 constexpr double MAX_OD = 20.0;
 constexpr double LIMIT_EXP = 20.0, LIMIT_LOG = 4.8e+08;
 
-template <class RT> __device__ inline RT rexp(RT x);
-template <> __device__ inline float rexp<float>(float x) { return __expf(x); }
-template <> __device__ inline double rexp<double>(double x) { return exp(x); }
-
-// Planck radiance B = c1 / (exp(c2/Te) - 1), Te = bc1 + bc2*T
-// (CRTM_Planck_Functions.f90:116-136); expm1 keeps the microwave case
-// (c2/Te ~ 0.01) accurate in FP32.
-__device__ inline float planck(float c1, float c2, float bc1, float bc2, double t, float& te) {
-  te = bc1 + bc2 * (float)t;
-  return __fdividef(c1, expm1f(__fdividef(c2, te)));
-}
-__device__ inline double planck(double c1, double c2, double bc1, double bc2, double t, double& te) {
-  te = bc1 + bc2 * t;
-  return c1 / (exp(c2 / te) - 1.0);
-}
-
 template <class RT, class KT>
 __device__ inline void store_k(void* base, int64_t idx, RT v) {
   reinterpret_cast<KT*>(base)[idx] = (KT)v;
@@ -437,9 +421,12 @@ clear_fwdk_kernel(DevSensor s, DevBatch bt, ProfWork<RT> w, DevOut o) {
   RT uacc = RT(0), tc = RT(1);
   const RT up0 = ok_prof ? upath(0) : RT(0);
   RT upp = up0;
+  // all-sky calls keep the gas OD of cloudy profiles for the scattering RT
+  RT* tau_out = (o.tau_g && valid && o.tau_mask[m] > 0) ? reinterpret_cast<RT*>(o.tau_g) + loff * s.L + l : nullptr;
   for (int k = 0; k < K; ++k) {
     const RT up = upath(k + 1);
     const RT tau = (up - upp) * isec;
+    if (tau_out) tau_out[(int64_t)k * s.L] = tau;
     upp = up;
     const RT t = rexp<RT>(-tau * iu);
     RT te;

@@ -25,6 +25,12 @@ template <int G, class RT>
 void launch_prep(const DevSensor&, const DevBatch&, const ProfWork<RT>&, hipStream_t);
 template <int G, class RT, class KT, bool DOK>
 void launch_fwdk(const DevSensor&, const DevBatch&, const ProfWork<RT>&, const DevOut&, hipStream_t);
+void launch_cloud_prep(const DevBatch&, const DevClouds&, const CloudWork&, hipStream_t);
+template <class RT>
+void launch_allsky(const DevSensor&, const DevBatch&, const DevClouds&, const CloudWork&, const DevCloudCoeff&,
+                   const LayerScratch<RT>&, const RT*, const double*, int, const int*, int, const DevOut&,
+                   const AllskyOut&, hipStream_t);
+void launch_clear_copy(const DevSensor&, int M, int Ltot, const DevOut&, const AllskyOut&, hipStream_t);
 }  // namespace crtm
 
 using namespace crtm;
@@ -524,6 +530,87 @@ void build_sensor(Sensor& S, int chan_offset) {
   d.bc1 = S.put(sc.bc1);
   d.bc2 = S.put(sc.bc2);
   d.cosmic = S.put(sc.cosmic);
+  d.frequency = S.put(sc.frequency);
+  d.wavenumber = S.put(sc.wavenumber);
+  d.sensor_type = sc.sensor_type;
+}
+
+// CloudCoeff release 3 (CloudCoeff_Binary_IO.f90:367-446): dimensions, axes,
+// MW liquid, MW solid, IR tables.  Only phase element 1 is kept.
+struct CloudCoeffH {
+  int nfm = 0, nrm = 0, nfi = 0, nri = 0, nt = 0, ndm = 0, nl = 0, np = 0;
+  std::vector<double> f_mw, f_ir, r_mw, r_ir, temp, dens;
+  std::vector<double> ke_l, w_l, pc_l, ke_s, w_s, pc_s, ke_i, w_i, pc_i;
+};
+
+CloudCoeffH read_cloudcoeff(const std::string& path) {
+  Records rec(path);
+  CloudCoeffH c;
+  Cursor r0(rec.next());
+  const int rel = r0.get<int32_t>();
+  r0.get<int32_t>();
+  r0.done("CloudCoeff release");
+  if (rel < 3) throw CrtmError(fmt("CloudCoeff Release check failed for %s (release %d)", path.c_str(), rel));
+  Cursor r1(rec.next());
+  c.nfm = r1.get<int32_t>(); c.nrm = r1.get<int32_t>(); c.nfi = r1.get<int32_t>(); c.nri = r1.get<int32_t>();
+  c.nt = r1.get<int32_t>(); c.ndm = r1.get<int32_t>(); c.nl = r1.get<int32_t>(); c.np = r1.get<int32_t>();
+  r1.done("CloudCoeff dimensions");
+  if (c.nfm < 4 || c.nrm < 4 || c.nfi < 4 || c.nri < 4 || c.nt < 4 || c.ndm < 3 || c.np < 1)
+    throw CrtmError("CloudCoeff tables need at least 4 points per interpolation axis");
+  if (c.nl < 37) throw CrtmError(fmt("CloudCoeff has %d Legendre terms; the stream sets need 37", c.nl));
+  const size_t ndi = c.ndm + 1;
+  Cursor r2(rec.next());
+  c.f_mw = r2.arr<double>(c.nfm); c.f_ir = r2.arr<double>(c.nfi); c.r_mw = r2.arr<double>(c.nrm);
+  c.r_ir = r2.arr<double>(c.nri); c.temp = r2.arr<double>(c.nt); c.dens = r2.arr<double>(c.ndm);
+  r2.done("CloudCoeff axes");
+  const size_t nL = (size_t)c.nfm * c.nrm * c.nt, nS = (size_t)c.nfm * c.nrm * c.ndm, nI = (size_t)c.nfi * c.nri * ndi;
+  Cursor r3(rec.next());
+  c.ke_l = r3.arr<double>(nL); c.w_l = r3.arr<double>(nL); r3.arr<double>(nL);
+  c.pc_l = r3.arr<double>(nL * (c.nl + 1) * c.np);
+  r3.done("CloudCoeff microwave liquid");
+  Cursor r4(rec.next());
+  c.ke_s = r4.arr<double>(nS); c.w_s = r4.arr<double>(nS); r4.arr<double>(nS);
+  c.pc_s = r4.arr<double>(nS * (c.nl + 1) * c.np);
+  r4.done("CloudCoeff microwave solid");
+  Cursor r5(rec.next());
+  c.ke_i = r5.arr<double>(nI); c.w_i = r5.arr<double>(nI); r5.arr<double>(nI);
+  c.pc_i = r5.arr<double>(nI * c.nl * c.np);
+  r5.done("CloudCoeff infrared");
+  // phase element 1 only: the leading block of each pcoeff array
+  c.pc_l.resize(nL * (c.nl + 1));
+  c.pc_s.resize(nS * (c.nl + 1));
+  c.pc_i.resize(nI * c.nl);
+  return c;
+}
+
+// Gauss-Legendre nodes and weights on [0,1], ascending, for 1..9 points
+// (Double_Gauss_Quadrature, src/CRTM_Utility/CRTM_Utility.f90:343-378);
+// layout [n][2][10].
+std::vector<double> gauss_table() {
+  std::vector<double> g(10 * 20, 0.0);
+  for (int n = 1; n <= 9; ++n) {
+    double* mu = &g[n * 20];
+    double* w = mu + 10;
+    for (int j = 1; j <= (n + 1) / 2; ++j) {
+      double x = std::cos(M_PI * (j - 0.25) / (n + 0.5)), dp = 0.0;
+      for (int it = 0; it < 10; ++it) {
+        double p1 = 1.0, p = x;
+        for (int l = 2; l <= n; ++l) {
+          const double p2 = p1;
+          p1 = p;
+          p = ((2 * l - 1) * x * p1 - (l - 1) * p2) / l;
+        }
+        dp = n * (x * p - p1) / (x * x - 1.0);
+        const double xp = x;
+        x = xp - p / dp;
+        if (std::fabs(x - xp) <= 3.0e-14) break;
+      }
+      mu[j - 1] = (1.0 - x) / 2.0;
+      mu[n - j] = (1.0 + x) / 2.0;
+      w[j - 1] = w[n - j] = 1.0 / ((1.0 - x * x) * dp * dp);
+    }
+  }
+  return g;
 }
 
 }  // namespace
@@ -543,10 +630,15 @@ struct crtm_context {
   DevBuf stage;  // host-mode staging of inputs/outputs
   DevBuf clim;   // model climatology table (AddLayers)
   DevBuf xnl, xloff, xnx, xlp, xp, xt, xa;   // profiles extended by AddLayers
+  // clouds: CloudCoeff tables (loaded at init when present) and per-call work
+  bool cloud_loaded = false;
+  std::vector<std::unique_ptr<DevBuf>> cloud_bufs;
+  DevCloudCoeff cc{};
+  DevBuf cwc, creff, ccount, cslot, cdesc, ctotal, ctcc, cfrac, cmaxr, ctau, cscr;
   // kernel timing (crtm_profile_enable): HIP events on the context stream
   bool profiling = false;
   std::vector<hipEvent_t> ev;      // pairs (start, stop) per recorded launch
-  std::vector<int> ev_kind;        // 0 = preparation kernel, 1 = forward/K kernel
+  std::vector<int> ev_kind;        // 0 = preparation kernel, 1 = forward/K kernel, 2 = scattering kernels
   size_t ev_used = 0;
   hipEvent_t mark() {
     if (ev_used == ev.size()) {
@@ -597,7 +689,7 @@ ProfWork<RT> make_work(crtm_context* c, int M, int Ko, int nx, int max_k) {
 }
 
 template <class RT, class KT, bool DOK>
-void run_sensor(crtm_context* c, Sensor& S, const DevBatch& b, const DevOut& o) {
+ProfWork<RT> run_sensor(crtm_context* c, Sensor& S, const DevBatch& b, const DevOut& o) {
   const int nx = S.d.group == 1 ? 89 : S.d.group == 2 ? 50 : 21;
   ProfWork<RT> w = make_work<RT>(c, b.M, S.d.Ko, nx, b.max_k);
   DevBatch bb = b;
@@ -630,6 +722,79 @@ void run_sensor(crtm_context* c, Sensor& S, const DevBatch& b, const DevOut& o) 
       timed(1, [&] { launch_fwdk<3, RT, KT, DOK>(S.d, bb, w, o, c->stream); });
       break;
   }
+  HIPCHECK(hipGetLastError());
+  return w;
+}
+
+// Cloud preparation for the whole batch (sensor independent); returns the
+// number of cloudy layers (one host synchronisation).
+int prepare_clouds(crtm_context* c, const DevBatch& be, const DevClouds& cl, CloudWork& cw) {
+  const int M = be.M;
+  const int64_t NTe = be.NT;
+  c->cwc.reserve(std::max<size_t>((size_t)cl.nc * NTe * sizeof(double), 16));
+  c->creff.reserve(std::max<size_t>((size_t)cl.nc * NTe * sizeof(double), 16));
+  c->ccount.reserve(M * sizeof(int));
+  c->cslot.reserve(NTe * sizeof(int));
+  c->cdesc.reserve(std::max<int64_t>(be.out_NT, 1) * sizeof(int2));
+  c->ctotal.reserve(sizeof(int));
+  c->ctcc.reserve(M * sizeof(double));
+  c->cfrac.reserve(M * sizeof(int));
+  c->cmaxr.reserve(M * sizeof(double));
+  cw.wc = c->cwc.as<double>();
+  cw.reff = c->creff.as<double>();
+  cw.cl_count = c->ccount.as<int>();
+  cw.cl_slot = c->cslot.as<int>();
+  cw.cl_desc = c->cdesc.as<int2>();
+  cw.total = c->ctotal.as<int>();
+  cw.tcc = c->ctcc.as<double>();
+  cw.frac = c->cfrac.as<int>();
+  cw.max_reff = c->cmaxr.as<double>();
+  cw.cap = (int)std::min<int64_t>(be.out_NT, INT32_MAX);
+  HIPCHECK(hipMemsetAsync(cw.total, 0, sizeof(int), c->stream));
+  launch_cloud_prep(be, cl, cw, c->stream);
+  HIPCHECK(hipGetLastError());
+  int total = 0;
+  HIPCHECK(hipMemcpyAsync(&total, cw.total, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  HIPCHECK(hipStreamSynchronize(c->stream));
+  return std::min(total, cw.cap);
+}
+
+// Angle counts the layer and sweep kernels are launched for: the stream
+// choice gives n quadrature angles plus the sensor angle (n alone when the
+// sensor angle coincides with a node).
+std::vector<int> angle_counts(int n_streams) {
+  if (n_streams > 0) {
+    const int ns = (n_streams + 2) / 2;
+    return {ns, ns + 1};
+  }
+  return {2, 3, 4, 5};   // Mie-parameter choice: 2, 4 or 6 streams
+}
+
+template <class RT>
+void run_allsky(crtm_context* c, Sensor& S, const DevBatch& be, const DevClouds& cl, const CloudWork& cw0,
+                int total_cl, const ProfWork<RT>& w, const DevOut& o, const AllskyOut& ao) {
+  CloudWork cw = cw0;
+  cw.status = w.status;
+  const int64_t items = (int64_t)total_cl * S.d.L;
+  if (items > INT32_MAX) throw CrtmError(fmt("%lld cloudy layer-channels exceed one launch", (long long)items));
+  const std::vector<int> nz = angle_counts(cl.n_streams);
+  LayerScratch<RT> sc{};
+  sc.nzc = *std::max_element(nz.begin(), nz.end());
+  sc.n_items = std::max<int64_t>(items, 1);
+  c->cscr.reserve((size_t)sc.n_elem() * sc.n_items * sizeof(RT));
+  sc.v = c->cscr.as<RT>();
+  auto timed = [&](auto&& fn) {
+    if (c->profiling) {
+      c->mark();
+      c->ev_kind.push_back(2);
+    }
+    fn();
+    if (c->profiling) c->mark();
+  };
+  timed([&] {
+    launch_allsky<RT>(S.d, be, cl, cw, c->cc, sc, reinterpret_cast<const RT*>(o.tau_g), w.geo, (int)items,
+                      nz.data(), (int)nz.size(), o, ao, c->stream);
+  });
   HIPCHECK(hipGetLastError());
 }
 
@@ -677,6 +842,40 @@ DevBatch add_layers(crtm_context* c, const DevBatch& b, const int* user_ids) {
   return be;
 }
 
+// All sensors of the context: clear-sky kernels, then the scattering path
+// for the cloudy profiles.
+struct CloudCall {
+  bool on = false;
+  DevClouds cl{};
+  CloudWork cw{};
+  int total = 0;
+  AllskyOut ao{};
+};
+
+template <class RT, class KT, bool DOK>
+void run_sensors(crtm_context* c, const DevBatch& be, DevOut o, const CloudCall& cc) {
+  for (auto& S : c->sensors) {
+    if (cc.on) {
+      c->ctau.reserve(std::max<size_t>((size_t)be.NT * S->d.L * sizeof(RT), 16));
+      o.tau_g = c->ctau.p;
+      o.tau_mask = cc.cw.cl_count;
+    }
+    ProfWork<RT> w = run_sensor<RT, KT, DOK>(c, *S, be, o);
+    if (cc.on) run_allsky<RT>(c, *S, be, cc.cl, cc.cw, cc.total, w, o, cc.ao);
+    else launch_clear_copy(S->d, be.M, be.Ltot, o, cc.ao, c->stream);   // R_Clear = Radiance without clouds
+  }
+}
+
+void dispatch(crtm_context* c, const DevBatch& be, const DevOut& o, const CloudCall& cc, bool split, bool kf32,
+              bool dok) {
+  if (dok) {
+    if (split) { if (kf32) run_sensors<float, float, true>(c, be, o, cc); else run_sensors<float, double, true>(c, be, o, cc); }
+    else { if (kf32) run_sensors<double, float, true>(c, be, o, cc); else run_sensors<double, double, true>(c, be, o, cc); }
+  } else {
+    if (split) run_sensors<float, float, false>(c, be, o, cc); else run_sensors<double, double, false>(c, be, o, cc);
+  }
+}
+
 int do_call(crtm_context* ctx, const crtm_atmosphere* atm, const crtm_surface* sfc, const crtm_geometry* geo,
             const crtm_options* opt, crtm_rtsolution* rts, crtm_k_output* k) {
   if (!ctx) return fail(nullptr, "null context");
@@ -688,6 +887,23 @@ int do_call(crtm_context* ctx, const crtm_atmosphere* atm, const crtm_surface* s
   if (atm->n_absorbers < 1 || atm->n_absorbers > NA_MAX)
     return fail(ctx, fmt("n_absorbers %d outside 1..%d", atm->n_absorbers, NA_MAX));
   if (!rts->radiance || !rts->brightness_temperature) return fail(ctx, "RTSolution radiance/BT arrays required");
+  CloudCall cc;
+  cc.on = atm->n_clouds > 0;
+  if (cc.on) {
+    if (!ctx->cloud_loaded)
+      return fail(ctx, "The CloudCoeff data must be loaded (CloudCoeff.bin in the crtm_init path) for cloudy profiles");
+    if (!atm->cloud_type || !atm->cloud_water_content || !atm->cloud_effective_radius)
+      return fail(ctx, "cloud type, water content and effective radius arrays required");
+    if (opt->n_streams != 0 && opt->n_streams != 2 && opt->n_streams != 4 && opt->n_streams != 6 &&
+        opt->n_streams != 8)
+      return fail(ctx, fmt("n_streams %d: the scattering kernels take 2, 4, 6 or 8 streams (0 = automatic)",
+                           opt->n_streams));
+    if (opt->overlap_id < 0 || opt->overlap_id > 5) return fail(ctx, "Invalid overlap assumption");
+    if (k) return fail(ctx, "the K-matrix of cloudy profiles is not part of this build yet");
+    cc.cl.nc = atm->n_clouds;
+    cc.cl.n_streams = opt->n_streams;
+    cc.cl.overlap = opt->overlap_id > 0 ? opt->overlap_id : 4;
+  }
   const bool host = opt->memory == CRTM_MEM_HOST;
   const bool split = opt->precision == CRTM_PREC_FP32_SPLIT;
   const bool kf32 = opt->jacobian_f32 != 0;
@@ -757,6 +973,16 @@ int do_call(crtm_context* ctx, const crtm_atmosphere* atm, const crtm_surface* s
     in(atm->temperature, NT * sizeof(double), (const void**)&b.t);
     in(atm->absorber, na * NT * sizeof(double), (const void**)&b.a);
     in(atm->climatology, M * sizeof(int), (const void**)&b.climatology);
+    if (cc.on) {
+      const size_t nc = atm->n_clouds;
+      in(atm->cloud_type, M * nc * sizeof(int), (const void**)&cc.cl.type);
+      in(atm->cloud_water_content, nc * NT * sizeof(double), (const void**)&cc.cl.wc);
+      in(atm->cloud_effective_radius, nc * NT * sizeof(double), (const void**)&cc.cl.reff);
+      in(atm->cloud_fraction, NT * sizeof(double), (const void**)&cc.cl.cf);
+      for (size_t i = 0; i < (size_t)M * nc; ++i)
+        if (atm->cloud_type[i] < 0 || atm->cloud_type[i] > 6)
+          return fail(ctx, fmt("profile #%d: invalid cloud type %d", (int)(i / nc) + 1, atm->cloud_type[i]));
+    }
     const double* cov[4] = {sfc->land_coverage, sfc->water_coverage, sfc->snow_coverage, sfc->ice_coverage};
     const double* st[4] = {sfc->land_temperature, sfc->water_temperature, sfc->snow_temperature, sfc->ice_temperature};
     for (int c = 0; c < 4; ++c) {
@@ -780,6 +1006,9 @@ int do_call(crtm_context* ctx, const crtm_atmosphere* atm, const crtm_surface* s
     out(rts->down_radiance, ML, (void**)&o.down_radiance);
     out(rts->surface_planck_radiance, ML, (void**)&o.sfc_planck);
     out(rts->surface_emissivity, ML, (void**)&o.sfc_emis);
+    out(rts->r_clear, ML, (void**)&cc.ao.r_clear);
+    out(rts->tb_clear, ML, (void**)&cc.ao.tb_clear);
+    out(rts->total_cloud_cover, ML, (void**)&cc.ao.tcc);
     if (dok) {
       out(k->radiance_k, ML, (void**)&o.rad_k);
       out(k->surface_emissivity_k, ML, (void**)&o.emis_k);
@@ -809,19 +1038,13 @@ int do_call(crtm_context* ctx, const crtm_atmosphere* atm, const crtm_surface* s
     }
     if (dok && !k->radiance_k) o.rad_k = nullptr;
     const DevBatch be = add_layers(ctx, b, abs_id.data());
-    for (auto& S : ctx->sensors) {
-      if (dok) {
-        if (split) { if (kf32) run_sensor<float, float, true>(ctx, *S, be, o); else run_sensor<float, double, true>(ctx, *S, be, o); }
-        else { if (kf32) run_sensor<double, float, true>(ctx, *S, be, o); else run_sensor<double, double, true>(ctx, *S, be, o); }
-      } else {
-        if (split) run_sensor<float, float, false>(ctx, *S, be, o); else run_sensor<double, double, false>(ctx, *S, be, o);
-      }
-      if (rts->status) {
-        std::vector<int> st(M);
-        HIPCHECK(hipMemcpyAsync(st.data(), ctx->wstatus.p, M * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
-        HIPCHECK(hipStreamSynchronize(ctx->stream));
-        for (int m = 0; m < M; ++m) rts->status[m] = std::max(rts->status[m], st[m] ? CRTM_FAILURE : CRTM_SUCCESS);
-      }
+    if (cc.on) cc.total = prepare_clouds(ctx, be, cc.cl, cc.cw);
+    dispatch(ctx, be, o, cc, split, kf32, dok);
+    if (rts->status) {
+      std::vector<int> st(M);
+      HIPCHECK(hipMemcpyAsync(st.data(), ctx->wstatus.p, M * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+      HIPCHECK(hipStreamSynchronize(ctx->stream));
+      for (int m = 0; m < M; ++m) rts->status[m] = std::max(rts->status[m], st[m] ? CRTM_FAILURE : CRTM_SUCCESS);
     }
     for (size_t i = 0; i < outs.size(); ++i)
       HIPCHECK(hipMemcpyAsync(outs[i].h, base + ooff[i], outs[i].n, hipMemcpyDeviceToHost, ctx->stream));
@@ -865,17 +1088,20 @@ int do_call(crtm_context* ctx, const crtm_atmosphere* atm, const crtm_surface* s
     o.a_k = k->absorber_k;
     if (!o.t_k || !o.a_k) return fail(ctx, "K-matrix temperature/absorber arrays required");
   }
-  const DevBatch be = add_layers(ctx, b, abs_id.data());
-  for (auto& S : ctx->sensors) {
-    if (dok) {
-      if (split) { if (kf32) run_sensor<float, float, true>(ctx, *S, be, o); else run_sensor<float, double, true>(ctx, *S, be, o); }
-      else { if (kf32) run_sensor<double, float, true>(ctx, *S, be, o); else run_sensor<double, double, true>(ctx, *S, be, o); }
-    } else {
-      if (split) run_sensor<float, float, false>(ctx, *S, be, o); else run_sensor<double, double, false>(ctx, *S, be, o);
-    }
-    if (rts->status)
-      HIPCHECK(hipMemcpyAsync(rts->status, ctx->wstatus.p, M * sizeof(int), hipMemcpyDeviceToDevice, ctx->stream));
+  cc.ao.r_clear = rts->r_clear;
+  cc.ao.tb_clear = rts->tb_clear;
+  cc.ao.tcc = rts->total_cloud_cover;
+  if (cc.on) {
+    cc.cl.type = atm->cloud_type;
+    cc.cl.wc = atm->cloud_water_content;
+    cc.cl.reff = atm->cloud_effective_radius;
+    cc.cl.cf = atm->cloud_fraction;
   }
+  const DevBatch be = add_layers(ctx, b, abs_id.data());
+  if (cc.on) cc.total = prepare_clouds(ctx, be, cc.cl, cc.cw);
+  dispatch(ctx, be, o, cc, split, kf32, dok);
+  if (rts->status)
+    HIPCHECK(hipMemcpyAsync(rts->status, ctx->wstatus.p, M * sizeof(int), hipMemcpyDeviceToDevice, ctx->stream));
   return CRTM_SUCCESS;
 }
 
@@ -907,6 +1133,24 @@ int crtm_init(int n_sensors, const char* const* sensor_ids, const char* file_pat
     }
     ctx->n_channels = off;
     upload(ctx->clim, std::vector<double>(CLIM_TABLE, CLIM_TABLE + CLIM_SIZE));
+    const std::string ccpath = dir + "CloudCoeff.bin";
+    if (std::ifstream(ccpath).good()) {
+      const CloudCoeffH h = read_cloudcoeff(ccpath);
+      auto put = [&](const std::vector<double>& v) {
+        ctx->cloud_bufs.emplace_back(new DevBuf);
+        upload(*ctx->cloud_bufs.back(), v);
+        return ctx->cloud_bufs.back()->as<double>();
+      };
+      DevCloudCoeff& c = ctx->cc;
+      c.nfm = h.nfm; c.nrm = h.nrm; c.nt = h.nt; c.ndm = h.ndm; c.nfi = h.nfi; c.nri = h.nri; c.ndi = h.ndm + 1;
+      c.nlf = h.nl;
+      c.f_mw = put(h.f_mw); c.r_mw = put(h.r_mw); c.temp = put(h.temp); c.f_ir = put(h.f_ir); c.r_ir = put(h.r_ir);
+      c.ke_l = put(h.ke_l); c.w_l = put(h.w_l); c.pc_l = put(h.pc_l);
+      c.ke_s = put(h.ke_s); c.w_s = put(h.w_s); c.pc_s = put(h.pc_s);
+      c.ke_i = put(h.ke_i); c.w_i = put(h.w_i); c.pc_i = put(h.pc_i);
+      c.gauss = put(gauss_table());
+      ctx->cloud_loaded = true;
+    }
     HIPCHECK(hipDeviceSynchronize());
   } catch (const std::exception& e) {
     std::fprintf(stderr, "CRTM_MI355X crtm_init: %s\n", e.what());
@@ -994,23 +1238,35 @@ int crtm_profile_enable(crtm_context* ctx, int on) {
   return CRTM_SUCCESS;
 }
 
-int crtm_profile_query(crtm_context* ctx, double* prep_ms, double* main_ms, int* n_launches) {
+int crtm_profile_query_kinds(crtm_context* ctx, double* ms, int* n_launches, int n_kinds) {
   if (!ctx) return CRTM_FAILURE;
   if (hipStreamSynchronize(ctx->stream) != hipSuccess) return fail(ctx, "stream synchronize failed");
-  double t[2] = {0.0, 0.0};
-  int n[2] = {0, 0};
+  double t[3] = {0.0, 0.0, 0.0};
+  int n[3] = {0, 0, 0};
   for (size_t i = 0; i < ctx->ev_kind.size(); ++i) {
-    float ms = 0.f;
-    if (hipEventElapsedTime(&ms, ctx->ev[2 * i], ctx->ev[2 * i + 1]) != hipSuccess)
+    float e = 0.f;
+    if (hipEventElapsedTime(&e, ctx->ev[2 * i], ctx->ev[2 * i + 1]) != hipSuccess)
       return fail(ctx, "hipEventElapsedTime failed");
-    t[ctx->ev_kind[i]] += ms;
+    t[ctx->ev_kind[i]] += e;
     n[ctx->ev_kind[i]] += 1;
   }
+  for (int q = 0; q < n_kinds && q < 3; ++q) {
+    if (ms) ms[q] = t[q];
+    if (n_launches) n_launches[q] = n[q];
+  }
+  ctx->ev_used = 0;
+  ctx->ev_kind.clear();
+  return CRTM_SUCCESS;
+}
+
+int crtm_profile_query(crtm_context* ctx, double* prep_ms, double* main_ms, int* n_launches) {
+  double t[3];
+  int n[3];
+  const int rc = crtm_profile_query_kinds(ctx, t, n, 3);
+  if (rc != CRTM_SUCCESS) return rc;
   if (prep_ms) *prep_ms = t[0];
   if (main_ms) *main_ms = t[1];
   if (n_launches) *n_launches = n[1];
-  ctx->ev_used = 0;
-  ctx->ev_kind.clear();
   return CRTM_SUCCESS;
 }
 

@@ -20,6 +20,22 @@ constexpr int N_OPTRAN_PRED = 14;
 constexpr int OPZ = 15;            // OPTRAN predictor rows incl. constant row 0
 constexpr int OPO = 11;            // OPTRAN polynomial orders 0..10
 
+template <class RT> __device__ inline RT rexp(RT x);
+template <> __device__ inline float rexp<float>(float x) { return __expf(x); }
+template <> __device__ inline double rexp<double>(double x) { return exp(x); }
+
+// Planck radiance B = c1 / (exp(c2/Te) - 1), Te = bc1 + bc2*T
+// (CRTM_Planck_Functions.f90:116-136); expm1 keeps the microwave case
+// (c2/Te ~ 0.01) accurate in FP32.
+__device__ inline float planck(float c1, float c2, float bc1, float bc2, double t, float& te) {
+  te = bc1 + bc2 * (float)t;
+  return __fdividef(c1, expm1f(__fdividef(c2, te)));
+}
+__device__ inline double planck(double c1, double c2, double bc1, double bc2, double t, double& te) {
+  te = bc1 + bc2 * t;
+  return c1 / (exp(c2 / te) - 1.0);
+}
+
 // ODPS predictor groups (src/AtmAbsorption/ODPS/ODPS_Predictor.f90:85-117)
 template <int G> struct GT;
 template <> struct GT<1> {
@@ -109,6 +125,9 @@ struct DevSensor {
   const double* bc1;
   const double* bc2;
   const double* cosmic;
+  const double* frequency;  // GHz (microwave cloud tables)
+  const double* wavenumber; // cm^-1 (infrared cloud tables, stream choice)
+  int sensor_type;          // 1 microwave, 2 infrared, ...
 };
 
 // blk_meta layout
@@ -192,6 +211,86 @@ struct DevOut {
   void* t_k;               // [NT][Ltot]
   void* a_k;               // [na][NT][Ltot]
   int k_f32;
+  // all-sky: nadir gas OD per layer of the profiles with clouds,
+  // [NT (extended layout)][L of the sensor], precision of the call
+  void* tau_g;
+  const int* tau_mask;     // [M]: write the profile's ODs when > 0
+};
+
+// ---------------------------------------------------------------------------
+// All-sky (clouds, ADA)
+// ---------------------------------------------------------------------------
+constexpr int NZ_AS = 6;           // largest angle count the all-sky kernels are built for (8 streams)
+constexpr int NL_AS = 9;           // Legendre terms 0..8 carried per layer
+
+// CloudCoeff tables (src/Coefficients/CloudCoeff/CloudCoeff_Define.f90:95-154),
+// column-major as in the file, phase element 1 only
+struct DevCloudCoeff {
+  int nfm, nrm, nt, ndm, nfi, nri, ndi, nlf;   // nlf: Legendre terms of the file
+  const double* f_mw;
+  const double* r_mw;
+  const double* temp;
+  const double* f_ir;
+  const double* r_ir;
+  const double* ke_l;   // (f, r, T)
+  const double* w_l;
+  const double* pc_l;   // (f, r, T, 0:nlf)
+  const double* ke_s;   // (f, r, D)
+  const double* w_s;
+  const double* pc_s;   // (f, r, D, 0:nlf)
+  const double* ke_i;   // (f, r, D+1)
+  const double* w_i;
+  const double* pc_i;   // (f, r, D+1, 1:nlf)
+  const double* gauss;  // [10][2][10]: nodes and weights on [0,1] for n = 0..9 streams
+};
+
+// Cloud inputs of a batch (caller's layer layout)
+struct DevClouds {
+  int nc;               // cloud slots per profile
+  const int* type;      // [M][nc]
+  const double* wc;     // [nc][out_NT] kg/m^2
+  const double* reff;   // [nc][out_NT] microns
+  const double* cf;     // [out_NT] or null
+  int n_streams;        // Options%n_Streams (0: chosen from the Mie parameter)
+  int overlap;          // Options%Overlap_Id 1..5
+};
+
+// Per-call cloud work (extended layer layout)
+struct CloudWork {
+  double* wc;           // [nc][NT] cleaned, cover-scaled water content
+  double* reff;         // [nc][NT]
+  int* cl_count;        // [M] cloudy layers of profile m
+  int* cl_slot;         // [NT] cloudy-layer id of (m, k), -1 for clear layers
+  int2* cl_desc;        // [cap] (m, k) of cloudy layer c
+  int* total;           // [1] cloudy layers in the batch
+  double* tcc;          // [M] total cloud cover of fractional profiles (else 0)
+  int* frac;            // [M] 1: fractional coverage
+  double* max_reff;     // [M] largest effective radius (stream choice)
+  int cap;
+  const int* status;    // [M] profile status of the preparation kernel (0 = good)
+};
+
+// Per-(cloudy layer, channel) layer results, structure of arrays
+// [element][item], item = c * L + l
+template <class RT> struct LayerScratch {
+  RT* v;
+  int64_t n_items;
+  int nzc;              // matrix size the slots are laid out for
+  // element offsets: R [nzc*nzc], T [nzc*nzc], e = 1 - Thermal_C [nzc],
+  // delta-scaled tau and ssa, phase coefficients P_0..P_{NL_AS-2}
+  __host__ __device__ int oR() const { return 0; }
+  __host__ __device__ int oT() const { return nzc * nzc; }
+  __host__ __device__ int oE() const { return 2 * nzc * nzc; }
+  __host__ __device__ int oTau() const { return 2 * nzc * nzc + nzc; }
+  __host__ __device__ int oSsa() const { return oTau() + 1; }
+  __host__ __device__ int oP() const { return oTau() + 2; }
+  __host__ __device__ int n_elem() const { return oP() + NL_AS - 1; }
+};
+
+struct AllskyOut {
+  double* r_clear;      // [M][Ltot] optional
+  double* tb_clear;
+  double* tcc;
 };
 
 }  // namespace crtm

@@ -2,7 +2,7 @@
 !
 ! A thin ISO_C_BINDING layer over libcrtm_mi355x.so (include/crtm_mi355x.h)
 ! that keeps the names, argument order and result codes of CRTM's public
-! API for the clear-sky forward / K-matrix path, so a caller written against
+! API for the forward / K-matrix path (clear and cloudy), so a caller written against
 ! CRTM_Module (src/CRTM_Module.F90:7-38) compiles unchanged for that path:
 !
 !   CRTM_Init(Sensor_ID, ChannelInfo, File_Path)          src/CRTM_LifeCycle.f90:520
@@ -40,6 +40,16 @@ MODULE CRTM_MI355X_Module
     TYPE(C_PTR) :: ctx = C_NULL_PTR      ! shared by all sensors of one CRTM_Init call
   END TYPE
 
+  ! CRTM_Cloud_type (src/Atmosphere/Cloud/CRTM_Cloud_Define.f90:295-310)
+  TYPE, PUBLIC :: CRTM_Cloud_type
+    INTEGER :: n_Layers = 0
+    INTEGER :: Type = 0                            ! WATER_CLOUD .. HAIL_CLOUD
+    REAL(fp), ALLOCATABLE :: Effective_Radius(:)   ! microns
+    REAL(fp), ALLOCATABLE :: Water_Content(:)      ! kg/m^2
+  END TYPE
+  INTEGER, PARAMETER, PUBLIC :: WATER_CLOUD = 1, ICE_CLOUD = 2, RAIN_CLOUD = 3, SNOW_CLOUD = 4, &
+                                GRAUPEL_CLOUD = 5, HAIL_CLOUD = 6
+
   TYPE, PUBLIC :: CRTM_Atmosphere_type
     INTEGER :: n_Layers = 0, n_Absorbers = 0, n_Clouds = 0, n_Aerosols = 0
     INTEGER :: Climatology = 6                     ! US_STANDARD_ATMOSPHERE
@@ -47,6 +57,8 @@ MODULE CRTM_MI355X_Module
     REAL(fp), ALLOCATABLE :: Level_Pressure(:)     ! (0:n_Layers)
     REAL(fp), ALLOCATABLE :: Pressure(:), Temperature(:)
     REAL(fp), ALLOCATABLE :: Absorber(:, :)        ! (n_Layers, n_Absorbers)
+    REAL(fp), ALLOCATABLE :: Cloud_Fraction(:)     ! (n_Layers)
+    TYPE(CRTM_Cloud_type), ALLOCATABLE :: Cloud(:) ! (n_Clouds)
   END TYPE
 
   TYPE, PUBLIC :: CRTM_Surface_type
@@ -61,12 +73,16 @@ MODULE CRTM_MI355X_Module
   TYPE, PUBLIC :: CRTM_RTSolution_type
     REAL(fp) :: Radiance = 0, Brightness_Temperature = 0
     REAL(fp) :: Up_Radiance = 0, Down_Radiance = 0, Surface_Planck_Radiance = 0, Surface_Emissivity = 0
+    REAL(fp) :: R_Clear = 0, Tb_Clear = 0, Total_Cloud_Cover = 0
   END TYPE
 
   TYPE, PUBLIC :: CRTM_Options_type
     LOGICAL :: Use_Emissivity = .FALSE.
     REAL(fp), ALLOCATABLE :: Emissivity(:)         ! (n_Channels)
     LOGICAL :: Check_Input = .TRUE.
+    LOGICAL :: Use_N_Streams = .FALSE.
+    INTEGER :: n_Streams = 0
+    INTEGER :: Overlap_Id = 4                      ! average overlap (CRTM_CloudCover_Define.f90:79)
     INTEGER :: Precision = CRTM_PREC_FP64          ! extension: arithmetic of the GPU path
   END TYPE
 
@@ -81,6 +97,8 @@ MODULE CRTM_MI355X_Module
     INTEGER(C_INT) :: n_absorbers
     TYPE(C_PTR) :: absorber_id, absorber_units, level_pressure, pressure, temperature, absorber
     TYPE(C_PTR) :: climatology
+    INTEGER(C_INT) :: n_clouds
+    TYPE(C_PTR) :: cloud_type, cloud_water_content, cloud_effective_radius, cloud_fraction
   END TYPE
   TYPE, BIND(C) :: c_surface
     TYPE(C_PTR) :: land_coverage, water_coverage, snow_coverage, ice_coverage, &
@@ -93,10 +111,12 @@ MODULE CRTM_MI355X_Module
     INTEGER(C_INT) :: use_emissivity
     TYPE(C_PTR) :: emissivity
     INTEGER(C_INT) :: precision, memory, jacobian_f32, check_input
+    INTEGER(C_INT) :: n_streams, overlap_id
   END TYPE
   TYPE, BIND(C) :: c_rtsolution
     TYPE(C_PTR) :: radiance, brightness_temperature, up_radiance, down_radiance, &
-                   surface_planck_radiance, surface_emissivity, status
+                   surface_planck_radiance, surface_emissivity, status, r_clear, tb_clear, &
+                   total_cloud_cover
   END TYPE
   TYPE, BIND(C) :: c_k_output
     TYPE(C_PTR) :: brightness_temperature_k, radiance_k, surface_emissivity_k, temperature_k, &
@@ -156,8 +176,9 @@ MODULE CRTM_MI355X_Module
     INTEGER(C_INT), ALLOCATABLE :: nl(:), ids(:), units(:), clim(:)
     INTEGER(C_INT64_T), ALLOCATABLE :: off(:)
     REAL(C_DOUBLE), ALLOCATABLE :: lp(:), p(:), t(:), a(:), cov(:, :), st(:, :), zen(:), scan(:), src(:), emis(:)
-    REAL(C_DOUBLE), ALLOCATABLE :: rad(:), bt(:), up(:), dn(:), sp(:), se(:)
-    INTEGER(C_INT), ALLOCATABLE :: status(:)
+    REAL(C_DOUBLE), ALLOCATABLE :: rad(:), bt(:), up(:), dn(:), sp(:), se(:), rcl(:), tbc(:), tcc(:)
+    INTEGER(C_INT), ALLOCATABLE :: status(:), ctype(:)
+    REAL(C_DOUBLE), ALLOCATABLE :: cwc(:), creff(:), cfrac(:)
     TYPE(c_atmosphere) :: atm
     TYPE(c_surface) :: sfc
     TYPE(c_geometry) :: geo
@@ -222,7 +243,7 @@ CONTAINS
   SUBROUTINE CRTM_Atmosphere_Create(Atm, n_Layers, n_Absorbers, n_Clouds, n_Aerosols)
     TYPE(CRTM_Atmosphere_type), INTENT(INOUT) :: Atm(:)
     INTEGER, INTENT(IN) :: n_Layers, n_Absorbers, n_Clouds, n_Aerosols
-    INTEGER :: m
+    INTEGER :: m, ic
     DO m = 1, SIZE(Atm)
       Atm(m)%n_Layers = n_Layers
       Atm(m)%n_Absorbers = n_Absorbers
@@ -235,6 +256,16 @@ CONTAINS
                Atm(m)%Absorber_Units(n_Absorbers))
       Atm(m)%Level_Pressure = 0; Atm(m)%Pressure = 0; Atm(m)%Temperature = 0; Atm(m)%Absorber = 0
       Atm(m)%Absorber_ID = 0; Atm(m)%Absorber_Units = 0
+      IF (ALLOCATED(Atm(m)%Cloud)) DEALLOCATE(Atm(m)%Cloud)
+      IF (ALLOCATED(Atm(m)%Cloud_Fraction)) DEALLOCATE(Atm(m)%Cloud_Fraction)
+      ALLOCATE(Atm(m)%Cloud(n_Clouds), Atm(m)%Cloud_Fraction(n_Layers))
+      Atm(m)%Cloud_Fraction = 0
+      DO ic = 1, n_Clouds
+        Atm(m)%Cloud(ic)%n_Layers = n_Layers
+        ALLOCATE(Atm(m)%Cloud(ic)%Effective_Radius(n_Layers), Atm(m)%Cloud(ic)%Water_Content(n_Layers))
+        Atm(m)%Cloud(ic)%Effective_Radius = 0
+        Atm(m)%Cloud(ic)%Water_Content = 0
+      END DO
     END DO
   END SUBROUTINE CRTM_Atmosphere_Create
 
@@ -355,7 +386,7 @@ CONTAINS
     TYPE(CRTM_Options_type), OPTIONAL, INTENT(IN) :: Opt(:)
     TYPE(batch_type), TARGET, INTENT(INOUT) :: b
     INTEGER, INTENT(OUT) :: L, M
-    INTEGER :: Error_Status, mm, NT, na, K0, K, j
+    INTEGER :: Error_Status, mm, NT, na, K0, K, j, nc
     Error_Status = FAILURE
     M = SIZE(Atm)
     L = SUM(ChannelInfo(:)%n_Channels)
@@ -374,7 +405,10 @@ CONTAINS
     NT = INT(b%off(M + 1))
     ALLOCATE(b%ids(na), b%units(na), b%lp(NT + M), b%p(NT), b%t(NT), b%a(na * NT), b%cov(M, 4), b%st(M, 4), &
              b%zen(M), b%scan(M), b%src(M), b%emis(L * M), b%rad(L * M), b%bt(L * M), b%up(L * M), &
-             b%dn(L * M), b%sp(L * M), b%se(L * M), b%status(M))
+             b%dn(L * M), b%sp(L * M), b%se(L * M), b%status(M), b%rcl(L * M), b%tbc(L * M), b%tcc(L * M))
+    nc = Atm(1)%n_Clouds
+    ALLOCATE(b%ctype(MAX(nc, 1) * M), b%cwc(MAX(nc, 1) * NT), b%creff(MAX(nc, 1) * NT), b%cfrac(NT))
+    b%ctype = 0; b%cwc = 0; b%creff = 0; b%cfrac = 0
     b%ids = Atm(1)%Absorber_ID
     b%units = Atm(1)%Absorber_Units
     DO mm = 1, M
@@ -395,6 +429,15 @@ CONTAINS
       IF (.NOT. Opt(mm)%Use_Emissivity) RETURN
       IF (SIZE(Opt(mm)%Emissivity) < L) RETURN
       b%emis((mm - 1) * L + 1:mm * L) = Opt(mm)%Emissivity(1:L)
+      IF (Atm(mm)%n_Clouds /= nc) RETURN
+      IF (nc > 0) THEN
+        b%cfrac(K0 + 1:K0 + K) = Atm(mm)%Cloud_Fraction(1:K)
+        DO j = 1, nc
+          b%ctype((mm - 1) * nc + j) = Atm(mm)%Cloud(j)%Type
+          b%cwc((j - 1) * NT + K0 + 1:(j - 1) * NT + K0 + K) = Atm(mm)%Cloud(j)%Water_Content(1:K)
+          b%creff((j - 1) * NT + K0 + 1:(j - 1) * NT + K0 + K) = Atm(mm)%Cloud(j)%Effective_Radius(1:K)
+        END DO
+      END IF
     END DO
     b%atm%n_profiles = M
     b%atm%max_layers = MAXVAL(b%nl)
@@ -409,6 +452,11 @@ CONTAINS
     b%atm%temperature = C_LOC(b%t)
     b%atm%absorber = C_LOC(b%a)
     b%atm%climatology = C_LOC(b%clim)
+    b%atm%n_clouds = nc
+    b%atm%cloud_type = C_LOC(b%ctype)
+    b%atm%cloud_water_content = C_LOC(b%cwc)
+    b%atm%cloud_effective_radius = C_LOC(b%creff)
+    b%atm%cloud_fraction = C_LOC(b%cfrac)
     b%sfc%land_coverage = C_LOC(b%cov(1, 1))
     b%sfc%water_coverage = C_LOC(b%cov(1, 2))
     b%sfc%snow_coverage = C_LOC(b%cov(1, 3))
@@ -427,6 +475,8 @@ CONTAINS
     b%opt%memory = 0
     b%opt%jacobian_f32 = 0
     b%opt%check_input = MERGE(1, 0, Opt(1)%Check_Input)
+    b%opt%n_streams = MERGE(Opt(1)%n_Streams, 0, Opt(1)%Use_N_Streams)
+    b%opt%overlap_id = Opt(1)%Overlap_Id
     b%rts%radiance = C_LOC(b%rad)
     b%rts%brightness_temperature = C_LOC(b%bt)
     b%rts%up_radiance = C_LOC(b%up)
@@ -434,6 +484,9 @@ CONTAINS
     b%rts%surface_planck_radiance = C_LOC(b%sp)
     b%rts%surface_emissivity = C_LOC(b%se)
     b%rts%status = C_LOC(b%status)
+    b%rts%r_clear = C_LOC(b%rcl)
+    b%rts%tb_clear = C_LOC(b%tbc)
+    b%rts%total_cloud_cover = C_LOC(b%tcc)
     Error_Status = SUCCESS
   END FUNCTION pack_batch
 
@@ -447,6 +500,9 @@ CONTAINS
     r%Down_Radiance = b%dn(i)
     r%Surface_Planck_Radiance = b%sp(i)
     r%Surface_Emissivity = b%se(i)
+    r%R_Clear = b%rcl(i)
+    r%Tb_Clear = b%tbc(i)
+    r%Total_Cloud_Cover = b%tcc(i)
   END SUBROUTINE unpack_rts
 
   SUBROUTINE report(ctx, who)

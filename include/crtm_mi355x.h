@@ -74,6 +74,15 @@ typedef struct {
   const int* climatology;          /* [M] model climatology 1..6 used to extend a profile
                                       whose top is below TOA_PRESSURE (CRTM_Atmosphere_AddLayers,
                                       src/Atmosphere/CRTM_Atmosphere.f90:248-400); NULL = US standard */
+  /* Clouds (Atmosphere%Cloud(n), src/Atmosphere/Cloud/CRTM_Cloud_Define.f90:295-310, and
+   * Atmosphere%Cloud_Fraction): n_clouds slots per profile; 0 = clear-sky call. */
+  int n_clouds;
+  const int* cloud_type;               /* [M][n_clouds] CRTM cloud type 1..6
+                                          (WATER, ICE, RAIN, SNOW, GRAUPEL, HAIL; 0 = empty slot) */
+  const double* cloud_water_content;   /* kg/m^2, [n_clouds][layer_offset[M]] */
+  const double* cloud_effective_radius;/* microns, [n_clouds][layer_offset[M]] */
+  const double* cloud_fraction;        /* [layer_offset[M]]; clouds of layers with a fraction below
+                                          1e-12 are ignored (src/CRTM_Forward_Module.f90:368-378) */
 } crtm_atmosphere;
 
 /* ---- Surface (src/Surface/CRTM_Surface_Define.f90:204-244): [M] each ---- */
@@ -104,6 +113,11 @@ typedef struct {
   int memory;                    /* CRTM_MEM_* */
   int jacobian_f32;              /* K outputs are float (1) or double (0) */
   int check_input;               /* Options%Check_Input */
+  int n_streams;                 /* Options%n_Streams with Use_N_Streams (2, 4, 6, 8);
+                                    0 = chosen per channel from the Mie parameter
+                                    (CRTM_Compute_nStreams, src/RTSolution/CRTM_RTSolution.f90:1204) */
+  int overlap_id;                /* Options%Overlap_Id 1..5 (maximum, random, maximum-random,
+                                    average, overcast); 0 = default (average) */
 } crtm_options;
 
 /* ---- RTSolution (src/RTSolution/CRTM_RTSolution_Define.f90:201-245) ---- */
@@ -115,6 +129,9 @@ typedef struct {
   double* surface_planck_radiance; /* [M][L], optional */
   double* surface_emissivity;      /* [M][L], optional */
   int* status;                     /* [M] per-profile status, optional */
+  double* r_clear;                 /* RTSolution%R_Clear  [M][L], optional */
+  double* tb_clear;                /* RTSolution%Tb_Clear [M][L], optional */
+  double* total_cloud_cover;       /* RTSolution%Total_Cloud_Cover [M][L], optional */
 } crtm_rtsolution;
 
 /* ---- K-matrix adjoint inputs/outputs ---- */
@@ -138,8 +155,10 @@ typedef struct {
 } crtm_k_output;
 
 /* CRTM_Init (src/CRTM_LifeCycle.f90:520): loads <id>.SpcCoeff.bin and
- * <id>.TauCoeff.bin (ODPS) for each sensor from file_path, uploads the
- * coefficients to the current GPU and creates the context stream. */
+ * <id>.TauCoeff.bin (ODPS) for each sensor from file_path, and CloudCoeff.bin
+ * (release 3, binary) when present there, uploads the coefficients to the
+ * current GPU and creates the context stream.  Cloudy calls need the
+ * CloudCoeff data (src/CRTM_Forward_Module.f90:379-386). */
 int crtm_init(int n_sensors, const char* const* sensor_ids, const char* file_path,
               crtm_context** ctx);
 
@@ -153,7 +172,12 @@ int crtm_channel_info(const crtm_context* ctx, int sensor_index, int* n_channels
                       int* sensor_channel /* [n_channels] or NULL */,
                       char* sensor_id /* >= 21 bytes or NULL */);
 
-/* CRTM_Forward (src/CRTM_Forward_Module.f90:229-245) */
+/* CRTM_Forward (src/CRTM_Forward_Module.f90:229-245).  With clouds the
+ * radiances follow the all-sky path: cloud optics, AtmOptics combination and
+ * the ADA scattering solver (or Emission when no layer scatters), combined
+ * with the clear column by the total cloud cover when any cloudy layer has a
+ * cloud fraction above 1e-6.  A cloudy device-memory call synchronises once
+ * with the host (the number of cloudy layers sizes the layer work). */
 int crtm_forward(crtm_context* ctx, const crtm_atmosphere* atm, const crtm_surface* sfc,
                  const crtm_geometry* geo, const crtm_options* opt, crtm_rtsolution* rts);
 
@@ -184,6 +208,9 @@ const char* crtm_last_error(const crtm_context* ctx);
  * last enable/query and resets them. */
 int crtm_profile_enable(crtm_context* ctx, int on);
 int crtm_profile_query(crtm_context* ctx, double* prep_ms, double* main_ms, int* n_launches);
+/* Per kind: [0] preparation, [1] forward/K-matrix, [2] all-sky scattering
+ * kernels (cloud optics, layer R/T, adding sweep); n_kinds <= 3. */
+int crtm_profile_query_kinds(crtm_context* ctx, double* ms, int* n_launches, int n_kinds);
 
 #ifdef __cplusplus
 }

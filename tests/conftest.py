@@ -28,6 +28,7 @@ def coeff_dir(tmp_path_factory):
     d = os.environ.get("CRTM_SYNTH_DIR") or str(tmp_path_factory.mktemp("coeffs"))
     for name in SMALL_SENSORS:
         synthetic.write_sensor(d, name)
+    synthetic.write_cloud_coeff(d)
     return d
 
 

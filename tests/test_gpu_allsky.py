@@ -1,0 +1,160 @@
+"""All-sky forward on the GPU against the oracle (through the C-ABI).
+
+Cloudy columns (liquid, ice, rain, snow) of the ATMS-like microwave sensor
+and the HIRS-like infrared sensor, overcast and partial cover under each
+overlap assumption, user and Mie-parameter stream counts, mixed clear and
+cloudy profiles, host and device memory.  Tolerances: FP64 path 1e-6 K
+(same algorithm, different operation order); FP32 split path 1e-2 K, the
+brightness-temperature bar of the split arithmetic.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from conftest import load_sensor
+from crtmv3_amd import api, coeffio, synthetic
+
+pytestmark = pytest.mark.gpu
+
+TOL = {api.PREC_FP64: 1e-6, api.PREC_FP32_SPLIT: 1e-2}
+
+
+@pytest.fixture(scope="module")
+def ctx_mw(coeff_dir):
+    synthetic.write_sensor(coeff_dir, "atms_syn")
+    c = api.crtm_init(["atms_syn"], coeff_dir)
+    yield c
+    c.destroy()
+
+
+@pytest.fixture(scope="module")
+def oracle_mw(coeff_dir, oracle_mod):
+    tau, spc = load_sensor(coeff_dir, "atms_syn")
+    cc = coeffio.read_cloudcoeff(synthetic.write_cloud_coeff(coeff_dir))
+    return oracle_mod.Sensor(tau, spc), oracle_mod.CloudCoeff(cc)
+
+
+def _oracle(orc, S, C, ps, eps, n_streams, overlap=0):
+    outs = [orc.run_allsky(S, C, ps, m, eps[m], n_streams=n_streams, overlap=overlap)
+            for m in range(ps.n_profiles)]
+    return {k: np.stack([o[k] for o in outs]) for k in outs[0]}
+
+
+def _emis(M, L, seed=3):
+    return np.random.default_rng(seed).uniform(0.45, 0.75, (M, L))
+
+
+@pytest.mark.parametrize("precision", [api.PREC_FP64, api.PREC_FP32_SPLIT])
+@pytest.mark.parametrize("n_streams", [4, 8])
+def test_overcast_matches_oracle(ctx_mw, oracle_mw, oracle_mod, precision, n_streams):
+    print(f"allsky overcast atms_syn M=12 n_streams={n_streams} precision={precision}", flush=True)
+    ps = synthetic.add_clouds(synthetic.make_profiles(12, 100, seed=21), seed=22)
+    eps = _emis(12, 22)
+    out = ctx_mw.forward(ps, eps, precision=precision, extras=True, n_streams=n_streams)
+    ref = _oracle(oracle_mod, *oracle_mw, ps, eps, n_streams)
+    err = np.abs(out["brightness_temperature"] - ref["bt"])
+    print(f"  max |dBT| = {err.max():.3e} K; cloud effect up to "
+          f"{np.abs(ref['bt'] - ref['tb_clear']).max():.1f} K", flush=True)
+    assert err.max() < TOL[precision]
+    np.testing.assert_allclose(out["total_cloud_cover"], ref["total_cloud_cover"], atol=1e-12)
+    np.testing.assert_allclose(out["tb_clear"], ref["tb_clear"], atol=TOL[precision])
+
+
+@pytest.mark.parametrize("overlap", [1, 2, 3, 4, 5])
+def test_partial_cover_overlaps(ctx_mw, oracle_mw, oracle_mod, overlap):
+    print(f"allsky partial cover overlap={overlap}", flush=True)
+    ps = synthetic.add_clouds(synthetic.make_profiles(8, 90, seed=31), seed=32, cover="random")
+    eps = _emis(8, 22, 4)
+    out = ctx_mw.forward(ps, eps, precision=api.PREC_FP64, extras=True, n_streams=4, overlap=overlap)
+    ref = _oracle(oracle_mod, *oracle_mw, ps, eps, 4, overlap)
+    assert np.abs(out["brightness_temperature"] - ref["bt"]).max() < 1e-6
+    np.testing.assert_allclose(out["r_clear"], ref["r_clear"], rtol=1e-10)
+    np.testing.assert_allclose(out["total_cloud_cover"], ref["total_cloud_cover"], atol=1e-12)
+
+
+@pytest.mark.parametrize("precision", [api.PREC_FP64, api.PREC_FP32_SPLIT])
+def test_mie_stream_choice_and_large_particles(ctx_mw, oracle_mw, oracle_mod, precision):
+    """n_streams = 0: 2/4/6 streams from the Mie parameter; large radii make
+    strongly scattering, optically thick layers (many doublings)."""
+    print(f"allsky auto streams precision={precision}", flush=True)
+    ps = synthetic.add_clouds(synthetic.make_profiles(10, 100, seed=41), seed=42, reff_scale=25.0,
+                              median_gm3=0.3)
+    eps = _emis(10, 22, 5)
+    out = ctx_mw.forward(ps, eps, precision=precision, n_streams=0)
+    ref = _oracle(oracle_mod, *oracle_mw, ps, eps, 0)
+    assert set(np.unique(ref["n_streams"])) - {0} <= {2, 4, 6}
+    assert np.abs(out["brightness_temperature"] - ref["bt"]).max() < TOL[precision]
+
+
+def test_mixed_clear_cloudy_batch(ctx_mw, oracle_mw, oracle_mod):
+    print("allsky mixed clear/cloudy batch, uneven layers", flush=True)
+    ks = [100, 73, 100, 61, 88, 100, 95, 70]
+    ps = synthetic.add_clouds(synthetic.make_profiles(8, ks, seed=51), seed=52, cloudy_fraction=0.5)
+    eps = _emis(8, 22, 6)
+    out = ctx_mw.forward(ps, eps, precision=api.PREC_FP64, extras=True, n_streams=8)
+    ref = _oracle(oracle_mod, *oracle_mw, ps, eps, 8)
+    assert np.abs(out["brightness_temperature"] - ref["bt"]).max() < 1e-6
+    clear = ctx_mw.forward(dataclasses_replace_no_clouds(ps), eps, precision=api.PREC_FP64)
+    cloudless = ps.cloud_water_content.reshape(4, -1).sum(0)
+    for m in range(8):
+        a, b = int(ps.layer_offset[m]), int(ps.layer_offset[m + 1])
+        if cloudless[a:b].sum() == 0:
+            np.testing.assert_array_equal(out["brightness_temperature"][m], clear["brightness_temperature"][m])
+
+
+def dataclasses_replace_no_clouds(ps):
+    import dataclasses
+    return dataclasses.replace(ps, cloud_type=None, cloud_water_content=None, cloud_effective_radius=None,
+                               cloud_fraction=None)
+
+
+def test_device_memory_matches_host(ctx_mw):
+    import torch
+    print("allsky device memory vs host", flush=True)
+    ps = synthetic.add_clouds(synthetic.make_profiles(16, 100, seed=61), seed=62)
+    eps = _emis(16, 22, 7)
+    host = ctx_mw.forward(ps, eps, precision=api.PREC_FP32_SPLIT, n_streams=4)
+    dev = ctx_mw.forward(api.device_profiles(ps), torch.as_tensor(eps, device="cuda"),
+                         precision=api.PREC_FP32_SPLIT, n_streams=4)
+    torch.cuda.synchronize()
+    np.testing.assert_array_equal(dev["brightness_temperature"].cpu().numpy(), host["brightness_temperature"])
+
+
+def test_infrared_clouds(coeff_dir, oracle_mod):
+    """Infrared tables: every cloud type scatters (2-D lookup over wavenumber and radius)."""
+    print("allsky infrared hirs_syn", flush=True)
+    tau, spc = load_sensor(coeff_dir, "hirs_syn")
+    cc = coeffio.read_cloudcoeff(synthetic.write_cloud_coeff(coeff_dir))
+    S, C = oracle_mod.Sensor(tau, spc), oracle_mod.CloudCoeff(cc)
+    ctx = api.crtm_init(["hirs_syn"], coeff_dir)
+    try:
+        ps = synthetic.add_clouds(synthetic.make_profiles(6, 100, seed=71), seed=72, median_gm3=0.005)
+        eps = np.full((6, 19), 0.97)
+        for prec in (api.PREC_FP64, api.PREC_FP32_SPLIT):
+            out = ctx.forward(ps, eps, precision=prec, n_streams=4)
+            ref = _oracle(oracle_mod, S, C, ps, eps, 4)
+            assert np.abs(out["brightness_temperature"] - ref["bt"]).max() < TOL[prec]
+    finally:
+        ctx.destroy()
+
+
+def test_cloudy_call_errors(ctx_mw, tmp_path):
+    ps = synthetic.add_clouds(synthetic.make_profiles(2, 50, seed=81), seed=82)
+    eps = _emis(2, 22)
+    with pytest.raises(api.CRTMError, match="n_streams 16"):
+        ctx_mw.forward(ps, eps, n_streams=16)
+    with pytest.raises(api.CRTMError, match="overlap"):
+        ctx_mw.forward(ps, eps, n_streams=4, overlap=9)
+    bad = synthetic.add_clouds(synthetic.make_profiles(2, 50, seed=81), seed=82)
+    bad.cloud_type = bad.cloud_type.copy()
+    bad.cloud_type[1, 0] = 9
+    with pytest.raises(api.CRTMError, match="invalid cloud type"):
+        ctx_mw.forward(bad, eps, n_streams=4)
+    synthetic.write_sensor(str(tmp_path), "atms_syn")     # no CloudCoeff.bin there
+    c2 = api.crtm_init(["atms_syn"], str(tmp_path))
+    try:
+        with pytest.raises(api.CRTMError, match="CloudCoeff"):
+            c2.forward(ps, eps, n_streams=4)
+    finally:
+        c2.destroy()
