@@ -135,9 +135,18 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 1 GiB device-to-device copy (3 times), the reference for "
                          "scaling the FETCH_SIZE / WRITE_SIZE counters to bytes")
+    ap.add_argument("--workload", choices=("clear-ir", "allsky-mw"), default="clear-ir",
+                    help="clear-ir: the headline (CrIS-like forward + K-matrix); allsky-mw: cloudy ATMS-like "
+                         "forward with the ADA scattering solver at 4 and 8 streams")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.workload == "allsky-mw":
+        if args.gpus != 1:
+            print("bench.py measures one GPU. Use --gpus 1.", file=sys.stderr)
+            sys.exit(2)
+        main_allsky(args)
         return
     if args.gpus != 1:
         print("bench.py measures one GPU: the MI355X path runs on a single device (profiles shard "
@@ -247,6 +256,97 @@ def main():
                        "parallelism": "single GPU"},
             "roofline": roof, "cpu_baseline": cpu}
     line.update(result)
+    print(json.dumps(line), flush=True)
+    ctx.destroy()
+
+
+def main_allsky(args):
+    """Cloudy microwave forward: 22-channel ATMS-like sensor, 8192 profiles x
+    100 layers with liquid, ice, rain and snow in 0-10 contiguous layers each
+    (overcast), ADA at 4 and 8 streams; FP32-split vs FP64 GPU, BT error
+    against the FP64 oracle on a sample, oracle timed on the host cores."""
+    import torch
+    from crtmv3_amd import api, coeffio, synthetic
+    from oracle import oracle as orc
+    dev = torch.device("cuda:0")
+    coeff_dir = synthetic.default_coeff_dir()
+    sensor = "atms_syn"
+    synthetic.write_sensor(coeff_dir, sensor)
+    synthetic.write_cloud_coeff(coeff_dir)
+    ctx = api.crtm_init([sensor], coeff_dir)
+    L = ctx.n_channels
+    M = args.profiles if args.profiles != 4096 else 8192
+    K = 100
+    log(f"allsky: {sensor} {L} channels, {M} cloudy profiles")
+    ps = synthetic.add_clouds(synthetic.make_profiles(M, K, seed=20261203 + 3), seed=20261203 + 4)
+    dps = api.device_profiles(ps, dev)
+    emis_h = np.tile(synthetic.default_emissivity(1, L), (M, 1))
+    emis = torch.as_tensor(emis_h, device=dev)
+    torch.cuda.synchronize()
+    by_streams = {}
+    tau = coeffio.read_odps(os.path.join(coeff_dir, f"{sensor}.TauCoeff.bin"))
+    spc = coeffio.read_spccoeff(os.path.join(coeff_dir, f"{sensor}.SpcCoeff.bin"))
+    S = orc.Sensor(tau, spc)
+    C = orc.CloudCoeff(coeffio.read_cloudcoeff(os.path.join(coeff_dir, synthetic.CLOUDCOEFF_FILE)))
+    n_sample = 256
+    sub = _subset(ps, 0, n_sample)
+    for ns in (4, 8):
+        def run(prec):
+            return ctx.forward(dps, emis, precision=prec, n_streams=ns)
+        res = {}
+        for prec, name, steps in ((api.PREC_FP32_SPLIT, "split", args.steps), (api.PREC_FP64, "fp64", args.fp64_steps)):
+            out = run(prec)
+            for _ in range(args.warmup):
+                out = run(prec)
+            ctx.synchronize()
+            ctx.profile_enable(True)
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                out = run(prec)
+            ctx.synchronize()
+            torch.cuda.synchronize()
+            t = (time.perf_counter() - t0) / steps
+            ms, nl = ctx.profile_query_kinds()
+            ctx.profile_enable(False)
+            res[name] = {"ms_per_step": t * 1e3, "profile_channels_per_s": M * L / t,
+                         "kernel_ms": {"prep": ms[0] / steps, "clear_fwd": ms[1] / steps,
+                                       "scattering": ms[2] / steps},
+                         "bt": out["brightness_temperature"].cpu().numpy()}
+            log(f"  {ns} streams {name}: {t * 1e3:.2f} ms/step  kernels {res[name]['kernel_ms']}")
+        _, bt_or = orc.forward_allsky_batch(S, C, sub, emis_h[:n_sample], n_streams=ns,
+                                            n_threads=min(16, os.cpu_count() or 1))
+        by_streams[ns] = {
+            "split": {k: v for k, v in res["split"].items() if k != "bt"},
+            "fp64": {k: v for k, v in res["fp64"].items() if k != "bt"},
+            "speedup_split_over_fp64": res["fp64"]["ms_per_step"] / res["split"]["ms_per_step"],
+            "bt_err_split_vs_fp64_K": float(np.abs(res["split"]["bt"] - res["fp64"]["bt"]).max()),
+            "bt_err_split_vs_oracle_K": float(np.abs(res["split"]["bt"][:n_sample] - bt_or).max()),
+            "bt_err_fp64_vs_oracle_K": float(np.abs(res["fp64"]["bt"][:n_sample] - bt_or).max()),
+        }
+    cpu = None
+    if not args.no_cpu_baseline:
+        threads = min(16, os.cpu_count() or 1)
+        n = 64
+        t0 = time.perf_counter()
+        orc.forward_allsky_batch(S, C, _subset(ps, 0, n), emis_h[:n], n_streams=8, n_threads=threads)
+        dt = time.perf_counter() - t0
+        n = min(M, max(n, int(n * args.cpu_seconds / max(dt, 1e-6))))
+        t0 = time.perf_counter()
+        orc.forward_allsky_batch(S, C, _subset(ps, 0, n), emis_h[:n], n_streams=8, n_threads=threads)
+        dt = time.perf_counter() - t0
+        cpu = {"value": n * L / dt, "unit": "profile-channels/s", "cores": threads, "kind": "port",
+               "sample": f"{n} of the {M} profiles x {L} channels, all-sky forward at 8 streams, FP64 oracle "
+                         f"(oracle_forward_allsky_batch), {dt:.1f} s"}
+    v = by_streams[8]["split"]["profile_channels_per_s"]
+    line = {"metric": "all-sky forward profile-channels/s, FP32-split vs FP64 GPU", "value": v,
+            "unit": "profile-channels/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": by_streams[8]["split"]["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "fp32+fp64-split",
+            "data": "synthetic (seeded cloudy profiles; synthetic ODPS/SpcCoeff/CloudCoeff files)",
+            "config": {"workload": f"all-sky microwave forward, ATMS-like 22-ch ODPS group 3, {M} profiles x "
+                                   f"{K} layers, liquid/ice/rain/snow in 0-10 layers each, ADA 4 and 8 streams",
+                       "n_profiles": M, "n_channels": L, "n_layers": K, "parallelism": "single GPU"},
+            "by_streams": by_streams, "roofline": None, "cpu_baseline": cpu}
     print(json.dumps(line), flush=True)
     ctx.destroy()
 

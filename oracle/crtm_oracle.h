@@ -145,6 +145,10 @@ typedef struct {
 int oracle_forward_allsky(const or_sensor* s, const or_cloudcoeff* cc, const or_profile* p, const or_clouds* c,
                           const double* emissivity, const or_scatter_opts* opt, or_allsky_out* o);
 
+int oracle_forward_allsky_batch(const or_sensor* s, const or_cloudcoeff* cc, int n_profiles, const or_profile* p,
+                                const or_clouds* c, const double* emissivity, const or_scatter_opts* opt,
+                                double* radiance, double* bt, int n_threads);
+
 /* White-box pieces of the all-sky path. */
 int oracle_cloud_optics(const or_cloudcoeff* cc, int microwave, double freq, int type, double reff,
                         double temperature, int n_legendre, double* ke, double* w, double* pcoeff);

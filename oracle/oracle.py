@@ -97,6 +97,10 @@ def lib():
         _lib.oracle_forward_allsky.argtypes = [ctypes.POINTER(OrSensor), ctypes.POINTER(OrCloudCoeff),
                                                ctypes.POINTER(OrProfile), ctypes.POINTER(OrClouds), _dp,
                                                ctypes.POINTER(OrScatterOpts), ctypes.POINTER(OrAllskyOut)]
+        _lib.oracle_forward_allsky_batch.argtypes = [ctypes.POINTER(OrSensor), ctypes.POINTER(OrCloudCoeff),
+                                                     ctypes.c_int, ctypes.POINTER(OrProfile),
+                                                     ctypes.POINTER(OrClouds), _dp, ctypes.POINTER(OrScatterOpts),
+                                                     _dp, _dp, ctypes.c_int]
         _lib.oracle_cloud_optics.argtypes = [ctypes.POINTER(OrCloudCoeff), ctypes.c_int, ctypes.c_double,
                                              ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int,
                                              _dp, _dp, _dp]
@@ -311,3 +315,21 @@ def run_allsky(sensor: Sensor, cloudcoeff: CloudCoeff, ps, m, emissivity, n_stre
     if rc != 0:
         raise RuntimeError(f"oracle_forward_allsky failed ({rc})")
     return out
+
+
+def forward_allsky_batch(sensor: Sensor, cloudcoeff: CloudCoeff, ps, emissivity, n_streams=0, overlap=0,
+                         n_threads=0):
+    """All profiles of a cloudy ProfileSet (OpenMP over profiles); (radiance, bt) [M, L]."""
+    M, L = ps.n_profiles, sensor.n_channels
+    keep = []
+    profs = (OrProfile * M)(*[make_profile(ps, m, keep) for m in range(M)])
+    clouds = (OrClouds * M)(*[make_clouds(ps, m, keep) or OrClouds() for m in range(M)])
+    rad, bt = np.zeros((M, L)), np.zeros((M, L))
+    eps = np.ascontiguousarray(np.broadcast_to(emissivity, (M, L)), np.float64)
+    opt = OrScatterOpts(int(n_streams), int(overlap))
+    rc = lib().oracle_forward_allsky_batch(ctypes.byref(sensor.s), ctypes.byref(cloudcoeff.c), M, profs, clouds,
+                                           _p(eps, _dp), ctypes.byref(opt), _p(rad, _dp), _p(bt, _dp),
+                                           n_threads)
+    if rc != 0:
+        raise RuntimeError(f"oracle_forward_allsky_batch failed ({rc})")
+    return rad, bt
