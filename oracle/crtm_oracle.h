@@ -141,10 +141,24 @@ typedef struct {
   int* n_streams;            /* [L] optional, Legendre terms of the scattering RT (0: Emission) */
 } or_allsky_out;
 
+typedef struct {
+  double* t_k;      /* [L][K] */
+  double* abs_k;    /* [L][nabs][K] */
+  double* wc_k;     /* [L][n_clouds][K] Atmosphere_K%Cloud(n)%Water_Content */
+  double* reff_k;   /* [L][n_clouds][K] Atmosphere_K%Cloud(n)%Effective_Radius */
+  double* cf_k;     /* [L][K] Atmosphere_K%Cloud_Fraction */
+  double* sfc_t_k;  /* [L][4] */
+  double* emis_k;   /* [L] */
+} or_allsky_k;
+
 /* CRTM_Forward with clouds for one profile, all channels. */
 int oracle_forward_allsky(const or_sensor* s, const or_cloudcoeff* cc, const or_profile* p, const or_clouds* c,
                           const double* emissivity, const or_scatter_opts* opt, or_allsky_out* o);
 
+/* CRTM_K_Matrix with clouds by complex-step differentiation (exact derivatives). */
+int oracle_k_matrix_allsky(const or_sensor* s, const or_cloudcoeff* cc, const or_profile* p, const or_clouds* c,
+                           const double* emissivity, const or_scatter_opts* opt, const double* bt_k,
+                           const double* rad_k, or_allsky_out* fo, or_allsky_k* ko);
 int oracle_forward_allsky_batch(const or_sensor* s, const or_cloudcoeff* cc, int n_profiles, const or_profile* p,
                                 const or_clouds* c, const double* emissivity, const or_scatter_opts* opt,
                                 double* radiance, double* bt, int n_threads);

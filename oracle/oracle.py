@@ -52,6 +52,10 @@ class OrScatterOpts(ctypes.Structure):
     _fields_ = [("n_streams", ctypes.c_int), ("overlap", ctypes.c_int)]
 
 
+class OrAllskyK(ctypes.Structure):
+    _fields_ = [(n, _dp) for n in ("t_k", "abs_k", "wc_k", "reff_k", "cf_k", "sfc_t_k", "emis_k")]
+
+
 class OrAllskyOut(ctypes.Structure):
     _fields_ = [(n, _dp) for n in ("radiance", "bt", "r_clear", "tb_clear", "total_cloud_cover", "sod")] + \
                [("n_streams", _ip)]
@@ -101,6 +105,10 @@ def lib():
                                                      ctypes.c_int, ctypes.POINTER(OrProfile),
                                                      ctypes.POINTER(OrClouds), _dp, ctypes.POINTER(OrScatterOpts),
                                                      _dp, _dp, ctypes.c_int]
+        _lib.oracle_k_matrix_allsky.argtypes = [ctypes.POINTER(OrSensor), ctypes.POINTER(OrCloudCoeff),
+                                                ctypes.POINTER(OrProfile), ctypes.POINTER(OrClouds), _dp,
+                                                ctypes.POINTER(OrScatterOpts), _dp, _dp,
+                                                ctypes.POINTER(OrAllskyOut), ctypes.POINTER(OrAllskyK)]
         _lib.oracle_cloud_optics.argtypes = [ctypes.POINTER(OrCloudCoeff), ctypes.c_int, ctypes.c_double,
                                              ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int,
                                              _dp, _dp, _dp]
@@ -333,3 +341,34 @@ def forward_allsky_batch(sensor: Sensor, cloudcoeff: CloudCoeff, ps, emissivity,
     if rc != 0:
         raise RuntimeError(f"oracle_forward_allsky_batch failed ({rc})")
     return rad, bt
+
+
+def k_matrix_allsky(sensor: Sensor, cloudcoeff: CloudCoeff, ps, m, emissivity, n_streams=0, overlap=0,
+                    bt_k=None, rad_k=None):
+    """CRTM_K_Matrix with clouds for profile m by complex-step differentiation."""
+    keep = []
+    p = make_profile(ps, m, keep)
+    c = make_clouds(ps, m, keep)
+    L, K, na = sensor.n_channels, p.n_layers, p.n_absorbers
+    nc = c.n_clouds if c is not None else 0
+    fwd = {k: np.zeros(L) for k in ("radiance", "bt", "r_clear", "tb_clear", "total_cloud_cover", "sod")}
+    fwd["n_streams"] = np.zeros(L, np.int32)
+    fo = OrAllskyOut()
+    for k, v in fwd.items():
+        setattr(fo, k, _p(v, _ip if k == "n_streams" else _dp))
+    kd = {"t_k": np.zeros((L, K)), "abs_k": np.zeros((L, na, K)), "wc_k": np.zeros((L, max(nc, 1), K)),
+          "reff_k": np.zeros((L, max(nc, 1), K)), "cf_k": np.zeros((L, K)), "sfc_t_k": np.zeros((L, 4)),
+          "emis_k": np.zeros(L)}
+    ko = OrAllskyK()
+    for k, v in kd.items():
+        setattr(ko, k, _p(v, _dp))
+    btk = np.ascontiguousarray(np.ones(L) if bt_k is None else bt_k, np.float64)
+    rk = np.ascontiguousarray(np.zeros(L) if rad_k is None else rad_k, np.float64)
+    opt = OrScatterOpts(int(n_streams), int(overlap))
+    eps = np.ascontiguousarray(emissivity, np.float64)
+    rc = lib().oracle_k_matrix_allsky(ctypes.byref(sensor.s), ctypes.byref(cloudcoeff.c), ctypes.byref(p),
+                                      ctypes.byref(c) if c is not None else None, _p(eps, _dp), ctypes.byref(opt),
+                                      _p(btk, _dp), _p(rk, _dp), ctypes.byref(fo), ctypes.byref(ko))
+    if rc != 0:
+        raise RuntimeError(f"oracle_k_matrix_allsky failed ({rc})")
+    return fwd, kd
