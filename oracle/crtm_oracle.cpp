@@ -1009,8 +1009,21 @@ void predictor_layer_ad(const or_sensor* s, const ProfileState<double>& st, int 
   w.tzp_ad[k] += tzp_ad;
 }
 
+// Adjoint seeds of one channel from the all-sky path (cloudy columns): the
+// clear-sky Emission adjoint is run with rad_ad_clear (the clear share of
+// the radiance adjoint under fractional cover, else 0) and the layer and
+// surface adjoints of the scattering RT are added on top.
+struct ChannelSeeds {
+  bool on = false;
+  double radiance = 0.0, bt = 0.0, rad_k = 0.0;   // all-sky radiance, BT, processed radiance adjoint
+  double rad_ad_clear = 0.0;
+  std::vector<double> tau_ad, b_ad, t_ad;          // [K] nadir OD, layer Planck and direct T adjoints
+  double emis_ad = 0.0, bs_ad = 0.0;               // surface emissivity and Planck adjoints
+};
+
 void k_matrix_profile(const or_sensor* s, const or_profile* p, const double* emissivity,
-                      const double* bt_k, const double* rad_k, or_out* o) {
+                      const double* bt_k, const double* rad_k, or_out* o,
+                      const std::vector<ChannelSeeds>* seeds = nullptr) {
   const int K = p->n_layers, L = s->n_channels, na = p->n_absorbers;
   ProfileState<double> st;
   map_and_predict<double>(s, p, p->temperature, p->absorber, st);
@@ -1020,14 +1033,16 @@ void k_matrix_profile(const or_sensor* s, const or_profile* p, const double* emi
   AdjWork w;
   for (int l = 0; l < L; ++l) {
     channel_forward<double>(s, p, p->temperature, tsfc, emissivity[l], l, st, cf);
-    o->radiance[l] = cf.radiance;
-    o->bt[l] = cf.bt;
+    const ChannelSeeds* sd = seeds ? &(*seeds)[l] : nullptr;
+    const bool ext = sd && sd->on;
+    o->radiance[l] = ext ? sd->radiance : cf.radiance;
+    o->bt[l] = ext ? sd->bt : cf.bt;
     if (o->tau) std::memcpy(o->tau + (size_t)l * K, cf.tau.data(), sizeof(double) * K);
     if (o->up_radiance) o->up_radiance[l] = cf.up_radiance;
     if (o->down_radiance) o->down_radiance[l] = cf.rd[K];
     // Pre_Process_RTSolution_K (src/CRTM_K_Matrix_Module.f90:1725-1761)
-    double rad_ad = rad_k[l] + planck_t_ad_factor(s, l, cf.radiance) * bt_k[l];
-    if (o->radiance_k) o->radiance_k[l] = rad_ad;
+    double rad_ad = ext ? sd->rad_ad_clear : rad_k[l] + planck_t_ad_factor(s, l, cf.radiance) * bt_k[l];
+    if (o->radiance_k) o->radiance_k[l] = ext ? sd->rad_k : rad_ad;
     // CRTM_Emission_AD (Emission_Module.f90:280-395)
     double u = st.cos_sensor, sec_down = 1.0 / u, eps = emissivity[l];
     w.tau_ad.assign(K, 0.0);
@@ -1051,10 +1066,20 @@ void k_matrix_profile(const or_sensor* s, const or_profile* p, const double* emi
       down_ad *= td;
       w.tau_ad[k - 1] -= a_ad * sec_down;
     }
+    if (ext) {
+      for (int k = 0; k < K; ++k) {
+        w.tau_ad[k] += sd->tau_ad[k];
+        w.b_ad[k] += sd->b_ad[k];
+      }
+      bs_ad += sd->bs_ad;
+      eps_ad += sd->emis_ad;
+    }
     // Planck AD (Assign_Common_Output_AD, Common_RTSolution.f90:1680-1700)
     double* tk = o->t_k ? o->t_k + (size_t)l * K : nullptr;
     double* ak = o->abs_k ? o->abs_k + (size_t)l * na * K : nullptr;
-    if (tk) for (int k = 0; k < K; ++k) tk[k] = planck_r_ad_factor(s, l, p->temperature[k]) * w.b_ad[k];
+    if (tk)
+      for (int k = 0; k < K; ++k)
+        tk[k] = planck_r_ad_factor(s, l, p->temperature[k]) * w.b_ad[k] + (ext ? sd->t_ad[k] : 0.0);
     if (ak) for (int i = 0; i < na * K; ++i) ak[i] = 0.0;
     double sfct_ad = planck_r_ad_factor(s, l, tsfc) * bs_ad;
     if (o->sfc_t_k) for (int c = 0; c < 4; ++c) o->sfc_t_k[(size_t)l * 4 + c] = p->coverage[c] * sfct_ad;
@@ -1286,3 +1311,4 @@ extern "C" int oracle_k_matrix_batch(const or_sensor* s, int n_profiles, const o
 }
 
 #include "crtm_oracle_allsky.inc"
+#include "crtm_oracle_allsky_ad.inc"

@@ -109,6 +109,7 @@ def lib():
                                                 ctypes.POINTER(OrProfile), ctypes.POINTER(OrClouds), _dp,
                                                 ctypes.POINTER(OrScatterOpts), _dp, _dp,
                                                 ctypes.POINTER(OrAllskyOut), ctypes.POINTER(OrAllskyK)]
+        _lib.oracle_k_matrix_allsky_ad.argtypes = _lib.oracle_k_matrix_allsky.argtypes
         _lib.oracle_cloud_optics.argtypes = [ctypes.POINTER(OrCloudCoeff), ctypes.c_int, ctypes.c_double,
                                              ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int,
                                              _dp, _dp, _dp]
@@ -344,8 +345,9 @@ def forward_allsky_batch(sensor: Sensor, cloudcoeff: CloudCoeff, ps, emissivity,
 
 
 def k_matrix_allsky(sensor: Sensor, cloudcoeff: CloudCoeff, ps, m, emissivity, n_streams=0, overlap=0,
-                    bt_k=None, rad_k=None):
-    """CRTM_K_Matrix with clouds for profile m by complex-step differentiation."""
+                    bt_k=None, rad_k=None, method="cstep"):
+    """CRTM_K_Matrix with clouds for profile m: complex-step differentiation
+    (method "cstep") or the hand-written adjoint (method "adjoint")."""
     keep = []
     p = make_profile(ps, m, keep)
     c = make_clouds(ps, m, keep)
@@ -366,9 +368,10 @@ def k_matrix_allsky(sensor: Sensor, cloudcoeff: CloudCoeff, ps, m, emissivity, n
     rk = np.ascontiguousarray(np.zeros(L) if rad_k is None else rad_k, np.float64)
     opt = OrScatterOpts(int(n_streams), int(overlap))
     eps = np.ascontiguousarray(emissivity, np.float64)
-    rc = lib().oracle_k_matrix_allsky(ctypes.byref(sensor.s), ctypes.byref(cloudcoeff.c), ctypes.byref(p),
-                                      ctypes.byref(c) if c is not None else None, _p(eps, _dp), ctypes.byref(opt),
-                                      _p(btk, _dp), _p(rk, _dp), ctypes.byref(fo), ctypes.byref(ko))
+    fn = lib().oracle_k_matrix_allsky_ad if method == "adjoint" else lib().oracle_k_matrix_allsky
+    rc = fn(ctypes.byref(sensor.s), ctypes.byref(cloudcoeff.c), ctypes.byref(p),
+            ctypes.byref(c) if c is not None else None, _p(eps, _dp), ctypes.byref(opt),
+            _p(btk, _dp), _p(rk, _dp), ctypes.byref(fo), ctypes.byref(ko))
     if rc != 0:
-        raise RuntimeError(f"oracle_k_matrix_allsky failed ({rc})")
+        raise RuntimeError(f"oracle_k_matrix_allsky ({method}) failed ({rc})")
     return fwd, kd
