@@ -76,7 +76,8 @@ class CRTSolution(ctypes.Structure):
 class CKOutput(ctypes.Structure):
     _fields_ = [(n, _vp) for n in ("brightness_temperature_k", "radiance_k", "surface_emissivity_k",
                                    "temperature_k", "absorber_k", "land_temperature_k",
-                                   "water_temperature_k", "snow_temperature_k", "ice_temperature_k")]
+                                   "water_temperature_k", "snow_temperature_k", "ice_temperature_k",
+                                   "cloud_water_content_k", "cloud_effective_radius_k", "cloud_fraction_k")]
 
 
 _lib = None
@@ -278,20 +279,25 @@ class Context:
         return out
 
     def k_matrix(self, prof, emissivity, bt_k=None, rad_k=None, precision=PREC_FP64,
-                 jacobian_f32=False, check_input=True, out=None):
+                 jacobian_f32=False, check_input=True, out=None, n_streams=0, overlap=0, extras=False):
         """CRTM_K_Matrix: returns (RTSolution dict, K dict).
 
         K layout: temperature_k [NT, L] (profile m rows layer_offset[m]:...),
-        absorber_k [n_absorbers, NT, L]; surface temperature K per type [M, L].
+        absorber_k [n_absorbers, NT, L]; surface temperature K per type [M, L];
+        with clouds also cloud_water_content_k and cloud_effective_radius_k
+        [n_clouds, NT, L] and cloud_fraction_k [NT, L].
         bt_k / rad_k are RTSolution_K inputs (BT_K defaults to 1, Radiance_K
         to 0).  `out` = (rts, k) dicts from a previous call are reused as the
-        output buffers (every element is overwritten)."""
+        output buffers (every element is overwritten).  n_streams / overlap as
+        in forward(); extras adds R_Clear, Tb_Clear and the cloud cover of a
+        cloudy call to the RTSolution dict."""
         memory = MEM_DEVICE if _is_torch(prof.temperature) else MEM_HOST
         if memory == MEM_DEVICE:
             self._use_torch_stream()
         keep = []
         atm, sfc, geo, opt = self._structs(prof, emissivity, precision, memory, jacobian_f32,
-                                           check_input, keep)
+                                           check_input, keep, n_streams, overlap)
+        nc = int(atm.n_clouds)
         M, L = prof.n_profiles, self.n_channels
         host_nl = getattr(prof, "n_layers_host", None)
         NT = int(np.sum(host_nl if host_nl is not None else np.asarray(prof.n_layers)))
@@ -309,6 +315,13 @@ class Context:
                   "absorber_k": self._alloc((na, NT, L), kdt, memory, like)}
             for f in ("land_temperature_k", "water_temperature_k", "snow_temperature_k", "ice_temperature_k"):
                 kd[f] = self._alloc((M, L), np.float64, memory, like)
+            if nc > 0:
+                kd["cloud_water_content_k"] = self._alloc((nc, NT, L), kdt, memory, like)
+                kd["cloud_effective_radius_k"] = self._alloc((nc, NT, L), kdt, memory, like)
+                kd["cloud_fraction_k"] = self._alloc((NT, L), kdt, memory, like)
+                if extras:
+                    for k in ("r_clear", "tb_clear", "total_cloud_cover"):
+                        rts_d[k] = self._alloc((M, L), np.float64, memory, like)
         if bt_k is None:
             bt_k = self._alloc((M, L), np.float64, memory, like) + 1.0
         if memory == MEM_HOST:

@@ -444,7 +444,11 @@ clear_fwdk_kernel(DevSensor s, DevBatch bt, ProfWork<RT> w, DevOut o) {
   // CRTM_Planck_Temperature in FP64 (CRTM_Planck_Functions.f90:470-492)
   const double c1d = s.c1[lc], c2d = s.c2[lc], bc1d = s.bc1[lc], bc2d = s.bc2[lc];
   const double bt_v = (c2d / log(c1d / R + 1.0) - bc1d) / bc2d;
-  if (valid) {
+  // profiles with cloudy layers in an all-sky K-matrix call: the scattering
+  // path has written the outputs and hands over adjoint seeds
+  const bool seeded = DOK && o.ks.mask && ok_prof && o.ks.mask[m] > 0;   // wave-uniform
+  const int64_t sl = (int64_t)m * s.L + lc;
+  if (valid && !seeded) {
     o.radiance[ol] = R;
     o.bt[ol] = bt_v;
     if (o.up_radiance) o.up_radiance[ol] = (double)uacc;
@@ -458,14 +462,17 @@ clear_fwdk_kernel(DevSensor s, DevBatch bt, ProfWork<RT> w, DevOut o) {
   const double arg = c1d / R + 1.0;
   const double lga = log(arg);
   const double F = c1d * c2d / (bc2d * arg * (R * lga) * (R * lga));
-  const double rad_ad_d = valid ? (o.rad_k ? o.rad_k[ol] : 0.0) + F * (o.bt_k ? o.bt_k[ol] : 0.0) : 0.0;
-  if (valid && o.rad_k) o.rad_k[ol] = rad_ad_d;
+  double rad_ad_d = 0.0;
+  if (seeded) rad_ad_d = valid ? o.ks.rad[sl] : 0.0;
+  else rad_ad_d = valid ? (o.rad_k ? o.rad_k[ol] : 0.0) + F * (o.bt_k ? o.bt_k[ol] : 0.0) : 0.0;
+  if (valid && o.rad_k && !seeded) o.rad_k[ol] = rad_ad_d;
   const RT rad_ad = (RT)rad_ad_d;
   const RT upad_sfc = rad_ad * tc;           // adjoint of the surface-leaving radiance
   if (valid) {
-    if (o.emis_k) o.emis_k[ol] = (double)(upad_sfc * Bs - upad_sfc * rd);
+    const double e_seed = seeded ? o.ks.emis[sl] : 0.0, bs_seed = seeded ? o.ks.bs[sl] : 0.0;
+    if (o.emis_k) o.emis_k[ol] = (double)(upad_sfc * Bs - upad_sfc * rd) + e_seed;
     const RT dbsdt = c2 * bc2 * Bs * (Bs + c1) * ic1 / (tes * tes);
-    const double sad = (double)(dbsdt * eps * upad_sfc);
+    const double sad = (double)(dbsdt * (eps * upad_sfc + (RT)bs_seed));
     for (int c = 0; c < 4; ++c)
       if (o.sfc_t_k[c]) o.sfc_t_k[c][ol] = bt.cov[c][m] * sad;
   }
@@ -511,14 +518,21 @@ clear_fwdk_kernel(DevSensor s, DevBatch bt, ProfWork<RT> w, DevOut o) {
       const RT upad = rad_ad * tabove;
       const RT a_up = t * (ru - B) * upad;
       const RT a_dn = gd - B * dn;
-      const RT b_ad = (upad + dn) * (RT(1) - t);
-      const RT tau_ad = -(a_up + a_dn) * iu;
+      RT b_ad = (upad + dn) * (RT(1) - t);
+      RT tau_ad = -(a_up + a_dn) * iu;
+      RT t_seed = RT(0);
+      if (seeded) {
+        const int64_t is = (loff + k) * s.L + lc;
+        b_ad += reinterpret_cast<const RT*>(o.ks.b)[is];
+        tau_ad += reinterpret_cast<const RT*>(o.ks.tau)[is];
+        t_seed = reinterpret_cast<const RT*>(o.ks.t)[is];
+      }
       gd -= B * (RT(1) - t) * dn;
       dn *= t;
       ru = ru * t + B * (RT(1) - t);
       const RT dbdt = c2 * bc2 * B * (B + c1) * ic1 / (te * te);
       if (valid && k >= nx) {
-        tko[(int64_t)k * Lt] = (KT)(dbdt * b_ad);
+        tko[(int64_t)k * Lt] = (KT)(dbdt * b_ad + t_seed);
         for (int ja = 0; ja < na; ++ja) ako[ja * astr + (int64_t)k * Lt] = KT(0);
       }
       const RT gq = tau_ad * isec;

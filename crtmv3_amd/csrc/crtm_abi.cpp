@@ -30,7 +30,13 @@ template <class RT>
 void launch_allsky(const DevSensor&, const DevBatch&, const DevClouds&, const CloudWork&, const DevCloudCoeff&,
                    const LayerScratch<RT>&, const RT*, const double*, int, const int*, int, const DevOut&,
                    const AllskyOut&, hipStream_t);
-void launch_clear_copy(const DevSensor&, int M, int Ltot, const DevOut&, const AllskyOut&, hipStream_t);
+void launch_clear_copy(const DevSensor&, int M, int Ltot, const int* cl_count, const DevOut&, const AllskyOut&,
+                       hipStream_t);
+template <class RT, class KT>
+void launch_allsky_k(const DevSensor&, const DevBatch&, const DevClouds&, const CloudWork&, const DevCloudCoeff&,
+                     const LayerScratch<RT>&, const LayerScratch<RT>&, const RT*, const double*, int, const int*, int,
+                     const DevOut&, const AllskyOut&, const CloudKOut&, RT*, int64_t, int, hipStream_t);
+int allsky_sweep_ad_blocks(int64_t n_lanes);
 }  // namespace crtm
 
 using namespace crtm;
@@ -635,6 +641,8 @@ struct crtm_context {
   std::vector<std::unique_ptr<DevBuf>> cloud_bufs;
   DevCloudCoeff cc{};
   DevBuf cwc, creff, ccount, cslot, cdesc, ctotal, ctcc, cfrac, cmaxr, ctau, cscr;
+  // all-sky K-matrix: layer adjoints, per-thread adding tape, seeds for the clear-sky K kernel
+  DevBuf cscr_ad, ctape, ks_tau, ks_b, ks_t, ks_ch;
   // kernel timing (crtm_profile_enable): HIP events on the context stream
   bool profiling = false;
   std::vector<hipEvent_t> ev;      // pairs (start, stop) per recorded launch
@@ -688,6 +696,36 @@ ProfWork<RT> make_work(crtm_context* c, int M, int Ko, int nx, int max_k) {
   return w;
 }
 
+// the batch as the sensor's kernels see it: ODPS absorber j -> user absorber
+DevBatch sensor_batch(const Sensor& S, const DevBatch& b) {
+  DevBatch bb = b;
+  for (int j = 0; j < NA_MAX; ++j) bb.jmap[j] = -1;
+  for (int j = 0; j < S.d.Jm; ++j)
+    for (int ju = 0; ju < b.na; ++ju)
+      if (S.tc.absorber_id[j] == b.jmap[ju]) {   // b.jmap carries the user absorber ids here
+        bb.jmap[j] = ju;
+        break;
+      }
+  return bb;
+}
+
+// the fused clear-sky forward / K-matrix kernel alone (preparation already run)
+template <class RT, class KT, bool DOK>
+void run_fwdk(crtm_context* c, Sensor& S, const DevBatch& b, const ProfWork<RT>& w, const DevOut& o) {
+  const DevBatch bb = sensor_batch(S, b);
+  if (c->profiling) {
+    c->mark();
+    c->ev_kind.push_back(1);
+  }
+  switch (S.d.group) {
+    case 1: launch_fwdk<1, RT, KT, DOK>(S.d, bb, w, o, c->stream); break;
+    case 2: launch_fwdk<2, RT, KT, DOK>(S.d, bb, w, o, c->stream); break;
+    case 3: launch_fwdk<3, RT, KT, DOK>(S.d, bb, w, o, c->stream); break;
+  }
+  if (c->profiling) c->mark();
+  HIPCHECK(hipGetLastError());
+}
+
 template <class RT, class KT, bool DOK>
 ProfWork<RT> run_sensor(crtm_context* c, Sensor& S, const DevBatch& b, const DevOut& o) {
   const int nx = S.d.group == 1 ? 89 : S.d.group == 2 ? 50 : 21;
@@ -728,7 +766,7 @@ ProfWork<RT> run_sensor(crtm_context* c, Sensor& S, const DevBatch& b, const Dev
 
 // Cloud preparation for the whole batch (sensor independent); returns the
 // number of cloudy layers (one host synchronisation).
-int prepare_clouds(crtm_context* c, const DevBatch& be, const DevClouds& cl, CloudWork& cw) {
+int prepare_clouds(crtm_context* c, const DevBatch& be, const DevClouds& cl, CloudWork& cw, int* max_k_cloudy) {
   const int M = be.M;
   const int64_t NTe = be.NT;
   c->cwc.reserve(std::max<size_t>((size_t)cl.nc * NTe * sizeof(double), 16));
@@ -736,7 +774,7 @@ int prepare_clouds(crtm_context* c, const DevBatch& be, const DevClouds& cl, Clo
   c->ccount.reserve(M * sizeof(int));
   c->cslot.reserve(NTe * sizeof(int));
   c->cdesc.reserve(std::max<int64_t>(be.out_NT, 1) * sizeof(int2));
-  c->ctotal.reserve(sizeof(int));
+  c->ctotal.reserve(2 * sizeof(int));
   c->ctcc.reserve(M * sizeof(double));
   c->cfrac.reserve(M * sizeof(int));
   c->cmaxr.reserve(M * sizeof(double));
@@ -750,13 +788,14 @@ int prepare_clouds(crtm_context* c, const DevBatch& be, const DevClouds& cl, Clo
   cw.frac = c->cfrac.as<int>();
   cw.max_reff = c->cmaxr.as<double>();
   cw.cap = (int)std::min<int64_t>(be.out_NT, INT32_MAX);
-  HIPCHECK(hipMemsetAsync(cw.total, 0, sizeof(int), c->stream));
+  HIPCHECK(hipMemsetAsync(cw.total, 0, 2 * sizeof(int), c->stream));
   launch_cloud_prep(be, cl, cw, c->stream);
   HIPCHECK(hipGetLastError());
-  int total = 0;
-  HIPCHECK(hipMemcpyAsync(&total, cw.total, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+  int tot[2] = {0, 0};
+  HIPCHECK(hipMemcpyAsync(tot, cw.total, 2 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
   HIPCHECK(hipStreamSynchronize(c->stream));
-  return std::min(total, cw.cap);
+  if (max_k_cloudy) *max_k_cloudy = tot[1];
+  return std::min(tot[0], cw.cap);
 }
 
 // Angle counts the layer and sweep kernels are launched for: the stream
@@ -849,12 +888,75 @@ struct CloudCall {
   DevClouds cl{};
   CloudWork cw{};
   int total = 0;
+  int max_k = 0;       // largest (extended) layer count of a profile with cloudy layers
   AllskyOut ao{};
+  CloudKOut ko{};      // cloud Jacobians (K-matrix calls)
 };
+
+// All-sky K-matrix for one sensor: clear-sky forward (radiances and gas
+// optical depths), the scattering forward + adjoint producing the cloud
+// Jacobians and the seeds, then the clear-sky K-matrix kernel completing the
+// gas, Planck and surface adjoints with those seeds.
+template <class RT, class KT>
+void run_sensor_k_allsky(crtm_context* c, Sensor& S, const DevBatch& be, const DevOut& o, const CloudCall& cc) {
+  const int L = S.d.L;
+  DevOut o1 = o;
+  c->ctau.reserve(std::max<size_t>((size_t)be.NT * L * sizeof(RT), 16));
+  o1.tau_g = c->ctau.p;
+  o1.tau_mask = cc.cw.cl_count;
+  o1.ks = KSeeds{};
+  ProfWork<RT> w = run_sensor<RT, RT, false>(c, S, be, o1);
+  CloudWork cw = cc.cw;
+  cw.status = w.status;
+  const int64_t items = (int64_t)cc.total * L;
+  if (items > INT32_MAX) throw CrtmError(fmt("%lld cloudy layer-channels exceed one launch", (long long)items));
+  const std::vector<int> nz = angle_counts(cc.cl.n_streams);
+  LayerScratch<RT> sc{}, sa{};
+  sc.nzc = sa.nzc = *std::max_element(nz.begin(), nz.end());
+  sc.n_items = sa.n_items = std::max<int64_t>(items, 1);
+  c->cscr.reserve((size_t)sc.n_elem() * sc.n_items * sizeof(RT));
+  c->cscr_ad.reserve((size_t)sa.n_elem() * sa.n_items * sizeof(RT));
+  sc.v = c->cscr.as<RT>();
+  sa.v = c->cscr_ad.as<RT>();
+  const int64_t ML = (int64_t)be.M * L;
+  const int64_t tape_lanes = (int64_t)allsky_sweep_ad_blocks(ML) * 256;
+  const int te = sc.nzc * sc.nzc + sc.nzc;
+  c->ctape.reserve(std::max<size_t>((size_t)tape_lanes * std::max(cc.max_k, 1) * te * sizeof(RT), 16));
+  c->ks_tau.reserve(std::max<size_t>((size_t)be.NT * L * sizeof(RT), 16));
+  c->ks_b.reserve(std::max<size_t>((size_t)be.NT * L * sizeof(RT), 16));
+  c->ks_t.reserve(std::max<size_t>((size_t)be.NT * L * sizeof(RT), 16));
+  c->ks_ch.reserve((size_t)4 * ML * sizeof(double));
+  DevOut o2 = o;
+  o2.tau_g = nullptr;
+  o2.tau_mask = nullptr;
+  o2.ks.mask = cc.cw.cl_count;
+  o2.ks.tau = c->ks_tau.p;
+  o2.ks.b = c->ks_b.p;
+  o2.ks.t = c->ks_t.p;
+  o2.ks.rad = c->ks_ch.as<double>();
+  o2.ks.emis = o2.ks.rad + ML;
+  o2.ks.bs = o2.ks.emis + ML;
+  o2.ks.tcc_ad = o2.ks.bs + ML;
+  if (c->profiling) {
+    c->mark();
+    c->ev_kind.push_back(2);
+  }
+  launch_allsky_k<RT, KT>(S.d, be, cc.cl, cw, c->cc, sc, sa, reinterpret_cast<const RT*>(o1.tau_g), w.geo,
+                          (int)items, nz.data(), (int)nz.size(), o2, cc.ao, cc.ko, c->ctape.as<RT>(), tape_lanes, te,
+                          c->stream);
+  launch_clear_copy(S.d, be.M, be.Ltot, cc.cw.cl_count, o1, cc.ao, c->stream);   // profiles without cloudy layers
+  if (c->profiling) c->mark();
+  HIPCHECK(hipGetLastError());
+  run_fwdk<RT, KT, true>(c, S, be, w, o2);
+}
 
 template <class RT, class KT, bool DOK>
 void run_sensors(crtm_context* c, const DevBatch& be, DevOut o, const CloudCall& cc) {
   for (auto& S : c->sensors) {
+    if (DOK && cc.on) {
+      run_sensor_k_allsky<RT, KT>(c, *S, be, o, cc);
+      continue;
+    }
     if (cc.on) {
       c->ctau.reserve(std::max<size_t>((size_t)be.NT * S->d.L * sizeof(RT), 16));
       o.tau_g = c->ctau.p;
@@ -862,7 +964,7 @@ void run_sensors(crtm_context* c, const DevBatch& be, DevOut o, const CloudCall&
     }
     ProfWork<RT> w = run_sensor<RT, KT, DOK>(c, *S, be, o);
     if (cc.on) run_allsky<RT>(c, *S, be, cc.cl, cc.cw, cc.total, w, o, cc.ao);
-    else launch_clear_copy(S->d, be.M, be.Ltot, o, cc.ao, c->stream);   // R_Clear = Radiance without clouds
+    else launch_clear_copy(S->d, be.M, be.Ltot, nullptr, o, cc.ao, c->stream);   // R_Clear = Radiance without clouds
   }
 }
 
@@ -874,6 +976,14 @@ void dispatch(crtm_context* c, const DevBatch& be, const DevOut& o, const CloudC
   } else {
     if (split) run_sensors<float, float, false>(c, be, o, cc); else run_sensors<double, double, false>(c, be, o, cc);
   }
+}
+
+// cloud Jacobians are written only where a cloud is present
+void zero_cloud_k(crtm_context* ctx, const CloudKOut& ko, int nc, int64_t NT, int L, size_t szK) {
+  const size_t n = (size_t)NT * L * szK;
+  HIPCHECK(hipMemsetAsync(ko.wc_k, 0, nc * n, ctx->stream));
+  HIPCHECK(hipMemsetAsync(ko.reff_k, 0, nc * n, ctx->stream));
+  HIPCHECK(hipMemsetAsync(ko.cf_k, 0, n, ctx->stream));
 }
 
 int do_call(crtm_context* ctx, const crtm_atmosphere* atm, const crtm_surface* sfc, const crtm_geometry* geo,
@@ -899,7 +1009,8 @@ int do_call(crtm_context* ctx, const crtm_atmosphere* atm, const crtm_surface* s
       return fail(ctx, fmt("n_streams %d: the scattering kernels take 2, 4, 6 or 8 streams (0 = automatic)",
                            opt->n_streams));
     if (opt->overlap_id < 0 || opt->overlap_id > 5) return fail(ctx, "Invalid overlap assumption");
-    if (k) return fail(ctx, "the K-matrix of cloudy profiles is not part of this build yet");
+    if (k && (!k->cloud_water_content_k || !k->cloud_effective_radius_k || !k->cloud_fraction_k))
+      return fail(ctx, "cloud Jacobian arrays (water content, effective radius, fraction) required");
     cc.cl.nc = atm->n_clouds;
     cc.cl.n_streams = opt->n_streams;
     cc.cl.overlap = opt->overlap_id > 0 ? opt->overlap_id : 4;
@@ -1016,6 +1127,12 @@ int do_call(crtm_context* ctx, const crtm_atmosphere* atm, const crtm_surface* s
       for (int c = 0; c < 4; ++c) out(stk[c], ML, (void**)&o.sfc_t_k[c]);
       out(k->temperature_k, (size_t)NT * L * szK, (void**)&o.t_k);
       out(k->absorber_k, (size_t)na * NT * L * szK, (void**)&o.a_k);
+      if (cc.on) {
+        const size_t nc = atm->n_clouds;
+        out(k->cloud_water_content_k, nc * NT * L * szK, &cc.ko.wc_k);
+        out(k->cloud_effective_radius_k, nc * NT * L * szK, &cc.ko.reff_k);
+        out(k->cloud_fraction_k, (size_t)NT * L * szK, &cc.ko.cf_k);
+      }
     }
     // rad_k is both input and output: give it one device slot
     size_t total = 0;
@@ -1037,8 +1154,9 @@ int do_call(crtm_context* ctx, const crtm_atmosphere* atm, const crtm_surface* s
       *outs[i].d = base + ooff[i];
     }
     if (dok && !k->radiance_k) o.rad_k = nullptr;
+    if (dok && cc.on) zero_cloud_k(ctx, cc.ko, atm->n_clouds, NT, L, szK);
     const DevBatch be = add_layers(ctx, b, abs_id.data());
-    if (cc.on) cc.total = prepare_clouds(ctx, be, cc.cl, cc.cw);
+    if (cc.on) cc.total = prepare_clouds(ctx, be, cc.cl, cc.cw, &cc.max_k);
     dispatch(ctx, be, o, cc, split, kf32, dok);
     if (rts->status) {
       std::vector<int> st(M);
@@ -1096,9 +1214,15 @@ int do_call(crtm_context* ctx, const crtm_atmosphere* atm, const crtm_surface* s
     cc.cl.wc = atm->cloud_water_content;
     cc.cl.reff = atm->cloud_effective_radius;
     cc.cl.cf = atm->cloud_fraction;
+    if (dok) {
+      cc.ko.wc_k = k->cloud_water_content_k;
+      cc.ko.reff_k = k->cloud_effective_radius_k;
+      cc.ko.cf_k = k->cloud_fraction_k;
+      zero_cloud_k(ctx, cc.ko, atm->n_clouds, NT, L, szK);
+    }
   }
   const DevBatch be = add_layers(ctx, b, abs_id.data());
-  if (cc.on) cc.total = prepare_clouds(ctx, be, cc.cl, cc.cw);
+  if (cc.on) cc.total = prepare_clouds(ctx, be, cc.cl, cc.cw, &cc.max_k);
   dispatch(ctx, be, o, cc, split, kf32, dok);
   if (rts->status)
     HIPCHECK(hipMemcpyAsync(rts->status, ctx->wstatus.p, M * sizeof(int), hipMemcpyDeviceToDevice, ctx->stream));

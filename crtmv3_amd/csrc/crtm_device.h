@@ -196,6 +196,21 @@ template <class RT> struct ProfWork {
   int* status;    // [M]
 };
 
+// All-sky K-matrix: adjoints the scattering path hands to the clear-sky
+// K-matrix kernel for the profiles with cloudy layers (mask > 0).  Layer
+// arrays are [NT (extended layout)][L of the sensor] in the call's precision,
+// per-channel arrays [M][L of the sensor].
+struct KSeeds {
+  const int* mask;   // [M] cloudy-layer count of the profile
+  void* tau;         // adjoint of the nadir gas optical depth of the layer
+  void* b;           // adjoint of the layer Planck radiance
+  void* t;           // direct temperature adjoint (temperature-dependent cloud tables)
+  double* rad;       // adjoint of the clear-column radiance (fractional cover), else 0
+  double* emis;      // surface emissivity adjoint of the cloudy RT
+  double* bs;        // surface Planck radiance adjoint of the cloudy RT
+  double* tcc_ad;    // adjoint of the total cloud cover
+};
+
 struct DevOut {
   double* radiance;        // [M][Ltot]
   double* bt;
@@ -215,6 +230,15 @@ struct DevOut {
   // [NT (extended layout)][L of the sensor], precision of the call
   void* tau_g;
   const int* tau_mask;     // [M]: write the profile's ODs when > 0
+  KSeeds ks;               // all-sky K-matrix seeds (ks.mask null: none)
+};
+
+// Cloud Jacobians of the K-matrix, caller's layer layout, element type KT:
+// water content and effective radius [nc][out_NT][Ltot], fraction [out_NT][Ltot]
+struct CloudKOut {
+  void* wc_k;
+  void* reff_k;
+  void* cf_k;
 };
 
 // ---------------------------------------------------------------------------
@@ -262,7 +286,7 @@ struct CloudWork {
   int* cl_count;        // [M] cloudy layers of profile m
   int* cl_slot;         // [NT] cloudy-layer id of (m, k), -1 for clear layers
   int2* cl_desc;        // [cap] (m, k) of cloudy layer c
-  int* total;           // [1] cloudy layers in the batch
+  int* total;           // [2] cloudy layers in the batch, largest extended layer count of a cloudy profile
   double* tcc;          // [M] total cloud cover of fractional profiles (else 0)
   int* frac;            // [M] 1: fractional coverage
   double* max_reff;     // [M] largest effective radius (stream choice)

@@ -120,7 +120,8 @@ MODULE CRTM_MI355X_Module
   END TYPE
   TYPE, BIND(C) :: c_k_output
     TYPE(C_PTR) :: brightness_temperature_k, radiance_k, surface_emissivity_k, temperature_k, &
-                   absorber_k, land_temperature_k, water_temperature_k, snow_temperature_k, ice_temperature_k
+                   absorber_k, land_temperature_k, water_temperature_k, snow_temperature_k, ice_temperature_k, &
+                   cloud_water_content_k, cloud_effective_radius_k, cloud_fraction_k
   END TYPE
 
   INTERFACE
@@ -271,11 +272,18 @@ CONTAINS
 
   SUBROUTINE CRTM_Atmosphere_Zero(Atm)
     TYPE(CRTM_Atmosphere_type), INTENT(INOUT) :: Atm(:, :)
-    INTEGER :: i, j
+    INTEGER :: i, j, n
     DO j = 1, SIZE(Atm, 2)
       DO i = 1, SIZE(Atm, 1)
         IF (ALLOCATED(Atm(i, j)%Temperature)) Atm(i, j)%Temperature = 0
         IF (ALLOCATED(Atm(i, j)%Absorber)) Atm(i, j)%Absorber = 0
+        IF (ALLOCATED(Atm(i, j)%Cloud_Fraction)) Atm(i, j)%Cloud_Fraction = 0
+        IF (ALLOCATED(Atm(i, j)%Cloud)) THEN
+          DO n = 1, SIZE(Atm(i, j)%Cloud)
+            Atm(i, j)%Cloud(n)%Water_Content = 0
+            Atm(i, j)%Cloud(n)%Effective_Radius = 0
+          END DO
+        END IF
       END DO
     END DO
   END SUBROUTINE CRTM_Atmosphere_Zero
@@ -329,7 +337,8 @@ CONTAINS
     TYPE(batch_type), TARGET :: b
     TYPE(c_k_output) :: k
     REAL(C_DOUBLE), ALLOCATABLE, TARGET :: btk(:), radk(:), emk(:), tk(:), ak(:), stk(:, :)
-    INTEGER :: L, M, mm, ic, j, kk, NT, na, K0
+    REAL(C_DOUBLE), ALLOCATABLE, TARGET :: cwk(:), crk(:), cfk(:)
+    INTEGER :: L, M, mm, ic, j, kk, NT, na, K0, nc
     Error_Status = pack_batch(Atmosphere, Surface, Geometry, ChannelInfo, RTSolution, Options, b, L, M)
     IF (Error_Status /= SUCCESS) RETURN
     NT = INT(b%off(M + 1))
@@ -350,6 +359,11 @@ CONTAINS
     k%water_temperature_k = C_LOC(stk(1, 2))
     k%snow_temperature_k = C_LOC(stk(1, 3))
     k%ice_temperature_k = C_LOC(stk(1, 4))
+    nc = Atmosphere(1)%n_Clouds
+    ALLOCATE(cwk(MAX(nc, 1) * NT * L), crk(MAX(nc, 1) * NT * L), cfk(NT * L))
+    k%cloud_water_content_k = C_LOC(cwk)
+    k%cloud_effective_radius_k = C_LOC(crk)
+    k%cloud_fraction_k = C_LOC(cfk)
     Error_Status = c_crtm_k_matrix(ChannelInfo(1)%ctx, b%atm, b%sfc, b%geo, b%opt, b%rts, k)
     IF (Error_Status /= SUCCESS) THEN
       CALL report(ChannelInfo(1)%ctx, 'CRTM_K_Matrix')
@@ -367,6 +381,17 @@ CONTAINS
             Atmosphere_K(ic, mm)%Absorber(kk, j) = Atmosphere_K(ic, mm)%Absorber(kk, j) + &
                 ak(((j - 1) * NT + K0 + kk - 1) * L + ic)
           END DO
+          ! Atmosphere_K%Cloud(j)%Water_Content / %Effective_Radius, %Cloud_Fraction
+          IF (nc > 0 .AND. ALLOCATED(Atmosphere_K(ic, mm)%Cloud)) THEN
+            DO j = 1, nc
+              Atmosphere_K(ic, mm)%Cloud(j)%Water_Content(kk) = Atmosphere_K(ic, mm)%Cloud(j)%Water_Content(kk) + &
+                  cwk(((j - 1) * NT + K0 + kk - 1) * L + ic)
+              Atmosphere_K(ic, mm)%Cloud(j)%Effective_Radius(kk) = &
+                  Atmosphere_K(ic, mm)%Cloud(j)%Effective_Radius(kk) + crk(((j - 1) * NT + K0 + kk - 1) * L + ic)
+            END DO
+            Atmosphere_K(ic, mm)%Cloud_Fraction(kk) = Atmosphere_K(ic, mm)%Cloud_Fraction(kk) + &
+                cfk((K0 + kk - 1) * L + ic)
+          END IF
         END DO
         Surface_K(ic, mm)%Land_Temperature = Surface_K(ic, mm)%Land_Temperature + stk((mm - 1) * L + ic, 1)
         Surface_K(ic, mm)%Water_Temperature = Surface_K(ic, mm)%Water_Temperature + stk((mm - 1) * L + ic, 2)

@@ -152,6 +152,15 @@ typedef struct {
   double* water_temperature_k;
   double* snow_temperature_k;
   double* ice_temperature_k;
+  /* Atmosphere_K%Cloud(n)%Water_Content and %Effective_Radius
+   * [n_clouds][NT][L], and Atmosphere_K%Cloud_Fraction [NT][L]
+   * (CRTM_Compute_CloudScatter_AD, src/AtmScatter/CRTM_CloudScatter.f90:737-907;
+   * Compute_CloudCover_AD, src/Atmosphere/Cloud/CRTM_CloudCover_Define.f90).
+   * Element type per options.jacobian_f32; required when the atmosphere has
+   * clouds, ignored otherwise; overwritten. */
+  void* cloud_water_content_k;
+  void* cloud_effective_radius_k;
+  void* cloud_fraction_k;
 } crtm_k_output;
 
 /* CRTM_Init (src/CRTM_LifeCycle.f90:520): loads <id>.SpcCoeff.bin and
