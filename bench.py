@@ -85,9 +85,9 @@ def _subset(ps, a, b):
             d[f.name] = off[a:b + 1] - off[a]
         elif f.name == "level_pressure":
             d[f.name] = v[la + a:lb + b]
-        elif f.name in ("pressure", "temperature"):
+        elif f.name in ("pressure", "temperature", "cloud_fraction"):
             d[f.name] = v[la:lb]
-        elif f.name == "absorber":
+        elif f.name in ("absorber", "cloud_water_content", "cloud_effective_radius"):
             d[f.name] = v[:, la:lb]
         elif f.name in ("absorber_id", "absorber_units"):
             d[f.name] = v
@@ -261,10 +261,11 @@ def main():
 
 
 def main_allsky(args):
-    """Cloudy microwave forward: 22-channel ATMS-like sensor, 8192 profiles x
-    100 layers with liquid, ice, rain and snow in 0-10 contiguous layers each
-    (overcast), ADA at 4 and 8 streams; FP32-split vs FP64 GPU, BT error
-    against the FP64 oracle on a sample, oracle timed on the host cores."""
+    """Cloudy microwave forward + K-matrix: 22-channel ATMS-like sensor, 8192
+    profiles x 100 layers with liquid, ice, rain and snow in 0-10 contiguous
+    layers each (overcast), ADA at 4 and 8 streams; FP32-split vs FP64 GPU, BT
+    and Jacobian errors against the FP64 oracle on a sample, oracle timed on the
+    host cores."""
     import torch
     from crtmv3_amd import api, coeffio, synthetic
     from oracle import oracle as orc
@@ -282,69 +283,114 @@ def main_allsky(args):
     dps = api.device_profiles(ps, dev)
     emis_h = np.tile(synthetic.default_emissivity(1, L), (M, 1))
     emis = torch.as_tensor(emis_h, device=dev)
+    btk = torch.ones((M, L), dtype=torch.float64, device=dev)
     torch.cuda.synchronize()
     by_streams = {}
     tau = coeffio.read_odps(os.path.join(coeff_dir, f"{sensor}.TauCoeff.bin"))
     spc = coeffio.read_spccoeff(os.path.join(coeff_dir, f"{sensor}.SpcCoeff.bin"))
     S = orc.Sensor(tau, spc)
     C = orc.CloudCoeff(coeffio.read_cloudcoeff(os.path.join(coeff_dir, synthetic.CLOUDCOEFF_FILE)))
-    n_sample = 256
+    threads = min(16, os.cpu_count() or 1)
+    n_sample = 256     # forward BT check
+    n_ksample = 32     # K-matrix check (hand-written oracle adjoint)
     sub = _subset(ps, 0, n_sample)
+    ksub = _subset(ps, 0, n_ksample)
     for ns in (4, 8):
-        def run(prec):
-            return ctx.forward(dps, emis, precision=prec, n_streams=ns)
         res = {}
         for prec, name, steps in ((api.PREC_FP32_SPLIT, "split", args.steps), (api.PREC_FP64, "fp64", args.fp64_steps)):
-            out = run(prec)
-            for _ in range(args.warmup):
-                out = run(prec)
-            ctx.synchronize()
-            ctx.profile_enable(True)
-            t0 = time.perf_counter()
-            for _ in range(steps):
-                out = run(prec)
-            ctx.synchronize()
-            torch.cuda.synchronize()
-            t = (time.perf_counter() - t0) / steps
-            ms, nl = ctx.profile_query_kinds()
-            ctx.profile_enable(False)
-            res[name] = {"ms_per_step": t * 1e3, "profile_channels_per_s": M * L / t,
-                         "kernel_ms": {"prep": ms[0] / steps, "clear_fwd": ms[1] / steps,
-                                       "scattering": ms[2] / steps},
-                         "bt": out["brightness_temperature"].cpu().numpy()}
-            log(f"  {ns} streams {name}: {t * 1e3:.2f} ms/step  kernels {res[name]['kernel_ms']}")
-        _, bt_or = orc.forward_allsky_batch(S, C, sub, emis_h[:n_sample], n_streams=ns,
-                                            n_threads=min(16, os.cpu_count() or 1))
+            r = {}
+            for mode in ("forward", "k_matrix"):
+                if mode == "forward":
+                    def run():
+                        return ctx.forward(dps, emis, precision=prec, n_streams=ns), None
+                else:
+                    prev = ctx.k_matrix(dps, emis, bt_k=btk, precision=prec, jacobian_f32=prec == api.PREC_FP32_SPLIT,
+                                        n_streams=ns)
+
+                    def run():
+                        return ctx.k_matrix(dps, emis, bt_k=btk, precision=prec,
+                                            jacobian_f32=prec == api.PREC_FP32_SPLIT, n_streams=ns, out=prev)
+                out = run()
+                for _ in range(args.warmup):
+                    out = run()
+                ctx.synchronize()
+                ctx.profile_enable(True)
+                t0 = time.perf_counter()
+                for _ in range(steps):
+                    out = run()
+                ctx.synchronize()
+                torch.cuda.synchronize()
+                t = (time.perf_counter() - t0) / steps
+                ms, nl = ctx.profile_query_kinds()
+                ctx.profile_enable(False)
+                r[mode] = {"ms_per_step": t * 1e3, "profile_channels_per_s": M * L / t,
+                           "kernel_ms": {"prep": ms[0] / steps, "clear": ms[1] / steps, "scattering": ms[2] / steps}}
+                rts, kd = out
+                r[mode + "_bt"] = rts["brightness_temperature"].cpu().numpy()
+                if kd is not None:
+                    r["k"] = {k: kd[k].double().cpu().numpy() for k in
+                              ("temperature_k", "cloud_water_content_k", "cloud_effective_radius_k")}
+                log(f"  {ns} streams {name} {mode}: {t * 1e3:.2f} ms/step  kernels {r[mode]['kernel_ms']}")
+            res[name] = r
+        _, bt_or = orc.forward_allsky_batch(S, C, sub, emis_h[:n_sample], n_streams=ns, n_threads=threads)
+        kbt_or, tk_or, wck_or = orc.k_matrix_allsky_batch(S, C, ksub, emis_h[:n_ksample], n_streams=ns,
+                                                         n_threads=threads)
+        nk = int(ps.layer_offset[n_ksample])
+        nc = ps.cloud_type.shape[1]
+
+        def kerr(prec_name):   # largest relative error per profile and block, vs the oracle adjoint
+            kk = res[prec_name]["k"]
+            worst = {"T": 0.0, "water_content": 0.0}
+            for m in range(n_ksample):
+                a, b = int(ps.layer_offset[m]), int(ps.layer_offset[m + 1])
+                want = tk_or[a * L:b * L].reshape(L, b - a)
+                got = kk["temperature_k"][a:b].T
+                worst["T"] = max(worst["T"], float(np.abs(got - want).max() / np.abs(want).max()))
+                want = wck_or[a * nc * L:b * nc * L].reshape(L, nc, b - a)
+                got = kk["cloud_water_content_k"][:, a:b, :].transpose(2, 0, 1)
+                sc = np.abs(want).max()
+                if sc > 0:
+                    worst["water_content"] = max(worst["water_content"], float(np.abs(got - want).max() / sc))
+            return worst
+        jac_split_vs_fp64 = {}
+        for key in ("temperature_k", "cloud_water_content_k", "cloud_effective_radius_k"):
+            a_, b_ = res["split"]["k"][key], res["fp64"]["k"][key]
+            jac_split_vs_fp64[key] = float(np.abs(a_ - b_).max() / max(np.abs(b_).max(), 1e-300))
         by_streams[ns] = {
-            "split": {k: v for k, v in res["split"].items() if k != "bt"},
-            "fp64": {k: v for k, v in res["fp64"].items() if k != "bt"},
-            "speedup_split_over_fp64": res["fp64"]["ms_per_step"] / res["split"]["ms_per_step"],
-            "bt_err_split_vs_fp64_K": float(np.abs(res["split"]["bt"] - res["fp64"]["bt"]).max()),
-            "bt_err_split_vs_oracle_K": float(np.abs(res["split"]["bt"][:n_sample] - bt_or).max()),
-            "bt_err_fp64_vs_oracle_K": float(np.abs(res["fp64"]["bt"][:n_sample] - bt_or).max()),
+            "split": {m_: res["split"][m_] for m_ in ("forward", "k_matrix")},
+            "fp64": {m_: res["fp64"][m_] for m_ in ("forward", "k_matrix")},
+            "speedup_split_over_fp64_k": res["fp64"]["k_matrix"]["ms_per_step"] / res["split"]["k_matrix"]["ms_per_step"],
+            "speedup_split_over_fp64_forward": res["fp64"]["forward"]["ms_per_step"] / res["split"]["forward"]["ms_per_step"],
+            "bt_err_split_vs_fp64_K": float(np.abs(res["split"]["forward_bt"] - res["fp64"]["forward_bt"]).max()),
+            "bt_err_split_vs_oracle_K": float(np.abs(res["split"]["forward_bt"][:n_sample] - bt_or).max()),
+            "bt_err_fp64_vs_oracle_K": float(np.abs(res["fp64"]["forward_bt"][:n_sample] - bt_or).max()),
+            "k_bt_err_fp64_vs_oracle_K": float(np.abs(res["fp64"]["k_matrix_bt"][:n_ksample] - kbt_or).max()),
+            "jacobian_rel_err_split_vs_fp64": jac_split_vs_fp64,
+            "jacobian_rel_err_fp64_vs_oracle": kerr("fp64"),
+            "jacobian_rel_err_split_vs_oracle": kerr("split"),
         }
+        _ = nk
     cpu = None
     if not args.no_cpu_baseline:
-        threads = min(16, os.cpu_count() or 1)
-        n = 64
+        n = 32
         t0 = time.perf_counter()
-        orc.forward_allsky_batch(S, C, _subset(ps, 0, n), emis_h[:n], n_streams=8, n_threads=threads)
+        orc.k_matrix_allsky_batch(S, C, _subset(ps, 0, n), emis_h[:n], n_streams=8, n_threads=threads)
         dt = time.perf_counter() - t0
         n = min(M, max(n, int(n * args.cpu_seconds / max(dt, 1e-6))))
         t0 = time.perf_counter()
-        orc.forward_allsky_batch(S, C, _subset(ps, 0, n), emis_h[:n], n_streams=8, n_threads=threads)
+        orc.k_matrix_allsky_batch(S, C, _subset(ps, 0, n), emis_h[:n], n_streams=8, n_threads=threads)
         dt = time.perf_counter() - t0
         cpu = {"value": n * L / dt, "unit": "profile-channels/s", "cores": threads, "kind": "port",
-               "sample": f"{n} of the {M} profiles x {L} channels, all-sky forward at 8 streams, FP64 oracle "
-                         f"(oracle_forward_allsky_batch), {dt:.1f} s"}
-    v = by_streams[8]["split"]["profile_channels_per_s"]
-    line = {"metric": "all-sky forward profile-channels/s, FP32-split vs FP64 GPU", "value": v,
+               "sample": f"{n} of the {M} profiles x {L} channels, all-sky forward + K-matrix at 8 streams, FP64 "
+                         f"oracle hand-written adjoint (oracle_k_matrix_allsky_ad_batch), {dt:.1f} s"}
+    v = by_streams[8]["split"]["k_matrix"]["profile_channels_per_s"]
+    line = {"metric": "all-sky forward+K-matrix profile-channels/s, FP32-split vs FP64 GPU", "value": v,
             "unit": "profile-channels/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
-            "ms_per_step": by_streams[8]["split"]["ms_per_step"], "higher_is_better": True, "scaling": "weak",
-            "vs_baseline": None, "dtype": "fp32+fp64-split",
+            "ms_per_step": by_streams[8]["split"]["k_matrix"]["ms_per_step"], "higher_is_better": True,
+            "scaling": "weak", "vs_baseline": None, "dtype": "fp32+fp64-split",
             "data": "synthetic (seeded cloudy profiles; synthetic ODPS/SpcCoeff/CloudCoeff files)",
-            "config": {"workload": f"all-sky microwave forward, ATMS-like 22-ch ODPS group 3, {M} profiles x "
-                                   f"{K} layers, liquid/ice/rain/snow in 0-10 layers each, ADA 4 and 8 streams",
+            "config": {"workload": f"all-sky microwave forward + K-matrix, ATMS-like 22-ch ODPS group 3, {M} profiles "
+                                   f"x {K} layers, liquid/ice/rain/snow in 0-10 layers each, ADA 4 and 8 streams",
                        "n_profiles": M, "n_channels": L, "n_layers": K, "parallelism": "single GPU"},
             "by_streams": by_streams, "roofline": None, "cpu_baseline": cpu}
     print(json.dumps(line), flush=True)

@@ -163,6 +163,10 @@ int oracle_k_matrix_allsky(const or_sensor* s, const or_cloudcoeff* cc, const or
 int oracle_k_matrix_allsky_ad(const or_sensor* s, const or_cloudcoeff* cc, const or_profile* p, const or_clouds* c,
                               const double* emissivity, const or_scatter_opts* opt, const double* bt_k,
                               const double* rad_k, or_allsky_out* fo, or_allsky_k* ko);
+int oracle_k_matrix_allsky_ad_batch(const or_sensor* s, const or_cloudcoeff* cc, int n_profiles, const or_profile* p,
+                                    const or_clouds* c, const double* emissivity, const or_scatter_opts* opt,
+                                    const double* bt_k, double* bt, double* t_k, double* wc_k,
+                                    const long long* offset, int nc_stride, int n_threads);
 int oracle_forward_allsky_batch(const or_sensor* s, const or_cloudcoeff* cc, int n_profiles, const or_profile* p,
                                 const or_clouds* c, const double* emissivity, const or_scatter_opts* opt,
                                 double* radiance, double* bt, int n_threads);

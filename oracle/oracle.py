@@ -110,6 +110,10 @@ def lib():
                                                 ctypes.POINTER(OrScatterOpts), _dp, _dp,
                                                 ctypes.POINTER(OrAllskyOut), ctypes.POINTER(OrAllskyK)]
         _lib.oracle_k_matrix_allsky_ad.argtypes = _lib.oracle_k_matrix_allsky.argtypes
+        _lib.oracle_k_matrix_allsky_ad_batch.argtypes = [
+            ctypes.POINTER(OrSensor), ctypes.POINTER(OrCloudCoeff), ctypes.c_int, ctypes.POINTER(OrProfile),
+            ctypes.POINTER(OrClouds), _dp, ctypes.POINTER(OrScatterOpts), _dp, _dp, _dp, _dp,
+            ctypes.POINTER(ctypes.c_longlong), ctypes.c_int, ctypes.c_int]
         _lib.oracle_cloud_optics.argtypes = [ctypes.POINTER(OrCloudCoeff), ctypes.c_int, ctypes.c_double,
                                              ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int,
                                              _dp, _dp, _dp]
@@ -342,6 +346,33 @@ def forward_allsky_batch(sensor: Sensor, cloudcoeff: CloudCoeff, ps, emissivity,
     if rc != 0:
         raise RuntimeError(f"oracle_forward_allsky_batch failed ({rc})")
     return rad, bt
+
+
+def k_matrix_allsky_batch(sensor: Sensor, cloudcoeff: CloudCoeff, ps, emissivity, n_streams=0, overlap=0,
+                          bt_k=None, n_threads=0):
+    """All-sky K-matrix of every profile by the hand-written adjoint (OpenMP
+    over profiles): (bt [M, L], t_k [NT, L, ...] packed per profile as [L, K],
+    water-content K packed per profile as [L, nc, K])."""
+    M, L = ps.n_profiles, sensor.n_channels
+    keep = []
+    profs = (OrProfile * M)(*[make_profile(ps, m, keep) for m in range(M)])
+    clouds = (OrClouds * M)(*[make_clouds(ps, m, keep) or OrClouds() for m in range(M)])
+    nc = int(ps.cloud_type.shape[1]) if getattr(ps, "cloud_type", None) is not None else 0
+    off = np.ascontiguousarray(ps.layer_offset[:M] - ps.layer_offset[0], np.int64)
+    NT = int(ps.layer_offset[M] - ps.layer_offset[0])
+    bt = np.zeros((M, L))
+    tk = np.zeros(NT * L)
+    wck = np.zeros(max(nc, 1) * NT * L)
+    eps = np.ascontiguousarray(np.broadcast_to(emissivity, (M, L)), np.float64)
+    btk = np.ascontiguousarray(np.ones((M, L)) if bt_k is None else bt_k, np.float64)
+    opt = OrScatterOpts(int(n_streams), int(overlap))
+    rc = lib().oracle_k_matrix_allsky_ad_batch(
+        ctypes.byref(sensor.s), ctypes.byref(cloudcoeff.c), M, profs, clouds, _p(eps, _dp), ctypes.byref(opt),
+        _p(btk, _dp), _p(bt, _dp), _p(tk, _dp), _p(wck, _dp), off.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)),
+        max(nc, 1), n_threads)
+    if rc != 0:
+        raise RuntimeError(f"oracle_k_matrix_allsky_ad_batch failed ({rc})")
+    return bt, tk, wck
 
 
 def k_matrix_allsky(sensor: Sensor, cloudcoeff: CloudCoeff, ps, m, emissivity, n_streams=0, overlap=0,
