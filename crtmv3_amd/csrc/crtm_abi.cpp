@@ -632,7 +632,7 @@ struct crtm_context {
   int n_channels = 0;
   std::string last_error;
   // work pool (grown on demand, reused across calls)
-  DevBuf wx, wu, wo, wacci, waccw, wuidx, wufrac, wgeo, wstatus, wuwin, wfrange;
+  DevBuf wx, wu, wo, wacci, waccw, wuidx, wufrac, wgeo, wstatus, wuwin, wfrange, waccw64, wlm;
   DevBuf stage;  // host-mode staging of inputs/outputs
   DevBuf clim;   // model climatology table (AddLayers)
   DevBuf xnl, xloff, xnx, xlp, xp, xt, xa;   // profiles extended by AddLayers
@@ -682,6 +682,10 @@ ProfWork<RT> make_work(crtm_context* c, int M, int Ko, int nx, int max_k) {
   c->wuwin.reserve((size_t)M * max_k * MAP_WIN * sizeof(RT));
   c->wfrange.reserve((size_t)M * Ko * 2 * sizeof(int));
   c->wstatus.reserve((size_t)M * sizeof(int));
+  c->waccw64.reserve((size_t)M * NZ_MAX * sizeof(double));
+  c->wlm.reserve((size_t)(3 * max_k + 1) * M * sizeof(double));
+  w.accw64 = c->waccw64.as<double>();
+  w.lm = c->wlm.as<double>();
   w.x = c->wx.as<RT>();
   w.u = c->wu.as<RT>();
   w.o = c->wo.as<RT>();

@@ -194,6 +194,9 @@ template <class RT> struct ProfWork {
   RT* uwin;       // [M][max_k][MAP_WIN]: Map_Input weights of user layer i from ODPS layers kmin(i)+c
   int* frange;    // [M][Ko][2]: user layers whose last Map_Input contribution comes from ODPS layer k
   int* status;    // [M]
+  double* accw64; // [M][NZ_MAX]: the LayerAvg weights in FP64 (mapping of T and absorbers)
+  double* lm;     // [3 * max_k + 1][M]: per-profile columns of the layer-mapping kernel
+                  // (user layer lnP, user level lnP, LayerAvg accumulation column)
 };
 
 // All-sky K-matrix: adjoints the scattering path hands to the clear-sky

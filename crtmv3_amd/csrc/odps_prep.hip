@@ -125,129 +125,240 @@ void launch_add_layers(const DevBatch& b, const ExtBatch& e, const double* clim,
                      ids[4], ids[5]);
 }
 
+// ---------------------------------------------------------------------------
+// Layer mapping, one lane per profile: LayerAvg (ODPS_CoordinateMapping.f90:
+// 1108-1320), Compute_Interp_Index (:963-1012) and the Map_Input_AD window
+// tables of the K-matrix kernel.  These are sequential recurrences (LayerAvg's
+// running search start, the bracket merge), so each profile's chain runs in
+// its own lane and the batch's chains run side by side; per-profile columns
+// live in HBM laid out [row][profile] so lanes stepping through the same rows
+// coalesce.
+// ---------------------------------------------------------------------------
+template <class RT>
+__global__ void __launch_bounds__(64) layer_map_kernel(DevSensor s, DevBatch bt, ProfWork<RT> w) {
+  __shared__ double rlnp[KO_MAX], rlnp_lev[KO_MAX + 1];
+  const int Ko = s.Ko;
+  for (int k = threadIdx.x; k < Ko && k < KO_MAX; k += 64) rlnp[k] = log(s.ref_p[k]);
+  for (int k = threadIdx.x; k <= Ko && k <= KO_MAX; k += 64) rlnp_lev[k] = log(s.ref_lp[k]);
+  __syncthreads();
+  const int m = blockIdx.x * 64 + threadIdx.x;
+  if (m >= bt.M) return;
+  const int K = bt.n_layers[m];
+  if (K < 1 || K > K_MAX || K > bt.max_k || Ko > KO_MAX) {
+    w.status[m] = ST_LAYERS;
+    return;
+  }
+  const int64_t M = bt.M, loff = bt.loff[m];
+  const double* lp_u = bt.level_p + loff + m;   // [K+1]
+  const double* p_u = bt.p + loff;
+  double* ul = w.lm + m;                                   // user layer lnP, row k
+  double* ulv = w.lm + (int64_t)bt.max_k * M + m;          // user level lnP, row k (0..K)
+  double* pz = w.lm + (int64_t)(2 * bt.max_k + 1) * M + m; // LayerAvg accumulation column
+  for (int k = 0; k < K; ++k) {
+    ul[k * M] = log(p_u[k]);
+    pz[k * M] = 0.0;
+  }
+  for (int k = 0; k <= K; ++k) ulv[k * M] = (k == 0 && lp_u[0] <= 0.0) ? log(lp_u[1] / 2.0) : log(lp_u[k]);
+  int* acc = w.acc_i + (size_t)m * Ko * 3;
+  double* aw64 = w.accw64 + (size_t)m * NZ_MAX;
+  RT* aw = w.acc_w + (size_t)m * NZ_MAX;
+
+  // ---- LayerAvg: weights of user layers in each ODPS layer ----
+  const int kn1 = Ko, kn2 = K;
+  auto P1 = [&](int i) { return rlnp[i - 1]; };
+  auto P2 = [&](int i) { return ul[(int64_t)(i - 1) * M]; };
+  auto PZ = [&](int i) -> double& { return pz[(int64_t)(i - 1) * M]; };
+  int istart = 1;
+  int st = ST_OK;
+  int nz = 0;
+  const double p2n = P2(kn2);
+  for (int ki = 1; ki <= kn1 && st == ST_OK; ++ki) {
+    double z2 = P1(ki);
+    double z1 = (ki == 1) ? 2.0 * z2 - P1(ki + 1) : P1(ki - 1);
+    double z3 = (ki == kn1) ? 2.0 * z2 - z1 : P1(ki + 1);
+    if (z3 > p2n) z3 = p2n;
+    int iskip = 0;
+    if (z2 >= p2n) { z3 = p2n; z2 = p2n; iskip = 1; }
+    int ic = 0, j, lo = kn2 + 1, hi = 0;
+    bool jumped = false;
+    double pj1 = P2(istart);
+    for (j = istart; j <= kn2 - 1; ++j) {
+      const double pj = pj1;   // P2(j)
+      if (pj > z3) { jumped = true; break; }
+      pj1 = P2(j + 1);
+      for (int side = 0; side < 2; ++side) {
+        // side 0: the upper half-layer (toward z1); side 1: the lower half (toward z3)
+        double ze = side == 0 ? z1 : z3;
+        bool take = side == 0 ? (pj <= z2 && pj1 > z1) : (pj < z3 && pj1 >= z2 && iskip == 0);
+        if (!take) continue;
+        double zo = side == 0 ? z3 : z1;
+        int itop = 0, ibot = 0;
+        double y1, y2;
+        if (ze < zo) {
+          y1 = ze; if (pj > ze) { y1 = pj; itop = 1; }
+          y2 = z2; if (pj1 < z2) { y2 = pj1; ibot = 1; }
+        } else {
+          y1 = z2; if (pj > z2) { y1 = pj; itop = 1; }
+          y2 = ze; if (pj1 < ze) { y2 = pj1; ibot = 1; }
+        }
+        double dy = y2 - y1, dz = ze - z2;
+        if (fabs(dz) < SMALLDIFF) { st = ST_LAYERAVG; break; }
+        double dzd = 1.0 / dz;
+        double zw1 = (ze - y1) * dzd * dy;
+        double zw2 = (ze - y2) * dzd * dy;
+        double dx = pj1 - pj;
+        if (fabs(dx) < SMALLDIFF) { st = ST_LAYERAVG; break; }
+        double d = (pj1 - z2) * (1.0 / dx);
+        if (ze < zo && ibot == 0) {
+          zw1 = zw1 + zw2 * d;
+          zw2 = zw2 * (1.0 - d);
+        } else if (ze > zo && itop == 0) {
+          zw2 = zw2 + zw1 * (1.0 - d);
+          zw1 = zw1 * d;
+        }
+        PZ(j) += zw1;
+        PZ(j + 1) += zw2;
+        lo = min(lo, j);
+        hi = max(hi, j + 1);
+        ic = 1;
+      }
+      if (st != ST_OK) break;
+    }
+    if (st != ST_OK) break;
+    if (!jumped) j = kn2;
+    if (ic == 0) { PZ(j) = 1.0; lo = min(lo, j); hi = max(hi, j); }
+    int i1 = istart;
+    for (int ii = istart; ii <= hi; ++ii)
+      if (PZ(ii) != 0.0) { i1 = ii; break; }
+    istart = i1;
+    int i2 = kn2;
+    for (int ii = i1 + 1; ii <= kn2; ++ii)
+      if (ii > hi || PZ(ii) == 0.0) { i2 = ii - 1; break; }
+    int n = i2 - i1 + 1;
+    if (nz + n > NZ_MAX) { st = ST_NZ; break; }
+    double zsum = 0.0;
+    for (int ii = i1; ii <= i2; ++ii) zsum += PZ(ii);
+    for (int ii = i1; ii <= i2; ++ii) {
+      const double v = PZ(ii) / zsum;
+      aw64[nz + ii - i1] = v;
+      aw[nz + ii - i1] = (RT)v;
+    }
+    acc[3 * (ki - 1)] = i1 - 1;
+    acc[3 * (ki - 1) + 1] = n;
+    acc[3 * (ki - 1) + 2] = nz;
+    nz += n;
+    for (int ii = lo; ii <= hi; ++ii) PZ(ii) = 0.0;
+  }
+  if (st != ST_OK) {
+    w.status[m] = st;
+    return;
+  }
+
+  // ---- ODPS level brackets of the user levels (Compute_Interp_Index) ----
+  {
+    const int nx = Ko + 1, nu = K + 1;
+    int* ui = w.uidx + (size_t)m * (bt.max_k + 1);
+    RT* uf = w.ufrac + (size_t)m * (bt.max_k + 1);
+    auto UL = [&](int i) { return ulv[(int64_t)i * M]; };
+    int k1 = nu + 1;
+    for (int iu = 1; iu <= nu; ++iu) {
+      if (UL(iu - 1) < rlnp_lev[0]) { ui[iu - 1] = 0; uf[iu - 1] = RT(-1); }
+      else { k1 = iu; break; }
+    }
+    int k2 = 0;
+    for (int iu = nu; iu >= k1; --iu) {
+      if (UL(iu - 1) > rlnp_lev[nx - 1]) { ui[iu - 1] = nx - 1; uf[iu - 1] = RT(-1); }
+      else { k2 = iu; break; }
+    }
+    int j = 1;
+    for (int iu = k1; iu <= k2; ++iu) {
+      bool found = false;
+      const double u = UL(iu - 1);
+      for (int ix = j; ix <= nx - 1; ++ix) {
+        if (u >= rlnp_lev[ix - 1] && u <= rlnp_lev[ix]) {
+          ui[iu - 1] = ix - 1;
+          uf[iu - 1] = (RT)((u - rlnp_lev[ix - 1]) / (rlnp_lev[ix] - rlnp_lev[ix - 1]));
+          j = ix;
+          found = true;
+          break;
+        }
+      }
+      if (!found) st = ST_BRACKET;
+    }
+  }
+  // widest run of ODPS layers whose averaging supports share one user
+  // layer: the K-matrix kernel folds Map_Input_AD into its sweep with a
+  // register window when this fits the window
+  int span = 0;
+  {
+    // user layer iu is complete after ODPS layer kmin(iu), the first whose
+    // support reaches it; its weights from layers kmin(iu)+c, c < MAP_WIN
+    RT* uw = w.uwin + (size_t)m * bt.max_k * MAP_WIN;
+    int kmin = 0, kmax = -1;
+    for (int iu = 0; iu < K; ++iu) {
+      while (kmax + 1 < Ko && acc[3 * (kmax + 1)] <= iu) ++kmax;
+      while (kmin < Ko && acc[3 * kmin] + acc[3 * kmin + 1] - 1 < iu) ++kmin;
+      span = max(span, kmax - kmin + 1);
+      for (int c = 0; c < MAP_WIN; ++c) {
+        const int kk = kmin + c;
+        double wv = 0.0;
+        if (kk < Ko && kk <= kmax && iu >= acc[3 * kk] && iu < acc[3 * kk] + acc[3 * kk + 1])
+          wv = aw64[acc[3 * kk + 2] + iu - acc[3 * kk]];
+        uw[(size_t)iu * MAP_WIN + c] = (RT)wv;
+      }
+    }
+    // rows completed at ODPS layer k: (i2(k-1), i2(k)], i2 = last user layer of the support
+    int* fr = w.frange + (size_t)m * Ko * 2;
+    for (int k = 0; k < Ko; ++k) {
+      const int i2 = acc[3 * k] + acc[3 * k + 1] - 1;
+      const int lo = k > 0 ? acc[3 * (k - 1)] + acc[3 * (k - 1) + 1] : 0;
+      if (i2 < lo - 1) span = 1 << 20;   // supports not ordered: the kernel takes the general path
+      fr[2 * k] = lo;
+      fr[2 * k + 1] = i2;
+    }
+  }
+  w.geo[(size_t)m * 4 + 3] = (double)span;
+  w.status[m] = st;
+}
+
+// ---------------------------------------------------------------------------
+// Per-layer preparation, one workgroup per profile (after layer_map_kernel):
+// geometry, mapped T / absorbers with the Min/Max clamps, secants, integrated
+// ratios, predictors, the adjoint table and the OPTRAN table.
+// ---------------------------------------------------------------------------
 template <int G, class RT>
 __global__ void __launch_bounds__(PREP_THREADS) odps_prep_kernel(DevSensor s, DevBatch bt, ProfWork<RT> w) {
   using T = GT<G>;
+  constexpr int NA = T::NABS;
+  constexpr int OPT = G == 3 ? 1 : KO_MAX;   // OPTRAN tables only for the infrared groups
   const int m = blockIdx.x;
   const int tid = threadIdx.x;
   const int Ko = s.Ko;
   const int K = bt.n_layers[m];
   const int64_t loff = bt.loff[m];
   const double* lp_u = bt.level_p + loff + m;   // [K+1]
-  const double* p_u = bt.p + loff;
   const double* t_u = bt.t + loff;
 
-  __shared__ double ulnp[K_MAX];          // user layer lnP
-  __shared__ double ulnp_lev[K_MAX + 1];  // user level lnP
-  __shared__ double rlnp[KO_MAX];
   __shared__ double rlnp_lev[KO_MAX + 1];
-  __shared__ double pz[K_MAX];            // LayerAvg scratch column
-  __shared__ double accw[NZ_MAX];          // compact LayerAvg weights, ODPS layer k at acc_off[k]
   __shared__ int acc_i1[KO_MAX], acc_n[KO_MAX], acc_off[KO_MAX];
   __shared__ double t_o[KO_MAX];
-  __shared__ double a_o[6][KO_MAX];
+  __shared__ double a_o[NA][KO_MAX];
   __shared__ double sec_o[KO_MAX];
-  __shared__ double tz[KO_MAX], tzp[KO_MAX], gazp[6][KO_MAX], gatzp[6][KO_MAX];
-  __shared__ double iv_s[KO_MAX], ave_s[KO_MAX], st_s[KO_MAX], sp_s[KO_MAX], xlt_s[KO_MAX], xlp_s[KO_MAX];
+  __shared__ double tz[KO_MAX], tzp[KO_MAX], gazp[NA][KO_MAX], gatzp[NA][KO_MAX];
+  __shared__ double iv_s[OPT], ave_s[OPT], st_s[OPT], sp_s[OPT], xlt_s[OPT], xlp_s[OPT];
   __shared__ double geo[4];
-  __shared__ int status;
 
-  if (tid == 0) status = (K < 1 || K > K_MAX || Ko > KO_MAX) ? ST_LAYERS : ST_OK;
-  __syncthreads();
-  if (status != ST_OK) {
-    if (tid == 0) w.status[m] = status;
-    return;
+  // status of the layer mapping (uniform over the workgroup)
+  if (w.status[m] != ST_OK) return;
+  const double* aw64 = w.accw64 + (size_t)m * NZ_MAX;
+  for (int k = tid; k < Ko; k += PREP_THREADS) {
+    const int* a = w.acc_i + ((size_t)m * Ko + k) * 3;
+    acc_i1[k] = a[0];
+    acc_n[k] = a[1];
+    acc_off[k] = a[2];
   }
-  for (int k = tid; k < K; k += PREP_THREADS) {
-    ulnp[k] = log(p_u[k]);
-    pz[k] = 0.0;
-  }
-  for (int k = tid; k <= K; k += PREP_THREADS)
-    ulnp_lev[k] = (k == 0 && lp_u[0] <= 0.0) ? log(lp_u[1] / 2.0) : log(lp_u[k]);
-  for (int k = tid; k < Ko; k += PREP_THREADS) rlnp[k] = log(s.ref_p[k]);
   for (int k = tid; k <= Ko; k += PREP_THREADS) rlnp_lev[k] = log(s.ref_lp[k]);
-  __syncthreads();
-
-  // ---- LayerAvg (lane 0): weights of user layers in each ODPS layer ----
   if (tid == 0) {
-    const int kn1 = Ko, kn2 = K;
-    auto P1 = [&](int i) { return rlnp[i - 1]; };
-    auto P2 = [&](int i) { return ulnp[i - 1]; };
-    int istart = 1;
-    int st = ST_OK;
-    int nz = 0;
-    for (int ki = 1; ki <= kn1 && st == ST_OK; ++ki) {
-      double z2 = P1(ki);
-      double z1 = (ki == 1) ? 2.0 * z2 - P1(ki + 1) : P1(ki - 1);
-      double z3 = (ki == kn1) ? 2.0 * z2 - z1 : P1(ki + 1);
-      if (z3 > P2(kn2)) z3 = P2(kn2);
-      int iskip = 0;
-      if (z2 >= P2(kn2)) { z3 = P2(kn2); z2 = P2(kn2); iskip = 1; }
-      int ic = 0, j, lo = kn2 + 1, hi = 0;
-      bool jumped = false;
-      for (j = istart; j <= kn2 - 1; ++j) {
-        if (P2(j) > z3) { jumped = true; break; }
-        for (int side = 0; side < 2; ++side) {
-          // side 0: the upper half-layer (toward z1); side 1: the lower half (toward z3)
-          double ze = side == 0 ? z1 : z3;
-          bool take = side == 0 ? (P2(j) <= z2 && P2(j + 1) > z1)
-                                : (P2(j) < z3 && P2(j + 1) >= z2 && iskip == 0);
-          if (!take) continue;
-          double zo = side == 0 ? z3 : z1;
-          int itop = 0, ibot = 0;
-          double y1, y2;
-          if (ze < zo) {
-            y1 = ze; if (P2(j) > ze) { y1 = P2(j); itop = 1; }
-            y2 = z2; if (P2(j + 1) < z2) { y2 = P2(j + 1); ibot = 1; }
-          } else {
-            y1 = z2; if (P2(j) > z2) { y1 = P2(j); itop = 1; }
-            y2 = ze; if (P2(j + 1) < ze) { y2 = P2(j + 1); ibot = 1; }
-          }
-          double dy = y2 - y1, dz = ze - z2;
-          if (fabs(dz) < SMALLDIFF) { st = ST_LAYERAVG; break; }
-          double dzd = 1.0 / dz;
-          double zw1 = (ze - y1) * dzd * dy;
-          double zw2 = (ze - y2) * dzd * dy;
-          double dx = P2(j + 1) - P2(j);
-          if (fabs(dx) < SMALLDIFF) { st = ST_LAYERAVG; break; }
-          double d = (P2(j + 1) - z2) * (1.0 / dx);
-          if (ze < zo && ibot == 0) {
-            zw1 = zw1 + zw2 * d;
-            zw2 = zw2 * (1.0 - d);
-          } else if (ze > zo && itop == 0) {
-            zw2 = zw2 + zw1 * (1.0 - d);
-            zw1 = zw1 * d;
-          }
-          pz[j - 1] += zw1;
-          pz[j] += zw2;
-          lo = min(lo, j);
-          hi = max(hi, j + 1);
-          ic = 1;
-        }
-        if (st != ST_OK) break;
-      }
-      if (st != ST_OK) break;
-      if (!jumped) j = kn2;
-      if (ic == 0) { pz[j - 1] = 1.0; lo = min(lo, j); hi = max(hi, j); }
-      int i1 = istart;
-      for (int ii = istart; ii <= hi; ++ii)
-        if (pz[ii - 1] != 0.0) { i1 = ii; break; }
-      istart = i1;
-      int i2 = kn2;
-      for (int ii = i1 + 1; ii <= kn2; ++ii)
-        if (ii > hi || pz[ii - 1] == 0.0) { i2 = ii - 1; break; }
-      int n = i2 - i1 + 1;
-      if (nz + n > NZ_MAX) { st = ST_NZ; break; }
-      double zsum = 0.0;
-      for (int ii = i1; ii <= i2; ++ii) zsum += pz[ii - 1];
-      for (int ii = i1; ii <= i2; ++ii) accw[nz + ii - i1] = pz[ii - 1] / zsum;
-      acc_off[ki - 1] = nz;
-      nz += n;
-      acc_i1[ki - 1] = i1 - 1;
-      acc_n[ki - 1] = n;
-      for (int ii = lo; ii <= hi; ++ii) pz[ii - 1] = 0.0;
-    }
-    status = st;
     // geometry (CRTM_GeometryInfo_Compute) and the surface temperature
     double zen = bt.zenith[m];
     double zen_rad = D2R * zen;
@@ -260,12 +371,12 @@ __global__ void __launch_bounds__(PREP_THREADS) odps_prep_kernel(DevSensor s, De
     for (int c = 0; c < 4; ++c) ts += bt.cov[c][m] * bt.sfc_t[c][m];
     geo[2] = ts;
     geo[3] = trans_rad;
+    double* g = w.geo + (size_t)m * 4;
+    g[0] = geo[0];
+    g[1] = geo[1];
+    g[2] = geo[2];
   }
   __syncthreads();
-  if (status != ST_OK) {
-    if (tid == 0) w.status[m] = status;
-    return;
-  }
 
   // ---- mapped temperature / absorbers on ODPS layers, secants ----
   const double alt = bt.altitude ? bt.altitude[m] : 0.0;
@@ -279,7 +390,7 @@ __global__ void __launch_bounds__(PREP_THREADS) odps_prep_kernel(DevSensor s, De
       for (int k = Ko; k >= 0; --k)
         if (s.ref_lp[k] < psfc) {
           sfc_idx = k + 1;
-          frac = (rlnp_lev[sfc_idx] - ulnp_lev[K]) / (rlnp_lev[sfc_idx] - rlnp_lev[k]);
+          frac = (rlnp_lev[sfc_idx] - log(lp_u[K])) / (rlnp_lev[sfc_idx] - rlnp_lev[k]);
           break;
         }
     }
@@ -292,7 +403,7 @@ __global__ void __launch_bounds__(PREP_THREADS) odps_prep_kernel(DevSensor s, De
   for (int k = tid; k < Ko; k += PREP_THREADS) {
     const int i1 = acc_i1[k], n = acc_n[k];
     double sum = 0.0;
-    for (int q = 0; q < n; ++q) sum += accw[acc_off[k] + q] * t_u[i1 + q];
+    for (int q = 0; q < n; ++q) sum += aw64[acc_off[k] + q] * t_u[i1 + q];
     t_o[k] = sum;
     for (int j = 0; j < T::NABS; ++j) {
       const int ju = bt.jmap[j];
@@ -300,7 +411,7 @@ __global__ void __launch_bounds__(PREP_THREADS) odps_prep_kernel(DevSensor s, De
       if (ju >= 0) {
         const double* au = bt.a + (size_t)ju * bt.NT + loff;
         double sm = 0.0;
-        for (int q = 0; q < n; ++q) sm += accw[acc_off[k] + q] * au[i1 + q];
+        for (int q = 0; q < n; ++q) sm += aw64[acc_off[k] + q] * au[i1 + q];
         v = sm;
         if (v <= TOLERANCE) v = TOLERANCE;
         const double mn = s.min_a[(size_t)j * Ko + k], mx = s.max_a[(size_t)j * Ko + k];
@@ -499,87 +610,12 @@ __global__ void __launch_bounds__(PREP_THREADS) odps_prep_kernel(DevSensor s, De
       RT* oo = w.o + ((size_t)m * Ko + k) * NO;
       for (int i = 0; i < NO; ++i) oo[i] = (RT)O[i];
     }
-    int* ai = w.acc_i + ((size_t)m * Ko + k) * 3;
-    ai[0] = acc_i1[k];
-    ai[1] = acc_n[k];
-    ai[2] = acc_off[k];
-    RT* aw = w.acc_w + (size_t)m * NZ_MAX + acc_off[k];
-    for (int q = 0; q < acc_n[k]; ++q) aw[q] = (RT)accw[acc_off[k] + q];
-  }
-
-  // ---- ODPS level brackets of the user levels (Compute_Interp_Index) ----
-  if (tid == 0) {
-    const int nx = Ko + 1, nu = K + 1;
-    int* ui = w.uidx + (size_t)m * (bt.max_k + 1);
-    RT* uf = w.ufrac + (size_t)m * (bt.max_k + 1);
-    int st = ST_OK;
-    int k1 = nu + 1;
-    for (int iu = 1; iu <= nu; ++iu) {
-      if (ulnp_lev[iu - 1] < rlnp_lev[0]) { ui[iu - 1] = 0; uf[iu - 1] = RT(-1); }
-      else { k1 = iu; break; }
-    }
-    int k2 = 0;
-    for (int iu = nu; iu >= k1; --iu) {
-      if (ulnp_lev[iu - 1] > rlnp_lev[nx - 1]) { ui[iu - 1] = nx - 1; uf[iu - 1] = RT(-1); }
-      else { k2 = iu; break; }
-    }
-    int j = 1;
-    for (int iu = k1; iu <= k2; ++iu) {
-      bool found = false;
-      for (int ix = j; ix <= nx - 1; ++ix) {
-        const double u = ulnp_lev[iu - 1];
-        if (u >= rlnp_lev[ix - 1] && u <= rlnp_lev[ix]) {
-          ui[iu - 1] = ix - 1;
-          uf[iu - 1] = (RT)((u - rlnp_lev[ix - 1]) / (rlnp_lev[ix] - rlnp_lev[ix - 1]));
-          j = ix;
-          found = true;
-          break;
-        }
-      }
-      if (!found) st = ST_BRACKET;
-    }
-    // widest run of ODPS layers whose averaging supports share one user
-    // layer: the K-matrix kernel folds Map_Input_AD into its sweep with a
-    // register window when this fits the window
-    int span = 0;
-    {
-      // user layer iu is complete after ODPS layer kmin(iu), the first whose
-      // support reaches it; its weights from layers kmin(iu)+c, c < MAP_WIN
-      RT* uw = w.uwin + (size_t)m * bt.max_k * MAP_WIN;
-      int kmin = 0, kmax = -1;
-      for (int iu = 0; iu < K; ++iu) {
-        while (kmax + 1 < Ko && acc_i1[kmax + 1] <= iu) ++kmax;
-        while (kmin < Ko && acc_i1[kmin] + acc_n[kmin] - 1 < iu) ++kmin;
-        span = max(span, kmax - kmin + 1);
-        for (int c = 0; c < MAP_WIN; ++c) {
-          const int kk = kmin + c;
-          double wv = 0.0;
-          if (kk < Ko && kk <= kmax && iu >= acc_i1[kk] && iu < acc_i1[kk] + acc_n[kk])
-            wv = accw[acc_off[kk] + iu - acc_i1[kk]];
-          uw[(size_t)iu * MAP_WIN + c] = (RT)wv;
-        }
-      }
-      // rows completed at ODPS layer k: (i2(k-1), i2(k)], i2 = last user layer of the support
-      int* fr = w.frange + (size_t)m * Ko * 2;
-      for (int k = 0; k < Ko; ++k) {
-        const int i2 = acc_i1[k] + acc_n[k] - 1;
-        const int lo = k > 0 ? acc_i1[k - 1] + acc_n[k - 1] : 0;
-        if (i2 < lo - 1) span = 1 << 20;   // supports not ordered: the kernel takes the general path
-        fr[2 * k] = lo;
-        fr[2 * k + 1] = i2;
-      }
-    }
-    double* g = w.geo + (size_t)m * 4;
-    g[0] = geo[0];
-    g[1] = geo[1];
-    g[2] = geo[2];
-    g[3] = (double)span;
-    w.status[m] = st;
   }
 }
 
 template <int G, class RT>
 void launch_prep(const DevSensor& s, const DevBatch& b, const ProfWork<RT>& w, hipStream_t stream) {
+  hipLaunchKernelGGL((layer_map_kernel<RT>), dim3((b.M + 63) / 64), dim3(64), 0, stream, s, b, w);
   hipLaunchKernelGGL((odps_prep_kernel<G, RT>), dim3(b.M), dim3(PREP_THREADS), 0, stream, s, b, w);
 }
 
