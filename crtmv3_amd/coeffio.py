@@ -388,3 +388,97 @@ def read_cloudcoeff(path: str) -> CloudCoeff:
     return CloudCoeff(fm.copy(), fi.copy(), rm.copy(), ri.copy(), t.copy(), dm.copy(),
                       *[a.copy() for a in lm], *[a.copy() for a in sm], *[a.copy() for a in ir],
                       release=int(rel), version=int(ver))
+
+
+# --------------------------------------------------------------------------
+# AerosolCoeff (binary, release 4, 'CRTM' aerosol scheme)
+# --------------------------------------------------------------------------
+AEROSOLCOEFF_RELEASE = 4
+AEROSOLCOEFF_VERSION = 301   # > 300: the file carries the backscatter table kb
+AEROSOL_STRLEN = 80          # CHARACTER(SL), SL = 80 (AerosolCoeff_Define.f90)
+
+
+@dataclasses.dataclass
+class AerosolCoeff:
+    """Aerosol optical property tables (AerosolCoeff_Define.f90:100-128).
+
+    Shapes (Fortran order): reff (nR, nT), rsig (nS, nT), rh (n_RH_Radii,);
+    ke/w/g/kb (nW, nRH, nR, nS, nT); pcoeff (nW, nRH, nR, nS, nT, nL+1, nP).
+    Wavelengths in microns, stored in descending order so that the
+    frequencies (10^4 / wavelength, cm^-1) the lookup interpolates in ascend.
+    """
+    type_id: np.ndarray
+    type_name: List[str]
+    wavelength: np.ndarray
+    reff: np.ndarray
+    rsig: np.ndarray
+    rh: np.ndarray
+    ke: np.ndarray
+    w: np.ndarray
+    g: np.ndarray
+    kb: np.ndarray
+    pcoeff: np.ndarray
+    data_source: str = "synthetic"
+    release: int = AEROSOLCOEFF_RELEASE
+    version: int = AEROSOLCOEFF_VERSION
+
+    @property
+    def frequency(self) -> np.ndarray:
+        """AerosolCoeff_Frequency: micron_to_inverse_cm of the wavelengths."""
+        return 10000.0 / self.wavelength
+
+    @property
+    def n_legendre_terms(self) -> int:
+        return int(self.pcoeff.shape[5] - 1)
+
+
+def _fixed(s: str, n: int = AEROSOL_STRLEN) -> bytes:
+    return s.encode("ascii").ljust(n)[:n]
+
+
+def write_aerosolcoeff(path: str, ac: AerosolCoeff) -> None:
+    """Record order of AerosolCoeff_Binary_ReadFile (AerosolCoeff_Binary_IO.f90:396-520)."""
+    f8 = lambda a: np.asarray(a, np.float64)
+    nW, nRH, nR, nS, nT = ac.ke.shape
+    with RecordWriter(path) as w:
+        w.record(np.int32(ac.release), np.int32(ac.version))
+        w.record(*[np.int32(x) for x in (nW, nR, nS, nT, nRH, ac.n_legendre_terms, ac.pcoeff.shape[6])])
+        w.record(np.int32(ac.rh.size))                 # n_RH_Radii ('CRTM' and 'CMAQ' schemes)
+        w.record(np.int32(AEROSOL_STRLEN))
+        w.record(_fixed(ac.data_source))
+        w.record(np.asarray(ac.type_id, np.int32))
+        w.record(np.int32(AEROSOL_STRLEN))
+        w.record(b"".join(_fixed(n) for n in ac.type_name))
+        w.record(f8(ac.wavelength), f8(ac.reff), f8(ac.rsig), f8(ac.rh))
+        if ac.version <= 300:
+            w.record(f8(ac.ke), f8(ac.w), f8(ac.g), f8(ac.pcoeff))
+        else:
+            w.record(f8(ac.ke), f8(ac.w), f8(ac.g), f8(ac.kb), f8(ac.pcoeff))
+
+
+def read_aerosolcoeff(path: str) -> AerosolCoeff:
+    r = RecordReader(path)
+    rel, ver = r.read([("i4", ()), ("i4", ())])
+    if rel != AEROSOLCOEFF_RELEASE:
+        raise ValueError(f"AerosolCoeff release {rel} unsupported (need {AEROSOLCOEFF_RELEASE})")
+    nW, nR, nS, nT, nRH, nL, nP = [int(x) for x in r.read([("i4", ())] * 7)]
+    (nRHR,) = r.read([("i4", ())])
+    (sl,) = r.read([("i4", ())])
+    if sl != AEROSOL_STRLEN:
+        raise ValueError("AerosolCoeff data source string length mismatch")
+    src = r._next().decode("ascii").rstrip()
+    (tid,) = r.read([("i4", (nT,))])
+    (sl,) = r.read([("i4", ())])
+    if sl != AEROSOL_STRLEN:
+        raise ValueError("AerosolCoeff type name string length mismatch")
+    raw = r._next()
+    names = [raw[i * sl:(i + 1) * sl].decode("ascii").rstrip() for i in range(nT)]
+    wl, reff, rsig, rh = r.read([("f8", (nW,)), ("f8", (nR, nT)), ("f8", (nS, nT)), ("f8", (int(nRHR),))])
+    shp = (nW, nRH, nR, nS, nT)
+    if ver <= 300:
+        ke, w, g, pc = r.read([("f8", shp)] * 3 + [("f8", shp + (nL + 1, nP))])
+        kb = np.zeros(shp)
+    else:
+        ke, w, g, kb, pc = r.read([("f8", shp)] * 4 + [("f8", shp + (nL + 1, nP))])
+    return AerosolCoeff(tid.copy(), names, wl.copy(), reff.copy(), rsig.copy(), rh.copy(), ke.copy(), w.copy(),
+                        g.copy(), kb.copy(), pc.copy(), data_source=src, release=int(rel), version=int(ver))

@@ -630,3 +630,89 @@ def add_clouds(ps: ProfileSet, seed: int = 20261204,
     out.cloud_effective_radius = re_
     out.cloud_fraction = cf
     return out
+
+
+# --------------------------------------------------------------------------
+# Aerosols
+# --------------------------------------------------------------------------
+AEROSOLCOEFF_FILE = "AerosolCoeff.bin"
+DUST_AEROSOL, SEASALT_SSAM_AEROSOL, SEASALT_SSCM1_AEROSOL, SEASALT_SSCM2_AEROSOL, SEASALT_SSCM3_AEROSOL, \
+    ORGANIC_CARBON_AEROSOL, BLACK_CARBON_AEROSOL, SULFATE_AEROSOL = range(1, 9)
+_AEROSOL_TYPES = [  # id, name, particle density kg/m^3, radius table (um), single-scatter albedo level
+    (DUST_AEROSOL, "Dust", 2600.0, (0.1, 10.0), 0.80),
+    (SEASALT_SSAM_AEROSOL, "Sea salt (SSAM)", 2200.0, (0.2, 20.0), 0.95),
+    (SEASALT_SSCM1_AEROSOL, "Sea salt (SSCM1)", 2200.0, (0.2, 20.0), 0.95),
+    (SEASALT_SSCM2_AEROSOL, "Sea salt (SSCM2)", 2200.0, (0.2, 20.0), 0.95),
+    (SEASALT_SSCM3_AEROSOL, "Sea salt (SSCM3)", 2200.0, (0.2, 20.0), 0.95),
+    (ORGANIC_CARBON_AEROSOL, "Organic carbon", 1800.0, (0.02, 2.0), 0.75),
+    (BLACK_CARBON_AEROSOL, "Black carbon", 1000.0, (0.02, 2.0), 0.25),
+    (SULFATE_AEROSOL, "Sulfate", 1700.0, (0.02, 3.0), 0.90),
+]
+
+
+def make_aerosol_coeff() -> coeffio.AerosolCoeff:
+    """Smooth synthetic aerosol tables in the 'CRTM' scheme layout (one RH
+    and one size-variance entry): extinction from a size-parameter
+    efficiency 2x^2/(1+x^2) over 4/3 rho r, type-dependent albedo, HG-like
+    Legendre coefficients with asymmetry g = x^2/2(1+x^2) <= 1/2."""
+    wl = np.geomspace(50.0, 2.0, 40)                     # microns, descending: frequency ascends
+    nR, nT = 12, len(_AEROSOL_TYPES)
+    nL1 = N_LEGENDRE_FILE + 1
+    reff = np.zeros((nR, nT))
+    shp = (wl.size, 1, nR, 1, nT)
+    ke, w, g = np.zeros(shp), np.zeros(shp), np.zeros(shp)
+    for t, (_, _, rho, (r0, r1), w0) in enumerate(_AEROSOL_TYPES):
+        r = np.geomspace(r0, r1, nR)
+        reff[:, t] = r
+        x = 2.0 * np.pi * r[None, :] / wl[:, None]                       # (wl, r)
+        q = 2.0 * x ** 2 / (1.0 + x ** 2)
+        ke[:, 0, :, 0, t] = 7.5e5 * q / (rho * r[None, :]) + 5.0
+        w[:, 0, :, 0, t] = w0 * (0.6 + 0.4 * x / (1.0 + x)) * (0.9 + 0.1 * np.sin(wl / 3.0))[:, None]
+        g[:, 0, :, 0, t] = 0.5 * x ** 2 / (1.0 + x ** 2)
+    pc = _hg_slots(g, nL1)[..., None]                                     # (wl, 1, r, 1, t, 0:38, 1)
+    return coeffio.AerosolCoeff(
+        type_id=np.array([a[0] for a in _AEROSOL_TYPES], np.int32), type_name=[a[1] for a in _AEROSOL_TYPES],
+        wavelength=wl, reff=reff, rsig=np.ones((1, nT)), rh=np.zeros(1), ke=ke, w=w, g=g, kb=ke * (1.0 - g),
+        pcoeff=pc)
+
+
+def write_aerosol_coeff(dirpath: str) -> str:
+    """Write AerosolCoeff.bin (idempotent); returns its path."""
+    os.makedirs(dirpath, exist_ok=True)
+    path = os.path.join(dirpath, AEROSOLCOEFF_FILE)
+    if not os.path.exists(path):
+        coeffio.write_aerosolcoeff(path + ".tmp", make_aerosol_coeff())
+        os.replace(path + ".tmp", path)
+    return path
+
+
+_AEROSOL_RADII = {DUST_AEROSOL: (0.5, 4.0), SEASALT_SSAM_AEROSOL: (0.5, 5.0), SULFATE_AEROSOL: (0.05, 0.5),
+                  ORGANIC_CARBON_AEROSOL: (0.05, 0.5), BLACK_CARBON_AEROSOL: (0.05, 0.3)}
+
+
+def add_aerosols(ps: ProfileSet, seed: int = 20261205,
+                 types: Sequence[int] = (DUST_AEROSOL, SEASALT_SSAM_AEROSOL, SULFATE_AEROSOL),
+                 n_layers: int = 20, median_kgm2: float = 2.0e-6, sigma: float = 1.0,
+                 aerosol_fraction: float = 1.0) -> ProfileSet:
+    """Copy of ps with aerosols: each type log-normal in mass concentration
+    (median_kgm2 * exp(N(0, sigma)) kg/m^2 per layer) in the lowest n_layers
+    layers, effective radius uniform in the type's range (microns)."""
+    rng = np.random.default_rng(seed)
+    M, na = ps.n_profiles, len(types)
+    ntot = int(ps.layer_offset[-1])
+    conc = np.zeros((na, ntot))
+    re_ = np.zeros((na, ntot))
+    for m in range(M):
+        a, b = int(ps.layer_offset[m]), int(ps.layer_offset[m + 1])
+        if rng.uniform() >= aerosol_fraction:
+            continue
+        k0 = max(a, b - n_layers)
+        for n, t in enumerate(types):
+            lo, hi = _AEROSOL_RADII.get(t, (0.1, 1.0))
+            conc[n, k0:b] = median_kgm2 * np.exp(sigma * rng.standard_normal(b - k0))
+            re_[n, k0:b] = rng.uniform(lo, hi, b - k0)
+    out = dataclasses.replace(ps)
+    out.aerosol_type = np.tile(np.asarray(types, np.int32), (M, 1))
+    out.aerosol_concentration = conc
+    out.aerosol_effective_radius = re_
+    return out

@@ -116,7 +116,19 @@ typedef struct {
   const double *ke_l_mw, *w_l_mw, *pcoeff_l_mw;  /* (f,r,T), (f,r,T,0:nL,nP) */
   const double *ke_s_mw, *w_s_mw, *pcoeff_s_mw;  /* (f,r,D), (f,r,D,0:nL,nP) */
   const double *ke_ir, *w_ir, *pcoeff_ir;        /* (f,r,Dir), (f,r,Dir,1:nL,nP) */
+  const struct or_aerosolcoeff* aerosol;         /* AerosolCoeff tables, NULL when not loaded */
 } or_cloudcoeff;
+
+/* AerosolCoeff release 4, 'CRTM' scheme (src/Coefficients/AerosolCoeff/
+ * AerosolCoeff_Define.f90:100-128); arrays column-major as read from the file. */
+typedef struct or_aerosolcoeff {
+  int n_wavelengths, n_radii, n_sigma, n_types, n_rh, n_legendre_terms, n_phase_elements;
+  const int* type_id;        /* [n_types] aerosol type ids of the table */
+  const double* frequency;   /* [n_wavelengths] cm^-1, ascending (10^4 / wavelength) */
+  const double* reff;        /* (n_radii, n_types) microns */
+  const double *ke, *w;      /* (n_wavelengths, n_rh, n_radii, n_sigma, n_types) */
+  const double* pcoeff;      /* (n_wavelengths, n_rh, n_radii, n_sigma, n_types, 0:nL, nP) */
+} or_aerosolcoeff;
 
 typedef struct {
   int n_clouds;
@@ -124,6 +136,10 @@ typedef struct {
   const double* wc;       /* [n_clouds*K] water content, kg/m^2 */
   const double* reff;     /* [n_clouds*K] effective radius, microns */
   const double* fraction; /* [K] layer cloud fraction (NULL: 0) */
+  int n_aerosols;                 /* Atmosphere%Aerosol(:) slots */
+  const int* aerosol_type;        /* [n_aerosols] type id (0: empty slot) */
+  const double* aerosol_conc;     /* [n_aerosols*K] concentration, kg/m^2 */
+  const double* aerosol_reff;     /* [n_aerosols*K] effective radius, microns */
 } or_clouds;
 
 typedef struct {
@@ -149,6 +165,8 @@ typedef struct {
   double* cf_k;     /* [L][K] Atmosphere_K%Cloud_Fraction */
   double* sfc_t_k;  /* [L][4] */
   double* emis_k;   /* [L] */
+  double* aconc_k;  /* [L][n_aerosols][K] Atmosphere_K%Aerosol(n)%Concentration */
+  double* areff_k;  /* [L][n_aerosols][K] Atmosphere_K%Aerosol(n)%Effective_Radius */
 } or_allsky_k;
 
 /* CRTM_Forward with clouds for one profile, all channels. */
@@ -172,6 +190,8 @@ int oracle_forward_allsky_batch(const or_sensor* s, const or_cloudcoeff* cc, int
                                 double* radiance, double* bt, int n_threads);
 
 /* White-box pieces of the all-sky path. */
+int oracle_aerosol_optics(const or_aerosolcoeff* ac, double wavenumber, int type_id, double reff, int n_legendre,
+                          double* ke, double* w, double* pcoeff);
 int oracle_cloud_optics(const or_cloudcoeff* cc, int microwave, double freq, int type, double reff,
                         double temperature, int n_legendre, double* ke, double* w, double* pcoeff);
 void oracle_lpoly_interp(const double* x, int n, double xi, const double* z, double* out);

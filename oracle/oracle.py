@@ -35,17 +35,25 @@ class OrSensor(ctypes.Structure):
                 ("frequency", _dp), ("wavenumber", _dp)]
 
 
+class OrAerosolCoeff(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int) for n in ("n_wavelengths", "n_radii", "n_sigma", "n_types", "n_rh",
+                                            "n_legendre_terms", "n_phase_elements")] + \
+               [("type_id", _ip)] + [(n, _dp) for n in ("frequency", "reff", "ke", "w", "pcoeff")]
+
+
 class OrCloudCoeff(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int) for n in ("n_mw_frequencies", "n_mw_radii", "n_ir_frequencies", "n_ir_radii",
                                             "n_temperatures", "n_mw_densities", "n_ir_densities",
                                             "n_legendre_terms", "n_phase_elements")] + \
                [(n, _dp) for n in ("frequency_mw", "frequency_ir", "reff_mw", "reff_ir", "temperature",
                                    "density_mw", "ke_l_mw", "w_l_mw", "pcoeff_l_mw", "ke_s_mw", "w_s_mw",
-                                   "pcoeff_s_mw", "ke_ir", "w_ir", "pcoeff_ir")]
+                                   "pcoeff_s_mw", "ke_ir", "w_ir", "pcoeff_ir")] + \
+               [("aerosol", ctypes.POINTER(OrAerosolCoeff))]
 
 
 class OrClouds(ctypes.Structure):
-    _fields_ = [("n_clouds", ctypes.c_int), ("type", _ip), ("wc", _dp), ("reff", _dp), ("fraction", _dp)]
+    _fields_ = [("n_clouds", ctypes.c_int), ("type", _ip), ("wc", _dp), ("reff", _dp), ("fraction", _dp),
+                ("n_aerosols", ctypes.c_int), ("aerosol_type", _ip), ("aerosol_conc", _dp), ("aerosol_reff", _dp)]
 
 
 class OrScatterOpts(ctypes.Structure):
@@ -53,7 +61,8 @@ class OrScatterOpts(ctypes.Structure):
 
 
 class OrAllskyK(ctypes.Structure):
-    _fields_ = [(n, _dp) for n in ("t_k", "abs_k", "wc_k", "reff_k", "cf_k", "sfc_t_k", "emis_k")]
+    _fields_ = [(n, _dp) for n in ("t_k", "abs_k", "wc_k", "reff_k", "cf_k", "sfc_t_k", "emis_k", "aconc_k",
+                                   "areff_k")]
 
 
 class OrAllskyOut(ctypes.Structure):
@@ -118,6 +127,8 @@ def lib():
                                              ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int,
                                              _dp, _dp, _dp]
         _lib.oracle_lpoly_interp.argtypes = [_dp, ctypes.c_int, ctypes.c_double, _dp, _dp]
+        _lib.oracle_aerosol_optics.argtypes = [ctypes.POINTER(OrAerosolCoeff), ctypes.c_double, ctypes.c_int,
+                                               ctypes.c_double, ctypes.c_int, _dp, _dp, _dp]
         _lib.oracle_gauss.argtypes = [ctypes.c_int, _dp, _dp]
         _lib.oracle_phase_matrix.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_int, _dp, _dp, _dp,
                                              _dp, _dp]
@@ -270,9 +281,10 @@ def k_matrix_batch(sensor: Sensor, ps, emissivity, n_threads=0, bt_k=None, rad_k
 
 
 class CloudCoeff:
-    """Oracle view of a coeffio.CloudCoeff (keeps arrays alive)."""
+    """Oracle view of a coeffio.CloudCoeff, with an optional
+    coeffio.AerosolCoeff (keeps arrays alive)."""
 
-    def __init__(self, cc):
+    def __init__(self, cc, aerosol=None):
         self._keep = []
 
         def keep(a):
@@ -289,22 +301,44 @@ class CloudCoeff:
                   "ke_l_mw", "w_l_mw", "pcoeff_l_mw", "ke_s_mw", "w_s_mw", "pcoeff_s_mw", "ke_ir", "w_ir",
                   "pcoeff_ir"):
             setattr(c, n, keep(getattr(cc, n)))
+        if aerosol is not None:
+            a = OrAerosolCoeff()
+            nW, nRH, nR, nS, nT = aerosol.ke.shape
+            a.n_wavelengths, a.n_radii, a.n_sigma, a.n_types, a.n_rh = nW, nR, nS, nT, nRH
+            a.n_legendre_terms, a.n_phase_elements = aerosol.n_legendre_terms, aerosol.pcoeff.shape[6]
+            tid = np.ascontiguousarray(aerosol.type_id, np.int32)
+            self._keep.append(tid)
+            a.type_id = _p(tid, _ip)
+            a.frequency = keep(aerosol.frequency)
+            a.reff, a.ke, a.w, a.pcoeff = keep(aerosol.reff), keep(aerosol.ke), keep(aerosol.w), keep(aerosol.pcoeff)
+            self._a = a
+            c.aerosol = ctypes.pointer(a)
         self.c = c
 
 
 def make_clouds(ps, m, keep):
-    """OrClouds of profile m (None when the set carries no clouds)."""
-    if ps.cloud_type is None:
+    """OrClouds of profile m: clouds and aerosols (None when the set carries neither)."""
+    has_c = getattr(ps, "cloud_type", None) is not None
+    has_a = getattr(ps, "aerosol_type", None) is not None
+    if not has_c and not has_a:
         return None
     a, b = int(ps.layer_offset[m]), int(ps.layer_offset[m + 1])
-    types = np.ascontiguousarray(ps.cloud_type[m], np.int32)
-    wc = np.ascontiguousarray(ps.cloud_water_content[:, a:b], np.float64)
-    re_ = np.ascontiguousarray(ps.cloud_effective_radius[:, a:b], np.float64)
-    cf = None if ps.cloud_fraction is None else np.ascontiguousarray(ps.cloud_fraction[a:b], np.float64)
-    keep.extend([types, wc, re_, cf])
     c = OrClouds()
-    c.n_clouds = types.size
-    c.type, c.wc, c.reff, c.fraction = _p(types, _ip), _p(wc, _dp), _p(re_, _dp), _p(cf, _dp)
+    if has_c:
+        types = np.ascontiguousarray(ps.cloud_type[m], np.int32)
+        wc = np.ascontiguousarray(ps.cloud_water_content[:, a:b], np.float64)
+        re_ = np.ascontiguousarray(ps.cloud_effective_radius[:, a:b], np.float64)
+        cf = None if ps.cloud_fraction is None else np.ascontiguousarray(ps.cloud_fraction[a:b], np.float64)
+        keep.extend([types, wc, re_, cf])
+        c.n_clouds = types.size
+        c.type, c.wc, c.reff, c.fraction = _p(types, _ip), _p(wc, _dp), _p(re_, _dp), _p(cf, _dp)
+    if has_a:
+        at = np.ascontiguousarray(ps.aerosol_type[m], np.int32)
+        ac = np.ascontiguousarray(ps.aerosol_concentration[:, a:b], np.float64)
+        ar = np.ascontiguousarray(ps.aerosol_effective_radius[:, a:b], np.float64)
+        keep.extend([at, ac, ar])
+        c.n_aerosols = at.size
+        c.aerosol_type, c.aerosol_conc, c.aerosol_reff = _p(at, _ip), _p(ac, _dp), _p(ar, _dp)
     return c
 
 
@@ -389,9 +423,10 @@ def k_matrix_allsky(sensor: Sensor, cloudcoeff: CloudCoeff, ps, m, emissivity, n
     fo = OrAllskyOut()
     for k, v in fwd.items():
         setattr(fo, k, _p(v, _ip if k == "n_streams" else _dp))
+    nae = c.n_aerosols if c is not None else 0
     kd = {"t_k": np.zeros((L, K)), "abs_k": np.zeros((L, na, K)), "wc_k": np.zeros((L, max(nc, 1), K)),
           "reff_k": np.zeros((L, max(nc, 1), K)), "cf_k": np.zeros((L, K)), "sfc_t_k": np.zeros((L, 4)),
-          "emis_k": np.zeros(L)}
+          "emis_k": np.zeros(L), "aconc_k": np.zeros((L, max(nae, 1), K)), "areff_k": np.zeros((L, max(nae, 1), K))}
     ko = OrAllskyK()
     for k, v in kd.items():
         setattr(ko, k, _p(v, _dp))

@@ -12,6 +12,9 @@
 !   Find_Index, LPoly, Interp_1D/2D/3D
 !                            src/Interpolation/CRTM_Interpolation.f90:225-860
 !   CRTM_AtmOptics_Combine   src/AtmOptics/CRTM_AtmOptics.f90:369
+!   AerosolCoeff_Binary_ReadFile, AerosolCoeff_Frequency, AerosolCoeff_typeID_to_index
+!                            src/Coefficients/AerosolCoeff/AerosolCoeff_Binary_IO.f90:352,
+!                            src/Coefficients/AerosolCoeff/AerosolCoeff_Define.f90:635-696
 MODULE ref_harness
   USE ISO_C_BINDING
   USE Type_Kinds, ONLY: fp
@@ -28,9 +31,13 @@ MODULE ref_harness
   USE CRTM_AtmOptics_Define, ONLY: CRTM_AtmOptics_type, CRTM_AtmOptics_Create
   USE AOvar_Define, ONLY: AOvar_type, AOvar_Create
   USE CRTM_AtmOptics, ONLY: CRTM_AtmOptics_Combine
+  USE AerosolCoeff_Define, ONLY: AerosolCoeff_type, AerosolCoeff_Destroy, AerosolCoeff_Frequency, &
+                                 AerosolCoeff_typeID_to_index
+  USE AerosolCoeff_Binary_IO, ONLY: AerosolCoeff_Binary_ReadFile
   IMPLICIT NONE
   TYPE(ODPS_type), SAVE :: tc
   TYPE(CloudCoeff_type), SAVE :: ccf
+  TYPE(AerosolCoeff_type), SAVE :: acf
 CONTAINS
 
   FUNCTION c2f(cstr, n) RESULT(s)
@@ -243,5 +250,44 @@ CONTAINS
     pc = ao%Phase_Coefficient(0:nl, 1, 1:k)
     sod = ao%Scattering_Optical_Depth
   END SUBROUTINE ref_combine
+
+  ! AerosolCoeff_Binary_ReadFile with the 'CRTM' scheme, then AerosolCoeff_Frequency;
+  ! dims = n_Wavelengths, n_Radii, n_Sigma, n_Types, n_RH, n_RH_Radii, n_Legendre_Terms,
+  ! n_Phase_Elements, Release, Version
+  INTEGER(C_INT) FUNCTION ref_aerosolcoeff_load(path, n, dims) BIND(C, NAME='ref_aerosolcoeff_load')
+    CHARACTER(KIND=C_CHAR), INTENT(IN) :: path(*)
+    INTEGER(C_INT), VALUE :: n
+    INTEGER(C_INT), INTENT(OUT) :: dims(10)
+    CALL AerosolCoeff_Destroy(acf)
+    ref_aerosolcoeff_load = AerosolCoeff_Binary_ReadFile('CRTM', c2f(path, n), acf, Quiet=.TRUE.)
+    IF (ref_aerosolcoeff_load /= 0) RETURN
+    CALL AerosolCoeff_Frequency(acf)
+    dims = (/ acf%n_Wavelengths, acf%n_Radii, acf%n_Sigma, acf%n_Types, acf%n_RH, acf%n_RH_Radii, &
+              acf%n_Legendre_Terms, acf%n_Phase_Elements, acf%Release, acf%Version /)
+  END FUNCTION ref_aerosolcoeff_load
+
+  ! Copy the loaded AerosolCoeff arrays out (column-major, as stored)
+  SUBROUTINE ref_aerosolcoeff_fetch(tid, wl, fr, reff, rsig, rh, ke, w, g, kb, pc) &
+      BIND(C, NAME='ref_aerosolcoeff_fetch')
+    INTEGER(C_INT), INTENT(OUT) :: tid(*)
+    REAL(C_DOUBLE), INTENT(OUT) :: wl(*), fr(*), reff(*), rsig(*), rh(*), ke(*), w(*), g(*), kb(*), pc(*)
+    tid(1:SIZE(acf%Type)) = acf%Type
+    wl(1:SIZE(acf%Wavelength)) = acf%Wavelength
+    fr(1:SIZE(acf%Frequency)) = acf%Frequency
+    reff(1:SIZE(acf%Reff)) = RESHAPE(acf%Reff, (/SIZE(acf%Reff)/))
+    rsig(1:SIZE(acf%Rsig)) = RESHAPE(acf%Rsig, (/SIZE(acf%Rsig)/))
+    rh(1:SIZE(acf%RH)) = acf%RH
+    ke(1:SIZE(acf%ke)) = RESHAPE(acf%ke, (/SIZE(acf%ke)/))
+    w(1:SIZE(acf%w)) = RESHAPE(acf%w, (/SIZE(acf%w)/))
+    g(1:SIZE(acf%g)) = RESHAPE(acf%g, (/SIZE(acf%g)/))
+    kb(1:SIZE(acf%kb)) = RESHAPE(acf%kb, (/SIZE(acf%kb)/))
+    pc(1:SIZE(acf%pcoeff)) = RESHAPE(acf%pcoeff, (/SIZE(acf%pcoeff)/))
+  END SUBROUTINE ref_aerosolcoeff_fetch
+
+  ! AerosolCoeff_typeID_to_index of the loaded table (1-based; -1 when absent)
+  INTEGER(C_INT) FUNCTION ref_aerosol_type_index(id) BIND(C, NAME='ref_aerosol_type_index')
+    INTEGER(C_INT), VALUE :: id
+    ref_aerosol_type_index = AerosolCoeff_typeID_to_index(acf, INT(id))
+  END FUNCTION ref_aerosol_type_index
 
 END MODULE ref_harness

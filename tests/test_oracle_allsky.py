@@ -45,6 +45,9 @@ def ref(oracle_mod):
     R.ref_combine.argtypes = [ctypes.c_int, ctypes.c_int, _dp, _dp, _dp, _dp]
     R.ref_cloudcoeff_load.argtypes = [ctypes.c_char_p, ctypes.c_int, _ip]
     R.ref_cloudcoeff_fetch.argtypes = [_dp] * 16
+    R.ref_aerosolcoeff_load.argtypes = [ctypes.c_char_p, ctypes.c_int, _ip]
+    R.ref_aerosolcoeff_fetch.argtypes = [_ip] + [_dp] * 10
+    R.ref_aerosol_type_index.argtypes = [ctypes.c_int]
     return R
 
 
@@ -562,3 +565,147 @@ def test_hand_adjoint_matches_complex_step(coeff_dir, oracle_mod, cloudcoeff, se
         scale = max(np.abs(k1[key]).max(), 1e-300)
         assert np.abs(k1[key] - k2[key]).max() < 1e-11 * scale, key
     assert np.abs(k1["wc_k"]).max() > 0 and np.abs(k1["reff_k"]).max() > 0
+
+
+# ------------------------------------------------------------- aerosols
+@pytest.fixture(scope="module")
+def aerosolcoeff(tmp_path_factory):
+    d = str(tmp_path_factory.mktemp("ac"))
+    path = synthetic.write_aerosol_coeff(d)
+    return path, coeffio.read_aerosolcoeff(path)
+
+
+def test_aerosolcoeff_reader_matches_reference(aerosolcoeff, ref):
+    """The synthetic AerosolCoeff file read by the reference's
+    AerosolCoeff_Binary_ReadFile ('CRTM' scheme) and AerosolCoeff_Frequency
+    gives the arrays the package reader returns, the frequencies the lookup
+    interpolates in, and the reference's type-id -> table-index mapping."""
+    path, ac = aerosolcoeff
+    dims = np.zeros(10, np.int32)
+    b = path.encode()
+    assert ref.ref_aerosolcoeff_load(b, len(b), dims.ctypes.data_as(_ip)) == 0
+    nW, nRH, nR, nS, nT = ac.ke.shape
+    assert list(dims) == [nW, nR, nS, nT, nRH, ac.rh.size, ac.n_legendre_terms, ac.pcoeff.shape[6],
+                          ac.release, ac.version]
+    tid = np.zeros(nT, np.int32)
+    out = [np.zeros(a.size) for a in (ac.wavelength, ac.frequency, ac.reff, ac.rsig, ac.rh, ac.ke, ac.w, ac.g,
+                                       ac.kb, ac.pcoeff)]
+    ref.ref_aerosolcoeff_fetch(tid.ctypes.data_as(_ip), *[o.ctypes.data_as(_dp) for o in out])
+    np.testing.assert_array_equal(tid, ac.type_id)
+    for o, a in zip(out, (ac.wavelength, ac.frequency, ac.reff, ac.rsig, ac.rh, ac.ke, ac.w, ac.g, ac.kb,
+                          ac.pcoeff)):
+        np.testing.assert_allclose(o, np.asarray(a).ravel(order="F"), rtol=1e-15, atol=0)
+    assert np.all(np.diff(out[1]) > 0)     # the lookup needs ascending frequencies
+    for q, t in enumerate(ac.type_id):
+        assert ref.ref_aerosol_type_index(int(t)) == q + 1
+
+
+def test_aerosol_table_indexing(aerosolcoeff, oracle_mod):
+    """'CRTM' scheme lookup on tables linear in frequency and radius (the
+    piecewise-quadratic interpolation is exact for them): the type's radius
+    grid, the first RH and size-variance entries, and Legendre slot l + offset
+    (4 / 6 / 8 / 16 streams -> 0 / 5 / 12 / 21) are the ones read."""
+    _, ac0 = aerosolcoeff
+    nW, nR, nT, nL1 = ac0.wavelength.size, ac0.reff.shape[0], ac0.type_id.size, ac0.pcoeff.shape[5]
+    nRH, nS = 2, 2
+    fr = ac0.frequency
+    ke = np.full((nW, nRH, nR, nS, nT), -7.0)     # RH / sigma entries 2 hold poison
+    w = np.full_like(ke, -7.0)
+    pc = np.full((nW, nRH, nR, nS, nT, nL1, 1), -7.0)
+    for t in range(nT):
+        r = ac0.reff[:, t]
+        ke[:, 0, :, 0, t] = 100.0 * (t + 1) + 0.01 * fr[:, None] + 3.0 * r[None, :]
+        w[:, 0, :, 0, t] = 0.5 + 1e-5 * fr[:, None] + 1e-3 * r[None, :]
+        for l in range(nL1):
+            pc[:, 0, :, 0, t, l, 0] = 1000.0 * l + t + 1e-6 * fr[:, None]
+    ac = coeffio.AerosolCoeff(ac0.type_id, ac0.type_name, ac0.wavelength, ac0.reff, np.ones((nS, nT)),
+                              np.zeros(nRH), ke, w, ke, ke, pc)
+    cc = oracle_mod.CloudCoeff(coeffio.read_cloudcoeff(synthetic.write_cloud_coeff(os.path.dirname(
+        aerosolcoeff[0]))), ac)
+    for t_id in (synthetic.DUST_AEROSOL, synthetic.SULFATE_AEROSOL):
+        t = int(np.nonzero(ac.type_id == t_id)[0][0])
+        r = float(np.sqrt(ac0.reff[3, t] * ac0.reff[4, t]))
+        for nl, off in ((4, 0), (6, 5), (8, 12), (16, 21)):
+            k_, w_ = ctypes.c_double(), ctypes.c_double()
+            p_ = np.zeros(17)
+            assert oracle_mod.lib().oracle_aerosol_optics(cc.c.aerosol, 987.6, t_id, r, nl, ctypes.byref(k_),
+                                                          ctypes.byref(w_), P(p_)) == 0
+            assert k_.value == pytest.approx(100.0 * (t + 1) + 0.01 * 987.6 + 3.0 * r, rel=1e-12)
+            assert w_.value == pytest.approx(0.5 + 1e-5 * 987.6 + 1e-3 * r, rel=1e-12)
+            assert p_[0] == 0.5
+            np.testing.assert_allclose(p_[1:nl + 1], [1000.0 * (l + off) + t + 1e-6 * 987.6 for l in range(1, nl + 1)],
+                                       rtol=1e-12)
+    k_, w_ = ctypes.c_double(), ctypes.c_double()
+    p_ = np.zeros(17)
+    assert oracle_mod.lib().oracle_aerosol_optics(cc.c.aerosol, 987.6, 1, 1.0, 2, ctypes.byref(k_), ctypes.byref(w_),
+                                                  P(p_)) == -1      # two streams: no aerosol phase function
+    assert oracle_mod.lib().oracle_aerosol_optics(cc.c.aerosol, 987.6, 99, 1.0, 4, ctypes.byref(k_), ctypes.byref(w_),
+                                                  P(p_)) == -6      # type id not in the table
+
+
+@pytest.fixture(scope="module")
+def hirs_aer(coeff_dir, oracle_mod, cloudcoeff, aerosolcoeff):
+    from conftest import load_sensor
+    tau, spc = load_sensor(coeff_dir, "hirs_syn")
+    return oracle_mod.Sensor(tau, spc), oracle_mod.CloudCoeff(cloudcoeff[1], aerosolcoeff[1])
+
+
+@pytest.mark.parametrize("clouds", [False, True])
+@pytest.mark.parametrize("n_streams", [0, 8])
+def test_aerosol_adjoint_matches_complex_step(hirs_aer, oracle_mod, clouds, n_streams):
+    """Aerosol concentration / radius Jacobians (and every other block) of the
+    hand-written adjoint equal the complex-step Jacobians, with and without
+    clouds in the column."""
+    S, C = hirs_aer
+    ps = synthetic.make_profiles(1, 60, seed=201)
+    if clouds:
+        ps = synthetic.add_clouds(ps, seed=202, cover="random", median_gm3=0.005)
+    ps = synthetic.add_aerosols(ps, seed=203)
+    eps = np.full(S.n_channels, 0.97)
+    btk = np.random.default_rng(3).normal(size=S.n_channels)
+    f1, k1 = oracle_mod.k_matrix_allsky(S, C, ps, 0, eps, n_streams, 0, btk)
+    f2, k2 = oracle_mod.k_matrix_allsky(S, C, ps, 0, eps, n_streams, 0, btk, method="adjoint")
+    for key in k1:
+        scale = max(np.abs(k1[key]).max(), 1e-300)
+        assert np.abs(k1[key] - k2[key]).max() < 1e-11 * scale, key
+    assert np.abs(k1["aconc_k"]).max() > 0 and np.abs(k1["areff_k"]).max() > 0
+
+
+def test_aerosol_complex_step_matches_central_differences(hirs_aer, oracle_mod):
+    import dataclasses
+    S, C = hirs_aer
+    ps = synthetic.add_aerosols(synthetic.make_profiles(1, 60, seed=211), seed=212)
+    eps = np.full(S.n_channels, 0.97)
+    _, k = oracle_mod.k_matrix_allsky(S, C, ps, 0, eps, 4, 0)
+
+    def bt(p_):
+        return oracle_mod.run_allsky(S, C, p_, 0, eps, 4)["bt"]
+    for n, kk in ((0, 55), (1, 50), (2, 58)):
+        for field, key in (("aerosol_concentration", "aconc_k"), ("aerosol_effective_radius", "areff_k")):
+            v = getattr(ps, field)
+            h = 1e-5 * v[n, kk]
+            a, b = dataclasses.replace(ps), dataclasses.replace(ps)
+            va, vb = v.copy(), v.copy()
+            va[n, kk] += h
+            vb[n, kk] -= h
+            setattr(a, field, va)
+            setattr(b, field, vb)
+            fd = (bt(a) - bt(b)) / (2 * h)
+            assert np.abs(fd - k[key][:, n, kk]).max() < 1e-5 * np.abs(fd).max(), (field, n, kk)
+
+
+def test_aerosols_skip_microwave_and_need_four_streams(atms, hirs_aer, oracle_mod, aerosolcoeff, cloudcoeff):
+    """Aerosol scattering is infrared-only (CRTM_AerosolScatter.f90:200); an
+    infrared channel at two streams has no aerosol phase function and the
+    reference fails it (:208-225)."""
+    import dataclasses
+    S, _ = atms
+    C = oracle_mod.CloudCoeff(cloudcoeff[1], aerosolcoeff[1])
+    ps = synthetic.make_profiles(1, 60, seed=221)
+    pa = synthetic.add_aerosols(ps, seed=222)
+    eps = np.full(22, 0.6)
+    np.testing.assert_array_equal(oracle_mod.run_allsky(S, C, pa, 0, eps, 4)["bt"],
+                                  oracle_mod.run_allsky(S, C, ps, 0, eps, 4)["bt"])
+    Sh, Ch = hirs_aer
+    with pytest.raises(RuntimeError, match="-5"):
+        oracle_mod.run_allsky(Sh, Ch, synthetic.add_aerosols(ps, seed=223), 0, np.full(Sh.n_channels, 0.97), 2)
