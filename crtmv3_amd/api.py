@@ -47,7 +47,9 @@ class CAtmosphere(ctypes.Structure):
                 ("n_absorbers", ctypes.c_int), ("absorber_id", _vp), ("absorber_units", _vp),
                 ("level_pressure", _vp), ("pressure", _vp), ("temperature", _vp), ("absorber", _vp),
                 ("climatology", _vp), ("n_clouds", ctypes.c_int), ("cloud_type", _vp),
-                ("cloud_water_content", _vp), ("cloud_effective_radius", _vp), ("cloud_fraction", _vp)]
+                ("cloud_water_content", _vp), ("cloud_effective_radius", _vp), ("cloud_fraction", _vp),
+                ("n_aerosols", ctypes.c_int), ("aerosol_type", _vp), ("aerosol_concentration", _vp),
+                ("aerosol_effective_radius", _vp)]
 
 
 class CSurface(ctypes.Structure):
@@ -77,7 +79,8 @@ class CKOutput(ctypes.Structure):
     _fields_ = [(n, _vp) for n in ("brightness_temperature_k", "radiance_k", "surface_emissivity_k",
                                    "temperature_k", "absorber_k", "land_temperature_k",
                                    "water_temperature_k", "snow_temperature_k", "ice_temperature_k",
-                                   "cloud_water_content_k", "cloud_effective_radius_k", "cloud_fraction_k")]
+                                   "cloud_water_content_k", "cloud_effective_radius_k", "cloud_fraction_k",
+                                   "aerosol_concentration_k", "aerosol_effective_radius_k")]
 
 
 _lib = None
@@ -223,6 +226,12 @@ class Context:
             atm.cloud_effective_radius = _ptr(arr(prof.cloud_effective_radius, np.float64))
             cf = getattr(prof, "cloud_fraction", None)
             atm.cloud_fraction = _ptr(arr(cf, np.float64)) if cf is not None else None
+        atype = getattr(prof, "aerosol_type", None)
+        if atype is not None and int(atype.shape[1]) > 0:
+            atm.n_aerosols = int(atype.shape[1])
+            atm.aerosol_type = _ptr(arr(atype, np.int32))
+            atm.aerosol_concentration = _ptr(arr(prof.aerosol_concentration, np.float64))
+            atm.aerosol_effective_radius = _ptr(arr(prof.aerosol_effective_radius, np.float64))
         sfc = CSurface()
         for f in ("land_coverage", "water_coverage", "snow_coverage", "ice_coverage", "land_temperature",
                   "water_temperature", "snow_temperature", "ice_temperature"):
@@ -297,7 +306,7 @@ class Context:
         keep = []
         atm, sfc, geo, opt = self._structs(prof, emissivity, precision, memory, jacobian_f32,
                                            check_input, keep, n_streams, overlap)
-        nc = int(atm.n_clouds)
+        nc, nae = int(atm.n_clouds), int(atm.n_aerosols)
         M, L = prof.n_profiles, self.n_channels
         host_nl = getattr(prof, "n_layers_host", None)
         NT = int(np.sum(host_nl if host_nl is not None else np.asarray(prof.n_layers)))
@@ -315,6 +324,9 @@ class Context:
                   "absorber_k": self._alloc((na, NT, L), kdt, memory, like)}
             for f in ("land_temperature_k", "water_temperature_k", "snow_temperature_k", "ice_temperature_k"):
                 kd[f] = self._alloc((M, L), np.float64, memory, like)
+            if nae > 0:
+                kd["aerosol_concentration_k"] = self._alloc((nae, NT, L), kdt, memory, like)
+                kd["aerosol_effective_radius_k"] = self._alloc((nae, NT, L), kdt, memory, like)
             if nc > 0:
                 kd["cloud_water_content_k"] = self._alloc((nc, NT, L), kdt, memory, like)
                 kd["cloud_effective_radius_k"] = self._alloc((nc, NT, L), kdt, memory, like)

@@ -116,11 +116,25 @@ __global__ void __launch_bounds__(64) cloud_prep_kernel(DevBatch be, DevClouds c
       for (int n = 0; n < nc; ++n)
         for (int kk = 0; kk < K; ++kk) cw.wc[n * NTe + le + kk] /= tcc;
   }
+  // aerosols (Atmosphere%Aerosol): every typed slot's radius enters the
+  // stream choice (CRTM_Compute_nStreams); layers holding aerosol are
+  // scattering layers like cloudy ones
+  auto aerosol_layer = [&](int kk) -> bool {
+    const int ku = kk - nx;
+    if (ku < 0) return false;
+    for (int n = 0; n < cl.na; ++n)
+      if (cl.atype[(int64_t)m * cl.na + n] != 0 && cl.aconc[n * NTu + lu + ku] > AEROSOL_CONTENT_THRESHOLD)
+        return true;
+    return false;
+  };
+  for (int n = 0; n < cl.na; ++n)
+    if (cl.atype[(int64_t)m * cl.na + n] != 0)
+      for (int ku = 0; ku < K - nx; ++ku) max_reff = fmax(max_reff, cl.areff[n * NTu + lu + ku]);
   int count = 0;
   for (int kk = 0; kk < K; ++kk) {
     bool cloudy = false;
     for (int n = 0; n < nc; ++n) cloudy |= cw.wc[n * NTe + le + kk] > WATER_CONTENT_THRESHOLD;
-    count += cloudy;
+    count += cloudy || aerosol_layer(kk);
   }
   const int base = count ? atomicAdd(cw.total, count) : 0;
   if (count) atomicMax(cw.total + 1, K);
@@ -128,6 +142,7 @@ __global__ void __launch_bounds__(64) cloud_prep_kernel(DevBatch be, DevClouds c
   for (int kk = 0; kk < K; ++kk) {
     bool cloudy = false;
     for (int n = 0; n < nc; ++n) cloudy |= cw.wc[n * NTe + le + kk] > WATER_CONTENT_THRESHOLD;
+    cloudy = cloudy || aerosol_layer(kk);
     int slot = -1;
     if (cloudy && base + j < cw.cap) {
       slot = base + j;
@@ -140,6 +155,7 @@ __global__ void __launch_bounds__(64) cloud_prep_kernel(DevBatch be, DevClouds c
   cw.tcc[m] = fractional ? tcc : 0.0;
   cw.frac[m] = fractional;
   cw.max_reff[m] = max_reff;
+  cw.aer_err[m] = 0;
 }
 
 // =========================================================================
@@ -170,6 +186,21 @@ __global__ void __launch_bounds__(256) cloud_optics_kernel(DevSensor s, DevBatch
       if (wc > WATER_CONTENT_THRESHOLD)
         add_cloud(cc, mw, freq, cl.type[(int64_t)m * cl.nc + n], cw.reff[n * be.NT + ik], be.t[ik], wc, nl, loff,
                   od, ssa, pl);
+    }
+    // CRTM_Compute_AerosolScatter (src/AtmScatter/CRTM_AerosolScatter.f90:170-317): infrared only
+    const int ku = k - be.n_extra[m];
+    if (!mw && cl.na > 0 && ku >= 0) {
+      const int aoff = aerosol_offset(nl);
+      const int64_t iu = be.out_loff[m] + ku;
+#pragma unroll 1
+      for (int n = 0; n < cl.na; ++n) {
+        const int ty = cl.atype[(int64_t)m * cl.na + n];
+        const double conc = cl.aconc[n * be.out_NT + iu];
+        if (ty == 0 || !(conc > AEROSOL_CONTENT_THRESHOLD)) continue;
+        const int ka = aerosol_index(cc.aero, ty);
+        if (aoff < 0 || ka < 0) continue;   // failed profile (aerosol_check_kernel)
+        add_aerosol(cc.aero, s.wavenumber[l], ka, cl.areff[n * be.out_NT + iu], conc, nl, aoff, od, ssa, pl);
+      }
     }
     // CRTM_AtmOptics_Combine: P_l /= ssa, P_0 = 1/2, delta truncation f = P_nl
     if (ssa > BS_THRESHOLD) {
@@ -350,7 +381,7 @@ __global__ void __launch_bounds__(256) allsky_sweep_kernel(DevSensor s, DevBatch
   const int64_t n = (int64_t)be.M * s.L;
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
     const int m = (int)(t / s.L), l = (int)(t - (int64_t)m * s.L);
-    if (cw.cl_count[m] == 0 || cw.status[m] != 0) continue;
+    if (cw.cl_count[m] == 0 || cw.status[m] != 0 || cw.aer_err[m] != 0) continue;
     const double cosu = geo[(int64_t)m * 4 + 1];
     const int nl = legendre_terms(cl.n_streams, cw.max_reff[m], s.wavenumber[l]);
     if (nl > NL_AS - 1) continue;
@@ -531,9 +562,55 @@ __global__ void allsky_clear_copy_kernel(DevSensor s, int M, int Ltot, const int
   }
 }
 
+// Infrared channels whose stream count has no aerosol phase function (two
+// streams) fail the profile when the atmosphere carries aerosol slots, as
+// CRTM_Compute_AerosolScatter does (CRTM_AerosolScatter.f90:208-225); so do
+// aerosol types the AerosolCoeff tables lack (AerosolCoeff_typeID_to_index).
+__global__ void aerosol_check_kernel(DevSensor s, int M, DevClouds cl, CloudWork cw, DevAerosolCoeff ac) {
+  const int64_t n = (int64_t)M * s.L;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+    const int m = (int)(t / s.L), l = (int)(t - (int64_t)m * s.L);
+    const int nl = legendre_terms(cl.n_streams, cw.max_reff[m], s.wavenumber[l]);
+    bool bad = aerosol_offset(nl) < 0;
+    for (int q = 0; q < cl.na && !bad; ++q) {
+      const int ty = cl.atype[(int64_t)m * cl.na + q];
+      if (ty != 0 && aerosol_index(ac, ty) < 0) bad = true;
+    }
+    if (bad) cw.aer_err[m] = 1;
+  }
+}
+// failed profiles: NaN outputs and a FAILURE status
+__global__ void aerosol_fail_kernel(DevSensor s, int M, int Ltot, const int* aer_err, int* status, DevOut o) {
+  const int64_t n = (int64_t)M * s.L;
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+    const int m = (int)(t / s.L), l = (int)(t - (int64_t)m * s.L);
+    if (!aer_err[m]) continue;
+    const int64_t ol = (int64_t)m * Ltot + s.chan_offset + l;
+    o.radiance[ol] = __builtin_nan("");
+    o.bt[ol] = __builtin_nan("");
+    if (l == 0) status[m] = 5;
+  }
+}
+
 // ------------------------------------------------------------------ launchers
 void launch_cloud_prep(const DevBatch& be, const DevClouds& cl, const CloudWork& cw, hipStream_t st) {
   hipLaunchKernelGGL(cloud_prep_kernel, dim3((be.M + 63) / 64), dim3(64), 0, st, be, cl, cw);
+}
+
+void launch_aerosol_check(const DevSensor& s, int M, const DevClouds& cl, const CloudWork& cw,
+                          const DevCloudCoeff& cc, hipStream_t st) {
+  if (s.sensor_type == 1 || cl.na == 0) return;   // aerosols scatter in the infrared only
+  const int64_t n = (int64_t)M * s.L;
+  hipLaunchKernelGGL(aerosol_check_kernel, dim3((unsigned)std::min<int64_t>((n + 255) / 256, 8192)), dim3(256), 0,
+                     st, s, M, cl, cw, cc.aero);
+}
+
+void launch_aerosol_fail(const DevSensor& s, int M, int Ltot, const DevClouds& cl, const CloudWork& cw, int* status,
+                         const DevOut& o, hipStream_t st) {
+  if (s.sensor_type == 1 || cl.na == 0) return;
+  const int64_t n = (int64_t)M * s.L;
+  hipLaunchKernelGGL(aerosol_fail_kernel, dim3((unsigned)std::min<int64_t>((n + 255) / 256, 8192)), dim3(256), 0,
+                     st, s, M, Ltot, (const int*)cw.aer_err, status, o);
 }
 
 static int persistent_blocks(int64_t n) {

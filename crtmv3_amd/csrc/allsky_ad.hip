@@ -504,6 +504,33 @@ __device__ CloudLU cloud_lookup(const DevCloudCoeff& cc, bool mw, double freq, i
   if (c.w >= 1.0) c.w = 1.0;
   return c;
 }
+// One aerosol (Get_Aerosol_Opt, 'CRTM' scheme) with its radius derivatives,
+// in the CloudLU form (no temperature dependence).
+__device__ CloudLU aerosol_lookup(const DevAerosolCoeff& ac, double wavenumber, int k, double reff, int aoff) {
+  CloudLU c{};
+  c.u = weights(ac.freq, ac.nw, wavenumber);
+  const W4D rd = weights_d(ac.reff + (int64_t)k * ac.nr, ac.nr, reff);
+  c.v = rd.v;
+  c.dv = rd.d;
+  const int64_t sR = (int64_t)ac.nw * ac.nrh, sT = sR * ac.nr * ac.ns, sL = sT * ac.nt;
+  const int64_t o = c.u.i1 + sR * c.v.i1 + sT * k;
+  c.ke = tab2(ac.ke + o, sR, c.u, c.v);
+  c.ke_r = tab2(ac.ke + o, sR, c.u, c.dv);
+  c.w = tab2(ac.w + o, sR, c.u, c.v);
+  c.w_r = tab2(ac.w + o, sR, c.u, c.dv);
+  c.pct = ac.pc + o + sL * aoff;
+  c.sl = sL;
+  c.s2 = sR;
+  c.dims = 2;
+  c.ke_on = c.ke > 0.0;
+  c.w_on = c.ke_on && c.w > 0.0 && c.w < 1.0;
+  c.pc_on = c.ke_on && c.w > 0.0;
+  if (!c.ke_on) { c.ke = 0.0; c.w = 0.0; }
+  if (c.w <= 0.0) c.w = 0.0;
+  if (c.w >= 1.0) c.w = 1.0;
+  return c;
+}
+
 // phase coefficient l of a looked-up cloud and its r / t derivatives
 __device__ inline double pc_val(const CloudLU& c, int l) {
   const double* z = c.pct + c.sl * l;
@@ -684,7 +711,7 @@ __global__ void __launch_bounds__(256) allsky_sweep_ad_kernel(DevSensor s, DevBa
   RT* t_s = reinterpret_cast<RT*>(o.ks.t);
   for (int64_t t = lane0; t < n; t += (int64_t)gridDim.x * blockDim.x) {
     const int m = (int)(t / s.L), l = (int)(t - (int64_t)m * s.L);
-    if (cw.cl_count[m] == 0 || cw.status[m] != 0) continue;
+    if (cw.cl_count[m] == 0 || cw.status[m] != 0 || cw.aer_err[m] != 0) continue;
     const double cosu = geo[(int64_t)m * 4 + 1];
     const int nl = legendre_terms(cl.n_streams, cw.max_reff[m], s.wavenumber[l]);
     if (nl > NL_AS - 1) continue;
@@ -1117,7 +1144,7 @@ __global__ void __launch_bounds__(256) cloud_rt_ad_kernel(DevSensor s, CloudWork
   for (int item = blockIdx.x * blockDim.x + threadIdx.x; item < n_items; item += gridDim.x * blockDim.x) {
     const int c = item / s.L, l = item - c * s.L;
     const int m = cw.cl_desc[c].x;
-    if (cw.status[m] != 0) continue;
+    if (cw.status[m] != 0 || cw.aer_err[m] != 0) continue;
     const int nl = legendre_terms(cl.n_streams, cw.max_reff[m], s.wavenumber[l]);
     if (nl > NL_AS - 1) continue;
     const double cosu = geo[(int64_t)m * 4 + 1];
@@ -1267,7 +1294,7 @@ __global__ void __launch_bounds__(256) cloud_optics_ad_kernel(DevSensor s, DevBa
     const int c = item / s.L, l = item - c * s.L;
     const int2 d = cw.cl_desc[c];
     const int m = d.x, k = d.y;
-    if (cw.status[m] != 0) continue;
+    if (cw.status[m] != 0 || cw.aer_err[m] != 0) continue;
     const int nl = legendre_terms(cl.n_streams, cw.max_reff[m], s.wavenumber[l]);
     if (nl > NL_AS - 1) continue;
     const int64_t ik = be.loff[m] + k;
@@ -1294,6 +1321,32 @@ __global__ void __launch_bounds__(256) cloud_optics_ad_kernel(DevSensor s, DevBa
 #pragma unroll
           for (int qq = 1; qq < NL_AS; ++qq)
             if (qq == q) pl0[qq] += bs * pv;
+        }
+      }
+    }
+    const int ku = k - be.n_extra[m];
+    const bool aer = !mw && cl.na > 0 && ku >= 0 && aerosol_offset(nl) >= 0;
+    const int aoff = aer ? aerosol_offset(nl) : 0;
+    const int64_t iu = be.out_loff[m] + ku;
+    if (aer) {
+#pragma unroll 1
+      for (int n = 0; n < cl.na; ++n) {
+        const int ty = cl.atype[(int64_t)m * cl.na + n];
+        const double conc = cl.aconc[n * be.out_NT + iu];
+        const int ka = ty != 0 ? aerosol_index(cc.aero, ty) : -1;
+        if (ka < 0 || !(conc > AEROSOL_CONTENT_THRESHOLD)) continue;
+        const CloudLU lu = aerosol_lookup(cc.aero, s.wavenumber[l], ka, cl.areff[n * be.out_NT + iu], aoff);
+        od0 += lu.ke * conc;
+        const double bs = conc * lu.ke * lu.w;
+        ssa0 += bs;
+        if (bs != 0.0) {
+#pragma unroll 1
+          for (int q = 1; q <= nl; ++q) {
+            const double pv = pc_val(lu, q);
+#pragma unroll
+            for (int qq = 1; qq < NL_AS; ++qq)
+              if (qq == q) pl0[qq] += bs * pv;
+          }
         }
       }
     }
@@ -1372,6 +1425,42 @@ __global__ void __launch_bounds__(256) cloud_optics_ad_kernel(DevSensor s, DevBa
       rek[oi] = (KT)r_ad;
       t_sum += t_ad;
     }
+    // CRTM_Compute_AerosolScatter_AD (src/AtmScatter/CRTM_AerosolScatter.f90:614-751)
+    if (aer) {
+      KT* ack = reinterpret_cast<KT*>(ko.aconc_k);
+      KT* ark = reinterpret_cast<KT*>(ko.areff_k);
+#pragma unroll 1
+      for (int n = 0; n < cl.na; ++n) {
+        const int ty = cl.atype[(int64_t)m * cl.na + n];
+        const double conc = cl.aconc[n * be.out_NT + iu];
+        const int ka = ty != 0 ? aerosol_index(cc.aero, ty) : -1;
+        if (ka < 0 || !(conc > AEROSOL_CONTENT_THRESHOLD)) continue;
+        const CloudLU lu = aerosol_lookup(cc.aero, s.wavenumber[l], ka, cl.areff[n * be.out_NT + iu], aoff);
+        const double bs = conc * lu.ke * lu.w;
+        double s0 = 0.0, sr = 0.0;
+        if (lu.pc_on) {
+#pragma unroll 1
+          for (int q = 1; q <= nl; ++q) {
+            double pa = 0.0;
+#pragma unroll
+            for (int qq = 1; qq < NL_AS; ++qq)
+              if (qq == q) pa = plpre_ad[qq];
+            s0 += pa * pc_val(lu, q);
+            sr += pa * pc_dr(lu, q);
+          }
+        }
+        const double bs_ad = ssa_pre_ad + s0;
+        const double ke_ad = od_ad * conc + bs_ad * conc * lu.w;
+        const double c_ad = od_ad * lu.ke + bs_ad * lu.ke * lu.w;
+        const double w_ad = bs_ad * conc * lu.ke;
+        double r_ad = bs * sr;
+        if (lu.ke_on) r_ad += ke_ad * lu.ke_r;
+        if (lu.w_on) r_ad += w_ad * lu.w_r;
+        const int64_t oi = ((int64_t)n * be.out_NT + row) * be.Ltot + lg;
+        ack[oi] = (KT)c_ad;
+        ark[oi] = (KT)r_ad;
+      }
+    }
     reinterpret_cast<RT*>(o.ks.t)[ik * s.L + l] = (RT)t_sum;
   }
 }
@@ -1387,7 +1476,7 @@ __global__ void __launch_bounds__(256) cloud_cover_ad_kernel(DevSensor s, DevBat
   const int64_t n = (int64_t)be.M * s.L;
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
     const int m = (int)(t / s.L), l = (int)(t - (int64_t)m * s.L);
-    if (cw.cl_count[m] == 0 || cw.status[m] != 0 || !cw.frac[m]) continue;
+    if (cw.cl_count[m] == 0 || cw.status[m] != 0 || cw.aer_err[m] != 0 || !cw.frac[m]) continue;
     const int K = be.n_layers[m], nx = be.n_extra[m], nc = cl.nc;
     const int64_t le = be.loff[m], lu = be.out_loff[m];
     const int64_t NTe = be.NT, NTu = be.out_NT, Lt = be.Ltot;

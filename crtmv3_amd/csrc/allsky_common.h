@@ -9,6 +9,7 @@ namespace crtm {
 
 namespace {
 constexpr double WATER_CONTENT_THRESHOLD = 1.0e-6;  // src/CRTM_Parameters.f90:256
+constexpr double AEROSOL_CONTENT_THRESHOLD = 1.0e-9;  // src/CRTM_Parameters.f90:255
 constexpr double MIN_COVERAGE_THRESHOLD = 1.0e-12;  // src/CRTM_Parameters.f90:257
 constexpr double MIN_COVERAGE_FLAG = 1.0e-6;        // src/Atmosphere/CRTM_Atmosphere.f90:164
 constexpr double BS_THRESHOLD = 1.0e-10;            // src/CRTM_Parameters.f90:260
@@ -181,6 +182,43 @@ __device__ void add_cloud(const DevCloudCoeff& cc, bool mw, double freq, int typ
     const double* z = pct + sl * l;
     pl[l] += bs * (dims == 3 ? tab3(z, s2, s3, u, v, t) : tab2(z, s2, u, v));
   }
+}
+
+// Legendre offset of the aerosol tables per stream set
+// (src/AtmScatter/CRTM_AerosolScatter.f90:208-225); -1: no aerosol phase
+// function at this stream count (two streams)
+__device__ inline int aerosol_offset(int nl) {
+  return nl == 4 ? 0 : nl == 6 ? 5 : nl == 8 ? 12 : nl == 16 ? 21 : -1;
+}
+// AerosolCoeff_typeID_to_index (0-based; -1 when the table lacks the type)
+__device__ inline int aerosol_index(const DevAerosolCoeff& ac, int type) {
+  int k = -1;
+  for (int q = 0; q < ac.nt; ++q)
+    if (ac.type_id[q] == type) k = q;
+  return k;
+}
+
+// Get_Aerosol_Opt, 'CRTM' scheme (CRTM_AerosolScatter.f90:772-840): 2-D
+// lookup over frequency and the type's radius grid at the first RH and
+// size-variance entries, with the CRTM_Compute_AerosolScatter clean-up and
+// accumulation into od, bs and pl[1..nl].
+__device__ void add_aerosol(const DevAerosolCoeff& ac, double wavenumber, int k, double reff, double conc, int nl,
+                            int aoff, double& od, double& bs_sum, double* pl) {
+  const W4 u = weights(ac.freq, ac.nw, wavenumber);
+  const W4 v = weights(ac.reff + (int64_t)k * ac.nr, ac.nr, reff);
+  const int64_t sR = (int64_t)ac.nw * ac.nrh, sT = sR * ac.nr * ac.ns, sL = sT * ac.nt;
+  const int64_t o = u.i1 + sR * v.i1 + sT * k;
+  double ke = tab2(ac.ke + o, sR, u, v);
+  double w = tab2(ac.w + o, sR, u, v);
+  if (ke <= 0.0) { ke = 0.0; w = 0.0; }
+  if (w <= 0.0) w = 0.0;
+  if (w >= 1.0) w = 1.0;
+  od += ke * conc;
+  const double bs = conc * ke * w;
+  bs_sum += bs;
+  if (bs == 0.0) return;
+#pragma unroll 1
+  for (int l = 1; l <= nl; ++l) pl[l] += bs * tab2(ac.pc + o + sL * (l + aoff), sR, u, v);
 }
 
 // ----------------------------------------------------------- small matrices

@@ -37,6 +37,10 @@ void launch_allsky_k(const DevSensor&, const DevBatch&, const DevClouds&, const 
                      const LayerScratch<RT>&, const LayerScratch<RT>&, const RT*, const double*, int, const int*, int,
                      const DevOut&, const AllskyOut&, const CloudKOut&, RT*, int64_t, int, hipStream_t);
 int allsky_sweep_ad_blocks(int64_t n_lanes);
+void launch_aerosol_check(const DevSensor&, int M, const DevClouds&, const CloudWork&, const DevCloudCoeff&,
+                          hipStream_t);
+void launch_aerosol_fail(const DevSensor&, int M, int Ltot, const DevClouds&, const CloudWork&, int* status,
+                         const DevOut&, hipStream_t);
 }  // namespace crtm
 
 using namespace crtm;
@@ -589,6 +593,65 @@ CloudCoeffH read_cloudcoeff(const std::string& path) {
   return c;
 }
 
+// AerosolCoeff, binary, release 4, 'CRTM' scheme: record order of
+// AerosolCoeff_Binary_ReadFile (src/Coefficients/AerosolCoeff/AerosolCoeff_Binary_IO.f90:396-520);
+// frequencies from AerosolCoeff_Frequency (10^4 / wavelength).
+struct AerosolCoeffH {
+  int nw = 0, nr = 0, ns = 0, nt = 0, nrh = 0, nl = 0, np = 0;
+  std::vector<int> type_id;
+  std::vector<double> freq, reff, ke, w, pc;
+};
+
+AerosolCoeffH read_aerosolcoeff(const std::string& path) {
+  Records rec(path);
+  AerosolCoeffH a;
+  Cursor r0(rec.next());
+  const int rel = r0.get<int32_t>();
+  const int ver = r0.get<int32_t>();
+  r0.done("AerosolCoeff release");
+  if (rel != 4) throw CrtmError(fmt("AerosolCoeff Release check failed for %s (release %d)", path.c_str(), rel));
+  Cursor r1(rec.next());
+  a.nw = r1.get<int32_t>(); a.nr = r1.get<int32_t>(); a.ns = r1.get<int32_t>(); a.nt = r1.get<int32_t>();
+  a.nrh = r1.get<int32_t>(); a.nl = r1.get<int32_t>(); a.np = r1.get<int32_t>();
+  r1.done("AerosolCoeff dimensions");
+  if (a.nw < 4 || a.nr < 4 || a.ns < 1 || a.nt < 1 || a.nrh < 1 || a.np < 1)
+    throw CrtmError("AerosolCoeff tables need at least 4 points per interpolation axis");
+  if (a.nl < 37) throw CrtmError(fmt("AerosolCoeff has %d Legendre terms; the stream sets need 37", a.nl));
+  Cursor r2(rec.next());
+  const int nrhr = r2.get<int32_t>();   // n_RH_Radii ('CRTM' scheme)
+  r2.done("AerosolCoeff n_RH_Radii");
+  Cursor r3(rec.next());
+  const int sl = r3.get<int32_t>();
+  r3.done("AerosolCoeff data source length");
+  rec.next();                           // data source string
+  Cursor r5(rec.next());
+  a.type_id = r5.arr<int32_t>(a.nt);
+  r5.done("AerosolCoeff type ids");
+  Cursor r6(rec.next());
+  if (r6.get<int32_t>() != sl) throw CrtmError("AerosolCoeff type name string length mismatch");
+  rec.next();                           // type names
+  Cursor r8(rec.next());
+  const std::vector<double> wl = r8.arr<double>(a.nw);
+  a.reff = r8.arr<double>((size_t)a.nr * a.nt);
+  r8.arr<double>((size_t)a.ns * a.nt);
+  r8.arr<double>(nrhr);
+  r8.done("AerosolCoeff axes");
+  a.freq.resize(a.nw);
+  for (int i = 0; i < a.nw; ++i) a.freq[i] = 10000.0 / wl[i];
+  for (int i = 1; i < a.nw; ++i)
+    if (!(a.freq[i] > a.freq[i - 1])) throw CrtmError("AerosolCoeff wavelengths must be descending");
+  const size_t n5 = (size_t)a.nw * a.nrh * a.nr * a.ns * a.nt;
+  Cursor r9(rec.next());
+  a.ke = r9.arr<double>(n5);
+  a.w = r9.arr<double>(n5);
+  r9.arr<double>(n5);                  // g
+  if (ver > 300) r9.arr<double>(n5);   // kb
+  a.pc = r9.arr<double>(n5 * (a.nl + 1) * a.np);
+  r9.done("AerosolCoeff tables");
+  a.pc.resize(n5 * (a.nl + 1));        // phase element 1
+  return a;
+}
+
 // Gauss-Legendre nodes and weights on [0,1], ascending, for 1..9 points
 // (Double_Gauss_Quadrature, src/CRTM_Utility/CRTM_Utility.f90:343-378);
 // layout [n][2][10].
@@ -638,9 +701,11 @@ struct crtm_context {
   DevBuf xnl, xloff, xnx, xlp, xp, xt, xa;   // profiles extended by AddLayers
   // clouds: CloudCoeff tables (loaded at init when present) and per-call work
   bool cloud_loaded = false;
+  bool aerosol_loaded = false;
+  std::vector<int> aerosol_types;   // type ids of the AerosolCoeff tables
   std::vector<std::unique_ptr<DevBuf>> cloud_bufs;
   DevCloudCoeff cc{};
-  DevBuf cwc, creff, ccount, cslot, cdesc, ctotal, ctcc, cfrac, cmaxr, ctau, cscr;
+  DevBuf cwc, creff, ccount, cslot, cdesc, ctotal, ctcc, cfrac, cmaxr, ctau, cscr, caerr;
   // all-sky K-matrix: layer adjoints, per-thread adding tape, seeds for the clear-sky K kernel
   DevBuf cscr_ad, ctape, ks_tau, ks_b, ks_t, ks_ch;
   // kernel timing (crtm_profile_enable): HIP events on the context stream
@@ -782,6 +847,7 @@ int prepare_clouds(crtm_context* c, const DevBatch& be, const DevClouds& cl, Clo
   c->ctcc.reserve(M * sizeof(double));
   c->cfrac.reserve(M * sizeof(int));
   c->cmaxr.reserve(M * sizeof(double));
+  c->caerr.reserve(M * sizeof(int));
   cw.wc = c->cwc.as<double>();
   cw.reff = c->creff.as<double>();
   cw.cl_count = c->ccount.as<int>();
@@ -791,6 +857,7 @@ int prepare_clouds(crtm_context* c, const DevBatch& be, const DevClouds& cl, Clo
   cw.tcc = c->ctcc.as<double>();
   cw.frac = c->cfrac.as<int>();
   cw.max_reff = c->cmaxr.as<double>();
+  cw.aer_err = c->caerr.as<int>();
   cw.cap = (int)std::min<int64_t>(be.out_NT, INT32_MAX);
   HIPCHECK(hipMemsetAsync(cw.total, 0, 2 * sizeof(int), c->stream));
   launch_cloud_prep(be, cl, cw, c->stream);
@@ -835,8 +902,10 @@ void run_allsky(crtm_context* c, Sensor& S, const DevBatch& be, const DevClouds&
     if (c->profiling) c->mark();
   };
   timed([&] {
+    launch_aerosol_check(S.d, be.M, cl, cw, c->cc, c->stream);
     launch_allsky<RT>(S.d, be, cl, cw, c->cc, sc, reinterpret_cast<const RT*>(o.tau_g), w.geo, (int)items,
                       nz.data(), (int)nz.size(), o, ao, c->stream);
+    launch_aerosol_fail(S.d, be.M, be.Ltot, cl, cw, w.status, o, c->stream);
   });
   HIPCHECK(hipGetLastError());
 }
@@ -945,9 +1014,11 @@ void run_sensor_k_allsky(crtm_context* c, Sensor& S, const DevBatch& be, const D
     c->mark();
     c->ev_kind.push_back(2);
   }
+  launch_aerosol_check(S.d, be.M, cc.cl, cw, c->cc, c->stream);
   launch_allsky_k<RT, KT>(S.d, be, cc.cl, cw, c->cc, sc, sa, reinterpret_cast<const RT*>(o1.tau_g), w.geo,
                           (int)items, nz.data(), (int)nz.size(), o2, cc.ao, cc.ko, c->ctape.as<RT>(), tape_lanes, te,
                           c->stream);
+  launch_aerosol_fail(S.d, be.M, be.Ltot, cc.cl, cw, w.status, o2, c->stream);
   launch_clear_copy(S.d, be.M, be.Ltot, cc.cw.cl_count, o1, cc.ao, c->stream);   // profiles without cloudy layers
   if (c->profiling) c->mark();
   HIPCHECK(hipGetLastError());
@@ -983,11 +1054,17 @@ void dispatch(crtm_context* c, const DevBatch& be, const DevOut& o, const CloudC
 }
 
 // cloud Jacobians are written only where a cloud is present
-void zero_cloud_k(crtm_context* ctx, const CloudKOut& ko, int nc, int64_t NT, int L, size_t szK) {
+void zero_cloud_k(crtm_context* ctx, const CloudKOut& ko, int nc, int na, int64_t NT, int L, size_t szK) {
   const size_t n = (size_t)NT * L * szK;
-  HIPCHECK(hipMemsetAsync(ko.wc_k, 0, nc * n, ctx->stream));
-  HIPCHECK(hipMemsetAsync(ko.reff_k, 0, nc * n, ctx->stream));
-  HIPCHECK(hipMemsetAsync(ko.cf_k, 0, n, ctx->stream));
+  if (nc > 0) {
+    HIPCHECK(hipMemsetAsync(ko.wc_k, 0, nc * n, ctx->stream));
+    HIPCHECK(hipMemsetAsync(ko.reff_k, 0, nc * n, ctx->stream));
+    HIPCHECK(hipMemsetAsync(ko.cf_k, 0, n, ctx->stream));
+  }
+  if (na > 0) {
+    HIPCHECK(hipMemsetAsync(ko.aconc_k, 0, na * n, ctx->stream));
+    HIPCHECK(hipMemsetAsync(ko.areff_k, 0, na * n, ctx->stream));
+  }
 }
 
 int do_call(crtm_context* ctx, const crtm_atmosphere* atm, const crtm_surface* sfc, const crtm_geometry* geo,
@@ -1002,20 +1079,31 @@ int do_call(crtm_context* ctx, const crtm_atmosphere* atm, const crtm_surface* s
     return fail(ctx, fmt("n_absorbers %d outside 1..%d", atm->n_absorbers, NA_MAX));
   if (!rts->radiance || !rts->brightness_temperature) return fail(ctx, "RTSolution radiance/BT arrays required");
   CloudCall cc;
-  cc.on = atm->n_clouds > 0;
+  cc.on = atm->n_clouds > 0 || atm->n_aerosols > 0;
   if (cc.on) {
-    if (!ctx->cloud_loaded)
+    if (atm->n_clouds > 0 && !ctx->cloud_loaded)
       return fail(ctx, "The CloudCoeff data must be loaded (CloudCoeff.bin in the crtm_init path) for cloudy profiles");
-    if (!atm->cloud_type || !atm->cloud_water_content || !atm->cloud_effective_radius)
+    if (atm->n_aerosols > 0 && !ctx->aerosol_loaded)
+      return fail(ctx, "The AerosolCoeff data must be loaded (AerosolCoeff.bin in the crtm_init path) for aerosol "
+                       "profiles");
+    if (atm->n_clouds > 0 && (!atm->cloud_type || !atm->cloud_water_content || !atm->cloud_effective_radius))
       return fail(ctx, "cloud type, water content and effective radius arrays required");
+    if (atm->n_aerosols > 0 &&
+        (!atm->aerosol_type || !atm->aerosol_concentration || !atm->aerosol_effective_radius))
+      return fail(ctx, "aerosol type, concentration and effective radius arrays required");
+    if (atm->n_clouds < 0 || atm->n_aerosols < 0) return fail(ctx, "negative cloud or aerosol count");
     if (opt->n_streams != 0 && opt->n_streams != 2 && opt->n_streams != 4 && opt->n_streams != 6 &&
         opt->n_streams != 8)
       return fail(ctx, fmt("n_streams %d: the scattering kernels take 2, 4, 6 or 8 streams (0 = automatic)",
                            opt->n_streams));
     if (opt->overlap_id < 0 || opt->overlap_id > 5) return fail(ctx, "Invalid overlap assumption");
-    if (k && (!k->cloud_water_content_k || !k->cloud_effective_radius_k || !k->cloud_fraction_k))
+    if (k && atm->n_clouds > 0 &&
+        (!k->cloud_water_content_k || !k->cloud_effective_radius_k || !k->cloud_fraction_k))
       return fail(ctx, "cloud Jacobian arrays (water content, effective radius, fraction) required");
+    if (k && atm->n_aerosols > 0 && (!k->aerosol_concentration_k || !k->aerosol_effective_radius_k))
+      return fail(ctx, "aerosol Jacobian arrays (concentration, effective radius) required");
     cc.cl.nc = atm->n_clouds;
+    cc.cl.na = atm->n_aerosols;
     cc.cl.n_streams = opt->n_streams;
     cc.cl.overlap = opt->overlap_id > 0 ? opt->overlap_id : 4;
   }
@@ -1097,6 +1185,15 @@ int do_call(crtm_context* ctx, const crtm_atmosphere* atm, const crtm_surface* s
       for (size_t i = 0; i < (size_t)M * nc; ++i)
         if (atm->cloud_type[i] < 0 || atm->cloud_type[i] > 6)
           return fail(ctx, fmt("profile #%d: invalid cloud type %d", (int)(i / nc) + 1, atm->cloud_type[i]));
+      const size_t na_ = atm->n_aerosols;
+      in(atm->aerosol_type, M * na_ * sizeof(int), (const void**)&cc.cl.atype);
+      in(atm->aerosol_concentration, na_ * NT * sizeof(double), (const void**)&cc.cl.aconc);
+      in(atm->aerosol_effective_radius, na_ * NT * sizeof(double), (const void**)&cc.cl.areff);
+      for (size_t i = 0; i < (size_t)M * na_; ++i) {
+        const int t = atm->aerosol_type[i];
+        if (t != 0 && std::find(ctx->aerosol_types.begin(), ctx->aerosol_types.end(), t) == ctx->aerosol_types.end())
+          return fail(ctx, fmt("Invalid aerosol type ID %d, not exist in specified aerosol coefficient file", t));
+      }
     }
     const double* cov[4] = {sfc->land_coverage, sfc->water_coverage, sfc->snow_coverage, sfc->ice_coverage};
     const double* st[4] = {sfc->land_temperature, sfc->water_temperature, sfc->snow_temperature, sfc->ice_temperature};
@@ -1132,10 +1229,16 @@ int do_call(crtm_context* ctx, const crtm_atmosphere* atm, const crtm_surface* s
       out(k->temperature_k, (size_t)NT * L * szK, (void**)&o.t_k);
       out(k->absorber_k, (size_t)na * NT * L * szK, (void**)&o.a_k);
       if (cc.on) {
-        const size_t nc = atm->n_clouds;
-        out(k->cloud_water_content_k, nc * NT * L * szK, &cc.ko.wc_k);
-        out(k->cloud_effective_radius_k, nc * NT * L * szK, &cc.ko.reff_k);
-        out(k->cloud_fraction_k, (size_t)NT * L * szK, &cc.ko.cf_k);
+        const size_t nc = atm->n_clouds, na_ = atm->n_aerosols;
+        if (nc > 0) {
+          out(k->cloud_water_content_k, nc * NT * L * szK, &cc.ko.wc_k);
+          out(k->cloud_effective_radius_k, nc * NT * L * szK, &cc.ko.reff_k);
+          out(k->cloud_fraction_k, (size_t)NT * L * szK, &cc.ko.cf_k);
+        }
+        if (na_ > 0) {
+          out(k->aerosol_concentration_k, na_ * NT * L * szK, &cc.ko.aconc_k);
+          out(k->aerosol_effective_radius_k, na_ * NT * L * szK, &cc.ko.areff_k);
+        }
       }
     }
     // rad_k is both input and output: give it one device slot
@@ -1158,7 +1261,7 @@ int do_call(crtm_context* ctx, const crtm_atmosphere* atm, const crtm_surface* s
       *outs[i].d = base + ooff[i];
     }
     if (dok && !k->radiance_k) o.rad_k = nullptr;
-    if (dok && cc.on) zero_cloud_k(ctx, cc.ko, atm->n_clouds, NT, L, szK);
+    if (dok && cc.on) zero_cloud_k(ctx, cc.ko, atm->n_clouds, atm->n_aerosols, NT, L, szK);
     const DevBatch be = add_layers(ctx, b, abs_id.data());
     if (cc.on) cc.total = prepare_clouds(ctx, be, cc.cl, cc.cw, &cc.max_k);
     dispatch(ctx, be, o, cc, split, kf32, dok);
@@ -1218,11 +1321,16 @@ int do_call(crtm_context* ctx, const crtm_atmosphere* atm, const crtm_surface* s
     cc.cl.wc = atm->cloud_water_content;
     cc.cl.reff = atm->cloud_effective_radius;
     cc.cl.cf = atm->cloud_fraction;
+    cc.cl.atype = atm->aerosol_type;
+    cc.cl.aconc = atm->aerosol_concentration;
+    cc.cl.areff = atm->aerosol_effective_radius;
     if (dok) {
       cc.ko.wc_k = k->cloud_water_content_k;
       cc.ko.reff_k = k->cloud_effective_radius_k;
       cc.ko.cf_k = k->cloud_fraction_k;
-      zero_cloud_k(ctx, cc.ko, atm->n_clouds, NT, L, szK);
+      cc.ko.aconc_k = k->aerosol_concentration_k;
+      cc.ko.areff_k = k->aerosol_effective_radius_k;
+      zero_cloud_k(ctx, cc.ko, atm->n_clouds, atm->n_aerosols, NT, L, szK);
     }
   }
   const DevBatch be = add_layers(ctx, b, abs_id.data());
@@ -1276,8 +1384,29 @@ int crtm_init(int n_sensors, const char* const* sensor_ids, const char* file_pat
       c.ke_l = put(h.ke_l); c.w_l = put(h.w_l); c.pc_l = put(h.pc_l);
       c.ke_s = put(h.ke_s); c.w_s = put(h.w_s); c.pc_s = put(h.pc_s);
       c.ke_i = put(h.ke_i); c.w_i = put(h.w_i); c.pc_i = put(h.pc_i);
-      c.gauss = put(gauss_table());
       ctx->cloud_loaded = true;
+    }
+    {
+      ctx->cloud_bufs.emplace_back(new DevBuf);
+      upload(*ctx->cloud_bufs.back(), gauss_table());
+      ctx->cc.gauss = ctx->cloud_bufs.back()->as<double>();
+    }
+    const std::string acpath = dir + "AerosolCoeff.bin";
+    if (std::ifstream(acpath).good()) {
+      const AerosolCoeffH h = read_aerosolcoeff(acpath);
+      auto put = [&](const std::vector<double>& v) {
+        ctx->cloud_bufs.emplace_back(new DevBuf);
+        upload(*ctx->cloud_bufs.back(), v);
+        return ctx->cloud_bufs.back()->as<double>();
+      };
+      DevAerosolCoeff& a = ctx->cc.aero;
+      a.nw = h.nw; a.nr = h.nr; a.ns = h.ns; a.nt = h.nt; a.nrh = h.nrh; a.nlf = h.nl;
+      ctx->cloud_bufs.emplace_back(new DevBuf);
+      upload(*ctx->cloud_bufs.back(), h.type_id);
+      a.type_id = ctx->cloud_bufs.back()->as<int>();
+      a.freq = put(h.freq); a.reff = put(h.reff); a.ke = put(h.ke); a.w = put(h.w); a.pc = put(h.pc);
+      ctx->aerosol_types = h.type_id;
+      ctx->aerosol_loaded = true;
     }
     HIPCHECK(hipDeviceSynchronize());
   } catch (const std::exception& e) {

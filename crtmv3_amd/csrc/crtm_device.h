@@ -242,6 +242,8 @@ struct CloudKOut {
   void* wc_k;
   void* reff_k;
   void* cf_k;
+  void* aconc_k;   // [na][out_NT][Ltot]
+  void* areff_k;
 };
 
 // ---------------------------------------------------------------------------
@@ -249,6 +251,18 @@ struct CloudKOut {
 // ---------------------------------------------------------------------------
 constexpr int NZ_AS = 6;           // largest angle count the all-sky kernels are built for (8 streams)
 constexpr int NL_AS = 9;           // Legendre terms 0..8 carried per layer
+
+// AerosolCoeff tables, 'CRTM' scheme (src/Coefficients/AerosolCoeff/
+// AerosolCoeff_Define.f90:100-128), column-major as in the file
+struct DevAerosolCoeff {
+  int nw, nr, ns, nt, nrh, nlf;   // wavelengths, radii, size variances, types, RH entries, Legendre terms
+  const int* type_id;   // [nt]
+  const double* freq;   // [nw] cm^-1 ascending
+  const double* reff;   // (nr, nt)
+  const double* ke;     // (nw, nrh, nr, ns, nt)
+  const double* w;
+  const double* pc;     // (nw, nrh, nr, ns, nt, 0:nlf, nP)
+};
 
 // CloudCoeff tables (src/Coefficients/CloudCoeff/CloudCoeff_Define.f90:95-154),
 // column-major as in the file, phase element 1 only
@@ -269,9 +283,10 @@ struct DevCloudCoeff {
   const double* w_i;
   const double* pc_i;   // (f, r, D+1, 1:nlf)
   const double* gauss;  // [10][2][10]: nodes and weights on [0,1] for n = 0..9 streams
+  DevAerosolCoeff aero; // aerosol tables (aero.nt = 0: not loaded)
 };
 
-// Cloud inputs of a batch (caller's layer layout)
+// Cloud and aerosol inputs of a batch (caller's layer layout)
 struct DevClouds {
   int nc;               // cloud slots per profile
   const int* type;      // [M][nc]
@@ -280,6 +295,10 @@ struct DevClouds {
   const double* cf;     // [out_NT] or null
   int n_streams;        // Options%n_Streams (0: chosen from the Mie parameter)
   int overlap;          // Options%Overlap_Id 1..5
+  int na;               // aerosol slots per profile
+  const int* atype;     // [M][na] aerosol type id (0: empty slot)
+  const double* aconc;  // [na][out_NT] kg/m^2
+  const double* areff;  // [na][out_NT] microns
 };
 
 // Per-call cloud work (extended layer layout)
@@ -293,6 +312,7 @@ struct CloudWork {
   double* tcc;          // [M] total cloud cover of fractional profiles (else 0)
   int* frac;            // [M] 1: fractional coverage
   double* max_reff;     // [M] largest effective radius (stream choice)
+  int* aer_err;         // [M] 1: an infrared channel's stream count has no aerosol phase function
   int cap;
   const int* status;    // [M] profile status of the preparation kernel (0 = good)
 };

@@ -83,6 +83,13 @@ typedef struct {
   const double* cloud_effective_radius;/* microns, [n_clouds][layer_offset[M]] */
   const double* cloud_fraction;        /* [layer_offset[M]]; clouds of layers with a fraction below
                                           1e-12 are ignored (src/CRTM_Forward_Module.f90:368-378) */
+  /* Aerosols (Atmosphere%Aerosol(n), src/Atmosphere/Aerosol/CRTM_Aerosol_Define.f90:119-133):
+   * n_aerosols slots per profile; infrared sensors only (CRTM_AerosolScatter.f90:200). */
+  int n_aerosols;
+  const int* aerosol_type;                 /* [M][n_aerosols] type id of the AerosolCoeff tables
+                                              (0 = empty slot) */
+  const double* aerosol_concentration;     /* kg/m^2, [n_aerosols][layer_offset[M]] */
+  const double* aerosol_effective_radius;  /* microns, [n_aerosols][layer_offset[M]] */
 } crtm_atmosphere;
 
 /* ---- Surface (src/Surface/CRTM_Surface_Define.f90:204-244): [M] each ---- */
@@ -161,11 +168,17 @@ typedef struct {
   void* cloud_water_content_k;
   void* cloud_effective_radius_k;
   void* cloud_fraction_k;
+  /* Atmosphere_K%Aerosol(n)%Concentration and %Effective_Radius [n_aerosols][NT][L]
+   * (CRTM_Compute_AerosolScatter_AD, src/AtmScatter/CRTM_AerosolScatter.f90:614-751);
+   * required when the atmosphere has aerosols; overwritten. */
+  void* aerosol_concentration_k;
+  void* aerosol_effective_radius_k;
 } crtm_k_output;
 
 /* CRTM_Init (src/CRTM_LifeCycle.f90:520): loads <id>.SpcCoeff.bin and
  * <id>.TauCoeff.bin (ODPS) for each sensor from file_path, and CloudCoeff.bin
- * (release 3, binary) when present there, uploads the coefficients to the
+ * (release 3, binary) and AerosolCoeff.bin (release 4, binary, 'CRTM' scheme)
+ * when present there, uploads the coefficients to the
  * current GPU and creates the context stream.  Cloudy calls need the
  * CloudCoeff data (src/CRTM_Forward_Module.f90:379-386). */
 int crtm_init(int n_sensors, const char* const* sensor_ids, const char* file_path,

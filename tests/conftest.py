@@ -29,6 +29,7 @@ def coeff_dir(tmp_path_factory):
     for name in SMALL_SENSORS:
         synthetic.write_sensor(d, name)
     synthetic.write_cloud_coeff(d)
+    synthetic.write_aerosol_coeff(d)
     return d
 
 
