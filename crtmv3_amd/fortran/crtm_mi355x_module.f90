@@ -50,6 +50,14 @@ MODULE CRTM_MI355X_Module
   INTEGER, PARAMETER, PUBLIC :: WATER_CLOUD = 1, ICE_CLOUD = 2, RAIN_CLOUD = 3, SNOW_CLOUD = 4, &
                                 GRAUPEL_CLOUD = 5, HAIL_CLOUD = 6
 
+  ! CRTM_Aerosol_type (src/Atmosphere/Aerosol/CRTM_Aerosol_Define.f90:119-133)
+  TYPE, PUBLIC :: CRTM_Aerosol_type
+    INTEGER :: n_Layers = 0
+    INTEGER :: Type = 0                            ! aerosol type id of the AerosolCoeff tables
+    REAL(fp), ALLOCATABLE :: Effective_Radius(:)   ! microns
+    REAL(fp), ALLOCATABLE :: Concentration(:)      ! kg/m^2
+  END TYPE
+
   TYPE, PUBLIC :: CRTM_Atmosphere_type
     INTEGER :: n_Layers = 0, n_Absorbers = 0, n_Clouds = 0, n_Aerosols = 0
     INTEGER :: Climatology = 6                     ! US_STANDARD_ATMOSPHERE
@@ -59,6 +67,7 @@ MODULE CRTM_MI355X_Module
     REAL(fp), ALLOCATABLE :: Absorber(:, :)        ! (n_Layers, n_Absorbers)
     REAL(fp), ALLOCATABLE :: Cloud_Fraction(:)     ! (n_Layers)
     TYPE(CRTM_Cloud_type), ALLOCATABLE :: Cloud(:) ! (n_Clouds)
+    TYPE(CRTM_Aerosol_type), ALLOCATABLE :: Aerosol(:) ! (n_Aerosols)
   END TYPE
 
   TYPE, PUBLIC :: CRTM_Surface_type
@@ -99,6 +108,8 @@ MODULE CRTM_MI355X_Module
     TYPE(C_PTR) :: climatology
     INTEGER(C_INT) :: n_clouds
     TYPE(C_PTR) :: cloud_type, cloud_water_content, cloud_effective_radius, cloud_fraction
+    INTEGER(C_INT) :: n_aerosols
+    TYPE(C_PTR) :: aerosol_type, aerosol_concentration, aerosol_effective_radius
   END TYPE
   TYPE, BIND(C) :: c_surface
     TYPE(C_PTR) :: land_coverage, water_coverage, snow_coverage, ice_coverage, &
@@ -121,7 +132,8 @@ MODULE CRTM_MI355X_Module
   TYPE, BIND(C) :: c_k_output
     TYPE(C_PTR) :: brightness_temperature_k, radiance_k, surface_emissivity_k, temperature_k, &
                    absorber_k, land_temperature_k, water_temperature_k, snow_temperature_k, ice_temperature_k, &
-                   cloud_water_content_k, cloud_effective_radius_k, cloud_fraction_k
+                   cloud_water_content_k, cloud_effective_radius_k, cloud_fraction_k, &
+                   aerosol_concentration_k, aerosol_effective_radius_k
   END TYPE
 
   INTERFACE
@@ -180,6 +192,8 @@ MODULE CRTM_MI355X_Module
     REAL(C_DOUBLE), ALLOCATABLE :: rad(:), bt(:), up(:), dn(:), sp(:), se(:), rcl(:), tbc(:), tcc(:)
     INTEGER(C_INT), ALLOCATABLE :: status(:), ctype(:)
     REAL(C_DOUBLE), ALLOCATABLE :: cwc(:), creff(:), cfrac(:)
+    INTEGER(C_INT), ALLOCATABLE :: atype(:)
+    REAL(C_DOUBLE), ALLOCATABLE :: aconc(:), areff(:)
     TYPE(c_atmosphere) :: atm
     TYPE(c_surface) :: sfc
     TYPE(c_geometry) :: geo
@@ -267,6 +281,14 @@ CONTAINS
         Atm(m)%Cloud(ic)%Effective_Radius = 0
         Atm(m)%Cloud(ic)%Water_Content = 0
       END DO
+      IF (ALLOCATED(Atm(m)%Aerosol)) DEALLOCATE(Atm(m)%Aerosol)
+      ALLOCATE(Atm(m)%Aerosol(n_Aerosols))
+      DO ic = 1, n_Aerosols
+        Atm(m)%Aerosol(ic)%n_Layers = n_Layers
+        ALLOCATE(Atm(m)%Aerosol(ic)%Effective_Radius(n_Layers), Atm(m)%Aerosol(ic)%Concentration(n_Layers))
+        Atm(m)%Aerosol(ic)%Effective_Radius = 0
+        Atm(m)%Aerosol(ic)%Concentration = 0
+      END DO
     END DO
   END SUBROUTINE CRTM_Atmosphere_Create
 
@@ -282,6 +304,12 @@ CONTAINS
           DO n = 1, SIZE(Atm(i, j)%Cloud)
             Atm(i, j)%Cloud(n)%Water_Content = 0
             Atm(i, j)%Cloud(n)%Effective_Radius = 0
+          END DO
+        END IF
+        IF (ALLOCATED(Atm(i, j)%Aerosol)) THEN
+          DO n = 1, SIZE(Atm(i, j)%Aerosol)
+            Atm(i, j)%Aerosol(n)%Concentration = 0
+            Atm(i, j)%Aerosol(n)%Effective_Radius = 0
           END DO
         END IF
       END DO
@@ -337,8 +365,8 @@ CONTAINS
     TYPE(batch_type), TARGET :: b
     TYPE(c_k_output) :: k
     REAL(C_DOUBLE), ALLOCATABLE, TARGET :: btk(:), radk(:), emk(:), tk(:), ak(:), stk(:, :)
-    REAL(C_DOUBLE), ALLOCATABLE, TARGET :: cwk(:), crk(:), cfk(:)
-    INTEGER :: L, M, mm, ic, j, kk, NT, na, K0, nc
+    REAL(C_DOUBLE), ALLOCATABLE, TARGET :: cwk(:), crk(:), cfk(:), ack(:), ark(:)
+    INTEGER :: L, M, mm, ic, j, kk, NT, na, K0, nc, nae
     Error_Status = pack_batch(Atmosphere, Surface, Geometry, ChannelInfo, RTSolution, Options, b, L, M)
     IF (Error_Status /= SUCCESS) RETURN
     NT = INT(b%off(M + 1))
@@ -364,6 +392,10 @@ CONTAINS
     k%cloud_water_content_k = C_LOC(cwk)
     k%cloud_effective_radius_k = C_LOC(crk)
     k%cloud_fraction_k = C_LOC(cfk)
+    nae = Atmosphere(1)%n_Aerosols
+    ALLOCATE(ack(MAX(nae, 1) * NT * L), ark(MAX(nae, 1) * NT * L))
+    k%aerosol_concentration_k = C_LOC(ack)
+    k%aerosol_effective_radius_k = C_LOC(ark)
     Error_Status = c_crtm_k_matrix(ChannelInfo(1)%ctx, b%atm, b%sfc, b%geo, b%opt, b%rts, k)
     IF (Error_Status /= SUCCESS) THEN
       CALL report(ChannelInfo(1)%ctx, 'CRTM_K_Matrix')
@@ -392,6 +424,15 @@ CONTAINS
             Atmosphere_K(ic, mm)%Cloud_Fraction(kk) = Atmosphere_K(ic, mm)%Cloud_Fraction(kk) + &
                 cfk((K0 + kk - 1) * L + ic)
           END IF
+          ! Atmosphere_K%Aerosol(j)%Concentration / %Effective_Radius
+          IF (nae > 0 .AND. ALLOCATED(Atmosphere_K(ic, mm)%Aerosol)) THEN
+            DO j = 1, nae
+              Atmosphere_K(ic, mm)%Aerosol(j)%Concentration(kk) = &
+                  Atmosphere_K(ic, mm)%Aerosol(j)%Concentration(kk) + ack(((j - 1) * NT + K0 + kk - 1) * L + ic)
+              Atmosphere_K(ic, mm)%Aerosol(j)%Effective_Radius(kk) = &
+                  Atmosphere_K(ic, mm)%Aerosol(j)%Effective_Radius(kk) + ark(((j - 1) * NT + K0 + kk - 1) * L + ic)
+            END DO
+          END IF
         END DO
         Surface_K(ic, mm)%Land_Temperature = Surface_K(ic, mm)%Land_Temperature + stk((mm - 1) * L + ic, 1)
         Surface_K(ic, mm)%Water_Temperature = Surface_K(ic, mm)%Water_Temperature + stk((mm - 1) * L + ic, 2)
@@ -411,7 +452,7 @@ CONTAINS
     TYPE(CRTM_Options_type), OPTIONAL, INTENT(IN) :: Opt(:)
     TYPE(batch_type), TARGET, INTENT(INOUT) :: b
     INTEGER, INTENT(OUT) :: L, M
-    INTEGER :: Error_Status, mm, NT, na, K0, K, j, nc
+    INTEGER :: Error_Status, mm, NT, na, K0, K, j, nc, nae
     Error_Status = FAILURE
     M = SIZE(Atm)
     L = SUM(ChannelInfo(:)%n_Channels)
@@ -434,6 +475,9 @@ CONTAINS
     nc = Atm(1)%n_Clouds
     ALLOCATE(b%ctype(MAX(nc, 1) * M), b%cwc(MAX(nc, 1) * NT), b%creff(MAX(nc, 1) * NT), b%cfrac(NT))
     b%ctype = 0; b%cwc = 0; b%creff = 0; b%cfrac = 0
+    nae = Atm(1)%n_Aerosols
+    ALLOCATE(b%atype(MAX(nae, 1) * M), b%aconc(MAX(nae, 1) * NT), b%areff(MAX(nae, 1) * NT))
+    b%atype = 0; b%aconc = 0; b%areff = 0
     b%ids = Atm(1)%Absorber_ID
     b%units = Atm(1)%Absorber_Units
     DO mm = 1, M
@@ -463,6 +507,12 @@ CONTAINS
           b%creff((j - 1) * NT + K0 + 1:(j - 1) * NT + K0 + K) = Atm(mm)%Cloud(j)%Effective_Radius(1:K)
         END DO
       END IF
+      IF (Atm(mm)%n_Aerosols /= nae) RETURN
+      DO j = 1, nae
+        b%atype((mm - 1) * nae + j) = Atm(mm)%Aerosol(j)%Type
+        b%aconc((j - 1) * NT + K0 + 1:(j - 1) * NT + K0 + K) = Atm(mm)%Aerosol(j)%Concentration(1:K)
+        b%areff((j - 1) * NT + K0 + 1:(j - 1) * NT + K0 + K) = Atm(mm)%Aerosol(j)%Effective_Radius(1:K)
+      END DO
     END DO
     b%atm%n_profiles = M
     b%atm%max_layers = MAXVAL(b%nl)
@@ -482,6 +532,10 @@ CONTAINS
     b%atm%cloud_water_content = C_LOC(b%cwc)
     b%atm%cloud_effective_radius = C_LOC(b%creff)
     b%atm%cloud_fraction = C_LOC(b%cfrac)
+    b%atm%n_aerosols = nae
+    b%atm%aerosol_type = C_LOC(b%atype)
+    b%atm%aerosol_concentration = C_LOC(b%aconc)
+    b%atm%aerosol_effective_radius = C_LOC(b%areff)
     b%sfc%land_coverage = C_LOC(b%cov(1, 1))
     b%sfc%water_coverage = C_LOC(b%cov(1, 2))
     b%sfc%snow_coverage = C_LOC(b%cov(1, 3))
