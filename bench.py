@@ -135,18 +135,19 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 1 GiB device-to-device copy (3 times), the reference for "
                          "scaling the FETCH_SIZE / WRITE_SIZE counters to bytes")
-    ap.add_argument("--workload", choices=("clear-ir", "allsky-mw"), default="clear-ir",
+    ap.add_argument("--workload", choices=("clear-ir", "allsky-mw", "allsky-ir"), default="clear-ir",
                     help="clear-ir: the headline (CrIS-like forward + K-matrix); allsky-mw: cloudy ATMS-like "
-                         "forward with the ADA scattering solver at 4 and 8 streams")
+                         "forward + K-matrix with the ADA scattering solver at 4 and 8 streams; allsky-ir: "
+                         "IASI-like forward + K-matrix, 60/40 clear/cloudy, 8 streams")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
         return
-    if args.workload == "allsky-mw":
+    if args.workload in ("allsky-mw", "allsky-ir"):
         if args.gpus != 1:
             print("bench.py measures one GPU. Use --gpus 1.", file=sys.stderr)
             sys.exit(2)
-        main_allsky(args)
+        (main_allsky if args.workload == "allsky-mw" else main_allsky_ir)(args)
         return
     if args.gpus != 1:
         print("bench.py measures one GPU: the MI355X path runs on a single device (profiles shard "
@@ -393,6 +394,127 @@ def main_allsky(args):
                                    f"x {K} layers, liquid/ice/rain/snow in 0-10 layers each, ADA 4 and 8 streams",
                        "n_profiles": M, "n_channels": L, "n_layers": K, "parallelism": "single GPU"},
             "by_streams": by_streams, "roofline": None, "cpu_baseline": cpu}
+    print(json.dumps(line), flush=True)
+    ctx.destroy()
+
+
+def main_allsky_ir(args):
+    """Infrared all-sky forward + K-matrix: 8461-channel IASI-like sensor,
+    1024 profiles x 100 layers, 40 % of them with liquid and ice clouds as in
+    the microwave workload (overcast), ADA at 8 streams; FP32-split vs FP64
+    GPU (Jacobians stored in FP32 in both), BT and Jacobian errors against the
+    FP64 oracle adjoint on a sample of cloudy profiles, oracle timed on the
+    host cores."""
+    import torch
+    from crtmv3_amd import api, coeffio, synthetic
+    from oracle import oracle as orc
+    dev = torch.device("cuda:0")
+    coeff_dir = synthetic.default_coeff_dir()
+    sensor = "iasi_syn"
+    synthetic.write_sensor(coeff_dir, sensor)
+    synthetic.write_cloud_coeff(coeff_dir)
+    ctx = api.crtm_init([sensor], coeff_dir)
+    L = ctx.n_channels
+    M = args.profiles if args.profiles != 4096 else 1024
+    K, ns = 100, 8
+    ps = synthetic.add_clouds(synthetic.make_profiles(M, K, seed=20261203 + 5), seed=20261203 + 6,
+                              types=(synthetic.WATER_CLOUD, synthetic.ICE_CLOUD), cloudy_fraction=0.4)
+    n_cloudy = int(sum(ps.cloud_water_content[:, int(ps.layer_offset[m]):int(ps.layer_offset[m + 1])].max() > 0
+                       for m in range(M)))
+    log(f"allsky-ir: {sensor} {L} channels, {M} profiles ({n_cloudy} cloudy)")
+    dps = api.device_profiles(ps, dev)
+    emis_h = np.tile(synthetic.default_emissivity(1, L), (M, 1))
+    emis = torch.as_tensor(emis_h, device=dev)
+    btk = torch.ones((M, L), dtype=torch.float64, device=dev)
+    res = {}
+    for prec, name, steps in ((api.PREC_FP32_SPLIT, "split", args.steps), (api.PREC_FP64, "fp64", args.fp64_steps)):
+        r = {}
+        for mode in ("forward", "k_matrix"):
+            if mode == "forward":
+                def run():
+                    return ctx.forward(dps, emis, precision=prec, n_streams=ns), None
+            else:
+                prev = ctx.k_matrix(dps, emis, bt_k=btk, precision=prec, jacobian_f32=True, n_streams=ns)
+
+                def run():
+                    return ctx.k_matrix(dps, emis, bt_k=btk, precision=prec, jacobian_f32=True, n_streams=ns,
+                                        out=prev)
+            out = run()
+            for _ in range(args.warmup):
+                out = run()
+            ctx.synchronize()
+            ctx.profile_enable(True)
+            t0 = time.perf_counter()
+            for _ in range(steps):
+                out = run()
+            ctx.synchronize()
+            torch.cuda.synchronize()
+            t = (time.perf_counter() - t0) / steps
+            ms, _ = ctx.profile_query_kinds()
+            ctx.profile_enable(False)
+            r[mode] = {"ms_per_step": t * 1e3, "profile_channels_per_s": M * L / t,
+                       "kernel_ms": {"prep": ms[0] / steps, "clear": ms[1] / steps, "scattering": ms[2] / steps}}
+            rts, kd = out
+            r[mode + "_bt"] = rts["brightness_temperature"].cpu().numpy()
+            if kd is not None:
+                sel = [m for m in range(M)
+                       if ps.cloud_water_content[:, int(ps.layer_offset[m]):int(ps.layer_offset[m + 1])].max() > 0][:2]
+                r["k_sel"] = sel
+                r["k"] = {key: [kd[key][..., int(ps.layer_offset[m]):int(ps.layer_offset[m + 1]), :].double()
+                                .cpu().numpy() for m in sel]
+                          for key in ("temperature_k", "cloud_water_content_k")}
+            log(f"  {name} {mode}: {t * 1e3:.2f} ms/step  kernels {r[mode]['kernel_ms']}")
+            del out
+        res[name] = r
+    # oracle on a sample of cloudy profiles: forward BT and the hand-written adjoint
+    tau = coeffio.read_odps(os.path.join(coeff_dir, f"{sensor}.TauCoeff.bin"))
+    spc = coeffio.read_spccoeff(os.path.join(coeff_dir, f"{sensor}.SpcCoeff.bin"))
+    S = orc.Sensor(tau, spc)
+    C = orc.CloudCoeff(coeffio.read_cloudcoeff(os.path.join(coeff_dir, synthetic.CLOUDCOEFF_FILE)))
+    sel = res["fp64"]["k_sel"]
+    worst = {"split": {"T": 0.0, "water_content": 0.0}, "fp64": {"T": 0.0, "water_content": 0.0}}
+    bt_err = {"split": 0.0, "fp64": 0.0}
+    for i, m in enumerate(sel):
+        f, k = orc.k_matrix_allsky(S, C, ps, m, emis_h[m], ns, 0, np.ones(L), None, method="adjoint")
+        for name in ("split", "fp64"):
+            bt_err[name] = max(bt_err[name], float(np.abs(res[name]["forward_bt"][m] - f["bt"]).max()))
+            got = res[name]["k"]["temperature_k"][i].T
+            worst[name]["T"] = max(worst[name]["T"], float(np.abs(got - k["t_k"]).max() / np.abs(k["t_k"]).max()))
+            got = res[name]["k"]["cloud_water_content_k"][i].transpose(2, 0, 1)
+            sc = np.abs(k["wc_k"]).max()
+            if sc > 0:
+                worst[name]["water_content"] = max(worst[name]["water_content"],
+                                                   float(np.abs(got - k["wc_k"]).max() / sc))
+    cpu = None
+    if not args.no_cpu_baseline:
+        threads = min(16, os.cpu_count() or 1)
+        n = 2
+        t0 = time.perf_counter()
+        orc.k_matrix_allsky_batch(S, C, _subset(ps, 0, n), emis_h[:n], n_streams=ns, n_threads=threads)
+        dt = time.perf_counter() - t0
+        n = min(M, max(n, int(n * args.cpu_seconds / max(dt, 1e-6))))
+        t0 = time.perf_counter()
+        orc.k_matrix_allsky_batch(S, C, _subset(ps, 0, n), emis_h[:n], n_streams=ns, n_threads=threads)
+        dt = time.perf_counter() - t0
+        cpu = {"value": n * L / dt, "unit": "profile-channels/s", "cores": threads, "kind": "port",
+               "sample": f"{n} of the {M} profiles x {L} channels (the first ones; 40 % cloudy overall), "
+                         f"all-sky forward + K-matrix at 8 streams, FP64 oracle hand-written adjoint, {dt:.1f} s"}
+    v = res["split"]["k_matrix"]["profile_channels_per_s"]
+    line = {"metric": "all-sky IR forward+K-matrix profile-channels/s, FP32-split vs FP64 GPU", "value": v,
+            "unit": "profile-channels/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+            "ms_per_step": res["split"]["k_matrix"]["ms_per_step"], "higher_is_better": True, "scaling": "weak",
+            "vs_baseline": None, "dtype": "fp32+fp64-split",
+            "data": "synthetic (seeded profiles; synthetic ODPS/SpcCoeff/CloudCoeff files)",
+            "config": {"workload": f"all-sky infrared forward + K-matrix, IASI-like {L}-ch ODPS group 1, {M} profiles "
+                                   f"x {K} layers, {n_cloudy} with liquid and ice clouds in 0-10 layers each, "
+                                   f"ADA 8 streams, FP32 Jacobians",
+                       "n_profiles": M, "n_channels": L, "n_layers": K, "parallelism": "single GPU"},
+            "split": {m_: res["split"][m_] for m_ in ("forward", "k_matrix")},
+            "fp64": {m_: res["fp64"][m_] for m_ in ("forward", "k_matrix")},
+            "speedup_split_over_fp64_k": res["fp64"]["k_matrix"]["ms_per_step"] / res["split"]["k_matrix"]["ms_per_step"],
+            "bt_err_split_vs_fp64_K": float(np.abs(res["split"]["forward_bt"] - res["fp64"]["forward_bt"]).max()),
+            "bt_err_vs_oracle_K": bt_err, "jacobian_rel_err_vs_oracle": worst,
+            "roofline": None, "cpu_baseline": cpu}
     print(json.dumps(line), flush=True)
     ctx.destroy()
 
