@@ -172,12 +172,12 @@ __global__ void __launch_bounds__(256) cloud_optics_kernel(DevSensor s, DevBatch
     const int m = d.x, k = d.y;
     if (cw.status[m] != 0) continue;
     const int nl = legendre_terms(cl.n_streams, cw.max_reff[m], s.wavenumber[l]);
-    if (nl > NL_AS - 1) continue;   // stream counts above the built tables are rejected by the host
+    if (nl > NL_W - 1) continue;   // stream counts above the built tables are rejected by the host
     const int64_t ik = be.loff[m] + k;
     // CRTM_Compute_CloudScatter: accumulate over the clouds of the layer
-    double od = (double)tau_g[ik * s.L + l], ssa = 0.0, pl[NL_AS];
+    double od = (double)tau_g[ik * s.L + l], ssa = 0.0, pl[NL_W];
 #pragma unroll
-    for (int q = 0; q < NL_AS; ++q) pl[q] = 0.0;
+    for (int q = 0; q < NL_W; ++q) pl[q] = 0.0;
     const int loff = legendre_offset(nl);
     const double freq = mw ? s.frequency[l] : s.wavenumber[l];
 #pragma unroll 1
@@ -207,11 +207,11 @@ __global__ void __launch_bounds__(256) cloud_optics_kernel(DevSensor s, DevBatch
       const double w = ssa / od;
       const double iss = 1.0 / ssa;
 #pragma unroll
-      for (int q = 1; q < NL_AS; ++q) pl[q] *= iss;
+      for (int q = 1; q < NL_W; ++q) pl[q] *= iss;
       pl[0] = 0.5;
       double f = 0.0;
 #pragma unroll
-      for (int q = 1; q < NL_AS; ++q)
+      for (int q = 1; q < NL_W; ++q)
         if (q == nl) f = pl[q];
       od = (1.0 - f * w) * od;
       ssa = (1.0 - f) * w / (1.0 - f * w);
@@ -221,7 +221,8 @@ __global__ void __launch_bounds__(256) cloud_optics_kernel(DevSensor s, DevBatch
     out[st * sc.oTau()] = (RT)od;
     out[st * sc.oSsa()] = (RT)ssa;
 #pragma unroll
-    for (int q = 0; q < NL_AS - 1; ++q) out[st * (sc.oP() + q)] = (RT)(q < nl ? pl[q] : 0.0);
+    for (int q = 0; q < NL_W - 1; ++q)
+      if (q < sc.npc) out[st * (sc.oP() + q)] = (RT)(q < nl ? pl[q] : 0.0);
   }
 }
 
@@ -613,6 +614,16 @@ void launch_aerosol_fail(const DevSensor& s, int M, int Ltot, const DevClouds& c
                      st, s, M, Ltot, (const int*)cw.aer_err, status, o);
 }
 
+// wavefront kernels for 9 and 10 angles (allsky_wide.hip)
+template <class RT>
+void launch_cloud_rt_wide(int nz, const DevSensor& s, const CloudWork& cw, const DevClouds& cl,
+                          const DevCloudCoeff& cc, const LayerScratch<RT>& sc, const double* geo, int n_items,
+                          hipStream_t st);
+template <class RT>
+void launch_sweep_wide(int nz, const DevSensor& s, const DevBatch& be, const DevClouds& cl, const CloudWork& cw,
+                       const DevCloudCoeff& cc, const LayerScratch<RT>& sc, const RT* tau_g, const double* geo,
+                       const DevOut& o, const AllskyOut& ao, hipStream_t st);
+
 static int persistent_blocks(int64_t n) {
   const int64_t want = (n + 255) / 256;
   return (int)(want < 8192 ? (want > 0 ? want : 1) : 8192);
@@ -644,6 +655,11 @@ void launch_allsky(const DevSensor& s, const DevBatch& be, const DevClouds& cl, 
       CRTM_NZ_CASE(5)
       CRTM_NZ_CASE(6)
 #undef CRTM_NZ_CASE
+      case 9:
+      case 10:
+        launch_cloud_rt_wide<RT>(nz_list[q], s, cw, cl, cc, sc, geo, n_items, st);
+        launch_sweep_wide<RT>(nz_list[q], s, be, cl, cw, cc, sc, tau_g, geo, o, ao, st);
+        break;
       default:
         break;
     }

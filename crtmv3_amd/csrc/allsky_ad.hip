@@ -1296,15 +1296,15 @@ __global__ void __launch_bounds__(256) cloud_optics_ad_kernel(DevSensor s, DevBa
     const int m = d.x, k = d.y;
     if (cw.status[m] != 0 || cw.aer_err[m] != 0) continue;
     const int nl = legendre_terms(cl.n_streams, cw.max_reff[m], s.wavenumber[l]);
-    if (nl > NL_AS - 1) continue;
+    if (nl > NL_W - 1) continue;
     const int64_t ik = be.loff[m] + k;
     const int loff = legendre_offset(nl);
     const double freq = mw ? s.frequency[l] : s.wavenumber[l];
     const double temp = be.t[ik];
     // forward accumulation (CRTM_Compute_CloudScatter)
-    double od0 = (double)tau_g[ik * s.L + l], ssa0 = 0.0, pl0[NL_AS];
+    double od0 = (double)tau_g[ik * s.L + l], ssa0 = 0.0, pl0[NL_W];
 #pragma unroll
-    for (int q = 0; q < NL_AS; ++q) pl0[q] = 0.0;
+    for (int q = 0; q < NL_W; ++q) pl0[q] = 0.0;
 #pragma unroll 1
     for (int n = 0; n < cl.nc; ++n) {
       const double wc = cw.wc[n * be.NT + ik];
@@ -1319,7 +1319,7 @@ __global__ void __launch_bounds__(256) cloud_optics_ad_kernel(DevSensor s, DevBa
         for (int q = 1; q <= nl; ++q) {
           const double pv = pc_val(lu, q);
 #pragma unroll
-          for (int qq = 1; qq < NL_AS; ++qq)
+          for (int qq = 1; qq < NL_W; ++qq)
             if (qq == q) pl0[qq] += bs * pv;
         }
       }
@@ -1344,7 +1344,7 @@ __global__ void __launch_bounds__(256) cloud_optics_ad_kernel(DevSensor s, DevBa
           for (int q = 1; q <= nl; ++q) {
             const double pv = pc_val(lu, q);
 #pragma unroll
-            for (int qq = 1; qq < NL_AS; ++qq)
+            for (int qq = 1; qq < NL_W; ++qq)
               if (qq == q) pl0[qq] += bs * pv;
           }
         }
@@ -1354,13 +1354,13 @@ __global__ void __launch_bounds__(256) cloud_optics_ad_kernel(DevSensor s, DevBa
     const int64_t st = sa.n_items;
     const double tau_ad = (double)sa.v[st * sa.oTau() + item];
     const double ssa_ad = (double)sa.v[st * sa.oSsa() + item];
-    double od_ad, ssa_pre_ad, plpre_ad[NL_AS];
+    double od_ad, ssa_pre_ad, plpre_ad[NL_W];
 #pragma unroll
-    for (int q = 0; q < NL_AS; ++q) plpre_ad[q] = 0.0;
+    for (int q = 0; q < NL_W; ++q) plpre_ad[q] = 0.0;
     if (ssa0 > BS_THRESHOLD) {
       double plnl = 0.0;
 #pragma unroll
-      for (int q = 1; q < NL_AS; ++q)
+      for (int q = 1; q < NL_W; ++q)
         if (q == nl) plnl = pl0[q];
       const double w = ssa0 / od0, f = plnl / ssa0, den = 1.0 - f * w;
       const double f_ad = -w * od0 * tau_ad + ssa_ad * w * (w - 1.0) / (den * den);
@@ -1368,7 +1368,7 @@ __global__ void __launch_bounds__(256) cloud_optics_ad_kernel(DevSensor s, DevBa
       od_ad = den * tau_ad;
       ssa_pre_ad = 0.0;
 #pragma unroll
-      for (int q = 1; q < NL_AS; ++q) {
+      for (int q = 1; q < NL_W; ++q) {
         double pa = q < nl ? (double)sa.v[st * (sa.oP() + q) + item] : 0.0;
         if (q == nl) pa = f_ad;
         if (q > nl) pa = 0.0;
@@ -1400,7 +1400,7 @@ __global__ void __launch_bounds__(256) cloud_optics_ad_kernel(DevSensor s, DevBa
         for (int q = 1; q <= nl; ++q) {
           double pa = 0.0;
 #pragma unroll
-          for (int qq = 1; qq < NL_AS; ++qq)
+          for (int qq = 1; qq < NL_W; ++qq)
             if (qq == q) pa = plpre_ad[qq];
           s0 += pa * pc_val(lu, q);
           sr += pa * pc_dr(lu, q);
@@ -1443,7 +1443,7 @@ __global__ void __launch_bounds__(256) cloud_optics_ad_kernel(DevSensor s, DevBa
           for (int q = 1; q <= nl; ++q) {
             double pa = 0.0;
 #pragma unroll
-            for (int qq = 1; qq < NL_AS; ++qq)
+            for (int qq = 1; qq < NL_W; ++qq)
               if (qq == q) pa = plpre_ad[qq];
             s0 += pa * pc_val(lu, q);
             sr += pa * pc_dr(lu, q);
@@ -1600,6 +1600,15 @@ template <class RT>
 void launch_cloud_rt(int nz, const DevSensor&, const CloudWork&, const DevClouds&, const DevCloudCoeff&,
                      const LayerScratch<RT>&, const double*, int, hipStream_t);
 
+// wavefront kernels for 9 and 10 angles (allsky_wide.hip, allsky_wide_ad.hip)
+template <class RT>
+void launch_cloud_rt_wide(int nz, const DevSensor&, const CloudWork&, const DevClouds&, const DevCloudCoeff&,
+                          const LayerScratch<RT>&, const double*, int, hipStream_t);
+template <class RT>
+void launch_allsky_ad_wide(int nz, const DevSensor&, const DevBatch&, const DevClouds&, const CloudWork&,
+                           const DevCloudCoeff&, const LayerScratch<RT>&, const LayerScratch<RT>&, const RT*,
+                           const double*, int, const DevOut&, const AllskyOut&, RT*, int, hipStream_t);
+
 static int blocks_for(int64_t n, int cap) {
   const int64_t want = (n + 255) / 256;
   return (int)(want < cap ? (want > 0 ? want : 1) : cap);
@@ -1611,13 +1620,18 @@ template <class RT, class KT>
 void launch_allsky_k(const DevSensor& s, const DevBatch& be, const DevClouds& cl, const CloudWork& cw,
                      const DevCloudCoeff& cc, const LayerScratch<RT>& sc, const LayerScratch<RT>& sa,
                      const RT* tau_g, const double* geo, int n_items, const int* nz_list, int n_nz, const DevOut& o,
-                     const AllskyOut& ao, const CloudKOut& ko, RT* tape, int64_t tape_lanes, int te,
-                     hipStream_t st) {
+                     const AllskyOut& ao, const CloudKOut& ko, RT* tape, int64_t tape_lanes, int te, RT* wtape,
+                     int kmax, hipStream_t st) {
   const int lb = blocks_for(n_items, 8192);
   const int sb = allsky_sweep_ad_blocks((int64_t)be.M * s.L);
   if ((int64_t)sb * 256 != tape_lanes) return;   // the host sizes the tape for this grid
   launch_cloud_optics<RT>(s, be, cl, cw, cc, sc, tau_g, n_items, st);
   for (int q = 0; q < n_nz; ++q) {
+    if (nz_list[q] > NZ_AS) {   // 16 streams: wavefront kernels
+      launch_cloud_rt_wide<RT>(nz_list[q], s, cw, cl, cc, sc, geo, n_items, st);
+      launch_allsky_ad_wide<RT>(nz_list[q], s, be, cl, cw, cc, sc, sa, tau_g, geo, n_items, o, ao, wtape, kmax, st);
+      continue;
+    }
     launch_cloud_rt<RT>(nz_list[q], s, cw, cl, cc, sc, geo, n_items, st);
     switch (nz_list[q]) {
 #define CRTM_NZ_CASE(N)                                                                                        \
@@ -1649,7 +1663,8 @@ void launch_allsky_k(const DevSensor& s, const DevBatch& be, const DevClouds& cl
   template void launch_allsky_k<RT, KT>(const DevSensor&, const DevBatch&, const DevClouds&, const CloudWork&, \
                                         const DevCloudCoeff&, const LayerScratch<RT>&, const LayerScratch<RT>&, \
                                         const RT*, const double*, int, const int*, int, const DevOut&,        \
-                                        const AllskyOut&, const CloudKOut&, RT*, int64_t, int, hipStream_t);
+                                        const AllskyOut&, const CloudKOut&, RT*, int64_t, int, RT*, int,        \
+                                        hipStream_t);
 CRTM_INST_K(float, float)
 CRTM_INST_K(float, double)
 CRTM_INST_K(double, float)

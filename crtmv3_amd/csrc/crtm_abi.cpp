@@ -35,8 +35,9 @@ void launch_clear_copy(const DevSensor&, int M, int Ltot, const int* cl_count, c
 template <class RT, class KT>
 void launch_allsky_k(const DevSensor&, const DevBatch&, const DevClouds&, const CloudWork&, const DevCloudCoeff&,
                      const LayerScratch<RT>&, const LayerScratch<RT>&, const RT*, const double*, int, const int*, int,
-                     const DevOut&, const AllskyOut&, const CloudKOut&, RT*, int64_t, int, hipStream_t);
+                     const DevOut&, const AllskyOut&, const CloudKOut&, RT*, int64_t, int, RT*, int, hipStream_t);
 int allsky_sweep_ad_blocks(int64_t n_lanes);
+int allsky_sweep_ad_wide_blocks(int64_t n_pc);
 void launch_aerosol_check(const DevSensor&, int M, const DevClouds&, const CloudWork&, const DevCloudCoeff&,
                           hipStream_t);
 void launch_aerosol_fail(const DevSensor&, int M, int Ltot, const DevClouds&, const CloudWork&, int* status,
@@ -705,9 +706,9 @@ struct crtm_context {
   std::vector<int> aerosol_types;   // type ids of the AerosolCoeff tables
   std::vector<std::unique_ptr<DevBuf>> cloud_bufs;
   DevCloudCoeff cc{};
-  DevBuf cwc, creff, ccount, cslot, cdesc, ctotal, ctcc, cfrac, cmaxr, ctau, cscr, caerr;
+  DevBuf cwc, creff, ccount, cslot, cdesc, ctotal, ctcc, cfrac, cmaxr, ctau, cscr, caerr, cwide;
   // all-sky K-matrix: layer adjoints, per-thread adding tape, seeds for the clear-sky K kernel
-  DevBuf cscr_ad, ctape, ks_tau, ks_b, ks_t, ks_ch;
+  DevBuf cscr_ad, cwide_ad, ctape, ctape_w, ks_tau, ks_b, ks_t, ks_ch;
   // kernel timing (crtm_profile_enable): HIP events on the context stream
   bool profiling = false;
   std::vector<hipEvent_t> ev;      // pairs (start, stop) per recorded launch
@@ -880,6 +881,27 @@ std::vector<int> angle_counts(int n_streams) {
   return {2, 3, 4, 5};   // Mie-parameter choice: 2, 4 or 6 streams
 }
 
+// Layer scratch for a set of angle counts: per-lane slots (SoA) for counts up
+// to NZ_AS, item-major wavefront slots for the 16-stream counts; the optics
+// part (tau, ssa, phase coefficients) is always SoA.
+template <class RT>
+LayerScratch<RT> layer_scratch(const std::vector<int>& nz, int64_t n_items, DevBuf& soa, DevBuf& wide) {
+  LayerScratch<RT> sc{};
+  for (int z : nz) {
+    if (z <= NZ_AS) sc.nzc = std::max(sc.nzc, z);
+    else sc.nzw = std::max(sc.nzw, z);
+  }
+  sc.npc = sc.nzw > 0 ? NL_W - 1 : NL_AS - 1;
+  sc.n_items = std::max<int64_t>(n_items, 1);
+  soa.reserve((size_t)sc.n_elem() * sc.n_items * sizeof(RT));
+  sc.v = soa.as<RT>();
+  if (sc.nzw > 0) {
+    wide.reserve((size_t)sc.w_stride() * sc.n_items * sizeof(RT));
+    sc.w = wide.as<RT>();
+  }
+  return sc;
+}
+
 template <class RT>
 void run_allsky(crtm_context* c, Sensor& S, const DevBatch& be, const DevClouds& cl, const CloudWork& cw0,
                 int total_cl, const ProfWork<RT>& w, const DevOut& o, const AllskyOut& ao) {
@@ -888,11 +910,7 @@ void run_allsky(crtm_context* c, Sensor& S, const DevBatch& be, const DevClouds&
   const int64_t items = (int64_t)total_cl * S.d.L;
   if (items > INT32_MAX) throw CrtmError(fmt("%lld cloudy layer-channels exceed one launch", (long long)items));
   const std::vector<int> nz = angle_counts(cl.n_streams);
-  LayerScratch<RT> sc{};
-  sc.nzc = *std::max_element(nz.begin(), nz.end());
-  sc.n_items = std::max<int64_t>(items, 1);
-  c->cscr.reserve((size_t)sc.n_elem() * sc.n_items * sizeof(RT));
-  sc.v = c->cscr.as<RT>();
+  const LayerScratch<RT> sc = layer_scratch<RT>(nz, items, c->cscr, c->cwide);
   auto timed = [&](auto&& fn) {
     if (c->profiling) {
       c->mark();
@@ -984,17 +1002,16 @@ void run_sensor_k_allsky(crtm_context* c, Sensor& S, const DevBatch& be, const D
   const int64_t items = (int64_t)cc.total * L;
   if (items > INT32_MAX) throw CrtmError(fmt("%lld cloudy layer-channels exceed one launch", (long long)items));
   const std::vector<int> nz = angle_counts(cc.cl.n_streams);
-  LayerScratch<RT> sc{}, sa{};
-  sc.nzc = sa.nzc = *std::max_element(nz.begin(), nz.end());
-  sc.n_items = sa.n_items = std::max<int64_t>(items, 1);
-  c->cscr.reserve((size_t)sc.n_elem() * sc.n_items * sizeof(RT));
-  c->cscr_ad.reserve((size_t)sa.n_elem() * sa.n_items * sizeof(RT));
-  sc.v = c->cscr.as<RT>();
-  sa.v = c->cscr_ad.as<RT>();
+  const LayerScratch<RT> sc = layer_scratch<RT>(nz, items, c->cscr, c->cwide);
+  const LayerScratch<RT> sa = layer_scratch<RT>(nz, items, c->cscr_ad, c->cwide_ad);
   const int64_t ML = (int64_t)be.M * L;
   const int64_t tape_lanes = (int64_t)allsky_sweep_ad_blocks(ML) * 256;
   const int te = sc.nzc * sc.nzc + sc.nzc;
   c->ctape.reserve(std::max<size_t>((size_t)tape_lanes * std::max(cc.max_k, 1) * te * sizeof(RT), 16));
+  // wavefront adding adjoint (16 streams): 8 x 64 values per layer and wavefront
+  const int kmax = std::max(cc.max_k, 1);
+  if (sc.nzw > 0)
+    c->ctape_w.reserve((size_t)allsky_sweep_ad_wide_blocks(ML) * kmax * 8 * 64 * sizeof(RT));
   c->ks_tau.reserve(std::max<size_t>((size_t)be.NT * L * sizeof(RT), 16));
   c->ks_b.reserve(std::max<size_t>((size_t)be.NT * L * sizeof(RT), 16));
   c->ks_t.reserve(std::max<size_t>((size_t)be.NT * L * sizeof(RT), 16));
@@ -1017,7 +1034,7 @@ void run_sensor_k_allsky(crtm_context* c, Sensor& S, const DevBatch& be, const D
   launch_aerosol_check(S.d, be.M, cc.cl, cw, c->cc, c->stream);
   launch_allsky_k<RT, KT>(S.d, be, cc.cl, cw, c->cc, sc, sa, reinterpret_cast<const RT*>(o1.tau_g), w.geo,
                           (int)items, nz.data(), (int)nz.size(), o2, cc.ao, cc.ko, c->ctape.as<RT>(), tape_lanes, te,
-                          c->stream);
+                          c->ctape_w.as<RT>(), kmax, c->stream);
   launch_aerosol_fail(S.d, be.M, be.Ltot, cc.cl, cw, w.status, o2, c->stream);
   launch_clear_copy(S.d, be.M, be.Ltot, cc.cw.cl_count, o1, cc.ao, c->stream);   // profiles without cloudy layers
   if (c->profiling) c->mark();
@@ -1093,8 +1110,8 @@ int do_call(crtm_context* ctx, const crtm_atmosphere* atm, const crtm_surface* s
       return fail(ctx, "aerosol type, concentration and effective radius arrays required");
     if (atm->n_clouds < 0 || atm->n_aerosols < 0) return fail(ctx, "negative cloud or aerosol count");
     if (opt->n_streams != 0 && opt->n_streams != 2 && opt->n_streams != 4 && opt->n_streams != 6 &&
-        opt->n_streams != 8)
-      return fail(ctx, fmt("n_streams %d: the scattering kernels take 2, 4, 6 or 8 streams (0 = automatic)",
+        opt->n_streams != 8 && opt->n_streams != 16)
+      return fail(ctx, fmt("n_streams %d: the scattering kernels take 2, 4, 6, 8 or 16 streams (0 = automatic)",
                            opt->n_streams));
     if (opt->overlap_id < 0 || opt->overlap_id > 5) return fail(ctx, "Invalid overlap assumption");
     if (k && atm->n_clouds > 0 &&

@@ -249,8 +249,10 @@ struct CloudKOut {
 // ---------------------------------------------------------------------------
 // All-sky (clouds, ADA)
 // ---------------------------------------------------------------------------
-constexpr int NZ_AS = 6;           // largest angle count the all-sky kernels are built for (8 streams)
-constexpr int NL_AS = 9;           // Legendre terms 0..8 carried per layer
+constexpr int NZ_AS = 6;           // largest angle count of the per-lane kernels (8 streams)
+constexpr int NL_AS = 9;           // Legendre terms 0..8 of the per-lane kernels
+constexpr int NZ_W = 10;           // largest angle count of the wavefront kernels (16 streams)
+constexpr int NL_W = 17;           // Legendre terms 0..16 of the optics and wavefront kernels
 
 // AerosolCoeff tables, 'CRTM' scheme (src/Coefficients/AerosolCoeff/
 // AerosolCoeff_Define.f90:100-128), column-major as in the file
@@ -322,16 +324,22 @@ struct CloudWork {
 template <class RT> struct LayerScratch {
   RT* v;
   int64_t n_items;
-  int nzc;              // matrix size the slots are laid out for
+  int nzc;              // matrix size the per-lane slots are laid out for (0: none)
+  int npc;              // phase coefficients carried: NL_AS - 1 or NL_W - 1
   // element offsets: R [nzc*nzc], T [nzc*nzc], e = 1 - Thermal_C [nzc],
-  // delta-scaled tau and ssa, phase coefficients P_0..P_{NL_AS-2}
+  // delta-scaled tau and ssa, phase coefficients P_0..P_{npc-1}
   __host__ __device__ int oR() const { return 0; }
   __host__ __device__ int oT() const { return nzc * nzc; }
   __host__ __device__ int oE() const { return 2 * nzc * nzc; }
   __host__ __device__ int oTau() const { return 2 * nzc * nzc + nzc; }
   __host__ __device__ int oSsa() const { return oTau() + 1; }
   __host__ __device__ int oP() const { return oTau() + 2; }
-  __host__ __device__ int n_elem() const { return oP() + NL_AS - 1; }
+  __host__ __device__ int n_elem() const { return oP() + npc; }
+  // wavefront kernels: per item R [nzw*nzw], T [nzw*nzw] row-major and
+  // e [nzw], item-major (one item's block contiguous)
+  RT* w;
+  int nzw;              // angle count of the wavefront slots (0: none)
+  __host__ __device__ int64_t w_stride() const { return 2 * (int64_t)nzw * nzw + nzw; }
 };
 
 struct AllskyOut {

@@ -120,7 +120,7 @@ typedef struct {
   int memory;                    /* CRTM_MEM_* */
   int jacobian_f32;              /* K outputs are float (1) or double (0) */
   int check_input;               /* Options%Check_Input */
-  int n_streams;                 /* Options%n_Streams with Use_N_Streams (2, 4, 6, 8);
+  int n_streams;                 /* Options%n_Streams with Use_N_Streams (2, 4, 6, 8, 16);
                                     0 = chosen per channel from the Mie parameter
                                     (CRTM_Compute_nStreams, src/RTSolution/CRTM_RTSolution.f90:1204) */
   int overlap_id;                /* Options%Overlap_Id 1..5 (maximum, random, maximum-random,

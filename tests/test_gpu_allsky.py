@@ -142,8 +142,8 @@ def test_infrared_clouds(coeff_dir, oracle_mod):
 def test_cloudy_call_errors(ctx_mw, tmp_path):
     ps = synthetic.add_clouds(synthetic.make_profiles(2, 50, seed=81), seed=82)
     eps = _emis(2, 22)
-    with pytest.raises(api.CRTMError, match="n_streams 16"):
-        ctx_mw.forward(ps, eps, n_streams=16)
+    with pytest.raises(api.CRTMError, match="n_streams 12"):
+        ctx_mw.forward(ps, eps, n_streams=12)
     with pytest.raises(api.CRTMError, match="overlap"):
         ctx_mw.forward(ps, eps, n_streams=4, overlap=9)
     bad = synthetic.add_clouds(synthetic.make_profiles(2, 50, seed=81), seed=82)

@@ -540,6 +540,8 @@ def test_allsky_complex_step_matches_central_differences(atms, oracle_mod):
     ("atms_syn", "overcast", 8, 0, False),
     ("atms_syn", "random", 0, 1, True),
     ("hirs_syn", "random", 6, 2, False),
+    ("hirs_syn", "random", 16, 4, False),
+    ("atms_syn", "random", 16, 3, True),
 ])
 def test_hand_adjoint_matches_complex_step(coeff_dir, oracle_mod, cloudcoeff, sensor, cover, n_streams, overlap,
                                            thick):
