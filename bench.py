@@ -135,6 +135,9 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 1 GiB device-to-device copy (3 times), the reference for "
                          "scaling the FETCH_SIZE / WRITE_SIZE counters to bytes")
+    ap.add_argument("--streams", type=int, default=8, help="allsky-ir: ADA stream count (8 or 16)")
+    ap.add_argument("--wavefront-angles", type=int, default=0,
+                    help="all-sky: angle counts >= this run on the wavefront (matrix-core) kernels (5, 6, 9)")
     ap.add_argument("--workload", choices=("clear-ir", "allsky-mw", "allsky-ir"), default="clear-ir",
                     help="clear-ir: the headline (CrIS-like forward + K-matrix); allsky-mw: cloudy ATMS-like "
                          "forward + K-matrix with the ADA scattering solver at 4 and 8 streams; allsky-ir: "
@@ -276,6 +279,8 @@ def main_allsky(args):
     synthetic.write_sensor(coeff_dir, sensor)
     synthetic.write_cloud_coeff(coeff_dir)
     ctx = api.crtm_init([sensor], coeff_dir)
+    if args.wavefront_angles:
+        ctx.set_wavefront_angles(args.wavefront_angles)
     L = ctx.n_channels
     M = args.profiles if args.profiles != 4096 else 8192
     K = 100
@@ -414,9 +419,11 @@ def main_allsky_ir(args):
     synthetic.write_sensor(coeff_dir, sensor)
     synthetic.write_cloud_coeff(coeff_dir)
     ctx = api.crtm_init([sensor], coeff_dir)
+    if args.wavefront_angles:
+        ctx.set_wavefront_angles(args.wavefront_angles)
     L = ctx.n_channels
     M = args.profiles if args.profiles != 4096 else 1024
-    K, ns = 100, 8
+    K, ns = 100, args.streams
     ps = synthetic.add_clouds(synthetic.make_profiles(M, K, seed=20261203 + 5), seed=20261203 + 6,
                               types=(synthetic.WATER_CLOUD, synthetic.ICE_CLOUD), cloudy_fraction=0.4)
     n_cloudy = int(sum(ps.cloud_water_content[:, int(ps.layer_offset[m]):int(ps.layer_offset[m + 1])].max() > 0
@@ -498,7 +505,7 @@ def main_allsky_ir(args):
         dt = time.perf_counter() - t0
         cpu = {"value": n * L / dt, "unit": "profile-channels/s", "cores": threads, "kind": "port",
                "sample": f"{n} of the {M} profiles x {L} channels (the first ones; 40 % cloudy overall), "
-                         f"all-sky forward + K-matrix at 8 streams, FP64 oracle hand-written adjoint, {dt:.1f} s"}
+                         f"all-sky forward + K-matrix at {ns} streams, FP64 oracle hand-written adjoint, {dt:.1f} s"}
     v = res["split"]["k_matrix"]["profile_channels_per_s"]
     line = {"metric": "all-sky IR forward+K-matrix profile-channels/s, FP32-split vs FP64 GPU", "value": v,
             "unit": "profile-channels/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
@@ -507,8 +514,9 @@ def main_allsky_ir(args):
             "data": "synthetic (seeded profiles; synthetic ODPS/SpcCoeff/CloudCoeff files)",
             "config": {"workload": f"all-sky infrared forward + K-matrix, IASI-like {L}-ch ODPS group 1, {M} profiles "
                                    f"x {K} layers, {n_cloudy} with liquid and ice clouds in 0-10 layers each, "
-                                   f"ADA 8 streams, FP32 Jacobians",
-                       "n_profiles": M, "n_channels": L, "n_layers": K, "parallelism": "single GPU"},
+                                   f"ADA {ns} streams, FP32 Jacobians",
+                       "n_profiles": M, "n_channels": L, "n_layers": K, "n_streams": ns,
+                       "wavefront_angles": args.wavefront_angles or 9, "parallelism": "single GPU"},
             "split": {m_: res["split"][m_] for m_ in ("forward", "k_matrix")},
             "fp64": {m_: res["fp64"][m_] for m_ in ("forward", "k_matrix")},
             "speedup_split_over_fp64_k": res["fp64"]["k_matrix"]["ms_per_step"] / res["split"]["k_matrix"]["ms_per_step"],

@@ -106,6 +106,7 @@ def lib():
         L.crtm_k_matrix.argtypes = L.crtm_forward.argtypes + [ctypes.POINTER(CKOutput)]
         L.crtm_synchronize.argtypes = [_vp]
         L.crtm_set_stream.argtypes = [_vp, _vp, ctypes.c_int]
+        L.crtm_set_wavefront_angles.argtypes = [_vp, ctypes.c_int]
         L.crtm_last_error.argtypes = [_vp]
         L.crtm_last_error.restype = ctypes.c_char_p
         L.crtm_profile_enable.argtypes = [_vp, ctypes.c_int]
@@ -178,6 +179,12 @@ class Context:
             if lib().crtm_set_stream(self._h, h, 0) != SUCCESS:
                 self._err("crtm_set_stream")
             self._stream = h
+
+    def set_wavefront_angles(self, min_angles: int):
+        """Angle counts >= min_angles (5, 6 or 9) run on the wavefront
+        (matrix-core) scattering kernels, smaller ones on the per-lane kernels."""
+        if lib().crtm_set_wavefront_angles(self._h, int(min_angles)) != SUCCESS:
+            self._err("crtm_set_wavefront_angles")
 
     def _err(self, what):
         msg = lib().crtm_last_error(self._h).decode()

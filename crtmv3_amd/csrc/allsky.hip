@@ -640,6 +640,11 @@ void launch_allsky(const DevSensor& s, const DevBatch& be, const DevClouds& cl, 
     hipLaunchKernelGGL((cloud_optics_kernel<RT>), dim3(lb), dim3(256), 0, st, s, be, cl, cw, cc, sc, tau_g,
                        n_items);
   for (int q = 0; q < n_nz; ++q) {
+    if (nz_list[q] > sc.nzc) {   // wavefront kernels
+      launch_cloud_rt_wide<RT>(nz_list[q], s, cw, cl, cc, sc, geo, n_items, st);
+      launch_sweep_wide<RT>(nz_list[q], s, be, cl, cw, cc, sc, tau_g, geo, o, ao, st);
+      continue;
+    }
     switch (nz_list[q]) {
 #define CRTM_NZ_CASE(N)                                                                                     \
   case N:                                                                                                   \
@@ -655,11 +660,6 @@ void launch_allsky(const DevSensor& s, const DevBatch& be, const DevClouds& cl, 
       CRTM_NZ_CASE(5)
       CRTM_NZ_CASE(6)
 #undef CRTM_NZ_CASE
-      case 9:
-      case 10:
-        launch_cloud_rt_wide<RT>(nz_list[q], s, cw, cl, cc, sc, geo, n_items, st);
-        launch_sweep_wide<RT>(nz_list[q], s, be, cl, cw, cc, sc, tau_g, geo, o, ao, st);
-        break;
       default:
         break;
     }

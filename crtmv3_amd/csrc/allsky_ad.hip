@@ -1627,7 +1627,7 @@ void launch_allsky_k(const DevSensor& s, const DevBatch& be, const DevClouds& cl
   if ((int64_t)sb * 256 != tape_lanes) return;   // the host sizes the tape for this grid
   launch_cloud_optics<RT>(s, be, cl, cw, cc, sc, tau_g, n_items, st);
   for (int q = 0; q < n_nz; ++q) {
-    if (nz_list[q] > NZ_AS) {   // 16 streams: wavefront kernels
+    if (nz_list[q] > sc.nzc) {   // wavefront kernels
       launch_cloud_rt_wide<RT>(nz_list[q], s, cw, cl, cc, sc, geo, n_items, st);
       launch_allsky_ad_wide<RT>(nz_list[q], s, be, cl, cw, cc, sc, sa, tau_g, geo, n_items, o, ao, wtape, kmax, st);
       continue;

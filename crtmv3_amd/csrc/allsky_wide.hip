@@ -187,17 +187,25 @@ __global__ void __launch_bounds__(64) allsky_sweep_wide_kernel(DevSensor s, DevB
 // ------------------------------------------------------------------ launchers
 static int wide_blocks(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>(n, 16384)); }
 
+// angle counts of the wavefront kernels: 8 streams (5, 6) and 16 streams (9, 10)
+#define CRTM_WIDE_NZ(X) X(5) X(6) X(9) X(10)
+
 template <class RT>
 void launch_cloud_rt_wide(int nz, const DevSensor& s, const CloudWork& cw, const DevClouds& cl,
                           const DevCloudCoeff& cc, const LayerScratch<RT>& sc, const double* geo, int n_items,
                           hipStream_t st) {
   if (n_items <= 0) return;
-  if (nz == 9)
-    hipLaunchKernelGGL((cloud_rt_wide_kernel<RT, 9>), dim3(wide_blocks(n_items)), dim3(64), 0, st, s, cw, cl, cc, sc,
-                       geo, n_items);
-  else if (nz == 10)
-    hipLaunchKernelGGL((cloud_rt_wide_kernel<RT, 10>), dim3(wide_blocks(n_items)), dim3(64), 0, st, s, cw, cl, cc,
-                       sc, geo, n_items);
+  switch (nz) {
+#define CRTM_CASE(N)                                                                                             \
+  case N:                                                                                                        \
+    hipLaunchKernelGGL((cloud_rt_wide_kernel<RT, N>), dim3(wide_blocks(n_items)), dim3(64), 0, st, s, cw, cl, cc, \
+                       sc, geo, n_items);                                                                        \
+    break;
+    CRTM_WIDE_NZ(CRTM_CASE)
+#undef CRTM_CASE
+    default:
+      break;
+  }
 }
 
 template <class RT>
@@ -205,12 +213,17 @@ void launch_sweep_wide(int nz, const DevSensor& s, const DevBatch& be, const Dev
                        const DevCloudCoeff& cc, const LayerScratch<RT>& sc, const RT* tau_g, const double* geo,
                        const DevOut& o, const AllskyOut& ao, hipStream_t st) {
   const int64_t n = (int64_t)be.M * s.L;
-  if (nz == 9)
-    hipLaunchKernelGGL((allsky_sweep_wide_kernel<RT, 9>), dim3(wide_blocks(n)), dim3(64), 0, st, s, be, cl, cw, cc, sc,
-                       tau_g, geo, o, ao);
-  else if (nz == 10)
-    hipLaunchKernelGGL((allsky_sweep_wide_kernel<RT, 10>), dim3(wide_blocks(n)), dim3(64), 0, st, s, be, cl, cw, cc,
-                       sc, tau_g, geo, o, ao);
+  switch (nz) {
+#define CRTM_CASE(N)                                                                                              \
+  case N:                                                                                                         \
+    hipLaunchKernelGGL((allsky_sweep_wide_kernel<RT, N>), dim3(wide_blocks(n)), dim3(64), 0, st, s, be, cl, cw, cc, \
+                       sc, tau_g, geo, o, ao);                                                                    \
+    break;
+    CRTM_WIDE_NZ(CRTM_CASE)
+#undef CRTM_CASE
+    default:
+      break;
+  }
 }
 
 template void launch_cloud_rt_wide<float>(int, const DevSensor&, const CloudWork&, const DevClouds&,

@@ -738,18 +738,22 @@ void launch_allsky_ad_wide(int nz, const DevSensor& s, const DevBatch& be, const
                            RT* tape, int kmax, hipStream_t st) {
   const int sb = wide_ad_blocks((int64_t)be.M * s.L);
   const int lb = wide_ad_blocks(n_items);
-  if (nz == 9) {
-    hipLaunchKernelGGL((allsky_sweep_ad_wide_kernel<RT, 9>), dim3(sb), dim3(64), 0, st, s, be, cl, cw, cc, sc, sa,
-                       tau_g, geo, o, ao, tape, kmax);
-    if (n_items > 0)
-      hipLaunchKernelGGL((cloud_rt_ad_wide_kernel<RT, 9>), dim3(lb), dim3(64), 0, st, s, cw, cl, cc, sc, sa, geo,
-                         n_items);
-  } else if (nz == 10) {
-    hipLaunchKernelGGL((allsky_sweep_ad_wide_kernel<RT, 10>), dim3(sb), dim3(64), 0, st, s, be, cl, cw, cc, sc, sa,
-                       tau_g, geo, o, ao, tape, kmax);
-    if (n_items > 0)
-      hipLaunchKernelGGL((cloud_rt_ad_wide_kernel<RT, 10>), dim3(lb), dim3(64), 0, st, s, cw, cl, cc, sc, sa, geo,
-                         n_items);
+  switch (nz) {
+#define CRTM_CASE(N)                                                                                               \
+  case N:                                                                                                          \
+    hipLaunchKernelGGL((allsky_sweep_ad_wide_kernel<RT, N>), dim3(sb), dim3(64), 0, st, s, be, cl, cw, cc, sc, sa, \
+                       tau_g, geo, o, ao, tape, kmax);                                                             \
+    if (n_items > 0)                                                                                               \
+      hipLaunchKernelGGL((cloud_rt_ad_wide_kernel<RT, N>), dim3(lb), dim3(64), 0, st, s, cw, cl, cc, sc, sa, geo,  \
+                         n_items);                                                                                 \
+    break;
+    CRTM_CASE(5)
+    CRTM_CASE(6)
+    CRTM_CASE(9)
+    CRTM_CASE(10)
+#undef CRTM_CASE
+    default:
+      break;
   }
 }
 

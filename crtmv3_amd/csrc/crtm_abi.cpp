@@ -709,6 +709,9 @@ struct crtm_context {
   DevBuf cwc, creff, ccount, cslot, cdesc, ctotal, ctcc, cfrac, cmaxr, ctau, cscr, caerr, cwide;
   // all-sky K-matrix: layer adjoints, per-thread adding tape, seeds for the clear-sky K kernel
   DevBuf cscr_ad, cwide_ad, ctape, ctape_w, ks_tau, ks_b, ks_t, ks_ch;
+  // smallest angle count run by the wavefront (matrix-core) kernels
+  // (crtm_set_wavefront_angles)
+  int wide_min = 9;
   // kernel timing (crtm_profile_enable): HIP events on the context stream
   bool profiling = false;
   std::vector<hipEvent_t> ev;      // pairs (start, stop) per recorded launch
@@ -881,14 +884,16 @@ std::vector<int> angle_counts(int n_streams) {
   return {2, 3, 4, 5};   // Mie-parameter choice: 2, 4 or 6 streams
 }
 
-// Layer scratch for a set of angle counts: per-lane slots (SoA) for counts up
-// to NZ_AS, item-major wavefront slots for the 16-stream counts; the optics
-// part (tau, ssa, phase coefficients) is always SoA.
+// Layer scratch for a set of angle counts: per-lane slots (SoA) for counts
+// below wide_min, item-major wavefront slots for the others; the optics part
+// (tau, ssa, phase coefficients) is always SoA.  The kernels take an angle
+// count to the wavefront kernels when it exceeds sc.nzc.
 template <class RT>
-LayerScratch<RT> layer_scratch(const std::vector<int>& nz, int64_t n_items, DevBuf& soa, DevBuf& wide) {
+LayerScratch<RT> layer_scratch(const std::vector<int>& nz, int64_t n_items, DevBuf& soa, DevBuf& wide,
+                               int wide_min) {
   LayerScratch<RT> sc{};
   for (int z : nz) {
-    if (z <= NZ_AS) sc.nzc = std::max(sc.nzc, z);
+    if (z < wide_min) sc.nzc = std::max(sc.nzc, z);
     else sc.nzw = std::max(sc.nzw, z);
   }
   sc.npc = sc.nzw > 0 ? NL_W - 1 : NL_AS - 1;
@@ -910,7 +915,7 @@ void run_allsky(crtm_context* c, Sensor& S, const DevBatch& be, const DevClouds&
   const int64_t items = (int64_t)total_cl * S.d.L;
   if (items > INT32_MAX) throw CrtmError(fmt("%lld cloudy layer-channels exceed one launch", (long long)items));
   const std::vector<int> nz = angle_counts(cl.n_streams);
-  const LayerScratch<RT> sc = layer_scratch<RT>(nz, items, c->cscr, c->cwide);
+  const LayerScratch<RT> sc = layer_scratch<RT>(nz, items, c->cscr, c->cwide, c->wide_min);
   auto timed = [&](auto&& fn) {
     if (c->profiling) {
       c->mark();
@@ -1002,8 +1007,8 @@ void run_sensor_k_allsky(crtm_context* c, Sensor& S, const DevBatch& be, const D
   const int64_t items = (int64_t)cc.total * L;
   if (items > INT32_MAX) throw CrtmError(fmt("%lld cloudy layer-channels exceed one launch", (long long)items));
   const std::vector<int> nz = angle_counts(cc.cl.n_streams);
-  const LayerScratch<RT> sc = layer_scratch<RT>(nz, items, c->cscr, c->cwide);
-  const LayerScratch<RT> sa = layer_scratch<RT>(nz, items, c->cscr_ad, c->cwide_ad);
+  const LayerScratch<RT> sc = layer_scratch<RT>(nz, items, c->cscr, c->cwide, c->wide_min);
+  const LayerScratch<RT> sa = layer_scratch<RT>(nz, items, c->cscr_ad, c->cwide_ad, c->wide_min);
   const int64_t ML = (int64_t)be.M * L;
   const int64_t tape_lanes = (int64_t)allsky_sweep_ad_blocks(ML) * 256;
   const int te = sc.nzc * sc.nzc + sc.nzc;
@@ -1479,6 +1484,14 @@ int crtm_k_matrix(crtm_context* ctx, const crtm_atmosphere* atm, const crtm_surf
   } catch (const std::exception& e) {
     return fail(ctx, e.what());
   }
+}
+
+int crtm_set_wavefront_angles(crtm_context* ctx, int min_angles) {
+  if (!ctx) return CRTM_FAILURE;
+  if (min_angles != 5 && min_angles != 6 && min_angles != 9)
+    return fail(ctx, fmt("wavefront angle threshold %d: 5, 6 or 9", min_angles));
+  ctx->wide_min = min_angles;
+  return CRTM_SUCCESS;
 }
 
 int crtm_set_stream(crtm_context* ctx, void* stream, int own_stream) {

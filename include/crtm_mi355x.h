@@ -217,6 +217,15 @@ int crtm_k_matrix(crtm_context* ctx, const crtm_atmosphere* atm, const crtm_surf
  * the OpenMP execution-context choice in src/CRTM_Forward_Module.f90:337-357. */
 int crtm_set_stream(crtm_context* ctx, void* stream, int own_stream);
 
+/* Scattering-kernel layout: angle counts >= min_angles run on the wavefront
+ * kernels (one 64-lane wavefront per layer or profile-channel, stream
+ * matrices on the matrix cores), smaller ones on the per-lane kernels (one
+ * lane per layer or profile-channel, matrices in registers).  min_angles is
+ * 5 (8 and 16 streams on the wavefront kernels), 6 or 9 (the default: only
+ * 16 streams).  Results agree to rounding either way.  No counterpart in the
+ * reference, whose solver has one serial form (src/RTSolution/ADA/ADA_Module.f90). */
+int crtm_set_wavefront_angles(crtm_context* ctx, int min_angles);
+
 /* Wait for the context stream (device-memory calls are asynchronous). */
 int crtm_synchronize(crtm_context* ctx);
 

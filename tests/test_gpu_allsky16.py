@@ -215,3 +215,39 @@ def test_k_infrared_aerosols_16_streams(coeff_dir, oracle_mod, precision):
                  aerosols=True)
     finally:
         ctx.destroy()
+
+
+# ------------------------------------------- wavefront kernels at 8 streams
+@pytest.mark.parametrize("precision", [api.PREC_FP64, api.PREC_FP32_SPLIT], ids=["fp64", "split"])
+@pytest.mark.parametrize("n_streams", [8, 0])
+def test_wavefront_kernels_at_8_streams(ctx_mw, coeff_dir, oracle_mod, precision, n_streams):
+    """crtm_set_wavefront_angles(5): the 6-stream (5 angles) and 8-stream (5, 6
+    angles) layers run on the wavefront kernels; forward and K against the
+    oracle, and against the per-lane kernels."""
+    print(f"wavefront kernels n_streams={n_streams} precision={precision}", flush=True)
+    S, C = _oracle_sensor(coeff_dir, oracle_mod, "atms_syn")
+    ps = synthetic.add_clouds(synthetic.make_profiles(5, 100, seed=281), seed=282, cover="random",
+                              reff_scale=25.0 if n_streams == 0 else 1.0)
+    rng = np.random.default_rng(28)
+    eps = rng.uniform(0.45, 0.75, (5, 22))
+    btk = rng.uniform(0.5, 1.5, (5, 22))
+    kw = dict(bt_k=btk, precision=precision, jacobian_f32=bool(precision), n_streams=n_streams, extras=True)
+    try:
+        ctx_mw.set_wavefront_angles(5)
+        rts_w, kd_w = ctx_mw.k_matrix(ps, eps, **kw)
+        fwd_w = ctx_mw.forward(ps, eps, precision=precision, n_streams=n_streams)
+    finally:
+        ctx_mw.set_wavefront_angles(9)
+    rts_l, kd_l = ctx_mw.k_matrix(ps, eps, **kw)
+    _check_k(ps, rts_w, kd_w, _oracle_k(oracle_mod, S, C, ps, eps, btk, n_streams), precision,
+             f"wavefront n={n_streams}")
+    tol = 1e-9 if precision == api.PREC_FP64 else 1e-3
+    np.testing.assert_allclose(fwd_w["brightness_temperature"], rts_l["brightness_temperature"], rtol=0, atol=tol)
+    for key in ("temperature_k", "cloud_water_content_k", "cloud_effective_radius_k"):
+        a, b = np.asarray(kd_w[key], np.float64), np.asarray(kd_l[key], np.float64)
+        assert np.abs(a - b).max() <= (1e-8 if precision == api.PREC_FP64 else 1e-3) * np.abs(b).max(), key
+
+
+def test_wavefront_angles_argument(ctx_mw):
+    with pytest.raises(api.CRTMError, match="wavefront angle threshold 7"):
+        ctx_mw.set_wavefront_angles(7)
