@@ -137,7 +137,7 @@ def main():
                          "scaling the FETCH_SIZE / WRITE_SIZE counters to bytes")
     ap.add_argument("--streams", type=int, default=8, help="allsky-ir: ADA stream count (8 or 16)")
     ap.add_argument("--wavefront-angles", type=int, default=0,
-                    help="all-sky: angle counts >= this run on the wavefront (matrix-core) kernels (5, 6, 9)")
+                    help="all-sky: angle counts >= this run on the wavefront (matrix-core) kernels (6, 9)")
     ap.add_argument("--workload", choices=("clear-ir", "allsky-mw", "allsky-ir"), default="clear-ir",
                     help="clear-ir: the headline (CrIS-like forward + K-matrix); allsky-mw: cloudy ATMS-like "
                          "forward + K-matrix with the ADA scattering solver at 4 and 8 streams; allsky-ir: "

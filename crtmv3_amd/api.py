@@ -66,7 +66,7 @@ class CGeometry(ctypes.Structure):
 class COptions(ctypes.Structure):
     _fields_ = [("use_emissivity", ctypes.c_int), ("emissivity", _vp), ("precision", ctypes.c_int),
                 ("memory", ctypes.c_int), ("jacobian_f32", ctypes.c_int), ("check_input", ctypes.c_int),
-                ("n_streams", ctypes.c_int), ("overlap_id", ctypes.c_int)]
+                ("n_streams", ctypes.c_int), ("overlap_id", ctypes.c_int), ("n_streams_profile", _vp)]
 
 
 class CRTSolution(ctypes.Structure):
@@ -181,7 +181,7 @@ class Context:
             self._stream = h
 
     def set_wavefront_angles(self, min_angles: int):
-        """Angle counts >= min_angles (5, 6 or 9) run on the wavefront
+        """Angle counts >= min_angles (6 or 9) run on the wavefront
         (matrix-core) scattering kernels, smaller ones on the per-lane kernels."""
         if lib().crtm_set_wavefront_angles(self._h, int(min_angles)) != SUCCESS:
             self._err("crtm_set_wavefront_angles")
@@ -254,7 +254,20 @@ class Context:
         opt.memory = memory
         opt.jacobian_f32 = int(jacobian_f32)
         opt.check_input = int(check_input)
-        opt.n_streams = int(n_streams)
+        if np.ndim(n_streams) == 0 and not _is_torch(n_streams):
+            opt.n_streams = int(n_streams)
+        else:   # Options(m)%n_Streams per profile
+            if memory == MEM_DEVICE:
+                import torch
+                ns = torch.as_tensor(n_streams, dtype=torch.int32, device=prof.temperature.device).contiguous()
+                keep.append(ns)
+            else:
+                ns = np.ascontiguousarray(np.asarray(n_streams), np.int32)
+                keep.append(ns)
+            if ns.shape[0] != prof.n_profiles:
+                raise CRTMError(f"n_streams: {ns.shape[0]} values for {prof.n_profiles} profiles")
+            opt.n_streams = 0
+            opt.n_streams_profile = _ptr(ns)
         opt.overlap_id = int(overlap)
         return atm, sfc, geo, opt
 
@@ -270,8 +283,9 @@ class Context:
         """CRTM_Forward over a ProfileSet (host numpy) or a device batch (torch).
 
         Profiles carrying clouds take the all-sky path; n_streams is
-        Options%n_Streams (0: chosen from the Mie parameter) and overlap
-        Options%Overlap_Id (0: average)."""
+        Options%n_Streams (0: chosen from the Mie parameter), or one value per
+        profile (Options(m)%n_Streams), and overlap Options%Overlap_Id
+        (0: average)."""
         memory = MEM_DEVICE if _is_torch(prof.temperature) else MEM_HOST
         if memory == MEM_DEVICE:
             self._use_torch_stream()
@@ -355,6 +369,85 @@ class Context:
             self._err("crtm_k_matrix")
         return rts_d, kd
 
+    def adjoint(self, prof, emissivity, bt_ad, precision=PREC_FP64, n_streams=0, overlap=0, jacobian_f32=None,
+                max_jacobian_bytes=8 << 30):
+        """CRTM_Adjoint (src/CRTM_Adjoint_Module.f90:299-325) for a device batch:
+        the channel-summed adjoint of the brightness temperatures seeded with
+        bt_ad [M, L] (RTSolution_AD%Brightness_Temperature).
+
+        Returns (bt [M, L], ad) with ad holding temperature_ad [NT],
+        absorber_ad [n_absorbers, NT], surface temperature adjoints per type
+        [M], surface_emissivity_ad [M, L] (per channel, as RTSolution_AD%
+        Surface_Emissivity) and, with clouds / aerosols, cloud_water_content_ad,
+        cloud_effective_radius_ad [n_clouds, NT], cloud_fraction_ad [NT],
+        aerosol_concentration_ad, aerosol_effective_radius_ad [n_aerosols, NT].
+
+        Computed as the K-matrix with RTSolution_K = bt_ad summed over
+        channels (the adjoint and the K-matrix share every _AD routine),
+        in profile chunks whose per-channel Jacobians stay within
+        max_jacobian_bytes, so batches far beyond the K-matrix's own memory
+        run in one call.  n_streams may be one value or one per profile."""
+        import torch
+        if not _is_torch(prof.temperature):
+            raise CRTMError("adjoint runs on device batches (api.device_profiles)")
+        if jacobian_f32 is None:
+            jacobian_f32 = precision == PREC_FP32_SPLIT
+        M, L = prof.n_profiles, self.n_channels
+        nl = np.asarray(prof.n_layers_host, np.int64)
+        loh = np.concatenate([[0], np.cumsum(nl)])
+        NT = int(loh[-1])
+        na = int(prof.absorber.shape[0])
+        nc = 0 if prof.cloud_type is None else int(prof.cloud_type.shape[1])
+        nae = 0 if prof.aerosol_type is None else int(prof.aerosol_type.shape[1])
+        fields = 2 + na + (3 * nc + 1 if nc else 0) + 2 * nae
+        per_layer = L * fields * (4 if jacobian_f32 else 8)
+        dev = prof.temperature.device
+        f64 = dict(dtype=torch.float64, device=dev)
+        ad = {"temperature_ad": torch.zeros(NT, **f64), "absorber_ad": torch.zeros(na, NT, **f64),
+              "surface_emissivity_ad": torch.zeros(M, L, **f64)}
+        for f in ("land", "water", "snow", "ice"):
+            ad[f + "_temperature_ad"] = torch.zeros(M, **f64)
+        if nc:
+            ad["cloud_water_content_ad"] = torch.zeros(nc, NT, **f64)
+            ad["cloud_effective_radius_ad"] = torch.zeros(nc, NT, **f64)
+            ad["cloud_fraction_ad"] = torch.zeros(NT, **f64)
+        if nae:
+            ad["aerosol_concentration_ad"] = torch.zeros(nae, NT, **f64)
+            ad["aerosol_effective_radius_ad"] = torch.zeros(nae, NT, **f64)
+        bt = torch.empty(M, L, **f64)
+        per_profile_streams = not (np.ndim(n_streams) == 0 and not _is_torch(n_streams))
+        m0 = 0
+        while m0 < M:
+            m1 = m0 + 1
+            while m1 < M and (loh[m1 + 1] - loh[m0]) * per_layer <= max_jacobian_bytes:
+                m1 += 1
+            a, b = int(loh[m0]), int(loh[m1])
+            sub = profile_slice(prof, m0, m1)
+            ns = n_streams[m0:m1] if per_profile_streams else n_streams
+            rts, kd = self.k_matrix(sub, emissivity[m0:m1].contiguous(), bt_k=bt_ad[m0:m1].contiguous(),
+                                    precision=precision, jacobian_f32=jacobian_f32, check_input=False,
+                                    n_streams=ns, overlap=overlap)
+            bt[m0:m1] = rts["brightness_temperature"]
+            ad["temperature_ad"][a:b] = torch.sum(kd["temperature_k"], dim=-1, dtype=torch.float64)
+            ad["absorber_ad"][:, a:b] = torch.sum(kd["absorber_k"], dim=-1, dtype=torch.float64)
+            ad["surface_emissivity_ad"][m0:m1] = kd["surface_emissivity_k"]
+            for f in ("land", "water", "snow", "ice"):
+                ad[f + "_temperature_ad"][m0:m1] = kd[f + "_temperature_k"].sum(dim=-1)
+            if nc:
+                ad["cloud_water_content_ad"][:, a:b] = torch.sum(kd["cloud_water_content_k"], dim=-1,
+                                                                 dtype=torch.float64)
+                ad["cloud_effective_radius_ad"][:, a:b] = torch.sum(kd["cloud_effective_radius_k"], dim=-1,
+                                                                    dtype=torch.float64)
+                ad["cloud_fraction_ad"][a:b] = torch.sum(kd["cloud_fraction_k"], dim=-1, dtype=torch.float64)
+            if nae:
+                ad["aerosol_concentration_ad"][:, a:b] = torch.sum(kd["aerosol_concentration_k"], dim=-1,
+                                                                   dtype=torch.float64)
+                ad["aerosol_effective_radius_ad"][:, a:b] = torch.sum(kd["aerosol_effective_radius_k"], dim=-1,
+                                                                      dtype=torch.float64)
+            del rts, kd
+            m0 = m1
+        return bt, ad
+
     def profile_enable(self, on=True):
         if lib().crtm_profile_enable(self._h, int(on)) != SUCCESS:
             self._err("crtm_profile_enable")
@@ -399,6 +492,43 @@ def crtm_init(sensor_ids: Sequence[str], file_path: str) -> Context:
         infos.append(ChannelInfo(sid.value.decode(), t.value, n.value, ch, off))
         off += n.value
     return Context(h, infos)
+
+
+def profile_slice(prof, m0, m1):
+    """Profiles [m0, m1) of a batch (host numpy or device torch) as a batch of
+    its own: per-profile arrays sliced, per-layer arrays cut at the layer
+    offsets, offsets rebased to the first profile."""
+    host_nl = getattr(prof, "n_layers_host", None)
+    nl = np.asarray(host_nl if host_nl is not None else prof.n_layers, np.int64)
+    loh = np.concatenate([[0], np.cumsum(nl)])
+    a, b = int(loh[m0]), int(loh[m1])
+    torch_batch = _is_torch(prof.temperature)
+
+    def cut(v):
+        return v.contiguous() if torch_batch else np.ascontiguousarray(v)
+    per_layer = ("pressure", "temperature", "cloud_fraction")
+    per_layer_2d = ("absorber", "cloud_water_content", "cloud_effective_radius", "aerosol_concentration",
+                    "aerosol_effective_radius")
+    shared = ("absorber_id", "absorber_units")
+    d = {}
+    for f in dataclasses.fields(prof):
+        v = getattr(prof, f.name)
+        if v is None or f.name in shared:
+            continue
+        if f.name == "layer_offset":
+            d[f.name] = cut(v[m0:m1 + 1] - v[m0])
+        elif f.name == "level_pressure":
+            d[f.name] = cut(v[a + m0:b + m1])
+        elif f.name in per_layer:
+            d[f.name] = cut(v[a:b])
+        elif f.name in per_layer_2d:
+            d[f.name] = cut(v[:, a:b])
+        else:   # per profile (and the cloud / aerosol type rows)
+            d[f.name] = cut(v[m0:m1])
+    out = dataclasses.replace(prof, **d)
+    if host_nl is not None:
+        out.n_layers_host = np.asarray(host_nl)[m0:m1]
+    return out
 
 
 def device_profiles(prof, device="cuda"):

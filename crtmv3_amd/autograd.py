@@ -1,21 +1,26 @@
 """torch.autograd.Function over the C-ABI: brightness temperatures as a
-differentiable function of the atmospheric state.
+differentiable function of the atmospheric state, clouds and aerosols.
 
 Forward is CRTM_Forward (src/CRTM_Forward_Module.f90:229-245); backward is
-the adjoint of the same model, obtained from CRTM_K_Matrix
-(src/CRTM_K_Matrix_Module.f90:300-325) seeded with the incoming gradient as
-RTSolution_K%Brightness_Temperature and summed over channels -- the
-channel-summed adjoint that CRTM_Adjoint (src/CRTM_Adjoint_Module.f90:299)
-returns.  Both run on the GPU in device memory on torch's current stream.
+CRTM_Adjoint (src/CRTM_Adjoint_Module.f90:299-325) seeded with the incoming
+gradient as RTSolution_AD%Brightness_Temperature: the channel-summed adjoint,
+computed by Context.adjoint from the K-matrix kernels in profile chunks, so
+the gradient of a large batch does not hold a full K-matrix.  Both run on
+the GPU in device memory on torch's current stream.
 
     bt = crtm_bt(ctx, profiles, temperature, absorber, water_temperature,
-                 emissivity, precision=api.PREC_FP32_SPLIT)
+                 emissivity, precision=api.PREC_FP32_SPLIT,
+                 cloud_water_content=wc, cloud_effective_radius=re,
+                 aerosol_concentration=ac, n_streams=ns)
     J = (((y - bt) / sigma) ** 2).sum(); J.backward()
 
 `profiles` is a device batch from api.device_profiles (it supplies the
-pressures, layer counts, coverages and geometry); the differentiable inputs
-replace its temperature [NT], absorber [n_absorbers, NT] and water
-temperature [M].  Gradients flow to those and to the emissivity [M, L].
+pressures, layer counts, coverages, cloud / aerosol types and geometry); the
+differentiable inputs replace its temperature [NT], absorber [n_absorbers,
+NT] and water temperature [M], and, when given, its cloud water content and
+effective radius [n_clouds, NT] and aerosol concentration and effective
+radius [n_aerosols, NT].  Gradients flow to those and to the emissivity
+[M, L].  n_streams is Options%n_Streams, one value or one per profile.
 """
 from __future__ import annotations
 
@@ -25,43 +30,54 @@ import torch
 
 from . import api
 
+_FIELDS = ("temperature", "absorber", "water_temperature", "cloud_water_content", "cloud_effective_radius",
+           "aerosol_concentration", "aerosol_effective_radius")
+_ADJOINT = ("temperature_ad", "absorber_ad", "water_temperature_ad", "cloud_water_content_ad",
+            "cloud_effective_radius_ad", "aerosol_concentration_ad", "aerosol_effective_radius_ad")
+
 
 class CRTMBrightnessTemperature(torch.autograd.Function):
     @staticmethod
-    def forward(ctx, temperature, absorber, water_temperature, emissivity, crtm_ctx, profiles, precision):
-        prof = _with_state(profiles, temperature, absorber, water_temperature)
-        rts = crtm_ctx.forward(prof, emissivity, precision=precision)
-        ctx.save_for_backward(temperature, absorber, water_temperature, emissivity)
-        ctx.crtm = (crtm_ctx, profiles, precision)
+    def forward(ctx, crtm_ctx, profiles, options, emissivity, *state):
+        precision, n_streams, overlap, max_bytes = options
+        prof = _with_state(profiles, state)
+        rts = crtm_ctx.forward(prof, emissivity.detach().to(torch.float64).contiguous(), precision=precision,
+                               n_streams=n_streams, overlap=overlap)
+        ctx.save_for_backward(emissivity, *[s for s in state if s is not None])
+        ctx.present = tuple(s is not None for s in state)
+        ctx.crtm = (crtm_ctx, profiles, options)
         return rts["brightness_temperature"]
 
     @staticmethod
     def backward(ctx, grad_bt):
-        temperature, absorber, water_temperature, emissivity = ctx.saved_tensors
-        crtm_ctx, profiles, precision = ctx.crtm
-        prof = _with_state(profiles, temperature, absorber, water_temperature)
+        saved = list(ctx.saved_tensors)
+        emissivity = saved.pop(0)
+        state = [saved.pop(0) if p else None for p in ctx.present]
+        crtm_ctx, profiles, (precision, n_streams, overlap, max_bytes) = ctx.crtm
+        prof = _with_state(profiles, state)
         seed = grad_bt.detach().to(torch.float64).contiguous()
-        _, k = crtm_ctx.k_matrix(prof, emissivity, bt_k=seed, precision=precision, jacobian_f32=False)
-        g_t = k["temperature_k"].sum(dim=1)
-        g_a = k["absorber_k"].sum(dim=2)
-        g_ts = k["water_temperature_k"].sum(dim=1)
-        g_e = k["surface_emissivity_k"]
-        return (g_t.to(temperature.dtype), g_a.to(absorber.dtype), g_ts.to(water_temperature.dtype),
-                g_e.to(emissivity.dtype), None, None, None)
+        _, ad = crtm_ctx.adjoint(prof, emissivity.detach().to(torch.float64).contiguous(), seed,
+                                 precision=precision, n_streams=n_streams, overlap=overlap,
+                                 max_jacobian_bytes=max_bytes)
+        grads = [None if s is None else ad[k].to(s.dtype) for s, k in zip(state, _ADJOINT)]
+        return (None, None, None, ad["surface_emissivity_ad"].to(emissivity.dtype), *grads)
 
 
-def _with_state(profiles, temperature, absorber, water_temperature):
-    prof = dataclasses.replace(profiles, temperature=temperature.detach().to(torch.float64).contiguous(),
-                               absorber=absorber.detach().to(torch.float64).contiguous(),
-                               water_temperature=water_temperature.detach().to(torch.float64).contiguous())
+def _with_state(profiles, state):
+    repl = {f: s.detach().to(torch.float64).contiguous() for f, s in zip(_FIELDS, state) if s is not None}
+    prof = dataclasses.replace(profiles, **repl)
     prof.n_layers_host = profiles.n_layers_host
     return prof
 
 
 def crtm_bt(crtm_ctx, profiles, temperature, absorber, water_temperature, emissivity,
-            precision=api.PREC_FP32_SPLIT):
+            precision=api.PREC_FP32_SPLIT, cloud_water_content=None, cloud_effective_radius=None,
+            aerosol_concentration=None, aerosol_effective_radius=None, n_streams=0, overlap=0,
+            max_jacobian_bytes=8 << 30):
     """Differentiable CRTM brightness temperatures [M, L]."""
     if not temperature.is_cuda:
         raise api.CRTMError("crtm_bt runs on the GPU: pass device tensors (api.device_profiles)")
-    return CRTMBrightnessTemperature.apply(temperature, absorber, water_temperature, emissivity, crtm_ctx,
-                                           profiles, precision)
+    state = (temperature, absorber, water_temperature, cloud_water_content, cloud_effective_radius,
+             aerosol_concentration, aerosol_effective_radius)
+    return CRTMBrightnessTemperature.apply(crtm_ctx, profiles, (precision, n_streams, overlap, max_jacobian_bytes),
+                                           emissivity, *state)

@@ -171,7 +171,7 @@ __global__ void __launch_bounds__(256) cloud_optics_kernel(DevSensor s, DevBatch
     const int2 d = cw.cl_desc[c];
     const int m = d.x, k = d.y;
     if (cw.status[m] != 0) continue;
-    const int nl = legendre_terms(cl.n_streams, cw.max_reff[m], s.wavenumber[l]);
+    const int nl = legendre_terms(profile_streams(cl, m), cw.max_reff[m], s.wavenumber[l]);
     if (nl > NL_W - 1) continue;   // stream counts above the built tables are rejected by the host
     const int64_t ik = be.loff[m] + k;
     // CRTM_Compute_CloudScatter: accumulate over the clouds of the layer
@@ -237,7 +237,7 @@ __global__ void __launch_bounds__(256) cloud_rt_kernel(DevSensor s, CloudWork cw
     const int c = item / s.L, l = item - c * s.L;
     const int m = cw.cl_desc[c].x;
     if (cw.status[m] != 0) continue;
-    const int nl = legendre_terms(cl.n_streams, cw.max_reff[m], s.wavenumber[l]);
+    const int nl = legendre_terms(profile_streams(cl, m), cw.max_reff[m], s.wavenumber[l]);
     if (nl > NL_AS - 1) continue;
     const double cosu = geo[(int64_t)m * 4 + 1];
     const AngleSet as = angle_set(cc.gauss, (nl + 2) / 2, cosu);
@@ -384,7 +384,7 @@ __global__ void __launch_bounds__(256) allsky_sweep_kernel(DevSensor s, DevBatch
     const int m = (int)(t / s.L), l = (int)(t - (int64_t)m * s.L);
     if (cw.cl_count[m] == 0 || cw.status[m] != 0 || cw.aer_err[m] != 0) continue;
     const double cosu = geo[(int64_t)m * 4 + 1];
-    const int nl = legendre_terms(cl.n_streams, cw.max_reff[m], s.wavenumber[l]);
+    const int nl = legendre_terms(profile_streams(cl, m), cw.max_reff[m], s.wavenumber[l]);
     if (nl > NL_AS - 1) continue;
     const AngleSet as = angle_set(cc.gauss, (nl + 2) / 2, cosu);
     if (as.nz != NZ) continue;
@@ -571,7 +571,7 @@ __global__ void aerosol_check_kernel(DevSensor s, int M, DevClouds cl, CloudWork
   const int64_t n = (int64_t)M * s.L;
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
     const int m = (int)(t / s.L), l = (int)(t - (int64_t)m * s.L);
-    const int nl = legendre_terms(cl.n_streams, cw.max_reff[m], s.wavenumber[l]);
+    const int nl = legendre_terms(profile_streams(cl, m), cw.max_reff[m], s.wavenumber[l]);
     bool bad = aerosol_offset(nl) < 0;
     for (int q = 0; q < cl.na && !bad; ++q) {
       const int ty = cl.atype[(int64_t)m * cl.na + q];

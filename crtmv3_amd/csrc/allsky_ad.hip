@@ -713,7 +713,7 @@ __global__ void __launch_bounds__(256) allsky_sweep_ad_kernel(DevSensor s, DevBa
     const int m = (int)(t / s.L), l = (int)(t - (int64_t)m * s.L);
     if (cw.cl_count[m] == 0 || cw.status[m] != 0 || cw.aer_err[m] != 0) continue;
     const double cosu = geo[(int64_t)m * 4 + 1];
-    const int nl = legendre_terms(cl.n_streams, cw.max_reff[m], s.wavenumber[l]);
+    const int nl = legendre_terms(profile_streams(cl, m), cw.max_reff[m], s.wavenumber[l]);
     if (nl > NL_AS - 1) continue;
     const AngleSet as = angle_set(cc.gauss, (nl + 2) / 2, cosu);
     if (as.nz != NZ) continue;
@@ -1145,7 +1145,7 @@ __global__ void __launch_bounds__(256) cloud_rt_ad_kernel(DevSensor s, CloudWork
     const int c = item / s.L, l = item - c * s.L;
     const int m = cw.cl_desc[c].x;
     if (cw.status[m] != 0 || cw.aer_err[m] != 0) continue;
-    const int nl = legendre_terms(cl.n_streams, cw.max_reff[m], s.wavenumber[l]);
+    const int nl = legendre_terms(profile_streams(cl, m), cw.max_reff[m], s.wavenumber[l]);
     if (nl > NL_AS - 1) continue;
     const double cosu = geo[(int64_t)m * 4 + 1];
     const AngleSet as = angle_set(cc.gauss, (nl + 2) / 2, cosu);
@@ -1295,7 +1295,7 @@ __global__ void __launch_bounds__(256) cloud_optics_ad_kernel(DevSensor s, DevBa
     const int2 d = cw.cl_desc[c];
     const int m = d.x, k = d.y;
     if (cw.status[m] != 0 || cw.aer_err[m] != 0) continue;
-    const int nl = legendre_terms(cl.n_streams, cw.max_reff[m], s.wavenumber[l]);
+    const int nl = legendre_terms(profile_streams(cl, m), cw.max_reff[m], s.wavenumber[l]);
     if (nl > NL_W - 1) continue;
     const int64_t ik = be.loff[m] + k;
     const int loff = legendre_offset(nl);

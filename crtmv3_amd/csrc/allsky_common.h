@@ -32,6 +32,9 @@ __device__ inline int legendre_terms(int user, double max_reff, double wavenumbe
   return mie < 0.01 ? 2 : (mie < 1.0 ? 4 : 6);
 }
 
+// Options(m)%n_Streams of profile m
+__device__ inline int profile_streams(const DevClouds& cl, int m) { return cl.ns_prof ? cl.ns_prof[m] : cl.n_streams; }
+
 // Angle set of a (profile, channel): ns Gauss nodes plus the sensor angle
 // unless it coincides with a node (Common_RTSolution.f90:255-281)
 struct AngleSet {

@@ -30,7 +30,7 @@ __global__ void __launch_bounds__(64) cloud_rt_wide_kernel(DevSensor s, CloudWor
     const int c = item / s.L, l = item - c * s.L;
     const int m = cw.cl_desc[c].x;
     if (cw.status[m] != 0) continue;
-    const int nl = legendre_terms(cl.n_streams, cw.max_reff[m], s.wavenumber[l]);
+    const int nl = legendre_terms(profile_streams(cl, m), cw.max_reff[m], s.wavenumber[l]);
     if (nl > NL_W - 1) continue;
     const double cosu = geo[(int64_t)m * 4 + 1];
     const AngleSet as = angle_set(cc.gauss, (nl + 2) / 2, cosu);
@@ -70,7 +70,7 @@ __global__ void __launch_bounds__(64) allsky_sweep_wide_kernel(DevSensor s, DevB
     const int m = (int)(t / s.L), l = (int)(t - (int64_t)m * s.L);
     if (cw.cl_count[m] == 0 || cw.status[m] != 0 || cw.aer_err[m] != 0) continue;
     const double cosu = geo[(int64_t)m * 4 + 1];
-    const int nl = legendre_terms(cl.n_streams, cw.max_reff[m], s.wavenumber[l]);
+    const int nl = legendre_terms(profile_streams(cl, m), cw.max_reff[m], s.wavenumber[l]);
     if (nl > NL_W - 1) continue;
     const AngleSet as = angle_set(cc.gauss, (nl + 2) / 2, cosu);
     if (as.nz != NZ) continue;
@@ -187,8 +187,8 @@ __global__ void __launch_bounds__(64) allsky_sweep_wide_kernel(DevSensor s, DevB
 // ------------------------------------------------------------------ launchers
 static int wide_blocks(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>(n, 16384)); }
 
-// angle counts of the wavefront kernels: 8 streams (5, 6) and 16 streams (9, 10)
-#define CRTM_WIDE_NZ(X) X(5) X(6) X(9) X(10)
+// angle counts of the wavefront kernels: 8 streams (6) and 16 streams (9, 10)
+#define CRTM_WIDE_NZ(X) X(6) X(9) X(10)
 
 template <class RT>
 void launch_cloud_rt_wide(int nz, const DevSensor& s, const CloudWork& cw, const DevClouds& cl,

@@ -294,7 +294,7 @@ __global__ void __launch_bounds__(64) allsky_sweep_ad_wide_kernel(DevSensor s, D
     const int m = (int)(t / s.L), l = (int)(t - (int64_t)m * s.L);
     if (cw.cl_count[m] == 0 || cw.status[m] != 0 || cw.aer_err[m] != 0) continue;
     const double cosu = geo[(int64_t)m * 4 + 1];
-    const int nl = legendre_terms(cl.n_streams, cw.max_reff[m], s.wavenumber[l]);
+    const int nl = legendre_terms(profile_streams(cl, m), cw.max_reff[m], s.wavenumber[l]);
     if (nl > NL_W - 1) continue;
     const AngleSet as = angle_set(cc.gauss, (nl + 2) / 2, cosu);
     if (as.nz != NZ) continue;
@@ -624,7 +624,7 @@ __global__ void __launch_bounds__(64) cloud_rt_ad_wide_kernel(DevSensor s, Cloud
     const int c = item / s.L, l = item - c * s.L;
     const int m = cw.cl_desc[c].x;
     if (cw.status[m] != 0 || cw.aer_err[m] != 0) continue;
-    const int nl = legendre_terms(cl.n_streams, cw.max_reff[m], s.wavenumber[l]);
+    const int nl = legendre_terms(profile_streams(cl, m), cw.max_reff[m], s.wavenumber[l]);
     if (nl > NL_W - 1) continue;
     const double cosu = geo[(int64_t)m * 4 + 1];
     const AngleSet as = angle_set(cc.gauss, (nl + 2) / 2, cosu);
@@ -747,7 +747,6 @@ void launch_allsky_ad_wide(int nz, const DevSensor& s, const DevBatch& be, const
       hipLaunchKernelGGL((cloud_rt_ad_wide_kernel<RT, N>), dim3(lb), dim3(64), 0, st, s, cw, cl, cc, sc, sa, geo,  \
                          n_items);                                                                                 \
     break;
-    CRTM_CASE(5)
     CRTM_CASE(6)
     CRTM_CASE(9)
     CRTM_CASE(10)

@@ -876,13 +876,20 @@ int prepare_clouds(crtm_context* c, const DevBatch& be, const DevClouds& cl, Clo
 // Angle counts the layer and sweep kernels are launched for: the stream
 // choice gives n quadrature angles plus the sensor angle (n alone when the
 // sensor angle coincides with a node).
-std::vector<int> angle_counts(int n_streams) {
+unsigned angle_mask(int n_streams) {
   if (n_streams > 0) {
     const int ns = (n_streams + 2) / 2;
-    return {ns, ns + 1};
+    return 3u << ns;
   }
-  return {2, 3, 4, 5};   // Mie-parameter choice: 2, 4 or 6 streams
+  return 0xFu << 2;   // Mie-parameter choice: 2, 4 or 6 streams
 }
+std::vector<int> angle_counts(unsigned mask) {
+  std::vector<int> v;
+  for (int z = 1; z < 32; ++z)
+    if (mask >> z & 1) v.push_back(z);
+  return v;
+}
+bool valid_streams(int n) { return n == 0 || n == 2 || n == 4 || n == 6 || n == 8 || n == 16; }
 
 // Layer scratch for a set of angle counts: per-lane slots (SoA) for counts
 // below wide_min, item-major wavefront slots for the others; the optics part
@@ -914,7 +921,7 @@ void run_allsky(crtm_context* c, Sensor& S, const DevBatch& be, const DevClouds&
   cw.status = w.status;
   const int64_t items = (int64_t)total_cl * S.d.L;
   if (items > INT32_MAX) throw CrtmError(fmt("%lld cloudy layer-channels exceed one launch", (long long)items));
-  const std::vector<int> nz = angle_counts(cl.n_streams);
+  const std::vector<int> nz = angle_counts(cl.nz_mask);
   const LayerScratch<RT> sc = layer_scratch<RT>(nz, items, c->cscr, c->cwide, c->wide_min);
   auto timed = [&](auto&& fn) {
     if (c->profiling) {
@@ -1006,7 +1013,7 @@ void run_sensor_k_allsky(crtm_context* c, Sensor& S, const DevBatch& be, const D
   cw.status = w.status;
   const int64_t items = (int64_t)cc.total * L;
   if (items > INT32_MAX) throw CrtmError(fmt("%lld cloudy layer-channels exceed one launch", (long long)items));
-  const std::vector<int> nz = angle_counts(cc.cl.n_streams);
+  const std::vector<int> nz = angle_counts(cc.cl.nz_mask);
   const LayerScratch<RT> sc = layer_scratch<RT>(nz, items, c->cscr, c->cwide, c->wide_min);
   const LayerScratch<RT> sa = layer_scratch<RT>(nz, items, c->cscr_ad, c->cwide_ad, c->wide_min);
   const int64_t ML = (int64_t)be.M * L;
@@ -1114,10 +1121,29 @@ int do_call(crtm_context* ctx, const crtm_atmosphere* atm, const crtm_surface* s
         (!atm->aerosol_type || !atm->aerosol_concentration || !atm->aerosol_effective_radius))
       return fail(ctx, "aerosol type, concentration and effective radius arrays required");
     if (atm->n_clouds < 0 || atm->n_aerosols < 0) return fail(ctx, "negative cloud or aerosol count");
-    if (opt->n_streams != 0 && opt->n_streams != 2 && opt->n_streams != 4 && opt->n_streams != 6 &&
-        opt->n_streams != 8 && opt->n_streams != 16)
+    if (!valid_streams(opt->n_streams))
       return fail(ctx, fmt("n_streams %d: the scattering kernels take 2, 4, 6, 8 or 16 streams (0 = automatic)",
                            opt->n_streams));
+    // per-profile stream counts (Options(m)%n_Streams): a host copy gives the
+    // angle counts to launch for
+    cc.cl.nz_mask = angle_mask(opt->n_streams);
+    if (opt->n_streams_profile) {
+      std::vector<int> nsp(M);
+      if (opt->memory == CRTM_MEM_HOST) {
+        std::memcpy(nsp.data(), opt->n_streams_profile, M * sizeof(int));
+      } else {
+        HIPCHECK(hipMemcpyAsync(nsp.data(), opt->n_streams_profile, M * sizeof(int), hipMemcpyDeviceToHost,
+                                ctx->stream));
+        HIPCHECK(hipStreamSynchronize(ctx->stream));
+      }
+      cc.cl.nz_mask = 0;
+      for (int m = 0; m < M; ++m) {
+        if (!valid_streams(nsp[m]))
+          return fail(ctx, fmt("profile #%d: n_streams %d: the scattering kernels take 2, 4, 6, 8 or 16 streams "
+                               "(0 = automatic)", m + 1, nsp[m]));
+        cc.cl.nz_mask |= angle_mask(nsp[m]);
+      }
+    }
     if (opt->overlap_id < 0 || opt->overlap_id > 5) return fail(ctx, "Invalid overlap assumption");
     if (k && atm->n_clouds > 0 &&
         (!k->cloud_water_content_k || !k->cloud_effective_radius_k || !k->cloud_fraction_k))
@@ -1204,6 +1230,7 @@ int do_call(crtm_context* ctx, const crtm_atmosphere* atm, const crtm_surface* s
       in(atm->cloud_water_content, nc * NT * sizeof(double), (const void**)&cc.cl.wc);
       in(atm->cloud_effective_radius, nc * NT * sizeof(double), (const void**)&cc.cl.reff);
       in(atm->cloud_fraction, NT * sizeof(double), (const void**)&cc.cl.cf);
+      in(opt->n_streams_profile, M * sizeof(int), (const void**)&cc.cl.ns_prof);
       for (size_t i = 0; i < (size_t)M * nc; ++i)
         if (atm->cloud_type[i] < 0 || atm->cloud_type[i] > 6)
           return fail(ctx, fmt("profile #%d: invalid cloud type %d", (int)(i / nc) + 1, atm->cloud_type[i]));
@@ -1346,6 +1373,7 @@ int do_call(crtm_context* ctx, const crtm_atmosphere* atm, const crtm_surface* s
     cc.cl.atype = atm->aerosol_type;
     cc.cl.aconc = atm->aerosol_concentration;
     cc.cl.areff = atm->aerosol_effective_radius;
+    cc.cl.ns_prof = opt->n_streams_profile;
     if (dok) {
       cc.ko.wc_k = k->cloud_water_content_k;
       cc.ko.reff_k = k->cloud_effective_radius_k;
@@ -1488,8 +1516,8 @@ int crtm_k_matrix(crtm_context* ctx, const crtm_atmosphere* atm, const crtm_surf
 
 int crtm_set_wavefront_angles(crtm_context* ctx, int min_angles) {
   if (!ctx) return CRTM_FAILURE;
-  if (min_angles != 5 && min_angles != 6 && min_angles != 9)
-    return fail(ctx, fmt("wavefront angle threshold %d: 5, 6 or 9", min_angles));
+  if (min_angles != 6 && min_angles != 9)
+    return fail(ctx, fmt("wavefront angle threshold %d: 6 or 9", min_angles));
   ctx->wide_min = min_angles;
   return CRTM_SUCCESS;
 }

@@ -296,6 +296,8 @@ struct DevClouds {
   const double* reff;   // [nc][out_NT] microns
   const double* cf;     // [out_NT] or null
   int n_streams;        // Options%n_Streams (0: chosen from the Mie parameter)
+  const int* ns_prof;   // [M] per-profile Options(m)%n_Streams, or null (n_streams for all)
+  unsigned nz_mask;     // host side: bit z set when angle count z occurs in the call
   int overlap;          // Options%Overlap_Id 1..5
   int na;               // aerosol slots per profile
   const int* atype;     // [M][na] aerosol type id (0: empty slot)

@@ -123,6 +123,7 @@ MODULE CRTM_MI355X_Module
     TYPE(C_PTR) :: emissivity
     INTEGER(C_INT) :: precision, memory, jacobian_f32, check_input
     INTEGER(C_INT) :: n_streams, overlap_id
+    TYPE(C_PTR) :: n_streams_profile
   END TYPE
   TYPE, BIND(C) :: c_rtsolution
     TYPE(C_PTR) :: radiance, brightness_temperature, up_radiance, down_radiance, &
@@ -194,6 +195,7 @@ MODULE CRTM_MI355X_Module
     REAL(C_DOUBLE), ALLOCATABLE :: cwc(:), creff(:), cfrac(:)
     INTEGER(C_INT), ALLOCATABLE :: atype(:)
     REAL(C_DOUBLE), ALLOCATABLE :: aconc(:), areff(:)
+    INTEGER(C_INT), ALLOCATABLE :: nsp(:)
     TYPE(c_atmosphere) :: atm
     TYPE(c_surface) :: sfc
     TYPE(c_geometry) :: geo
@@ -556,6 +558,14 @@ CONTAINS
     b%opt%check_input = MERGE(1, 0, Opt(1)%Check_Input)
     b%opt%n_streams = MERGE(Opt(1)%n_Streams, 0, Opt(1)%Use_N_Streams)
     b%opt%overlap_id = Opt(1)%Overlap_Id
+    b%opt%n_streams_profile = C_NULL_PTR
+    IF (SIZE(Opt) == M .AND. M > 1) THEN   ! Options(m)%n_Streams per profile
+      ALLOCATE(b%nsp(M))
+      DO mm = 1, M
+        b%nsp(mm) = MERGE(Opt(mm)%n_Streams, 0, Opt(mm)%Use_N_Streams)
+      END DO
+      b%opt%n_streams_profile = C_LOC(b%nsp)
+    END IF
     b%rts%radiance = C_LOC(b%rad)
     b%rts%brightness_temperature = C_LOC(b%bt)
     b%rts%up_radiance = C_LOC(b%up)

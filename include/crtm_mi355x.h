@@ -125,6 +125,9 @@ typedef struct {
                                     (CRTM_Compute_nStreams, src/RTSolution/CRTM_RTSolution.f90:1204) */
   int overlap_id;                /* Options%Overlap_Id 1..5 (maximum, random, maximum-random,
                                     average, overcast); 0 = default (average) */
+  const int* n_streams_profile;  /* [M] or NULL: Options(m)%n_Streams per profile (the
+                                    reference's Options is per profile), overriding
+                                    n_streams; same values */
 } crtm_options;
 
 /* ---- RTSolution (src/RTSolution/CRTM_RTSolution_Define.f90:201-245) ---- */
@@ -221,9 +224,10 @@ int crtm_set_stream(crtm_context* ctx, void* stream, int own_stream);
  * kernels (one 64-lane wavefront per layer or profile-channel, stream
  * matrices on the matrix cores), smaller ones on the per-lane kernels (one
  * lane per layer or profile-channel, matrices in registers).  min_angles is
- * 5 (8 and 16 streams on the wavefront kernels), 6 or 9 (the default: only
- * 16 streams).  Results agree to rounding either way.  No counterpart in the
- * reference, whose solver has one serial form (src/RTSolution/ADA/ADA_Module.f90). */
+ * 6 (8 streams with the sensor angle appended, and 16 streams, on the
+ * wavefront kernels) or 9 (the default: only 16 streams).  Results agree to
+ * rounding either way.  No counterpart in the reference, whose solver has one
+ * serial form (src/RTSolution/ADA/ADA_Module.f90). */
 int crtm_set_wavefront_angles(crtm_context* ctx, int min_angles);
 
 /* Wait for the context stream (device-memory calls are asynchronous). */

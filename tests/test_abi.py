@@ -81,3 +81,21 @@ def test_missing_library_fails_loudly(monkeypatch, tmp_path):
     monkeypatch.setattr(api, "_lib", None)
     with pytest.raises(api.CRTMError):
         api.lib()
+
+
+def test_profile_slice_rebases_offsets():
+    """api.profile_slice: a sub-batch of profiles with its own layer offsets."""
+    import numpy as np
+    from crtmv3_amd import api, synthetic
+    ps = synthetic.add_aerosols(synthetic.add_clouds(synthetic.make_profiles(5, [60, 80, 100, 70, 90], seed=1),
+                                                     seed=2), seed=3)
+    sub = api.profile_slice(ps, 1, 4)
+    a, b = int(ps.layer_offset[1]), int(ps.layer_offset[4])
+    np.testing.assert_array_equal(sub.n_layers, [80, 100, 70])
+    np.testing.assert_array_equal(sub.layer_offset, [0, 80, 180, 250])
+    np.testing.assert_array_equal(sub.temperature, ps.temperature[a:b])
+    np.testing.assert_array_equal(sub.absorber, ps.absorber[:, a:b])
+    np.testing.assert_array_equal(sub.cloud_water_content, ps.cloud_water_content[:, a:b])
+    np.testing.assert_array_equal(sub.aerosol_type, ps.aerosol_type[1:4])
+    np.testing.assert_array_equal(sub.level_pressure, ps.level_pressure[a + 1:b + 4])
+    np.testing.assert_array_equal(sub.water_temperature, ps.water_temperature[1:4])

@@ -183,6 +183,18 @@ def test_k_overcast_16_streams(ctx_mw, coeff_dir, oracle_mod, precision):
                                atol=1e-9 if precision == api.PREC_FP64 else 1e-3)
 
 
+def test_k_thick_random_cover_16_streams(ctx_mw, coeff_dir, oracle_mod):
+    """Large snow and rain particles under random cover, 100 layers."""
+    print("allsky K 16 streams thick random cover", flush=True)
+    S, C = _oracle_sensor(coeff_dir, oracle_mod, "atms_syn")
+    ps = synthetic.add_clouds(synthetic.make_profiles(5, 100, seed=281), seed=282, cover="random", reff_scale=25.0)
+    rng = np.random.default_rng(28)
+    eps = rng.uniform(0.45, 0.75, (5, 22))
+    btk = rng.uniform(0.5, 1.5, (5, 22))
+    rts, kd = ctx_mw.k_matrix(ps, eps, bt_k=btk, precision=api.PREC_FP64, n_streams=16)
+    _check_k(ps, rts, kd, _oracle_k(oracle_mod, S, C, ps, eps, btk, 16), api.PREC_FP64, "thick random 16")
+
+
 @pytest.mark.parametrize("overlap", [1, 3])
 def test_k_partial_cover_thick_16_streams(ctx_mw, coeff_dir, oracle_mod, overlap):
     """Partial cover, large particles and heavy water content (several
@@ -219,21 +231,23 @@ def test_k_infrared_aerosols_16_streams(coeff_dir, oracle_mod, precision):
 
 # ------------------------------------------- wavefront kernels at 8 streams
 @pytest.mark.parametrize("precision", [api.PREC_FP64, api.PREC_FP32_SPLIT], ids=["fp64", "split"])
-@pytest.mark.parametrize("n_streams", [8, 0])
-def test_wavefront_kernels_at_8_streams(ctx_mw, coeff_dir, oracle_mod, precision, n_streams):
-    """crtm_set_wavefront_angles(5): the 6-stream (5 angles) and 8-stream (5, 6
-    angles) layers run on the wavefront kernels; forward and K against the
-    oracle, and against the per-lane kernels."""
-    print(f"wavefront kernels n_streams={n_streams} precision={precision}", flush=True)
+@pytest.mark.parametrize("thick", [False, True], ids=["thin", "thick"])
+def test_wavefront_kernels_at_8_streams(ctx_mw, coeff_dir, oracle_mod, precision, thick):
+    """crtm_set_wavefront_angles(6): the 8-stream layers with the sensor angle
+    appended (6 angles) run on the wavefront kernels, a coinciding sensor angle
+    (5 angles) on the per-lane ones; forward and K against the oracle, and
+    against the per-lane kernels alone."""
+    n_streams = 8
+    print(f"wavefront kernels n_streams={n_streams} thick={thick} precision={precision}", flush=True)
     S, C = _oracle_sensor(coeff_dir, oracle_mod, "atms_syn")
     ps = synthetic.add_clouds(synthetic.make_profiles(5, 100, seed=281), seed=282, cover="random",
-                              reff_scale=25.0 if n_streams == 0 else 1.0)
+                              reff_scale=25.0 if thick else 1.0)
     rng = np.random.default_rng(28)
     eps = rng.uniform(0.45, 0.75, (5, 22))
     btk = rng.uniform(0.5, 1.5, (5, 22))
     kw = dict(bt_k=btk, precision=precision, jacobian_f32=bool(precision), n_streams=n_streams, extras=True)
     try:
-        ctx_mw.set_wavefront_angles(5)
+        ctx_mw.set_wavefront_angles(6)
         rts_w, kd_w = ctx_mw.k_matrix(ps, eps, **kw)
         fwd_w = ctx_mw.forward(ps, eps, precision=precision, n_streams=n_streams)
     finally:
@@ -249,5 +263,6 @@ def test_wavefront_kernels_at_8_streams(ctx_mw, coeff_dir, oracle_mod, precision
 
 
 def test_wavefront_angles_argument(ctx_mw):
-    with pytest.raises(api.CRTMError, match="wavefront angle threshold 7"):
-        ctx_mw.set_wavefront_angles(7)
+    for bad in (5, 7):
+        with pytest.raises(api.CRTMError, match=f"wavefront angle threshold {bad}"):
+            ctx_mw.set_wavefront_angles(bad)

@@ -1,0 +1,155 @@
+"""Batched all-sky workload behind torch.autograd: several sensors in one
+context, ragged layer counts (60-137), liquid / ice clouds and dust / sea-salt
+/ sulfate aerosols, Options(m)%n_Streams varying per profile (2, 4, 8, 16 and
+the Mie-parameter choice) in one call.
+
+* per-profile stream counts against the oracle, profile by profile;
+* Context.adjoint (CRTM_Adjoint: channel-summed, in profile chunks) against
+  the channel sums of the full K-matrix;
+* the gradient of J = sum((y - H(x))^2 / sigma^2) through crtm_bt against
+  central finite differences of the GPU forward model.
+
+Tolerances: FP64 BT 1e-6 K against the oracle; adjoint vs K-matrix sums 1e-10
+relative (same kernels, different summation); finite differences 1e-5
+relative for temperature, 1e-4 for water content and aerosol concentration
+(O(h^2) truncation of the central difference).
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from conftest import load_sensor
+from crtmv3_amd import api, coeffio, synthetic
+
+pytestmark = pytest.mark.gpu
+
+SENSORS = ["atms_syn", "hirs_syn"]
+
+
+@pytest.fixture(scope="module")
+def batch(coeff_dir):
+    import torch
+    for s in SENSORS:
+        synthetic.write_sensor(coeff_dir, s)
+    ctx = api.crtm_init(SENSORS, coeff_dir)
+    rng = np.random.default_rng(301)
+    M = 8
+    ks = rng.integers(60, 138, M).tolist()
+    ps = synthetic.add_clouds(synthetic.make_profiles(M, ks, seed=302), seed=303,
+                              types=(synthetic.WATER_CLOUD, synthetic.ICE_CLOUD), cover="random",
+                              median_gm3=0.02, cloudy_fraction=0.75)
+    ps = synthetic.add_aerosols(ps, seed=304, aerosol_fraction=0.6)
+    # two streams only where there is no aerosol (an infrared two-stream
+    # channel has no aerosol phase function: the reference fails the profile)
+    has_aer = [bool(ps.aerosol_concentration[:, int(ps.layer_offset[m]):int(ps.layer_offset[m + 1])].max() > 0)
+               for m in range(M)]
+    choices = [16, 8, 4, 0, 16, 8, 2, 4]
+    ns = np.array([c if (c != 2 or not has_aer[m]) else 4 for m, c in enumerate(choices)], np.int32)
+    L = ctx.n_channels
+    eps = np.concatenate([np.full((M, 22), 0.6), np.full((M, L - 22), 0.97)], axis=1)
+    dev = torch.device("cuda:0")
+    yield ctx, ps, ns, eps, api.device_profiles(ps, dev), dev
+    ctx.destroy()
+
+
+def test_per_profile_streams_match_oracle(batch, coeff_dir, oracle_mod):
+    ctx, ps, ns, eps, dps, dev = batch
+    print(f"batched per-profile streams {ns.tolist()} vs oracle", flush=True)
+    out = ctx.forward(ps, eps, precision=api.PREC_FP64, n_streams=ns)
+    cc = coeffio.read_cloudcoeff(synthetic.write_cloud_coeff(coeff_dir))
+    ac = coeffio.read_aerosolcoeff(synthetic.write_aerosol_coeff(coeff_dir))
+    C = oracle_mod.CloudCoeff(cc, ac)
+    col = 0
+    for s in SENSORS:
+        tau, spc = load_sensor(coeff_dir, s)
+        S = oracle_mod.Sensor(tau, spc)
+        L = S.n_channels
+        for m in range(ps.n_profiles):
+            ref = oracle_mod.run_allsky(S, C, ps, m, eps[m, col:col + L], n_streams=int(ns[m]))
+            err = np.abs(out["brightness_temperature"][m, col:col + L] - ref["bt"]).max()
+            assert err < 1e-6, (s, m, int(ns[m]), err)
+        col += L
+
+
+@pytest.mark.parametrize("precision", [api.PREC_FP64, api.PREC_FP32_SPLIT], ids=["fp64", "split"])
+def test_adjoint_equals_k_matrix_channel_sums(batch, precision):
+    import torch
+    ctx, ps, ns, eps, dps, dev = batch
+    print(f"batched adjoint vs K-matrix sums precision={precision}", flush=True)
+    M, L = ps.n_profiles, ctx.n_channels
+    g = torch.Generator(device="cpu").manual_seed(5)
+    bt_ad = torch.randn(M, L, generator=g, dtype=torch.float64).to(dev)
+    e = torch.as_tensor(eps, device=dev)
+    # a small budget forces several profile chunks
+    bt, ad = ctx.adjoint(dps, e, bt_ad, precision=precision, n_streams=ns, jacobian_f32=False,
+                         max_jacobian_bytes=64 << 20)
+    rts, kd = ctx.k_matrix(dps, e, bt_k=bt_ad, precision=precision, n_streams=ns)
+    torch.cuda.synchronize()
+    np.testing.assert_allclose(bt.cpu().numpy(), rts["brightness_temperature"].cpu().numpy(), rtol=0, atol=1e-9)
+    pairs = [("temperature_ad", "temperature_k"), ("absorber_ad", "absorber_k"),
+             ("cloud_water_content_ad", "cloud_water_content_k"),
+             ("cloud_effective_radius_ad", "cloud_effective_radius_k"), ("cloud_fraction_ad", "cloud_fraction_k"),
+             ("aerosol_concentration_ad", "aerosol_concentration_k"),
+             ("aerosol_effective_radius_ad", "aerosol_effective_radius_k"),
+             ("water_temperature_ad", "water_temperature_k")]
+    for a_key, k_key in pairs:
+        want = kd[k_key].sum(dim=-1).cpu().numpy()
+        got = ad[a_key].cpu().numpy()
+        scale = np.abs(want).max()
+        assert scale > 0, k_key
+        assert np.abs(got - want).max() <= 1e-10 * scale, (a_key, np.abs(got - want).max() / scale)
+    np.testing.assert_array_equal(ad["surface_emissivity_ad"].cpu().numpy(),
+                                  kd["surface_emissivity_k"].cpu().numpy())
+
+
+def test_gradient_of_cost_all_sky(batch):
+    import torch
+    from crtmv3_amd.autograd import crtm_bt
+    ctx, ps, ns, eps, dps, dev = batch
+    print("batched autograd gradient of J vs central differences", flush=True)
+    M, L = ps.n_profiles, ctx.n_channels
+    e = torch.as_tensor(eps, device=dev)
+    kw = dict(precision=api.PREC_FP64, n_streams=ns, max_jacobian_bytes=64 << 20)
+    with torch.no_grad():
+        bt0 = crtm_bt(ctx, dps, dps.temperature, dps.absorber, dps.water_temperature, e, **kw)
+    g = torch.Generator(device="cpu").manual_seed(6)
+    y = bt0 + 0.2 * torch.randn(M, L, generator=g, dtype=torch.float64).to(dev) + 0.5
+
+    def cost(T, A, Ts, E, W, C):
+        bt = crtm_bt(ctx, dps, T, A, Ts, E, cloud_water_content=W, aerosol_concentration=C, **kw)
+        return (((y - bt) / 0.2) ** 2).sum()
+    T = dps.temperature.clone().requires_grad_(True)
+    A = dps.absorber.clone().requires_grad_(True)
+    Ts = dps.water_temperature.clone().requires_grad_(True)
+    E = e.clone().requires_grad_(True)
+    W = dps.cloud_water_content.clone().requires_grad_(True)
+    C = dps.aerosol_concentration.clone().requires_grad_(True)
+    cost(T, A, Ts, E, W, C).backward()
+    torch.cuda.synchronize()
+    base = [t.detach() for t in (T, A, Ts, E, W, C)]
+
+    def fd(which, idx, h):
+        p_, m_ = [b.clone() for b in base], [b.clone() for b in base]
+        p_[which][idx] += h
+        m_[which][idx] -= h
+        with torch.no_grad():
+            return float((cost(*p_) - cost(*m_)) / (2 * h))
+    wc = ps.cloud_water_content
+    ac = ps.aerosol_concentration
+    checks = []
+    rng = np.random.default_rng(7)
+    for i in rng.choice(int(ps.layer_offset[-1]), 3, replace=False):
+        checks.append(("T", 0, int(i), 1e-3, T.grad[i], 1e-5))
+    # the largest water contents and aerosol concentrations (clear of the
+    # thresholds, which are kinks of the forward model)
+    for flat in np.argsort(wc, axis=None)[-3:]:
+        n, i = np.unravel_index(flat, wc.shape)
+        checks.append(("water content", 4, (int(n), int(i)), 1e-3 * wc[n, i], W.grad[n, i], 1e-4))
+    for flat in np.argsort(ac, axis=None)[-3:]:
+        n, i = np.unravel_index(flat, ac.shape)
+        checks.append(("aerosol", 5, (int(n), int(i)), 1e-3 * ac[n, i], C.grad[n, i], 1e-4))
+    assert len(checks) >= 7
+    for name, which, idx, h, got, tol in checks:
+        want = fd(which, idx, float(h))
+        assert abs(float(got) - want) <= tol * max(abs(want), 1e-6), (name, idx, float(got), want)
