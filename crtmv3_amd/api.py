@@ -448,6 +448,73 @@ class Context:
             m0 = m1
         return bt, ad
 
+    def tangent_linear(self, prof, emissivity, tl, precision=PREC_FP64, n_streams=0, overlap=0,
+                       jacobian_f32=None, max_jacobian_bytes=8 << 30):
+        """CRTM_Tangent_Linear (src/CRTM_Tangent_Linear_Module.f90:271-296) for a
+        device batch: brightness-temperature perturbations [M, L] for the
+        state perturbations in `tl`, a dict with any of temperature [NT],
+        absorber [n_absorbers, NT], land/water/snow/ice_temperature [M],
+        surface_emissivity [M, L], cloud_water_content /
+        cloud_effective_radius [n_clouds, NT], cloud_fraction [NT],
+        aerosol_concentration / aerosol_effective_radius [n_aerosols, NT].
+
+        Computed as K-matrix (RTSolution_K = 1) contracted with the
+        perturbations over layers, in profile chunks; returns (bt, bt_tl)."""
+        import torch
+        if not _is_torch(prof.temperature):
+            raise CRTMError("tangent_linear runs on device batches (api.device_profiles)")
+        if jacobian_f32 is None:
+            jacobian_f32 = precision == PREC_FP32_SPLIT
+        M, L = prof.n_profiles, self.n_channels
+        nl = np.asarray(prof.n_layers_host, np.int64)
+        loh = np.concatenate([[0], np.cumsum(nl)])
+        na = int(prof.absorber.shape[0])
+        nc = 0 if prof.cloud_type is None else int(prof.cloud_type.shape[1])
+        nae = 0 if prof.aerosol_type is None else int(prof.aerosol_type.shape[1])
+        per_layer = L * (2 + na + (3 * nc + 1 if nc else 0) + 2 * nae) * (4 if jacobian_f32 else 8)
+        dev = prof.temperature.device
+        bt = torch.empty(M, L, dtype=torch.float64, device=dev)
+        bt_tl = torch.zeros(M, L, dtype=torch.float64, device=dev)
+        per_profile_streams = not (np.ndim(n_streams) == 0 and not _is_torch(n_streams))
+        layer_fields = {"temperature": ("temperature_k", None), "cloud_fraction": ("cloud_fraction_k", None),
+                        "absorber": ("absorber_k", 1), "cloud_water_content": ("cloud_water_content_k", 1),
+                        "cloud_effective_radius": ("cloud_effective_radius_k", 1),
+                        "aerosol_concentration": ("aerosol_concentration_k", 1),
+                        "aerosol_effective_radius": ("aerosol_effective_radius_k", 1)}
+        m0 = 0
+        while m0 < M:
+            m1 = m0 + 1
+            while m1 < M and (loh[m1 + 1] - loh[m0]) * per_layer <= max_jacobian_bytes:
+                m1 += 1
+            a, b = int(loh[m0]), int(loh[m1])
+            sub = profile_slice(prof, m0, m1)
+            ns = n_streams[m0:m1] if per_profile_streams else n_streams
+            rts, kd = self.k_matrix(sub, emissivity[m0:m1].contiguous(), precision=precision,
+                                    jacobian_f32=jacobian_f32, check_input=False, n_streams=ns, overlap=overlap)
+            bt[m0:m1] = rts["brightness_temperature"]
+            acc = bt_tl[m0:m1]
+            # layer fields: profile m's rows [loff[m], loff[m+1]) of the chunk
+            seg = torch.as_tensor(np.repeat(np.arange(m1 - m0), nl[m0:m1]), device=dev)
+            for name, (kname, stacked) in layer_fields.items():
+                x = tl.get(name)
+                if x is None or kname not in kd:
+                    continue
+                k = kd[kname].to(torch.float64)
+                if stacked is None:
+                    contrib = k * x[a:b, None]                         # [NTc, L]
+                else:
+                    contrib = (k * x[:, a:b, None]).sum(dim=0)         # [NTc, L]
+                acc.index_add_(0, seg, contrib)
+            for f in ("land", "water", "snow", "ice"):
+                x = tl.get(f + "_temperature")
+                if x is not None:
+                    acc += kd[f + "_temperature_k"] * x[m0:m1, None]
+            if tl.get("surface_emissivity") is not None:
+                acc += kd["surface_emissivity_k"] * tl["surface_emissivity"][m0:m1]
+            del rts, kd
+            m0 = m1
+        return bt, bt_tl
+
     def profile_enable(self, on=True):
         if lib().crtm_profile_enable(self._h, int(on)) != SUCCESS:
             self._err("crtm_profile_enable")

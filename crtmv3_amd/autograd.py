@@ -39,21 +39,33 @@ _ADJOINT = ("temperature_ad", "absorber_ad", "water_temperature_ad", "cloud_wate
 class CRTMBrightnessTemperature(torch.autograd.Function):
     @staticmethod
     def forward(ctx, crtm_ctx, profiles, options, emissivity, *state):
-        precision, n_streams, overlap, max_bytes = options
+        precision, n_streams, overlap, max_bytes, chunk = options
         prof = _with_state(profiles, state)
-        rts = crtm_ctx.forward(prof, emissivity.detach().to(torch.float64).contiguous(), precision=precision,
-                               n_streams=n_streams, overlap=overlap)
+        e = emissivity.detach().to(torch.float64).contiguous()
+        M = prof.n_profiles
+        if not chunk or chunk >= M:
+            bt = crtm_ctx.forward(prof, e, precision=precision, n_streams=n_streams,
+                                  overlap=overlap)["brightness_temperature"]
+        else:   # profile chunks bound the scattering work space of one call
+            bt = torch.empty(M, crtm_ctx.n_channels, dtype=torch.float64, device=e.device)
+            per_profile = not (isinstance(n_streams, int) or getattr(n_streams, "ndim", 1) == 0)
+            for m0 in range(0, M, chunk):
+                m1 = min(M, m0 + chunk)
+                ns = n_streams[m0:m1] if per_profile else n_streams
+                bt[m0:m1] = crtm_ctx.forward(api.profile_slice(prof, m0, m1), e[m0:m1].contiguous(),
+                                             precision=precision, n_streams=ns,
+                                             overlap=overlap)["brightness_temperature"]
         ctx.save_for_backward(emissivity, *[s for s in state if s is not None])
         ctx.present = tuple(s is not None for s in state)
         ctx.crtm = (crtm_ctx, profiles, options)
-        return rts["brightness_temperature"]
+        return bt
 
     @staticmethod
     def backward(ctx, grad_bt):
         saved = list(ctx.saved_tensors)
         emissivity = saved.pop(0)
         state = [saved.pop(0) if p else None for p in ctx.present]
-        crtm_ctx, profiles, (precision, n_streams, overlap, max_bytes) = ctx.crtm
+        crtm_ctx, profiles, (precision, n_streams, overlap, max_bytes, chunk) = ctx.crtm
         prof = _with_state(profiles, state)
         seed = grad_bt.detach().to(torch.float64).contiguous()
         _, ad = crtm_ctx.adjoint(prof, emissivity.detach().to(torch.float64).contiguous(), seed,
@@ -73,11 +85,14 @@ def _with_state(profiles, state):
 def crtm_bt(crtm_ctx, profiles, temperature, absorber, water_temperature, emissivity,
             precision=api.PREC_FP32_SPLIT, cloud_water_content=None, cloud_effective_radius=None,
             aerosol_concentration=None, aerosol_effective_radius=None, n_streams=0, overlap=0,
-            max_jacobian_bytes=8 << 30):
-    """Differentiable CRTM brightness temperatures [M, L]."""
+            max_jacobian_bytes=8 << 30, forward_chunk=0):
+    """Differentiable CRTM brightness temperatures [M, L].  forward_chunk > 0
+    runs the forward model in chunks of that many profiles (the backward is
+    chunked by max_jacobian_bytes)."""
     if not temperature.is_cuda:
         raise api.CRTMError("crtm_bt runs on the GPU: pass device tensors (api.device_profiles)")
     state = (temperature, absorber, water_temperature, cloud_water_content, cloud_effective_radius,
              aerosol_concentration, aerosol_effective_radius)
-    return CRTMBrightnessTemperature.apply(crtm_ctx, profiles, (precision, n_streams, overlap, max_jacobian_bytes),
+    return CRTMBrightnessTemperature.apply(crtm_ctx, profiles,
+                                           (precision, n_streams, overlap, max_jacobian_bytes, forward_chunk),
                                            emissivity, *state)

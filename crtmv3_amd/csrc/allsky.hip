@@ -572,11 +572,14 @@ __global__ void aerosol_check_kernel(DevSensor s, int M, DevClouds cl, CloudWork
   for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
     const int m = (int)(t / s.L), l = (int)(t - (int64_t)m * s.L);
     const int nl = legendre_terms(profile_streams(cl, m), cw.max_reff[m], s.wavenumber[l]);
-    bool bad = aerosol_offset(nl) < 0;
-    for (int q = 0; q < cl.na && !bad; ++q) {
+    // the profile's n_Aerosols counts its typed slots (type 0: unused slot)
+    bool typed = false, bad = false;
+    for (int q = 0; q < cl.na; ++q) {
       const int ty = cl.atype[(int64_t)m * cl.na + q];
+      typed |= ty != 0;
       if (ty != 0 && aerosol_index(ac, ty) < 0) bad = true;
     }
+    if (typed && aerosol_offset(nl) < 0) bad = true;
     if (bad) cw.aer_err[m] = 1;
   }
 }

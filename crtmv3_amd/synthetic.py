@@ -696,15 +696,19 @@ def add_aerosols(ps: ProfileSet, seed: int = 20261205,
                  aerosol_fraction: float = 1.0) -> ProfileSet:
     """Copy of ps with aerosols: each type log-normal in mass concentration
     (median_kgm2 * exp(N(0, sigma)) kg/m^2 per layer) in the lowest n_layers
-    layers, effective radius uniform in the type's range (microns)."""
+    layers, effective radius uniform in the type's range (microns).  A
+    fraction aerosol_fraction of the profiles carries them; the others get
+    unused slots (type 0: no aerosols, n_Aerosols = 0)."""
     rng = np.random.default_rng(seed)
     M, na = ps.n_profiles, len(types)
     ntot = int(ps.layer_offset[-1])
     conc = np.zeros((na, ntot))
     re_ = np.zeros((na, ntot))
+    atype = np.tile(np.asarray(types, np.int32), (M, 1))
     for m in range(M):
         a, b = int(ps.layer_offset[m]), int(ps.layer_offset[m + 1])
         if rng.uniform() >= aerosol_fraction:
+            atype[m] = 0
             continue
         k0 = max(a, b - n_layers)
         for n, t in enumerate(types):
@@ -712,7 +716,7 @@ def add_aerosols(ps: ProfileSet, seed: int = 20261205,
             conc[n, k0:b] = median_kgm2 * np.exp(sigma * rng.standard_normal(b - k0))
             re_[n, k0:b] = rng.uniform(lo, hi, b - k0)
     out = dataclasses.replace(ps)
-    out.aerosol_type = np.tile(np.asarray(types, np.int32), (M, 1))
+    out.aerosol_type = atype
     out.aerosol_concentration = conc
     out.aerosol_effective_radius = re_
     return out

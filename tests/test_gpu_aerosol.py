@@ -121,3 +121,26 @@ def test_aerosol_failures(ir):
         ctx.forward(bad, eps, precision=api.PREC_FP64, n_streams=4)
     out = ctx.forward(ps, eps, precision=api.PREC_FP64, n_streams=4)   # the context is still usable
     assert np.isfinite(out["brightness_temperature"]).all()
+
+
+def test_empty_aerosol_slots_two_streams(coeff_dir, oracle_mod):
+    """Profiles whose aerosol slots are all unused (type 0) run at two
+    streams next to profiles with aerosols at four, against the oracle."""
+    print("aerosol empty slots, per-profile streams 2 / 4", flush=True)
+    tau, spc = load_sensor(coeff_dir, "hirs_syn")
+    cc = coeffio.read_cloudcoeff(synthetic.write_cloud_coeff(coeff_dir))
+    ac = coeffio.read_aerosolcoeff(synthetic.write_aerosol_coeff(coeff_dir))
+    S, C = oracle_mod.Sensor(tau, spc), oracle_mod.CloudCoeff(cc, ac)
+    ctx = api.crtm_init(["hirs_syn"], coeff_dir)
+    try:
+        ps = synthetic.add_clouds(synthetic.make_profiles(6, 80, seed=241), seed=242, median_gm3=0.005)
+        ps = synthetic.add_aerosols(ps, seed=243, aerosol_fraction=0.5)
+        ns = np.where((ps.aerosol_type != 0).any(axis=1), 4, 2).astype(np.int32)
+        assert (ns == 2).any() and (ns == 4).any()
+        eps = np.full((6, 19), 0.97)
+        out = ctx.forward(ps, eps, precision=api.PREC_FP64, n_streams=ns)
+        for m in range(6):
+            ref = oracle_mod.run_allsky(S, C, ps, m, eps[m], n_streams=int(ns[m]))
+            assert np.abs(out["brightness_temperature"][m] - ref["bt"]).max() < 1e-6, m
+    finally:
+        ctx.destroy()

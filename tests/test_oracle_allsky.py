@@ -711,3 +711,20 @@ def test_aerosols_skip_microwave_and_need_four_streams(atms, hirs_aer, oracle_mo
     Sh, Ch = hirs_aer
     with pytest.raises(RuntimeError, match="-5"):
         oracle_mod.run_allsky(Sh, Ch, synthetic.add_aerosols(ps, seed=223), 0, np.full(Sh.n_channels, 0.97), 2)
+
+
+def test_empty_aerosol_slots_are_no_aerosols(hirs_aer, oracle_mod):
+    """Aerosol type 0 marks an unused slot: a profile whose slots are all
+    unused has n_Aerosols = 0, so two streams are allowed (the failure of
+    CRTM_AerosolScatter.f90:208-225 needs an aerosol) and the result equals
+    the aerosol-free one."""
+    import dataclasses
+    Sh, Ch = hirs_aer
+    ps = synthetic.make_profiles(1, 60, seed=231)
+    ps = synthetic.add_clouds(ps, seed=232, median_gm3=0.005)
+    pa = synthetic.add_aerosols(ps, seed=233, aerosol_fraction=0.0)
+    assert (pa.aerosol_type == 0).all()
+    eps = np.full(Sh.n_channels, 0.97)
+    for ns in (2, 4):
+        np.testing.assert_array_equal(oracle_mod.run_allsky(Sh, Ch, pa, 0, eps, ns)["bt"],
+                                      oracle_mod.run_allsky(Sh, Ch, ps, 0, eps, ns)["bt"])
