@@ -604,6 +604,9 @@ def device_profiles(prof, device="cuda"):
     d = {}
     for f in dataclasses.fields(prof):
         v = getattr(prof, f.name)
+        if f.name in ("absorber_id", "absorber_units"):   # read by the host side of the call
+            d[f.name] = None if v is None else np.ascontiguousarray(v)
+            continue
         d[f.name] = None if v is None else torch.as_tensor(np.ascontiguousarray(v)).to(device)
     out = dataclasses.replace(prof, **d)
     # host copy of the layer counts: lets device calls size their work without a sync

@@ -1107,6 +1107,14 @@ int do_call(crtm_context* ctx, const crtm_atmosphere* atm, const crtm_surface* s
   if (atm->n_absorbers < 1 || atm->n_absorbers > NA_MAX)
     return fail(ctx, fmt("n_absorbers %d outside 1..%d", atm->n_absorbers, NA_MAX));
   if (!rts->radiance || !rts->brightness_temperature) return fail(ctx, "RTSolution radiance/BT arrays required");
+  // under HIP graph capture only the clear-sky device-memory call is valid:
+  // host staging and the cloud-layer count each synchronise with the host
+  hipStreamCaptureStatus capture = hipStreamCaptureStatusNone;
+  HIPCHECK(hipStreamIsCapturing(ctx->stream, &capture));
+  if (capture != hipStreamCaptureStatusNone &&
+      (opt->memory == CRTM_MEM_HOST || atm->n_clouds > 0 || atm->n_aerosols > 0))
+    return fail(ctx, "graph capture: only device-memory clear-sky calls can be captured (host staging and the "
+                     "cloud-layer count synchronise with the host)");
   CloudCall cc;
   cc.on = atm->n_clouds > 0 || atm->n_aerosols > 0;
   if (cc.on) {
