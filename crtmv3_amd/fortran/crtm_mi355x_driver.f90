@@ -1,8 +1,11 @@
 ! crtm_mi355x_driver.f90 -- Fortran caller of CRTM_MI355X_Module written the
 ! way a CRTM application calls CRTM_Module: CRTM_Init, CRTM_Forward,
-! CRTM_K_Matrix with zeroed Atmosphere_K / Surface_K, CRTM_Destroy.
+! CRTM_K_Matrix with zeroed Atmosphere_K / Surface_K, CRTM_Adjoint with
+! zeroed Atmosphere_AD / Surface_AD, CRTM_Tangent_Linear, CRTM_Destroy.
 ! Reads a batch from a text file and writes BT, T_K, absorber_K, water
-! temperature K and emissivity K as text (used by the Fortran tests).
+! temperature K and emissivity K as text, and the adjoint (seeded with
+! BT_AD(l) = 0.5 + l/L) and tangent linear (T_TL = 1 K, absorber_TL = 1 %,
+! water temperature_TL = 0.5 K) to <output.txt>.ad (used by the Fortran tests).
 !
 !   crtm_mi355x_driver <input.txt> <output.txt>
 PROGRAM crtm_mi355x_driver
@@ -16,6 +19,9 @@ PROGRAM crtm_mi355x_driver
   TYPE(CRTM_Geometry_type), ALLOCATABLE :: geo(:)
   TYPE(CRTM_RTSolution_type), ALLOCATABLE :: rts(:, :), rts_k(:, :), rts_f(:, :)
   TYPE(CRTM_Options_type), ALLOCATABLE :: opt(:)
+  TYPE(CRTM_Atmosphere_type), ALLOCATABLE :: atm_ad(:, :), atm_tl(:, :)
+  TYPE(CRTM_Surface_type), ALLOCATABLE :: sfc_ad(:, :), sfc_tl(:)
+  TYPE(CRTM_RTSolution_type), ALLOCATABLE :: rts_ad(:, :), rts_tl(:, :)
   INTEGER :: M, na, prec, m1, K, L, j, l1, k1, err, u
   REAL(fp) :: zen, ts
 
@@ -61,6 +67,38 @@ PROGRAM crtm_mi355x_driver
   rts_k(:, :)%Radiance = 0.0_fp
   err = CRTM_K_Matrix(atm, sfc, rts_k, geo, chinfo, atm_k, sfc_k, rts, opt)
   IF (err /= SUCCESS) STOP 'CRTM_K_Matrix failed'
+
+  ! adjoint and tangent linear of the same batch
+  ALLOCATE(atm_ad(1, M), atm_tl(1, M), sfc_ad(1, M), sfc_tl(M), rts_ad(L, M), rts_tl(L, M))
+  DO m1 = 1, M
+    K = atm(m1)%n_Layers
+    CALL CRTM_Atmosphere_Create(atm_ad(1:1, m1), K, na, 0, 0)
+    CALL CRTM_Atmosphere_Create(atm_tl(1:1, m1), K, na, 0, 0)
+    atm_tl(1, m1)%Temperature = 1.0_fp
+    atm_tl(1, m1)%Absorber = 0.01_fp * atm(m1)%Absorber
+    sfc_tl(m1)%Water_Temperature = 0.5_fp
+    DO l1 = 1, L
+      rts_ad(l1, m1)%Brightness_Temperature = 0.5_fp + REAL(l1, fp) / REAL(L, fp)
+      rts_ad(l1, m1)%Radiance = 0.0_fp
+    END DO
+  END DO
+  CALL CRTM_Atmosphere_Zero(atm_ad)
+  CALL CRTM_Surface_Zero(sfc_ad)
+  err = CRTM_Adjoint(atm, sfc, rts_ad, geo, chinfo, atm_ad(1, :), sfc_ad(1, :), rts, opt)
+  IF (err /= SUCCESS) STOP 'CRTM_Adjoint failed'
+  err = CRTM_Tangent_Linear(atm, sfc, atm_tl(1, :), sfc_tl, geo, chinfo, rts, rts_tl, opt)
+  IF (err /= SUCCESS) STOP 'CRTM_Tangent_Linear failed'
+  OPEN(NEWUNIT=u, FILE=TRIM(fout)//'.ad', STATUS='REPLACE', ACTION='WRITE')
+  DO m1 = 1, M
+    K = atm(m1)%n_Layers
+    WRITE(u, '(*(ES25.16E3))') atm_ad(1, m1)%Temperature(1:K)
+    DO j = 1, na
+      WRITE(u, '(*(ES25.16E3))') atm_ad(1, m1)%Absorber(1:K, j)
+    END DO
+    WRITE(u, '(*(ES25.16E3))') sfc_ad(1, m1)%Water_Temperature
+    WRITE(u, '(*(ES25.16E3))') (rts_tl(l1, m1)%Brightness_Temperature, l1 = 1, L)
+  END DO
+  CLOSE(u)
 
   OPEN(NEWUNIT=u, FILE=TRIM(fout), STATUS='REPLACE', ACTION='WRITE')
   WRITE(u, '(I8, I8)') M, L

@@ -95,7 +95,7 @@ MODULE CRTM_MI355X_Module
     INTEGER :: Precision = CRTM_PREC_FP64          ! extension: arithmetic of the GPU path
   END TYPE
 
-  PUBLIC :: CRTM_Init, CRTM_Destroy, CRTM_Forward, CRTM_K_Matrix
+  PUBLIC :: CRTM_Init, CRTM_Destroy, CRTM_Forward, CRTM_K_Matrix, CRTM_Adjoint, CRTM_Tangent_Linear
   PUBLIC :: CRTM_Atmosphere_Create, CRTM_Atmosphere_Zero, CRTM_Surface_Zero
 
   ! ---- C-ABI mirrors (include/crtm_mi355x.h) ----
@@ -201,6 +201,14 @@ MODULE CRTM_MI355X_Module
     TYPE(c_geometry) :: geo
     TYPE(c_options) :: opt
     TYPE(c_rtsolution) :: rts
+  END TYPE
+
+  ! K-matrix outputs of one profile chunk, flat in the C layout
+  TYPE :: k_buffers
+    TYPE(batch_type) :: b
+    INTEGER :: na = 0, nc = 0, nae = 0
+    REAL(C_DOUBLE), ALLOCATABLE :: btk(:), radk(:), emk(:), tk(:), ak(:), stk(:, :)
+    REAL(C_DOUBLE), ALLOCATABLE :: cwk(:), crk(:), cfk(:), ack(:), ark(:)
   END TYPE
 
 CONTAINS
@@ -443,6 +451,230 @@ CONTAINS
       END DO
     END DO
   END FUNCTION CRTM_K_Matrix
+
+  ! CRTM_Adjoint (src/CRTM_Adjoint_Module.f90:299-325): the channel-summed
+  ! adjoint for RTSolution_AD%Brightness_Temperature / %Radiance, i.e. the
+  ! K-matrix seeded with RTSolution_AD and summed over channels.  Runs the
+  ! K-matrix on profile chunks (at most K_CHUNK_BYTES of per-channel
+  ! Jacobians at a time).  Atmosphere_AD and Surface_AD are accumulated into;
+  ! the emissivity adjoint is per channel, in RTSolution_AD%Surface_Emissivity.
+  FUNCTION CRTM_Adjoint(Atmosphere, Surface, RTSolution_AD, Geometry, ChannelInfo, Atmosphere_AD, Surface_AD, &
+                        RTSolution, Options) RESULT(Error_Status)
+    TYPE(CRTM_Atmosphere_type), INTENT(IN) :: Atmosphere(:)
+    TYPE(CRTM_Surface_type), INTENT(IN) :: Surface(:)
+    TYPE(CRTM_RTSolution_type), INTENT(INOUT) :: RTSolution_AD(:, :)
+    TYPE(CRTM_Geometry_type), INTENT(IN) :: Geometry(:)
+    TYPE(CRTM_ChannelInfo_type), INTENT(IN) :: ChannelInfo(:)
+    TYPE(CRTM_Atmosphere_type), INTENT(INOUT) :: Atmosphere_AD(:)
+    TYPE(CRTM_Surface_type), INTENT(INOUT) :: Surface_AD(:)
+    TYPE(CRTM_RTSolution_type), INTENT(INOUT) :: RTSolution(:, :)
+    TYPE(CRTM_Options_type), OPTIONAL, INTENT(IN) :: Options(:)
+    INTEGER :: Error_Status
+    TYPE(k_buffers) :: kb
+    INTEGER :: M, L, m0, m1, mm, ic, kk, j, K0, NT
+    IF (.NOT. PRESENT(Options)) THEN
+      Error_Status = FAILURE
+      RETURN
+    END IF
+    M = SIZE(Atmosphere)
+    m0 = 1
+    DO WHILE (m0 <= M)
+      m1 = chunk_end(Atmosphere, ChannelInfo, m0)
+      Error_Status = run_k_chunk(Atmosphere(m0:m1), Surface(m0:m1), Geometry(m0:m1), ChannelInfo, &
+                                 RTSolution(:, m0:m1), Options, m0, m1, RTSolution_AD(:, m0:m1), kb, L, NT)
+      IF (Error_Status /= SUCCESS) RETURN
+      DO mm = m0, m1
+        K0 = INT(kb%b%off(mm - m0 + 1))
+        DO ic = 1, L
+          RTSolution_AD(ic, mm)%Surface_Emissivity = RTSolution_AD(ic, mm)%Surface_Emissivity + &
+              kb%emk((mm - m0) * L + ic)
+        END DO
+        Surface_AD(mm)%Land_Temperature = Surface_AD(mm)%Land_Temperature + SUM(kb%stk((mm - m0) * L + 1:(mm - m0 + 1) * L, 1))
+        Surface_AD(mm)%Water_Temperature = Surface_AD(mm)%Water_Temperature + &
+            SUM(kb%stk((mm - m0) * L + 1:(mm - m0 + 1) * L, 2))
+        Surface_AD(mm)%Snow_Temperature = Surface_AD(mm)%Snow_Temperature + SUM(kb%stk((mm - m0) * L + 1:(mm - m0 + 1) * L, 3))
+        Surface_AD(mm)%Ice_Temperature = Surface_AD(mm)%Ice_Temperature + SUM(kb%stk((mm - m0) * L + 1:(mm - m0 + 1) * L, 4))
+        DO kk = 1, Atmosphere(mm)%n_Layers
+          Atmosphere_AD(mm)%Temperature(kk) = Atmosphere_AD(mm)%Temperature(kk) + &
+              SUM(kb%tk((K0 + kk - 1) * L + 1:(K0 + kk) * L))
+          DO j = 1, kb%na
+            Atmosphere_AD(mm)%Absorber(kk, j) = Atmosphere_AD(mm)%Absorber(kk, j) + &
+                SUM(kb%ak(((j - 1) * NT + K0 + kk - 1) * L + 1:((j - 1) * NT + K0 + kk) * L))
+          END DO
+          IF (kb%nc > 0 .AND. ALLOCATED(Atmosphere_AD(mm)%Cloud)) THEN
+            DO j = 1, kb%nc
+              Atmosphere_AD(mm)%Cloud(j)%Water_Content(kk) = Atmosphere_AD(mm)%Cloud(j)%Water_Content(kk) + &
+                  SUM(kb%cwk(((j - 1) * NT + K0 + kk - 1) * L + 1:((j - 1) * NT + K0 + kk) * L))
+              Atmosphere_AD(mm)%Cloud(j)%Effective_Radius(kk) = Atmosphere_AD(mm)%Cloud(j)%Effective_Radius(kk) + &
+                  SUM(kb%crk(((j - 1) * NT + K0 + kk - 1) * L + 1:((j - 1) * NT + K0 + kk) * L))
+            END DO
+            Atmosphere_AD(mm)%Cloud_Fraction(kk) = Atmosphere_AD(mm)%Cloud_Fraction(kk) + &
+                SUM(kb%cfk((K0 + kk - 1) * L + 1:(K0 + kk) * L))
+          END IF
+          IF (kb%nae > 0 .AND. ALLOCATED(Atmosphere_AD(mm)%Aerosol)) THEN
+            DO j = 1, kb%nae
+              Atmosphere_AD(mm)%Aerosol(j)%Concentration(kk) = Atmosphere_AD(mm)%Aerosol(j)%Concentration(kk) + &
+                  SUM(kb%ack(((j - 1) * NT + K0 + kk - 1) * L + 1:((j - 1) * NT + K0 + kk) * L))
+              Atmosphere_AD(mm)%Aerosol(j)%Effective_Radius(kk) = Atmosphere_AD(mm)%Aerosol(j)%Effective_Radius(kk) + &
+                  SUM(kb%ark(((j - 1) * NT + K0 + kk - 1) * L + 1:((j - 1) * NT + K0 + kk) * L))
+            END DO
+          END IF
+        END DO
+      END DO
+      m0 = m1 + 1
+    END DO
+  END FUNCTION CRTM_Adjoint
+
+  ! CRTM_Tangent_Linear (src/CRTM_Tangent_Linear_Module.f90:271-296):
+  ! RTSolution_TL%Brightness_Temperature = K . (Atmosphere_TL, Surface_TL),
+  ! with the K-matrix (BT seeded with 1) run on profile chunks.  Radiance_TL
+  ! is not formed (the brightness temperature is this path's observable).
+  FUNCTION CRTM_Tangent_Linear(Atmosphere, Surface, Atmosphere_TL, Surface_TL, Geometry, ChannelInfo, RTSolution, &
+                               RTSolution_TL, Options) RESULT(Error_Status)
+    TYPE(CRTM_Atmosphere_type), INTENT(IN) :: Atmosphere(:)
+    TYPE(CRTM_Surface_type), INTENT(IN) :: Surface(:)
+    TYPE(CRTM_Atmosphere_type), INTENT(IN) :: Atmosphere_TL(:)
+    TYPE(CRTM_Surface_type), INTENT(IN) :: Surface_TL(:)
+    TYPE(CRTM_Geometry_type), INTENT(IN) :: Geometry(:)
+    TYPE(CRTM_ChannelInfo_type), INTENT(IN) :: ChannelInfo(:)
+    TYPE(CRTM_RTSolution_type), INTENT(INOUT) :: RTSolution(:, :)
+    TYPE(CRTM_RTSolution_type), INTENT(INOUT) :: RTSolution_TL(:, :)
+    TYPE(CRTM_Options_type), OPTIONAL, INTENT(IN) :: Options(:)
+    INTEGER :: Error_Status
+    TYPE(k_buffers) :: kb
+    TYPE(CRTM_RTSolution_type), ALLOCATABLE :: seed(:, :)
+    INTEGER :: M, L, m0, m1, mm, ic, kk, j, K0, NT, nl
+    REAL(fp) :: s
+    IF (.NOT. PRESENT(Options)) THEN
+      Error_Status = FAILURE
+      RETURN
+    END IF
+    M = SIZE(Atmosphere)
+    ALLOCATE(seed(SIZE(RTSolution, 1), M))
+    seed(:, :)%Brightness_Temperature = 1.0_fp
+    seed(:, :)%Radiance = 0.0_fp
+    m0 = 1
+    DO WHILE (m0 <= M)
+      m1 = chunk_end(Atmosphere, ChannelInfo, m0)
+      Error_Status = run_k_chunk(Atmosphere(m0:m1), Surface(m0:m1), Geometry(m0:m1), ChannelInfo, &
+                                 RTSolution(:, m0:m1), Options, m0, m1, seed(:, m0:m1), kb, L, NT)
+      IF (Error_Status /= SUCCESS) RETURN
+      DO mm = m0, m1
+        K0 = INT(kb%b%off(mm - m0 + 1))
+        nl = Atmosphere(mm)%n_Layers
+        DO ic = 1, L
+          s = kb%stk((mm - m0) * L + ic, 1) * Surface_TL(mm)%Land_Temperature + &
+              kb%stk((mm - m0) * L + ic, 2) * Surface_TL(mm)%Water_Temperature + &
+              kb%stk((mm - m0) * L + ic, 3) * Surface_TL(mm)%Snow_Temperature + &
+              kb%stk((mm - m0) * L + ic, 4) * Surface_TL(mm)%Ice_Temperature
+          DO kk = 1, nl
+            s = s + kb%tk((K0 + kk - 1) * L + ic) * Atmosphere_TL(mm)%Temperature(kk)
+            DO j = 1, kb%na
+              s = s + kb%ak(((j - 1) * NT + K0 + kk - 1) * L + ic) * Atmosphere_TL(mm)%Absorber(kk, j)
+            END DO
+            IF (kb%nc > 0 .AND. ALLOCATED(Atmosphere_TL(mm)%Cloud)) THEN
+              DO j = 1, kb%nc
+                s = s + kb%cwk(((j - 1) * NT + K0 + kk - 1) * L + ic) * Atmosphere_TL(mm)%Cloud(j)%Water_Content(kk) &
+                      + kb%crk(((j - 1) * NT + K0 + kk - 1) * L + ic) * Atmosphere_TL(mm)%Cloud(j)%Effective_Radius(kk)
+              END DO
+              IF (ALLOCATED(Atmosphere_TL(mm)%Cloud_Fraction)) &
+                s = s + kb%cfk((K0 + kk - 1) * L + ic) * Atmosphere_TL(mm)%Cloud_Fraction(kk)
+            END IF
+            IF (kb%nae > 0 .AND. ALLOCATED(Atmosphere_TL(mm)%Aerosol)) THEN
+              DO j = 1, kb%nae
+                s = s + kb%ack(((j - 1) * NT + K0 + kk - 1) * L + ic) * Atmosphere_TL(mm)%Aerosol(j)%Concentration(kk) &
+                      + kb%ark(((j - 1) * NT + K0 + kk - 1) * L + ic) * Atmosphere_TL(mm)%Aerosol(j)%Effective_Radius(kk)
+              END DO
+            END IF
+          END DO
+          RTSolution_TL(ic, mm)%Brightness_Temperature = s
+        END DO
+      END DO
+      m0 = m1 + 1
+    END DO
+  END FUNCTION CRTM_Tangent_Linear
+
+  ! last profile of the chunk starting at m0: at most K_CHUNK_BYTES of
+  ! per-channel Jacobians (and at least one profile)
+  FUNCTION chunk_end(Atm, ChannelInfo, m0) RESULT(m1)
+    TYPE(CRTM_Atmosphere_type), INTENT(IN) :: Atm(:)
+    TYPE(CRTM_ChannelInfo_type), INTENT(IN) :: ChannelInfo(:)
+    INTEGER, INTENT(IN) :: m0
+    INTEGER :: m1, L, fields
+    INTEGER(C_INT64_T) :: bytes
+    INTEGER(C_INT64_T), PARAMETER :: K_CHUNK_BYTES = 2_C_INT64_T * 1024 * 1024 * 1024
+    L = SUM(ChannelInfo(:)%n_Channels)
+    fields = 1 + Atm(m0)%n_Absorbers + 3 * Atm(m0)%n_Clouds + 2 * Atm(m0)%n_Aerosols
+    m1 = m0
+    bytes = INT(Atm(m0)%n_Layers, C_INT64_T) * L * fields * 8
+    DO WHILE (m1 < SIZE(Atm))
+      bytes = bytes + INT(Atm(m1 + 1)%n_Layers, C_INT64_T) * L * fields * 8
+      IF (bytes > K_CHUNK_BYTES) EXIT
+      m1 = m1 + 1
+    END DO
+  END FUNCTION chunk_end
+
+  ! K-matrix of profiles m0..m1 (their sections passed in) with RTSolution_K
+  ! from Seed, into the flat buffers of kb; forward results into RTS
+  FUNCTION run_k_chunk(Atm, Sfc, Geo, ChannelInfo, RTS, Opt, m0, m1, Seed, kb, L, NT) RESULT(Error_Status)
+    TYPE(CRTM_Atmosphere_type), INTENT(IN) :: Atm(:)
+    TYPE(CRTM_Surface_type), INTENT(IN) :: Sfc(:)
+    TYPE(CRTM_Geometry_type), INTENT(IN) :: Geo(:)
+    TYPE(CRTM_ChannelInfo_type), INTENT(IN) :: ChannelInfo(:)
+    TYPE(CRTM_RTSolution_type), INTENT(INOUT) :: RTS(:, :)
+    TYPE(CRTM_Options_type), INTENT(IN) :: Opt(:)
+    INTEGER, INTENT(IN) :: m0, m1
+    TYPE(CRTM_RTSolution_type), INTENT(IN) :: Seed(:, :)
+    TYPE(k_buffers), TARGET, INTENT(INOUT) :: kb
+    INTEGER, INTENT(OUT) :: L, NT
+    INTEGER :: Error_Status, M, mm, ic
+    TYPE(c_k_output) :: k
+    IF (SIZE(Opt) >= m1) THEN
+      Error_Status = pack_batch(Atm, Sfc, Geo, ChannelInfo, RTS, Opt(m0:m1), kb%b, L, M)
+    ELSE
+      Error_Status = pack_batch(Atm, Sfc, Geo, ChannelInfo, RTS, Opt(1:1), kb%b, L, M)
+    END IF
+    IF (Error_Status /= SUCCESS) RETURN
+    NT = INT(kb%b%off(M + 1))
+    kb%na = Atm(1)%n_Absorbers
+    kb%nc = Atm(1)%n_Clouds
+    kb%nae = Atm(1)%n_Aerosols
+    IF (ALLOCATED(kb%btk)) DEALLOCATE(kb%btk, kb%radk, kb%emk, kb%tk, kb%ak, kb%stk, kb%cwk, kb%crk, kb%cfk, &
+                                      kb%ack, kb%ark)
+    ALLOCATE(kb%btk(L * M), kb%radk(L * M), kb%emk(L * M), kb%tk(NT * L), kb%ak(kb%na * NT * L), kb%stk(L * M, 4))
+    ALLOCATE(kb%cwk(MAX(kb%nc, 1) * NT * L), kb%crk(MAX(kb%nc, 1) * NT * L), kb%cfk(NT * L))
+    ALLOCATE(kb%ack(MAX(kb%nae, 1) * NT * L), kb%ark(MAX(kb%nae, 1) * NT * L))
+    DO mm = 1, M
+      DO ic = 1, L
+        kb%btk((mm - 1) * L + ic) = Seed(ic, mm)%Brightness_Temperature
+        kb%radk((mm - 1) * L + ic) = Seed(ic, mm)%Radiance
+      END DO
+    END DO
+    k%brightness_temperature_k = C_LOC(kb%btk)
+    k%radiance_k = C_LOC(kb%radk)
+    k%surface_emissivity_k = C_LOC(kb%emk)
+    k%temperature_k = C_LOC(kb%tk)
+    k%absorber_k = C_LOC(kb%ak)
+    k%land_temperature_k = C_LOC(kb%stk(1, 1))
+    k%water_temperature_k = C_LOC(kb%stk(1, 2))
+    k%snow_temperature_k = C_LOC(kb%stk(1, 3))
+    k%ice_temperature_k = C_LOC(kb%stk(1, 4))
+    k%cloud_water_content_k = C_LOC(kb%cwk)
+    k%cloud_effective_radius_k = C_LOC(kb%crk)
+    k%cloud_fraction_k = C_LOC(kb%cfk)
+    k%aerosol_concentration_k = C_LOC(kb%ack)
+    k%aerosol_effective_radius_k = C_LOC(kb%ark)
+    Error_Status = c_crtm_k_matrix(ChannelInfo(1)%ctx, kb%b%atm, kb%b%sfc, kb%b%geo, kb%b%opt, kb%b%rts, k)
+    IF (Error_Status /= SUCCESS) THEN
+      CALL report(ChannelInfo(1)%ctx, 'CRTM_Adjoint / CRTM_Tangent_Linear')
+      RETURN
+    END IF
+    DO mm = 1, M
+      DO ic = 1, L
+        CALL unpack_rts(kb%b, (mm - 1) * L + ic, RTS(ic, mm))
+      END DO
+    END DO
+  END FUNCTION run_k_chunk
 
   ! ---- helpers ----
   FUNCTION pack_batch(Atm, Sfc, Geo, ChannelInfo, RTS, Opt, b, L, M) RESULT(Error_Status)

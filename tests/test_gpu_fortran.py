@@ -70,3 +70,20 @@ def test_fortran_driver_matches_oracle(coeff_dir, oracle_mod, tmp_path, prec):
             assert err <= jac_tol, (m, key, err)
         assert np.abs(got[m]["ts_k"] - ref["sfc_t_k"][:, 1]).max() <= jac_tol * np.abs(ref["sfc_t_k"][:, 1]).max()
         assert np.abs(got[m]["emis_k"] - ref["emis_k"]).max() <= jac_tol * np.abs(ref["emis_k"]).max()
+    # CRTM_Adjoint and CRTM_Tangent_Linear against the driver's own K-matrix
+    vals = iter(float(v) for v in open(fout + ".ad").read().split())
+    w = 0.5 + np.arange(1, L + 1) / L
+    tol = 1e-10 if prec == 0 else 1e-5
+    for m in range(3):
+        K = int(ps.n_layers[m])
+        a, b = int(ps.layer_offset[m]), int(ps.layer_offset[m + 1])
+        take = lambda n: np.array([next(vals) for _ in range(n)])
+        ad_t, ad_a, ad_ts, tl_bt = take(K), take(2 * K).reshape(2, K), take(1)[0], take(L)
+        g = got[m]
+        np.testing.assert_allclose(ad_t, (w[:, None] * g["t_k"]).sum(0), rtol=tol, atol=tol * np.abs(g["t_k"]).max())
+        want_a = (w[:, None, None] * g["abs_k"]).sum(0)
+        np.testing.assert_allclose(ad_a, want_a, rtol=tol, atol=tol * np.abs(want_a).max())
+        assert abs(ad_ts - (w * g["ts_k"]).sum()) <= tol * abs((w * g["ts_k"]).sum())
+        want_tl = g["t_k"].sum(1) + (g["abs_k"] * (0.01 * ps.absorber[:, a:b])[None]).sum((1, 2)) + 0.5 * g["ts_k"]
+        np.testing.assert_allclose(tl_bt, want_tl, rtol=tol, atol=tol * np.abs(want_tl).max())
+
