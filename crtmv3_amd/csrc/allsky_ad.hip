@@ -1607,7 +1607,7 @@ void launch_cloud_rt_wide(int nz, const DevSensor&, const CloudWork&, const DevC
 template <class RT>
 void launch_allsky_ad_wide(int nz, const DevSensor&, const DevBatch&, const DevClouds&, const CloudWork&,
                            const DevCloudCoeff&, const LayerScratch<RT>&, const LayerScratch<RT>&, const RT*,
-                           const double*, int, const DevOut&, const AllskyOut&, RT*, int, hipStream_t);
+                           const double*, int, const DevOut&, const AllskyOut&, RT*, int, RT*, hipStream_t);
 
 static int blocks_for(int64_t n, int cap) {
   const int64_t want = (n + 255) / 256;
@@ -1621,7 +1621,7 @@ void launch_allsky_k(const DevSensor& s, const DevBatch& be, const DevClouds& cl
                      const DevCloudCoeff& cc, const LayerScratch<RT>& sc, const LayerScratch<RT>& sa,
                      const RT* tau_g, const double* geo, int n_items, const int* nz_list, int n_nz, const DevOut& o,
                      const AllskyOut& ao, const CloudKOut& ko, RT* tape, int64_t tape_lanes, int te, RT* wtape,
-                     int kmax, hipStream_t st) {
+                     int kmax, RT* dtape, hipStream_t st) {
   const int lb = blocks_for(n_items, 8192);
   const int sb = allsky_sweep_ad_blocks((int64_t)be.M * s.L);
   if ((int64_t)sb * 256 != tape_lanes) return;   // the host sizes the tape for this grid
@@ -1629,7 +1629,8 @@ void launch_allsky_k(const DevSensor& s, const DevBatch& be, const DevClouds& cl
   for (int q = 0; q < n_nz; ++q) {
     if (nz_list[q] > sc.nzc) {   // wavefront kernels
       launch_cloud_rt_wide<RT>(nz_list[q], s, cw, cl, cc, sc, geo, n_items, st);
-      launch_allsky_ad_wide<RT>(nz_list[q], s, be, cl, cw, cc, sc, sa, tau_g, geo, n_items, o, ao, wtape, kmax, st);
+      launch_allsky_ad_wide<RT>(nz_list[q], s, be, cl, cw, cc, sc, sa, tau_g, geo, n_items, o, ao, wtape, kmax, dtape,
+                                st);
       continue;
     }
     launch_cloud_rt<RT>(nz_list[q], s, cw, cl, cc, sc, geo, n_items, st);
@@ -1663,7 +1664,7 @@ void launch_allsky_k(const DevSensor& s, const DevBatch& be, const DevClouds& cl
   template void launch_allsky_k<RT, KT>(const DevSensor&, const DevBatch&, const DevClouds&, const CloudWork&, \
                                         const DevCloudCoeff&, const LayerScratch<RT>&, const LayerScratch<RT>&, \
                                         const RT*, const double*, int, const int*, int, const DevOut&,        \
-                                        const AllskyOut&, const CloudKOut&, RT*, int64_t, int, RT*, int,        \
+                                        const AllskyOut&, const CloudKOut&, RT*, int64_t, int, RT*, int, RT*,   \
                                         hipStream_t);
 CRTM_INST_K(float, float)
 CRTM_INST_K(float, double)

@@ -614,11 +614,32 @@ __global__ void __launch_bounds__(64) allsky_sweep_ad_wide_kernel(DevSensor s, D
 // =========================================================================
 // layer R/T adjoint: one wavefront per scattering (layer, channel)
 // =========================================================================
+// Doubling tape of one wavefront: the state (R, T) entering doubling step d,
+// d < WDCAP, as [d][matrix][r][lane]; steps beyond the cap are recomputed
+// from the series start.
+constexpr int WDCAP = 24;
+
 template <class RT, int NZ>
 __global__ void __launch_bounds__(64) cloud_rt_ad_wide_kernel(DevSensor s, CloudWork cw, DevClouds cl,
                                                               DevCloudCoeff cc, LayerScratch<RT> sc,
-                                                              LayerScratch<RT> sa, const double* geo, int n_items) {
+                                                              LayerScratch<RT> sa, const double* geo, int n_items,
+                                                              RT* dtape) {
   __shared__ RT buf[16 * WLD];
+  RT* dt = dtape + (int64_t)blockIdx.x * WDCAP * 2 * 4 * 64 + threadIdx.x;
+  auto put = [&](int d, const WM<RT>& R_, const WM<RT>& T_) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      dt[((d * 2 + 0) * 4 + r) * 64] = R_.v[r];
+      dt[((d * 2 + 1) * 4 + r) * 64] = T_.v[r];
+    }
+  };
+  auto get = [&](int d, WM<RT>& R_, WM<RT>& T_) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      R_.v[r] = dt[((d * 2 + 0) * 4 + r) * 64];
+      T_.v[r] = dt[((d * 2 + 1) * 4 + r) * 64];
+    }
+  };
   const int j = wj(), g = wg();
   for (int item = blockIdx.x; item < n_items; item += gridDim.x) {
     const int c = item / s.L, l = item - c * s.L;
@@ -694,9 +715,22 @@ __global__ void __launch_bounds__(64) cloud_rt_ad_wide_kernel(DevSensor s, Cloud
       if (nd > 0) {
         WM<RT> R0, T0;
         wmexp_start(apb, amb, t, NZ, buf, R0, T0);
-        for (int d = nd - 1; d >= 0; --d) {
+        {   // tape the states entering the first WDCAP steps
           WM<RT> Rd = R0, Td = T0;
-          for (int q = 0; q < d; ++q) wdouble_step(Rd, Td, NZ, buf);
+          const int nt = nd < WDCAP ? nd : WDCAP;
+          for (int d = 0; d < nt; ++d) {
+            put(d, Rd, Td);
+            if (d + 1 < nt) wdouble_step(Rd, Td, NZ, buf);
+          }
+        }
+        for (int d = nd - 1; d >= 0; --d) {
+          WM<RT> Rd, Td;
+          if (d < WDCAP) {
+            get(d, Rd, Td);
+          } else {
+            get(WDCAP - 1, Rd, Td);
+            for (int q = WDCAP - 1; q < d; ++q) wdouble_step(Rd, Td, NZ, buf);
+          }
           wdouble_step_ad(Rd, Td, Ra, Ta, NZ, buf);
         }
       }
@@ -730,12 +764,14 @@ __global__ void __launch_bounds__(64) cloud_rt_ad_wide_kernel(DevSensor s, Cloud
 static int wide_ad_blocks(int64_t n) { return (int)std::max<int64_t>(1, std::min<int64_t>(n, 8192)); }
 
 int allsky_sweep_ad_wide_blocks(int64_t n_pc) { return wide_ad_blocks(n_pc); }
+// elements of the doubling tape of the layer adjoint for n_items items
+int64_t wide_doubling_tape_elems(int64_t n_items) { return (int64_t)wide_ad_blocks(n_items) * WDCAP * 2 * 4 * 64; }
 
 template <class RT>
 void launch_allsky_ad_wide(int nz, const DevSensor& s, const DevBatch& be, const DevClouds& cl, const CloudWork& cw,
                            const DevCloudCoeff& cc, const LayerScratch<RT>& sc, const LayerScratch<RT>& sa,
                            const RT* tau_g, const double* geo, int n_items, const DevOut& o, const AllskyOut& ao,
-                           RT* tape, int kmax, hipStream_t st) {
+                           RT* tape, int kmax, RT* dtape, hipStream_t st) {
   const int sb = wide_ad_blocks((int64_t)be.M * s.L);
   const int lb = wide_ad_blocks(n_items);
   switch (nz) {
@@ -745,7 +781,7 @@ void launch_allsky_ad_wide(int nz, const DevSensor& s, const DevBatch& be, const
                        tau_g, geo, o, ao, tape, kmax);                                                             \
     if (n_items > 0)                                                                                               \
       hipLaunchKernelGGL((cloud_rt_ad_wide_kernel<RT, N>), dim3(lb), dim3(64), 0, st, s, cw, cl, cc, sc, sa, geo,  \
-                         n_items);                                                                                 \
+                         n_items, dtape);                                                                          \
     break;
     CRTM_CASE(6)
     CRTM_CASE(9)
@@ -759,10 +795,10 @@ void launch_allsky_ad_wide(int nz, const DevSensor& s, const DevBatch& be, const
 template void launch_allsky_ad_wide<float>(int, const DevSensor&, const DevBatch&, const DevClouds&,
                                            const CloudWork&, const DevCloudCoeff&, const LayerScratch<float>&,
                                            const LayerScratch<float>&, const float*, const double*, int,
-                                           const DevOut&, const AllskyOut&, float*, int, hipStream_t);
+                                           const DevOut&, const AllskyOut&, float*, int, float*, hipStream_t);
 template void launch_allsky_ad_wide<double>(int, const DevSensor&, const DevBatch&, const DevClouds&,
                                             const CloudWork&, const DevCloudCoeff&, const LayerScratch<double>&,
                                             const LayerScratch<double>&, const double*, const double*, int,
-                                            const DevOut&, const AllskyOut&, double*, int, hipStream_t);
+                                            const DevOut&, const AllskyOut&, double*, int, double*, hipStream_t);
 
 }  // namespace crtm

@@ -35,9 +35,11 @@ void launch_clear_copy(const DevSensor&, int M, int Ltot, const int* cl_count, c
 template <class RT, class KT>
 void launch_allsky_k(const DevSensor&, const DevBatch&, const DevClouds&, const CloudWork&, const DevCloudCoeff&,
                      const LayerScratch<RT>&, const LayerScratch<RT>&, const RT*, const double*, int, const int*, int,
-                     const DevOut&, const AllskyOut&, const CloudKOut&, RT*, int64_t, int, RT*, int, hipStream_t);
+                     const DevOut&, const AllskyOut&, const CloudKOut&, RT*, int64_t, int, RT*, int, RT*,
+                     hipStream_t);
 int allsky_sweep_ad_blocks(int64_t n_lanes);
 int allsky_sweep_ad_wide_blocks(int64_t n_pc);
+int64_t wide_doubling_tape_elems(int64_t n_items);
 void launch_aerosol_check(const DevSensor&, int M, const DevClouds&, const CloudWork&, const DevCloudCoeff&,
                           hipStream_t);
 void launch_aerosol_fail(const DevSensor&, int M, int Ltot, const DevClouds&, const CloudWork&, int* status,
@@ -708,7 +710,7 @@ struct crtm_context {
   DevCloudCoeff cc{};
   DevBuf cwc, creff, ccount, cslot, cdesc, ctotal, ctcc, cfrac, cmaxr, ctau, cscr, caerr, cwide;
   // all-sky K-matrix: layer adjoints, per-thread adding tape, seeds for the clear-sky K kernel
-  DevBuf cscr_ad, cwide_ad, ctape, ctape_w, ks_tau, ks_b, ks_t, ks_ch;
+  DevBuf cscr_ad, cwide_ad, ctape, ctape_w, ctape_d, ks_tau, ks_b, ks_t, ks_ch;
   // smallest angle count run by the wavefront (matrix-core) kernels
   // (crtm_set_wavefront_angles)
   int wide_min = 9;
@@ -1022,8 +1024,10 @@ void run_sensor_k_allsky(crtm_context* c, Sensor& S, const DevBatch& be, const D
   c->ctape.reserve(std::max<size_t>((size_t)tape_lanes * std::max(cc.max_k, 1) * te * sizeof(RT), 16));
   // wavefront adding adjoint (16 streams): 8 x 64 values per layer and wavefront
   const int kmax = std::max(cc.max_k, 1);
-  if (sc.nzw > 0)
+  if (sc.nzw > 0) {
     c->ctape_w.reserve((size_t)allsky_sweep_ad_wide_blocks(ML) * kmax * 8 * 64 * sizeof(RT));
+    c->ctape_d.reserve((size_t)wide_doubling_tape_elems(std::max<int64_t>(items, 1)) * sizeof(RT));
+  }
   c->ks_tau.reserve(std::max<size_t>((size_t)be.NT * L * sizeof(RT), 16));
   c->ks_b.reserve(std::max<size_t>((size_t)be.NT * L * sizeof(RT), 16));
   c->ks_t.reserve(std::max<size_t>((size_t)be.NT * L * sizeof(RT), 16));
@@ -1046,7 +1050,7 @@ void run_sensor_k_allsky(crtm_context* c, Sensor& S, const DevBatch& be, const D
   launch_aerosol_check(S.d, be.M, cc.cl, cw, c->cc, c->stream);
   launch_allsky_k<RT, KT>(S.d, be, cc.cl, cw, c->cc, sc, sa, reinterpret_cast<const RT*>(o1.tau_g), w.geo,
                           (int)items, nz.data(), (int)nz.size(), o2, cc.ao, cc.ko, c->ctape.as<RT>(), tape_lanes, te,
-                          c->ctape_w.as<RT>(), kmax, c->stream);
+                          c->ctape_w.as<RT>(), kmax, c->ctape_d.as<RT>(), c->stream);
   launch_aerosol_fail(S.d, be.M, be.Ltot, cc.cl, cw, w.status, o2, c->stream);
   launch_clear_copy(S.d, be.M, be.Ltot, cc.cw.cl_count, o1, cc.ao, c->stream);   // profiles without cloudy layers
   if (c->profiling) c->mark();
