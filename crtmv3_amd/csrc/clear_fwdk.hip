@@ -175,21 +175,22 @@ __device__ inline void finish_layer(const RT* u, LayerAdj<RT>& a) {
   a.T += a.T2 * RT(2) * u[U_T];
 }
 
+// The layer's predictor adjoint is linear in X_AD(k, :, :) = C(:, :, k) * OD_AD(k)
+// with wave-uniform partials, so it runs on the coefficients alone and the
+// caller scales the layer's results by OD_AD(k) once.
 template <int G, class RT, int J>
-__device__ inline void comp_layer_ad(const float* ck, const int* npb, RT od_ad, const RT* uk,
-                                     LayerAdj<RT>& a) {
+__device__ inline void comp_layer_ad(const float* ck, const int* npb, const RT* uk, LayerAdj<RT>& a) {
   using Tg = GT<G>;
   if constexpr (J < Tg::NCOMP) {
-    const int np = npb[J];
-    if (np > 0) {
+    if (npb[J] > 0) {   // block-uniform: no channel of the block uses component J
       RT xa[Tg::NP[J]];
       const float* cj = ck + Tg::XOFF[J] * WAVE;
       // slots beyond a channel's predictor count hold zeros
 #pragma unroll
-      for (int i = 0; i < Tg::NP[J]; ++i) xa[i] = (RT)cj[i * WAVE] * od_ad;
+      for (int i = 0; i < Tg::NP[J]; ++i) xa[i] = (RT)cj[i * WAVE];
       comp_adjoint<G, J, RT>(xa, uk, a);
     }
-    comp_layer_ad<G, RT, J + 1>(ck, npb, od_ad, uk, a);
+    comp_layer_ad<G, RT, J + 1>(ck, npb, uk, a);
   }
 }
 
@@ -658,7 +659,7 @@ clear_fwdk_kernel(DevSensor s, DevBatch bt, ProfWork<RT> w, DevOut o) {
       const RT* uk = tab[wv][k % NBUF];
       LayerAdj<RT> a{};
       const float* ck = cs[k % NBUF] + lane;
-      comp_layer_ad<G, RT, 0>(ck, npb, od_ad, uk, a);
+      comp_layer_ad<G, RT, 0>(ck, npb, uk, a);
       RT t_ad = RT(0), vap_ad = RT(0);
       if (blk_opt) {
         // Add_OPTRAN_wloOD_AD (ODPS_AtmAbsorption.f90:911-984) with chi recomputed
@@ -704,21 +705,21 @@ clear_fwdk_kernel(DevSensor s, DevBatch bt, ProfWork<RT> w, DevOut o) {
         vap_ad += okr[O_DPSEC] * dab_ad;
       }
       finish_layer<G, RT>(uk, a);
-      t_ad += a.T * uk[U_IRT] + a.dt;
+      t_ad += (a.T * uk[U_IRT] + a.dt) * od_ad;
       RT a_ad[Tg::NABS];
 #pragma unroll
-      for (int j = 0; j < Tg::NABS; ++j) a_ad[j] = a.ra[j] * uk[U_IRA0 + j];
+      for (int j = 0; j < Tg::NABS; ++j) a_ad[j] = a.ra[j] * od_ad * uk[U_IRA0 + j];
       if (s.optran) a_ad[h2o_j] += vap_ad;
       const RT pdp = uk[U_PDP];
 #pragma unroll
       for (int j = 0; j < Tg::NABS; ++j) {
-        gts[j] += a.gt[j] * uk[U_IGATZP0 + j];
-        gzs[j] += a.gz[j] * uk[U_IGAZP0 + j];
+        gts[j] += a.gt[j] * od_ad * uk[U_IGATZP0 + j];
+        gzs[j] += a.gz[j] * od_ad * uk[U_IGAZP0 + j];
         t_ad += gts[j] * pdp * uk[U_AODPS0 + j];
         a_ad[j] += gzs[j] * pdp + gts[j] * pdp * uk[U_TODPS];
       }
-      tzps += a.tzp * uk[U_ITZPREF];
-      tzs += a.tz * uk[U_ITZREF];
+      tzps += a.tzp * od_ad * uk[U_ITZPREF];
+      tzs += a.tz * od_ad * uk[U_ITZREF];
       t_ad += tzs + pdp * tzps;
       // Map_Input_AD (ODPS_CoordinateMapping.f90:480-547): add onto the user
       // layers averaged into this ODPS layer (their Planck part is already stored)
