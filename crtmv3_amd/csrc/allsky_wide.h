@@ -126,12 +126,39 @@ template <class T> __device__ inline WM<T> wadd(const WM<T>& a, const WM<T>& b, 
   return x;
 }
 
+// ---------------------------------------------------- cross-lane moves
+// DPP move within each 16-lane row (the lane's group g): CTRL is a DPP
+// control word, e.g. quad_perm, row_half_mirror (0x141) or row_mirror (0x140).
+// 64-bit values move as two 32-bit halves.
+template <int CTRL> __device__ inline float wdpp(float x) {
+  return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(x), CTRL, 0xf, 0xf, false));
+}
+template <int CTRL> __device__ inline double wdpp(double x) {
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)b, CTRL, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(b >> 32), CTRL, 0xf, 0xf, false);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+// element of lane `lane` (wavefront-uniform) to every lane
+__device__ inline float wreadlane(float x, int lane) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(x), lane));
+}
+__device__ inline double wreadlane(double x, int lane) {
+  const long long b = __double_as_longlong(x);
+  const int lo = __builtin_amdgcn_readlane((int)b, lane), hi = __builtin_amdgcn_readlane((int)(b >> 32), lane);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+
 // ------------------------------------------------------------ reductions
-template <class T> __device__ inline T wsum16(T x) {   // over j within the lane's group
-  x += __shfl_xor(x, 1);
-  x += __shfl_xor(x, 2);
-  x += __shfl_xor(x, 4);
-  x += __shfl_xor(x, 8);
+// Sum over j within the lane's group: the butterfly pairs lanes j^1, j^2, then
+// the two quads of each half-row (half mirror) and the two half-rows (mirror),
+// so every lane of a row adds the same operands in the same pairing and holds
+// the identical sum.
+template <class T> __device__ inline T wsum16(T x) {
+  x += wdpp<0xB1>(x);    // quad_perm [1, 0, 3, 2]
+  x += wdpp<0x4E>(x);    // quad_perm [2, 3, 0, 1]
+  x += wdpp<0x141>(x);   // row_half_mirror
+  x += wdpp<0x140>(x);   // row_mirror
   return x;
 }
 template <class T> __device__ inline T wsum4(T x) {    // over g at the lane's j
@@ -158,11 +185,11 @@ template <class T> __device__ inline T wsel(const T (&v)[4], int r) {
 }
 // X[i][k] for wavefront-uniform i, k
 template <class T> __device__ inline T wat(const WM<T>& X, int i, int k) {
-  return __shfl(wsel(X.v, WL<T>::reg(i)), k + 16 * WL<T>::grp(i));
+  return wreadlane(wsel(X.v, WL<T>::reg(i)), k + 16 * WL<T>::grp(i));
 }
 // element i of a column vector, i wavefront-uniform
 template <class T> __device__ inline T wat(const WCol<T>& x, int i) {
-  return __shfl(wsel(x.v, WL<T>::reg(i)), 16 * WL<T>::grp(i));
+  return wreadlane(wsel(x.v, WL<T>::reg(i)), 16 * WL<T>::grp(i));
 }
 // column vector -> row vector and back
 template <class T> __device__ inline T wc2r(const WCol<T>& x) {
@@ -192,7 +219,10 @@ template <class T> __device__ inline T wmtv(const WM<T>& X, const WCol<T>& y) {
 
 // In-place Gauss-Jordan inverse of the leading n x n block without pivoting,
 // the arithmetic of the per-lane inv_inplace (every matrix inverted here is
-// diagonally dominant).  Row k and column k are broadcast by shuffles.
+// diagonally dominant).  Row k and column k are broadcast by shuffles: a
+// variant with DPP row broadcasts for column k matched this one bit for bit in
+// isolation but gave wrong adjoints inside the 6-angle FP64 layer kernels, so
+// the inverse keeps the shuffles.
 template <class T> __device__ inline void winv(WM<T>& b, int n) {
   const int j = wj(), g = wg();
   for (int k = 0; k < n; ++k) {
