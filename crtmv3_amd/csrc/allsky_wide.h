@@ -161,10 +161,35 @@ template <class T> __device__ inline T wsum16(T x) {
   x += wdpp<0x140>(x);   // row_mirror
   return x;
 }
-template <class T> __device__ inline T wsum4(T x) {    // over g at the lane's j
-  x += __shfl_xor(x, 16);
-  x += __shfl_xor(x, 32);
-  return x;
+// Sum over g at the lane's j: permlane16_swap pairs lanes l and l ^ 16,
+// permlane32_swap lanes l and l ^ 32 (the two results of each swap are the
+// pair's two values), with no trip through the LDS crossbar.
+__device__ inline float wsum4(float x) {
+  const auto a = __builtin_amdgcn_permlane16_swap(__float_as_uint(x), __float_as_uint(x), false, false);
+  x = __uint_as_float(a[0]) + __uint_as_float(a[1]);
+  const auto b = __builtin_amdgcn_permlane32_swap(__float_as_uint(x), __float_as_uint(x), false, false);
+  return __uint_as_float(b[0]) + __uint_as_float(b[1]);
+}
+__device__ inline double wsum4(double x) {
+  auto pair16 = [](double v, double& u, double& w) {
+    const long long bits = __double_as_longlong(v);
+    const auto lo = __builtin_amdgcn_permlane16_swap((unsigned)bits, (unsigned)bits, false, false);
+    const auto hi = __builtin_amdgcn_permlane16_swap((unsigned)(bits >> 32), (unsigned)(bits >> 32), false, false);
+    u = __longlong_as_double(((long long)hi[0] << 32) | (long long)lo[0]);
+    w = __longlong_as_double(((long long)hi[1] << 32) | (long long)lo[1]);
+  };
+  auto pair32 = [](double v, double& u, double& w) {
+    const long long bits = __double_as_longlong(v);
+    const auto lo = __builtin_amdgcn_permlane32_swap((unsigned)bits, (unsigned)bits, false, false);
+    const auto hi = __builtin_amdgcn_permlane32_swap((unsigned)(bits >> 32), (unsigned)(bits >> 32), false, false);
+    u = __longlong_as_double(((long long)hi[0] << 32) | (long long)lo[0]);
+    w = __longlong_as_double(((long long)hi[1] << 32) | (long long)lo[1]);
+  };
+  double u, w;
+  pair16(x, u, w);
+  x = u + w;
+  pair32(x, u, w);
+  return u + w;
 }
 template <class T> __device__ inline T wmax64(T x) {
 #pragma unroll
