@@ -14,8 +14,8 @@
 //                                (ADA_Module.f90:963-1109, 1760-2081) and the
 //                                phase matrix + Normalize_Phase adjoint
 //                                (src/RTSolution/Common_RTSolution.f90:2548-2879,
-//                                3044-3116), doubling steps recomputed from the
-//                                series start.
+//                                3044-3116); the states entering the doubling
+//                                steps come from a per-wavefront tape.
 //
 // The reverse recurrences are those of the per-lane adjoints in allsky_ad.hip;
 // the optics, cover and clear-sky adjoints that follow are shared with them.
