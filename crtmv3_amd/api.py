@@ -107,6 +107,7 @@ def lib():
         L.crtm_synchronize.argtypes = [_vp]
         L.crtm_set_stream.argtypes = [_vp, _vp, ctypes.c_int]
         L.crtm_set_wavefront_angles.argtypes = [_vp, ctypes.c_int]
+        L.crtm_set_cloudy_layer_capacity.argtypes = [_vp, ctypes.c_int64]
         L.crtm_last_error.argtypes = [_vp]
         L.crtm_last_error.restype = ctypes.c_char_p
         L.crtm_profile_enable.argtypes = [_vp, ctypes.c_int]
@@ -185,6 +186,14 @@ class Context:
         (matrix-core) scattering kernels, smaller ones on the per-lane kernels."""
         if lib().crtm_set_wavefront_angles(self._h, int(min_angles)) != SUCCESS:
             self._err("crtm_set_wavefront_angles")
+
+    def set_cloudy_layer_capacity(self, max_layers: int):
+        """Size the scattering launches for max_layers scattering layers of a
+        batch instead of reading the count back (0: read back, the default).
+        With a capacity, device-memory calls with clouds or aerosols can be
+        captured in a HIP graph; profiles whose layers do not fit fail."""
+        if lib().crtm_set_cloudy_layer_capacity(self._h, int(max_layers)) != SUCCESS:
+            self._err("crtm_set_cloudy_layer_capacity")
 
     def _err(self, what):
         msg = lib().crtm_last_error(self._h).decode()

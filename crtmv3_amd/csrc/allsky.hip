@@ -155,7 +155,8 @@ __global__ void __launch_bounds__(64) cloud_prep_kernel(DevBatch be, DevClouds c
   cw.tcc[m] = fractional ? tcc : 0.0;
   cw.frac[m] = fractional;
   cw.max_reff[m] = max_reff;
-  cw.aer_err[m] = 0;
+  // a profile whose scattering layers did not all find a slot fails
+  cw.aer_err[m] = (count > 0 && base + count > cw.cap) ? 2 : 0;
 }
 
 // =========================================================================
@@ -166,7 +167,8 @@ __global__ void __launch_bounds__(256) cloud_optics_kernel(DevSensor s, DevBatch
                                                             DevCloudCoeff cc, LayerScratch<RT> sc, const RT* tau_g,
                                                             int n_items) {
   const bool mw = s.sensor_type == 1;
-  for (int item = blockIdx.x * blockDim.x + threadIdx.x; item < n_items; item += gridDim.x * blockDim.x) {
+  const int n_live = live_items(cw, s.L, n_items);
+  for (int item = blockIdx.x * blockDim.x + threadIdx.x; item < n_live; item += gridDim.x * blockDim.x) {
     const int c = item / s.L, l = item - c * s.L;
     const int2 d = cw.cl_desc[c];
     const int m = d.x, k = d.y;
@@ -233,7 +235,8 @@ __global__ void __launch_bounds__(256) cloud_optics_kernel(DevSensor s, DevBatch
 template <class RT, int NZ>
 __global__ void __launch_bounds__(256) cloud_rt_kernel(DevSensor s, CloudWork cw, DevClouds cl, DevCloudCoeff cc,
                                                         LayerScratch<RT> sc, const double* geo, int n_items) {
-  for (int item = blockIdx.x * blockDim.x + threadIdx.x; item < n_items; item += gridDim.x * blockDim.x) {
+  const int n_live = live_items(cw, s.L, n_items);
+  for (int item = blockIdx.x * blockDim.x + threadIdx.x; item < n_live; item += gridDim.x * blockDim.x) {
     const int c = item / s.L, l = item - c * s.L;
     const int m = cw.cl_desc[c].x;
     if (cw.status[m] != 0) continue;
@@ -592,8 +595,14 @@ __global__ void aerosol_fail_kernel(DevSensor s, int M, int Ltot, const int* aer
     const int64_t ol = (int64_t)m * Ltot + s.chan_offset + l;
     o.radiance[ol] = __builtin_nan("");
     o.bt[ol] = __builtin_nan("");
-    if (l == 0) status[m] = 5;
+    if (l == 0) status[m] = aer_err[m] == 2 ? 6 : 5;
   }
+}
+// zero fill of n bytes (4-byte aligned), a kernel so that capturable calls
+// record no memset nodes
+__global__ void zero_fill_kernel(uint32_t* p, int64_t n_words) {
+  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n_words; t += (int64_t)gridDim.x * blockDim.x)
+    p[t] = 0u;
 }
 
 // ------------------------------------------------------------------ launchers
@@ -609,9 +618,17 @@ void launch_aerosol_check(const DevSensor& s, int M, const DevClouds& cl, const 
                      st, s, M, cl, cw, cc.aero);
 }
 
+void launch_zero(void* p, size_t bytes, hipStream_t st) {
+  const int64_t n = (int64_t)(bytes / 4);
+  if (n > 0)
+    hipLaunchKernelGGL(zero_fill_kernel, dim3((unsigned)std::min<int64_t>((n + 255) / 256, 16384)), dim3(256), 0, st,
+                       static_cast<uint32_t*>(p), n);
+}
+
 void launch_aerosol_fail(const DevSensor& s, int M, int Ltot, const DevClouds& cl, const CloudWork& cw, int* status,
                          const DevOut& o, hipStream_t st) {
-  if (s.sensor_type == 1 || cl.na == 0) return;
+  // aerosol failures (infrared), and layer-capacity overflows of capacity-sized launches
+  if ((s.sensor_type == 1 || cl.na == 0) && !cw.capmode) return;
   const int64_t n = (int64_t)M * s.L;
   hipLaunchKernelGGL(aerosol_fail_kernel, dim3((unsigned)std::min<int64_t>((n + 255) / 256, 8192)), dim3(256), 0,
                      st, s, M, Ltot, (const int*)cw.aer_err, status, o);

@@ -1141,7 +1141,8 @@ template <class RT, int NZ>
 __global__ void __launch_bounds__(256) cloud_rt_ad_kernel(DevSensor s, CloudWork cw, DevClouds cl, DevCloudCoeff cc,
                                                            LayerScratch<RT> sc, LayerScratch<RT> sa,
                                                            const double* geo, int n_items) {
-  for (int item = blockIdx.x * blockDim.x + threadIdx.x; item < n_items; item += gridDim.x * blockDim.x) {
+  const int n_live = live_items(cw, s.L, n_items);
+  for (int item = blockIdx.x * blockDim.x + threadIdx.x; item < n_live; item += gridDim.x * blockDim.x) {
     const int c = item / s.L, l = item - c * s.L;
     const int m = cw.cl_desc[c].x;
     if (cw.status[m] != 0 || cw.aer_err[m] != 0) continue;
@@ -1290,7 +1291,8 @@ __global__ void __launch_bounds__(256) cloud_optics_ad_kernel(DevSensor s, DevBa
   const bool mw = s.sensor_type == 1;
   KT* wck = reinterpret_cast<KT*>(ko.wc_k);
   KT* rek = reinterpret_cast<KT*>(ko.reff_k);
-  for (int item = blockIdx.x * blockDim.x + threadIdx.x; item < n_items; item += gridDim.x * blockDim.x) {
+  const int n_live = live_items(cw, s.L, n_items);
+  for (int item = blockIdx.x * blockDim.x + threadIdx.x; item < n_live; item += gridDim.x * blockDim.x) {
     const int c = item / s.L, l = item - c * s.L;
     const int2 d = cw.cl_desc[c];
     const int m = d.x, k = d.y;

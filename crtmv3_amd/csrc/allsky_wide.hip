@@ -26,7 +26,8 @@ template <class RT, int NZ>
 __global__ void __launch_bounds__(64) cloud_rt_wide_kernel(DevSensor s, CloudWork cw, DevClouds cl, DevCloudCoeff cc,
                                                            LayerScratch<RT> sc, const double* geo, int n_items) {
   __shared__ RT buf[16 * WLD];
-  for (int item = blockIdx.x; item < n_items; item += gridDim.x) {
+  const int n_live = live_items(cw, s.L, n_items);
+  for (int item = blockIdx.x; item < n_live; item += gridDim.x) {
     const int c = item / s.L, l = item - c * s.L;
     const int m = cw.cl_desc[c].x;
     if (cw.status[m] != 0) continue;

@@ -641,7 +641,8 @@ __global__ void __launch_bounds__(64) cloud_rt_ad_wide_kernel(DevSensor s, Cloud
     }
   };
   const int j = wj(), g = wg();
-  for (int item = blockIdx.x; item < n_items; item += gridDim.x) {
+  const int n_live = live_items(cw, s.L, n_items);
+  for (int item = blockIdx.x; item < n_live; item += gridDim.x) {
     const int c = item / s.L, l = item - c * s.L;
     const int m = cw.cl_desc[c].x;
     if (cw.status[m] != 0 || cw.aer_err[m] != 0) continue;

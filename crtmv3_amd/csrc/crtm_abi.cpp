@@ -44,6 +44,7 @@ void launch_aerosol_check(const DevSensor&, int M, const DevClouds&, const Cloud
                           hipStream_t);
 void launch_aerosol_fail(const DevSensor&, int M, int Ltot, const DevClouds&, const CloudWork&, int* status,
                          const DevOut&, hipStream_t);
+void launch_zero(void* p, size_t bytes, hipStream_t st);
 }  // namespace crtm
 
 using namespace crtm;
@@ -714,6 +715,9 @@ struct crtm_context {
   // smallest angle count run by the wavefront (matrix-core) kernels
   // (crtm_set_wavefront_angles)
   int wide_min = 9;
+  // scattering-layer slots of capturable cloudy calls (crtm_set_cloudy_layer_capacity);
+  // 0: the count is read back and the launches are sized for it
+  int64_t cloud_cap = 0;
   // kernel timing (crtm_profile_enable): HIP events on the context stream
   bool profiling = false;
   std::vector<hipEvent_t> ev;      // pairs (start, stop) per recorded launch
@@ -840,7 +844,10 @@ ProfWork<RT> run_sensor(crtm_context* c, Sensor& S, const DevBatch& b, const Dev
 }
 
 // Cloud preparation for the whole batch (sensor independent); returns the
-// number of cloudy layers (one host synchronisation).
+// number of cloudy layers the scattering launches are sized for: the counted
+// layers (one host synchronisation), or with a layer capacity set the
+// capacity itself (no synchronisation; the kernels stop at the device-side
+// count, and profiles that do not fit fail).
 int prepare_clouds(crtm_context* c, const DevBatch& be, const DevClouds& cl, CloudWork& cw, int* max_k_cloudy) {
   const int M = be.M;
   const int64_t NTe = be.NT;
@@ -864,10 +871,15 @@ int prepare_clouds(crtm_context* c, const DevBatch& be, const DevClouds& cl, Clo
   cw.frac = c->cfrac.as<int>();
   cw.max_reff = c->cmaxr.as<double>();
   cw.aer_err = c->caerr.as<int>();
-  cw.cap = (int)std::min<int64_t>(be.out_NT, INT32_MAX);
-  HIPCHECK(hipMemsetAsync(cw.total, 0, 2 * sizeof(int), c->stream));
+  cw.capmode = c->cloud_cap > 0;
+  cw.cap = (int)std::min<int64_t>(cw.capmode ? std::min<int64_t>(c->cloud_cap, be.out_NT) : be.out_NT, INT32_MAX);
+  launch_zero(cw.total, 2 * sizeof(int), c->stream);
   launch_cloud_prep(be, cl, cw, c->stream);
   HIPCHECK(hipGetLastError());
+  if (cw.capmode) {
+    if (max_k_cloudy) *max_k_cloudy = be.max_k;
+    return cw.cap;
+  }
   int tot[2] = {0, 0};
   HIPCHECK(hipMemcpyAsync(tot, cw.total, 2 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
   HIPCHECK(hipStreamSynchronize(c->stream));
@@ -1090,14 +1102,15 @@ void dispatch(crtm_context* c, const DevBatch& be, const DevOut& o, const CloudC
 void zero_cloud_k(crtm_context* ctx, const CloudKOut& ko, int nc, int na, int64_t NT, int L, size_t szK) {
   const size_t n = (size_t)NT * L * szK;
   if (nc > 0) {
-    HIPCHECK(hipMemsetAsync(ko.wc_k, 0, nc * n, ctx->stream));
-    HIPCHECK(hipMemsetAsync(ko.reff_k, 0, nc * n, ctx->stream));
-    HIPCHECK(hipMemsetAsync(ko.cf_k, 0, n, ctx->stream));
+    launch_zero(ko.wc_k, nc * n, ctx->stream);
+    launch_zero(ko.reff_k, nc * n, ctx->stream);
+    launch_zero(ko.cf_k, n, ctx->stream);
   }
   if (na > 0) {
-    HIPCHECK(hipMemsetAsync(ko.aconc_k, 0, na * n, ctx->stream));
-    HIPCHECK(hipMemsetAsync(ko.areff_k, 0, na * n, ctx->stream));
+    launch_zero(ko.aconc_k, na * n, ctx->stream);
+    launch_zero(ko.areff_k, na * n, ctx->stream);
   }
+  HIPCHECK(hipGetLastError());
 }
 
 int do_call(crtm_context* ctx, const crtm_atmosphere* atm, const crtm_surface* sfc, const crtm_geometry* geo,
@@ -1111,14 +1124,22 @@ int do_call(crtm_context* ctx, const crtm_atmosphere* atm, const crtm_surface* s
   if (atm->n_absorbers < 1 || atm->n_absorbers > NA_MAX)
     return fail(ctx, fmt("n_absorbers %d outside 1..%d", atm->n_absorbers, NA_MAX));
   if (!rts->radiance || !rts->brightness_temperature) return fail(ctx, "RTSolution radiance/BT arrays required");
-  // under HIP graph capture only the clear-sky device-memory call is valid:
-  // host staging and the cloud-layer count each synchronise with the host
+  // under HIP graph capture only device-memory calls are valid, cloudy ones
+  // with a layer capacity and one stream count for the batch: host staging,
+  // the cloud-layer count and per-profile stream counts in device memory each
+  // synchronise with the host
   hipStreamCaptureStatus capture = hipStreamCaptureStatusNone;
   HIPCHECK(hipStreamIsCapturing(ctx->stream, &capture));
-  if (capture != hipStreamCaptureStatusNone &&
-      (opt->memory == CRTM_MEM_HOST || atm->n_clouds > 0 || atm->n_aerosols > 0))
-    return fail(ctx, "graph capture: only device-memory clear-sky calls can be captured (host staging and the "
-                     "cloud-layer count synchronise with the host)");
+  if (capture != hipStreamCaptureStatusNone) {
+    const bool cloudy = atm->n_clouds > 0 || atm->n_aerosols > 0;
+    if (opt->memory == CRTM_MEM_HOST)
+      return fail(ctx, "graph capture: host-memory calls synchronise with the host; use device memory");
+    if (cloudy && ctx->cloud_cap <= 0)
+      return fail(ctx, "graph capture: a cloudy call reads its cloud-layer count back unless a layer capacity is "
+                       "set (crtm_set_cloudy_layer_capacity)");
+    if (cloudy && opt->n_streams_profile)
+      return fail(ctx, "graph capture: per-profile stream counts are read back; give one n_streams for the batch");
+  }
   CloudCall cc;
   cc.on = atm->n_clouds > 0 || atm->n_aerosols > 0;
   if (cc.on) {
@@ -1531,6 +1552,14 @@ int crtm_set_wavefront_angles(crtm_context* ctx, int min_angles) {
   if (min_angles != 6 && min_angles != 9)
     return fail(ctx, fmt("wavefront angle threshold %d: 6 or 9", min_angles));
   ctx->wide_min = min_angles;
+  return CRTM_SUCCESS;
+}
+
+int crtm_set_cloudy_layer_capacity(crtm_context* ctx, int64_t max_layers) {
+  if (!ctx) return CRTM_FAILURE;
+  if (max_layers < 0 || max_layers > INT32_MAX)
+    return fail(ctx, fmt("cloudy layer capacity %lld: 0 (off) .. %d", (long long)max_layers, INT32_MAX));
+  ctx->cloud_cap = max_layers;
   return CRTM_SUCCESS;
 }
 

@@ -316,10 +316,21 @@ struct CloudWork {
   double* tcc;          // [M] total cloud cover of fractional profiles (else 0)
   int* frac;            // [M] 1: fractional coverage
   double* max_reff;     // [M] largest effective radius (stream choice)
-  int* aer_err;         // [M] 1: an infrared channel's stream count has no aerosol phase function
-  int cap;
+  int* aer_err;         // [M] 1: an infrared channel's stream count has no aerosol phase function,
+                        //     2: the profile's scattering layers exceed the layer capacity
+  int cap;              // cloudy-layer slots
+  int capmode;          // 1: launches are sized for cap, not for the counted layers
   const int* status;    // [M] profile status of the preparation kernel (0 = good)
 };
+
+// Items (cloudy layer x channel) of a scattering launch that hold a layer.  A
+// launch sized for the slot capacity (capturable calls) stops at the count
+// the preparation kernel left on the device; otherwise the two agree.
+__device__ inline int live_items(const CloudWork& cw, int L, int n_items) {
+  const int n = cw.total[0] < cw.cap ? cw.total[0] : cw.cap;
+  const int64_t live = (int64_t)n * L;
+  return live < n_items ? (int)live : n_items;
+}
 
 // Per-(cloudy layer, channel) layer results, structure of arrays
 // [element][item], item = c * L + l

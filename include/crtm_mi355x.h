@@ -230,6 +230,18 @@ int crtm_set_stream(crtm_context* ctx, void* stream, int own_stream);
  * serial form (src/RTSolution/ADA/ADA_Module.f90). */
 int crtm_set_wavefront_angles(crtm_context* ctx, int min_angles);
 
+/* Scattering-layer capacity for calls with clouds or aerosols.  0 (the
+ * default): each such call counts its scattering layers on the device and
+ * reads the count back once to size the scattering launches.  max_layers > 0:
+ * the launches and their work space are sized for max_layers scattering
+ * layers of the batch, nothing is read back, and a device-memory call can be
+ * captured in a HIP graph (after one eager call of the same shape has sized
+ * the work space, with one n_streams for the batch).  A profile whose
+ * scattering layers do not fit fails (status FAILURE, NaN radiances).  No
+ * counterpart in the reference, which allocates per profile on the host
+ * (src/CRTM_Forward_Module.f90:515-663). */
+int crtm_set_cloudy_layer_capacity(crtm_context* ctx, int64_t max_layers);
+
 /* Wait for the context stream (device-memory calls are asynchronous). */
 int crtm_synchronize(crtm_context* ctx);
 

@@ -3,8 +3,10 @@ torch.cuda.CUDAGraph): the clear-sky path issues kernels and device copies
 only, so a warmed-up call of fixed shape can be captured once and replayed
 with new inputs written in place.  Every replay is checked against an eager
 call on the same inputs, with eager work and synchronisations between the
-replays.  Cloudy and host-memory calls, which synchronise with the host, fail
-cleanly under capture."""
+replays.  A cloudy call reads its scattering-layer count back unless the
+context has a layer capacity (crtm_set_cloudy_layer_capacity): without one it
+fails cleanly under capture, with one it is captured and replayed like the
+clear-sky call, and profiles whose layers exceed the capacity fail."""
 from __future__ import annotations
 
 import numpy as np
@@ -73,3 +75,123 @@ def test_cloudy_call_refuses_capture(coeff_dir):
                 ctx.forward(dps, emis, n_streams=4)
     finally:
         ctx.destroy()
+
+
+def _cloudy_batch(coeff_dir, dev):
+    synthetic.write_sensor(coeff_dir, "atms_syn")
+    synthetic.write_sensor(coeff_dir, "hirs_syn")
+    rng = np.random.default_rng(411)
+    ps = synthetic.make_profiles(24, rng.integers(70, 111, 24).tolist(), seed=412)
+    ps = synthetic.add_clouds(ps, seed=413, types=(synthetic.WATER_CLOUD, synthetic.ICE_CLOUD), cover="random",
+                              cloudy_fraction=0.6)
+    ps = synthetic.add_aerosols(ps, seed=414, aerosol_fraction=0.5)
+    return ps, api.device_profiles(ps, dev)
+
+
+def _k_keys(kd):
+    return [k for k in ("temperature_k", "absorber_k", "cloud_water_content_k", "cloud_effective_radius_k",
+                        "cloud_fraction_k", "aerosol_concentration_k") if k in kd]
+
+
+def test_capacity_mode_matches_counted_launches(coeff_dir):
+    """Launches sized for a layer capacity give the same bits as launches sized
+    for the counted layers."""
+    import torch
+    print("cloudy K-matrix with a layer capacity vs the counted layers", flush=True)
+    dev = torch.device("cuda:0")
+    ps, dps = _cloudy_batch(coeff_dir, dev)
+    ref = api.crtm_init(["atms_syn", "hirs_syn"], coeff_dir)
+    cap = api.crtm_init(["atms_syn", "hirs_syn"], coeff_dir)
+    try:
+        cap.set_cloudy_layer_capacity(int(ps.layer_offset[-1]))
+        L = ref.n_channels
+        emis = torch.full((24, L), 0.9, dtype=torch.float64, device=dev)
+        btk = torch.ones((24, L), dtype=torch.float64, device=dev)
+        kw = dict(bt_k=btk, precision=api.PREC_FP32_SPLIT, jacobian_f32=True, n_streams=4)
+        r0, k0 = ref.k_matrix(dps, emis, **kw)
+        r1, k1 = cap.k_matrix(dps, emis, **kw)
+        torch.cuda.synchronize()
+        assert int((r0["status"] != 0).sum()) == 0
+        assert torch.equal(r0["brightness_temperature"], r1["brightness_temperature"])
+        assert torch.equal(r0["status"], r1["status"])
+        for key in _k_keys(k0):
+            assert torch.equal(k0[key], k1[key]), key
+    finally:
+        ref.destroy()
+        cap.destroy()
+
+
+def test_captured_cloudy_k_matrix_replays_match_eager(coeff_dir):
+    """A cloudy K-matrix call (clouds and aerosols, two sensors) captured with a
+    layer capacity; each replay on new cloud water contents and temperatures
+    equals an eager call of a context without a capacity."""
+    import torch
+    print("graph capture of a cloudy K-matrix call, 4 replays", flush=True)
+    dev = torch.device("cuda:0")
+    ps, dps = _cloudy_batch(coeff_dir, dev)
+    ref = api.crtm_init(["atms_syn", "hirs_syn"], coeff_dir)
+    ctx = api.crtm_init(["atms_syn", "hirs_syn"], coeff_dir)
+    try:
+        ctx.set_cloudy_layer_capacity(int(ps.layer_offset[-1]))
+        L = ctx.n_channels
+        emis = torch.full((24, L), 0.9, dtype=torch.float64, device=dev)
+        btk = torch.ones((24, L), dtype=torch.float64, device=dev)
+        kw = dict(bt_k=btk, precision=api.PREC_FP32_SPLIT, jacobian_f32=True, n_streams=4, check_input=False)
+        t0, w0 = dps.temperature.clone(), dps.cloud_water_content.clone()
+        s = torch.cuda.Stream()
+        with torch.cuda.stream(s):
+            out = ctx.k_matrix(dps, emis, **kw)          # sizes the work space, binds the stream
+            out = ctx.k_matrix(dps, emis, out=out, **kw)
+        s.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=s):
+            captured = ctx.k_matrix(dps, emis, out=out, **kw)
+        rng = np.random.default_rng(415)
+        for rep in range(4):
+            dps.temperature.copy_(t0 + torch.as_tensor(rng.normal(0.0, 2.0, t0.shape[0]), device=dev))
+            dps.cloud_water_content.copy_(w0 * torch.as_tensor(rng.uniform(0.5, 2.0, tuple(w0.shape)), device=dev))
+            g.replay()
+            torch.cuda.synchronize()
+            got_r = {k: v.clone() for k, v in captured[0].items() if k in ("brightness_temperature", "status")}
+            got_k = {k: captured[1][k].clone() for k in _k_keys(captured[1])}
+            r_e, k_e = ref.k_matrix(dps, emis, **kw)     # eager work between replays
+            torch.cuda.synchronize()
+            assert int((r_e["status"] != 0).sum()) == 0, rep
+            assert torch.equal(got_r["status"], r_e["status"]), rep
+            assert torch.equal(got_r["brightness_temperature"], r_e["brightness_temperature"]), rep
+            for key in got_k:
+                assert torch.equal(got_k[key], k_e[key]), (rep, key)
+        dps.temperature.copy_(t0)
+        dps.cloud_water_content.copy_(w0)
+    finally:
+        ref.destroy()
+        ctx.destroy()
+
+
+def test_layer_capacity_overflow_fails_profiles(coeff_dir):
+    """With room for one scattering layer, profiles whose layers did not find
+    slots fail (nonzero status, NaN brightness temperatures); every other
+    profile equals the counted-layer result."""
+    import torch
+    print("layer capacity overflow", flush=True)
+    dev = torch.device("cuda:0")
+    ps, dps = _cloudy_batch(coeff_dir, dev)
+    ref = api.crtm_init(["atms_syn", "hirs_syn"], coeff_dir)
+    cap = api.crtm_init(["atms_syn", "hirs_syn"], coeff_dir)
+    try:
+        cap.set_cloudy_layer_capacity(1)
+        L = ref.n_channels
+        emis = torch.full((24, L), 0.9, dtype=torch.float64, device=dev)
+        r0 = ref.forward(dps, emis, precision=api.PREC_FP64, n_streams=4)
+        r1 = cap.forward(dps, emis, precision=api.PREC_FP64, n_streams=4)
+        torch.cuda.synchronize()
+        bad = (r1["status"] != 0).cpu().numpy()
+        assert bad.sum() >= 1
+        bt0, bt1 = r0["brightness_temperature"].cpu().numpy(), r1["brightness_temperature"].cpu().numpy()
+        assert np.isnan(bt1[bad]).all()
+        np.testing.assert_array_equal(bt1[~bad], bt0[~bad])
+        with pytest.raises(api.CRTMError, match="capacity"):
+            cap.set_cloudy_layer_capacity(-1)
+    finally:
+        ref.destroy()
+        cap.destroy()
