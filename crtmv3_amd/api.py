@@ -265,14 +265,14 @@ class Context:
         opt.check_input = int(check_input)
         if np.ndim(n_streams) == 0 and not _is_torch(n_streams):
             opt.n_streams = int(n_streams)
-        else:   # Options(m)%n_Streams per profile
-            if memory == MEM_DEVICE:
+        else:   # Options(m)%n_Streams per profile: host memory unless given as a device tensor
+            if _is_torch(n_streams) and n_streams.is_cuda:
                 import torch
-                ns = torch.as_tensor(n_streams, dtype=torch.int32, device=prof.temperature.device).contiguous()
-                keep.append(ns)
+                ns = n_streams.to(torch.int32).contiguous()
             else:
-                ns = np.ascontiguousarray(np.asarray(n_streams), np.int32)
-                keep.append(ns)
+                ns = np.ascontiguousarray(np.asarray(n_streams.cpu() if _is_torch(n_streams) else n_streams),
+                                          np.int32)
+            keep.append(ns)
             if ns.shape[0] != prof.n_profiles:
                 raise CRTMError(f"n_streams: {ns.shape[0]} values for {prof.n_profiles} profiles")
             opt.n_streams = 0

@@ -718,6 +718,8 @@ struct crtm_context {
   // scattering-layer slots of capturable cloudy calls (crtm_set_cloudy_layer_capacity);
   // 0: the count is read back and the launches are sized for it
   int64_t cloud_cap = 0;
+  // device copy of per-profile stream counts given in host memory to a device-memory call
+  DevBuf nsprof;
   // kernel timing (crtm_profile_enable): HIP events on the context stream
   bool profiling = false;
   std::vector<hipEvent_t> ev;      // pairs (start, stop) per recorded launch
@@ -902,6 +904,16 @@ std::vector<int> angle_counts(unsigned mask) {
   for (int z = 1; z < 32; ++z)
     if (mask >> z & 1) v.push_back(z);
   return v;
+}
+// A caller's array in host memory (pageable or pinned) is read directly; in
+// device memory it is copied back with one synchronisation.
+bool host_resident(const void* p) {
+  hipPointerAttribute_t a{};
+  if (hipPointerGetAttributes(&a, p) != hipSuccess) {
+    (void)hipGetLastError();   // pageable host memory is unknown to the runtime
+    return true;
+  }
+  return a.type == hipMemoryTypeHost || a.type == hipMemoryTypeUnregistered;
 }
 bool valid_streams(int n) { return n == 0 || n == 2 || n == 4 || n == 6 || n == 8 || n == 16; }
 
@@ -1138,7 +1150,8 @@ int do_call(crtm_context* ctx, const crtm_atmosphere* atm, const crtm_surface* s
       return fail(ctx, "graph capture: a cloudy call reads its cloud-layer count back unless a layer capacity is "
                        "set (crtm_set_cloudy_layer_capacity)");
     if (cloudy && opt->n_streams_profile)
-      return fail(ctx, "graph capture: per-profile stream counts are read back; give one n_streams for the batch");
+      return fail(ctx, "graph capture: per-profile stream counts are staged through the host; give one n_streams "
+                       "for the batch");
   }
   CloudCall cc;
   cc.on = atm->n_clouds > 0 || atm->n_aerosols > 0;
@@ -1162,7 +1175,7 @@ int do_call(crtm_context* ctx, const crtm_atmosphere* atm, const crtm_surface* s
     cc.cl.nz_mask = angle_mask(opt->n_streams);
     if (opt->n_streams_profile) {
       std::vector<int> nsp(M);
-      if (opt->memory == CRTM_MEM_HOST) {
+      if (opt->memory == CRTM_MEM_HOST || host_resident(opt->n_streams_profile)) {
         std::memcpy(nsp.data(), opt->n_streams_profile, M * sizeof(int));
       } else {
         HIPCHECK(hipMemcpyAsync(nsp.data(), opt->n_streams_profile, M * sizeof(int), hipMemcpyDeviceToHost,
@@ -1407,6 +1420,12 @@ int do_call(crtm_context* ctx, const crtm_atmosphere* atm, const crtm_surface* s
     cc.cl.aconc = atm->aerosol_concentration;
     cc.cl.areff = atm->aerosol_effective_radius;
     cc.cl.ns_prof = opt->n_streams_profile;
+    if (opt->n_streams_profile && host_resident(opt->n_streams_profile)) {   // stage host values for the kernels
+      ctx->nsprof.reserve((size_t)M * sizeof(int));
+      HIPCHECK(hipMemcpyAsync(ctx->nsprof.p, opt->n_streams_profile, (size_t)M * sizeof(int), hipMemcpyHostToDevice,
+                              ctx->stream));
+      cc.cl.ns_prof = ctx->nsprof.as<int>();
+    }
     if (dok) {
       cc.ko.wc_k = k->cloud_water_content_k;
       cc.ko.reff_k = k->cloud_effective_radius_k;

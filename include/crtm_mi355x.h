@@ -127,7 +127,10 @@ typedef struct {
                                     average, overcast); 0 = default (average) */
   const int* n_streams_profile;  /* [M] or NULL: Options(m)%n_Streams per profile (the
                                     reference's Options is per profile), overriding
-                                    n_streams; same values */
+                                    n_streams; same values.  Host memory is read
+                                    directly in either memory mode (and under graph
+                                    capture); device memory is copied back with one
+                                    synchronisation */
 } crtm_options;
 
 /* ---- RTSolution (src/RTSolution/CRTM_RTSolution_Define.f90:201-245) ---- */
