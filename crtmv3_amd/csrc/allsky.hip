@@ -598,11 +598,20 @@ __global__ void aerosol_fail_kernel(DevSensor s, int M, int Ltot, const int* aer
     if (l == 0) status[m] = aer_err[m] == 2 ? 6 : 5;
   }
 }
-// zero fill of n bytes (4-byte aligned), a kernel so that capturable calls
-// record no memset nodes
+// zero fill of n words (4-byte aligned), a kernel so that capturable calls
+// record no memset nodes: 16-byte stores from the first 16-byte boundary,
+// single words before it and after the last full quad
 __global__ void zero_fill_kernel(uint32_t* p, int64_t n_words) {
-  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n_words; t += (int64_t)gridDim.x * blockDim.x)
-    p[t] = 0u;
+  const int64_t lead = (int64_t)((16 - ((uintptr_t)p & 15)) & 15) / 4;   // words before the boundary
+  const int64_t head = lead < n_words ? lead : n_words;
+  const int64_t nq = (n_words - head) / 4;
+  uint4* q = reinterpret_cast<uint4*>(p + head);
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  const int64_t t0 = blockIdx.x * (int64_t)blockDim.x + threadIdx.x;
+  for (int64_t t = t0; t < nq; t += stride) q[t] = make_uint4(0u, 0u, 0u, 0u);
+  if (t0 < head) p[t0] = 0u;
+  const int64_t tail = head + 4 * nq;
+  if (t0 < n_words - tail) p[tail + t0] = 0u;
 }
 
 // ------------------------------------------------------------------ launchers
@@ -621,8 +630,8 @@ void launch_aerosol_check(const DevSensor& s, int M, const DevClouds& cl, const 
 void launch_zero(void* p, size_t bytes, hipStream_t st) {
   const int64_t n = (int64_t)(bytes / 4);
   if (n > 0)
-    hipLaunchKernelGGL(zero_fill_kernel, dim3((unsigned)std::min<int64_t>((n + 255) / 256, 16384)), dim3(256), 0, st,
-                       static_cast<uint32_t*>(p), n);
+    hipLaunchKernelGGL(zero_fill_kernel, dim3((unsigned)std::min<int64_t>((n / 4 + 255) / 256 + 1, 16384)), dim3(256),
+                       0, st, static_cast<uint32_t*>(p), n);
 }
 
 void launch_aerosol_fail(const DevSensor& s, int M, int Ltot, const DevClouds& cl, const CloudWork& cw, int* status,
