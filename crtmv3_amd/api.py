@@ -436,6 +436,7 @@ class Context:
             rts, kd = self.k_matrix(sub, emissivity[m0:m1].contiguous(), bt_k=bt_ad[m0:m1].contiguous(),
                                     precision=precision, jacobian_f32=jacobian_f32, check_input=False,
                                     n_streams=ns, overlap=overlap)
+            _raise_on_failed(rts, m0, "adjoint")
             bt[m0:m1] = rts["brightness_temperature"]
             ad["temperature_ad"][a:b] = torch.sum(kd["temperature_k"], dim=-1, dtype=torch.float64)
             ad["absorber_ad"][:, a:b] = torch.sum(kd["absorber_k"], dim=-1, dtype=torch.float64)
@@ -500,6 +501,7 @@ class Context:
             ns = n_streams[m0:m1] if per_profile_streams else n_streams
             rts, kd = self.k_matrix(sub, emissivity[m0:m1].contiguous(), precision=precision,
                                     jacobian_f32=jacobian_f32, check_input=False, n_streams=ns, overlap=overlap)
+            _raise_on_failed(rts, m0, "tangent_linear")
             bt[m0:m1] = rts["brightness_temperature"]
             acc = bt_tl[m0:m1]
             # layer fields: profile m's rows [loff[m], loff[m+1]) of the chunk
@@ -547,6 +549,18 @@ class Context:
     def synchronize(self):
         if lib().crtm_synchronize(self._h) != SUCCESS:
             self._err("crtm_synchronize")
+
+
+def _raise_on_failed(rts, m0, what):
+    """A device-memory call reports per-profile failures only in its status
+    array; a failed profile has no Jacobian, so no adjoint or gradient may be
+    built from the call (the reference returns FAILURE for it,
+    src/CRTM_Adjoint_Module.f90:404-423)."""
+    st = rts["status"]
+    st = st.cpu().numpy() if _is_torch(st) else np.asarray(st)
+    bad = np.flatnonzero(st)
+    if bad.size:
+        raise CRTMError(f"{what}: profile #{m0 + int(bad[0]) + 1} failed (status {int(st[bad[0]])})")
 
 
 def crtm_init(sensor_ids: Sequence[str], file_path: str) -> Context:

@@ -134,28 +134,60 @@ __global__ void __launch_bounds__(64) cloud_prep_kernel(DevBatch be, DevClouds c
   for (int kk = 0; kk < K; ++kk) {
     bool cloudy = false;
     for (int n = 0; n < nc; ++n) cloudy |= cw.wc[n * NTe + le + kk] > WATER_CONTENT_THRESHOLD;
-    count += cloudy || aerosol_layer(kk);
+    const bool scat = cloudy || aerosol_layer(kk);
+    count += scat;
+    cw.cl_slot[le + kk] = scat ? 0 : -1;   // slot numbers follow in cloud_slot_kernel
   }
-  const int base = count ? atomicAdd(cw.total, count) : 0;
   if (count) atomicMax(cw.total + 1, K);
-  int j = 0;
-  for (int kk = 0; kk < K; ++kk) {
-    bool cloudy = false;
-    for (int n = 0; n < nc; ++n) cloudy |= cw.wc[n * NTe + le + kk] > WATER_CONTENT_THRESHOLD;
-    cloudy = cloudy || aerosol_layer(kk);
-    int slot = -1;
-    if (cloudy && base + j < cw.cap) {
-      slot = base + j;
-      cw.cl_desc[slot] = make_int2(m, kk);
-    }
-    j += cloudy;
-    cw.cl_slot[le + kk] = slot;
-  }
   cw.cl_count[m] = count;
   cw.tcc[m] = fractional ? tcc : 0.0;
   cw.frac[m] = fractional;
   cw.max_reff[m] = max_reff;
-  // a profile whose scattering layers did not all find a slot fails
+}
+
+// Exclusive prefix sum of the per-profile scattering-layer counts, one
+// workgroup: slots are numbered in profile order, so which profiles overflow
+// a layer capacity is the same on every call and every graph replay.
+constexpr int SCAN_THREADS = 1024;
+__global__ void __launch_bounds__(SCAN_THREADS) cloud_scan_kernel(int M, CloudWork cw) {
+  __shared__ int part[SCAN_THREADS];
+  int carry = 0;
+  for (int m0 = 0; m0 < M; m0 += SCAN_THREADS) {
+    const int m = m0 + (int)threadIdx.x;
+    const int v = m < M ? cw.cl_count[m] : 0;
+    part[threadIdx.x] = v;
+    __syncthreads();
+    for (int d = 1; d < SCAN_THREADS; d <<= 1) {   // Hillis-Steele inclusive scan
+      const int add = threadIdx.x >= (unsigned)d ? part[threadIdx.x - d] : 0;
+      __syncthreads();
+      part[threadIdx.x] += add;
+      __syncthreads();
+    }
+    if (m < M) cw.cl_base[m] = carry + part[threadIdx.x] - v;
+    carry += part[SCAN_THREADS - 1];
+    __syncthreads();   // part[] is rewritten by the next chunk
+  }
+  if (threadIdx.x == 0) cw.total[0] = carry;
+}
+
+// Slot numbers of each profile's scattering layers (after the scan); a
+// profile whose layers do not all fit the slot capacity fails.
+__global__ void __launch_bounds__(64) cloud_slot_kernel(DevBatch be, CloudWork cw) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= be.M) return;
+  const int K = be.n_layers[m], count = cw.cl_count[m], base = cw.cl_base[m];
+  const int64_t le = be.loff[m];
+  int j = 0;
+  for (int kk = 0; kk < K && j < count; ++kk) {
+    if (cw.cl_slot[le + kk] < 0) continue;
+    int slot = -1;
+    if (base + j < cw.cap) {
+      slot = base + j;
+      cw.cl_desc[slot] = make_int2(m, kk);
+    }
+    ++j;
+    cw.cl_slot[le + kk] = slot;
+  }
   cw.aer_err[m] = (count > 0 && base + count > cw.cap) ? 2 : 0;
 }
 
@@ -583,20 +615,25 @@ __global__ void aerosol_check_kernel(DevSensor s, int M, DevClouds cl, CloudWork
       if (ty != 0 && aerosol_index(ac, ty) < 0) bad = true;
     }
     if (typed && aerosol_offset(nl) < 0) bad = true;
-    if (bad) cw.aer_err[m] = 1;
+    if (bad) atomicMax(cw.aer_err + m, 1);   // a capacity overflow (2) keeps its code
   }
 }
-// failed profiles: NaN outputs and a FAILURE status
-__global__ void aerosol_fail_kernel(DevSensor s, int M, int Ltot, const int* aer_err, int* status, DevOut o) {
-  const int64_t n = (int64_t)M * s.L;
-  for (int64_t t = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
-    const int m = (int)(t / s.L), l = (int)(t - (int64_t)m * s.L);
-    if (!aer_err[m]) continue;
-    const int64_t ol = (int64_t)m * Ltot + s.chan_offset + l;
-    o.radiance[ol] = __builtin_nan("");
-    o.bt[ol] = __builtin_nan("");
-    if (l == 0) status[m] = aer_err[m] == 2 ? 6 : 5;
+// Call-level profile status, merged after each sensor: the preparation
+// status of the sensor (layer mapping) and the profile failures of the
+// scattering path (aerosol check, layer capacity).  The status only rises,
+// so a failure of one sensor fails the profile for the whole call, as a
+// channel error fails the profile in the reference
+// (src/CRTM_Forward_Module.f90:412-421).
+__global__ void status_merge_kernel(int M, const int* prep_status, const int* aer_err, int* call_status) {
+  const int m = blockIdx.x * blockDim.x + threadIdx.x;
+  if (m >= M) return;
+  int st = call_status[m];
+  if (prep_status[m] > st) st = prep_status[m];
+  if (aer_err && aer_err[m]) {
+    const int a = aer_err[m] == 2 ? 6 : 5;
+    if (a > st) st = a;
   }
+  call_status[m] = st;
 }
 // zero fill of n words (4-byte aligned), a kernel so that capturable calls
 // record no memset nodes: 16-byte stores from the first 16-byte boundary,
@@ -617,6 +654,8 @@ __global__ void zero_fill_kernel(uint32_t* p, int64_t n_words) {
 // ------------------------------------------------------------------ launchers
 void launch_cloud_prep(const DevBatch& be, const DevClouds& cl, const CloudWork& cw, hipStream_t st) {
   hipLaunchKernelGGL(cloud_prep_kernel, dim3((be.M + 63) / 64), dim3(64), 0, st, be, cl, cw);
+  hipLaunchKernelGGL(cloud_scan_kernel, dim3(1), dim3(SCAN_THREADS), 0, st, be.M, cw);
+  hipLaunchKernelGGL(cloud_slot_kernel, dim3((be.M + 63) / 64), dim3(64), 0, st, be, cw);
 }
 
 void launch_aerosol_check(const DevSensor& s, int M, const DevClouds& cl, const CloudWork& cw,
@@ -634,13 +673,9 @@ void launch_zero(void* p, size_t bytes, hipStream_t st) {
                        0, st, static_cast<uint32_t*>(p), n);
 }
 
-void launch_aerosol_fail(const DevSensor& s, int M, int Ltot, const DevClouds& cl, const CloudWork& cw, int* status,
-                         const DevOut& o, hipStream_t st) {
-  // aerosol failures (infrared), and layer-capacity overflows of capacity-sized launches
-  if ((s.sensor_type == 1 || cl.na == 0) && !cw.capmode) return;
-  const int64_t n = (int64_t)M * s.L;
-  hipLaunchKernelGGL(aerosol_fail_kernel, dim3((unsigned)std::min<int64_t>((n + 255) / 256, 8192)), dim3(256), 0,
-                     st, s, M, Ltot, (const int*)cw.aer_err, status, o);
+void launch_status_merge(int M, const int* prep_status, const int* aer_err, int* call_status, hipStream_t st) {
+  hipLaunchKernelGGL(status_merge_kernel, dim3((M + 255) / 256), dim3(256), 0, st, M, prep_status, aer_err,
+                     call_status);
 }
 
 // wavefront kernels for 9 and 10 angles (allsky_wide.hip)

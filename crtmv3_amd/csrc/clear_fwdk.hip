@@ -271,7 +271,7 @@ clear_fwdk_kernel(DevSensor s, DevBatch bt, ProfWork<RT> w, DevOut o) {
   const int m_raw = (int)(tix % ng) * PW + wv;
   const bool in_batch = m_raw < bt.M;
   const int m = in_batch ? m_raw : bt.M - 1;
-  const bool ok_prof = in_batch && w.status[m] == 0;   // wave-uniform
+  const bool ok_prof = in_batch && w.status[m] == 0 && !(o.fail && o.fail[m]);   // wave-uniform
   const int l = b * WAVE + lane;
   const bool lane_in = l < s.L;
   const bool valid = ok_prof && lane_in;
@@ -279,8 +279,19 @@ clear_fwdk_kernel(DevSensor s, DevBatch bt, ProfWork<RT> w, DevOut o) {
   const int lg = s.chan_offset + lc;
   const int64_t ol = (int64_t)m * bt.Ltot + lg;
   if (in_batch && !ok_prof && lane_in) {
+    // a failed profile: NaN radiances, and NaN Jacobian rows so that no
+    // gradient built from them passes silently
     o.radiance[ol] = __builtin_nan("");
     o.bt[ol] = __builtin_nan("");
+    if constexpr (DOK) {
+      const int64_t r0 = bt.out_loff[m], r1 = bt.out_loff[m + 1];
+      const int64_t astr_f = bt.out_NT * bt.Ltot;
+      for (int64_t r = r0; r < r1; ++r) {
+        reinterpret_cast<KT*>(o.t_k)[r * bt.Ltot + lg] = KT(__builtin_nan(""));
+        for (int ja = 0; ja < bt.na; ++ja)
+          reinterpret_cast<KT*>(o.a_k)[ja * astr_f + r * bt.Ltot + lg] = KT(__builtin_nan(""));
+      }
+    }
   }
   const int K = ok_prof ? bt.n_layers[m] : 0;
   const int64_t loff = bt.loff[m];

@@ -234,6 +234,8 @@ struct DevOut {
   void* tau_g;
   const int* tau_mask;     // [M]: write the profile's ODs when > 0
   KSeeds ks;               // all-sky K-matrix seeds (ks.mask null: none)
+  const int* fail;         // [M] or null: nonzero fails the profile (scattering-path
+                           // failures found before the clear-sky kernels run)
 };
 
 // Cloud Jacobians of the K-matrix, caller's layer layout, element type KT:
@@ -310,6 +312,7 @@ struct CloudWork {
   double* wc;           // [nc][NT] cleaned, cover-scaled water content
   double* reff;         // [nc][NT]
   int* cl_count;        // [M] cloudy layers of profile m
+  int* cl_base;         // [M] first slot of profile m (exclusive prefix sum of cl_count)
   int* cl_slot;         // [NT] cloudy-layer id of (m, k), -1 for clear layers
   int2* cl_desc;        // [cap] (m, k) of cloudy layer c
   int* total;           // [2] cloudy layers in the batch, largest extended layer count of a cloudy profile

@@ -123,6 +123,52 @@ def test_aerosol_failures(ir):
     assert np.isfinite(out["brightness_temperature"]).all()
 
 
+def test_aerosol_failure_is_sticky_across_sensors(coeff_dir):
+    """A profile that fails in an infrared sensor's aerosol step fails for the
+    whole call, whatever the sensor order: FAILURE, status set, NaN
+    brightness temperatures on every channel of every sensor, and NaN
+    Jacobian rows; profiles without aerosols are computed."""
+    import torch
+    print("aerosol failure across sensors (hirs before atms), two streams", flush=True)
+    synthetic.write_sensor(coeff_dir, "atms_syn")
+    synthetic.write_sensor(coeff_dir, "hirs_syn")
+    ctx = api.crtm_init(["hirs_syn", "atms_syn"], coeff_dir)
+    try:
+        ps = synthetic.add_aerosols(synthetic.make_profiles(6, 80, seed=351), seed=352, aerosol_fraction=0.5)
+        fails = (ps.aerosol_type != 0).any(axis=1)
+        assert fails.any() and (~fails).any()
+        L = ctx.n_channels
+        eps = np.full((6, L), 0.9)
+        with pytest.raises(api.CRTMError, match="failed"):
+            ctx.forward(ps, eps, precision=api.PREC_FP64, n_streams=2)
+        dev = torch.device("cuda:0")
+        dps = api.device_profiles(ps, dev)
+        demis = torch.full((6, L), 0.9, dtype=torch.float64, device=dev)
+        for prec in (api.PREC_FP64, api.PREC_FP32_SPLIT):
+            rts = ctx.forward(dps, demis, precision=prec, n_streams=2)
+            rk, kd = ctx.k_matrix(dps, demis, precision=prec, n_streams=2)
+            torch.cuda.synchronize()
+            for r in (rts, rk):
+                st = r["status"].cpu().numpy()
+                bt = r["brightness_temperature"].cpu().numpy()
+                np.testing.assert_array_equal(st != 0, fails)
+                assert np.isnan(bt[fails]).all()
+                assert np.isfinite(bt[~fails]).all()
+            tk = kd["temperature_k"].cpu().numpy()
+            ak = kd["absorber_k"].cpu().numpy()
+            for m in range(6):
+                a, b = int(ps.layer_offset[m]), int(ps.layer_offset[m + 1])
+                if fails[m]:
+                    assert np.isnan(tk[a:b]).all() and np.isnan(ak[:, a:b]).all(), m
+                else:
+                    assert np.isfinite(tk[a:b]).all() and np.isfinite(ak[:, a:b]).all(), m
+            with pytest.raises(api.CRTMError, match="failed"):
+                ctx.adjoint(dps, demis, torch.ones((6, L), dtype=torch.float64, device=dev), precision=prec,
+                            n_streams=2)
+    finally:
+        ctx.destroy()
+
+
 def test_empty_aerosol_slots_two_streams(coeff_dir, oracle_mod):
     """Profiles whose aerosol slots are all unused (type 0) run at two
     streams next to profiles with aerosols at four, against the oracle."""

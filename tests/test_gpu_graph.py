@@ -168,25 +168,54 @@ def test_captured_cloudy_k_matrix_replays_match_eager(coeff_dir):
         ctx.destroy()
 
 
+def _scattering_layer_counts(ps):
+    """Scattering layers per profile as the cloud preparation counts them:
+    a typed cloud with water content above 1e-6 kg/m^2 in a layer whose cloud
+    fraction is at least 1e-12, or a typed aerosol above 1e-9 kg/m^2."""
+    M = ps.n_profiles
+    counts = np.zeros(M, np.int64)
+    for m in range(M):
+        a, b = int(ps.layer_offset[m]), int(ps.layer_offset[m + 1])
+        scat = np.zeros(b - a, bool)
+        for n in range(ps.cloud_type.shape[1]):
+            if 1 <= ps.cloud_type[m, n] <= 6:
+                scat |= (ps.cloud_water_content[n, a:b] > 1e-6) & (ps.cloud_fraction[a:b] >= 1e-12)
+        for n in range(ps.aerosol_type.shape[1]):
+            if ps.aerosol_type[m, n] != 0:
+                scat |= ps.aerosol_concentration[n, a:b] > 1e-9
+        counts[m] = int(scat.sum())
+    return counts
+
+
 def test_layer_capacity_overflow_fails_profiles(coeff_dir):
-    """With room for one scattering layer, profiles whose layers did not find
-    slots fail (nonzero status, NaN brightness temperatures); every other
-    profile equals the counted-layer result."""
+    """With room for the scattering layers of the first half of the batch,
+    slots are numbered in profile order: exactly the profiles whose layers end
+    beyond the capacity fail (status 6, NaN brightness temperatures), the same
+    ones on every call; every other profile equals the counted-layer result."""
     import torch
     print("layer capacity overflow", flush=True)
     dev = torch.device("cuda:0")
     ps, dps = _cloudy_batch(coeff_dir, dev)
+    counts = _scattering_layer_counts(ps)
+    base = np.concatenate([[0], np.cumsum(counts)[:-1]])
+    capacity = int(np.cumsum(counts)[11])          # the first 12 profiles fit
+    expect_bad = (counts > 0) & (base + counts > capacity)
+    assert expect_bad.any() and (~expect_bad).any()
     ref = api.crtm_init(["atms_syn", "hirs_syn"], coeff_dir)
     cap = api.crtm_init(["atms_syn", "hirs_syn"], coeff_dir)
     try:
-        cap.set_cloudy_layer_capacity(1)
+        cap.set_cloudy_layer_capacity(capacity)
         L = ref.n_channels
         emis = torch.full((24, L), 0.9, dtype=torch.float64, device=dev)
         r0 = ref.forward(dps, emis, precision=api.PREC_FP64, n_streams=4)
         r1 = cap.forward(dps, emis, precision=api.PREC_FP64, n_streams=4)
+        r2 = cap.forward(dps, emis, precision=api.PREC_FP64, n_streams=4)
         torch.cuda.synchronize()
-        bad = (r1["status"] != 0).cpu().numpy()
-        assert bad.sum() >= 1
+        st1, st2 = r1["status"].cpu().numpy(), r2["status"].cpu().numpy()
+        np.testing.assert_array_equal(st1 != 0, expect_bad)
+        np.testing.assert_array_equal(st1, st2)
+        assert (st1[expect_bad] == 6).all()
+        bad = expect_bad
         bt0, bt1 = r0["brightness_temperature"].cpu().numpy(), r1["brightness_temperature"].cpu().numpy()
         assert np.isnan(bt1[bad]).all()
         np.testing.assert_array_equal(bt1[~bad], bt0[~bad])
