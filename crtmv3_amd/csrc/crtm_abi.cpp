@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cstdint>
 #include <cmath>
 #include <cstdarg>
@@ -11,6 +12,7 @@
 #include <cstring>
 #include <fstream>
 #include <memory>
+#include <mutex>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -42,8 +44,7 @@ int allsky_sweep_ad_wide_blocks(int64_t n_pc);
 int64_t wide_doubling_tape_elems(int64_t n_items);
 void launch_aerosol_check(const DevSensor&, int M, const DevClouds&, const CloudWork&, const DevCloudCoeff&,
                           hipStream_t);
-void launch_aerosol_fail(const DevSensor&, int M, int Ltot, const DevClouds&, const CloudWork&, int* status,
-                         const DevOut&, hipStream_t);
+void launch_status_merge(int M, const int* prep_status, const int* aer_err, int* call_status, hipStream_t st);
 void launch_zero(void* p, size_t bytes, hipStream_t st);
 }  // namespace crtm
 
@@ -709,7 +710,16 @@ struct crtm_context {
   std::vector<int> aerosol_types;   // type ids of the AerosolCoeff tables
   std::vector<std::unique_ptr<DevBuf>> cloud_bufs;
   DevCloudCoeff cc{};
-  DevBuf cwc, creff, ccount, cslot, cdesc, ctotal, ctcc, cfrac, cmaxr, ctau, cscr, caerr, cwide;
+  DevBuf cwc, creff, ccount, cbase, cslot, cdesc, ctotal, ctcc, cfrac, cmaxr, ctau, cscr, caerr, cwide;
+  // call-level profile status: the per-sensor preparation status and the
+  // scattering-path failures merged after each sensor (only ever raised)
+  DevBuf cstatus;
+  int* call_status = nullptr;   // the current call's call-level status (pool or scratch array)
+  int* prep_status = nullptr;   // preparation status of the sensor being run
+  // crtm_k_matrix_noalloc: work space is carved from the device's scratch
+  // array (arena non-null) instead of the pool
+  char* arena = nullptr;
+  size_t arena_cap = 0, arena_used = 0;
   // all-sky K-matrix: layer adjoints, per-thread adding tape, seeds for the clear-sky K kernel
   DevBuf cscr_ad, cwide_ad, ctape, ctape_w, ctape_d, ks_tau, ks_b, ks_t, ks_ch;
   // smallest angle count run by the wavefront (matrix-core) kernels
@@ -745,35 +755,55 @@ int fail(crtm_context* ctx, const std::string& msg) {
   return CRTM_FAILURE;
 }
 
+// Work space of a call: a pool buffer grown on demand, or in a no-allocation
+// call the next 256-byte-aligned piece of the device's scratch array.
+void* take(crtm_context* c, DevBuf& b, size_t bytes) {
+  if (!c->arena) {
+    b.reserve(bytes);
+    return b.p;
+  }
+  const size_t at = (c->arena_used + 255) & ~size_t(255);
+  if (at + bytes > c->arena_cap)
+    throw CrtmError(fmt("work space of %zu bytes beyond the %zu-byte scratch array", at + bytes, c->arena_cap));
+  c->arena_used = at + bytes;
+  return c->arena + at;
+}
+
+// Bytes of each per-profile work array of one sensor's preparation and fused
+// kernels (ProfWork), in the order make_work takes them.
+constexpr int N_WORK = 13;
+template <class RT>
+std::array<size_t, N_WORK> work_sizes(int M, int Ko, int nx, int max_k) {
+  const size_t m = (size_t)M, ko = (size_t)Ko, mk = (size_t)max_k;
+  return {m * ko * nx * sizeof(RT),       m * ko * NU * sizeof(RT),        m * ko * NO * sizeof(RT),
+          m * ko * 3 * sizeof(int),       m * NZ_MAX * sizeof(RT),         m * (mk + 1) * sizeof(int),
+          m * (mk + 1) * sizeof(RT),      m * 4 * sizeof(double),          m * mk * MAP_WIN * sizeof(RT),
+          m * ko * 2 * sizeof(int),       m * sizeof(int),                 m * NZ_MAX * sizeof(double),
+          (3 * mk + 1) * m * sizeof(double)};
+}
+
 template <class RT>
 ProfWork<RT> make_work(crtm_context* c, int M, int Ko, int nx, int max_k) {
+  const std::array<size_t, N_WORK> sz = work_sizes<RT>(M, Ko, nx, max_k);
+  DevBuf* bufs[N_WORK] = {&c->wx,    &c->wu,   &c->wo,     &c->wacci,   &c->waccw,   &c->wuidx, &c->wufrac,
+                          &c->wgeo,  &c->wuwin, &c->wfrange, &c->wstatus, &c->waccw64, &c->wlm};
+  void* p[N_WORK];
+  for (int i = 0; i < N_WORK; ++i) p[i] = take(c, *bufs[i], sz[i]);
   ProfWork<RT> w{};
-  c->wx.reserve((size_t)M * Ko * nx * sizeof(RT));
-  c->wu.reserve((size_t)M * Ko * NU * sizeof(RT));
-  c->wo.reserve((size_t)M * Ko * NO * sizeof(RT));
-  c->wacci.reserve((size_t)M * Ko * 3 * sizeof(int));
-  c->waccw.reserve((size_t)M * NZ_MAX * sizeof(RT));
-  c->wuidx.reserve((size_t)M * (max_k + 1) * sizeof(int));
-  c->wufrac.reserve((size_t)M * (max_k + 1) * sizeof(RT));
-  c->wgeo.reserve((size_t)M * 4 * sizeof(double));
-  c->wuwin.reserve((size_t)M * max_k * MAP_WIN * sizeof(RT));
-  c->wfrange.reserve((size_t)M * Ko * 2 * sizeof(int));
-  c->wstatus.reserve((size_t)M * sizeof(int));
-  c->waccw64.reserve((size_t)M * NZ_MAX * sizeof(double));
-  c->wlm.reserve((size_t)(3 * max_k + 1) * M * sizeof(double));
-  w.accw64 = c->waccw64.as<double>();
-  w.lm = c->wlm.as<double>();
-  w.x = c->wx.as<RT>();
-  w.u = c->wu.as<RT>();
-  w.o = c->wo.as<RT>();
-  w.acc_i = c->wacci.as<int>();
-  w.acc_w = c->waccw.as<RT>();
-  w.uidx = c->wuidx.as<int>();
-  w.ufrac = c->wufrac.as<RT>();
-  w.geo = c->wgeo.as<double>();
-  w.uwin = c->wuwin.as<RT>();
-  w.frange = c->wfrange.as<int>();
-  w.status = c->wstatus.as<int>();
+  w.x = static_cast<RT*>(p[0]);
+  w.u = static_cast<RT*>(p[1]);
+  w.o = static_cast<RT*>(p[2]);
+  w.acc_i = static_cast<int*>(p[3]);
+  w.acc_w = static_cast<RT*>(p[4]);
+  w.uidx = static_cast<int*>(p[5]);
+  w.ufrac = static_cast<RT*>(p[6]);
+  w.geo = static_cast<double*>(p[7]);
+  w.uwin = static_cast<RT*>(p[8]);
+  w.frange = static_cast<int*>(p[9]);
+  w.status = static_cast<int*>(p[10]);
+  w.accw64 = static_cast<double*>(p[11]);
+  w.lm = static_cast<double*>(p[12]);
+  c->prep_status = w.status;
   return w;
 }
 
@@ -856,6 +886,7 @@ int prepare_clouds(crtm_context* c, const DevBatch& be, const DevClouds& cl, Clo
   c->cwc.reserve(std::max<size_t>((size_t)cl.nc * NTe * sizeof(double), 16));
   c->creff.reserve(std::max<size_t>((size_t)cl.nc * NTe * sizeof(double), 16));
   c->ccount.reserve(M * sizeof(int));
+  c->cbase.reserve(M * sizeof(int));
   c->cslot.reserve(NTe * sizeof(int));
   c->cdesc.reserve(std::max<int64_t>(be.out_NT, 1) * sizeof(int2));
   c->ctotal.reserve(2 * sizeof(int));
@@ -866,6 +897,7 @@ int prepare_clouds(crtm_context* c, const DevBatch& be, const DevClouds& cl, Clo
   cw.wc = c->cwc.as<double>();
   cw.reff = c->creff.as<double>();
   cw.cl_count = c->ccount.as<int>();
+  cw.cl_base = c->cbase.as<int>();
   cw.cl_slot = c->cslot.as<int>();
   cw.cl_desc = c->cdesc.as<int2>();
   cw.total = c->ctotal.as<int>();
@@ -877,6 +909,9 @@ int prepare_clouds(crtm_context* c, const DevBatch& be, const DevClouds& cl, Clo
   cw.cap = (int)std::min<int64_t>(cw.capmode ? std::min<int64_t>(c->cloud_cap, be.out_NT) : be.out_NT, INT32_MAX);
   launch_zero(cw.total, 2 * sizeof(int), c->stream);
   launch_cloud_prep(be, cl, cw, c->stream);
+  // aerosol failures of every infrared sensor before any sensor runs, so the
+  // profile fails on all sensors of the context whatever their order
+  for (auto& S : c->sensors) launch_aerosol_check(S->d, M, cl, cw, c->cc, c->stream);
   HIPCHECK(hipGetLastError());
   if (cw.capmode) {
     if (max_k_cloudy) *max_k_cloudy = be.max_k;
@@ -958,10 +993,8 @@ void run_allsky(crtm_context* c, Sensor& S, const DevBatch& be, const DevClouds&
     if (c->profiling) c->mark();
   };
   timed([&] {
-    launch_aerosol_check(S.d, be.M, cl, cw, c->cc, c->stream);
     launch_allsky<RT>(S.d, be, cl, cw, c->cc, sc, reinterpret_cast<const RT*>(o.tau_g), w.geo, (int)items,
                       nz.data(), (int)nz.size(), o, ao, c->stream);
-    launch_aerosol_fail(S.d, be.M, be.Ltot, cl, cw, w.status, o, c->stream);
   });
   HIPCHECK(hipGetLastError());
 }
@@ -979,20 +1012,26 @@ struct Stager {
   }
 };
 
+// Bytes of the arrays of the batch extended by AddLayers, in add_layers' order.
+constexpr int N_EXT = 7;
+std::array<size_t, N_EXT> ext_sizes(int M, int64_t NT, int na) {
+  const size_t NTe = (size_t)(NT + (int64_t)M * N_EXTRA_MAX);
+  return {M * sizeof(int),          (M + 1) * sizeof(int64_t), M * sizeof(int),  (NTe + M) * sizeof(double),
+          NTe * sizeof(double),     NTe * sizeof(double),      std::max(na, 1) * NTe * sizeof(double)};
+}
+
 // CRTM_Atmosphere_AddLayers for the whole batch: the returned batch
 // describes the extended profiles; Jacobian outputs keep the caller's layout.
 DevBatch add_layers(crtm_context* c, const DevBatch& b, const int* user_ids) {
   const int M = b.M;
   const int64_t NTe = b.NT + (int64_t)M * N_EXTRA_MAX;
-  c->xnl.reserve(M * sizeof(int));
-  c->xloff.reserve((M + 1) * sizeof(int64_t));
-  c->xnx.reserve(M * sizeof(int));
-  c->xlp.reserve((NTe + M) * sizeof(double));
-  c->xp.reserve(NTe * sizeof(double));
-  c->xt.reserve(NTe * sizeof(double));
-  c->xa.reserve(std::max(b.na, 1) * NTe * sizeof(double));
-  ExtBatch e{c->xnl.as<int>(), c->xloff.as<int64_t>(), c->xnx.as<int>(), c->xlp.as<double>(), c->xp.as<double>(),
-             c->xt.as<double>(), c->xa.as<double>(), NTe};
+  const std::array<size_t, N_EXT> sz = ext_sizes(M, b.NT, b.na);
+  DevBuf* bufs[N_EXT] = {&c->xnl, &c->xloff, &c->xnx, &c->xlp, &c->xp, &c->xt, &c->xa};
+  void* p[N_EXT];
+  for (int i = 0; i < N_EXT; ++i) p[i] = take(c, *bufs[i], sz[i]);
+  ExtBatch e{static_cast<int*>(p[0]),    static_cast<int64_t*>(p[1]), static_cast<int*>(p[2]),
+             static_cast<double*>(p[3]), static_cast<double*>(p[4]),  static_cast<double*>(p[5]),
+             static_cast<double*>(p[6]), NTe};
   launch_add_layers(b, e, c->clim.as<double>(), user_ids, c->stream);
   HIPCHECK(hipGetLastError());
   DevBatch be = b;
@@ -1071,11 +1110,9 @@ void run_sensor_k_allsky(crtm_context* c, Sensor& S, const DevBatch& be, const D
     c->mark();
     c->ev_kind.push_back(2);
   }
-  launch_aerosol_check(S.d, be.M, cc.cl, cw, c->cc, c->stream);
   launch_allsky_k<RT, KT>(S.d, be, cc.cl, cw, c->cc, sc, sa, reinterpret_cast<const RT*>(o1.tau_g), w.geo,
                           (int)items, nz.data(), (int)nz.size(), o2, cc.ao, cc.ko, c->ctape.as<RT>(), tape_lanes, te,
                           c->ctape_w.as<RT>(), kmax, c->ctape_d.as<RT>(), c->stream);
-  launch_aerosol_fail(S.d, be.M, be.Ltot, cc.cl, cw, w.status, o2, c->stream);
   launch_clear_copy(S.d, be.M, be.Ltot, cc.cw.cl_count, o1, cc.ao, c->stream);   // profiles without cloudy layers
   if (c->profiling) c->mark();
   HIPCHECK(hipGetLastError());
@@ -1084,9 +1121,22 @@ void run_sensor_k_allsky(crtm_context* c, Sensor& S, const DevBatch& be, const D
 
 template <class RT, class KT, bool DOK>
 void run_sensors(crtm_context* c, const DevBatch& be, DevOut o, const CloudCall& cc) {
+  c->call_status = static_cast<int*>(take(c, c->cstatus, (size_t)be.M * sizeof(int)));
+  launch_zero(c->call_status, (size_t)be.M * sizeof(int), c->stream);
+  const int* aer_err = cc.on ? cc.cw.aer_err : nullptr;
+  o.fail = aer_err;
+  auto merge = [&] {
+    launch_status_merge(be.M, c->prep_status, aer_err, c->call_status, c->stream);
+    HIPCHECK(hipGetLastError());
+  };
   for (auto& S : c->sensors) {
+    // the sensor's work space is dead once its kernels are enqueued (stream
+    // order), so the next sensor reuses it
+    const size_t mark = c->arena_used;
     if (DOK && cc.on) {
       run_sensor_k_allsky<RT, KT>(c, *S, be, o, cc);
+      merge();
+      c->arena_used = mark;
       continue;
     }
     if (cc.on) {
@@ -1097,6 +1147,8 @@ void run_sensors(crtm_context* c, const DevBatch& be, DevOut o, const CloudCall&
     ProfWork<RT> w = run_sensor<RT, KT, DOK>(c, *S, be, o);
     if (cc.on) run_allsky<RT>(c, *S, be, cc.cl, cc.cw, cc.total, w, o, cc.ao);
     else launch_clear_copy(S->d, be.M, be.Ltot, nullptr, o, cc.ao, c->stream);   // R_Clear = Radiance without clouds
+    merge();
+    c->arena_used = mark;
   }
 }
 
@@ -1360,18 +1412,19 @@ int do_call(crtm_context* ctx, const crtm_atmosphere* atm, const crtm_surface* s
     const DevBatch be = add_layers(ctx, b, abs_id.data());
     if (cc.on) cc.total = prepare_clouds(ctx, be, cc.cl, cc.cw, &cc.max_k);
     dispatch(ctx, be, o, cc, split, kf32, dok);
-    if (rts->status) {
-      std::vector<int> st(M);
-      HIPCHECK(hipMemcpyAsync(st.data(), ctx->wstatus.p, M * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
-      HIPCHECK(hipStreamSynchronize(ctx->stream));
-      for (int m = 0; m < M; ++m) rts->status[m] = std::max(rts->status[m], st[m] ? CRTM_FAILURE : CRTM_SUCCESS);
-    }
+    // per-profile failures: the outputs of the good profiles are written and
+    // the call returns FAILURE (src/CRTM_Forward_Module.f90:412-421)
+    std::vector<int> pst(M);
+    HIPCHECK(hipMemcpyAsync(pst.data(), ctx->call_status, M * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
     for (size_t i = 0; i < outs.size(); ++i)
       HIPCHECK(hipMemcpyAsync(outs[i].h, base + ooff[i], outs[i].n, hipMemcpyDeviceToHost, ctx->stream));
     HIPCHECK(hipStreamSynchronize(ctx->stream));
-    if (rts->status)
-      for (int m = 0; m < M; ++m)
-        if (rts->status[m] != CRTM_SUCCESS) return fail(ctx, fmt("profile #%d failed", m + 1));
+    int first_bad = -1;
+    for (int m = 0; m < M; ++m) {
+      if (rts->status) rts->status[m] = std::max(rts->status[m], pst[m] ? CRTM_FAILURE : CRTM_SUCCESS);
+      if (pst[m] && first_bad < 0) first_bad = m;
+    }
+    if (first_bad >= 0) return fail(ctx, fmt("profile #%d failed", first_bad + 1));
     return CRTM_SUCCESS;
   }
   // device mode: pointers are used as given, asynchronously on the context stream
@@ -1439,7 +1492,7 @@ int do_call(crtm_context* ctx, const crtm_atmosphere* atm, const crtm_surface* s
   if (cc.on) cc.total = prepare_clouds(ctx, be, cc.cl, cc.cw, &cc.max_k);
   dispatch(ctx, be, o, cc, split, kf32, dok);
   if (rts->status)
-    HIPCHECK(hipMemcpyAsync(rts->status, ctx->wstatus.p, M * sizeof(int), hipMemcpyDeviceToDevice, ctx->stream));
+    HIPCHECK(hipMemcpyAsync(rts->status, ctx->call_status, M * sizeof(int), hipMemcpyDeviceToDevice, ctx->stream));
   return CRTM_SUCCESS;
 }
 
