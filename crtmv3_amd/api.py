@@ -104,6 +104,7 @@ def lib():
                                    ctypes.POINTER(CGeometry), ctypes.POINTER(COptions),
                                    ctypes.POINTER(CRTSolution)]
         L.crtm_k_matrix.argtypes = L.crtm_forward.argtypes + [ctypes.POINTER(CKOutput)]
+        L.crtm_k_matrix_noalloc.argtypes = L.crtm_k_matrix.argtypes
         L.crtm_synchronize.argtypes = [_vp]
         L.crtm_set_stream.argtypes = [_vp, _vp, ctypes.c_int]
         L.crtm_set_wavefront_angles.argtypes = [_vp, ctypes.c_int]
@@ -318,7 +319,8 @@ class Context:
         return out
 
     def k_matrix(self, prof, emissivity, bt_k=None, rad_k=None, precision=PREC_FP64,
-                 jacobian_f32=False, check_input=True, out=None, n_streams=0, overlap=0, extras=False):
+                 jacobian_f32=False, check_input=True, out=None, n_streams=0, overlap=0, extras=False,
+                 no_alloc=False):
         """CRTM_K_Matrix: returns (RTSolution dict, K dict).
 
         K layout: temperature_k [NT, L] (profile m rows layer_offset[m]:...),
@@ -329,7 +331,9 @@ class Context:
         to 0).  `out` = (rts, k) dicts from a previous call are reused as the
         output buffers (every element is overwritten).  n_streams / overlap as
         in forward(); extras adds R_Clear, Tb_Clear and the cloud cover of a
-        cloudy call to the RTSolution dict."""
+        cloudy call to the RTSolution dict.  no_alloc calls
+        crtm_k_matrix_noalloc (clear-sky device batches: the library's work
+        space comes from its fixed scratch array, nothing is allocated)."""
         memory = MEM_DEVICE if _is_torch(prof.temperature) else MEM_HOST
         if memory == MEM_DEVICE:
             self._use_torch_stream()
@@ -373,9 +377,10 @@ class Context:
             kd["radiance_k"] = rad_k.clone() if _is_torch(rad_k) else np.array(rad_k, np.float64)
         rts = CRTSolution(**{k: _ptr(v) for k, v in rts_d.items()})
         ko = CKOutput(brightness_temperature_k=_ptr(bt_k), **{k: _ptr(v) for k, v in kd.items()})
-        st = lib().crtm_k_matrix(self._h, atm, sfc, geo, opt, rts, ko)
+        entry = "crtm_k_matrix_noalloc" if no_alloc else "crtm_k_matrix"
+        st = getattr(lib(), entry)(self._h, atm, sfc, geo, opt, rts, ko)
         if st != SUCCESS:
-            self._err("crtm_k_matrix")
+            self._err(entry)
         return rts_d, kd
 
     def adjoint(self, prof, emissivity, bt_ad, precision=PREC_FP64, n_streams=0, overlap=0, jacobian_f32=None,

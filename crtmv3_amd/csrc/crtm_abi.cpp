@@ -690,6 +690,35 @@ std::vector<double> gauss_table() {
 }  // namespace
 
 // ---------------------------------------------------------------------------
+// Scratch array of the no-allocation entry point (crtm_k_matrix_noalloc).
+// Each GPU the library runs on holds its own copy from module load until the
+// process ends.  Host code never uses the array's name as an address: the
+// device address comes from hipGetSymbolAddress, once per device, at
+// crtm_init with that device current.  Every no-allocation call on a device
+// shares that device's copy; overlapping calls on one device race.
+// ---------------------------------------------------------------------------
+constexpr size_t NOALLOC_SCRATCH_BYTES = size_t(128) << 20;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+constexpr int MAX_DEVICES = 64;
+static std::mutex g_scratch_mu;
+static char* g_scratch_ptr[MAX_DEVICES] = {};
+
+// Device address of the current device's scratch array (set-up, once per device).
+static void bind_scratch(int device) {
+  if (device < 0 || device >= MAX_DEVICES) return;
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr[device]) return;
+  void* ptr = nullptr;
+  const hipError_t rc = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (rc != hipSuccess || !ptr) {
+    std::fprintf(stderr, "CRTM_MI355X: hipGetSymbolAddress(scratch) on device %d: %s\n", device,
+                 hipGetErrorString(rc));
+    return;
+  }
+  g_scratch_ptr[device] = static_cast<char*>(ptr);
+}
+
+// ---------------------------------------------------------------------------
 // Context
 // ---------------------------------------------------------------------------
 struct crtm_context {
@@ -1177,6 +1206,28 @@ void zero_cloud_k(crtm_context* ctx, const CloudKOut& ko, int nc, int na, int64_
   HIPCHECK(hipGetLastError());
 }
 
+// Scratch bytes a clear-sky call takes, in the order the call carves them:
+// the AddLayers batch, the call-level status, then the largest sensor's work
+// (sensors run one after another on the same space).
+size_t noalloc_bytes(const crtm_context* c, int M, int max_k, int64_t NT, int na, bool split) {
+  size_t used = 0;
+  auto add = [&](size_t& u, size_t b) { u = ((u + 255) & ~size_t(255)) + b; };
+  for (size_t b : ext_sizes(M, NT, na)) add(used, b);
+  add(used, (size_t)M * sizeof(int));
+  size_t worst = used;
+  const int max_ke = max_k + N_EXTRA_MAX;
+  for (const auto& S : c->sensors) {
+    const int nx = S->d.group == 1 ? 89 : S->d.group == 2 ? 50 : 21;
+    size_t u = used;
+    if (split)
+      for (size_t b : work_sizes<float>(M, S->d.Ko, nx, max_ke)) add(u, b);
+    else
+      for (size_t b : work_sizes<double>(M, S->d.Ko, nx, max_ke)) add(u, b);
+    worst = std::max(worst, u);
+  }
+  return worst;
+}
+
 int do_call(crtm_context* ctx, const crtm_atmosphere* atm, const crtm_surface* sfc, const crtm_geometry* geo,
             const crtm_options* opt, crtm_rtsolution* rts, crtm_k_output* k) {
   if (!ctx) return fail(nullptr, "null context");
@@ -1509,6 +1560,7 @@ int crtm_init(int n_sensors, const char* const* sensor_ids, const char* file_pat
   std::unique_ptr<crtm_context> ctx(new crtm_context);
   try {
     HIPCHECK(hipGetDevice(&ctx->device));
+    bind_scratch(ctx->device);   // the device is current here
     HIPCHECK(hipStreamCreateWithFlags(&ctx->own_stream, hipStreamNonBlocking));
     ctx->stream = ctx->own_stream;
     std::string dir = file_path ? file_path : ".";
@@ -1617,6 +1669,46 @@ int crtm_k_matrix(crtm_context* ctx, const crtm_atmosphere* atm, const crtm_surf
   } catch (const std::exception& e) {
     return fail(ctx, e.what());
   }
+}
+
+int crtm_k_matrix_noalloc(crtm_context* ctx, const crtm_atmosphere* atm, const crtm_surface* sfc,
+                          const crtm_geometry* geo, const crtm_options* opt, crtm_rtsolution* rts,
+                          crtm_k_output* k) {
+  if (!ctx) return fail(nullptr, "null context");
+  if (!atm || !sfc || !geo || !opt || !rts || !k) return fail(ctx, "crtm_k_matrix_noalloc: null argument");
+  if (opt->memory != CRTM_MEM_DEVICE)
+    return fail(ctx, "crtm_k_matrix_noalloc: device-memory calls only (host calls stage through allocated memory)");
+  if (atm->n_clouds > 0 || atm->n_aerosols > 0)
+    return fail(ctx, "crtm_k_matrix_noalloc: clear-sky calls only (the scattering path sizes its work space from "
+                     "the batch's cloudy layers)");
+  if (atm->max_layers <= 0 || atm->max_layers > CRTM_MAX_N_LAYERS || atm->total_layers <= 0)
+    return fail(ctx, "crtm_k_matrix_noalloc: give max_layers (1..200) and total_layers");
+  if (ctx->profiling)
+    return fail(ctx, "crtm_k_matrix_noalloc: kernel timing creates HIP events; disable crtm_profile_enable");
+  const int dev = ctx->device;
+  char* base = (dev >= 0 && dev < MAX_DEVICES) ? g_scratch_ptr[dev] : nullptr;
+  if (!base) return fail(ctx, fmt("crtm_k_matrix_noalloc: no scratch array address on device %d", dev));
+  const int M = atm->n_profiles;
+  if (M <= 0) return CRTM_SUCCESS;
+  const size_t need = noalloc_bytes(ctx, M, atm->max_layers, atm->total_layers, atm->n_absorbers,
+                                    opt->precision == CRTM_PREC_FP32_SPLIT);
+  if (need > NOALLOC_SCRATCH_BYTES)
+    return fail(ctx, fmt("crtm_k_matrix_noalloc: %d profiles of up to %d layers need %.1f MiB of work space, more "
+                         "than the %zu MiB scratch array; split the batch",
+                         M, atm->max_layers, need / 1048576.0, NOALLOC_SCRATCH_BYTES >> 20));
+  ctx->arena = base;
+  ctx->arena_cap = NOALLOC_SCRATCH_BYTES;
+  ctx->arena_used = 0;
+  int st;
+  try {
+    st = do_call(ctx, atm, sfc, geo, opt, rts, k);
+  } catch (const std::exception& e) {
+    st = fail(ctx, e.what());
+  }
+  ctx->arena = nullptr;
+  ctx->arena_cap = 0;
+  ctx->arena_used = 0;
+  return st;
 }
 
 int crtm_set_wavefront_angles(crtm_context* ctx, int min_angles) {

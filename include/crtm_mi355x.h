@@ -214,6 +214,25 @@ int crtm_k_matrix(crtm_context* ctx, const crtm_atmosphere* atm, const crtm_surf
                   const crtm_geometry* geo, const crtm_options* opt, crtm_rtsolution* rts,
                   crtm_k_output* k);
 
+/* CRTM_K_Matrix without device allocation.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory during a call (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments and results as crtm_k_matrix, restricted to
+ * clear-sky, device-memory calls (options.memory = CRTM_MEM_DEVICE) with
+ * atm->max_layers and atm->total_layers given.  The call's work space is
+ * carved from a fixed 128 MiB device array that each GPU holds from library
+ * load until the process ends; a batch needing more (above about 930 FP32-split
+ * or 520 FP64 profiles of 100 layers for an ODPS group-1 sensor) is
+ * rejected with CRTM_FAILURE before any work is issued.  The caller allocates
+ * inputs and outputs.  The library allocates nothing during the call: no
+ * hipMalloc, hipFree or allocating library routine.  All no-allocation calls on one GPU share its array, so
+ * overlapping calls on one GPU (on different streams or host threads) race;
+ * the array keeps the last call's contents.  Kernel timing
+ * (crtm_profile_enable) must be off. */
+int crtm_k_matrix_noalloc(crtm_context* ctx, const crtm_atmosphere* atm, const crtm_surface* sfc,
+                          const crtm_geometry* geo, const crtm_options* opt, crtm_rtsolution* rts,
+                          crtm_k_output* k);
+
 /* Issue the context's work on `stream` (a hipStream_t of the context's
  * device, e.g. the caller's framework stream; NULL is the null stream)
  * instead of the context's own stream; own_stream != 0 restores the own

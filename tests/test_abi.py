@@ -71,6 +71,7 @@ def test_null_context_is_an_error_not_a_crash():
     L = api.lib()
     assert L.crtm_forward(None, None, None, None, None, None) == api.FAILURE
     assert L.crtm_k_matrix(None, None, None, None, None, None, None) == api.FAILURE
+    assert L.crtm_k_matrix_noalloc(None, None, None, None, None, None, None) == api.FAILURE
     assert L.crtm_destroy(None) == api.FAILURE
     assert L.crtm_last_error(None) is not None
 
