@@ -512,8 +512,15 @@ clear_fwdk_kernel(DevSensor s, DevBatch bt, ProfWork<RT> w, DevOut o) {
   KT* tko = reinterpret_cast<KT*>(o.t_k) + (bt.out_loff[m] - nx) * Lt + lg;
   KT* ako = reinterpret_cast<KT*>(o.a_k) + (bt.out_loff[m] - nx) * Lt + lg;
   const int64_t astr = bt.out_NT * Lt;
-  // The Emission_AD pass stores each user layer's Planck part of T_K (and
-  // zero absorber K) in the output rows; Map_Input_AD then completes them.
+  // Map_Input_AD in the sweep: user layer i is complete once the lowest ODPS
+  // layer whose support holds it has been processed; with at most MW such
+  // layers per user layer, their adjoints sit in a register window.  Wider
+  // supports (very fine user grids) or more absorbers take the gather path.
+  const bool winmap = ok_prof && geo[3] <= (double)MW && na <= NAW;
+  // The Emission_AD pass stores each user layer's Planck part of T_K in the
+  // output rows (and, for the gather path, zero absorber rows to add onto);
+  // Map_Input_AD then completes them, writing the absorber rows of the
+  // register-window path once.
   RT carry = RT(0);
   RT up_next = ok_prof ? upath(K) : RT(0);   // path at user level iu_ of the next step
   auto ustep = [&](int iu_) {
@@ -545,7 +552,8 @@ clear_fwdk_kernel(DevSensor s, DevBatch bt, ProfWork<RT> w, DevOut o) {
       const RT dbdt = c2 * bc2 * B * (B + c1) * ic1 / (te * te);
       if (valid && k >= nx) {
         tko[(int64_t)k * Lt] = (KT)(dbdt * b_ad + t_seed);
-        for (int ja = 0; ja < na; ++ja) ako[ja * astr + (int64_t)k * Lt] = KT(0);
+        if (!winmap)
+          for (int ja = 0; ja < na; ++ja) ako[ja * astr + (int64_t)k * Lt] = KT(0);
       }
       const RT gq = tau_ad * isec;
       v = carry + gq;
@@ -578,11 +586,6 @@ clear_fwdk_kernel(DevSensor s, DevBatch bt, ProfWork<RT> w, DevOut o) {
   RT pad_next = ok_prof ? pad[Ko] : RT(0);
   const int* acc_i = w.acc_i + (size_t)m * Ko * 3;
   const RT* acc_w = w.acc_w + (size_t)m * NZ_MAX;
-  // Map_Input_AD in the sweep: user layer i is complete once the lowest ODPS
-  // layer whose support holds it has been processed; with at most MW such
-  // layers per user layer, their adjoints sit in a register window.  Wider
-  // supports (very fine user grids) or more absorbers take the gather path.
-  const bool winmap = ok_prof && geo[3] <= (double)MW && na <= NAW;
   RT tw[MW], aw[MW][NAW];
 #pragma unroll
   for (int q = 0; q < MW; ++q) {
