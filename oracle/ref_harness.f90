@@ -15,6 +15,7 @@
 !   AerosolCoeff_Binary_ReadFile, AerosolCoeff_Frequency, AerosolCoeff_typeID_to_index
 !                            src/Coefficients/AerosolCoeff/AerosolCoeff_Binary_IO.f90:352,
 !                            src/Coefficients/AerosolCoeff/AerosolCoeff_Define.f90:635-696
+!   SpcCoeff_Binary_ReadFile src/Coefficients/SpcCoeff/SpcCoeff_Binary_IO.f90:325
 MODULE ref_harness
   USE ISO_C_BINDING
   USE Type_Kinds, ONLY: fp
@@ -34,10 +35,13 @@ MODULE ref_harness
   USE AerosolCoeff_Define, ONLY: AerosolCoeff_type, AerosolCoeff_Destroy, AerosolCoeff_Frequency, &
                                  AerosolCoeff_typeID_to_index
   USE AerosolCoeff_Binary_IO, ONLY: AerosolCoeff_Binary_ReadFile
+  USE SpcCoeff_Define, ONLY: SpcCoeff_type, SpcCoeff_Destroy
+  USE SpcCoeff_Binary_IO, ONLY: SpcCoeff_Binary_ReadFile
   IMPLICIT NONE
   TYPE(ODPS_type), SAVE :: tc
   TYPE(CloudCoeff_type), SAVE :: ccf
   TYPE(AerosolCoeff_type), SAVE :: acf
+  TYPE(SpcCoeff_type), SAVE :: scf
 CONTAINS
 
   FUNCTION c2f(cstr, n) RESULT(s)
@@ -289,5 +293,43 @@ CONTAINS
     INTEGER(C_INT), VALUE :: id
     ref_aerosol_type_index = AerosolCoeff_typeID_to_index(acf, INT(id))
   END FUNCTION ref_aerosol_type_index
+
+  ! SpcCoeff_Binary_ReadFile; dims = Release, Version, n_Channels, Sensor_Type,
+  ! WMO_Satellite_ID, WMO_Sensor_ID
+  INTEGER(C_INT) FUNCTION ref_spccoeff_load(path, n, dims) BIND(C, NAME='ref_spccoeff_load')
+    CHARACTER(KIND=C_CHAR), INTENT(IN) :: path(*)
+    INTEGER(C_INT), VALUE :: n
+    INTEGER(C_INT), INTENT(OUT) :: dims(6)
+    CALL SpcCoeff_Destroy(scf)
+    ref_spccoeff_load = SpcCoeff_Binary_ReadFile(c2f(path, n), scf, Quiet=.TRUE.)
+    IF (ref_spccoeff_load /= 0) RETURN
+    dims = (/ scf%Release, scf%Version, scf%n_Channels, scf%Sensor_Type, scf%WMO_Satellite_ID, &
+              scf%WMO_Sensor_ID /)
+  END FUNCTION ref_spccoeff_load
+
+  ! Copy the loaded SpcCoeff arrays out; sid receives the 20-character Sensor_Id
+  SUBROUTINE ref_spccoeff_fetch(sid, chan, pol, pang, flag, f, nu, c1, c2, bc1, bc2, cosmic, solar) &
+      BIND(C, NAME='ref_spccoeff_fetch')
+    CHARACTER(KIND=C_CHAR), INTENT(OUT) :: sid(20)
+    INTEGER(C_INT), INTENT(OUT) :: chan(*), pol(*), flag(*)
+    REAL(C_DOUBLE), INTENT(OUT) :: pang(*), f(*), nu(*), c1(*), c2(*), bc1(*), bc2(*), cosmic(*), solar(*)
+    INTEGER :: i, L
+    L = scf%n_Channels
+    DO i = 1, 20
+      sid(i) = scf%Sensor_Id(i:i)
+    END DO
+    chan(1:L) = scf%Sensor_Channel
+    pol(1:L) = scf%Polarization
+    pang(1:L) = scf%PolAngle
+    flag(1:L) = scf%Channel_Flag
+    f(1:L) = scf%Frequency
+    nu(1:L) = scf%Wavenumber
+    c1(1:L) = scf%Planck_C1
+    c2(1:L) = scf%Planck_C2
+    bc1(1:L) = scf%Band_C1
+    bc2(1:L) = scf%Band_C2
+    cosmic(1:L) = scf%Cosmic_Background_Radiance
+    solar(1:L) = scf%Solar_Irradiance
+  END SUBROUTINE ref_spccoeff_fetch
 
 END MODULE ref_harness

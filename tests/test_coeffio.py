@@ -105,3 +105,35 @@ def test_reference_reader_agrees(coeff_dir, name):
         np.testing.assert_array_equal(opidx, np.ravel(tau.op_index, order="F"))
         np.testing.assert_array_equal(a["oc"], tau.oc)
         np.testing.assert_array_equal(a["al"], [tau.alpha, tau.alpha_c1, tau.alpha_c2])
+
+
+@pytest.mark.parametrize("name", SMALL_SENSORS + ("cris_syn", "atms_syn"))
+def test_spccoeff_reader_matches_reference(coeff_dir, name):
+    """The package's SpcCoeff reader and the reference's own
+    SpcCoeff_Binary_ReadFile (src/Coefficients/SpcCoeff/SpcCoeff_Binary_IO.f90:325,
+    built unmodified under oracle/_ref with its ACCoeff / NLTECoeff readers)
+    load the same file to identical values."""
+    from crtmv3_amd import coeffio, synthetic
+    L = _ref_lib()
+    L.ref_spccoeff_load.argtypes = [ctypes.c_char_p, ctypes.c_int, ctypes.c_void_p]
+    synthetic.write_sensor(coeff_dir, name)
+    path = os.path.join(coeff_dir, f"{name}.SpcCoeff.bin")
+    spc = coeffio.read_spccoeff(path)
+    dims = np.zeros(6, np.int32)
+    assert L.ref_spccoeff_load(path.encode(), len(path), dims.ctypes.data) == 0
+    rel, ver, n, stype, wsat, wsen = (int(x) for x in dims)
+    assert rel == coeffio.SPCCOEFF_RELEASE
+    assert (n, stype, wsat, wsen) == (spc.n_channels, spc.sensor_type, spc.wmo_satellite_id, spc.wmo_sensor_id)
+    sid = ctypes.create_string_buffer(20)
+    ints = {k: np.zeros(n, np.int32) for k in ("chan", "pol", "flag")}
+    dbl = {k: np.zeros(n) for k in ("pang", "f", "nu", "c1", "c2", "bc1", "bc2", "cosmic", "solar")}
+    L.ref_spccoeff_fetch(sid, *[a.ctypes.data_as(ctypes.c_void_p) for a in (
+        ints["chan"], ints["pol"], dbl["pang"], ints["flag"], dbl["f"], dbl["nu"], dbl["c1"], dbl["c2"],
+        dbl["bc1"], dbl["bc2"], dbl["cosmic"], dbl["solar"])])
+    assert sid.raw.decode().strip() == spc.sensor_id.strip()
+    for k, f in (("chan", "sensor_channel"), ("pol", "polarization"), ("flag", "channel_flag")):
+        np.testing.assert_array_equal(ints[k], getattr(spc, f), err_msg=f)
+    for k, f in (("pang", "pol_angle"), ("f", "frequency"), ("nu", "wavenumber"), ("c1", "planck_c1"),
+                 ("c2", "planck_c2"), ("bc1", "band_c1"), ("bc2", "band_c2"),
+                 ("cosmic", "cosmic_background_radiance"), ("solar", "solar_irradiance")):
+        np.testing.assert_array_equal(dbl[k], getattr(spc, f), err_msg=f)
