@@ -73,3 +73,69 @@ def profiles_from_arrays(level_pressure, temperature, absorber, water_t, zenith,
         water_temperature=np.asarray(water_t, float), snow_temperature=np.full(M, 260.0),
         ice_temperature=np.full(M, 260.0), sensor_zenith=np.asarray(zenith, float),
         sensor_scan=np.zeros(M), source_zenith=np.full(M, 100.0))
+
+
+# Element-wise Jacobian bar of every GPU K-matrix test, besides the
+# block-relative bar each test states: every element of a block (one
+# Jacobian variable of one profile, all channels and layers) within
+#     |got - ref| <= rtol * |ref| + atol * max|ref of the block|.
+# FP64: rtol 1e-7, atol 1e-10.  FP32-split: rtol 1e-3, atol 1e-5 for the
+# gas and surface variables, 1e-4 for the scattering-path variables (cloud
+# water content, radius and fraction, aerosol concentration and radius).
+# Where terms cancel, FP32 leaves those an absolute error of up to about
+# 6e-5 of the block maximum: the worst case seen is the overcast cloud-fraction
+# Jacobian, where the cover term (R_cloudy - R_clear) nearly cancels the
+# water-content rescaling term; elements above about 0.1 of the block maximum
+# are within 1e-3 relative.
+SCATTERING_VARIABLES = ("water content", "radius", "fraction", "aerosol")
+JAC_ELEM = {0: dict(rtol=1e-7, atol=1e-10, atol_scattering=1e-10),
+            1: dict(rtol=1e-3, atol=1e-5, atol_scattering=1e-4)}
+
+
+def assert_jacobian_close(got, want, rtol_block, what, precision):
+    """(block-relative error, worst element-wise relative error among the
+    elements where the relative part of the bar dominates)."""
+    got = np.asarray(got, np.float64)
+    want = np.asarray(want, np.float64)
+    scale = np.abs(want).max() if want.size else 0.0
+    if scale == 0.0:
+        err = np.abs(got).max() if got.size else 0.0
+        assert err <= rtol_block, f"{what}: reference zero, got max {err:.3e}"
+        return err, 0.0
+    diff = np.abs(got - want)
+    blk = diff.max() / scale
+    assert blk <= rtol_block, f"{what}: block-relative error {blk:.3e}"
+    bar = JAC_ELEM[int(precision)]
+    scattering = any(v in what for v in SCATTERING_VARIABLES)
+    atol = (bar["atol_scattering"] if scattering else bar["atol"]) * scale
+    allowed = bar["rtol"] * np.abs(want) + atol
+    if (diff > allowed).any():
+        i = np.unravel_index(np.argmax(diff - allowed), diff.shape)
+        raise AssertionError(f"{what}: element {i} off by {diff[i]:.3e} = {diff[i] / abs(want[i]):.3e} relative "
+                             f"(element {abs(want[i]) / scale:.3e} of the block max); bar rtol {bar['rtol']:g}, "
+                             f"atol {atol / scale:g} of the block max")
+    rel_regime = bar["rtol"] * np.abs(want) >= atol
+    elem = float((diff[rel_regime] / np.abs(want[rel_regime])).max()) if rel_regime.any() else 0.0
+    return blk, elem
+
+
+def subset_sensor(tau, spc, idx):
+    """ODPS and SpcCoeff copies holding only channels idx (the coefficient
+    arrays are kept whole: the per-channel start indices still point into
+    them), so the oracle can check a wide sensor on a channel sample."""
+    import copy
+    idx = np.asarray(idx)
+    t = copy.deepcopy(tau)
+    t.sensor_channel = tau.sensor_channel[idx].copy()
+    t.n_predictors = tau.n_predictors[:, idx].copy()
+    t.pos_index = tau.pos_index[:, idx].copy()
+    if tau.osignificance is not None:
+        t.osignificance = tau.osignificance[idx].copy()
+        t.order = tau.order[idx].copy()
+        t.op_index = tau.op_index[:, idx].copy()
+        t.opos_index = tau.opos_index[idx].copy()
+    s = copy.deepcopy(spc)
+    for f in ("sensor_channel", "polarization", "pol_angle", "channel_flag", "frequency", "wavenumber",
+              "planck_c1", "planck_c2", "band_c1", "band_c2", "cosmic_background_radiance", "solar_irradiance"):
+        setattr(s, f, getattr(spc, f)[idx].copy())
+    return t, s

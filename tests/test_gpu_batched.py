@@ -199,3 +199,61 @@ def test_tangent_linear_matches_finite_differences_and_adjoint(batch):
                 + (tl["aerosol_concentration"] * ad["aerosol_concentration_ad"]).sum())
     print(f"  <TL, AD> {lhs:.12e} vs <dx, x_AD> {rhs:.12e}", flush=True)
     assert abs(lhs - rhs) <= 1e-10 * abs(lhs)
+
+
+@pytest.mark.parametrize("precision", [api.PREC_FP64, api.PREC_FP32_SPLIT], ids=["fp64", "split"])
+def test_microwave_with_group1_infrared_batch(coeff_dir, oracle_mod, precision):
+    """The microwave sensor and the CrIS-like 2211-channel group-1 OPTRAN
+    sensor in one context: ragged 60-137-layer profiles, clouds in most,
+    aerosols in half, per-profile stream counts 4 / 8 / 16 / Mie choice, one
+    K-matrix call against the oracle's adjoint (all microwave channels, a
+    sample of 80 infrared channels with 30 OPTRAN ones)."""
+    from conftest import assert_jacobian_close, subset_sensor
+    print(f"batched atms_syn + cris_syn K-matrix precision={precision}", flush=True)
+    for s in ("atms_syn", "cris_syn"):
+        synthetic.write_sensor(coeff_dir, s)
+    ctx = api.crtm_init(["atms_syn", "cris_syn"], coeff_dir)
+    try:
+        rng = np.random.default_rng(311)
+        M = 6
+        ks = rng.integers(60, 138, M).tolist()
+        ps = synthetic.add_clouds(synthetic.make_profiles(M, ks, seed=312), seed=313,
+                                  types=(synthetic.WATER_CLOUD, synthetic.ICE_CLOUD), cover="random",
+                                  median_gm3=0.02, cloudy_fraction=0.8)
+        ps = synthetic.add_aerosols(ps, seed=314, aerosol_fraction=0.5)
+        ns = np.array([4, 8, 16, 0, 8, 16], np.int32)
+        L = ctx.n_channels
+        eps = np.concatenate([np.full((M, 22), 0.6), np.full((M, L - 22), 0.97)], axis=1)
+        btk = rng.uniform(0.5, 1.5, (M, L))
+        rts, kd = ctx.k_matrix(ps, eps, bt_k=btk, precision=precision, jacobian_f32=bool(precision), n_streams=ns)
+        cc = coeffio.read_cloudcoeff(synthetic.write_cloud_coeff(coeff_dir))
+        ac = coeffio.read_aerosolcoeff(synthetic.write_aerosol_coeff(coeff_dir))
+        C = oracle_mod.CloudCoeff(cc, ac)
+        tau_m, spc_m = load_sensor(coeff_dir, "atms_syn")
+        tau_i, spc_i = load_sensor(coeff_dir, "cris_syn")
+        opt = np.flatnonzero(tau_i.osignificance == 1)
+        rest = np.setdiff1d(np.arange(tau_i.n_channels), opt)
+        idx = np.sort(np.concatenate([opt[:: max(1, opt.size // 30)][:30],
+                                      rest[np.linspace(0, rest.size - 1, 50).astype(int)]]))
+        parts = [(oracle_mod.Sensor(tau_m, spc_m), np.arange(22)),
+                 (oracle_mod.Sensor(*subset_sensor(tau_i, spc_i, idx)), 22 + idx)]
+        tol = {api.PREC_FP64: dict(bt=1e-6, jac=1e-8), api.PREC_FP32_SPLIT: dict(bt=1e-2, jac=1e-3)}[precision]
+        bt = np.asarray(rts["brightness_temperature"])
+        st = np.asarray(rts["status"])
+        for m in range(M):
+            assert st[m] == 0
+            a, b = int(ps.layer_offset[m]), int(ps.layer_offset[m + 1])
+            for S, cols in parts:
+                f, k = oracle_mod.k_matrix_allsky(S, C, ps, m, eps[m, cols], int(ns[m]), 0, btk[m, cols], None,
+                                                  method="adjoint")
+                assert np.abs(bt[m, cols] - f["bt"]).max() <= tol["bt"], (m, cols[0])
+                for what, got, want in (
+                        ("T", np.asarray(kd["temperature_k"][a:b])[:, cols].T, k["t_k"]),
+                        ("absorber", np.asarray(kd["absorber_k"][:, a:b, :])[:, :, cols].transpose(2, 0, 1),
+                         k["abs_k"]),
+                        ("water content",
+                         np.asarray(kd["cloud_water_content_k"][:, a:b, :])[:, :, cols].transpose(2, 0, 1),
+                         k["wc_k"][:, :ps.cloud_type.shape[1]])):
+                    assert_jacobian_close(got, want, tol["jac"], f"profile {m} sensor@{cols[0]} {what}_K", precision)
+    finally:
+        ctx.destroy()
