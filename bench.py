@@ -29,6 +29,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "forward+K-matrix profile-channels/s, FP32-split vs FP64 GPU, at 0.01 K BT err"
 PEAK_FP32_TFLOPS = 157.3     # MI355X FP32 (vector incl. dual issue / MFMA)
+JAC_SAMPLE = 32              # profiles whose Jacobians are compared with the oracle
 PEAK_HBM_GBS = 8000.0
 C_PAD = 250                  # adjoint flops per ODPS layer of the group-1 predictor chain
 
@@ -47,8 +48,40 @@ def algorithmic_bytes_per_pc(k, n_abs, kbytes):
     return (1 + n_abs) * k * kbytes + 4 * 8
 
 
+def host_threads():
+    """CPUs this process may run on (its affinity), capped by OMP_NUM_THREADS
+    when that is set (a shared host gives each GPU a share of its cores)."""
+    n = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    return min(n, int(omp)) if omp.isdigit() and int(omp) > 0 else n
+
+
+def jacobian_errors(gpu, ref, ps, n, L, na):
+    """Block-relative and element-wise relative errors (elements above 1e-3
+    of the block maximum) of T and absorber Jacobians of the first n
+    profiles: gpu = (t_k [NT, L], a_k [na, NT, L]) host arrays, ref = oracle
+    batch output (per profile [L, K] and [L, na, K])."""
+    off = ps.layer_offset
+    res = {"T": [0.0, 0.0], "absorbers": [0.0, 0.0]}
+    for m in range(n):
+        a, b = int(off[m]), int(off[m + 1])
+        K = b - a
+        rt = ref["t_k"][a * L:b * L].reshape(L, K)
+        ra = ref["abs_k"][a * L * na:b * L * na].reshape(L, na, K)
+        for key, g, r in (("T", gpu[0][a:b].T, rt), ("absorbers", gpu[1][:, a:b, :].transpose(2, 0, 1), ra)):
+            g = np.asarray(g, np.float64)
+            scale = np.abs(r).max()
+            d = np.abs(g - r)
+            big = np.abs(r) > 1e-3 * scale
+            res[key][0] = max(res[key][0], float(d.max() / scale))
+            res[key][1] = max(res[key][1], float((d[big] / np.abs(r[big])).max()))
+    return {k: {"block_relative": v[0], "element_relative_above_1e-3_of_max": v[1]} for k, v in res.items()}
+
+
 def cpu_baseline(sensor_name, coeff_dir, ps, emissivity, bt_gpu, seconds, threads):
-    """Time the oracle (FP64 CPU restatement, OpenMP over profiles) on a sample."""
+    """Time the oracle (FP64 CPU restatement, OpenMP over profiles) on a
+    sample; returns the baseline record, the BT error of the GPU split path
+    on the sample, and the oracle's outputs for it."""
     from crtmv3_amd import coeffio, synthetic
     from oracle import oracle as orc
     tau = coeffio.read_odps(os.path.join(coeff_dir, f"{sensor_name}.TauCoeff.bin"))
@@ -68,8 +101,9 @@ def cpu_baseline(sensor_name, coeff_dir, ps, emissivity, bt_gpu, seconds, thread
     dt = time.perf_counter() - t0
     bt_err = float(np.max(np.abs(out["bt"] - bt_gpu[:n])))
     return {"value": n * L / dt, "unit": "profile-channels/s", "cores": threads, "kind": "port",
+            "host_cpus": os.cpu_count(),
             "sample": f"{n} of the {ps.n_profiles} profiles x {L} channels, forward+K-matrix, "
-                      f"FP64 oracle (oracle_k_matrix_batch), {dt:.1f} s"}, bt_err
+                      f"FP64 oracle (oracle_k_matrix_batch), {dt:.1f} s"}, bt_err, out, n
 
 
 def _subset(ps, a, b):
@@ -110,15 +144,31 @@ def copy_calibration():
     print(json.dumps({"copy_bytes": COPY_BYTES, "launches": 3}), flush=True)
 
 
+KERNEL_SOURCES = [os.path.join(ROOT, "crtmv3_amd", "csrc", f)
+                  for f in ("clear_fwdk.hip", "odps_prep.hip", "crtm_device.h")]
+
+
+def kernel_fingerprint():
+    """sha256 of the fused kernel's sources: the traffic summary applies to
+    the build it was measured on only."""
+    import hashlib
+    h = hashlib.sha256()
+    for f in KERNEL_SOURCES:
+        h.update(open(f, "rb").read())
+    return h.hexdigest()[:16]
+
+
 def measured_traffic(workload):
     """HBM bytes per launch of the fused kernel from the committed counter
     summary (FETCH_SIZE and WRITE_SIZE passes, scaled by a plain copy), when
-    it was taken on this workload; else None."""
+    it was taken on this workload with these kernel sources; else None."""
     try:
         t = json.load(open(TRAFFIC_FILE))
     except (OSError, ValueError):
         return None
-    return t["traffic_bytes"] if t.get("workload") == workload else None
+    if t.get("workload") != workload or t.get("kernel_sources_sha256") != kernel_fingerprint():
+        return None
+    return t["traffic_bytes"]
 
 
 def main():
@@ -199,6 +249,9 @@ def main():
     main_ms /= max(nl, 1)
     prep_ms /= max(nl, 1)
     bt_split = out[0]["brightness_temperature"].cpu().numpy()
+    nt_j = int(ps.layer_offset[JAC_SAMPLE])
+    jac_split = (out[1]["temperature_k"][:nt_j].cpu().numpy(), out[1]["absorber_k"][:, :nt_j].cpu().numpy())
+    jac64 = None
     log(f"split: {t_split * 1e3:.2f} ms/step (fwd/K kernel {main_ms:.2f} ms, prep {prep_ms:.2f} ms)")
 
     result = {}
@@ -222,6 +275,9 @@ def main():
                           "speedup_split_over_fp64": t64 / t_split}
         result["bt_err_split_vs_fp64_K"] = bt_err
         result["jacobian_relerr_split_vs_fp64"] = jac
+        jac64 = (out64[1]["temperature_k"][:nt_j].cpu().numpy(), out64[1]["absorber_k"][:, :nt_j].cpu().numpy())
+        result["bt_err_fp64_vs_oracle_K"] = None
+        bt64_sample = bt64
         log(f"fp64: {t64 * 1e3:.2f} ms/step; BT err {bt_err:.2e} K; jac {jac}")
         del out64
         torch.cuda.empty_cache()
@@ -235,7 +291,7 @@ def main():
     achieved = fpc * M * L / (main_ms * 1e-3) / 1e12
     workload = (f"clear-sky forward+K-matrix, CrIS-like 2211-ch ODPS group 1, {M} profiles x {K} layers, "
                 "dBT/dT,dq,dO3")
-    roof = {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+    roof = {"bound": "valu", "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
             "frac": achieved / PEAK_FP32_TFLOPS, "traffic": measured_traffic(workload),
             "kernel": "clear_fwdk_kernel<1,float,float,true>", "kernel_ms": main_ms,
             "flops_per_pc": fpc, "bytes_per_pc": bpc,
@@ -244,12 +300,19 @@ def main():
     cpu = None
     if not args.no_cpu_baseline:
         try:
-            threads = min(16, os.cpu_count() or 1)
-            cpu, bt_oracle_err = cpu_baseline(args.sensor, coeff_dir, ps, emis_h, bt_split,
-                                              args.cpu_seconds, threads)
+            threads = host_threads()
+            cpu, bt_oracle_err, ref, n_ref = cpu_baseline(args.sensor, coeff_dir, ps, emis_h, bt_split,
+                                                          args.cpu_seconds, threads)
             result["bt_err_split_vs_oracle_K"] = bt_oracle_err
+            nj = min(JAC_SAMPLE, n_ref)
+            na = ps.absorber.shape[0]
+            result["jacobian_relerr_split_vs_oracle"] = jacobian_errors(jac_split, ref, ps, nj, L, na)
+            if jac64 is not None:
+                result["bt_err_fp64_vs_oracle_K"] = float(np.max(np.abs(bt64_sample[:n_ref] - ref["bt"])))
+                result["jacobian_relerr_fp64_vs_oracle"] = jacobian_errors(jac64, ref, ps, nj, L, na)
+            result["jacobian_oracle_sample_profiles"] = nj
             log(f"cpu baseline: {cpu['value']:.3e} pc/s on {threads} threads; split-vs-oracle BT err "
-                f"{bt_oracle_err:.2e} K")
+                f"{bt_oracle_err:.2e} K; Jacobians vs oracle: split {result['jacobian_relerr_split_vs_oracle']}")
         except Exception as e:  # the baseline is reported, never required
             log(f"cpu baseline unavailable: {e}")
 
@@ -298,7 +361,7 @@ def main_allsky(args):
     spc = coeffio.read_spccoeff(os.path.join(coeff_dir, f"{sensor}.SpcCoeff.bin"))
     S = orc.Sensor(tau, spc)
     C = orc.CloudCoeff(coeffio.read_cloudcoeff(os.path.join(coeff_dir, synthetic.CLOUDCOEFF_FILE)))
-    threads = min(16, os.cpu_count() or 1)
+    threads = host_threads()
     n_sample = 256     # forward BT check
     n_ksample = 32     # K-matrix check (hand-written oracle adjoint)
     sub = _subset(ps, 0, n_sample)
@@ -389,6 +452,7 @@ def main_allsky(args):
         orc.k_matrix_allsky_batch(S, C, _subset(ps, 0, n), emis_h[:n], n_streams=8, n_threads=threads)
         dt = time.perf_counter() - t0
         cpu = {"value": n * L / dt, "unit": "profile-channels/s", "cores": threads, "kind": "port",
+               "host_cpus": os.cpu_count(),
                "sample": f"{n} of the {M} profiles x {L} channels, all-sky forward + K-matrix at 8 streams, FP64 "
                          f"oracle hand-written adjoint (oracle_k_matrix_allsky_ad_batch), {dt:.1f} s"}
     v = by_streams[8]["split"]["k_matrix"]["profile_channels_per_s"]
@@ -496,7 +560,7 @@ def main_allsky_ir(args):
                                                    float(np.abs(got - k["wc_k"]).max() / sc))
     cpu = None
     if not args.no_cpu_baseline:
-        threads = min(16, os.cpu_count() or 1)
+        threads = host_threads()
         n = 2
         t0 = time.perf_counter()
         orc.k_matrix_allsky_batch(S, C, _subset(ps, 0, n), emis_h[:n], n_streams=ns, n_threads=threads)
@@ -506,6 +570,7 @@ def main_allsky_ir(args):
         orc.k_matrix_allsky_batch(S, C, _subset(ps, 0, n), emis_h[:n], n_streams=ns, n_threads=threads)
         dt = time.perf_counter() - t0
         cpu = {"value": n * L / dt, "unit": "profile-channels/s", "cores": threads, "kind": "port",
+               "host_cpus": os.cpu_count(),
                "sample": f"{n} of the {M} profiles x {L} channels (the first ones; 40 % cloudy overall), "
                          f"all-sky forward + K-matrix at {ns} streams, FP64 oracle hand-written adjoint, {dt:.1f} s"}
     v = res["split"]["k_matrix"]["profile_channels_per_s"]
