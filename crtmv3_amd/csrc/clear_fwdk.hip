@@ -22,6 +22,7 @@
 // split path, double for the plain FP64 path); Planck and the BT inversion
 // are always FP64.
 #include "crtm_device.h"
+#include "pred_adjoint.h"
 
 namespace crtm {
 
@@ -35,162 +36,37 @@ __device__ inline void store_k(void* base, int64_t idx, RT v) {
 }
 }  // namespace
 
-// Adjoint accumulators of the per-layer predictor intermediates
-template <class RT> struct LayerAdj {
-  RT T, T2, dt, dt2, h2o, h2o_a, h2o_s, h2o_r, h2o_r4, h2o_q, tz, tzp;
-  RT gz[6], gt[6], ra[6];
-  RT o3, o3_a, o3_r;
-  RT co, co_a, co_r, co_s, co_q, ch4, ch4_a, ch4_r, ch4_q, n2o, n2o_a, n2o_r, n2o_s;
-};
-
-// Adjoint of component j's predictors at one layer: xa[i] = X_AD(k, i, j).
-// Partial derivatives use the wave-uniform table u (ODPS_Predictor.f90:1894-2641).
-template <int G, int J, class RT>
-__device__ inline void comp_adjoint(const RT* xa, const RT* u, LayerAdj<RT>& a) {
-  const RT sec = u[U_SEC];
-  if constexpr (J == 0) {
-    a.T += xa[1] * sec + xa[3];
-    a.T2 += xa[2] * sec + xa[5];
-    a.tz += xa[6];
-  } else if constexpr (G == 3 && J == 1) {
-    const RT iT = u[U_IT], iT2 = u[U_IT2], iT4 = u[U_IT4], h2o = u[U_H2O], h2o_a = u[U_H2OA];
-    a.h2o_a += xa[0] * iT + xa[1] * h2o * iT + xa[2] * h2o * iT4 + xa[3] * iT2 + xa[4] * h2o * iT2 +
-               xa[5] * iT4 + xa[6] + xa[7] * u[U_DT] + xa[11] * u[U_H2OS];
-    a.h2o += (xa[1] * iT + xa[2] * iT4 + xa[4] * iT2) * h2o_a;
-    a.T -= (xa[0] + xa[1] * h2o) * h2o_a * iT * iT;
-    a.T2 -= (RT(2) * xa[2] * h2o * iT4 * iT2 + xa[3] * iT4 + xa[4] * h2o * iT4 + RT(2) * xa[5] * iT4 * iT2) * h2o_a;
-    a.dt += xa[7] * h2o_a;
-    a.gz[0] += xa[8] * RT(2) * sec * sec * u[U_H2OP] + xa[9] * sec;
-    a.h2o_s += xa[11] * h2o_a + xa[12] * RT(2) * u[U_H2OS];
-    a.h2o_q += xa[13];
-  } else if constexpr (J == 1) {  // IR water-vapour lines
-    const RT h2o_a = u[U_H2OA], dt = u[U_DT], h2o_r = u[U_H2OR], h2o_s = u[U_H2OS];
-    a.h2o_a += xa[0] + xa[1] * dt + xa[3] * u[U_DT2] + xa[5] * h2o_s;
-    a.dt += xa[1] * h2o_a + xa[7] * h2o_r;
-    a.h2o_s += xa[2] + xa[5] * h2o_a + xa[8] * RT(2) * h2o_s;
-    a.dt2 += xa[3] * h2o_a;
-    a.h2o_r4 += xa[4];
-    a.h2o_r += xa[6] + xa[7] * dt + xa[10] * u[U_H2ODTZP];
-    a.h2o_q += xa[9] + xa[10] * h2o_r;
-    a.gz[0] += xa[11] * RT(2) * sec * sec * u[U_H2OP] + xa[12] * sec;
-    a.ra[2] += xa[14] * sec;
-    if constexpr (G == 1) {
-      a.ch4_a += xa[15] + xa[16] * RT(2) * u[U_CH4A] * dt;
-      a.dt += xa[16] * u[U_CH4A] * u[U_CH4A];
-      a.co_a += xa[17];
-    }
-  } else if constexpr (J == 2) {  // IR water-vapour continuum
-    const RT iT = u[U_IT], iT2 = u[U_IT2], iT4 = u[U_IT4], h2o = u[U_H2O], h2o_a = u[U_H2OA];
-    a.h2o_a += xa[0] * iT + xa[1] * h2o * iT + xa[2] * h2o * iT4 + xa[3] * iT2 + xa[4] * h2o * iT2 +
-               xa[5] * iT4 + xa[6];
-    a.h2o += (xa[1] * iT + xa[2] * iT4 + xa[4] * iT2) * h2o_a;
-    a.T -= (xa[0] + xa[1] * h2o) * h2o_a * iT * iT;
-    a.T2 -= (RT(2) * xa[2] * h2o * iT4 * iT2 + xa[3] * iT4 + xa[4] * h2o * iT4 + RT(2) * xa[5] * iT4 * iT2) * h2o_a;
-  } else if constexpr (J == 3) {  // ozone
-    const RT o3 = u[U_O3], o3_a = u[U_O3A], o3_r = u[U_O3R], o3p = u[U_O3P], dt = u[U_DT];
-    a.o3_a += xa[0] + xa[1] * dt + xa[2] * o3 * o3p + RT(2) * xa[3] * o3_a + xa[4] * o3p + xa[5] * u[U_SP];
-    a.dt += xa[1] * o3_a + xa[6] * o3_r;
-    const RT io3p = u[U_IO3P];
-    a.o3_r += xa[6] * dt + xa[7] + xa[8] * o3 * io3p;
-    a.o3 += xa[2] * o3_a * o3p + xa[8] * o3_r * io3p;
-    a.gz[1] += xa[2] * o3_a * o3 + xa[4] * o3_a + xa[5] * RT(0.5) * o3_a * u[U_SQSO3P] -
-               xa[8] * o3_r * o3 * io3p * io3p + xa[9] * sec + xa[10] * RT(2) * sec * sec * o3p;
-  } else if constexpr (J == 4) {  // CO2
-    const RT tzp = u[U_TZP], sT = u[U_ST];
-    a.T += xa[0] * sec + xa[2] + xa[9] * sec * tzp * RT(0.5) * u[U_IST];
-    a.T2 += xa[1] * sec + xa[3];
-    a.ra[2] += xa[5] * sec;
-    a.tzp += xa[6] * sec + xa[8] * RT(3) * tzp * tzp + xa[9] * sec * sT;
-    a.gz[2] += xa[7] * RT(2) * sec * sec * u[U_CO2P];
-    if constexpr (G == 1) a.co_a += xa[10];
-  } else if constexpr (J == 5) {  // N2O
-    const RT dt = u[U_DT], n2o_a = u[U_N2OA], n2o = u[U_N2O], n2o_r = u[U_N2OR], n2ozp = u[U_N2OZP];
-    a.n2o_a += xa[0] * dt + xa[3] * RT(0.25) * u[U_N2OPOW] + xa[4];
-    a.dt += xa[0] * n2o_a + xa[2] * n2o;
-    const RT in2ozp = u[U_IN2OZP];
-    a.n2o_r += xa[1] + xa[9] * n2o * in2ozp;
-    a.n2o += xa[2] * dt + xa[9] * n2o_r * in2ozp;
-    a.gz[3] += xa[5] * sec - xa[9] * n2o_r * n2o * in2ozp * in2ozp;
-    a.gt[3] += xa[6] * sec + xa[8];
-    a.n2o_s += xa[7];
-    a.ch4_a += xa[10] + xa[11] * u[U_CH4ZP];
-    a.gz[5] += xa[11] * u[U_CH4A];
-    a.co_a += xa[12] + xa[13] * sec * u[U_COZP];
-    a.gz[4] += xa[13] * u[U_COA] * sec;
-  } else if constexpr (J == 6) {  // CO
-    const RT dt = u[U_DT], co_a = u[U_COA], co_r = u[U_COR], co_q = u[U_COQ];
-    a.co_a += xa[0] + xa[1] * dt + xa[6] * u[U_DT2];
-    a.dt += xa[1] * co_a + xa[3] * co_r;
-    const RT icor = u[U_ICOR];
-    a.co_r += xa[2] * RT(0.5) * u[U_ISQCOR] + xa[3] * dt + xa[5] - xa[8] * co_q * icor * icor;
-    a.co_s += xa[4];
-    a.dt2 += xa[6] * co_a;
-    a.co_q += xa[7] + xa[8] * icor + xa[9] * u[U_SQCOZP];
-    a.gz[4] += xa[9] * RT(0.5) * co_q * u[U_ISQCOZP];
-  } else if constexpr (J == 7) {  // CH4
-    const RT dt = u[U_DT], ch4_a = u[U_CH4A], ch4 = u[U_CH4], ch4_r = u[U_CH4R], ch4zp = u[U_CH4ZP];
-    a.ch4_a += xa[0] * dt + xa[2] * RT(2) * ch4_a + xa[3];
-    a.dt += xa[0] * ch4_a + xa[4] * ch4;
-    const RT ich4zp = u[U_ICH4ZP];
-    a.ch4_r += xa[1] + xa[7] * RT(0.5) * u[U_ISQCH4R] + xa[10] * ch4 * ich4zp;
-    a.ch4 += xa[4] * dt + xa[10] * ch4_r * ich4zp;
-    a.ch4_q += xa[5] + xa[6] * RT(2) * u[U_CH4Q];
-    a.gt[5] += xa[8] + xa[9] * sec;
-    a.gz[5] -= xa[10] * ch4_r * ch4 * ich4zp * ich4zp;
-  }
-}
-
-// Close the layer: chain the intermediates back to T, the absorber ratios
-// and the integrated ratios (tail of ODPS_Compute_Predictor_AD).
-template <int G, class RT>
-__device__ inline void finish_layer(const RT* u, LayerAdj<RT>& a) {
-  const RT sec = u[U_SEC];
-  if constexpr (G == 1) {
-    a.gz[5] += sec * a.ch4_q;
-    a.ch4_a += RT(0.5) * u[U_ICH4R] * a.ch4_r;
-    a.ch4 += sec * a.ch4_a;
-    const RT co_a = u[U_COA], co = u[U_CO], cozp = u[U_COZP];
-    const RT icozp = u[U_ICOZP];
-    a.gz[4] -= a.co_q * co_a * co * icozp * icozp;
-    a.co_a += a.co_q * co * icozp + a.co_s * RT(2) * co_a + a.co_r * RT(0.5) * u[U_ICOR];
-    a.co += a.co_q * co_a * icozp + a.co_a * sec;
-    a.n2o_a += a.n2o_r * RT(0.5) * u[U_IN2OR] + a.n2o_s * RT(2) * u[U_N2OA];
-    a.n2o += a.n2o_a * sec;
-    a.ra[5] += a.ch4;
-    a.ra[3] += a.n2o;
-    a.ra[4] += a.co;
-  }
-  if constexpr (G == 1 || G == 2) {
-    a.o3_a += a.o3_r * RT(0.5) * u[U_IO3R];
-    a.o3 += a.o3_a * sec;
-    a.ra[1] += a.o3;
-  }
-  const RT igt0 = u[U_IGT0];
-  a.gt[0] -= a.h2o_q * u[U_H2O] * igt0 * igt0;
-  a.h2o_r += a.h2o_r4 * RT(0.5) * u[U_IH2OR4];
-  a.h2o_a += a.h2o_s * RT(2) * u[U_H2OA] + a.h2o_r * RT(0.5) * u[U_IH2OR];
-  a.h2o += a.h2o_a * sec + a.h2o_q * igt0;
-  a.ra[0] += a.h2o;
-  a.dt += a.dt2 * RT(2) * u[U_ADT];
-  a.T += a.T2 * RT(2) * u[U_T];
-}
-
 // The layer's predictor adjoint is linear in X_AD(k, :, :) = C(:, :, k) * OD_AD(k)
-// with wave-uniform partials, so it runs on the coefficients alone and the
-// caller scales the layer's results by OD_AD(k) once.
+// with per-layer factors, so the final quantities are y[q] = sum_s C(s) D(s, q)
+// with D the profile's sensitivity table (prepared per layer) and C the
+// channel's coefficients; the caller scales y by OD_AD(k) once.  Only the
+// structurally nonzero (s, q) pairs are stored and visited, as compile-time
+// indices, so y lives in registers.
+template <int G, class RT, int S, int E, int Q>
+__device__ inline void sens_q(RT c, const RT* dr, RT* y) {
+  if constexpr (Q < QIdx<G>::NQ) {
+    constexpr bool on = (SensPattern<G>::tab.mask[S] >> Q) & 1u;
+    if constexpr (on) y[Q] += c * dr[E];
+    sens_q<G, RT, S, E + (on ? 1 : 0), Q + 1>(c, dr, y);
+  }
+}
+template <int G, class RT, int S, int SEND>
+__device__ inline void sens_slots(const float* ck, const RT* dr, RT* y) {
+  if constexpr (S < SEND) {
+    if constexpr (SensPattern<G>::tab.mask[S] != 0u) {
+      const RT c = (RT)ck[S * WAVE];
+      sens_q<G, RT, S, SensPattern<G>::tab.off[S], 0>(c, dr, y);
+    }
+    sens_slots<G, RT, S + 1, SEND>(ck, dr, y);
+  }
+}
 template <int G, class RT, int J>
-__device__ inline void comp_layer_ad(const float* ck, const int* npb, const RT* uk, LayerAdj<RT>& a) {
+__device__ inline void sens_layer(const float* ck, const int* npb, const RT* dr, RT* y) {
   using Tg = GT<G>;
   if constexpr (J < Tg::NCOMP) {
-    if (npb[J] > 0) {   // block-uniform: no channel of the block uses component J
-      RT xa[Tg::NP[J]];
-      const float* cj = ck + Tg::XOFF[J] * WAVE;
-      // slots beyond a channel's predictor count hold zeros
-#pragma unroll
-      for (int i = 0; i < Tg::NP[J]; ++i) xa[i] = (RT)cj[i * WAVE];
-      comp_adjoint<G, J, RT>(xa, uk, a);
-    }
-    comp_layer_ad<G, RT, J + 1>(ck, npb, uk, a);
+    if (npb[J] > 0)   // block-uniform: some channel of the block uses component J
+      sens_slots<G, RT, Tg::XOFF[J], Tg::XOFF[J] + Tg::NP[J]>(ck, dr, y);
+    sens_layer<G, RT, J + 1>(ck, npb, dr, y);
   }
 }
 
@@ -213,18 +89,17 @@ __device__ inline RT comp_layer_od(const float* ck, const int* npb, const RT* xk
 }
 
 // Workgroup shape per arithmetic type, from measurements at the benchmark
-// shape (2211 channels x 1024 profiles): FP32 runs best with 16 profiles per
-// workgroup at 4 waves per SIMD (<= 128 VGPRs); FP64 needs more registers and
-// runs best with 8 profiles, a single LDS buffer and no wave-count target.
+// shape: FP32 runs best with 16 profiles per workgroup at 4 waves per SIMD
+// (<= 128 VGPRs); FP64 needs more registers and runs best with 8 profiles
+// and no wave-count target.
 //   PW   profiles (waves) per workgroup, all on one 64-channel block
-//   NBUF LDS buffers per staged layer (2: the next layer's stores overlap
-//        the current layer's reads; 1: half the LDS, one more barrier)
 //   WPE  waves per SIMD the register allocation targets (0: compiler's choice)
-template <class RT> struct KCfg { static constexpr int PW = 16, NBUF = 2, WPE = 4; };
-template <> struct KCfg<double> { static constexpr int PW = 8, NBUF = 1, WPE = 0; };
-// LDS row of one wave: predictors X or adjoint table U, followed by the OPTRAN table O
-constexpr int TABW = (NX_MAX > NU ? NX_MAX : NU) + NO;
-constexpr int RQ = (TABW + WAVE - 1) / WAVE;   // row elements loaded per lane
+// Each layer is staged in one of two LDS buffers while the other is read.
+template <class RT> struct KCfg { static constexpr int PW = 16, WPE = 4; };
+template <> struct KCfg<double> { static constexpr int PW = 8, WPE = 0; };
+constexpr int NBUF = 2;
+// LDS row of one wave: predictors X or the adjoint row, followed by the OPTRAN table O
+constexpr int TABW = (NX_MAX > NR_MAX ? NX_MAX : NR_MAX) + NO;
 constexpr int OC_LDS_BYTES = 16384;            // OPTRAN coefficients staged when they fit
 constexpr int MW = MAP_WIN;                    // Map_Input_AD register window (ODPS layers)
 constexpr int NAW = 2;                         // user absorbers the window carries
@@ -250,7 +125,7 @@ __global__ void __attribute__((amdgpu_flat_work_group_size(KCfg<RT>::PW * WAVE, 
                                amdgpu_waves_per_eu(KCfg<RT>::WPE > 0 ? KCfg<RT>::WPE : 1)))
 clear_fwdk_kernel(DevSensor s, DevBatch bt, ProfWork<RT> w, DevOut o) {
   using Tg = GT<G>;
-  constexpr int PW = KCfg<RT>::PW, NBUF = KCfg<RT>::NBUF, TPB = PW * WAVE;
+  constexpr int PW = KCfg<RT>::PW, TPB = PW * WAVE;
   // Workgroup = one 64-channel block x PW profiles (one wave each).  The
   // block's coefficients of the current layer are staged once in LDS and
   // read by all PW waves, so each coefficient leaves L2 once per PW
@@ -304,7 +179,9 @@ clear_fwdk_kernel(DevSensor s, DevBatch bt, ProfWork<RT> w, DevOut o) {
   const bool blk_opt = ob >= 0;
   const bool lane_opt = blk_opt && s.lane_optran[b * WAVE + lane] != 0;
   const RT* X = w.x + (size_t)m * Ko * Tg::NX;
-  const RT* Ut = w.u + (size_t)m * Ko * NU;
+  using RR = RRow<G>;
+  using Qi = QIdx<G>;
+  const RT* Ut = w.u + (size_t)m * Ko * RR::NR;
   const RT* Ot = w.o + (size_t)m * Ko * NO;
   const int optw = s.optran ? NO : 0;
 
@@ -313,46 +190,41 @@ clear_fwdk_kernel(DevSensor s, DevBatch bt, ProfWork<RT> w, DevOut o) {
   constexpr int OCAP = OC_LDS_BYTES / (WAVE * (int)sizeof(RT));
   __shared__ RT ocs[OCAP * WAVE];                 // OPTRAN coefficients [row*order][lane]
 
-  // ---- coefficient staging: layer k's [NX][64] floats, float4 per thread ----
-  constexpr int CQ = (Tg::NX * WAVE / 4 + TPB - 1) / TPB;
+  // ---- staging of layer k straight into LDS (global_load_lds): lane i of a
+  // wave's load lands at base + size*i, so neither the block's coefficient
+  // tile nor the wave's rows pass through VGPRs.  Each wave waits for its own
+  // loads before the barrier that publishes the buffer (stage_wait). ----
+  using lds_ptr = __attribute__((address_space(3))) void*;
   const float4* cg = reinterpret_cast<const float4*>(s.cblk + s.blk_base[b]);
-  constexpr int n4 = Tg::NX * (WAVE / 4);
-  float4 creg[CQ];
-  auto cload = [&](int k) {
-#pragma unroll
-    for (int q = 0; q < CQ; ++q) {
-      const int e = tid + q * TPB;
-      if (e < n4) creg[q] = cg[(int64_t)k * n4 + e];
+  constexpr int n4 = Tg::NX * (WAVE / 4);          // float4 of one layer's coefficients
+  constexpr int NCH = (n4 + WAVE - 1) / WAVE;      // 1 KB pieces, spread over the waves
+  auto cload = [&](int k, int buf) {
+    for (int c = wv; c < NCH; c += PW) {           // wave-uniform
+      const int e = c * WAVE + lane;
+      if (e < n4)
+        __builtin_amdgcn_global_load_lds(cg + (int64_t)k * n4 + e,
+                                         (lds_ptr)(reinterpret_cast<float4*>(cs[buf]) + c * WAVE), 16, 0, 0);
     }
   };
-  auto cstore = [&](int buf) {
-#pragma unroll
-    for (int q = 0; q < CQ; ++q) {
-      const int e = tid + q * TPB;
-      if (e < n4) reinterpret_cast<float4*>(cs[buf])[e] = creg[q];
+  // element t of the row staged for layer k: [X or adjoint row | O row], as dwords
+  constexpr int DW = (int)(sizeof(RT) / 4);
+  auto rload = [&](const RT* base, int nbase, int k, int buf) {
+    const uint32_t* src = reinterpret_cast<const uint32_t*>(base + (size_t)k * nbase);
+    uint32_t* dst = reinterpret_cast<uint32_t*>(tab[wv][buf]);
+    const int nb = nbase * DW;
+    for (int c = 0; c * WAVE < nb; ++c)
+      if (c * WAVE + lane < nb)
+        __builtin_amdgcn_global_load_lds(src + c * WAVE + lane, (lds_ptr)(dst + c * WAVE), 4, 0, 0);
+    if (optw) {
+      const uint32_t* so = reinterpret_cast<const uint32_t*>(Ot + (size_t)k * NO);
+      constexpr int no = NO * DW;
+      for (int c = 0; c * WAVE < no; ++c)
+        if (c * WAVE + lane < no)
+          __builtin_amdgcn_global_load_lds(so + c * WAVE + lane, (lds_ptr)(dst + nb + c * WAVE), 4, 0, 0);
     }
   };
-  // ---- per-wave row staging: element t of layer k is [X or U row | O row] ----
-  RT rreg[RQ];
-  auto rload = [&](const RT* base, int nbase, int k) {
-#pragma unroll
-    for (int q = 0; q < RQ; ++q) {
-      const int t = lane + q * WAVE;
-      RT v = RT(0);
-      if (ok_prof) {
-        if (t < nbase) v = base[(size_t)k * nbase + t];
-        else if (t < nbase + optw) v = Ot[(size_t)k * NO + (t - nbase)];
-      }
-      rreg[q] = v;
-    }
-  };
-  auto rstore = [&](int buf) {
-#pragma unroll
-    for (int q = 0; q < RQ; ++q) {
-      const int t = lane + q * WAVE;
-      if (t < TABW) tab[wv][buf][t] = rreg[q];
-    }
-  };
+  // all of this wave's memory operations done (the staged loads land in LDS)
+  auto stage_wait = [] { __builtin_amdgcn_s_waitcnt(0); };
   // ---- OPTRAN coefficients: LDS copy when the block's table fits ----
   const int onq = blk_opt ? meta[BM_ONROW] * (meta[BM_OMAXO] + 1) : 0;
   const bool oc_lds = onq <= OCAP;
@@ -367,15 +239,14 @@ clear_fwdk_kernel(DevSensor s, DevBatch bt, ProfWork<RT> w, DevOut o) {
   RT path[KO_MAX + 1];
   uint64_t odclamp[2] = {0, 0}, lcclamp[2] = {0, 0};
   path[0] = RT(0);
-  cload(0);
-  rload(X, Tg::NX, 0);
-  cstore(0);
-  rstore(0);
+  cload(0, 0);
+  rload(X, Tg::NX, 0, 0);
+  stage_wait();
   __syncthreads();
   for (int k = 0; k < Ko; ++k) {
     if (k + 1 < Ko) {
-      cload(k + 1);
-      rload(X, Tg::NX, k + 1);
+      cload(k + 1, (k + 1) % NBUF);
+      rload(X, Tg::NX, k + 1, (k + 1) % NBUF);
     }
     const RT* xk = tab[wv][k % NBUF];
     const float* ck = cs[k % NBUF] + lane;
@@ -406,11 +277,7 @@ clear_fwdk_kernel(DevSensor s, DevBatch bt, ProfWork<RT> w, DevOut o) {
       odclamp[k >> 6] |= 1ull << (k & 63);
     }
     path[k + 1] = path[k] + odc;
-    if constexpr (NBUF == 1) __syncthreads();   // every wave is done with the buffer
-    if (k + 1 < Ko) {
-      cstore((k + 1) % NBUF);
-      rstore((k + 1) % NBUF);
-    }
+    stage_wait();
     __syncthreads();
   }
 
@@ -650,15 +517,14 @@ clear_fwdk_kernel(DevSensor s, DevBatch bt, ProfWork<RT> w, DevOut o) {
   RT ivn = RT(0), sta = RT(0), spa = RT(0), xltc = RT(0), xlpc = RT(0);
   const int h2o_j = s.h2o_idx - 1;
   // the forward loop ended on a barrier, so both LDS buffers are free
-  cload(Ko - 1);
-  rload(Ut, NU, Ko - 1);
-  cstore((Ko - 1) % NBUF);
-  rstore((Ko - 1) % NBUF);
+  cload(Ko - 1, (Ko - 1) % NBUF);
+  rload(Ut, RR::NR, Ko - 1, (Ko - 1) % NBUF);
+  stage_wait();
   __syncthreads();
   for (int k = Ko - 1; k >= 0; --k) {
     if (k > 0) {
-      cload(k - 1);
-      rload(Ut, NU, k - 1);
+      cload(k - 1, (k - 1) % NBUF);
+      rload(Ut, RR::NR, k - 1, (k - 1) % NBUF);
     }
     if (winmap) {
 #pragma unroll
@@ -671,13 +537,15 @@ clear_fwdk_kernel(DevSensor s, DevBatch bt, ProfWork<RT> w, DevOut o) {
       pad_next = k > 0 ? pad[k] : RT(0);
       const RT od_ad = ((odclamp[k >> 6] >> (k & 63)) & 1) ? RT(0) : S;
       const RT* uk = tab[wv][k % NBUF];
-      LayerAdj<RT> a{};
       const float* ck = cs[k % NBUF] + lane;
-      comp_layer_ad<G, RT, 0>(ck, npb, uk, a);
-      RT t_ad = RT(0), vap_ad = RT(0);
+      RT y[Qi::NQ];
+#pragma unroll
+      for (int q = 0; q < Qi::NQ; ++q) y[q] = RT(0);
+      sens_layer<G, RT, 0>(ck, npb, uk, y);
+      RT t_ad = y[Qi::TT] * od_ad, vap_ad = RT(0);
       if (blk_opt) {
         // Add_OPTRAN_wloOD_AD (ODPS_AtmAbsorption.f90:911-984) with chi recomputed
-        const RT* okr = uk + NU;
+        const RT* okr = uk + RR::NR;
         const RT lnchi = oc_lds ? optran_lnchi<RT>(oc_l, meta, okr) : optran_lnchi<RT>(oc_g, meta, okr);
         const bool lcc = (lcclamp[k >> 6] >> (k & 63)) & 1;
         const RT chi = lane_opt ? (lcc ? (lnchi > RT(0) ? RT(LIMIT_LOG) : RT(0)) : rexp<RT>(lnchi)) : RT(0);
@@ -718,22 +586,22 @@ clear_fwdk_kernel(DevSensor s, DevBatch bt, ProfWork<RT> w, DevOut o) {
         t_ad += okr[O_DA] * sta;
         vap_ad += okr[O_DPSEC] * dab_ad;
       }
-      finish_layer<G, RT>(uk, a);
-      t_ad += (a.T * uk[U_IRT] + a.dt) * od_ad;
       RT a_ad[Tg::NABS];
 #pragma unroll
-      for (int j = 0; j < Tg::NABS; ++j) a_ad[j] = a.ra[j] * od_ad * uk[U_IRA0 + j];
+      for (int j = 0; j < Tg::NABS; ++j) a_ad[j] = y[Qi::RA0 + j] * od_ad;
       if (s.optran) a_ad[h2o_j] += vap_ad;
-      const RT pdp = uk[U_PDP];
+      // the integrated ratios accumulate from the top, so their adjoints
+      // carry down the sweep (ODPS_Predictor.f90:707-759 reversed)
+      const RT pdp = uk[RR::PDP];
 #pragma unroll
       for (int j = 0; j < Tg::NABS; ++j) {
-        gts[j] += a.gt[j] * od_ad * uk[U_IGATZP0 + j];
-        gzs[j] += a.gz[j] * od_ad * uk[U_IGAZP0 + j];
-        t_ad += gts[j] * pdp * uk[U_AODPS0 + j];
-        a_ad[j] += gzs[j] * pdp + gts[j] * pdp * uk[U_TODPS];
+        gts[j] += y[Qi::GT0 + j] * od_ad;
+        gzs[j] += y[Qi::GZ0 + j] * od_ad;
+        t_ad += gts[j] * pdp * uk[RR::AODPS0 + j];
+        a_ad[j] += gzs[j] * pdp + gts[j] * pdp * uk[RR::TODPS];
       }
-      tzps += a.tzp * od_ad * uk[U_ITZPREF];
-      tzs += a.tz * od_ad * uk[U_ITZREF];
+      tzps += y[Qi::TZP] * od_ad;
+      tzs += y[Qi::TZ] * od_ad;
       t_ad += tzs + pdp * tzps;
       // Map_Input_AD (ODPS_CoordinateMapping.f90:480-547): add onto the user
       // layers averaged into this ODPS layer (their Planck part is already stored)
@@ -752,7 +620,7 @@ clear_fwdk_kernel(DevSensor s, DevBatch bt, ProfWork<RT> w, DevOut o) {
           const int ju = bt.jmap[j];
 #pragma unroll
           for (int q = 0; q < NAW; ++q)
-            if (ju == q && uk[U_MASK0 + j] != RT(0)) aw[0][q] = a_ad[j];
+            if (ju == q && uk[RR::MASK0 + j] != RT(0)) aw[0][q] = a_ad[j];
         }
         // user layers (i2(k-1), i2(k)] are complete
         const int ihi = frange[2 * k + 1];
@@ -769,17 +637,13 @@ clear_fwdk_kernel(DevSensor s, DevBatch bt, ProfWork<RT> w, DevOut o) {
 #pragma unroll
         for (int j = 0; j < Tg::NABS; ++j) {
           const int ju = bt.jmap[j];
-          if (ju >= 0 && uk[U_MASK0 + j] != RT(0))
+          if (ju >= 0 && uk[RR::MASK0 + j] != RT(0))
             for (int q = 0; q < n; ++q)
               if (i1 + q >= nx) unsafeAtomicAdd(ako + ju * astr + (int64_t)(i1 + q) * Lt, (KT)(awk[q] * a_ad[j]));
         }
       }
     }
-    if constexpr (NBUF == 1) __syncthreads();
-    if (k > 0) {
-      cstore((k - 1) % NBUF);
-      rstore((k - 1) % NBUF);
-    }
+    stage_wait();
     __syncthreads();
   }
 }

@@ -19,6 +19,7 @@
 
 #include "../../include/crtm_mi355x.h"
 #include "crtm_device.h"
+#include "pred_adjoint.h"
 #include "model_profiles.inc"   // CLIM_TABLE[CLIM_SIZE], generated from the model climatologies
 
 namespace crtm {
@@ -804,7 +805,7 @@ constexpr int N_WORK = 13;
 template <class RT>
 std::array<size_t, N_WORK> work_sizes(int M, int Ko, int nx, int max_k) {
   const size_t m = (size_t)M, ko = (size_t)Ko, mk = (size_t)max_k;
-  return {m * ko * nx * sizeof(RT),       m * ko * NU * sizeof(RT),        m * ko * NO * sizeof(RT),
+  return {m * ko * nx * sizeof(RT),       m * ko * NR_MAX * sizeof(RT),    m * ko * NO * sizeof(RT),
           m * ko * 3 * sizeof(int),       m * NZ_MAX * sizeof(RT),         m * (mk + 1) * sizeof(int),
           m * (mk + 1) * sizeof(RT),      m * 4 * sizeof(double),          m * mk * MAP_WIN * sizeof(RT),
           m * ko * 2 * sizeof(int),       m * sizeof(int),                 m * NZ_MAX * sizeof(double),

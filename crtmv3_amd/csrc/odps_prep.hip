@@ -15,6 +15,7 @@
 // :607-634, :963-1012, :1108-1320; src/AtmAbsorption/ODPS/ODPS_Predictor.f90:
 // 290-357, 645-1095, 2716-2816; src/GeometryInfo/CRTM_GeometryInfo.f90:100-143.
 #include "crtm_device.h"
+#include "pred_adjoint.h"
 
 namespace crtm {
 
@@ -321,6 +322,38 @@ __global__ void __launch_bounds__(64) layer_map_kernel(DevSensor s, DevBatch bt,
   w.status[m] = st;
 }
 
+// Component of predictor slot s of group G.
+template <int G> constexpr int comp_of(int s) {
+  int j = 0;
+  while (j + 1 < GT<G>::NCOMP && GT<G>::XOFF[j + 1] <= s) ++j;
+  return j;
+}
+
+// Sensitivity-table entries of slot S onwards: the predictor adjoint chain
+// (comp_adjoint, finish_layer, final_quantities) run on the unit predictor
+// adjoint of each slot, which the compiler folds to the slot's own terms.
+template <int G, int S>
+__device__ inline void sens_slot_row(const double* U, double* R) {
+  if constexpr (S < GT<G>::NX) {
+    constexpr unsigned msk = SensPattern<G>::tab.mask[S];
+    if constexpr (msk != 0) {
+      constexpr int J = comp_of<G>(S), I = S - GT<G>::XOFF[J];
+      double xa[18] = {};
+      xa[I] = 1.0;
+      LayerAdj<double> a{};
+      comp_adjoint<G, J, double>(xa, U, a);
+      finish_layer<G, double>(U, a);
+      double q[QIdx<G>::NQ];
+      final_quantities<G, double>(a, U, q);
+      int e = SensPattern<G>::tab.off[S];
+#pragma unroll
+      for (int c = 0; c < QIdx<G>::NQ; ++c)
+        if ((msk >> c) & 1) R[e++] = q[c];
+    }
+    sens_slot_row<G, S + 1>(U, R);
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Per-layer preparation, one workgroup per profile (after layer_map_kernel):
 // geometry, mapped T / absorbers with the Min/Max clamps, secants, integrated
@@ -571,8 +604,19 @@ __global__ void __launch_bounds__(PREP_THREADS) odps_prep_kernel(DevSensor s, De
     }
     RT* xo = w.x + ((size_t)m * Ko + k) * T::NX;
     for (int i = 0; i < T::NX; ++i) xo[i] = (RT)X[i];
-    RT* uo = w.u + ((size_t)m * Ko + k) * NU;
-    for (int i = 0; i < NU; ++i) uo[i] = (RT)U[i];
+    // adjoint row: sensitivities of the final quantities to each predictor,
+    // then the cumulative-chain factors and the Map_Input_AD masks
+    using RR = RRow<G>;
+    double R[RR::NR];
+    sens_slot_row<G, 0>(U, R);
+    R[RR::PDP] = U[U_PDP];
+    R[RR::TODPS] = U[U_TODPS];
+    for (int j = 0; j < T::NABS; ++j) {
+      R[RR::AODPS0 + j] = U[U_AODPS0 + j];
+      R[RR::MASK0 + j] = U[U_MASK0 + j];
+    }
+    RT* uo = w.u + ((size_t)m * Ko + k) * RR::NR;
+    for (int i = 0; i < RR::NR; ++i) uo[i] = (RT)R[i];
     if (s.optran) {
       double O[NO];
       const double tk = t_o[k], t2 = tk * tk, pk = s.ref_p[k], p2 = pk * pk;
