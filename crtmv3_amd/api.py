@@ -66,7 +66,8 @@ class CGeometry(ctypes.Structure):
 class COptions(ctypes.Structure):
     _fields_ = [("use_emissivity", ctypes.c_int), ("emissivity", _vp), ("precision", ctypes.c_int),
                 ("memory", ctypes.c_int), ("jacobian_f32", ctypes.c_int), ("check_input", ctypes.c_int),
-                ("n_streams", ctypes.c_int), ("overlap_id", ctypes.c_int), ("n_streams_profile", _vp)]
+                ("n_streams", ctypes.c_int), ("overlap_id", ctypes.c_int), ("n_streams_profile", _vp),
+                ("no_scattering", ctypes.c_int), ("skip_profile", _vp)]
 
 
 class CRTSolution(ctypes.Structure):
@@ -202,7 +203,7 @@ class Context:
 
     # -- argument marshalling ------------------------------------------------
     def _structs(self, prof, emissivity, precision, memory, jacobian_f32, check_input, keep, n_streams=0,
-                 overlap=0):
+                 overlap=0, include_scattering=True, skip_profile=None):
         def arr(x, dt):
             if x is None:
                 return None
@@ -279,6 +280,17 @@ class Context:
             opt.n_streams = 0
             opt.n_streams_profile = _ptr(ns)
         opt.overlap_id = int(overlap)
+        opt.no_scattering = 0 if include_scattering else 1
+        if skip_profile is not None:   # Options(m)%Skip_Profile, in the call's memory
+            if _is_torch(skip_profile):
+                import torch
+                sk = skip_profile.to(torch.int32).contiguous()
+            else:
+                sk = np.ascontiguousarray(np.asarray(skip_profile), np.int32)
+            if sk.shape[0] != prof.n_profiles:
+                raise CRTMError(f"skip_profile: {sk.shape[0]} values for {prof.n_profiles} profiles")
+            keep.append(sk)
+            opt.skip_profile = _ptr(sk)
         return atm, sfc, geo, opt
 
     def _alloc(self, shape, dtype, memory, like=None):
@@ -289,19 +301,21 @@ class Context:
         return np.zeros(shape, dtype)
 
     def forward(self, prof, emissivity, precision=PREC_FP64, extras=False, check_input=True, n_streams=0,
-                overlap=0):
+                overlap=0, include_scattering=True, skip_profile=None):
         """CRTM_Forward over a ProfileSet (host numpy) or a device batch (torch).
 
         Profiles carrying clouds take the all-sky path; n_streams is
         Options%n_Streams (0: chosen from the Mie parameter), or one value per
         profile (Options(m)%n_Streams), and overlap Options%Overlap_Id
-        (0: average)."""
+        (0: average).  include_scattering=False is Options%Include_Scattering
+        = .FALSE. (clouds and aerosols absorb only); skip_profile [M] holds
+        Options(m)%Skip_Profile (skipped profiles: zero outputs, SUCCESS)."""
         memory = MEM_DEVICE if _is_torch(prof.temperature) else MEM_HOST
         if memory == MEM_DEVICE:
             self._use_torch_stream()
         keep = []
         atm, sfc, geo, opt = self._structs(prof, emissivity, precision, memory, False, check_input, keep,
-                                           n_streams, overlap)
+                                           n_streams, overlap, include_scattering, skip_profile)
         M, L = prof.n_profiles, self.n_channels
         out = {k: self._alloc((M, L), np.float64, memory, prof.temperature)
                for k in ("radiance", "brightness_temperature")}
@@ -320,7 +334,7 @@ class Context:
 
     def k_matrix(self, prof, emissivity, bt_k=None, rad_k=None, precision=PREC_FP64,
                  jacobian_f32=False, check_input=True, out=None, n_streams=0, overlap=0, extras=False,
-                 no_alloc=False):
+                 no_alloc=False, include_scattering=True, skip_profile=None):
         """CRTM_K_Matrix: returns (RTSolution dict, K dict).
 
         K layout: temperature_k [NT, L] (profile m rows layer_offset[m]:...),
@@ -333,13 +347,16 @@ class Context:
         in forward(); extras adds R_Clear, Tb_Clear and the cloud cover of a
         cloudy call to the RTSolution dict.  no_alloc calls
         crtm_k_matrix_noalloc (clear-sky device batches: the library's work
-        space comes from its fixed scratch array, nothing is allocated)."""
+        space comes from its fixed scratch array, nothing is allocated).
+        include_scattering / skip_profile as in forward(); a skipped profile's
+        Jacobians are zero and its radiance_k is left as given."""
         memory = MEM_DEVICE if _is_torch(prof.temperature) else MEM_HOST
         if memory == MEM_DEVICE:
             self._use_torch_stream()
         keep = []
         atm, sfc, geo, opt = self._structs(prof, emissivity, precision, memory, jacobian_f32,
-                                           check_input, keep, n_streams, overlap)
+                                           check_input, keep, n_streams, overlap, include_scattering,
+                                           skip_profile)
         nc, nae = int(atm.n_clouds), int(atm.n_aerosols)
         M, L = prof.n_profiles, self.n_channels
         host_nl = getattr(prof, "n_layers_host", None)

@@ -75,13 +75,13 @@ def _compile(src: str, force: bool) -> str:
 
 
 FORTRAN = os.environ.get("FC_AMD", "/opt/rocm/bin/amdflang")
-F_SOURCES = ["crtm_mi355x_module.f90", "crtm_mi355x_driver.f90"]
+F_SOURCES = ["crtm_mi355x_module.f90", "crtm_module.f90", "crtm_mi355x_driver.f90"]
 DRIVER = os.path.join(HERE, "crtm_mi355x_driver")
 
 
 def build_fortran() -> str:
-    """Fortran host interface (CRTM_MI355X_Module) and its example driver,
-    linked against libcrtm_mi355x.so."""
+    """Fortran host interface (CRTM_MI355X_Module, re-exported as
+    CRTM_Module) and its example driver, linked against libcrtm_mi355x.so."""
     fdir = os.path.join(OBJ, "fortran")
     os.makedirs(fdir, exist_ok=True)
     objs = []

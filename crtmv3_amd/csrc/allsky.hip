@@ -130,11 +130,14 @@ __global__ void __launch_bounds__(64) cloud_prep_kernel(DevBatch be, DevClouds c
   for (int n = 0; n < cl.na; ++n)
     if (cl.atype[(int64_t)m * cl.na + n] != 0)
       for (int ku = 0; ku < K - nx; ++ku) max_reff = fmax(max_reff, cl.areff[n * NTu + lu + ku]);
+  // a skipped profile (Options(m)%Skip_Profile) takes no scattering slots
+  const bool skip = be.skip && be.skip[m];
+  if (skip) fractional = false;
   int count = 0;
   for (int kk = 0; kk < K; ++kk) {
     bool cloudy = false;
     for (int n = 0; n < nc; ++n) cloudy |= cw.wc[n * NTe + le + kk] > WATER_CONTENT_THRESHOLD;
-    const bool scat = cloudy || aerosol_layer(kk);
+    const bool scat = !skip && (cloudy || aerosol_layer(kk));
     count += scat;
     cw.cl_slot[le + kk] = scat ? 0 : -1;   // slot numbers follow in cloud_slot_kernel
   }
@@ -219,7 +222,7 @@ __global__ void __launch_bounds__(256) cloud_optics_kernel(DevSensor s, DevBatch
       const double wc = cw.wc[n * be.NT + ik];
       if (wc > WATER_CONTENT_THRESHOLD)
         add_cloud(cc, mw, freq, cl.type[(int64_t)m * cl.nc + n], cw.reff[n * be.NT + ik], be.t[ik], wc, nl, loff,
-                  od, ssa, pl);
+                  cl.no_scat != 0, od, ssa, pl);
     }
     // CRTM_Compute_AerosolScatter (src/AtmScatter/CRTM_AerosolScatter.f90:170-317): infrared only
     const int ku = k - be.n_extra[m];
@@ -233,7 +236,8 @@ __global__ void __launch_bounds__(256) cloud_optics_kernel(DevSensor s, DevBatch
         if (ty == 0 || !(conc > AEROSOL_CONTENT_THRESHOLD)) continue;
         const int ka = aerosol_index(cc.aero, ty);
         if (aoff < 0 || ka < 0) continue;   // failed profile (aerosol_check_kernel)
-        add_aerosol(cc.aero, s.wavenumber[l], ka, cl.areff[n * be.out_NT + iu], conc, nl, aoff, od, ssa, pl);
+        add_aerosol(cc.aero, s.wavenumber[l], ka, cl.areff[n * be.out_NT + iu], conc, nl, aoff, cl.no_scat != 0, od,
+                    ssa, pl);
       }
     }
     // CRTM_AtmOptics_Combine: P_l /= ssa, P_0 = 1/2, delta truncation f = P_nl
@@ -627,6 +631,7 @@ __global__ void aerosol_check_kernel(DevSensor s, int M, DevClouds cl, CloudWork
 __global__ void status_merge_kernel(int M, const int* prep_status, const int* aer_err, int* call_status) {
   const int m = blockIdx.x * blockDim.x + threadIdx.x;
   if (m >= M) return;
+  if (prep_status[m] == ST_SKIP) return;   // a skipped profile does not fail the call
   int st = call_status[m];
   if (prep_status[m] > st) st = prep_status[m];
   if (aer_err && aer_err[m]) {

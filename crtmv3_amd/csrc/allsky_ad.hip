@@ -424,8 +424,25 @@ struct CloudLU {
   W4 u, v, t, dv, dt;
 };
 
+// Options%Include_Scattering = .FALSE.: the extinction is the absorption
+// ke (1 - w), taken before the clean-up, with its derivatives; nothing
+// scatters (Get_Cloud_Opt_IR/MW and their adjoints,
+// src/AtmScatter/CRTM_CloudScatter.f90:994-997, 1213-1221, 1815-1822;
+// Get_Aerosol_Opt, CRTM_AerosolScatter.f90:845-848, 1260-1268).  Types
+// without an albedo table carry w = 0 and keep their ke.
+__device__ inline void absorption_only(CloudLU& c) {
+  const double a = 1.0 - c.w;
+  c.ke_r = c.ke_r * a - c.ke * c.w_r;
+  c.ke_t = c.ke_t * a - c.ke * c.w_t;
+  c.ke *= a;
+  c.w = 0.0;
+  c.w_r = 0.0;
+  c.w_t = 0.0;
+  c.pct = nullptr;
+}
+
 __device__ CloudLU cloud_lookup(const DevCloudCoeff& cc, bool mw, double freq, int type, double reff, double temp,
-                                int loff) {
+                                int loff, bool absorb) {
   CloudLU c{};
   c.w = 0.0;
   if (mw) {
@@ -496,6 +513,7 @@ __device__ CloudLU cloud_lookup(const DevCloudCoeff& cc, bool mw, double freq, i
     c.s2 = nF;
     c.dims = 2;
   }
+  if (absorb) absorption_only(c);
   c.ke_on = c.ke > 0.0;
   c.w_on = c.ke_on && c.w > 0.0 && c.w < 1.0;
   c.pc_on = c.ke_on && c.w > 0.0 && c.pct != nullptr;
@@ -506,7 +524,8 @@ __device__ CloudLU cloud_lookup(const DevCloudCoeff& cc, bool mw, double freq, i
 }
 // One aerosol (Get_Aerosol_Opt, 'CRTM' scheme) with its radius derivatives,
 // in the CloudLU form (no temperature dependence).
-__device__ CloudLU aerosol_lookup(const DevAerosolCoeff& ac, double wavenumber, int k, double reff, int aoff) {
+__device__ CloudLU aerosol_lookup(const DevAerosolCoeff& ac, double wavenumber, int k, double reff, int aoff,
+                                  bool absorb) {
   CloudLU c{};
   c.u = weights(ac.freq, ac.nw, wavenumber);
   const W4D rd = weights_d(ac.reff + (int64_t)k * ac.nr, ac.nr, reff);
@@ -522,6 +541,7 @@ __device__ CloudLU aerosol_lookup(const DevAerosolCoeff& ac, double wavenumber, 
   c.sl = sL;
   c.s2 = sR;
   c.dims = 2;
+  if (absorb) absorption_only(c);
   c.ke_on = c.ke > 0.0;
   c.w_on = c.ke_on && c.w > 0.0 && c.w < 1.0;
   c.pc_on = c.ke_on && c.w > 0.0;
@@ -1312,7 +1332,7 @@ __global__ void __launch_bounds__(256) cloud_optics_ad_kernel(DevSensor s, DevBa
       const double wc = cw.wc[n * be.NT + ik];
       if (!(wc > WATER_CONTENT_THRESHOLD)) continue;
       const CloudLU lu = cloud_lookup(cc, mw, freq, cl.type[(int64_t)m * cl.nc + n], cw.reff[n * be.NT + ik], temp,
-                                      loff);
+                                      loff, cl.no_scat != 0);
       od0 += lu.ke * wc;
       const double bs = wc * lu.ke * lu.w;
       ssa0 += bs;
@@ -1337,7 +1357,8 @@ __global__ void __launch_bounds__(256) cloud_optics_ad_kernel(DevSensor s, DevBa
         const double conc = cl.aconc[n * be.out_NT + iu];
         const int ka = ty != 0 ? aerosol_index(cc.aero, ty) : -1;
         if (ka < 0 || !(conc > AEROSOL_CONTENT_THRESHOLD)) continue;
-        const CloudLU lu = aerosol_lookup(cc.aero, s.wavenumber[l], ka, cl.areff[n * be.out_NT + iu], aoff);
+        const CloudLU lu = aerosol_lookup(cc.aero, s.wavenumber[l], ka, cl.areff[n * be.out_NT + iu], aoff,
+                                                 cl.no_scat != 0);
         od0 += lu.ke * conc;
         const double bs = conc * lu.ke * lu.w;
         ssa0 += bs;
@@ -1394,7 +1415,7 @@ __global__ void __launch_bounds__(256) cloud_optics_ad_kernel(DevSensor s, DevBa
       const double wc = cw.wc[n * be.NT + ik];
       if (!(wc > WATER_CONTENT_THRESHOLD)) continue;
       const CloudLU lu = cloud_lookup(cc, mw, freq, cl.type[(int64_t)m * cl.nc + n], cw.reff[n * be.NT + ik], temp,
-                                      loff);
+                                      loff, cl.no_scat != 0);
       const double bs = wc * lu.ke * lu.w;
       double s0 = 0.0, sr = 0.0, stt = 0.0;
       if (lu.pc_on) {
@@ -1437,7 +1458,8 @@ __global__ void __launch_bounds__(256) cloud_optics_ad_kernel(DevSensor s, DevBa
         const double conc = cl.aconc[n * be.out_NT + iu];
         const int ka = ty != 0 ? aerosol_index(cc.aero, ty) : -1;
         if (ka < 0 || !(conc > AEROSOL_CONTENT_THRESHOLD)) continue;
-        const CloudLU lu = aerosol_lookup(cc.aero, s.wavenumber[l], ka, cl.areff[n * be.out_NT + iu], aoff);
+        const CloudLU lu = aerosol_lookup(cc.aero, s.wavenumber[l], ka, cl.areff[n * be.out_NT + iu], aoff,
+                                                 cl.no_scat != 0);
         const double bs = conc * lu.ke * lu.w;
         double s0 = 0.0, sr = 0.0;
         if (lu.pc_on) {

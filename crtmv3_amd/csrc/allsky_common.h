@@ -115,9 +115,12 @@ __device__ inline double tab3(const double* z, int64_t s2, int64_t s3, const W4&
 
 // Get_Cloud_Opt_MW / Get_Cloud_Opt_IR for one cloud of one layer, with the
 // CRTM_Compute_CloudScatter clean-up and accumulation (ke <= 0, w <= 0,
-// w >= 1) into od, bs and pl[1..nl].
+// w >= 1) into od, bs and pl[1..nl].  absorb (Options%Include_Scattering =
+// .FALSE.): the extinction becomes the absorption ke (1 - w) before the
+// clean-up and nothing is scattered (src/AtmScatter/CRTM_CloudScatter.f90:
+// 986-997, 1372-1386, 1395-1407).
 __device__ void add_cloud(const DevCloudCoeff& cc, bool mw, double freq, int type, double reff, double temp,
-                          double wc, int nl, int loff, double& od, double& bs_sum, double* pl) {
+                          double wc, int nl, int loff, bool absorb, double& od, double& bs_sum, double* pl) {
   double ke, w = 0.0;
   const double* pct = nullptr;   // phase-coefficient table of slot l at pct + l * sl
   int64_t sl = 0, s2 = 0, s3 = 0;
@@ -173,6 +176,11 @@ __device__ void add_cloud(const DevCloudCoeff& cc, bool mw, double freq, int typ
     s2 = nF;
     dims = 2;
   }
+  if (absorb) {   // types without an albedo table keep w = 0 and their ke
+    ke *= 1.0 - w;
+    w = 0.0;
+    pct = nullptr;
+  }
   if (ke <= 0.0) { ke = 0.0; w = 0.0; }
   if (w <= 0.0) w = 0.0;
   if (w >= 1.0) w = 1.0;
@@ -204,15 +212,20 @@ __device__ inline int aerosol_index(const DevAerosolCoeff& ac, int type) {
 // Get_Aerosol_Opt, 'CRTM' scheme (CRTM_AerosolScatter.f90:772-840): 2-D
 // lookup over frequency and the type's radius grid at the first RH and
 // size-variance entries, with the CRTM_Compute_AerosolScatter clean-up and
-// accumulation into od, bs and pl[1..nl].
+// accumulation into od, bs and pl[1..nl]; absorb as for add_cloud
+// (CRTM_AerosolScatter.f90:838-848).
 __device__ void add_aerosol(const DevAerosolCoeff& ac, double wavenumber, int k, double reff, double conc, int nl,
-                            int aoff, double& od, double& bs_sum, double* pl) {
+                            int aoff, bool absorb, double& od, double& bs_sum, double* pl) {
   const W4 u = weights(ac.freq, ac.nw, wavenumber);
   const W4 v = weights(ac.reff + (int64_t)k * ac.nr, ac.nr, reff);
   const int64_t sR = (int64_t)ac.nw * ac.nrh, sT = sR * ac.nr * ac.ns, sL = sT * ac.nt;
   const int64_t o = u.i1 + sR * v.i1 + sT * k;
   double ke = tab2(ac.ke + o, sR, u, v);
   double w = tab2(ac.w + o, sR, u, v);
+  if (absorb) {
+    ke *= 1.0 - w;
+    w = 0.0;
+  }
   if (ke <= 0.0) { ke = 0.0; w = 0.0; }
   if (w <= 0.0) w = 0.0;
   if (w >= 1.0) w = 1.0;

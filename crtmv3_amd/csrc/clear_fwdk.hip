@@ -155,16 +155,31 @@ clear_fwdk_kernel(DevSensor s, DevBatch bt, ProfWork<RT> w, DevOut o) {
   const int64_t ol = (int64_t)m * bt.Ltot + lg;
   if (in_batch && !ok_prof && lane_in) {
     // a failed profile: NaN radiances, and NaN Jacobian rows so that no
-    // gradient built from them passes silently
-    o.radiance[ol] = __builtin_nan("");
-    o.bt[ol] = __builtin_nan("");
+    // gradient built from them passes silently.  A skipped profile
+    // (Options(m)%Skip_Profile): the zeroed RTSolution and untouched (zero)
+    // Jacobians of the reference (src/CRTM_K_Matrix_Module.f90:594, 631)
+    const bool skipped = w.status[m] == ST_SKIP;
+    const double fill = skipped ? 0.0 : __builtin_nan("");
+    o.radiance[ol] = fill;
+    o.bt[ol] = fill;
+    if (skipped) {
+      if (o.up_radiance) o.up_radiance[ol] = 0.0;
+      if (o.down_radiance) o.down_radiance[ol] = 0.0;
+      if (o.sfc_planck) o.sfc_planck[ol] = 0.0;
+      if (o.sfc_emis) o.sfc_emis[ol] = 0.0;
+    }
     if constexpr (DOK) {
       const int64_t r0 = bt.out_loff[m], r1 = bt.out_loff[m + 1];
       const int64_t astr_f = bt.out_NT * bt.Ltot;
       for (int64_t r = r0; r < r1; ++r) {
-        reinterpret_cast<KT*>(o.t_k)[r * bt.Ltot + lg] = KT(__builtin_nan(""));
+        reinterpret_cast<KT*>(o.t_k)[r * bt.Ltot + lg] = KT(fill);
         for (int ja = 0; ja < bt.na; ++ja)
-          reinterpret_cast<KT*>(o.a_k)[ja * astr_f + r * bt.Ltot + lg] = KT(__builtin_nan(""));
+          reinterpret_cast<KT*>(o.a_k)[ja * astr_f + r * bt.Ltot + lg] = KT(fill);
+      }
+      if (skipped) {
+        if (o.emis_k) o.emis_k[ol] = 0.0;
+        for (int c = 0; c < 4; ++c)
+          if (o.sfc_t_k[c]) o.sfc_t_k[c][ol] = 0.0;
       }
     }
   }

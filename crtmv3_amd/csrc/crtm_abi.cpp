@@ -760,6 +760,7 @@ struct crtm_context {
   int64_t cloud_cap = 0;
   // device copy of per-profile stream counts given in host memory to a device-memory call
   DevBuf nsprof;
+  DevBuf skipbuf;      // Skip_Profile flags staged from host memory in device-memory calls
   // kernel timing (crtm_profile_enable): HIP events on the context stream
   bool profiling = false;
   std::vector<hipEvent_t> ev;      // pairs (start, stop) per recorded launch
@@ -1304,6 +1305,7 @@ int do_call(crtm_context* ctx, const crtm_atmosphere* atm, const crtm_surface* s
     cc.cl.na = atm->n_aerosols;
     cc.cl.n_streams = opt->n_streams;
     cc.cl.overlap = opt->overlap_id > 0 ? opt->overlap_id : 4;
+    cc.cl.no_scat = opt->no_scattering != 0;
   }
   const bool host = opt->memory == CRTM_MEM_HOST;
   const bool split = opt->precision == CRTM_PREC_FP32_SPLIT;
@@ -1343,6 +1345,7 @@ int do_call(crtm_context* ctx, const crtm_atmosphere* atm, const crtm_surface* s
   abs_id.resize(NA_MAX, 0);
   if (opt->check_input && host) {
     for (int m = 0; m < M; ++m) {
+      if (opt->skip_profile && opt->skip_profile[m]) continue;   // checked after the skip in the reference
       const double* lp = atm->level_pressure + off[m] + m;
       for (int kk = 0; kk < nl[m]; ++kk)
         if (!(lp[kk + 1] > lp[kk]) || !(atm->temperature[off[m] + kk] > 0.0))
@@ -1374,6 +1377,7 @@ int do_call(crtm_context* ctx, const crtm_atmosphere* atm, const crtm_surface* s
     in(atm->temperature, NT * sizeof(double), (const void**)&b.t);
     in(atm->absorber, na * NT * sizeof(double), (const void**)&b.a);
     in(atm->climatology, M * sizeof(int), (const void**)&b.climatology);
+    in(opt->skip_profile, M * sizeof(int), (const void**)&b.skip);
     if (cc.on) {
       const size_t nc = atm->n_clouds;
       in(atm->cloud_type, M * nc * sizeof(int), (const void**)&cc.cl.type);
@@ -1487,6 +1491,14 @@ int do_call(crtm_context* ctx, const crtm_atmosphere* atm, const crtm_surface* s
   b.t = atm->temperature;
   b.a = atm->absorber;
   b.climatology = atm->climatology;
+  b.skip = opt->skip_profile;
+  if (b.skip && host_resident(b.skip)) {   // stage host flags for the kernels
+    if (capture != hipStreamCaptureStatusNone)
+      return fail(ctx, "graph capture: skip_profile must be device memory in a captured call");
+    int* d = static_cast<int*>(take(ctx, ctx->skipbuf, (size_t)M * sizeof(int)));
+    HIPCHECK(hipMemcpyAsync(d, b.skip, (size_t)M * sizeof(int), hipMemcpyHostToDevice, ctx->stream));
+    b.skip = d;
+  }
   b.cov[0] = sfc->land_coverage; b.cov[1] = sfc->water_coverage; b.cov[2] = sfc->snow_coverage; b.cov[3] = sfc->ice_coverage;
   b.sfc_t[0] = sfc->land_temperature; b.sfc_t[1] = sfc->water_temperature;
   b.sfc_t[2] = sfc->snow_temperature; b.sfc_t[3] = sfc->ice_temperature;
@@ -1686,6 +1698,8 @@ int crtm_k_matrix_noalloc(crtm_context* ctx, const crtm_atmosphere* atm, const c
     return fail(ctx, "crtm_k_matrix_noalloc: give max_layers (1..200) and total_layers");
   if (ctx->profiling)
     return fail(ctx, "crtm_k_matrix_noalloc: kernel timing creates HIP events; disable crtm_profile_enable");
+  if (opt->skip_profile && host_resident(opt->skip_profile))
+    return fail(ctx, "crtm_k_matrix_noalloc: skip_profile must be device memory");
   const int dev = ctx->device;
   char* base = (dev >= 0 && dev < MAX_DEVICES) ? g_scratch_ptr[dev] : nullptr;
   if (!base) return fail(ctx, fmt("crtm_k_matrix_noalloc: no scratch array address on device %d", dev));

@@ -19,6 +19,9 @@ constexpr int MAX_OPTRAN_ORDER = 10;
 constexpr int N_OPTRAN_PRED = 14;
 constexpr int OPZ = 15;            // OPTRAN predictor rows incl. constant row 0
 constexpr int OPO = 11;            // OPTRAN polynomial orders 0..10
+// Preparation status of a profile skipped by Options(m)%Skip_Profile: no
+// kernel computes it, its outputs are zero and it does not fail the call
+constexpr int ST_SKIP = 7;
 
 template <class RT> __device__ inline RT rexp(RT x);
 template <> __device__ inline float rexp<float>(float x) { return __expf(x); }
@@ -159,6 +162,7 @@ struct DevBatch {
   const int* n_extra;       // [M] layers added on top of profile m
   const int64_t* out_loff;  // [M+1] caller's layer offsets (Jacobian rows)
   int64_t out_NT;           // caller's total layers (absorber Jacobian stride)
+  const int* skip;          // [M] or null: Options(m)%Skip_Profile
 };
 
 // Extended profiles written by the AddLayers kernel: profile m occupies
@@ -305,6 +309,7 @@ struct DevClouds {
   const int* atype;     // [M][na] aerosol type id (0: empty slot)
   const double* aconc;  // [na][out_NT] kg/m^2
   const double* areff;  // [na][out_NT] microns
+  int no_scat;          // Options%Include_Scattering = .FALSE.: absorption only
 };
 
 // Per-call cloud work (extended layer layout)

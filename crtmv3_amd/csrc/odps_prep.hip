@@ -53,7 +53,7 @@ __global__ void __launch_bounds__(64) add_layers_kernel(DevBatch in, ExtBatch e,
                                                         int ids0, int ids1, int ids2, int ids3, int ids4, int ids5) {
   const int m = blockIdx.x, tid = threadIdx.x;
   const int ids[NA_MAX] = {ids0, ids1, ids2, ids3, ids4, ids5};
-  const int K = in.n_layers[m];
+  const int K = (in.skip && in.skip[m]) ? 0 : in.n_layers[m];   // a skipped profile is not extended
   const int64_t lo = in.loff[m];
   const double* lp = in.level_p + lo + m;
   const int na = in.na;
@@ -144,6 +144,10 @@ __global__ void __launch_bounds__(64) layer_map_kernel(DevSensor s, DevBatch bt,
   __syncthreads();
   const int m = blockIdx.x * 64 + threadIdx.x;
   if (m >= bt.M) return;
+  if (bt.skip && bt.skip[m]) {   // Options(m)%Skip_Profile: no kernel computes the profile
+    w.status[m] = ST_SKIP;
+    return;
+  }
   const int K = bt.n_layers[m];
   if (K < 1 || K > K_MAX || K > bt.max_k || Ko > KO_MAX) {
     w.status[m] = ST_LAYERS;

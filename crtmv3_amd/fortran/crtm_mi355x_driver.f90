@@ -1,15 +1,17 @@
-! crtm_mi355x_driver.f90 -- Fortran caller of CRTM_MI355X_Module written the
-! way a CRTM application calls CRTM_Module: CRTM_Init, CRTM_Forward,
+! crtm_mi355x_driver.f90 -- Fortran caller written the way a CRTM application
+! is: USE CRTM_Module, then CRTM_Init, CRTM_Forward,
 ! CRTM_K_Matrix with zeroed Atmosphere_K / Surface_K, CRTM_Adjoint with
 ! zeroed Atmosphere_AD / Surface_AD, CRTM_Tangent_Linear, CRTM_Destroy.
 ! Reads a batch from a text file and writes BT, T_K, absorber_K, water
 ! temperature K and emissivity K as text, and the adjoint (seeded with
 ! BT_AD(l) = 0.5 + l/L) and tangent linear (T_TL = 1 K, absorber_TL = 1 %,
-! water temperature_TL = 0.5 K) to <output.txt>.ad (used by the Fortran tests).
+! water temperature_TL = 0.5 K) to <output.txt>.ad, and a forward call with
+! Options%Skip_Profile on the last profile to <output.txt>.skip (used by the
+! Fortran tests).
 !
 !   crtm_mi355x_driver <input.txt> <output.txt>
 PROGRAM crtm_mi355x_driver
-  USE CRTM_MI355X_Module
+  USE CRTM_Module
   IMPLICIT NONE
   CHARACTER(LEN=1024) :: fin, fout, cdir
   CHARACTER(LEN=20) :: sid(1)
@@ -116,6 +118,18 @@ PROGRAM crtm_mi355x_driver
         WRITE(u, '(*(ES25.16E3))') (atm_k(l1, m1)%Absorber(k1, j), l1 = 1, L)
       END DO
     END DO
+  END DO
+  CLOSE(u)
+
+  ! Options%Skip_Profile on the last profile: its RTSolution is zero and the
+  ! others are unchanged; RTSolution carries the sensor channel numbers
+  opt(M)%Skip_Profile = .TRUE.
+  err = CRTM_Forward(atm, sfc, geo, chinfo, rts_tl, opt)
+  IF (err /= SUCCESS) STOP 'CRTM_Forward with Skip_Profile failed'
+  OPEN(NEWUNIT=u, FILE=TRIM(fout)//'.skip', STATUS='REPLACE', ACTION='WRITE')
+  WRITE(u, '(*(I8))') (rts_tl(l1, 1)%Sensor_Channel, l1 = 1, L)
+  DO m1 = 1, M
+    WRITE(u, '(*(ES25.16E3))') (rts_tl(l1, m1)%Brightness_Temperature, l1 = 1, L)
   END DO
   CLOSE(u)
   err = CRTM_Destroy(chinfo)

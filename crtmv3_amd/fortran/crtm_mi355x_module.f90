@@ -2,8 +2,9 @@
 !
 ! A thin ISO_C_BINDING layer over libcrtm_mi355x.so (include/crtm_mi355x.h)
 ! that keeps the names, argument order and result codes of CRTM's public
-! API for the forward / K-matrix path (clear and cloudy), so a caller written against
-! CRTM_Module (src/CRTM_Module.F90:7-38) compiles unchanged for that path:
+! API for the forward / K-matrix path (clear and cloudy); crtm_module.f90
+! re-exports it as MODULE CRTM_Module (src/CRTM_Module.F90:7-38), so a caller
+! written against CRTM_Module compiles unchanged for that path:
 !
 !   CRTM_Init(Sensor_ID, ChannelInfo, File_Path)          src/CRTM_LifeCycle.f90:520
 !   CRTM_Forward(Atm, Sfc, Geo, ChannelInfo, RTS, Opt)    src/CRTM_Forward_Module.f90:229-245
@@ -20,15 +21,37 @@
 ! src/RTSolution/CRTM_RTSolution_Define.f90:201-245,
 ! src/Options/CRTM_Options_Define.f90:122-179).  As in the reference,
 ! Atmosphere_K / Surface_K are accumulated into (callers zero them first).
+! Components of the reference types that feed models outside this path
+! (surface optics, NLTE, antenna correction, solar geometry) are present so
+! that callers compile; the path needs Options%Use_Emissivity and ignores
+! them.  Helpers a CRTM caller uses around the calls (CRTM_Version,
+! Display_Message, CRTM_Geometry_SetValue, CRTM_ChannelInfo_n_Channels,
+! CRTM_*_Associated / _Destroy / _Inspect) are provided with the reference's
+! names and arguments.
 MODULE CRTM_MI355X_Module
   USE ISO_C_BINDING
   IMPLICIT NONE
   PRIVATE
 
   INTEGER, PARAMETER, PUBLIC :: fp = C_DOUBLE
-  INTEGER, PARAMETER, PUBLIC :: SUCCESS = 0, FAILURE = 3
-  INTEGER, PARAMETER, PUBLIC :: H2O_ID = 1, CO2_ID = 2, O3_ID = 3
-  INTEGER, PARAMETER, PUBLIC :: VOLUME_MIXING_RATIO_UNITS = 1, MASS_MIXING_RATIO_UNITS = 3
+  REAL(fp), PARAMETER, PUBLIC :: ZERO = 0.0_fp, ONE = 1.0_fp
+  ! src/Utility/Message_Handler.f90:27-31
+  INTEGER, PARAMETER, PUBLIC :: SUCCESS = 0, INFORMATION = 1, WARNING = 2, FAILURE = 3, UNDEFINED = 4
+  ! src/Coefficients/SensorInfo/SensorInfo_Parameters.f90 (sensor types)
+  INTEGER, PARAMETER, PUBLIC :: INVALID_SENSOR = 0, MICROWAVE_SENSOR = 1, INFRARED_SENSOR = 2, &
+                                VISIBLE_SENSOR = 3, ULTRAVIOLET_SENSOR = 4
+  INTEGER, PARAMETER, PUBLIC :: H2O_ID = 1, CO2_ID = 2, O3_ID = 3, N2O_ID = 4, CO_ID = 5, CH4_ID = 6
+  INTEGER, PARAMETER, PUBLIC :: VOLUME_MIXING_RATIO_UNITS = 1, NUMBER_DENSITY_UNITS = 2, &
+                                MASS_MIXING_RATIO_UNITS = 3, MASS_DENSITY_UNITS = 4, PARTIAL_PRESSURE_UNITS = 5
+  ! src/Atmosphere/CRTM_Atmosphere_Define.f90:404-411
+  INTEGER, PARAMETER, PUBLIC :: INVALID_MODEL = 0, TROPICAL = 1, MIDLATITUDE_SUMMER = 2, MIDLATITUDE_WINTER = 3, &
+                                SUBARCTIC_SUMMER = 4, SUBARCTIC_WINTER = 5, US_STANDARD_ATMOSPHERE = 6
+  ! src/CRTM_Parameters.f90:286-294
+  INTEGER, PARAMETER, PUBLIC :: INVALID_AEROSOL = 0, DUST_AEROSOL = 1, SEASALT_SSAM_AEROSOL = 2, &
+                                SEASALT_SSCM1_AEROSOL = 3, SEASALT_SSCM2_AEROSOL = 4, SEASALT_SSCM3_AEROSOL = 5, &
+                                ORGANIC_CARBON_AEROSOL = 6, BLACK_CARBON_AEROSOL = 7, SULFATE_AEROSOL = 8
+  ! src/CRTM_Parameters.f90:247-250: the ADA solver is the one this path implements
+  INTEGER, PARAMETER, PUBLIC :: RT_ADA = 56, RT_AMOM = 56, RT_SOI = 168
   INTEGER, PARAMETER, PUBLIC :: CRTM_PREC_FP64 = 0, CRTM_PREC_FP32_SPLIT = 1
 
   TYPE, PUBLIC :: CRTM_ChannelInfo_type
@@ -70,33 +93,58 @@ MODULE CRTM_MI355X_Module
     TYPE(CRTM_Aerosol_type), ALLOCATABLE :: Aerosol(:) ! (n_Aerosols)
   END TYPE
 
+  ! src/Surface/CRTM_Surface_Define.f90:204-244: the coverages and
+  ! temperatures enter this path; the type and state components feed the
+  ! surface-optics models, which this path replaces by Options%Emissivity
   TYPE, PUBLIC :: CRTM_Surface_type
     REAL(fp) :: Land_Coverage = 0, Water_Coverage = 0, Snow_Coverage = 0, Ice_Coverage = 0
     REAL(fp) :: Land_Temperature = 283, Water_Temperature = 283, Snow_Temperature = 263, Ice_Temperature = 263
+    INTEGER :: Land_Type = 0, Water_Type = 1, Snow_Type = 0, Ice_Type = 0
+    INTEGER :: Soil_Type = 0, Vegetation_Type = 0
+    REAL(fp) :: Soil_Moisture_Content = 0.05_fp, Canopy_Water_Content = 0.05_fp, Vegetation_Fraction = 0.3_fp
+    REAL(fp) :: Soil_Temperature = 283, Lai = 3.5_fp
+    REAL(fp) :: Wind_Speed = 5, Wind_Direction = 0, Salinity = 33
+    REAL(fp) :: Snow_Depth = 50, Snow_Density = 0.2_fp, Snow_Grain_Size = 2
+    REAL(fp) :: Ice_Thickness = 10, Ice_Density = 0.9_fp, Ice_Roughness = 0
   END TYPE
 
+  ! src/GeometryInfo/CRTM_Geometry_Define.f90:99-121
   TYPE, PUBLIC :: CRTM_Geometry_type
-    REAL(fp) :: Sensor_Zenith_Angle = 0, Sensor_Scan_Angle = 0, Source_Zenith_Angle = 100
+    INTEGER :: iFOV = 0
+    REAL(fp) :: Longitude = 0, Latitude = 0, Surface_Altitude = 0   ! degrees, degrees, metres
+    REAL(fp) :: Sensor_Scan_Angle = 0, Sensor_Zenith_Angle = 0, Sensor_Azimuth_Angle = 999.9_fp
+    REAL(fp) :: Source_Zenith_Angle = 100, Source_Azimuth_Angle = 0, Flux_Zenith_Angle = 53.130102_fp
+    INTEGER :: Year = 2001, Month = 1, Day = 1
   END TYPE
 
+  ! src/RTSolution/CRTM_RTSolution_Define.f90:201-245 (the components this path produces)
   TYPE, PUBLIC :: CRTM_RTSolution_type
+    CHARACTER(LEN=20) :: Sensor_Id = ''
+    INTEGER :: WMO_Satellite_Id = 1023, WMO_Sensor_Id = 2047, Sensor_Channel = 0
     REAL(fp) :: Radiance = 0, Brightness_Temperature = 0
     REAL(fp) :: Up_Radiance = 0, Down_Radiance = 0, Surface_Planck_Radiance = 0, Surface_Emissivity = 0
     REAL(fp) :: R_Clear = 0, Tb_Clear = 0, Total_Cloud_Cover = 0
   END TYPE
 
+  ! src/Options/CRTM_Options_Define.f90:122-179 (the options this path honours)
   TYPE, PUBLIC :: CRTM_Options_type
+    LOGICAL :: Check_Input = .TRUE.
     LOGICAL :: Use_Emissivity = .FALSE.
     REAL(fp), ALLOCATABLE :: Emissivity(:)         ! (n_Channels)
-    LOGICAL :: Check_Input = .TRUE.
     LOGICAL :: Use_N_Streams = .FALSE.
     INTEGER :: n_Streams = 0
+    LOGICAL :: Include_Scattering = .TRUE.
     INTEGER :: Overlap_Id = 4                      ! average overlap (CRTM_CloudCover_Define.f90:79)
+    INTEGER :: RT_Algorithm_Id = RT_ADA
+    LOGICAL :: Skip_Profile = .FALSE.
     INTEGER :: Precision = CRTM_PREC_FP64          ! extension: arithmetic of the GPU path
   END TYPE
 
   PUBLIC :: CRTM_Init, CRTM_Destroy, CRTM_Forward, CRTM_K_Matrix, CRTM_Adjoint, CRTM_Tangent_Linear
   PUBLIC :: CRTM_Atmosphere_Create, CRTM_Atmosphere_Zero, CRTM_Surface_Zero
+  PUBLIC :: CRTM_Atmosphere_Associated, CRTM_Atmosphere_Destroy, CRTM_Atmosphere_Inspect
+  PUBLIC :: CRTM_Surface_Inspect, CRTM_RTSolution_Inspect, CRTM_Geometry_SetValue
+  PUBLIC :: CRTM_ChannelInfo_n_Channels, CRTM_Version, Display_Message, Program_Message
 
   ! ---- C-ABI mirrors (include/crtm_mi355x.h) ----
   TYPE, BIND(C) :: c_atmosphere
@@ -124,6 +172,8 @@ MODULE CRTM_MI355X_Module
     INTEGER(C_INT) :: precision, memory, jacobian_f32, check_input
     INTEGER(C_INT) :: n_streams, overlap_id
     TYPE(C_PTR) :: n_streams_profile
+    INTEGER(C_INT) :: no_scattering
+    TYPE(C_PTR) :: skip_profile
   END TYPE
   TYPE, BIND(C) :: c_rtsolution
     TYPE(C_PTR) :: radiance, brightness_temperature, up_radiance, down_radiance, &
@@ -190,6 +240,8 @@ MODULE CRTM_MI355X_Module
     INTEGER(C_INT), ALLOCATABLE :: nl(:), ids(:), units(:), clim(:)
     INTEGER(C_INT64_T), ALLOCATABLE :: off(:)
     REAL(C_DOUBLE), ALLOCATABLE :: lp(:), p(:), t(:), a(:), cov(:, :), st(:, :), zen(:), scan(:), src(:), emis(:)
+    REAL(C_DOUBLE), ALLOCATABLE :: alt(:)
+    INTEGER(C_INT), ALLOCATABLE :: skip(:)
     REAL(C_DOUBLE), ALLOCATABLE :: rad(:), bt(:), up(:), dn(:), sp(:), se(:), rcl(:), tbc(:), tcc(:)
     INTEGER(C_INT), ALLOCATABLE :: status(:), ctype(:)
     REAL(C_DOUBLE), ALLOCATABLE :: cwc(:), creff(:), cfrac(:)
@@ -358,6 +410,7 @@ CONTAINS
         CALL unpack_rts(b, (mm - 1) * L + ic, RTSolution(ic, mm))
       END DO
     END DO
+    CALL label_rts(ChannelInfo, RTSolution)
   END FUNCTION CRTM_Forward
 
   FUNCTION CRTM_K_Matrix(Atmosphere, Surface, RTSolution_K, Geometry, ChannelInfo, Atmosphere_K, Surface_K, &
@@ -450,6 +503,7 @@ CONTAINS
         Surface_K(ic, mm)%Ice_Temperature = Surface_K(ic, mm)%Ice_Temperature + stk((mm - 1) * L + ic, 4)
       END DO
     END DO
+    CALL label_rts(ChannelInfo, RTSolution)
   END FUNCTION CRTM_K_Matrix
 
   ! CRTM_Adjoint (src/CRTM_Adjoint_Module.f90:299-325): the channel-summed
@@ -676,6 +730,150 @@ CONTAINS
     END DO
   END FUNCTION run_k_chunk
 
+  ! ---- companions of the calls (names and arguments of the reference's) ----
+
+  ! CRTM_Version (src/CRTM_Version.inc): identifies this implementation
+  SUBROUTINE CRTM_Version(Id)
+    CHARACTER(*), INTENT(OUT) :: Id
+    Id = 'CRTM v3 interface, MI355X forward / K-matrix path (libcrtm_mi355x)'
+  END SUBROUTINE CRTM_Version
+
+  ! Display_Message (src/Utility/Message_Handler.f90:110-160): one line
+  ! "routine(STATE) : message" on standard output or appended to Message_Log
+  SUBROUTINE Display_Message(Routine_Name, Message, Error_State, Message_Log)
+    CHARACTER(*), INTENT(IN) :: Routine_Name, Message
+    INTEGER, INTENT(IN) :: Error_State
+    CHARACTER(*), OPTIONAL, INTENT(IN) :: Message_Log
+    CHARACTER(LEN=11), PARAMETER :: STATE(0:4) = (/ 'SUCCESS    ', 'INFORMATION', 'WARNING    ', 'FAILURE    ', &
+                                                   'UNDEFINED  ' /)
+    CHARACTER(*), PARAMETER :: FMT = '( 1x, a, "(", a, ") : ", a )'
+    INTEGER :: st, u, ios
+    st = Error_State
+    IF (st < 0 .OR. st > 4) st = UNDEFINED
+    IF (PRESENT(Message_Log)) THEN
+      OPEN(NEWUNIT=u, FILE=TRIM(Message_Log), POSITION='APPEND', ACTION='WRITE', IOSTAT=ios)
+      IF (ios == 0) THEN
+        WRITE(u, FMT) TRIM(Routine_Name), TRIM(STATE(st)), TRIM(Message)
+        CLOSE(u)
+        RETURN
+      END IF
+    END IF
+    WRITE(*, FMT) TRIM(Routine_Name), TRIM(STATE(st)), TRIM(Message)
+  END SUBROUTINE Display_Message
+
+  ! Program_Message (src/Utility/Message_Handler.f90:48-100): program banner
+  SUBROUTINE Program_Message(Name, Description, Revision)
+    CHARACTER(*), INTENT(IN) :: Name, Description, Revision
+    CHARACTER(*), PARAMETER :: BAR = '**********************************************************'
+    WRITE(*, '(/5x, a)') BAR
+    WRITE(*, '(5x, a)') TRIM(ADJUSTL(Name))
+    WRITE(*, '(/5x, a)') TRIM(Description)
+    WRITE(*, '(/5x, a)') TRIM(Revision)
+    WRITE(*, '(5x, a/)') BAR
+  END SUBROUTINE Program_Message
+
+  ! CRTM_ChannelInfo_n_Channels (src/ChannelInfo/CRTM_ChannelInfo_Define.f90)
+  ELEMENTAL FUNCTION CRTM_ChannelInfo_n_Channels(ChannelInfo) RESULT(n)
+    TYPE(CRTM_ChannelInfo_type), INTENT(IN) :: ChannelInfo
+    INTEGER :: n
+    n = ChannelInfo%n_Channels
+  END FUNCTION CRTM_ChannelInfo_n_Channels
+
+  ! CRTM_Atmosphere_Associated / _Destroy (src/Atmosphere/CRTM_Atmosphere_Define.f90)
+  ELEMENTAL FUNCTION CRTM_Atmosphere_Associated(Atm) RESULT(yes)
+    TYPE(CRTM_Atmosphere_type), INTENT(IN) :: Atm
+    LOGICAL :: yes
+    yes = ALLOCATED(Atm%Level_Pressure) .AND. ALLOCATED(Atm%Temperature) .AND. ALLOCATED(Atm%Absorber)
+  END FUNCTION CRTM_Atmosphere_Associated
+
+  ELEMENTAL SUBROUTINE CRTM_Atmosphere_Destroy(Atm)
+    TYPE(CRTM_Atmosphere_type), INTENT(OUT) :: Atm   ! INTENT(OUT) deallocates every component
+    Atm%n_Layers = 0
+  END SUBROUTINE CRTM_Atmosphere_Destroy
+
+  ! CRTM_Geometry_SetValue (src/GeometryInfo/CRTM_Geometry_Define.f90:365-411)
+  ELEMENTAL SUBROUTINE CRTM_Geometry_SetValue(geo, iFOV, Longitude, Latitude, Surface_Altitude, Sensor_Scan_Angle, &
+                                              Sensor_Zenith_Angle, Sensor_Azimuth_Angle, Source_Zenith_Angle, &
+                                              Source_Azimuth_Angle, Flux_Zenith_Angle, Year, Month, Day)
+    TYPE(CRTM_Geometry_type), INTENT(INOUT) :: geo
+    INTEGER, OPTIONAL, INTENT(IN) :: iFOV, Year, Month, Day
+    REAL(fp), OPTIONAL, INTENT(IN) :: Longitude, Latitude, Surface_Altitude, Sensor_Scan_Angle, &
+                                      Sensor_Zenith_Angle, Sensor_Azimuth_Angle, Source_Zenith_Angle, &
+                                      Source_Azimuth_Angle, Flux_Zenith_Angle
+    IF (PRESENT(iFOV)) geo%iFOV = iFOV
+    IF (PRESENT(Longitude)) geo%Longitude = Longitude
+    IF (PRESENT(Latitude)) geo%Latitude = Latitude
+    IF (PRESENT(Surface_Altitude)) geo%Surface_Altitude = Surface_Altitude
+    IF (PRESENT(Sensor_Scan_Angle)) geo%Sensor_Scan_Angle = Sensor_Scan_Angle
+    IF (PRESENT(Sensor_Zenith_Angle)) geo%Sensor_Zenith_Angle = Sensor_Zenith_Angle
+    IF (PRESENT(Sensor_Azimuth_Angle)) geo%Sensor_Azimuth_Angle = Sensor_Azimuth_Angle
+    IF (PRESENT(Source_Zenith_Angle)) geo%Source_Zenith_Angle = Source_Zenith_Angle
+    IF (PRESENT(Source_Azimuth_Angle)) geo%Source_Azimuth_Angle = Source_Azimuth_Angle
+    IF (PRESENT(Flux_Zenith_Angle)) geo%Flux_Zenith_Angle = Flux_Zenith_Angle
+    IF (PRESENT(Year)) geo%Year = Year
+    IF (PRESENT(Month)) geo%Month = Month
+    IF (PRESENT(Day)) geo%Day = Day
+  END SUBROUTINE CRTM_Geometry_SetValue
+
+  ! CRTM_RTSolution_Inspect / CRTM_Surface_Inspect / CRTM_Atmosphere_Inspect:
+  ! the components of this path, one per line
+  SUBROUTINE CRTM_RTSolution_Inspect(RTS)
+    TYPE(CRTM_RTSolution_type), INTENT(IN) :: RTS
+    WRITE(*, '(1x, "RTSolution OBJECT")')
+    WRITE(*, '(3x, "Sensor Id                     : ", a)') TRIM(RTS%Sensor_Id)
+    WRITE(*, '(3x, "Channel                       : ", i0)') RTS%Sensor_Channel
+    WRITE(*, '(3x, "Surface Emissivity            : ", es22.15)') RTS%Surface_Emissivity
+    WRITE(*, '(3x, "Up Radiance                   : ", es22.15)') RTS%Up_Radiance
+    WRITE(*, '(3x, "Down Radiance                 : ", es22.15)') RTS%Down_Radiance
+    WRITE(*, '(3x, "Surface Planck Radiance       : ", es22.15)') RTS%Surface_Planck_Radiance
+    WRITE(*, '(3x, "Total Cloud Cover             : ", es22.15)') RTS%Total_Cloud_Cover
+    WRITE(*, '(3x, "Radiance (clear)              : ", es22.15)') RTS%R_Clear
+    WRITE(*, '(3x, "Brightness Temperature (clear): ", es22.15)') RTS%Tb_Clear
+    WRITE(*, '(3x, "Radiance                      : ", es22.15)') RTS%Radiance
+    WRITE(*, '(3x, "Brightness Temperature        : ", es22.15)') RTS%Brightness_Temperature
+  END SUBROUTINE CRTM_RTSolution_Inspect
+
+  SUBROUTINE CRTM_Surface_Inspect(Sfc)
+    TYPE(CRTM_Surface_type), INTENT(IN) :: Sfc
+    WRITE(*, '(1x, "Surface OBJECT")')
+    WRITE(*, '(3x, "Land, water, snow, ice coverage   : ", 4es13.6)') Sfc%Land_Coverage, Sfc%Water_Coverage, &
+        Sfc%Snow_Coverage, Sfc%Ice_Coverage
+    WRITE(*, '(3x, "Land, water, snow, ice temperature: ", 4es13.6)') Sfc%Land_Temperature, Sfc%Water_Temperature, &
+        Sfc%Snow_Temperature, Sfc%Ice_Temperature
+  END SUBROUTINE CRTM_Surface_Inspect
+
+  SUBROUTINE CRTM_Atmosphere_Inspect(Atm)
+    TYPE(CRTM_Atmosphere_type), INTENT(IN) :: Atm
+    INTEGER :: j
+    WRITE(*, '(1x, "ATMOSPHERE OBJECT")')
+    WRITE(*, '(3x, "Dimensions (layers, absorbers, clouds, aerosols): ", 4(1x, i0))') Atm%n_Layers, &
+        Atm%n_Absorbers, Atm%n_Clouds, Atm%n_Aerosols
+    IF (.NOT. ALLOCATED(Atm%Temperature)) RETURN
+    WRITE(*, '(3x, "Temperature:")')
+    WRITE(*, '(5(1x, es13.6))') Atm%Temperature
+    DO j = 1, Atm%n_Absorbers
+      WRITE(*, '(3x, "Absorber #", i0, ":")') j
+      WRITE(*, '(5(1x, es13.6))') Atm%Absorber(:, j)
+    END DO
+  END SUBROUTINE CRTM_Atmosphere_Inspect
+
+  ! Sensor_Id / Sensor_Channel of RTSolution(l, :) from the channel order of
+  ! the CRTM_Init call
+  SUBROUTINE label_rts(ChannelInfo, RTS)
+    TYPE(CRTM_ChannelInfo_type), INTENT(IN) :: ChannelInfo(:)
+    TYPE(CRTM_RTSolution_type), INTENT(INOUT) :: RTS(:, :)
+    INTEGER :: n, l, ln
+    ln = 0
+    DO n = 1, SIZE(ChannelInfo)
+      DO l = 1, ChannelInfo(n)%n_Channels
+        ln = ln + 1
+        IF (ln > SIZE(RTS, 1)) RETURN
+        RTS(ln, :)%Sensor_Id = ChannelInfo(n)%Sensor_ID
+        RTS(ln, :)%Sensor_Channel = ChannelInfo(n)%Sensor_Channel(l)
+      END DO
+    END DO
+  END SUBROUTINE label_rts
+
   ! ---- helpers ----
   FUNCTION pack_batch(Atm, Sfc, Geo, ChannelInfo, RTS, Opt, b, L, M) RESULT(Error_Status)
     TYPE(CRTM_Atmosphere_type), INTENT(IN) :: Atm(:)
@@ -704,7 +902,7 @@ CONTAINS
     END DO
     NT = INT(b%off(M + 1))
     ALLOCATE(b%ids(na), b%units(na), b%lp(NT + M), b%p(NT), b%t(NT), b%a(na * NT), b%cov(M, 4), b%st(M, 4), &
-             b%zen(M), b%scan(M), b%src(M), b%emis(L * M), b%rad(L * M), b%bt(L * M), b%up(L * M), &
+             b%zen(M), b%scan(M), b%src(M), b%alt(M), b%skip(M), b%emis(L * M), b%rad(L * M), b%bt(L * M), b%up(L * M), &
              b%dn(L * M), b%sp(L * M), b%se(L * M), b%status(M), b%rcl(L * M), b%tbc(L * M), b%tcc(L * M))
     nc = Atm(1)%n_Clouds
     ALLOCATE(b%ctype(MAX(nc, 1) * M), b%cwc(MAX(nc, 1) * NT), b%creff(MAX(nc, 1) * NT), b%cfrac(NT))
@@ -729,6 +927,17 @@ CONTAINS
       b%zen(mm) = Geo(mm)%Sensor_Zenith_Angle
       b%scan(mm) = Geo(mm)%Sensor_Scan_Angle
       b%src(mm) = Geo(mm)%Source_Zenith_Angle
+      b%alt(mm) = Geo(mm)%Surface_Altitude
+      b%skip(mm) = MERGE(1, 0, Opt(mm)%Skip_Profile)
+      IF (Opt(mm)%Skip_Profile) CYCLE                ! no inputs read (src/CRTM_Forward_Module.f90:536)
+      IF (Opt(mm)%RT_Algorithm_Id /= RT_ADA) THEN
+        CALL Display_Message('CRTM', 'only the ADA solver (RT_ADA) is implemented', FAILURE)
+        RETURN
+      END IF
+      IF (Opt(mm)%Include_Scattering .NEQV. Opt(1)%Include_Scattering) THEN
+        CALL Display_Message('CRTM', 'Include_Scattering must be the same for all profiles of a call', FAILURE)
+        RETURN
+      END IF
       IF (.NOT. Opt(mm)%Use_Emissivity) RETURN
       IF (SIZE(Opt(mm)%Emissivity) < L) RETURN
       b%emis((mm - 1) * L + 1:mm * L) = Opt(mm)%Emissivity(1:L)
@@ -781,7 +990,7 @@ CONTAINS
     b%geo%sensor_zenith_angle = C_LOC(b%zen)
     b%geo%sensor_scan_angle = C_LOC(b%scan)
     b%geo%source_zenith_angle = C_LOC(b%src)
-    b%geo%surface_altitude = C_NULL_PTR
+    b%geo%surface_altitude = C_LOC(b%alt)
     b%opt%use_emissivity = 1
     b%opt%emissivity = C_LOC(b%emis)
     b%opt%precision = Opt(1)%Precision
@@ -791,6 +1000,9 @@ CONTAINS
     b%opt%n_streams = MERGE(Opt(1)%n_Streams, 0, Opt(1)%Use_N_Streams)
     b%opt%overlap_id = Opt(1)%Overlap_Id
     b%opt%n_streams_profile = C_NULL_PTR
+    b%opt%no_scattering = MERGE(0, 1, Opt(1)%Include_Scattering)
+    b%opt%skip_profile = C_NULL_PTR
+    IF (ANY(b%skip /= 0)) b%opt%skip_profile = C_LOC(b%skip)
     IF (SIZE(Opt) == M .AND. M > 1) THEN   ! Options(m)%n_Streams per profile
       ALLOCATE(b%nsp(M))
       DO mm = 1, M

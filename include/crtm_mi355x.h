@@ -1,9 +1,10 @@
 /*
  * crtm_mi355x.h -- C-ABI of the MI355X implementation of CRTM's forward and
  * K-matrix hot path.  Plain pointers and sizes only; every entry point
- * returns a CRTM status code.  This is the boundary a Fortran ISO_C_BINDING
- * shim (crtmv3_amd/fortran/crtm_module.f90) or a Python ctypes binding
- * (crtmv3_amd/api.py) binds.
+ * returns a CRTM status code.  This is the boundary that the Fortran
+ * ISO_C_BINDING layer (crtmv3_amd/fortran/crtm_mi355x_module.f90, re-exported
+ * under the reference's module name by crtmv3_amd/fortran/crtm_module.f90) and
+ * the Python ctypes binding (crtmv3_amd/api.py) bind.
  *
  * Layout conventions
  *   - M profiles, each with K_m layers (top-down, index 0 = top of atmosphere)
@@ -128,9 +129,26 @@ typedef struct {
   const int* n_streams_profile;  /* [M] or NULL: Options(m)%n_Streams per profile (the
                                     reference's Options is per profile), overriding
                                     n_streams; same values.  Host memory is read
-                                    directly in either memory mode (and under graph
-                                    capture); device memory is copied back with one
-                                    synchronisation */
+                                    directly in either memory mode; device memory is
+                                    copied back with one synchronisation.  Under HIP
+                                    graph capture a cloudy call rejects it (the counts
+                                    choose the launches on the host): give one
+                                    n_streams for the batch there */
+  int no_scattering;             /* nonzero: Options%Include_Scattering = .FALSE.: clouds and
+                                    aerosols absorb only (extinction ke (1 - w), no phase
+                                    function) and the radiance is the emission solution
+                                    (src/AtmScatter/CRTM_CloudScatter.f90:986-997, 1372-1407;
+                                    src/RTSolution/Common_RTSolution.f90:263).  0 keeps the
+                                    reference default .TRUE., so zero-initialised options
+                                    behave as the reference's defaults */
+  const int* skip_profile;       /* [M] or NULL: Options(m)%Skip_Profile.  A nonzero entry
+                                    skips profile m: its RTSolution is zero, its Jacobians
+                                    are zero, radiance_k is left as given, and it neither
+                                    fails input checks nor fails the call
+                                    (src/CRTM_Forward_Module.f90:498, 536;
+                                    src/CRTM_K_Matrix_Module.f90:594, 631).  Host memory in
+                                    host-memory calls; device memory in device-memory calls
+                                    (host memory there is staged, outside graph capture) */
 } crtm_options;
 
 /* ---- RTSolution (src/RTSolution/CRTM_RTSolution_Define.f90:201-245) ---- */

@@ -145,6 +145,7 @@ typedef struct {
 typedef struct {
   int n_streams;  /* Options%n_Streams (0: chosen from the Mie parameter) */
   int overlap;    /* Options%Overlap_Id 1..5 (0: default, average) */
+  int no_scattering; /* nonzero: Options%Include_Scattering = .FALSE. (absorption only) */
 } or_scatter_opts;
 
 typedef struct {

@@ -57,7 +57,7 @@ class OrClouds(ctypes.Structure):
 
 
 class OrScatterOpts(ctypes.Structure):
-    _fields_ = [("n_streams", ctypes.c_int), ("overlap", ctypes.c_int)]
+    _fields_ = [("n_streams", ctypes.c_int), ("overlap", ctypes.c_int), ("no_scattering", ctypes.c_int)]
 
 
 class OrAllskyK(ctypes.Structure):
@@ -342,8 +342,10 @@ def make_clouds(ps, m, keep):
     return c
 
 
-def run_allsky(sensor: Sensor, cloudcoeff: CloudCoeff, ps, m, emissivity, n_streams=0, overlap=0):
-    """CRTM_Forward with clouds for profile m; dict of per-channel outputs."""
+def run_allsky(sensor: Sensor, cloudcoeff: CloudCoeff, ps, m, emissivity, n_streams=0, overlap=0,
+               include_scattering=True):
+    """CRTM_Forward with clouds for profile m; dict of per-channel outputs.
+    include_scattering=False: Options%Include_Scattering = .FALSE."""
     keep = []
     p = make_profile(ps, m, keep)
     c = make_clouds(ps, m, keep)
@@ -353,7 +355,7 @@ def run_allsky(sensor: Sensor, cloudcoeff: CloudCoeff, ps, m, emissivity, n_stre
     o = OrAllskyOut()
     for k, v in out.items():
         setattr(o, k, _p(v, _ip if k == "n_streams" else _dp))
-    opt = OrScatterOpts(int(n_streams), int(overlap))
+    opt = OrScatterOpts(int(n_streams), int(overlap), 0 if include_scattering else 1)
     eps = np.ascontiguousarray(emissivity, np.float64)
     rc = lib().oracle_forward_allsky(ctypes.byref(sensor.s),
                                      ctypes.byref(cloudcoeff.c) if cloudcoeff is not None else None,
@@ -365,7 +367,7 @@ def run_allsky(sensor: Sensor, cloudcoeff: CloudCoeff, ps, m, emissivity, n_stre
 
 
 def forward_allsky_batch(sensor: Sensor, cloudcoeff: CloudCoeff, ps, emissivity, n_streams=0, overlap=0,
-                         n_threads=0):
+                         n_threads=0, include_scattering=True):
     """All profiles of a cloudy ProfileSet (OpenMP over profiles); (radiance, bt) [M, L]."""
     M, L = ps.n_profiles, sensor.n_channels
     keep = []
@@ -373,7 +375,7 @@ def forward_allsky_batch(sensor: Sensor, cloudcoeff: CloudCoeff, ps, emissivity,
     clouds = (OrClouds * M)(*[make_clouds(ps, m, keep) or OrClouds() for m in range(M)])
     rad, bt = np.zeros((M, L)), np.zeros((M, L))
     eps = np.ascontiguousarray(np.broadcast_to(emissivity, (M, L)), np.float64)
-    opt = OrScatterOpts(int(n_streams), int(overlap))
+    opt = OrScatterOpts(int(n_streams), int(overlap), 0 if include_scattering else 1)
     rc = lib().oracle_forward_allsky_batch(ctypes.byref(sensor.s), ctypes.byref(cloudcoeff.c), M, profs, clouds,
                                            _p(eps, _dp), ctypes.byref(opt), _p(rad, _dp), _p(bt, _dp),
                                            n_threads)
@@ -383,7 +385,7 @@ def forward_allsky_batch(sensor: Sensor, cloudcoeff: CloudCoeff, ps, emissivity,
 
 
 def k_matrix_allsky_batch(sensor: Sensor, cloudcoeff: CloudCoeff, ps, emissivity, n_streams=0, overlap=0,
-                          bt_k=None, n_threads=0):
+                          bt_k=None, n_threads=0, include_scattering=True):
     """All-sky K-matrix of every profile by the hand-written adjoint (OpenMP
     over profiles): (bt [M, L], t_k [NT, L, ...] packed per profile as [L, K],
     water-content K packed per profile as [L, nc, K])."""
@@ -399,7 +401,7 @@ def k_matrix_allsky_batch(sensor: Sensor, cloudcoeff: CloudCoeff, ps, emissivity
     wck = np.zeros(max(nc, 1) * NT * L)
     eps = np.ascontiguousarray(np.broadcast_to(emissivity, (M, L)), np.float64)
     btk = np.ascontiguousarray(np.ones((M, L)) if bt_k is None else bt_k, np.float64)
-    opt = OrScatterOpts(int(n_streams), int(overlap))
+    opt = OrScatterOpts(int(n_streams), int(overlap), 0 if include_scattering else 1)
     rc = lib().oracle_k_matrix_allsky_ad_batch(
         ctypes.byref(sensor.s), ctypes.byref(cloudcoeff.c), M, profs, clouds, _p(eps, _dp), ctypes.byref(opt),
         _p(btk, _dp), _p(bt, _dp), _p(tk, _dp), _p(wck, _dp), off.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)),
@@ -410,7 +412,7 @@ def k_matrix_allsky_batch(sensor: Sensor, cloudcoeff: CloudCoeff, ps, emissivity
 
 
 def k_matrix_allsky(sensor: Sensor, cloudcoeff: CloudCoeff, ps, m, emissivity, n_streams=0, overlap=0,
-                    bt_k=None, rad_k=None, method="cstep"):
+                    bt_k=None, rad_k=None, method="cstep", include_scattering=True):
     """CRTM_K_Matrix with clouds for profile m: complex-step differentiation
     (method "cstep") or the hand-written adjoint (method "adjoint")."""
     keep = []
@@ -432,7 +434,7 @@ def k_matrix_allsky(sensor: Sensor, cloudcoeff: CloudCoeff, ps, m, emissivity, n
         setattr(ko, k, _p(v, _dp))
     btk = np.ascontiguousarray(np.ones(L) if bt_k is None else bt_k, np.float64)
     rk = np.ascontiguousarray(np.zeros(L) if rad_k is None else rad_k, np.float64)
-    opt = OrScatterOpts(int(n_streams), int(overlap))
+    opt = OrScatterOpts(int(n_streams), int(overlap), 0 if include_scattering else 1)
     eps = np.ascontiguousarray(emissivity, np.float64)
     fn = lib().oracle_k_matrix_allsky_ad if method == "adjoint" else lib().oracle_k_matrix_allsky
     rc = fn(ctypes.byref(sensor.s), ctypes.byref(cloudcoeff.c), ctypes.byref(p),

@@ -80,16 +80,18 @@ def profiles_from_arrays(level_pressure, temperature, absorber, water_t, zenith,
 # Jacobian variable of one profile, all channels and layers) within
 #     |got - ref| <= rtol * |ref| + atol * max|ref of the block|.
 # FP64: rtol 1e-7, atol 1e-10.  FP32-split: rtol 1e-3, atol 1e-5 for the
-# gas and surface variables, 1e-4 for the scattering-path variables (cloud
+# gas and surface variables, 2e-4 for the scattering-path variables (cloud
 # water content, radius and fraction, aerosol concentration and radius).
 # Where terms cancel, FP32 leaves those an absolute error of up to about
-# 6e-5 of the block maximum: the worst case seen is the overcast cloud-fraction
-# Jacobian, where the cover term (R_cloudy - R_clear) nearly cancels the
-# water-content rescaling term; elements above about 0.1 of the block maximum
-# are within 1e-3 relative.
+# 1.1e-4 of the block maximum.  The worst cases seen are cloud-fraction
+# Jacobians: overcast, where the cover term (R_cloudy - R_clear) nearly
+# cancels the water-content rescaling term (6e-5), and absorption-only
+# clouds (Options%Include_Scattering = .FALSE.) in microwave channels that
+# the cloud barely affects, where R_cloudy - R_clear is ~1e-6 of R (1.1e-4).
+# Elements above about 0.1 of the block maximum are within 1e-3 relative.
 SCATTERING_VARIABLES = ("water content", "radius", "fraction", "aerosol")
 JAC_ELEM = {0: dict(rtol=1e-7, atol=1e-10, atol_scattering=1e-10),
-            1: dict(rtol=1e-3, atol=1e-5, atol_scattering=1e-4)}
+            1: dict(rtol=1e-3, atol=1e-5, atol_scattering=2e-4)}
 
 
 def assert_jacobian_close(got, want, rtol_block, what, precision):

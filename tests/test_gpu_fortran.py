@@ -1,7 +1,8 @@
-"""The Fortran host interface (CRTM_MI355X_Module): a Fortran program calls
-CRTM_Init / CRTM_Forward / CRTM_K_Matrix / CRTM_Destroy the way a CRTM
-application does, through the ISO_C_BINDING layer, and its results match
-the FP64 oracle."""
+"""The Fortran host interface: a Fortran program that USEs CRTM_Module calls
+CRTM_Init / CRTM_Forward / CRTM_K_Matrix / CRTM_Adjoint /
+CRTM_Tangent_Linear / CRTM_Destroy the way a CRTM application does, through
+the ISO_C_BINDING layer, and its results match the FP64 oracle; a call with
+Options%Skip_Profile set on one profile zeroes that profile only."""
 from __future__ import annotations
 
 import os
@@ -86,4 +87,12 @@ def test_fortran_driver_matches_oracle(coeff_dir, oracle_mod, tmp_path, prec):
         assert abs(ad_ts - (w * g["ts_k"]).sum()) <= tol * abs((w * g["ts_k"]).sum())
         want_tl = g["t_k"].sum(1) + (g["abs_k"] * (0.01 * ps.absorber[:, a:b])[None]).sum((1, 2)) + 0.5 * g["ts_k"]
         np.testing.assert_allclose(tl_bt, want_tl, rtol=tol, atol=tol * np.abs(want_tl).max())
+    # Options%Skip_Profile on the last profile (through the Fortran options
+    # mirror): zero RTSolution there, the others bit-identical; channel labels
+    sk = open(fout + ".skip").read().split()
+    np.testing.assert_array_equal(np.array(sk[:L], int), spc.sensor_channel)
+    bt_skip = np.array(sk[L:], float).reshape(3, L)
+    assert np.all(bt_skip[2] == 0.0)
+    for m in range(2):
+        np.testing.assert_array_equal(bt_skip[m], got[m]["bt_fwd"])
 

@@ -728,3 +728,116 @@ def test_empty_aerosol_slots_are_no_aerosols(hirs_aer, oracle_mod):
     for ns in (2, 4):
         np.testing.assert_array_equal(oracle_mod.run_allsky(Sh, Ch, pa, 0, eps, ns)["bt"],
                                       oracle_mod.run_allsky(Sh, Ch, ps, 0, eps, ns)["bt"])
+
+
+# ------------------------------------------------ Include_Scattering = .FALSE.
+def _emission(tau, B, Bs, eps, cosmic, u):
+    """Specular-surface emission along the sensor angle (numpy): downwelling
+    from the top with the cosmic background, reflected, then upwelling."""
+    t = np.exp(-tau / u)
+    rd = cosmic
+    for k in range(tau.size):
+        rd = rd * t[k] + B[k] * (1.0 - t[k])
+    ru = eps * Bs + (1.0 - eps) * rd
+    for k in range(tau.size - 1, -1, -1):
+        ru = ru * t[k] + B[k] * (1.0 - t[k])
+    return ru
+
+
+def test_no_scattering_is_emission_with_absorption_depths(coeff_dir, oracle_mod, hirs_aer):
+    """Include_Scattering = .FALSE. (CRTM_CloudScatter.f90:986-997,
+    CRTM_AerosolScatter.f90:845-848, Common_RTSolution.f90:263): the radiance
+    is the emission solution of the gas optical depth plus, per cloud and
+    aerosol, the amount times the absorption ke (1 - w) of the raw table
+    lookup.  Restated here in numpy from the white-box table lookups and the
+    clear-sky gas depths; a (1 + w), a clamp applied before the scaling or
+    an albedo left in the column fails it."""
+    from conftest import load_sensor
+    S, C = hirs_aer
+    tau_c, spc = load_sensor(coeff_dir, "hirs_syn")
+    ps = synthetic.add_clouds(synthetic.make_profiles(1, 60, seed=231), seed=232, median_gm3=0.005)
+    ps = synthetic.add_aerosols(ps, seed=233)
+    L, K = S.n_channels, 60
+    eps = np.full(L, 0.97)
+    clr = oracle_mod.run_profile(S, ps, 0, eps)
+    assert clr["tau"].shape == (L, K)      # the profile reaches the top: no added layers
+    o = oracle_mod.run_allsky(S, C, ps, 0, eps, n_streams=4, include_scattering=False)
+    scat = oracle_mod.run_allsky(S, C, ps, 0, eps, n_streams=4)
+    assert np.all(o["n_streams"] == 0)     # the emission solution, not ADA
+    u = np.cos(np.radians(ps.sensor_zenith[0]))
+    t = ps.temperature[:K]
+    tsfc = float(np.dot([ps.land_coverage[0], ps.water_coverage[0], ps.snow_coverage[0], ps.ice_coverage[0]],
+                        [ps.land_temperature[0], ps.water_temperature[0], ps.snow_temperature[0],
+                         ps.ice_temperature[0]]))
+    ke, w = ctypes.c_double(), ctypes.c_double()
+    pc = np.zeros(17)
+    absorbed = 0
+    for l in range(L):
+        wn = float(spc.wavenumber[l])
+        tau = clr["tau"][l].copy()
+        for n, typ in enumerate(ps.cloud_type[0]):
+            for k in np.nonzero(ps.cloud_water_content[n, :K] > 1e-6)[0]:
+                assert oracle_mod.lib().oracle_cloud_optics(ctypes.byref(C.c), 0, wn, int(typ),
+                                                            float(ps.cloud_effective_radius[n, k]), float(t[k]), 4,
+                                                            ctypes.byref(ke), ctypes.byref(w), P(pc)) == 0
+                tau[k] += ps.cloud_water_content[n, k] * max(ke.value * (1.0 - w.value), 0.0)
+                absorbed += 1
+        for n, typ in enumerate(ps.aerosol_type[0]):
+            for k in np.nonzero(ps.aerosol_concentration[n, :K] > 1e-15)[0]:
+                assert oracle_mod.lib().oracle_aerosol_optics(C.c.aerosol, wn, int(typ),
+                                                              float(ps.aerosol_effective_radius[n, k]), 4,
+                                                              ctypes.byref(ke), ctypes.byref(w), P(pc)) == 0
+                tau[k] += ps.aerosol_concentration[n, k] * max(ke.value * (1.0 - w.value), 0.0)
+        c1, c2 = spc.planck_c1[l], spc.planck_c2[l]
+        bc1, bc2 = spc.band_c1[l], spc.band_c2[l]
+        B = c1 / (np.exp(c2 / (bc1 + bc2 * t)) - 1.0)
+        Bs = c1 / (np.exp(c2 / (bc1 + bc2 * tsfc)) - 1.0)
+        want = _emission(tau, B, Bs, eps[l], spc.cosmic_background_radiance[l], u)
+        np.testing.assert_allclose(o["radiance"][l], want, rtol=1e-12)
+    assert absorbed > 0
+    assert np.abs(o["bt"] - scat["bt"]).max() > 1e-3      # the switch changes the answer
+
+
+def test_no_scattering_leaves_absorbing_microwave_clouds_unchanged(atms, oracle_mod):
+    """Microwave liquid and ice clouds have no albedo table (absorption only,
+    CRTM_CloudScatter.f90:1363-1367, 1389-1392): the switch changes nothing
+    for them, forward and K-matrix, to the last bit."""
+    S, C = atms
+    ps = synthetic.add_clouds(synthetic.make_profiles(1, 70, seed=241), seed=242,
+                              types=(synthetic.WATER_CLOUD, synthetic.ICE_CLOUD))
+    eps = np.full(22, 0.6)
+    assert ps.cloud_water_content.max() > 0
+    for ns in (0, 4):
+        f1, k1 = oracle_mod.k_matrix_allsky(S, C, ps, 0, eps, ns)
+        f2, k2 = oracle_mod.k_matrix_allsky(S, C, ps, 0, eps, ns, include_scattering=False)
+        np.testing.assert_array_equal(f1["radiance"], f2["radiance"])
+        for key in k1:
+            np.testing.assert_array_equal(k1[key], k2[key])
+
+
+@pytest.mark.parametrize("sensor", ["atms_syn", "hirs_syn"])
+def test_no_scattering_adjoint_matches_complex_step(coeff_dir, oracle_mod, cloudcoeff, aerosolcoeff, sensor):
+    """The hand-written adjoint of the absorption-only optics,
+    d[ke (1 - w)] = (1 - w) dke - ke dw, equals the complex-step Jacobians
+    (clouds of every type; aerosols on the infrared sensor)."""
+    from conftest import load_sensor
+    tau, spc = load_sensor(coeff_dir, sensor)
+    S, C = oracle_mod.Sensor(tau, spc), oracle_mod.CloudCoeff(cloudcoeff[1], aerosolcoeff[1])
+    mw = sensor == "atms_syn"
+    ps = synthetic.add_clouds(synthetic.make_profiles(1, 60, seed=251), seed=252, cover="random",
+                              reff_scale=8.0 if mw else 1.0, median_gm3=0.05 if mw else 0.005)
+    if not mw:
+        ps = synthetic.add_aerosols(ps, seed=253)
+    L = S.n_channels
+    rng = np.random.default_rng(6)
+    eps = rng.uniform(0.5, 0.9, L)
+    btk = rng.normal(size=L)
+    f1, k1 = oracle_mod.k_matrix_allsky(S, C, ps, 0, eps, 4, 0, btk, include_scattering=False)
+    f2, k2 = oracle_mod.k_matrix_allsky(S, C, ps, 0, eps, 4, 0, btk, method="adjoint", include_scattering=False)
+    np.testing.assert_allclose(f2["bt"], f1["bt"], rtol=0, atol=1e-10)
+    for key in k1:
+        scale = max(np.abs(k1[key]).max(), 1e-300)
+        assert np.abs(k1[key] - k2[key]).max() < 1e-11 * scale, key
+    assert np.abs(k1["wc_k"]).max() > 0 and np.abs(k1["reff_k"]).max() > 0
+    if not mw:
+        assert np.abs(k1["areff_k"]).max() > 0
