@@ -633,7 +633,9 @@ def main_batched(args):
     emis = torch.as_tensor(np.concatenate([np.full((M, nL[0]), 0.6), np.full((M, L - nL[0]), 0.98)], axis=1),
                            device=dev)
     prec = api.PREC_FP32_SPLIT
-    kw = dict(precision=prec, n_streams=ns, forward_chunk=256, max_jacobian_bytes=32 << 30)
+    # a progress line per profile chunk when a step takes long (full size)
+    kw = dict(precision=prec, n_streams=ns, forward_chunk=256, max_work_bytes=32 << 30,
+              progress=log if M > 2048 else None)
     # observations: H(x_truth) + N(0, 0.2 K), x_truth = x + 1 K temperature noise
     g = torch.Generator(device="cpu").manual_seed(11)
     with torch.no_grad():

@@ -84,6 +84,19 @@ class CKOutput(ctypes.Structure):
                                    "aerosol_concentration_k", "aerosol_effective_radius_k")]
 
 
+# adjoint outputs laid out per layer ([NT] or [n, NT]); the others are per profile
+_LAYER_AD = ("temperature_ad", "absorber_ad", "cloud_water_content_ad", "cloud_effective_radius_ad",
+             "cloud_fraction_ad", "aerosol_concentration_ad", "aerosol_effective_radius_ad")
+
+
+class CADOutput(ctypes.Structure):
+    _fields_ = [(n, _vp) for n in ("brightness_temperature_ad", "radiance_ad", "surface_emissivity_ad",
+                                   "temperature_ad", "absorber_ad", "land_temperature_ad",
+                                   "water_temperature_ad", "snow_temperature_ad", "ice_temperature_ad",
+                                   "cloud_water_content_ad", "cloud_effective_radius_ad", "cloud_fraction_ad",
+                                   "aerosol_concentration_ad", "aerosol_effective_radius_ad")]
+
+
 _lib = None
 
 
@@ -106,6 +119,7 @@ def lib():
                                    ctypes.POINTER(CRTSolution)]
         L.crtm_k_matrix.argtypes = L.crtm_forward.argtypes + [ctypes.POINTER(CKOutput)]
         L.crtm_k_matrix_noalloc.argtypes = L.crtm_k_matrix.argtypes
+        L.crtm_adjoint.argtypes = L.crtm_forward.argtypes + [ctypes.POINTER(CADOutput)]
         L.crtm_synchronize.argtypes = [_vp]
         L.crtm_set_stream.argtypes = [_vp, _vp, ctypes.c_int]
         L.crtm_set_wavefront_angles.argtypes = [_vp, ctypes.c_int]
@@ -400,85 +414,109 @@ class Context:
             self._err(entry)
         return rts_d, kd
 
-    def adjoint(self, prof, emissivity, bt_ad, precision=PREC_FP64, n_streams=0, overlap=0, jacobian_f32=None,
-                max_jacobian_bytes=8 << 30):
-        """CRTM_Adjoint (src/CRTM_Adjoint_Module.f90:299-325) for a device batch:
-        the channel-summed adjoint of the brightness temperatures seeded with
-        bt_ad [M, L] (RTSolution_AD%Brightness_Temperature).
+    def adjoint(self, prof, emissivity, bt_ad, precision=PREC_FP64, n_streams=0, overlap=0,
+                include_scattering=True, skip_profile=None, max_work_bytes=16 << 30, progress=None):
+        """CRTM_Adjoint (src/CRTM_Adjoint_Module.f90:299-325) through
+        crtm_adjoint: the channel-summed adjoint of the brightness
+        temperatures seeded with bt_ad [M, L] (RTSolution_AD%
+        Brightness_Temperature).  The kernels sum over channels before any
+        per-profile row is written; no per-channel Jacobian is formed.
 
         Returns (bt [M, L], ad) with ad holding temperature_ad [NT],
         absorber_ad [n_absorbers, NT], surface temperature adjoints per type
-        [M], surface_emissivity_ad [M, L] (per channel, as RTSolution_AD%
-        Surface_Emissivity) and, with clouds / aerosols, cloud_water_content_ad,
-        cloud_effective_radius_ad [n_clouds, NT], cloud_fraction_ad [NT],
-        aerosol_concentration_ad, aerosol_effective_radius_ad [n_aerosols, NT].
-
-        Computed as the K-matrix with RTSolution_K = bt_ad summed over
-        channels (the adjoint and the K-matrix share every _AD routine),
-        in profile chunks whose per-channel Jacobians stay within
-        max_jacobian_bytes, so batches far beyond the K-matrix's own memory
-        run in one call.  n_streams may be one value or one per profile."""
-        import torch
-        if not _is_torch(prof.temperature):
-            raise CRTMError("adjoint runs on device batches (api.device_profiles)")
-        if jacobian_f32 is None:
-            jacobian_f32 = precision == PREC_FP32_SPLIT
+        [M], surface_emissivity_ad [M, L] (per channel) and, with clouds /
+        aerosols, cloud_water_content_ad, cloud_effective_radius_ad
+        [n_clouds, NT], cloud_fraction_ad [NT], aerosol_concentration_ad,
+        aerosol_effective_radius_ad [n_aerosols, NT].  Host (numpy) or device
+        (torch) batches.  A batch with clouds or aerosols runs in profile
+        chunks whose scattering work space (per layer and channel: the optical
+        depth, Planck and temperature seeds of the scattering adjoint and the
+        cloudy layers' optics and tapes, estimated at 64 bytes FP32-split and
+        128 FP64) stays within max_work_bytes; a clear-sky batch is one call.  n_streams may
+        be one value or one per profile; progress(message) is called after
+        each chunk."""
+        memory = MEM_DEVICE if _is_torch(prof.temperature) else MEM_HOST
         M, L = prof.n_profiles, self.n_channels
-        nl = np.asarray(prof.n_layers_host, np.int64)
+        nl = np.asarray(getattr(prof, "n_layers_host", None) if memory == MEM_DEVICE else prof.n_layers, np.int64)
         loh = np.concatenate([[0], np.cumsum(nl)])
         NT = int(loh[-1])
         na = int(prof.absorber.shape[0])
         nc = 0 if prof.cloud_type is None else int(prof.cloud_type.shape[1])
         nae = 0 if prof.aerosol_type is None else int(prof.aerosol_type.shape[1])
-        fields = 2 + na + (3 * nc + 1 if nc else 0) + 2 * nae
-        per_layer = L * fields * (4 if jacobian_f32 else 8)
-        dev = prof.temperature.device
-        f64 = dict(dtype=torch.float64, device=dev)
-        ad = {"temperature_ad": torch.zeros(NT, **f64), "absorber_ad": torch.zeros(na, NT, **f64),
-              "surface_emissivity_ad": torch.zeros(M, L, **f64)}
+        like = prof.temperature
+        ad = {"temperature_ad": self._alloc((NT,), np.float64, memory, like),
+              "absorber_ad": self._alloc((na, NT), np.float64, memory, like),
+              "surface_emissivity_ad": self._alloc((M, L), np.float64, memory, like)}
         for f in ("land", "water", "snow", "ice"):
-            ad[f + "_temperature_ad"] = torch.zeros(M, **f64)
+            ad[f + "_temperature_ad"] = self._alloc((M,), np.float64, memory, like)
         if nc:
-            ad["cloud_water_content_ad"] = torch.zeros(nc, NT, **f64)
-            ad["cloud_effective_radius_ad"] = torch.zeros(nc, NT, **f64)
-            ad["cloud_fraction_ad"] = torch.zeros(NT, **f64)
+            ad["cloud_water_content_ad"] = self._alloc((nc, NT), np.float64, memory, like)
+            ad["cloud_effective_radius_ad"] = self._alloc((nc, NT), np.float64, memory, like)
+            ad["cloud_fraction_ad"] = self._alloc((NT,), np.float64, memory, like)
         if nae:
-            ad["aerosol_concentration_ad"] = torch.zeros(nae, NT, **f64)
-            ad["aerosol_effective_radius_ad"] = torch.zeros(nae, NT, **f64)
-        bt = torch.empty(M, L, **f64)
+            ad["aerosol_concentration_ad"] = self._alloc((nae, NT), np.float64, memory, like)
+            ad["aerosol_effective_radius_ad"] = self._alloc((nae, NT), np.float64, memory, like)
+        bt = self._alloc((M, L), np.float64, memory, like)
+        scattering = nc > 0 or nae > 0
+        # scattering work per layer and channel: three seeds and the gas OD in
+        # the call's precision, plus the cloudy layers' optics, R/T matrices
+        # and tapes, taken here as a few percent of the layers at up to ~2 KB
+        per_layer = L * 64 * (1 if precision == PREC_FP32_SPLIT else 2)
         per_profile_streams = not (np.ndim(n_streams) == 0 and not _is_torch(n_streams))
         m0 = 0
         while m0 < M:
-            m1 = m0 + 1
-            while m1 < M and (loh[m1 + 1] - loh[m0]) * per_layer <= max_jacobian_bytes:
-                m1 += 1
+            m1 = M
+            if scattering:
+                m1 = m0 + 1
+                while m1 < M and (loh[m1 + 1] - loh[m0]) * per_layer <= max_work_bytes:
+                    m1 += 1
             a, b = int(loh[m0]), int(loh[m1])
-            sub = profile_slice(prof, m0, m1)
-            ns = n_streams[m0:m1] if per_profile_streams else n_streams
-            rts, kd = self.k_matrix(sub, emissivity[m0:m1].contiguous(), bt_k=bt_ad[m0:m1].contiguous(),
-                                    precision=precision, jacobian_f32=jacobian_f32, check_input=False,
-                                    n_streams=ns, overlap=overlap)
-            _raise_on_failed(rts, m0, "adjoint")
-            bt[m0:m1] = rts["brightness_temperature"]
-            ad["temperature_ad"][a:b] = torch.sum(kd["temperature_k"], dim=-1, dtype=torch.float64)
-            ad["absorber_ad"][:, a:b] = torch.sum(kd["absorber_k"], dim=-1, dtype=torch.float64)
-            ad["surface_emissivity_ad"][m0:m1] = kd["surface_emissivity_k"]
-            for f in ("land", "water", "snow", "ice"):
-                ad[f + "_temperature_ad"][m0:m1] = kd[f + "_temperature_k"].sum(dim=-1)
-            if nc:
-                ad["cloud_water_content_ad"][:, a:b] = torch.sum(kd["cloud_water_content_k"], dim=-1,
-                                                                 dtype=torch.float64)
-                ad["cloud_effective_radius_ad"][:, a:b] = torch.sum(kd["cloud_effective_radius_k"], dim=-1,
-                                                                    dtype=torch.float64)
-                ad["cloud_fraction_ad"][a:b] = torch.sum(kd["cloud_fraction_k"], dim=-1, dtype=torch.float64)
-            if nae:
-                ad["aerosol_concentration_ad"][:, a:b] = torch.sum(kd["aerosol_concentration_k"], dim=-1,
-                                                                   dtype=torch.float64)
-                ad["aerosol_effective_radius_ad"][:, a:b] = torch.sum(kd["aerosol_effective_radius_k"], dim=-1,
-                                                                      dtype=torch.float64)
-            del rts, kd
+            whole = m0 == 0 and m1 == M
+            sub = prof if whole else profile_slice(prof, m0, m1)
+            ns = n_streams[m0:m1] if (per_profile_streams and not whole) else n_streams
+            sk = skip_profile if (skip_profile is None or whole) else skip_profile[m0:m1]
+            part = {k: (v[..., a:b] if k in _LAYER_AD else v[m0:m1]) for k, v in ad.items()}
+            self._adjoint_call(sub, emissivity if whole else emissivity[m0:m1], bt_ad if whole else bt_ad[m0:m1],
+                               precision, ns, overlap, include_scattering, sk, bt[m0:m1], part, m0)
+            if progress:
+                progress(f"adjoint profiles {m0}-{m1} of {M} done")
             m0 = m1
         return bt, ad
+
+    def _adjoint_call(self, prof, emissivity, bt_ad, precision, n_streams, overlap, include_scattering,
+                      skip_profile, bt_out, ad, m0):
+        memory = MEM_DEVICE if _is_torch(prof.temperature) else MEM_HOST
+        if memory == MEM_DEVICE:
+            self._use_torch_stream()
+        keep = []
+        if memory == MEM_DEVICE:
+            emissivity, bt_ad = emissivity.contiguous(), bt_ad.contiguous()
+        else:
+            emissivity, bt_ad = np.ascontiguousarray(emissivity, np.float64), np.ascontiguousarray(bt_ad, np.float64)
+        atm, sfc, geo, opt = self._structs(prof, emissivity, precision, memory, False, False, keep, n_streams,
+                                           overlap, include_scattering, skip_profile)
+        M, L = prof.n_profiles, self.n_channels
+        rts_d = {"radiance": self._alloc((M, L), np.float64, memory, prof.temperature),
+                 "brightness_temperature": bt_out, "status": self._alloc((M,), np.int32, memory, prof.temperature)}
+        # a chunk's layer slice of an [n, NT] array is strided: the library
+        # accumulates into a contiguous copy, copied back below
+        cont = {}
+        for k, v in ad.items():
+            if _is_torch(v):
+                cont[k] = v if v.is_contiguous() else v.contiguous()
+            else:
+                cont[k] = v if v.flags.c_contiguous else np.ascontiguousarray(v)
+        rts = CRTSolution(**{k: _ptr(v) for k, v in rts_d.items()})
+        out = CADOutput(brightness_temperature_ad=_ptr(bt_ad), **{k: _ptr(v) for k, v in cont.items()})
+        keep.append(bt_ad)
+        st = lib().crtm_adjoint(self._h, atm, sfc, geo, opt, rts, out)
+        if st != SUCCESS:
+            self._err("crtm_adjoint")
+        for k, v in ad.items():
+            if cont[k] is not v:
+                v[...] = cont[k]
+        if memory == MEM_DEVICE:
+            _raise_on_failed(rts_d, m0, "adjoint")
 
     def tangent_linear(self, prof, emissivity, tl, precision=PREC_FP64, n_streams=0, overlap=0,
                        jacobian_f32=None, max_jacobian_bytes=8 << 30):

@@ -3,10 +3,9 @@ differentiable function of the atmospheric state, clouds and aerosols.
 
 Forward is CRTM_Forward (src/CRTM_Forward_Module.f90:229-245); backward is
 CRTM_Adjoint (src/CRTM_Adjoint_Module.f90:299-325) seeded with the incoming
-gradient as RTSolution_AD%Brightness_Temperature: the channel-summed adjoint,
-computed by Context.adjoint from the K-matrix kernels in profile chunks, so
-the gradient of a large batch does not hold a full K-matrix.  Both run on
-the GPU in device memory on torch's current stream.
+gradient as RTSolution_AD%Brightness_Temperature: the channel-summed adjoint
+of crtm_adjoint (Context.adjoint), which never holds a per-channel Jacobian.
+Both run on the GPU in device memory on torch's current stream.
 
     bt = crtm_bt(ctx, profiles, temperature, absorber, water_temperature,
                  emissivity, precision=api.PREC_FP32_SPLIT,
@@ -39,7 +38,7 @@ _ADJOINT = ("temperature_ad", "absorber_ad", "water_temperature_ad", "cloud_wate
 class CRTMBrightnessTemperature(torch.autograd.Function):
     @staticmethod
     def forward(ctx, crtm_ctx, profiles, options, emissivity, *state):
-        precision, n_streams, overlap, max_bytes, chunk = options
+        precision, n_streams, overlap, max_bytes, chunk, progress = options
         prof = _with_state(profiles, state)
         e = emissivity.detach().to(torch.float64).contiguous()
         M = prof.n_profiles
@@ -55,6 +54,8 @@ class CRTMBrightnessTemperature(torch.autograd.Function):
                 bt[m0:m1] = crtm_ctx.forward(api.profile_slice(prof, m0, m1), e[m0:m1].contiguous(),
                                              precision=precision, n_streams=ns,
                                              overlap=overlap)["brightness_temperature"]
+                if progress:
+                    progress(f"forward profiles {m0}-{m1} of {M} issued")
         ctx.save_for_backward(emissivity, *[s for s in state if s is not None])
         ctx.present = tuple(s is not None for s in state)
         ctx.crtm = (crtm_ctx, profiles, options)
@@ -65,12 +66,12 @@ class CRTMBrightnessTemperature(torch.autograd.Function):
         saved = list(ctx.saved_tensors)
         emissivity = saved.pop(0)
         state = [saved.pop(0) if p else None for p in ctx.present]
-        crtm_ctx, profiles, (precision, n_streams, overlap, max_bytes, chunk) = ctx.crtm
+        crtm_ctx, profiles, (precision, n_streams, overlap, max_bytes, chunk, progress) = ctx.crtm
         prof = _with_state(profiles, state)
         seed = grad_bt.detach().to(torch.float64).contiguous()
         _, ad = crtm_ctx.adjoint(prof, emissivity.detach().to(torch.float64).contiguous(), seed,
                                  precision=precision, n_streams=n_streams, overlap=overlap,
-                                 max_jacobian_bytes=max_bytes)
+                                 max_work_bytes=max_bytes, progress=progress)
         grads = [None if s is None else ad[k].to(s.dtype) for s, k in zip(state, _ADJOINT)]
         return (None, None, None, ad["surface_emissivity_ad"].to(emissivity.dtype), *grads)
 
@@ -85,14 +86,15 @@ def _with_state(profiles, state):
 def crtm_bt(crtm_ctx, profiles, temperature, absorber, water_temperature, emissivity,
             precision=api.PREC_FP32_SPLIT, cloud_water_content=None, cloud_effective_radius=None,
             aerosol_concentration=None, aerosol_effective_radius=None, n_streams=0, overlap=0,
-            max_jacobian_bytes=8 << 30, forward_chunk=0):
+            max_work_bytes=16 << 30, forward_chunk=0, progress=None):
     """Differentiable CRTM brightness temperatures [M, L].  forward_chunk > 0
-    runs the forward model in chunks of that many profiles (the backward is
-    chunked by max_jacobian_bytes)."""
+    runs the forward model in chunks of that many profiles; the backward is
+    crtm_adjoint, whose scattering work space per call max_work_bytes
+    bounds.  progress(message) is called after each chunk is issued."""
     if not temperature.is_cuda:
         raise api.CRTMError("crtm_bt runs on the GPU: pass device tensors (api.device_profiles)")
     state = (temperature, absorber, water_temperature, cloud_water_content, cloud_effective_radius,
              aerosol_concentration, aerosol_effective_radius)
     return CRTMBrightnessTemperature.apply(crtm_ctx, profiles,
-                                           (precision, n_streams, overlap, max_jacobian_bytes, forward_chunk),
+                                           (precision, n_streams, overlap, max_work_bytes, forward_chunk, progress),
                                            emissivity, *state)

@@ -1443,9 +1443,24 @@ __global__ void __launch_bounds__(256) cloud_optics_ad_kernel(DevSensor s, DevBa
         r_ad += w_ad * lu.w_r;
         t_ad += w_ad * lu.w_t;
       }
-      const int64_t oi = ((int64_t)n * be.out_NT + row) * be.Ltot + lg;
-      wck[oi] = (KT)wc_ad;
-      rek[oi] = (KT)r_ad;
+      if (ko.reduce) {
+        // adjoint: channel sums [nc][out_NT]; the cover rescaling of the
+        // water content (wc / tcc) is applied here, and its share of the
+        // cover adjoint goes to the (profile, channel) seed that
+        // cloud_cover_ad_kernel reads after this kernel
+        const int64_t ri = (int64_t)n * be.out_NT + row;
+        double wa = wc_ad;
+        if (cw.frac[m] && cw.tcc[m] > MIN_COVERAGE_THRESHOLD) {
+          unsafeAtomicAdd(o.ks.tcc_ad + (int64_t)m * s.L + l, -wc_ad * wc / cw.tcc[m]);
+          wa = wc_ad / cw.tcc[m];
+        }
+        unsafeAtomicAdd(reinterpret_cast<double*>(ko.wc_k) + ri, wa);
+        unsafeAtomicAdd(reinterpret_cast<double*>(ko.reff_k) + ri, r_ad);
+      } else {
+        const int64_t oi = ((int64_t)n * be.out_NT + row) * be.Ltot + lg;
+        wck[oi] = (KT)wc_ad;
+        rek[oi] = (KT)r_ad;
+      }
       t_sum += t_ad;
     }
     // CRTM_Compute_AerosolScatter_AD (src/AtmScatter/CRTM_AerosolScatter.f90:614-751)
@@ -1480,9 +1495,15 @@ __global__ void __launch_bounds__(256) cloud_optics_ad_kernel(DevSensor s, DevBa
         double r_ad = bs * sr;
         if (lu.ke_on) r_ad += ke_ad * lu.ke_r;
         if (lu.w_on) r_ad += w_ad * lu.w_r;
-        const int64_t oi = ((int64_t)n * be.out_NT + row) * be.Ltot + lg;
-        ack[oi] = (KT)c_ad;
-        ark[oi] = (KT)r_ad;
+        if (ko.reduce) {
+          const int64_t ri = (int64_t)n * be.out_NT + row;
+          unsafeAtomicAdd(reinterpret_cast<double*>(ko.aconc_k) + ri, c_ad);
+          unsafeAtomicAdd(reinterpret_cast<double*>(ko.areff_k) + ri, r_ad);
+        } else {
+          const int64_t oi = ((int64_t)n * be.out_NT + row) * be.Ltot + lg;
+          ack[oi] = (KT)c_ad;
+          ark[oi] = (KT)r_ad;
+        }
       }
     }
     reinterpret_cast<RT*>(o.ks.t)[ik * s.L + l] = (RT)t_sum;
@@ -1518,9 +1539,9 @@ __global__ void __launch_bounds__(256) cloud_cover_ad_kernel(DevSensor s, DevBat
       if (ty < WATER_CLOUD || ty > HAIL_CLOUD || !cl.cf || cl.cf[lu + ku] < MIN_COVERAGE_THRESHOLD) return 0.0;
       return cl.wc[nn * NTu + lu + ku];
     };
-    // wc_scaled = wc / tcc
+    // wc_scaled = wc / tcc (adjoint calls: applied by cloud_optics_ad_kernel)
     double tot_ad = o.ks.tcc_ad[(int64_t)m * s.L + l];
-    for (int nn = 0; nn < nc; ++nn)
+    for (int nn = 0; nn < nc && !ko.reduce; ++nn)
       for (int ku = 0; ku < K - nx; ++ku) {
         const double wcs = cw.wc[nn * NTe + le + ku + nx];
         if (!(wcs > WATER_CONTENT_THRESHOLD)) continue;
@@ -1534,6 +1555,10 @@ __global__ void __launch_bounds__(256) cloud_cover_ad_kernel(DevSensor s, DevBat
     auto add_cf = [&](int kk, double v) {
       const int ku = kk - nx;
       if (ku < 0) return;
+      if (ko.reduce) {
+        unsafeAtomicAdd(reinterpret_cast<double*>(cfk) + lu + ku, v);
+        return;
+      }
       KT* p = cfk + (lu + ku) * Lt + lg;
       *p = (KT)((double)*p + v);
     };
@@ -1605,8 +1630,13 @@ __global__ void __launch_bounds__(256) cloud_cover_ad_kernel(DevSensor s, DevBat
               const double w = wcu(nn, ku);
               if (w > WATER_CONTENT_THRESHOLD) {
                 lwc += w;
-                const int64_t oi = ((int64_t)nn * NTu + lu + ku) * Lt + lg;
-                wck[oi] = (KT)((double)wck[oi] + tot_ad * (cf_at(ku + nx) - tcc) / ws);
+                const double add = tot_ad * (cf_at(ku + nx) - tcc) / ws;
+                if (ko.reduce) {
+                  unsafeAtomicAdd(reinterpret_cast<double*>(wck) + (int64_t)nn * NTu + lu + ku, add);
+                } else {
+                  const int64_t oi = ((int64_t)nn * NTu + lu + ku) * Lt + lg;
+                  wck[oi] = (KT)((double)wck[oi] + add);
+                }
               }
             }
             add_cf(ku + nx, tot_ad * lwc / ws);

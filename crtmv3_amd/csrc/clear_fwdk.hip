@@ -9,7 +9,7 @@
 //   forward:  ODPS regression + OPTRAN WLO  -> clamped level-to-space path
 //             (ODPS_AtmAbsorption.f90:133-256, :664-727)
 //             -> path interpolated to user levels, nadir layer OD
-//             -> Planck (FP64), Emission RT (Emission_Module.f90:54-174)
+//             -> Planck, Emission RT (Emission_Module.f90:54-174)
 //             -> BT (FP64, CRTM_Planck_Functions.f90:470-492)
 //   adjoint:  Pre_Process_RTSolution_K (CRTM_K_Matrix_Module.f90:1725-1761),
 //             CRTM_Emission_AD (Emission_Module.f90:280-395), Planck AD,
@@ -18,9 +18,9 @@
 //             (ODPS_Predictor.f90:1783-2643), ODPS_Compute_Predictor_ODAS_AD
 //             (:3092-3231), Map_Input_AD (ODPS_CoordinateMapping.f90:480-547)
 // all in one pass over the profile's layers, with nothing per-layer stored
-// in HBM.  RT is the arithmetic type of absorption and RT (float for the
-// split path, double for the plain FP64 path); Planck and the BT inversion
-// are always FP64.
+// in HBM.  RT is the arithmetic type of absorption, Planck and RT (float for
+// the split path, double for the plain FP64 path); the radiance is
+// accumulated into FP64 and the BT inversion is always FP64.
 #include "crtm_device.h"
 #include "pred_adjoint.h"
 
@@ -120,10 +120,16 @@ __device__ inline RT optran_lnchi(const OC& oc, const int* meta, const RT* ok) {
   return lnchi;
 }
 
-template <int G, class RT, class KT, bool DOK>
+// RED (adjoint): the Jacobian outputs are channel sums, CRTM_Adjoint's
+// Atmosphere_AD / Surface_AD seeded by RTSolution_AD (src/CRTM_Adjoint_Module.f90:
+// 1378-1396): each wave sums its 64 channels and adds the sum to the profile's
+// row (FP64 atomics across channel blocks), so no per-channel Jacobian is
+// stored; KT is double.
+template <int G, class RT, class KT, bool DOK, bool RED>
 __global__ void __attribute__((amdgpu_flat_work_group_size(KCfg<RT>::PW * WAVE, KCfg<RT>::PW * WAVE),
                                amdgpu_waves_per_eu(KCfg<RT>::WPE > 0 ? KCfg<RT>::WPE : 1)))
 clear_fwdk_kernel(DevSensor s, DevBatch bt, ProfWork<RT> w, DevOut o) {
+  static_assert(!RED || (DOK && sizeof(KT) == 8), "the adjoint accumulates FP64 channel sums");
   using Tg = GT<G>;
   constexpr int PW = KCfg<RT>::PW, TPB = PW * WAVE;
   // Workgroup = one 64-channel block x PW profiles (one wave each).  The
@@ -168,7 +174,14 @@ clear_fwdk_kernel(DevSensor s, DevBatch bt, ProfWork<RT> w, DevOut o) {
       if (o.sfc_planck) o.sfc_planck[ol] = 0.0;
       if (o.sfc_emis) o.sfc_emis[ol] = 0.0;
     }
-    if constexpr (DOK) {
+    if constexpr (RED) {   // a skipped profile adds nothing; a failed one's rows become NaN
+      if (!skipped && lane == 0 && b == 0)
+        for (int64_t r = bt.out_loff[m]; r < bt.out_loff[m + 1]; ++r) {
+          reinterpret_cast<double*>(o.t_k)[r] = fill;
+          for (int ja = 0; ja < bt.na; ++ja) reinterpret_cast<double*>(o.a_k)[ja * bt.out_NT + r] = fill;
+        }
+      if (skipped && o.emis_k) o.emis_k[ol] = 0.0;
+    } else if constexpr (DOK) {
       const int64_t r0 = bt.out_loff[m], r1 = bt.out_loff[m + 1];
       const int64_t astr_f = bt.out_NT * bt.Ltot;
       for (int64_t r = r0; r < r1; ++r) {
@@ -362,7 +375,20 @@ clear_fwdk_kernel(DevSensor s, DevBatch bt, ProfWork<RT> w, DevOut o) {
   if (valid && o.rad_k && !seeded) o.rad_k[ol] = rad_ad_d;
   const RT rad_ad = (RT)rad_ad_d;
   const RT upad_sfc = rad_ad * tc;           // adjoint of the surface-leaving radiance
-  if (valid) {
+  if constexpr (RED) {   // surface temperature adjoints [M]: channel sums
+    if (ok_prof) {
+      const double bs_seed = (seeded && valid) ? o.ks.bs[sl] : 0.0;
+      const RT dbsdt = c2 * bc2 * Bs * (Bs + c1) * ic1 / (tes * tes);
+      const double sad = wave_sum(valid ? (double)(dbsdt * (eps * upad_sfc + (RT)bs_seed)) : 0.0);
+      if (lane == 0)
+        for (int c = 0; c < 4; ++c)
+          if (o.sfc_t_k[c]) unsafeAtomicAdd(o.sfc_t_k[c] + m, bt.cov[c][m] * sad);
+    }
+    if (valid && o.emis_k) {
+      const double e_seed = seeded ? o.ks.emis[sl] : 0.0;
+      o.emis_k[ol] = (double)(upad_sfc * Bs - upad_sfc * rd) + e_seed;
+    }
+  } else if (valid) {
     const double e_seed = seeded ? o.ks.emis[sl] : 0.0, bs_seed = seeded ? o.ks.bs[sl] : 0.0;
     if (o.emis_k) o.emis_k[ol] = (double)(upad_sfc * Bs - upad_sfc * rd) + e_seed;
     const RT dbsdt = c2 * bc2 * Bs * (Bs + c1) * ic1 / (tes * tes);
@@ -394,6 +420,14 @@ clear_fwdk_kernel(DevSensor s, DevBatch bt, ProfWork<RT> w, DevOut o) {
   KT* tko = reinterpret_cast<KT*>(o.t_k) + (bt.out_loff[m] - nx) * Lt + lg;
   KT* ako = reinterpret_cast<KT*>(o.a_k) + (bt.out_loff[m] - nx) * Lt + lg;
   const int64_t astr = bt.out_NT * Lt;
+  // adjoint rows [out_NT] (RED): row of user layer k at tad[k]
+  double* tad = reinterpret_cast<double*>(o.t_k) + (bt.out_loff[m] - nx);
+  double* aad = reinterpret_cast<double*>(o.a_k) + (bt.out_loff[m] - nx);
+  // wave sum of a lane value (channels past L give 0), added to row p by lane 0
+  auto red_add = [&](double* p, RT v) {
+    const double sv = wave_sum(lane_in ? (double)v : 0.0);
+    if (lane == 0) unsafeAtomicAdd(p, sv);
+  };
   // Map_Input_AD in the sweep: user layer i is complete once the lowest ODPS
   // layer whose support holds it has been processed; with at most MW such
   // layers per user layer, their adjoints sit in a register window.  Wider
@@ -432,7 +466,9 @@ clear_fwdk_kernel(DevSensor s, DevBatch bt, ProfWork<RT> w, DevOut o) {
       dn *= t;
       ru = ru * t + B * (RT(1) - t);
       const RT dbdt = c2 * bc2 * B * (B + c1) * ic1 / (te * te);
-      if (valid && k >= nx) {
+      if constexpr (RED) {
+        if (k >= nx) red_add(tad + k, dbdt * b_ad + t_seed);   // wave-uniform: every lane takes part
+      } else if (valid && k >= nx) {
         tko[(int64_t)k * Lt] = (KT)(dbdt * b_ad + t_seed);
         if (!winmap)
           for (int ja = 0; ja < na; ++ja) ako[ja * astr + (int64_t)k * Lt] = KT(0);
@@ -488,10 +524,29 @@ clear_fwdk_kernel(DevSensor s, DevBatch bt, ProfWork<RT> w, DevOut o) {
 #pragma unroll
     for (int q = 0; q < PF; ++q) {
       const int iu = pf_lo_n + q;
-      pfn[q] = (valid && iu >= nx && iu < K) ? (RT)tko[(int64_t)iu * Lt] : RT(0);
+      pfn[q] = (!RED && valid && iu >= nx && iu < K) ? (RT)tko[(int64_t)iu * Lt] : RT(0);
     }
   };
   auto finish_user_layer = [&](int iu) {   // completed at the ODPS layer in window slot 0
+    if constexpr (RED) {   // the Planck part was summed in the Emission_AD pass
+      if (iu < nx) return;                         // wave-uniform
+      const RT* wr = uwin + (size_t)iu * MW;
+      RT st = RT(0), sa[NAW];
+#pragma unroll
+      for (int ju = 0; ju < NAW; ++ju) sa[ju] = RT(0);
+#pragma unroll
+      for (int q = 0; q < MW; ++q) {
+        const RT wk = wr[q];
+        st += wk * tw[q];
+#pragma unroll
+        for (int ju = 0; ju < NAW; ++ju) sa[ju] += wk * aw[q][ju];
+      }
+      red_add(tad + iu, st);
+#pragma unroll
+      for (int ju = 0; ju < NAW; ++ju)
+        if (ju < na) red_add(aad + ju * bt.out_NT + iu, sa[ju]);
+      return;
+    }
     if (!valid || iu < nx) return;
     RT st;                                         // Planck part from the Emission_AD pass
     if ((unsigned)(iu - pf_lo) < (unsigned)PF) {
@@ -640,6 +695,19 @@ clear_fwdk_kernel(DevSensor s, DevBatch bt, ProfWork<RT> w, DevOut o) {
         // user layers (i2(k-1), i2(k)] are complete
         const int ihi = frange[2 * k + 1];
         for (int iu = pf_lo; iu <= ihi; ++iu) finish_user_layer(iu);
+      } else if (RED) {
+        // wide supports or more absorbers, adjoint: channel sums per row
+        const int i1 = acc_i[3 * k], n = acc_i[3 * k + 1];
+        const RT* awk = acc_w + acc_i[3 * k + 2];
+        for (int q = 0; q < n; ++q)
+          if (i1 + q >= nx) red_add(tad + i1 + q, awk[q] * t_ad);
+#pragma unroll
+        for (int j = 0; j < Tg::NABS; ++j) {
+          const int ju = bt.jmap[j];
+          if (ju >= 0 && uk[RR::MASK0 + j] != RT(0))
+            for (int q = 0; q < n; ++q)
+              if (i1 + q >= nx) red_add(aad + ju * bt.out_NT + i1 + q, awk[q] * a_ad[j]);
+        }
       } else if (valid) {
         // wide supports or more absorbers: add onto the output rows.  Each
         // lane owns its channel's column, so the adds need no atomicity; the
@@ -663,22 +731,24 @@ clear_fwdk_kernel(DevSensor s, DevBatch bt, ProfWork<RT> w, DevOut o) {
   }
 }
 
-template <int G, class RT, class KT, bool DOK>
+template <int G, class RT, class KT, bool DOK, bool RED>
 void launch_fwdk(const DevSensor& s, const DevBatch& b, const ProfWork<RT>& w, const DevOut& o,
                  hipStream_t stream) {
   constexpr int PW = KCfg<RT>::PW;
   const int64_t ng = (b.M + PW - 1) / PW;
   const int64_t n = 8 * ((s.n_blocks * ng + 7) / 8);
-  hipLaunchKernelGGL((clear_fwdk_kernel<G, RT, KT, DOK>), dim3((unsigned)n), dim3(PW * WAVE), 0, stream, s, b, w,
-                     o);
+  hipLaunchKernelGGL((clear_fwdk_kernel<G, RT, KT, DOK, RED>), dim3((unsigned)n), dim3(PW * WAVE), 0, stream, s, b,
+                     w, o);
 }
 
-#define INST(G, RT, KT, DOK) \
-  template void launch_fwdk<G, RT, KT, DOK>(const DevSensor&, const DevBatch&, const ProfWork<RT>&, \
-                                            const DevOut&, hipStream_t);
-#define INST_G(G)                                                                        \
-  INST(G, float, float, true) INST(G, float, double, true) INST(G, float, float, false)  \
-  INST(G, double, double, true) INST(G, double, float, true) INST(G, double, double, false)
+#define INST(G, RT, KT, DOK, RED) \
+  template void launch_fwdk<G, RT, KT, DOK, RED>(const DevSensor&, const DevBatch&, const ProfWork<RT>&, \
+                                                 const DevOut&, hipStream_t);
+#define INST_G(G)                                                                                     \
+  INST(G, float, float, true, false) INST(G, float, double, true, false)                              \
+  INST(G, float, float, false, false) INST(G, double, double, true, false)                            \
+  INST(G, double, float, true, false) INST(G, double, double, false, false)                           \
+  INST(G, float, double, true, true) INST(G, double, double, true, true)
 INST_G(1)
 INST_G(2)
 INST_G(3)

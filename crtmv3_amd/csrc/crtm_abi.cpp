@@ -26,7 +26,7 @@ namespace crtm {
 void launch_add_layers(const DevBatch&, const ExtBatch&, const double* clim, const int* ids, hipStream_t);
 template <int G, class RT>
 void launch_prep(const DevSensor&, const DevBatch&, const ProfWork<RT>&, hipStream_t);
-template <int G, class RT, class KT, bool DOK>
+template <int G, class RT, class KT, bool DOK, bool RED>
 void launch_fwdk(const DevSensor&, const DevBatch&, const ProfWork<RT>&, const DevOut&, hipStream_t);
 void launch_cloud_prep(const DevBatch&, const DevClouds&, const CloudWork&, hipStream_t);
 template <class RT>
@@ -852,7 +852,7 @@ DevBatch sensor_batch(const Sensor& S, const DevBatch& b) {
 }
 
 // the fused clear-sky forward / K-matrix kernel alone (preparation already run)
-template <class RT, class KT, bool DOK>
+template <class RT, class KT, bool DOK, bool RED = false>
 void run_fwdk(crtm_context* c, Sensor& S, const DevBatch& b, const ProfWork<RT>& w, const DevOut& o) {
   const DevBatch bb = sensor_batch(S, b);
   if (c->profiling) {
@@ -860,15 +860,15 @@ void run_fwdk(crtm_context* c, Sensor& S, const DevBatch& b, const ProfWork<RT>&
     c->ev_kind.push_back(1);
   }
   switch (S.d.group) {
-    case 1: launch_fwdk<1, RT, KT, DOK>(S.d, bb, w, o, c->stream); break;
-    case 2: launch_fwdk<2, RT, KT, DOK>(S.d, bb, w, o, c->stream); break;
-    case 3: launch_fwdk<3, RT, KT, DOK>(S.d, bb, w, o, c->stream); break;
+    case 1: launch_fwdk<1, RT, KT, DOK, RED>(S.d, bb, w, o, c->stream); break;
+    case 2: launch_fwdk<2, RT, KT, DOK, RED>(S.d, bb, w, o, c->stream); break;
+    case 3: launch_fwdk<3, RT, KT, DOK, RED>(S.d, bb, w, o, c->stream); break;
   }
   if (c->profiling) c->mark();
   HIPCHECK(hipGetLastError());
 }
 
-template <class RT, class KT, bool DOK>
+template <class RT, class KT, bool DOK, bool RED = false>
 ProfWork<RT> run_sensor(crtm_context* c, Sensor& S, const DevBatch& b, const DevOut& o) {
   const int nx = S.d.group == 1 ? 89 : S.d.group == 2 ? 50 : 21;
   ProfWork<RT> w = make_work<RT>(c, b.M, S.d.Ko, nx, b.max_k);
@@ -891,15 +891,15 @@ ProfWork<RT> run_sensor(crtm_context* c, Sensor& S, const DevBatch& b, const Dev
   switch (S.d.group) {
     case 1:
       timed(0, [&] { launch_prep<1, RT>(S.d, bb, w, c->stream); });
-      timed(1, [&] { launch_fwdk<1, RT, KT, DOK>(S.d, bb, w, o, c->stream); });
+      timed(1, [&] { launch_fwdk<1, RT, KT, DOK, RED>(S.d, bb, w, o, c->stream); });
       break;
     case 2:
       timed(0, [&] { launch_prep<2, RT>(S.d, bb, w, c->stream); });
-      timed(1, [&] { launch_fwdk<2, RT, KT, DOK>(S.d, bb, w, o, c->stream); });
+      timed(1, [&] { launch_fwdk<2, RT, KT, DOK, RED>(S.d, bb, w, o, c->stream); });
       break;
     case 3:
       timed(0, [&] { launch_prep<3, RT>(S.d, bb, w, c->stream); });
-      timed(1, [&] { launch_fwdk<3, RT, KT, DOK>(S.d, bb, w, o, c->stream); });
+      timed(1, [&] { launch_fwdk<3, RT, KT, DOK, RED>(S.d, bb, w, o, c->stream); });
       break;
   }
   HIPCHECK(hipGetLastError());
@@ -1096,7 +1096,7 @@ struct CloudCall {
 // optical depths), the scattering forward + adjoint producing the cloud
 // Jacobians and the seeds, then the clear-sky K-matrix kernel completing the
 // gas, Planck and surface adjoints with those seeds.
-template <class RT, class KT>
+template <class RT, class KT, bool RED>
 void run_sensor_k_allsky(crtm_context* c, Sensor& S, const DevBatch& be, const DevOut& o, const CloudCall& cc) {
   const int L = S.d.L;
   DevOut o1 = o;
@@ -1147,10 +1147,10 @@ void run_sensor_k_allsky(crtm_context* c, Sensor& S, const DevBatch& be, const D
   launch_clear_copy(S.d, be.M, be.Ltot, cc.cw.cl_count, o1, cc.ao, c->stream);   // profiles without cloudy layers
   if (c->profiling) c->mark();
   HIPCHECK(hipGetLastError());
-  run_fwdk<RT, KT, true>(c, S, be, w, o2);
+  run_fwdk<RT, KT, true, RED>(c, S, be, w, o2);
 }
 
-template <class RT, class KT, bool DOK>
+template <class RT, class KT, bool DOK, bool RED = false>
 void run_sensors(crtm_context* c, const DevBatch& be, DevOut o, const CloudCall& cc) {
   c->call_status = static_cast<int*>(take(c, c->cstatus, (size_t)be.M * sizeof(int)));
   launch_zero(c->call_status, (size_t)be.M * sizeof(int), c->stream);
@@ -1165,7 +1165,7 @@ void run_sensors(crtm_context* c, const DevBatch& be, DevOut o, const CloudCall&
     // order), so the next sensor reuses it
     const size_t mark = c->arena_used;
     if (DOK && cc.on) {
-      run_sensor_k_allsky<RT, KT>(c, *S, be, o, cc);
+      run_sensor_k_allsky<RT, KT, RED>(c, *S, be, o, cc);
       merge();
       c->arena_used = mark;
       continue;
@@ -1175,7 +1175,7 @@ void run_sensors(crtm_context* c, const DevBatch& be, DevOut o, const CloudCall&
       o.tau_g = c->ctau.p;
       o.tau_mask = cc.cw.cl_count;
     }
-    ProfWork<RT> w = run_sensor<RT, KT, DOK>(c, *S, be, o);
+    ProfWork<RT> w = run_sensor<RT, KT, DOK, RED>(c, *S, be, o);
     if (cc.on) run_allsky<RT>(c, *S, be, cc.cl, cc.cw, cc.total, w, o, cc.ao);
     else launch_clear_copy(S->d, be.M, be.Ltot, nullptr, o, cc.ao, c->stream);   // R_Clear = Radiance without clouds
     merge();
@@ -1184,8 +1184,11 @@ void run_sensors(crtm_context* c, const DevBatch& be, DevOut o, const CloudCall&
 }
 
 void dispatch(crtm_context* c, const DevBatch& be, const DevOut& o, const CloudCall& cc, bool split, bool kf32,
-              bool dok) {
-  if (dok) {
+              bool dok, bool adj = false) {
+  if (adj) {   // CRTM_Adjoint: FP64 channel sums
+    if (split) run_sensors<float, double, true, true>(c, be, o, cc);
+    else run_sensors<double, double, true, true>(c, be, o, cc);
+  } else if (dok) {
     if (split) { if (kf32) run_sensors<float, float, true>(c, be, o, cc); else run_sensors<float, double, true>(c, be, o, cc); }
     else { if (kf32) run_sensors<double, float, true>(c, be, o, cc); else run_sensors<double, double, true>(c, be, o, cc); }
   } else {
@@ -1230,8 +1233,11 @@ size_t noalloc_bytes(const crtm_context* c, int M, int max_k, int64_t NT, int na
   return worst;
 }
 
+// k: K-matrix outputs, or with adj the adjoint outputs in the same fields
+// (FP64 channel sums: [NT] / [n][NT] rows, [M] surface temperatures,
+// accumulated into)
 int do_call(crtm_context* ctx, const crtm_atmosphere* atm, const crtm_surface* sfc, const crtm_geometry* geo,
-            const crtm_options* opt, crtm_rtsolution* rts, crtm_k_output* k) {
+            const crtm_options* opt, crtm_rtsolution* rts, crtm_k_output* k, bool adj = false) {
   if (!ctx) return fail(nullptr, "null context");
   if (!atm || !sfc || !geo || !opt || !rts) return fail(ctx, "null argument");
   if (!opt->use_emissivity || !opt->emissivity)
@@ -1309,7 +1315,7 @@ int do_call(crtm_context* ctx, const crtm_atmosphere* atm, const crtm_surface* s
   }
   const bool host = opt->memory == CRTM_MEM_HOST;
   const bool split = opt->precision == CRTM_PREC_FP32_SPLIT;
-  const bool kf32 = opt->jacobian_f32 != 0;
+  const bool kf32 = !adj && opt->jacobian_f32 != 0;
   const int L = ctx->n_channels;
   HIPCHECK(hipSetDevice(ctx->device));
   // layer bookkeeping: host copies of n_layers / offsets (host mode), or the
@@ -1411,9 +1417,14 @@ int do_call(crtm_context* ctx, const crtm_atmosphere* atm, const crtm_surface* s
       in(k->brightness_temperature_k, (size_t)M * L * sizeof(double), (const void**)&o.bt_k);
       in(k->radiance_k, (size_t)M * L * sizeof(double), (const void**)&o.rad_k);
     }
-    struct Out { void* h; size_t n; void** d; };
+    // acc: an adjoint output, added onto the caller's array after the call
+    struct Out { void* h; size_t n; void** d; bool acc; };
     std::vector<Out> outs;
-    auto out = [&](void* h, size_t n, void** d) { if (h) outs.push_back({h, n, d}); else *d = nullptr; };
+    auto out = [&](void* h, size_t n, void** d, bool acc = false) {
+      if (h) outs.push_back({h, n, d, acc}); else *d = nullptr;
+    };
+    const size_t rowsL = adj ? 1 : (size_t)L;   // adjoint rows are channel sums
+    const size_t szO = adj ? sizeof(double) : szK;
     const size_t ML = (size_t)M * L * sizeof(double);
     out(rts->radiance, ML, (void**)&o.radiance);
     out(rts->brightness_temperature, ML, (void**)&o.bt);
@@ -1428,19 +1439,19 @@ int do_call(crtm_context* ctx, const crtm_atmosphere* atm, const crtm_surface* s
       out(k->radiance_k, ML, (void**)&o.rad_k);
       out(k->surface_emissivity_k, ML, (void**)&o.emis_k);
       double* stk[4] = {k->land_temperature_k, k->water_temperature_k, k->snow_temperature_k, k->ice_temperature_k};
-      for (int c = 0; c < 4; ++c) out(stk[c], ML, (void**)&o.sfc_t_k[c]);
-      out(k->temperature_k, (size_t)NT * L * szK, (void**)&o.t_k);
-      out(k->absorber_k, (size_t)na * NT * L * szK, (void**)&o.a_k);
+      for (int c = 0; c < 4; ++c) out(stk[c], adj ? M * sizeof(double) : ML, (void**)&o.sfc_t_k[c], adj);
+      out(k->temperature_k, (size_t)NT * rowsL * szO, (void**)&o.t_k, adj);
+      out(k->absorber_k, (size_t)na * NT * rowsL * szO, (void**)&o.a_k, adj);
       if (cc.on) {
         const size_t nc = atm->n_clouds, na_ = atm->n_aerosols;
         if (nc > 0) {
-          out(k->cloud_water_content_k, nc * NT * L * szK, &cc.ko.wc_k);
-          out(k->cloud_effective_radius_k, nc * NT * L * szK, &cc.ko.reff_k);
-          out(k->cloud_fraction_k, (size_t)NT * L * szK, &cc.ko.cf_k);
+          out(k->cloud_water_content_k, nc * NT * rowsL * szO, &cc.ko.wc_k, adj);
+          out(k->cloud_effective_radius_k, nc * NT * rowsL * szO, &cc.ko.reff_k, adj);
+          out(k->cloud_fraction_k, (size_t)NT * rowsL * szO, &cc.ko.cf_k, adj);
         }
         if (na_ > 0) {
-          out(k->aerosol_concentration_k, na_ * NT * L * szK, &cc.ko.aconc_k);
-          out(k->aerosol_effective_radius_k, na_ * NT * L * szK, &cc.ko.areff_k);
+          out(k->aerosol_concentration_k, na_ * NT * rowsL * szO, &cc.ko.aconc_k, adj);
+          out(k->aerosol_effective_radius_k, na_ * NT * rowsL * szO, &cc.ko.areff_k, adj);
         }
       }
     }
@@ -1464,17 +1475,35 @@ int do_call(crtm_context* ctx, const crtm_atmosphere* atm, const crtm_surface* s
       *outs[i].d = base + ooff[i];
     }
     if (dok && !k->radiance_k) o.rad_k = nullptr;
-    if (dok && cc.on) zero_cloud_k(ctx, cc.ko, atm->n_clouds, atm->n_aerosols, NT, L, szK);
+    if (adj) {   // the device sums start from zero; the host adds them to the caller's arrays
+      for (size_t i = 0; i < outs.size(); ++i)
+        if (outs[i].acc) launch_zero(base + ooff[i], outs[i].n, ctx->stream);
+    } else if (dok && cc.on) {
+      zero_cloud_k(ctx, cc.ko, atm->n_clouds, atm->n_aerosols, NT, L, szK);
+    }
+    cc.ko.reduce = adj;
     const DevBatch be = add_layers(ctx, b, abs_id.data());
     if (cc.on) cc.total = prepare_clouds(ctx, be, cc.cl, cc.cw, &cc.max_k);
-    dispatch(ctx, be, o, cc, split, kf32, dok);
+    dispatch(ctx, be, o, cc, split, kf32, dok, adj);
     // per-profile failures: the outputs of the good profiles are written and
     // the call returns FAILURE (src/CRTM_Forward_Module.f90:412-421)
     std::vector<int> pst(M);
     HIPCHECK(hipMemcpyAsync(pst.data(), ctx->call_status, M * sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
-    for (size_t i = 0; i < outs.size(); ++i)
-      HIPCHECK(hipMemcpyAsync(outs[i].h, base + ooff[i], outs[i].n, hipMemcpyDeviceToHost, ctx->stream));
+    std::vector<std::vector<double>> sums(outs.size());
+    for (size_t i = 0; i < outs.size(); ++i) {
+      void* dst = outs[i].h;
+      if (outs[i].acc) {
+        sums[i].resize(outs[i].n / sizeof(double));
+        dst = sums[i].data();
+      }
+      HIPCHECK(hipMemcpyAsync(dst, base + ooff[i], outs[i].n, hipMemcpyDeviceToHost, ctx->stream));
+    }
     HIPCHECK(hipStreamSynchronize(ctx->stream));
+    for (size_t i = 0; i < outs.size(); ++i)
+      if (outs[i].acc) {
+        double* h = static_cast<double*>(outs[i].h);
+        for (size_t j = 0; j < sums[i].size(); ++j) h[j] += sums[i][j];
+      }
     int first_bad = -1;
     for (int m = 0; m < M; ++m) {
       if (rts->status) rts->status[m] = std::max(rts->status[m], pst[m] ? CRTM_FAILURE : CRTM_SUCCESS);
@@ -1549,12 +1578,13 @@ int do_call(crtm_context* ctx, const crtm_atmosphere* atm, const crtm_surface* s
       cc.ko.cf_k = k->cloud_fraction_k;
       cc.ko.aconc_k = k->aerosol_concentration_k;
       cc.ko.areff_k = k->aerosol_effective_radius_k;
-      zero_cloud_k(ctx, cc.ko, atm->n_clouds, atm->n_aerosols, NT, L, szK);
+      if (!adj) zero_cloud_k(ctx, cc.ko, atm->n_clouds, atm->n_aerosols, NT, L, szK);
+      cc.ko.reduce = adj;
     }
   }
   const DevBatch be = add_layers(ctx, b, abs_id.data());
   if (cc.on) cc.total = prepare_clouds(ctx, be, cc.cl, cc.cw, &cc.max_k);
-  dispatch(ctx, be, o, cc, split, kf32, dok);
+  dispatch(ctx, be, o, cc, split, kf32, dok, adj);
   if (rts->status)
     HIPCHECK(hipMemcpyAsync(rts->status, ctx->call_status, M * sizeof(int), hipMemcpyDeviceToDevice, ctx->stream));
   return CRTM_SUCCESS;
@@ -1679,6 +1709,33 @@ int crtm_k_matrix(crtm_context* ctx, const crtm_atmosphere* atm, const crtm_surf
   if (!k) return fail(ctx, "null K-matrix output");
   try {
     return do_call(ctx, atm, sfc, geo, opt, rts, k);
+  } catch (const std::exception& e) {
+    return fail(ctx, e.what());
+  }
+}
+
+int crtm_adjoint(crtm_context* ctx, const crtm_atmosphere* atm, const crtm_surface* sfc, const crtm_geometry* geo,
+                 const crtm_options* opt, crtm_rtsolution* rts, crtm_ad_output* ad) {
+  if (!ctx) return fail(nullptr, "null context");
+  if (!ad) return fail(ctx, "null adjoint output");
+  if (!ad->temperature_ad || !ad->absorber_ad) return fail(ctx, "adjoint temperature/absorber arrays required");
+  crtm_k_output k{};   // the same kernels with channel-summed outputs
+  k.brightness_temperature_k = ad->brightness_temperature_ad;
+  k.radiance_k = ad->radiance_ad;
+  k.surface_emissivity_k = ad->surface_emissivity_ad;
+  k.temperature_k = ad->temperature_ad;
+  k.absorber_k = ad->absorber_ad;
+  k.land_temperature_k = ad->land_temperature_ad;
+  k.water_temperature_k = ad->water_temperature_ad;
+  k.snow_temperature_k = ad->snow_temperature_ad;
+  k.ice_temperature_k = ad->ice_temperature_ad;
+  k.cloud_water_content_k = ad->cloud_water_content_ad;
+  k.cloud_effective_radius_k = ad->cloud_effective_radius_ad;
+  k.cloud_fraction_k = ad->cloud_fraction_ad;
+  k.aerosol_concentration_k = ad->aerosol_concentration_ad;
+  k.aerosol_effective_radius_k = ad->aerosol_effective_radius_ad;
+  try {
+    return do_call(ctx, atm, sfc, geo, opt, rts, &k, true);
   } catch (const std::exception& e) {
     return fail(ctx, e.what());
   }

@@ -218,6 +218,14 @@ struct KSeeds {
   double* tcc_ad;    // adjoint of the total cloud cover
 };
 
+// Sum over the 64 lanes of a wavefront; every lane of the wave must be active
+template <class T>
+__device__ inline T wave_sum(T v) {
+#pragma unroll
+  for (int off = WAVE / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, WAVE);
+  return v;
+}
+
 struct DevOut {
   double* radiance;        // [M][Ltot]
   double* bt;
@@ -230,8 +238,9 @@ struct DevOut {
   double* rad_k;           // [M][Ltot] in/out
   double* emis_k;
   double* sfc_t_k[4];
-  void* t_k;               // [NT][Ltot]
-  void* a_k;               // [na][NT][Ltot]
+  void* t_k;               // [NT][Ltot]; adjoint calls: double [NT], channel sums
+  void* a_k;               // [na][NT][Ltot]; adjoint calls: double [na][NT]
+                           // (adjoint calls: sfc_t_k[c] are double [M], channel sums)
   int k_f32;
   // all-sky: nadir gas OD per layer of the profiles with clouds,
   // [NT (extended layout)][L of the sensor], precision of the call
@@ -243,13 +252,16 @@ struct DevOut {
 };
 
 // Cloud Jacobians of the K-matrix, caller's layer layout, element type KT:
-// water content and effective radius [nc][out_NT][Ltot], fraction [out_NT][Ltot]
+// water content and effective radius [nc][out_NT][Ltot], fraction [out_NT][Ltot].
+// reduce (adjoint calls): double channel sums [nc][out_NT] / [out_NT],
+// accumulated into with atomics
 struct CloudKOut {
   void* wc_k;
   void* reff_k;
   void* cf_k;
   void* aconc_k;   // [na][out_NT][Ltot]
   void* areff_k;
+  int reduce;
 };
 
 // ---------------------------------------------------------------------------

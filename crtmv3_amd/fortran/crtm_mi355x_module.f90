@@ -180,6 +180,12 @@ MODULE CRTM_MI355X_Module
                    surface_planck_radiance, surface_emissivity, status, r_clear, tb_clear, &
                    total_cloud_cover
   END TYPE
+  TYPE, BIND(C) :: c_ad_output
+    TYPE(C_PTR) :: brightness_temperature_ad, radiance_ad, surface_emissivity_ad, temperature_ad, absorber_ad, &
+                   land_temperature_ad, water_temperature_ad, snow_temperature_ad, ice_temperature_ad, &
+                   cloud_water_content_ad, cloud_effective_radius_ad, cloud_fraction_ad, &
+                   aerosol_concentration_ad, aerosol_effective_radius_ad
+  END TYPE
   TYPE, BIND(C) :: c_k_output
     TYPE(C_PTR) :: brightness_temperature_k, radiance_k, surface_emissivity_k, temperature_k, &
                    absorber_k, land_temperature_k, water_temperature_k, snow_temperature_k, ice_temperature_k, &
@@ -227,6 +233,16 @@ MODULE CRTM_MI355X_Module
       TYPE(c_options), INTENT(IN) :: opt
       TYPE(c_rtsolution), INTENT(INOUT) :: rts
       TYPE(c_k_output), INTENT(INOUT) :: k
+    END FUNCTION
+    INTEGER(C_INT) FUNCTION c_crtm_adjoint(ctx, atm, sfc, geo, opt, rts, ad) BIND(C, NAME='crtm_adjoint')
+      IMPORT
+      TYPE(C_PTR), VALUE :: ctx
+      TYPE(c_atmosphere), INTENT(IN) :: atm
+      TYPE(c_surface), INTENT(IN) :: sfc
+      TYPE(c_geometry), INTENT(IN) :: geo
+      TYPE(c_options), INTENT(IN) :: opt
+      TYPE(c_rtsolution), INTENT(INOUT) :: rts
+      TYPE(c_ad_output), INTENT(INOUT) :: ad
     END FUNCTION
     FUNCTION c_crtm_last_error(ctx) RESULT(p) BIND(C, NAME='crtm_last_error')
       IMPORT
@@ -506,12 +522,11 @@ CONTAINS
     CALL label_rts(ChannelInfo, RTSolution)
   END FUNCTION CRTM_K_Matrix
 
-  ! CRTM_Adjoint (src/CRTM_Adjoint_Module.f90:299-325): the channel-summed
-  ! adjoint for RTSolution_AD%Brightness_Temperature / %Radiance, i.e. the
-  ! K-matrix seeded with RTSolution_AD and summed over channels.  Runs the
-  ! K-matrix on profile chunks (at most K_CHUNK_BYTES of per-channel
-  ! Jacobians at a time).  Atmosphere_AD and Surface_AD are accumulated into;
-  ! the emissivity adjoint is per channel, in RTSolution_AD%Surface_Emissivity.
+  ! CRTM_Adjoint (src/CRTM_Adjoint_Module.f90:299-325) through crtm_adjoint:
+  ! the channel-summed adjoint of RTSolution_AD%Brightness_Temperature /
+  ! %Radiance, summed over channels in the kernels (no per-channel Jacobian
+  ! is formed).  Atmosphere_AD and Surface_AD are accumulated into; the
+  ! emissivity adjoint is per channel, added to RTSolution_AD%Surface_Emissivity.
   FUNCTION CRTM_Adjoint(Atmosphere, Surface, RTSolution_AD, Geometry, ChannelInfo, Atmosphere_AD, Surface_AD, &
                         RTSolution, Options) RESULT(Error_Status)
     TYPE(CRTM_Atmosphere_type), INTENT(IN) :: Atmosphere(:)
@@ -524,59 +539,81 @@ CONTAINS
     TYPE(CRTM_RTSolution_type), INTENT(INOUT) :: RTSolution(:, :)
     TYPE(CRTM_Options_type), OPTIONAL, INTENT(IN) :: Options(:)
     INTEGER :: Error_Status
-    TYPE(k_buffers) :: kb
-    INTEGER :: M, L, m0, m1, mm, ic, kk, j, K0, NT
-    IF (.NOT. PRESENT(Options)) THEN
-      Error_Status = FAILURE
+    TYPE(batch_type), TARGET :: b
+    TYPE(c_ad_output) :: ad
+    REAL(C_DOUBLE), ALLOCATABLE, TARGET :: btad(:), radad(:), emad(:), tad(:), aad(:), stad(:, :)
+    REAL(C_DOUBLE), ALLOCATABLE, TARGET :: cwad(:), crad(:), cfad(:), acad(:), arad(:)
+    INTEGER :: L, M, mm, ic, kk, j, K0, NT, na, nc, nae
+    Error_Status = pack_batch(Atmosphere, Surface, Geometry, ChannelInfo, RTSolution, Options, b, L, M)
+    IF (Error_Status /= SUCCESS) RETURN
+    NT = INT(b%off(M + 1))
+    na = Atmosphere(1)%n_Absorbers
+    nc = Atmosphere(1)%n_Clouds
+    nae = Atmosphere(1)%n_Aerosols
+    ALLOCATE(btad(L * M), radad(L * M), emad(L * M), tad(NT), aad(na * NT), stad(M, 4))
+    ALLOCATE(cwad(MAX(nc, 1) * NT), crad(MAX(nc, 1) * NT), cfad(NT), acad(MAX(nae, 1) * NT), arad(MAX(nae, 1) * NT))
+    tad = 0; aad = 0; stad = 0; emad = 0; cwad = 0; crad = 0; cfad = 0; acad = 0; arad = 0
+    DO mm = 1, M
+      DO ic = 1, L
+        btad((mm - 1) * L + ic) = RTSolution_AD(ic, mm)%Brightness_Temperature
+        radad((mm - 1) * L + ic) = RTSolution_AD(ic, mm)%Radiance
+      END DO
+    END DO
+    ad%brightness_temperature_ad = C_LOC(btad)
+    ad%radiance_ad = C_LOC(radad)
+    ad%surface_emissivity_ad = C_LOC(emad)
+    ad%temperature_ad = C_LOC(tad)
+    ad%absorber_ad = C_LOC(aad)
+    ad%land_temperature_ad = C_LOC(stad(1, 1))
+    ad%water_temperature_ad = C_LOC(stad(1, 2))
+    ad%snow_temperature_ad = C_LOC(stad(1, 3))
+    ad%ice_temperature_ad = C_LOC(stad(1, 4))
+    ad%cloud_water_content_ad = C_LOC(cwad)
+    ad%cloud_effective_radius_ad = C_LOC(crad)
+    ad%cloud_fraction_ad = C_LOC(cfad)
+    ad%aerosol_concentration_ad = C_LOC(acad)
+    ad%aerosol_effective_radius_ad = C_LOC(arad)
+    Error_Status = c_crtm_adjoint(ChannelInfo(1)%ctx, b%atm, b%sfc, b%geo, b%opt, b%rts, ad)
+    IF (Error_Status /= SUCCESS) THEN
+      CALL report(ChannelInfo(1)%ctx, 'CRTM_Adjoint')
       RETURN
     END IF
-    M = SIZE(Atmosphere)
-    m0 = 1
-    DO WHILE (m0 <= M)
-      m1 = chunk_end(Atmosphere, ChannelInfo, m0)
-      Error_Status = run_k_chunk(Atmosphere(m0:m1), Surface(m0:m1), Geometry(m0:m1), ChannelInfo, &
-                                 RTSolution(:, m0:m1), Options, m0, m1, RTSolution_AD(:, m0:m1), kb, L, NT)
-      IF (Error_Status /= SUCCESS) RETURN
-      DO mm = m0, m1
-        K0 = INT(kb%b%off(mm - m0 + 1))
-        DO ic = 1, L
-          RTSolution_AD(ic, mm)%Surface_Emissivity = RTSolution_AD(ic, mm)%Surface_Emissivity + &
-              kb%emk((mm - m0) * L + ic)
-        END DO
-        Surface_AD(mm)%Land_Temperature = Surface_AD(mm)%Land_Temperature + SUM(kb%stk((mm - m0) * L + 1:(mm - m0 + 1) * L, 1))
-        Surface_AD(mm)%Water_Temperature = Surface_AD(mm)%Water_Temperature + &
-            SUM(kb%stk((mm - m0) * L + 1:(mm - m0 + 1) * L, 2))
-        Surface_AD(mm)%Snow_Temperature = Surface_AD(mm)%Snow_Temperature + SUM(kb%stk((mm - m0) * L + 1:(mm - m0 + 1) * L, 3))
-        Surface_AD(mm)%Ice_Temperature = Surface_AD(mm)%Ice_Temperature + SUM(kb%stk((mm - m0) * L + 1:(mm - m0 + 1) * L, 4))
-        DO kk = 1, Atmosphere(mm)%n_Layers
-          Atmosphere_AD(mm)%Temperature(kk) = Atmosphere_AD(mm)%Temperature(kk) + &
-              SUM(kb%tk((K0 + kk - 1) * L + 1:(K0 + kk) * L))
-          DO j = 1, kb%na
-            Atmosphere_AD(mm)%Absorber(kk, j) = Atmosphere_AD(mm)%Absorber(kk, j) + &
-                SUM(kb%ak(((j - 1) * NT + K0 + kk - 1) * L + 1:((j - 1) * NT + K0 + kk) * L))
-          END DO
-          IF (kb%nc > 0 .AND. ALLOCATED(Atmosphere_AD(mm)%Cloud)) THEN
-            DO j = 1, kb%nc
-              Atmosphere_AD(mm)%Cloud(j)%Water_Content(kk) = Atmosphere_AD(mm)%Cloud(j)%Water_Content(kk) + &
-                  SUM(kb%cwk(((j - 1) * NT + K0 + kk - 1) * L + 1:((j - 1) * NT + K0 + kk) * L))
-              Atmosphere_AD(mm)%Cloud(j)%Effective_Radius(kk) = Atmosphere_AD(mm)%Cloud(j)%Effective_Radius(kk) + &
-                  SUM(kb%crk(((j - 1) * NT + K0 + kk - 1) * L + 1:((j - 1) * NT + K0 + kk) * L))
-            END DO
-            Atmosphere_AD(mm)%Cloud_Fraction(kk) = Atmosphere_AD(mm)%Cloud_Fraction(kk) + &
-                SUM(kb%cfk((K0 + kk - 1) * L + 1:(K0 + kk) * L))
-          END IF
-          IF (kb%nae > 0 .AND. ALLOCATED(Atmosphere_AD(mm)%Aerosol)) THEN
-            DO j = 1, kb%nae
-              Atmosphere_AD(mm)%Aerosol(j)%Concentration(kk) = Atmosphere_AD(mm)%Aerosol(j)%Concentration(kk) + &
-                  SUM(kb%ack(((j - 1) * NT + K0 + kk - 1) * L + 1:((j - 1) * NT + K0 + kk) * L))
-              Atmosphere_AD(mm)%Aerosol(j)%Effective_Radius(kk) = Atmosphere_AD(mm)%Aerosol(j)%Effective_Radius(kk) + &
-                  SUM(kb%ark(((j - 1) * NT + K0 + kk - 1) * L + 1:((j - 1) * NT + K0 + kk) * L))
-            END DO
-          END IF
-        END DO
+    DO mm = 1, M
+      K0 = INT(b%off(mm))
+      DO ic = 1, L
+        CALL unpack_rts(b, (mm - 1) * L + ic, RTSolution(ic, mm))
+        RTSolution_AD(ic, mm)%Radiance = radad((mm - 1) * L + ic)
+        RTSolution_AD(ic, mm)%Surface_Emissivity = RTSolution_AD(ic, mm)%Surface_Emissivity + emad((mm - 1) * L + ic)
       END DO
-      m0 = m1 + 1
+      Surface_AD(mm)%Land_Temperature = Surface_AD(mm)%Land_Temperature + stad(mm, 1)
+      Surface_AD(mm)%Water_Temperature = Surface_AD(mm)%Water_Temperature + stad(mm, 2)
+      Surface_AD(mm)%Snow_Temperature = Surface_AD(mm)%Snow_Temperature + stad(mm, 3)
+      Surface_AD(mm)%Ice_Temperature = Surface_AD(mm)%Ice_Temperature + stad(mm, 4)
+      DO kk = 1, Atmosphere(mm)%n_Layers
+        Atmosphere_AD(mm)%Temperature(kk) = Atmosphere_AD(mm)%Temperature(kk) + tad(K0 + kk)
+        DO j = 1, na
+          Atmosphere_AD(mm)%Absorber(kk, j) = Atmosphere_AD(mm)%Absorber(kk, j) + aad((j - 1) * NT + K0 + kk)
+        END DO
+        IF (nc > 0 .AND. ALLOCATED(Atmosphere_AD(mm)%Cloud)) THEN
+          DO j = 1, nc
+            Atmosphere_AD(mm)%Cloud(j)%Water_Content(kk) = Atmosphere_AD(mm)%Cloud(j)%Water_Content(kk) + &
+                cwad((j - 1) * NT + K0 + kk)
+            Atmosphere_AD(mm)%Cloud(j)%Effective_Radius(kk) = Atmosphere_AD(mm)%Cloud(j)%Effective_Radius(kk) + &
+                crad((j - 1) * NT + K0 + kk)
+          END DO
+          Atmosphere_AD(mm)%Cloud_Fraction(kk) = Atmosphere_AD(mm)%Cloud_Fraction(kk) + cfad(K0 + kk)
+        END IF
+        IF (nae > 0 .AND. ALLOCATED(Atmosphere_AD(mm)%Aerosol)) THEN
+          DO j = 1, nae
+            Atmosphere_AD(mm)%Aerosol(j)%Concentration(kk) = Atmosphere_AD(mm)%Aerosol(j)%Concentration(kk) + &
+                acad((j - 1) * NT + K0 + kk)
+            Atmosphere_AD(mm)%Aerosol(j)%Effective_Radius(kk) = Atmosphere_AD(mm)%Aerosol(j)%Effective_Radius(kk) + &
+                arad((j - 1) * NT + K0 + kk)
+          END DO
+        END IF
+      END DO
     END DO
+    CALL label_rts(ChannelInfo, RTSolution)
   END FUNCTION CRTM_Adjoint
 
   ! CRTM_Tangent_Linear (src/CRTM_Tangent_Linear_Module.f90:271-296):
@@ -720,7 +757,7 @@ CONTAINS
     k%aerosol_effective_radius_k = C_LOC(kb%ark)
     Error_Status = c_crtm_k_matrix(ChannelInfo(1)%ctx, kb%b%atm, kb%b%sfc, kb%b%geo, kb%b%opt, kb%b%rts, k)
     IF (Error_Status /= SUCCESS) THEN
-      CALL report(ChannelInfo(1)%ctx, 'CRTM_Adjoint / CRTM_Tangent_Linear')
+      CALL report(ChannelInfo(1)%ctx, 'CRTM_Tangent_Linear')
       RETURN
     END IF
     DO mm = 1, M

@@ -232,6 +232,39 @@ int crtm_k_matrix(crtm_context* ctx, const crtm_atmosphere* atm, const crtm_surf
                   const crtm_geometry* geo, const crtm_options* opt, crtm_rtsolution* rts,
                   crtm_k_output* k);
 
+/* ---- Adjoint (CRTM_Adjoint) inputs and outputs ----
+ * Atmosphere_AD and Surface_AD are FP64 channel sums, accumulated into the
+ * caller's arrays as the reference's INTENT(IN OUT) arguments are (callers
+ * zero them first).  Layout: caller's layers, [NT] per variable, absorbers
+ * and clouds [n][NT]; surface temperatures [M]. */
+typedef struct {
+  const double* brightness_temperature_ad;  /* RTSolution_AD%Brightness_Temperature [M][L] */
+  double* radiance_ad;                      /* RTSolution_AD%Radiance [M][L], optional; overwritten with the
+                                               total radiance adjoint as radiance_k is */
+  double* surface_emissivity_ad;            /* [M][L] per-channel emissivity adjoint, optional, overwritten */
+  double* temperature_ad;                   /* Atmosphere_AD%Temperature [NT] */
+  double* absorber_ad;                      /* Atmosphere_AD%Absorber [n_absorbers][NT] */
+  double* land_temperature_ad;              /* Surface_AD temperatures [M] each, optional */
+  double* water_temperature_ad;
+  double* snow_temperature_ad;
+  double* ice_temperature_ad;
+  double* cloud_water_content_ad;           /* Atmosphere_AD%Cloud(n)%Water_Content [n_clouds][NT] */
+  double* cloud_effective_radius_ad;        /* Atmosphere_AD%Cloud(n)%Effective_Radius [n_clouds][NT] */
+  double* cloud_fraction_ad;                /* Atmosphere_AD%Cloud_Fraction [NT] */
+  double* aerosol_concentration_ad;         /* Atmosphere_AD%Aerosol(n)%Concentration [n_aerosols][NT] */
+  double* aerosol_effective_radius_ad;      /* Atmosphere_AD%Aerosol(n)%Effective_Radius [n_aerosols][NT] */
+} crtm_ad_output;
+
+/* CRTM_Adjoint (src/CRTM_Adjoint_Module.f90:299-325): the forward RTSolution
+ * and the channel-summed adjoint of RTSolution_AD.  The kernels reduce over
+ * each 64-channel block before the per-profile rows are touched (FP64 atomic
+ * adds across blocks, so the last bits depend on the order the blocks
+ * finish); no per-channel Jacobian is stored.  Cloud and aerosol arrays are
+ * required when the atmosphere carries them. */
+int crtm_adjoint(crtm_context* ctx, const crtm_atmosphere* atm, const crtm_surface* sfc,
+                 const crtm_geometry* geo, const crtm_options* opt, crtm_rtsolution* rts,
+                 crtm_ad_output* ad);
+
 /* CRTM_K_Matrix without device allocation.  No counterpart in the reference:
  * it is for callers that cannot allocate device memory during a call (code
  * under HIP graph capture, tight serving loops, harnesses that forbid

@@ -58,7 +58,8 @@ def test_ctypes_structs_match_header():
     from crtmv3_amd import api
     hs = _header_structs()
     pairs = {"crtm_atmosphere": api.CAtmosphere, "crtm_surface": api.CSurface, "crtm_geometry": api.CGeometry,
-             "crtm_options": api.COptions, "crtm_rtsolution": api.CRTSolution, "crtm_k_output": api.CKOutput}
+             "crtm_options": api.COptions, "crtm_rtsolution": api.CRTSolution, "crtm_k_output": api.CKOutput,
+             "crtm_ad_output": api.CADOutput}
     for cname, py in pairs.items():
         assert [f[0] for f in py._fields_] == hs[cname], cname
 
@@ -72,6 +73,7 @@ def test_null_context_is_an_error_not_a_crash():
     assert L.crtm_forward(None, None, None, None, None, None) == api.FAILURE
     assert L.crtm_k_matrix(None, None, None, None, None, None, None) == api.FAILURE
     assert L.crtm_k_matrix_noalloc(None, None, None, None, None, None, None) == api.FAILURE
+    assert L.crtm_adjoint(None, None, None, None, None, None, None) == api.FAILURE
     assert L.crtm_destroy(None) == api.FAILURE
     assert L.crtm_last_error(None) is not None
 

@@ -73,6 +73,8 @@ def test_per_profile_streams_match_oracle(batch, coeff_dir, oracle_mod):
 
 @pytest.mark.parametrize("precision", [api.PREC_FP64, api.PREC_FP32_SPLIT], ids=["fp64", "split"])
 def test_adjoint_equals_k_matrix_channel_sums(batch, precision):
+    """crtm_adjoint (channel sums formed in the kernels) equals the channel
+    sums of the K-matrix seeded with the same RTSolution_AD."""
     import torch
     ctx, ps, ns, eps, dps, dev = batch
     print(f"batched adjoint vs K-matrix sums precision={precision}", flush=True)
@@ -81,8 +83,7 @@ def test_adjoint_equals_k_matrix_channel_sums(batch, precision):
     bt_ad = torch.randn(M, L, generator=g, dtype=torch.float64).to(dev)
     e = torch.as_tensor(eps, device=dev)
     # a small budget forces several profile chunks
-    bt, ad = ctx.adjoint(dps, e, bt_ad, precision=precision, n_streams=ns, jacobian_f32=False,
-                         max_jacobian_bytes=64 << 20)
+    bt, ad = ctx.adjoint(dps, e, bt_ad, precision=precision, n_streams=ns, max_work_bytes=64 << 20)
     rts, kd = ctx.k_matrix(dps, e, bt_k=bt_ad, precision=precision, n_streams=ns)
     torch.cuda.synchronize()
     np.testing.assert_allclose(bt.cpu().numpy(), rts["brightness_temperature"].cpu().numpy(), rtol=0, atol=1e-9)
@@ -97,7 +98,10 @@ def test_adjoint_equals_k_matrix_channel_sums(batch, precision):
         got = ad[a_key].cpu().numpy()
         scale = np.abs(want).max()
         assert scale > 0, k_key
-        assert np.abs(got - want).max() <= 1e-10 * scale, (a_key, np.abs(got - want).max() / scale)
+        # FP64: summation order only; split: the adjoint sums the Planck and
+        # Map_Input parts of T_AD separately where the K row adds them in FP32
+        tol = 1e-10 if precision == api.PREC_FP64 else 2e-6
+        assert np.abs(got - want).max() <= tol * scale, (a_key, np.abs(got - want).max() / scale)
     np.testing.assert_array_equal(ad["surface_emissivity_ad"].cpu().numpy(),
                                   kd["surface_emissivity_k"].cpu().numpy())
 
@@ -109,7 +113,7 @@ def test_gradient_of_cost_all_sky(batch):
     print("batched autograd gradient of J vs central differences", flush=True)
     M, L = ps.n_profiles, ctx.n_channels
     e = torch.as_tensor(eps, device=dev)
-    kw = dict(precision=api.PREC_FP64, n_streams=ns, max_jacobian_bytes=64 << 20)
+    kw = dict(precision=api.PREC_FP64, n_streams=ns, max_work_bytes=64 << 20)
     with torch.no_grad():
         bt0 = crtm_bt(ctx, dps, dps.temperature, dps.absorber, dps.water_temperature, e, **kw)
     g = torch.Generator(device="cpu").manual_seed(6)
@@ -171,8 +175,7 @@ def test_tangent_linear_matches_finite_differences_and_adjoint(batch):
           "cloud_water_content": 0.01 * dps.cloud_water_content * rnd(tuple(dps.cloud_water_content.shape)),
           "aerosol_concentration": 0.01 * dps.aerosol_concentration * rnd(tuple(dps.aerosol_concentration.shape))}
     e = torch.as_tensor(eps, device=dev)
-    kw = dict(precision=api.PREC_FP64, n_streams=ns, max_jacobian_bytes=64 << 20)
-    bt, bt_tl = ctx.tangent_linear(dps, e, tl, **kw)
+    bt, bt_tl = ctx.tangent_linear(dps, e, tl, precision=api.PREC_FP64, n_streams=ns, max_jacobian_bytes=64 << 20)
     # central differences along the perturbation
     import dataclasses
 
@@ -191,7 +194,7 @@ def test_tangent_linear_matches_finite_differences_and_adjoint(batch):
     assert err < 1e-5
     # dot-product test with the adjoint
     bt_ad = rnd((M, L))
-    _, ad = ctx.adjoint(dps, e, bt_ad, **kw)
+    _, ad = ctx.adjoint(dps, e, bt_ad, precision=api.PREC_FP64, n_streams=ns, max_work_bytes=64 << 20)
     lhs = float((bt_tl * bt_ad).sum())
     rhs = float((tl["temperature"] * ad["temperature_ad"]).sum() + (tl["absorber"] * ad["absorber_ad"]).sum()
                 + (tl["water_temperature"] * ad["water_temperature_ad"]).sum()
