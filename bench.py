@@ -145,7 +145,7 @@ def copy_calibration():
 
 
 KERNEL_SOURCES = [os.path.join(ROOT, "crtmv3_amd", "csrc", f)
-                  for f in ("clear_fwdk.hip", "odps_prep.hip", "crtm_device.h")]
+                  for f in ("clear_fwdk.hip", "odps_prep.hip", "crtm_device.h", "pred_adjoint.h")]
 
 
 def kernel_fingerprint():
