@@ -330,6 +330,21 @@ def _add_optran(t, sd, rng, wlo_shape, lvl_p, lay_p, ref_t, ref_q):
     t.ocomponent_index = 2
 
 
+# polarisation flags (src/InstrumentInfo/SensorInfo/SensorInfo_Parameters.f90:92-108)
+UNPOLARIZED, SECOND_STOKES_COMPONENT, THIRD_STOKES_COMPONENT, FOURTH_STOKES_COMPONENT = 1, 2, 3, 4
+VL_POLARIZATION, HL_POLARIZATION, PLUS45L_POLARIZATION, MINUS45L_POLARIZATION = 5, 6, 7, 8
+VL_MIXED_POLARIZATION, HL_MIXED_POLARIZATION, RC_POLARIZATION, LC_POLARIZATION = 9, 10, 11, 12
+CONST_MIXED_POLARIZATION, PRA_POLARIZATION = 13, 14
+# ATMS-like: quasi-V at 23.8/31.4/88.2 GHz, quasi-H elsewhere; the AMSU-A-like
+# set cycles through every flag the surface optics combine
+_MW_POLARIZATION = {
+    "atms_syn": (9, 9, 10, 10, 10, 10, 10, 10, 10, 10, 10, 10, 10, 10, 10, 9, 10, 10, 10, 10, 10, 10),
+    "amsua_syn": (9, 10, 5, 6, 1, 2, 3, 4, 7, 8, 11, 12, 13, 14, 9),
+}
+# bumped when the content of the synthetic sensor files changes
+_SENSOR_FILE_REV = 2
+
+
 def make_spc_coeff(sd: SensorDef, seed: int = 20261203) -> coeffio.SpcCoeff:
     rng = np.random.default_rng(seed + 11 + sum(map(ord, sd.sensor_id)))
     L = sd.wavenumber.size
@@ -346,10 +361,16 @@ def make_spc_coeff(sd: SensorDef, seed: int = 20261203) -> coeffio.SpcCoeff:
         cosmic = c1 / (np.exp(c2 / (bc1 + bc2 * COSMIC_T)) - 1.0)
     else:
         cosmic = np.zeros(L)
+    pol, pol_angle = np.ones(L, np.int32), np.zeros(L)
+    if sd.sensor_type == coeffio.MICROWAVE_SENSOR:
+        pattern = _MW_POLARIZATION.get(sd.sensor_id, (UNPOLARIZED,))
+        pol = np.resize(np.asarray(pattern, np.int32), L)
+        pol_angle = np.where(np.isin(pol, (CONST_MIXED_POLARIZATION, PRA_POLARIZATION)),
+                             20.0 + 3.0 * np.arange(L), 0.0)
     return coeffio.SpcCoeff(
         sensor_id=sd.sensor_id, sensor_type=sd.sensor_type, wmo_satellite_id=1023,
         wmo_sensor_id=2047, sensor_channel=np.arange(1, L + 1, dtype=np.int32),
-        polarization=np.ones(L, np.int32), pol_angle=np.zeros(L), channel_flag=np.zeros(L, np.int32),
+        polarization=pol, pol_angle=pol_angle, channel_flag=np.zeros(L, np.int32),
         frequency=sd.frequency, wavenumber=nu, planck_c1=c1, planck_c2=c2, band_c1=bc1,
         band_c2=bc2, cosmic_background_radiance=cosmic, solar_irradiance=np.zeros(L))
 
@@ -361,7 +382,7 @@ def write_sensor(dirpath: str, name: str, seed: int = 20261203) -> str:
     tau = os.path.join(dirpath, f"{name}.TauCoeff.bin")
     stamp = os.path.join(dirpath, f"{name}.seed")
     if os.path.exists(spc) and os.path.exists(tau) and os.path.exists(stamp) \
-            and open(stamp).read().strip() == str(seed):
+            and open(stamp).read().strip() == f"{seed}:{_SENSOR_FILE_REV}":
         return dirpath
     sd = sensor_def(name, seed)
     coeffio.write_spccoeff(spc + ".tmp", make_spc_coeff(sd, seed))
@@ -369,7 +390,7 @@ def write_sensor(dirpath: str, name: str, seed: int = 20261203) -> str:
     os.replace(spc + ".tmp", spc)
     os.replace(tau + ".tmp", tau)
     with open(stamp, "w") as f:
-        f.write(str(seed))
+        f.write(f"{seed}:{_SENSOR_FILE_REV}")
     return dirpath
 
 
@@ -720,3 +741,71 @@ def add_aerosols(ps: ProfileSet, seed: int = 20261205,
     out.aerosol_concentration = conc
     out.aerosol_effective_radius = re_
     return out
+
+
+# ----------------------------------------------------------------------------
+# Surface emissivity model coefficients: FASTEM6 (microwave water) and the
+# IR sea-surface emissivity table.  Synthetic values of physically plausible
+# size, so that every term of both models is exercised.
+# ----------------------------------------------------------------------------
+MWWATERCOEFF_FILE = "FASTEM6.MWwater.EmisCoeff.bin"
+IRWATERCOEFF_FILE = "Nalli.IRwater.EmisCoeff.bin"
+
+
+def make_mwwater_coeff(seed: int = 20261206) -> coeffio.MWwaterCoeff:
+    rng = np.random.default_rng(seed)
+    u = lambda *shape: rng.uniform(0.6, 1.4, shape)
+    fc = np.array([2.95e-6, 3.52])                         # Monahan-O'Muircheartaigh whitecap law
+    fr = np.array([0.93, -2.0e-4, -2.0e-5, 1.0e-8, 1.0, -2.0e-3])
+    # reflection correction: each of the 7 predictors (1, var, var/cos, 1/cos,
+    # (var/cos)^2, 1/cos^2, var^2; slope variance var ~ 0.003 + 0.005 U) adds
+    # at most ~0.04 to the roughness exponent
+    zmag = np.array([1.0, 0.05, 0.07, 1.5, 4.0e-3, 2.2, 2.5e-3])
+    rc = np.empty((3, 7, 2))
+    for t, odmax in enumerate((1.0, 4.6, 21.0)):
+        rc[t] = 0.04 / (odmax * zmag)[:, None] * u(7, 2) * rng.choice((-1.0, 1.0), (7, 2))
+    az = np.zeros((6, 6, 2))
+    fscale = u(6)
+    az[0, :, 0] = 1.0
+    az[1:4, :, 0] = np.array([-3.0e-4, 2.0e-5, -1.0e-6])[:, None] * fscale
+    az[4, :, 0] = 0.02
+    az[5, :, 0] = 1.0e-4 * fscale
+    az[0, :, 1] = 1.2e-4 * fscale
+    az[1, :, 1] = 1.0
+    az[2:5, :, 1] = np.array([-2.5e-4, 1.5e-5, -8.0e-7])[:, None] * fscale
+    az[5, :, 1] = 0.025
+    ssc = np.array([1.0e-5, 1.0e-8, 1.0e-6, -1.0e-9, 1.0e-3, 1.0e-2, 1.0e-3, 1.0e-4]) * u(8)
+    z0 = np.array([[-1.0e-2, 1.2e-2, -3.0e-3, -2.0e-4, 5.0e-6, 1.0e-4],
+                   [-1.2e-2, 1.5e-2, -3.5e-3, 2.5e-4, -4.0e-6, -1.5e-4]]).T      # (term, pol)
+    lsc = np.stack([z0, 2.0e-3 * z0, -1.0e-5 * z0]) * u(3, 6, 2)
+    return coeffio.MWwaterCoeff(fc=fc, fr=fr, rc=rc, az=az, ssc=ssc, lsc=lsc)
+
+
+def make_irwater_coeff(with_temperature: bool = True) -> coeffio.IRwaterCoeff:
+    ang = np.array([0.0, 10.0, 20.0, 30.0, 40.0, 50.0, 55.0, 60.0, 65.0, 70.0, 75.0, 80.0, 85.0])
+    nu = np.linspace(600.0, 3000.0, 49)
+    v = np.array([0.0, 2.0, 4.0, 6.0, 8.0, 10.0, 12.0, 15.0])
+    t = np.array([271.0, 280.0, 290.0, 300.0, 310.0]) if with_temperature else np.array([300.0])
+    A, F, V, T = np.meshgrid(ang, nu, v, t, indexing="ij")
+    base = 0.976 - 0.010 * np.sin(F / 310.0) - 0.004 * np.cos(F / 90.0)
+    fall = (1.0 - np.cos(np.radians(A))) ** 3 * (0.62 - 0.012 * V)
+    e = base * (1.0 - fall) + 1.5e-4 * (T - 290.0) * (1.0 + 0.5 * np.cos(F / 500.0))
+    return coeffio.IRwaterCoeff("Nalli2" if with_temperature else "Nalli", ang, nu, v, t, e)
+
+
+def write_mwwater_coeff(dirpath: str) -> str:
+    os.makedirs(dirpath, exist_ok=True)
+    path = os.path.join(dirpath, MWWATERCOEFF_FILE)
+    if not os.path.exists(path):
+        coeffio.write_mwwatercoeff(path + ".tmp", make_mwwater_coeff())
+        os.replace(path + ".tmp", path)
+    return path
+
+
+def write_irwater_coeff(dirpath: str, with_temperature: bool = True, name: str = IRWATERCOEFF_FILE) -> str:
+    os.makedirs(dirpath, exist_ok=True)
+    path = os.path.join(dirpath, name)
+    if not os.path.exists(path):
+        coeffio.write_irwatercoeff(path + ".tmp", make_irwater_coeff(with_temperature))
+        os.replace(path + ".tmp", path)
+    return path

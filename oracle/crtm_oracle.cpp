@@ -496,6 +496,7 @@ template <class R>
 struct ChannelFwd {
   std::vector<R> od, path, upath, tau, B, td, tu, rd, ru;
   R bs, radiance, bt, up_radiance;
+  R eps, rho, trans;           // surface emissivity, reflectivity, column transmittance
   std::vector<R> ln_chi, chi;  // OPTRAN
   std::vector<R> b;            // OPTRAN b(k, 0:np)
   bool optran_ch = false;
@@ -536,9 +537,20 @@ void optran_od(const or_sensor* s, int l, const ProfileState<R>& st, ChannelFwd<
   }
 }
 
+#include "crtm_oracle_sfc.inc"
+
+// Surface inputs of computed optics for one channel (the optics need the
+// column transmittance, known only inside channel_forward)
+template <class R>
+struct SfcArgs {
+  SfcPoint sp;
+  R tw, sal, wind, az;
+};
+
 template <class R>
 void channel_forward(const or_sensor* s, const or_profile* p, const R* t_user, const R& tsfc,
-                     const R& eps, int l, const ProfileState<R>& st, ChannelFwd<R>& cf) {
+                     const R& eps, int l, const ProfileState<R>& st, ChannelFwd<R>& cf,
+                     const SfcArgs<R>* sa = nullptr) {
   const int K = st.K, Ko = st.Ko, J = st.J;
   // ODPS_Compute_AtmAbsorption (ODPS_AtmAbsorption.f90:133-256)
   cf.od.assign(Ko, R(0.0));
@@ -595,7 +607,18 @@ void channel_forward(const or_sensor* s, const or_profile* p, const R* t_user, c
     cf.rd[k] = cf.rd[k - 1] * cf.td[k - 1] + cf.B[k - 1] * (1.0 - cf.td[k - 1]);
     cf.tu[k - 1] = std::exp(-cf.tau[k - 1] / u);
   }
-  cf.ru[K] = eps * cf.bs + (1.0 - eps) * cf.rd[K];
+  if (sa) {
+    // CRTM_Compute_Transmittance (CRTM_AtmOptics.f90:210-218) feeds the
+    // FASTEM reflection correction
+    R od_sum(0.0);
+    for (int k = 0; k < K; ++k) od_sum = od_sum + cf.tau[k];
+    cf.trans = std::exp(-od_sum);
+    sfc_eval(sa->sp, sa->tw, sa->sal, sa->wind, sa->az, cf.trans, cf.eps, cf.rho);
+  } else {
+    cf.eps = eps;
+    cf.rho = 1.0 - eps;
+  }
+  cf.ru[K] = cf.eps * cf.bs + cf.rho * cf.rd[K];
   cf.up_radiance = R(0.0);
   for (int k = K; k >= 1; --k) {
     R src = cf.B[k - 1] * (1.0 - cf.tu[k - 1]);
@@ -611,6 +634,27 @@ double sfc_temperature(const or_profile* p) {
   // CRTM_Compute_SurfaceT (src/SfcOptics/CRTM_SfcOptics.f90:204-217)
   return p->coverage[0] * p->sfc_temperature[0] + p->coverage[1] * p->sfc_temperature[1] +
          p->coverage[2] * p->sfc_temperature[2] + p->coverage[3] * p->sfc_temperature[3];
+}
+
+// The computed-optics inputs of profile p, channel l (SfcOptics%Angle of the
+// sensor, src/RTSolution/Common_RTSolution.f90:290-352).  False for an
+// unknown polarisation flag.
+template <class R>
+bool make_sfc_args(const or_sensor* s, const or_profile* p, const or_surface_optics* so, int l, double cos_sensor,
+                   SfcArgs<R>& sa) {
+  SfcPoint& sp = sa.sp;
+  sp.so = so;
+  sp.microwave = s->sensor_type == 1;
+  sp.f = sp.microwave ? s->frequency[l] : s->wavenumber[l];
+  sp.angle = std::acos(cos_sensor) / D2R;
+  sp.coverage = p->coverage[1];
+  sa.tw = R(p->sfc_temperature[1]);
+  sa.sal = R(so->salinity);
+  sa.wind = R(so->wind_speed);
+  sa.az = R(so->wind_direction - so->sensor_azimuth);
+  if (!sp.microwave) return true;
+  return pol_weights(so->polarization[l], sp.angle, distance_ratio(p->sensor_zenith, p->sensor_scan),
+                     so->pol_angle[l], D2R * p->sensor_scan, sp.wv, sp.wh);
 }
 
 // Radiance adjoint of BT (CRTM_Planck_Temperature_AD, CRTM_Planck_Functions.f90:697-723)
@@ -761,14 +805,19 @@ struct ExtOut {
 // ===========================================================================
 // C API
 // ===========================================================================
-static int forward_ext(const or_sensor* s, const or_profile* p, const double* emissivity, or_out* o) {
+static int forward_ext(const or_sensor* s, const or_profile* p, const double* emissivity, or_out* o,
+                       const or_surface_optics* so = nullptr, double* emis = nullptr, double* refl = nullptr) {
   const int K = p->n_layers, L = s->n_channels;
   ProfileState<double> st;
   map_and_predict<double>(s, p, p->temperature, p->absorber, st);
   double tsfc = sfc_temperature(p);
   ChannelFwd<double> cf;
   for (int l = 0; l < L; ++l) {
-    channel_forward<double>(s, p, p->temperature, tsfc, emissivity[l], l, st, cf);
+    SfcArgs<double> sa;
+    if (so && !make_sfc_args(s, p, so, l, st.cos_sensor, sa)) return 3;
+    channel_forward<double>(s, p, p->temperature, tsfc, so ? 0.0 : emissivity[l], l, st, cf, so ? &sa : nullptr);
+    if (emis) emis[l] = cf.eps;
+    if (refl) refl[l] = cf.rho;
     o->radiance[l] = cf.radiance;
     o->bt[l] = cf.bt;
     if (o->tau) std::memcpy(o->tau + (size_t)l * K, cf.tau.data(), sizeof(double) * K);
@@ -779,26 +828,43 @@ static int forward_ext(const or_sensor* s, const or_profile* p, const double* em
 }
 
 static int cstep_ext(const or_sensor* s, const or_profile* p, const double* emissivity, const double* bt_k,
-                     const double* rad_k, or_out* o) {
+                     const double* rad_k, or_out* o, const or_surface_optics* so = nullptr,
+                     double* sfc_k = nullptr) {
   const int K = p->n_layers, L = s->n_channels, na = p->n_absorbers;
-  forward_ext(s, p, emissivity, o);
+  if (int r = forward_ext(s, p, emissivity, o, so)) return r;
   const double h = 1.0e-30;
-  const int ndir = K * (1 + na) + 2;
+  // directions: layer temperatures, absorbers, surface temperature, then
+  // either the user emissivities or (computed optics) salinity, wind speed
+  // and wind direction; the surface temperature direction perturbs the
+  // water temperature of the optics too
+  const int nd0 = K * (1 + na) + 1;
+  const int ndir = nd0 + (so ? 3 : 1);
   std::vector<cplx> tu(K), au((size_t)na * K), eps(L);
   for (int d = 0; d < ndir; ++d) {
     for (int k = 0; k < K; ++k) tu[k] = cplx(p->temperature[k], 0.0);
     for (int i = 0; i < na * K; ++i) au[i] = cplx(p->absorber[i], 0.0);
     cplx tsfc(sfc_temperature(p), 0.0);
-    for (int l = 0; l < L; ++l) eps[l] = cplx(emissivity[l], 0.0);
+    for (int l = 0; l < L; ++l) eps[l] = cplx(so ? 0.0 : emissivity[l], 0.0);
+    cplx dtw(0.0), dsal(0.0), dwind(0.0), daz(0.0);
     if (d < K) tu[d] += cplx(0.0, h);
     else if (d < K * (1 + na)) au[d - K] += cplx(0.0, h);
-    else if (d == K * (1 + na)) tsfc += cplx(0.0, h);
-    else for (int l = 0; l < L; ++l) eps[l] += cplx(0.0, h);
+    else if (d == K * (1 + na)) {
+      if (so) { tsfc += cplx(0.0, h * p->coverage[1]); dtw = cplx(0.0, h); }
+      else tsfc += cplx(0.0, h);
+    } else if (!so) for (int l = 0; l < L; ++l) eps[l] += cplx(0.0, h);
+    else if (d == nd0) dsal = cplx(0.0, h);
+    else if (d == nd0 + 1) dwind = cplx(0.0, h);
+    else daz = cplx(0.0, h);
     ProfileState<cplx> st;
     map_and_predict<cplx>(s, p, tu.data(), au.data(), st);
     ChannelFwd<cplx> cf;
     for (int l = 0; l < L; ++l) {
-      channel_forward<cplx>(s, p, tu.data(), tsfc, eps[l], l, st, cf);
+      SfcArgs<cplx> sa;
+      if (so) {
+        make_sfc_args(s, p, so, l, st.cos_sensor, sa);
+        sa.tw += dtw; sa.sal += dsal; sa.wind += dwind; sa.az += daz;
+      }
+      channel_forward<cplx>(s, p, tu.data(), tsfc, eps[l], l, st, cf, so ? &sa : nullptr);
       double dbt = im(cf.bt) / h, drad = im(cf.radiance) / h;
       double g = dbt * bt_k[l] + drad * rad_k[l];
       if (d < K) {
@@ -807,9 +873,12 @@ static int cstep_ext(const or_sensor* s, const or_profile* p, const double* emis
         if (o->abs_k) o->abs_k[(size_t)l * na * K + (d - K)] = g;
       } else if (d == K * (1 + na)) {
         if (o->sfc_t_k)
-          for (int c = 0; c < 4; ++c) o->sfc_t_k[(size_t)l * 4 + c] = p->coverage[c] * g;
-      } else {
+          for (int c = 0; c < 4; ++c) o->sfc_t_k[(size_t)l * 4 + c] = so ? (c == 1 ? g : 0.0) : p->coverage[c] * g;
+      } else if (!so) {
         if (o->emis_k) o->emis_k[l] = g;
+      } else {
+        if (sfc_k) sfc_k[(size_t)l * 3 + (d - nd0)] = g;
+        if (o->emis_k) o->emis_k[l] = 0.0;
       }
     }
   }
@@ -1023,7 +1092,8 @@ struct ChannelSeeds {
 
 void k_matrix_profile(const or_sensor* s, const or_profile* p, const double* emissivity,
                       const double* bt_k, const double* rad_k, or_out* o,
-                      const std::vector<ChannelSeeds>* seeds = nullptr) {
+                      const std::vector<ChannelSeeds>* seeds = nullptr,
+                      const or_surface_optics* so = nullptr, double* sfc_k = nullptr) {
   const int K = p->n_layers, L = s->n_channels, na = p->n_absorbers;
   ProfileState<double> st;
   map_and_predict<double>(s, p, p->temperature, p->absorber, st);
@@ -1032,7 +1102,9 @@ void k_matrix_profile(const or_sensor* s, const or_profile* p, const double* emi
   ChannelFwd<double> cf;
   AdjWork w;
   for (int l = 0; l < L; ++l) {
-    channel_forward<double>(s, p, p->temperature, tsfc, emissivity[l], l, st, cf);
+    SfcArgs<double> sa;
+    if (so) make_sfc_args(s, p, so, l, st.cos_sensor, sa);
+    channel_forward<double>(s, p, p->temperature, tsfc, so ? 0.0 : emissivity[l], l, st, cf, so ? &sa : nullptr);
     const ChannelSeeds* sd = seeds ? &(*seeds)[l] : nullptr;
     const bool ext = sd && sd->on;
     o->radiance[l] = ext ? sd->radiance : cf.radiance;
@@ -1044,7 +1116,7 @@ void k_matrix_profile(const or_sensor* s, const or_profile* p, const double* emi
     double rad_ad = ext ? sd->rad_ad_clear : rad_k[l] + planck_t_ad_factor(s, l, cf.radiance) * bt_k[l];
     if (o->radiance_k) o->radiance_k[l] = ext ? sd->rad_k : rad_ad;
     // CRTM_Emission_AD (Emission_Module.f90:280-395)
-    double u = st.cos_sensor, sec_down = 1.0 / u, eps = emissivity[l];
+    double u = st.cos_sensor, sec_down = 1.0 / u, eps = cf.eps, rho = cf.rho;
     w.tau_ad.assign(K, 0.0);
     w.b_ad.assign(K, 0.0);
     double up_ad = rad_ad;
@@ -1058,7 +1130,7 @@ void k_matrix_profile(const or_sensor* s, const or_profile* p, const double* emi
     double eps_ad = up_ad * cf.bs;
     double bs_ad = eps * up_ad;
     double r_ad = up_ad * cf.rd[K];
-    double down_ad = (1.0 - eps) * up_ad;
+    double down_ad = rho * up_ad;
     for (int k = K; k >= 1; --k) {
       double td = cf.td[k - 1];
       double a_ad = td * (cf.rd[k - 1] - cf.B[k - 1]) * down_ad;
@@ -1074,6 +1146,24 @@ void k_matrix_profile(const or_sensor* s, const or_profile* p, const double* emi
       bs_ad += sd->bs_ad;
       eps_ad += sd->emis_ad;
     }
+    // computed optics: CRTM_Compute_SfcOptics_AD through the emissivity and
+    // reflectivity adjoints (derivatives of the optics by complex step), then
+    // CRTM_Compute_Transmittance_AD (CRTM_AtmOptics.f90:314-368)
+    double sg[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    if (so) {
+      const double h = 1.0e-30;
+      for (int d = 0; d < 5; ++d) {
+        cplx in[5] = {cplx(sa.tw), cplx(sa.sal), cplx(sa.wind), cplx(sa.az), cplx(cf.trans)};
+        in[d] += cplx(0.0, h);
+        cplx e, r;
+        SfcPoint spc = sa.sp;
+        sfc_eval<cplx>(spc, in[0], in[1], in[2], in[3], in[4], e, r);
+        sg[d] = im(e) / h * eps_ad + im(r) / h * r_ad;
+      }
+      for (int k = 0; k < K; ++k) w.tau_ad[k] -= cf.trans * sg[4];
+      if (sfc_k)
+        for (int d = 0; d < 3; ++d) sfc_k[(size_t)l * 3 + d] = sg[1 + d];
+    }
     // Planck AD (Assign_Common_Output_AD, Common_RTSolution.f90:1680-1700)
     double* tk = o->t_k ? o->t_k + (size_t)l * K : nullptr;
     double* ak = o->abs_k ? o->abs_k + (size_t)l * na * K : nullptr;
@@ -1082,8 +1172,9 @@ void k_matrix_profile(const or_sensor* s, const or_profile* p, const double* emi
         tk[k] = planck_r_ad_factor(s, l, p->temperature[k]) * w.b_ad[k] + (ext ? sd->t_ad[k] : 0.0);
     if (ak) for (int i = 0; i < na * K; ++i) ak[i] = 0.0;
     double sfct_ad = planck_r_ad_factor(s, l, tsfc) * bs_ad;
-    if (o->sfc_t_k) for (int c = 0; c < 4; ++c) o->sfc_t_k[(size_t)l * 4 + c] = p->coverage[c] * sfct_ad;
-    if (o->emis_k) o->emis_k[l] = eps_ad - r_ad;
+    if (o->sfc_t_k)
+      for (int c = 0; c < 4; ++c) o->sfc_t_k[(size_t)l * 4 + c] = p->coverage[c] * sfct_ad + (c == 1 ? sg[0] : 0.0);
+    if (o->emis_k) o->emis_k[l] = so ? 0.0 : eps_ad - r_ad;
     // ODPS_Compute_AtmAbsorption_AD (ODPS_AtmAbsorption.f90:487-600)
     w.upath_ad.assign(K + 1, 0.0);
     for (int k = K; k >= 1; --k) {
@@ -1282,6 +1373,61 @@ extern "C" int oracle_k_matrix(const or_sensor* s, const or_profile* p, const do
     k_matrix_profile(s, q, emissivity, bt_k, rad_k, oo);
     return 0;
   });
+}
+
+extern "C" int oracle_forward_sfc(const or_sensor* s, const or_profile* p, const or_surface_optics* so, or_out* o,
+                                  double* emis, double* refl) {
+  return with_added_layers(s, p, o,
+                           [&](const or_profile* q, or_out* oo) { return forward_ext(s, q, nullptr, oo, so, emis, refl); });
+}
+
+extern "C" int oracle_k_matrix_sfc(const or_sensor* s, const or_profile* p, const or_surface_optics* so,
+                                   const double* bt_k, const double* rad_k, or_out* o, double* sfc_k) {
+  return with_added_layers(s, p, o, [&](const or_profile* q, or_out* oo) {
+    ProfileState<double> st;
+    map_and_predict<double>(s, q, q->temperature, q->absorber, st);
+    SfcArgs<double> sa;
+    for (int l = 0; l < s->n_channels; ++l)
+      if (!make_sfc_args(s, q, so, l, st.cos_sensor, sa)) return 3;
+    k_matrix_profile(s, q, nullptr, bt_k, rad_k, oo, nullptr, so, sfc_k);
+    return 0;
+  });
+}
+
+extern "C" int oracle_k_matrix_sfc_cstep(const or_sensor* s, const or_profile* p, const or_surface_optics* so,
+                                         const double* bt_k, const double* rad_k, or_out* o, double* sfc_k) {
+  return with_added_layers(s, p, o, [&](const or_profile* q, or_out* oo) {
+    return cstep_ext(s, q, nullptr, bt_k, rad_k, oo, so, sfc_k);
+  });
+}
+
+extern "C" void oracle_fastemx(const or_mwwater* c, double frequency, int n_angles, double zenith, const double* in5,
+                               double* emis, double* refl, double* d_emis, double* d_refl) {
+  fastemx<double>(c, frequency, n_angles, zenith, in5[0], in5[1], in5[2], in5[3], in5[4], emis, refl);
+  const double h = 1.0e-30;
+  for (int d = 0; d < 5; ++d) {
+    cplx in[5];
+    for (int i = 0; i < 5; ++i) in[i] = cplx(in5[i], i == d ? h : 0.0);
+    cplx e[4], r[4];
+    fastemx<cplx>(c, frequency, n_angles, zenith, in[0], in[1], in[2], in[3], in[4], e, r);
+    for (int j = 0; j < 4; ++j) {
+      d_emis[j * 5 + d] = im(e[j]) / h;
+      d_refl[j * 5 + d] = im(r[j]) / h;
+    }
+  }
+}
+
+extern "C" void oracle_irssem(const or_irwater* c, double temperature, double wind, double frequency, double angle,
+                              double* emis, double* d2) {
+  *emis = irssem<double>(c, temperature, wind, frequency, angle);
+  const double h = 1.0e-30;
+  d2[0] = im(irssem<cplx>(c, cplx(temperature), cplx(wind, h), frequency, angle)) / h;
+  d2[1] = im(irssem<cplx>(c, cplx(temperature, h), cplx(wind), frequency, angle)) / h;
+}
+
+extern "C" int oracle_pol_weights(int pol, double angle, double distance_ratio_, double pol_angle, double scan_rad,
+                                  double* w2) {
+  return pol_weights(pol, angle, distance_ratio_, pol_angle, scan_rad, w2[0], w2[1]) ? 0 : 3;
 }
 
 extern "C" int oracle_k_matrix_batch(const or_sensor* s, int n_profiles, const or_profile* p,

@@ -92,6 +92,51 @@ int oracle_k_matrix_batch(const or_sensor* s, int n_profiles, const or_profile* 
                           const double* emissivity, const double* bt_k, const double* rad_k,
                           or_out* o, int n_threads);
 
+/* ---- Surface optics of a water surface (computed emissivity) ---- */
+typedef struct {
+  /* FASTEM FitCoeff arrays, Fortran order (src/Coefficients/EmisCoeff/MW_Water/MWwaterCoeff_Define.f90:78-92):
+   * fc[2], fr[6], rc(3,7,2), az(6,6,2), ssc[8], lsc(3,6,2) */
+  const double *fc, *fr, *rc, *az, *ssc, *lsc;
+  int version;
+} or_mwwater;
+
+typedef struct {
+  /* IR sea-surface emissivity table (src/Coefficients/EmisCoeff/IR_Water/IRwaterCoeff_Define.f90:84-106) */
+  int n_angle, n_frequency, n_wind, n_temperature, nalli2;
+  const double *angle, *frequency, *wind_speed, *temperature;
+  const double* emissivity;   /* (angle, frequency, wind, temperature) column-major */
+} or_irwater;
+
+typedef struct {
+  const or_mwwater* mw;       /* microwave sensors */
+  const or_irwater* ir;       /* infrared sensors */
+  const int* polarization;    /* [L] SpcCoeff%Polarization */
+  const double* pol_angle;    /* [L] SpcCoeff%PolAngle, degrees */
+  double salinity, wind_speed, wind_direction, sensor_azimuth;
+} or_surface_optics;
+
+/* Compute_FastemX at one angle; in5 = temperature, salinity, wind speed,
+ * relative azimuth, transmittance; d_emis/d_refl [4][5] the derivatives of
+ * the four Stokes components with respect to in5 (complex step). */
+void oracle_fastemx(const or_mwwater* c, double frequency, int n_angles, double zenith, const double* in5,
+                    double* emis, double* refl, double* d_emis, double* d_refl);
+/* CRTM_Compute_IRSSEM at one angle; d2 = d/d(wind speed), d/d(temperature) */
+void oracle_irssem(const or_irwater* c, double temperature, double wind, double frequency, double angle,
+                   double* emis, double* d2);
+/* Stokes weights (V, H) of a polarisation flag; returns nonzero for an unknown flag */
+int oracle_pol_weights(int pol, double angle, double distance_ratio, double pol_angle, double scan_rad,
+                       double* w2);
+/* Clear-sky forward / K-matrix with the surface optics computed (Water_Coverage
+ * only); emis/refl [L] the emissivity and reflectivity at the sensor angle,
+ * sfc_k [L][3] the salinity, wind speed and wind direction Jacobians (the water
+ * temperature one is in sfc_t_k). */
+int oracle_forward_sfc(const or_sensor* s, const or_profile* p, const or_surface_optics* so, or_out* o,
+                       double* emis, double* refl);
+int oracle_k_matrix_sfc(const or_sensor* s, const or_profile* p, const or_surface_optics* so,
+                        const double* bt_k, const double* rad_k, or_out* o, double* sfc_k);
+int oracle_k_matrix_sfc_cstep(const or_sensor* s, const or_profile* p, const or_surface_optics* so,
+                              const double* bt_k, const double* rad_k, or_out* o, double* sfc_k);
+
 /* The six model climatologies on the 101 model levels
  * (src/Atmosphere/CRTM_Model_Profiles.f90): p[101] hPa, t[6][101] K,
  * a[6 models][6 absorbers in id order H2O,CO2,O3,N2O,CO,CH4][101].  Must be

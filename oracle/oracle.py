@@ -78,6 +78,21 @@ class OrProfile(ctypes.Structure):
                 ("surface_altitude", ctypes.c_double), ("climatology", ctypes.c_int)]
 
 
+class OrMWwater(ctypes.Structure):
+    _fields_ = [(n, _dp) for n in ("fc", "fr", "rc", "az", "ssc", "lsc")] + [("version", ctypes.c_int)]
+
+
+class OrIRwater(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int) for n in ("n_angle", "n_frequency", "n_wind", "n_temperature", "nalli2")] + \
+               [(n, _dp) for n in ("angle", "frequency", "wind_speed", "temperature", "emissivity")]
+
+
+class OrSurfaceOptics(ctypes.Structure):
+    _fields_ = [("mw", ctypes.POINTER(OrMWwater)), ("ir", ctypes.POINTER(OrIRwater)),
+                ("polarization", _ip), ("pol_angle", _dp)] + \
+               [(n, ctypes.c_double) for n in ("salinity", "wind_speed", "wind_direction", "sensor_azimuth")]
+
+
 class OrOut(ctypes.Structure):
     _fields_ = [(n, _dp) for n in ("radiance", "bt", "tau", "up_radiance", "down_radiance", "t_k",
                                    "abs_k", "sfc_t_k", "emis_k", "radiance_k")]
@@ -141,6 +156,17 @@ def lib():
                                     _dp, ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp]
         _lib.oracle_amom_layer.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_int, _dp, ctypes.c_double,
                                            ctypes.c_double, ctypes.c_double, _dp, _dp, _dp]
+        _lib.oracle_fastemx.argtypes = [ctypes.POINTER(OrMWwater), ctypes.c_double, ctypes.c_int,
+                                        ctypes.c_double, _dp, _dp, _dp, _dp, _dp]
+        _lib.oracle_irssem.argtypes = [ctypes.POINTER(OrIRwater), ctypes.c_double, ctypes.c_double,
+                                       ctypes.c_double, ctypes.c_double, _dp, _dp]
+        _lib.oracle_pol_weights.argtypes = [ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                            ctypes.c_double, _dp]
+        _lib.oracle_forward_sfc.argtypes = [ctypes.POINTER(OrSensor), ctypes.POINTER(OrProfile),
+                                            ctypes.POINTER(OrSurfaceOptics), ctypes.POINTER(OrOut), _dp, _dp]
+        for fn in ("oracle_k_matrix_sfc", "oracle_k_matrix_sfc_cstep"):
+            getattr(_lib, fn).argtypes = [ctypes.POINTER(OrSensor), ctypes.POINTER(OrProfile),
+                                          ctypes.POINTER(OrSurfaceOptics), _dp, _dp, ctypes.POINTER(OrOut), _dp]
         _set_climatology(_lib)
     return _lib
 
@@ -443,3 +469,95 @@ def k_matrix_allsky(sensor: Sensor, cloudcoeff: CloudCoeff, ps, m, emissivity, n
     if rc != 0:
         raise RuntimeError(f"oracle_k_matrix_allsky ({method}) failed ({rc})")
     return fwd, kd
+
+
+# ---------------------------------------------------------------------------
+# Surface optics (FASTEM6 microwave water, IR sea-surface emissivity table)
+# ---------------------------------------------------------------------------
+class MWwater:
+    """Oracle view of a coeffio.MWwaterCoeff (keeps arrays alive)."""
+
+    def __init__(self, c):
+        self._keep = [np.ascontiguousarray(np.asarray(a).ravel(order="F"), np.float64)
+                      for a in (c.fc, c.fr, c.rc, c.az, c.ssc, c.lsc)]
+        self.s = OrMWwater(*[_p(a, _dp) for a in self._keep], int(c.version))
+
+
+class IRwater:
+    """Oracle view of a coeffio.IRwaterCoeff."""
+
+    def __init__(self, c):
+        self._keep = [np.ascontiguousarray(np.asarray(a).ravel(order="F"), np.float64)
+                      for a in (c.angle, c.frequency, c.wind_speed, c.temperature, c.emissivity)]
+        self.s = OrIRwater(c.angle.size, c.frequency.size, c.wind_speed.size, c.temperature.size,
+                           int(c.classification == "Nalli2"), *[_p(a, _dp) for a in self._keep])
+
+
+def fastemx(mw: MWwater, frequency, zenith, temperature, salinity, wind, azimuth, transmittance, n_angles=1):
+    """Compute_FastemX: (emis[4], refl[4], d_emis[4,5], d_refl[4,5]); derivatives
+    with respect to temperature, salinity, wind speed, azimuth, transmittance."""
+    in5 = np.array([temperature, salinity, wind, azimuth, transmittance], np.float64)
+    e, r = np.zeros(4), np.zeros(4)
+    de, dr = np.zeros((4, 5)), np.zeros((4, 5))
+    lib().oracle_fastemx(ctypes.byref(mw.s), frequency, n_angles, zenith, _p(in5, _dp), _p(e, _dp), _p(r, _dp),
+                         _p(de, _dp), _p(dr, _dp))
+    return e, r, de, dr
+
+
+def irssem(ir: IRwater, temperature, wind, frequency, angle):
+    """CRTM_Compute_IRSSEM at one angle: (emissivity, d/d wind, d/d temperature)."""
+    e = np.zeros(1)
+    d = np.zeros(2)
+    lib().oracle_irssem(ctypes.byref(ir.s), temperature, wind, frequency, angle, _p(e, _dp), _p(d, _dp))
+    return e[0], d[0], d[1]
+
+
+def pol_weights(pol, angle, distance_ratio, pol_angle=0.0, scan_rad=0.0):
+    w = np.zeros(2)
+    err = lib().oracle_pol_weights(int(pol), angle, distance_ratio, pol_angle, scan_rad, _p(w, _dp))
+    return None if err else w
+
+
+class SurfaceOptics:
+    """Computed-optics inputs of one profile: coefficient set, the sensor's
+    polarisation flags and the profile's salinity / wind / azimuths."""
+
+    def __init__(self, spc, mw=None, ir=None, salinity=33.0, wind_speed=5.0, wind_direction=0.0,
+                 sensor_azimuth=999.9):
+        self._keep = [np.ascontiguousarray(spc.polarization, np.int32), np.ascontiguousarray(spc.pol_angle, np.float64),
+                      mw, ir]
+        self.s = OrSurfaceOptics(ctypes.pointer(mw.s) if mw is not None else None,
+                                 ctypes.pointer(ir.s) if ir is not None else None,
+                                 _p(self._keep[0], _ip), _p(self._keep[1], _dp),
+                                 salinity, wind_speed, wind_direction, sensor_azimuth)
+
+
+def run_profile_sfc(sensor: Sensor, ps, m, so: SurfaceOptics, mode="forward", bt_k=None, rad_k=None):
+    """Clear-sky profile with computed surface optics; mode 'forward', 'k'
+    (hand-written adjoint) or 'cstep' (complex step).  Adds 'emis', 'refl'
+    (forward) and 'sfc_k' [L, 3] (salinity, wind speed, wind direction)."""
+    keep = []
+    p = make_profile(ps, m, keep)
+    L, K, na = sensor.n_channels, p.n_layers, p.n_absorbers
+    out = {"radiance": np.zeros(L), "bt": np.zeros(L), "tau": np.zeros((L, K)),
+           "up_radiance": np.zeros(L), "down_radiance": np.zeros(L)}
+    if mode != "forward":
+        out.update(t_k=np.zeros((L, K)), abs_k=np.zeros((L, na, K)), sfc_t_k=np.zeros((L, 4)),
+                   emis_k=np.zeros(L), radiance_k=np.zeros(L))
+    o = OrOut()
+    for k, v in out.items():
+        setattr(o, k, _p(v, _dp))
+    if mode == "forward":
+        out["emis"], out["refl"] = np.zeros(L), np.zeros(L)
+        err = lib().oracle_forward_sfc(ctypes.byref(sensor.s), ctypes.byref(p), ctypes.byref(so.s), ctypes.byref(o),
+                                       _p(out["emis"], _dp), _p(out["refl"], _dp))
+    else:
+        btk = np.ascontiguousarray(np.ones(L) if bt_k is None else bt_k, np.float64)
+        rk = np.ascontiguousarray(np.zeros(L) if rad_k is None else rad_k, np.float64)
+        out["sfc_k"] = np.zeros((L, 3))
+        fn = lib().oracle_k_matrix_sfc if mode == "k" else lib().oracle_k_matrix_sfc_cstep
+        err = fn(ctypes.byref(sensor.s), ctypes.byref(p), ctypes.byref(so.s), _p(btk, _dp), _p(rk, _dp),
+                 ctypes.byref(o), _p(out["sfc_k"], _dp))
+    if err:
+        raise ValueError(f"oracle surface optics failed ({err})")
+    return out

@@ -16,6 +16,13 @@
 !                            src/Coefficients/AerosolCoeff/AerosolCoeff_Binary_IO.f90:352,
 !                            src/Coefficients/AerosolCoeff/AerosolCoeff_Define.f90:635-696
 !   SpcCoeff_Binary_ReadFile src/Coefficients/SpcCoeff/SpcCoeff_Binary_IO.f90:325
+!   MWwaterCoeff_ReadFile, Compute_FastemX(_AD)
+!                            src/Coefficients/EmisCoeff/MW_Water/MWwaterCoeff_Define.f90:699,
+!                            src/SfcOptics/MW_Water/FASTEM_MWSSEM/CRTM_FastemX.f90:302,803
+!   IRwaterCoeff_ReadFile, CRTM_Compute_IRSSEM(_AD)
+!                            src/Coefficients/EmisCoeff/IR_Water/IRwaterCoeff_Define.f90:805,
+!                            src/SfcOptics/IR_Water/IRSSEM/CRTM_IRSSEM.f90:204,586
+!   CRTM_Compute_Transmittance(_AD) src/AtmOptics/CRTM_AtmOptics.f90:210-368
 MODULE ref_harness
   USE ISO_C_BINDING
   USE Type_Kinds, ONLY: fp
@@ -37,11 +44,20 @@ MODULE ref_harness
   USE AerosolCoeff_Binary_IO, ONLY: AerosolCoeff_Binary_ReadFile
   USE SpcCoeff_Define, ONLY: SpcCoeff_type, SpcCoeff_Destroy
   USE SpcCoeff_Binary_IO, ONLY: SpcCoeff_Binary_ReadFile
+  USE CRTM_AtmOptics, ONLY: CRTM_Compute_Transmittance, CRTM_Compute_Transmittance_AD
+  USE MWwaterCoeff_Define, ONLY: MWwaterCoeff_type, MWwaterCoeff_Destroy, MWwaterCoeff_ReadFile
+  USE CRTM_FastemX, ONLY: FastemX_iVar_type => iVar_type, Compute_FastemX, Compute_FastemX_AD
+  USE IRwaterCoeff_Define, ONLY: IRwaterCoeff_type, IRwaterCoeff_Destroy, IRwaterCoeff_ReadFile
+  USE CRTM_IRSSEM, ONLY: IRSSEM_iVar_type => iVar_type, CRTM_Compute_IRSSEM, CRTM_Compute_IRSSEM_AD
   IMPLICIT NONE
   TYPE(ODPS_type), SAVE :: tc
   TYPE(CloudCoeff_type), SAVE :: ccf
   TYPE(AerosolCoeff_type), SAVE :: acf
   TYPE(SpcCoeff_type), SAVE :: scf
+  TYPE(MWwaterCoeff_type), SAVE :: mwc
+  TYPE(FastemX_iVar_type), SAVE :: fx_var
+  TYPE(IRwaterCoeff_type), SAVE :: irc
+  TYPE(IRSSEM_iVar_type), SAVE :: ir_var
 CONTAINS
 
   FUNCTION c2f(cstr, n) RESULT(s)
@@ -331,5 +347,96 @@ CONTAINS
     cosmic(1:L) = scf%Cosmic_Background_Radiance
     solar(1:L) = scf%Solar_Irradiance
   END SUBROUTINE ref_spccoeff_fetch
+
+  ! MWwaterCoeff_ReadFile (FASTEM coefficient file)
+  INTEGER(C_INT) FUNCTION ref_mwwater_load(path, n) BIND(C, NAME='ref_mwwater_load')
+    CHARACTER(KIND=C_CHAR), INTENT(IN) :: path(*)
+    INTEGER(C_INT), VALUE :: n
+    CALL MWwaterCoeff_Destroy(mwc)
+    ref_mwwater_load = MWwaterCoeff_ReadFile(mwc, c2f(path, n), Quiet=.TRUE.)
+  END FUNCTION ref_mwwater_load
+
+  ! Compute_FastemX with both optional inputs given (an azimuth beyond +-360
+  ! or a transmittance outside (0,1) switches that part off, as in the
+  ! reference); the internal variables are kept for ref_fastemx_ad
+  SUBROUTINE ref_fastemx(freq, n_angles, zenith, t, s, u, az, tr, emis, refl) BIND(C, NAME='ref_fastemx')
+    REAL(C_DOUBLE), VALUE :: freq, zenith, t, s, u, az, tr
+    INTEGER(C_INT), VALUE :: n_angles
+    REAL(C_DOUBLE), INTENT(OUT) :: emis(4), refl(4)
+    REAL(fp) :: e(4), r(4)
+    CALL Compute_FastemX(mwc, freq, INT(n_angles), zenith, t, s, u, fx_var, e, r, &
+                         Azimuth_Angle=az, Transmittance=tr)
+    emis = e
+    refl = r
+  END SUBROUTINE ref_fastemx
+
+  ! Compute_FastemX_AD of the last ref_fastemx call; out5 = adjoints of
+  ! temperature, salinity, wind speed, azimuth angle, transmittance
+  SUBROUTINE ref_fastemx_ad(emis_ad, refl_ad, out5) BIND(C, NAME='ref_fastemx_ad')
+    REAL(C_DOUBLE), INTENT(IN) :: emis_ad(4), refl_ad(4)
+    REAL(C_DOUBLE), INTENT(OUT) :: out5(5)
+    REAL(fp) :: e(4), r(4), ta, sa, ua, aa, tra
+    e = emis_ad
+    r = refl_ad
+    ta = 0.0_fp; sa = 0.0_fp; ua = 0.0_fp; aa = 0.0_fp; tra = 0.0_fp
+    CALL Compute_FastemX_AD(mwc, e, r, fx_var, ta, sa, ua, Azimuth_Angle_AD=aa, Transmittance_AD=tra)
+    out5 = (/ ta, sa, ua, aa, tra /)
+  END SUBROUTINE ref_fastemx_ad
+
+  ! IRwaterCoeff_ReadFile; dims = n_Angles, n_Frequencies, n_Wind_Speeds, n_Temperature
+  INTEGER(C_INT) FUNCTION ref_irwater_load(path, n, dims) BIND(C, NAME='ref_irwater_load')
+    CHARACTER(KIND=C_CHAR), INTENT(IN) :: path(*)
+    INTEGER(C_INT), VALUE :: n
+    INTEGER(C_INT), INTENT(OUT) :: dims(4)
+    CALL IRwaterCoeff_Destroy(irc)
+    ref_irwater_load = IRwaterCoeff_ReadFile(irc, c2f(path, n), Quiet=.TRUE.)
+    IF (ref_irwater_load /= 0) RETURN
+    dims = (/ irc%n_Angles, irc%n_Frequencies, irc%n_Wind_Speeds, irc%n_Temperature /)
+  END FUNCTION ref_irwater_load
+
+  ! CRTM_Compute_IRSSEM at n angles (degrees)
+  INTEGER(C_INT) FUNCTION ref_irssem(t, u, f, n, angles, emis) BIND(C, NAME='ref_irssem')
+    REAL(C_DOUBLE), VALUE :: t, u, f
+    INTEGER(C_INT), VALUE :: n
+    REAL(C_DOUBLE), INTENT(IN) :: angles(n)
+    REAL(C_DOUBLE), INTENT(OUT) :: emis(n)
+    REAL(fp) :: a(n), e(n)
+    a = angles
+    ref_irssem = CRTM_Compute_IRSSEM(irc, t, u, f, a, ir_var, e)
+    emis = e
+  END FUNCTION ref_irssem
+
+  ! CRTM_Compute_IRSSEM_AD of the last ref_irssem call; out2 = wind speed and
+  ! water temperature adjoints
+  INTEGER(C_INT) FUNCTION ref_irssem_ad(n, emis_ad, out2) BIND(C, NAME='ref_irssem_ad')
+    INTEGER(C_INT), VALUE :: n
+    REAL(C_DOUBLE), INTENT(IN) :: emis_ad(n)
+    REAL(C_DOUBLE), INTENT(OUT) :: out2(2)
+    REAL(fp) :: e(n), ua, ta
+    e = emis_ad
+    ua = 0.0_fp
+    ta = 0.0_fp
+    ref_irssem_ad = CRTM_Compute_IRSSEM_AD(irc, e, ir_var, ua, ta)
+    out2 = (/ ua, ta /)
+  END FUNCTION ref_irssem_ad
+
+  ! CRTM_Compute_Transmittance and its adjoint on k layers of optical depth
+  SUBROUTINE ref_transmittance(k, od, tr_ad, tr, od_ad) BIND(C, NAME='ref_transmittance')
+    INTEGER(C_INT), VALUE :: k
+    REAL(C_DOUBLE), INTENT(IN) :: od(k)
+    REAL(C_DOUBLE), VALUE :: tr_ad
+    REAL(C_DOUBLE), INTENT(OUT) :: tr, od_ad(k)
+    TYPE(CRTM_AtmOptics_type) :: ao, ao_ad
+    REAL(fp) :: t, ta
+    CALL CRTM_AtmOptics_Create(ao, k, 1, 1)
+    CALL CRTM_AtmOptics_Create(ao_ad, k, 1, 1)
+    ao%Optical_Depth = od
+    ao_ad%Optical_Depth = 0.0_fp
+    CALL CRTM_Compute_Transmittance(ao, t)
+    tr = t
+    ta = tr_ad
+    CALL CRTM_Compute_Transmittance_AD(ao, ta, ao_ad)
+    od_ad = ao_ad%Optical_Depth
+  END SUBROUTINE ref_transmittance
 
 END MODULE ref_harness
