@@ -55,12 +55,12 @@ class CAtmosphere(ctypes.Structure):
 class CSurface(ctypes.Structure):
     _fields_ = [(n, _vp) for n in ("land_coverage", "water_coverage", "snow_coverage", "ice_coverage",
                                    "land_temperature", "water_temperature", "snow_temperature",
-                                   "ice_temperature")]
+                                   "ice_temperature", "wind_speed", "wind_direction", "salinity")]
 
 
 class CGeometry(ctypes.Structure):
     _fields_ = [(n, _vp) for n in ("sensor_zenith_angle", "sensor_scan_angle", "source_zenith_angle",
-                                   "surface_altitude")]
+                                   "surface_altitude", "sensor_azimuth_angle")]
 
 
 class COptions(ctypes.Structure):
@@ -73,7 +73,7 @@ class COptions(ctypes.Structure):
 class CRTSolution(ctypes.Structure):
     _fields_ = [(n, _vp) for n in ("radiance", "brightness_temperature", "up_radiance", "down_radiance",
                                    "surface_planck_radiance", "surface_emissivity", "status", "r_clear",
-                                   "tb_clear", "total_cloud_cover")]
+                                   "tb_clear", "total_cloud_cover", "surface_reflectivity")]
 
 
 class CKOutput(ctypes.Structure):
@@ -81,7 +81,8 @@ class CKOutput(ctypes.Structure):
                                    "temperature_k", "absorber_k", "land_temperature_k",
                                    "water_temperature_k", "snow_temperature_k", "ice_temperature_k",
                                    "cloud_water_content_k", "cloud_effective_radius_k", "cloud_fraction_k",
-                                   "aerosol_concentration_k", "aerosol_effective_radius_k")]
+                                   "aerosol_concentration_k", "aerosol_effective_radius_k", "salinity_k",
+                                   "wind_speed_k", "wind_direction_k")]
 
 
 # adjoint outputs laid out per layer ([NT] or [n, NT]); the others are per profile
@@ -94,7 +95,8 @@ class CADOutput(ctypes.Structure):
                                    "temperature_ad", "absorber_ad", "land_temperature_ad",
                                    "water_temperature_ad", "snow_temperature_ad", "ice_temperature_ad",
                                    "cloud_water_content_ad", "cloud_effective_radius_ad", "cloud_fraction_ad",
-                                   "aerosol_concentration_ad", "aerosol_effective_radius_ad")]
+                                   "aerosol_concentration_ad", "aerosol_effective_radius_ad", "salinity_ad",
+                                   "wind_speed_ad", "wind_direction_ad")]
 
 
 _lib = None
@@ -268,12 +270,18 @@ class Context:
         for f in ("land_coverage", "water_coverage", "snow_coverage", "ice_coverage", "land_temperature",
                   "water_temperature", "snow_temperature", "ice_temperature"):
             setattr(sfc, f, _ptr(arr(getattr(prof, f), np.float64)))
+        for f in ("wind_speed", "wind_direction", "salinity"):   # optional (reference defaults)
+            v = getattr(prof, f, None)
+            setattr(sfc, f, _ptr(arr(v, np.float64)) if v is not None else None)
         geo = CGeometry()
         geo.sensor_zenith_angle = _ptr(arr(prof.sensor_zenith, np.float64))
         geo.sensor_scan_angle = _ptr(arr(prof.sensor_scan, np.float64))
         geo.source_zenith_angle = _ptr(arr(prof.source_zenith, np.float64))
+        az = getattr(prof, "sensor_azimuth", None)
+        geo.sensor_azimuth_angle = _ptr(arr(az, np.float64)) if az is not None else None
         opt = COptions()
-        opt.use_emissivity = 1
+        # emissivity None: Options%Use_Emissivity = .FALSE., the surface optics are computed
+        opt.use_emissivity = 0 if emissivity is None else 1
         opt.emissivity = _ptr(arr(emissivity, np.float64))
         opt.precision = precision
         opt.memory = memory
@@ -334,7 +342,8 @@ class Context:
         out = {k: self._alloc((M, L), np.float64, memory, prof.temperature)
                for k in ("radiance", "brightness_temperature")}
         if extras:
-            for k in ("up_radiance", "down_radiance", "surface_planck_radiance", "surface_emissivity"):
+            for k in ("up_radiance", "down_radiance", "surface_planck_radiance", "surface_emissivity",
+                      "surface_reflectivity"):
                 out[k] = self._alloc((M, L), np.float64, memory, prof.temperature)
             if atm.n_clouds > 0:
                 for k in ("r_clear", "tb_clear", "total_cloud_cover"):
@@ -389,6 +398,9 @@ class Context:
                   "absorber_k": self._alloc((na, NT, L), kdt, memory, like)}
             for f in ("land_temperature_k", "water_temperature_k", "snow_temperature_k", "ice_temperature_k"):
                 kd[f] = self._alloc((M, L), np.float64, memory, like)
+            if emissivity is None:   # computed surface optics: their input sensitivities
+                for f in ("salinity_k", "wind_speed_k", "wind_direction_k"):
+                    kd[f] = self._alloc((M, L), np.float64, memory, like)
             if nae > 0:
                 kd["aerosol_concentration_k"] = self._alloc((nae, NT, L), kdt, memory, like)
                 kd["aerosol_effective_radius_k"] = self._alloc((nae, NT, L), kdt, memory, like)
@@ -449,6 +461,9 @@ class Context:
               "surface_emissivity_ad": self._alloc((M, L), np.float64, memory, like)}
         for f in ("land", "water", "snow", "ice"):
             ad[f + "_temperature_ad"] = self._alloc((M,), np.float64, memory, like)
+        if emissivity is None:
+            for f in ("salinity_ad", "wind_speed_ad", "wind_direction_ad"):
+                ad[f] = self._alloc((M,), np.float64, memory, like)
         if nc:
             ad["cloud_water_content_ad"] = self._alloc((nc, NT), np.float64, memory, like)
             ad["cloud_effective_radius_ad"] = self._alloc((nc, NT), np.float64, memory, like)
@@ -476,7 +491,8 @@ class Context:
             ns = n_streams[m0:m1] if (per_profile_streams and not whole) else n_streams
             sk = skip_profile if (skip_profile is None or whole) else skip_profile[m0:m1]
             part = {k: (v[..., a:b] if k in _LAYER_AD else v[m0:m1]) for k, v in ad.items()}
-            self._adjoint_call(sub, emissivity if whole else emissivity[m0:m1], bt_ad if whole else bt_ad[m0:m1],
+            self._adjoint_call(sub, emissivity if (whole or emissivity is None) else emissivity[m0:m1],
+                               bt_ad if whole else bt_ad[m0:m1],
                                precision, ns, overlap, include_scattering, sk, bt[m0:m1], part, m0)
             if progress:
                 progress(f"adjoint profiles {m0}-{m1} of {M} done")
@@ -489,10 +505,9 @@ class Context:
         if memory == MEM_DEVICE:
             self._use_torch_stream()
         keep = []
-        if memory == MEM_DEVICE:
-            emissivity, bt_ad = emissivity.contiguous(), bt_ad.contiguous()
-        else:
-            emissivity, bt_ad = np.ascontiguousarray(emissivity, np.float64), np.ascontiguousarray(bt_ad, np.float64)
+        if emissivity is not None:
+            emissivity = emissivity.contiguous() if memory == MEM_DEVICE else np.ascontiguousarray(emissivity, np.float64)
+        bt_ad = bt_ad.contiguous() if memory == MEM_DEVICE else np.ascontiguousarray(bt_ad, np.float64)
         atm, sfc, geo, opt = self._structs(prof, emissivity, precision, memory, False, False, keep, n_streams,
                                            overlap, include_scattering, skip_profile)
         M, L = prof.n_profiles, self.n_channels

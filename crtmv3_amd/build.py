@@ -18,8 +18,9 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_obj")
 LIB = os.path.join(HERE, "libcrtm_mi355x.so")
-SOURCES = ["odps_prep.hip", "clear_fwdk.hip", "allsky.hip", "allsky_ad.hip", "allsky_wide.hip", "allsky_wide_ad.hip", "crtm_abi.cpp"]
-HEADERS = ["crtm_device.h", "allsky_common.h", "allsky_wide.h", os.path.join("..", "..", "include", "crtm_mi355x.h")]
+SOURCES = ["odps_prep.hip", "clear_fwdk.hip", "sfc_optics.hip", "allsky.hip", "allsky_ad.hip", "allsky_wide.hip",
+           "allsky_wide_ad.hip", "crtm_abi.cpp"]
+HEADERS = ["crtm_device.h", "sfc_optics.h", "allsky_common.h", "allsky_wide.h", os.path.join("..", "..", "include", "crtm_mi355x.h")]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("CRTM_OFFLOAD_ARCH", "gfx950")

@@ -173,6 +173,7 @@ clear_fwdk_kernel(DevSensor s, DevBatch bt, ProfWork<RT> w, DevOut o) {
       if (o.down_radiance) o.down_radiance[ol] = 0.0;
       if (o.sfc_planck) o.sfc_planck[ol] = 0.0;
       if (o.sfc_emis) o.sfc_emis[ol] = 0.0;
+      if (o.sfc_refl) o.sfc_refl[ol] = 0.0;
     }
     if constexpr (RED) {   // a skipped profile adds nothing; a failed one's rows become NaN
       if (!skipped && lane == 0 && b == 0)
@@ -193,6 +194,8 @@ clear_fwdk_kernel(DevSensor s, DevBatch bt, ProfWork<RT> w, DevOut o) {
         if (o.emis_k) o.emis_k[ol] = 0.0;
         for (int c = 0; c < 4; ++c)
           if (o.sfc_t_k[c]) o.sfc_t_k[c][ol] = 0.0;
+        for (int j = 0; j < 3; ++j)
+          if (o.sfc_k[j]) o.sfc_k[j][ol] = 0.0;
       }
     }
   }
@@ -330,9 +333,11 @@ clear_fwdk_kernel(DevSensor s, DevBatch bt, ProfWork<RT> w, DevOut o) {
   RT upp = up0;
   // all-sky calls keep the gas OD of cloudy profiles for the scattering RT
   RT* tau_out = (o.tau_g && valid && o.tau_mask[m] > 0) ? reinterpret_cast<RT*>(o.tau_g) + loff * s.L + l : nullptr;
+  double odsum = 0.0;   // column nadir optical depth (CRTM_Compute_Transmittance)
   for (int k = 0; k < K; ++k) {
     const RT up = upath(k + 1);
     const RT tau = (up - upp) * isec;
+    odsum += (double)tau;
     if (tau_out) tau_out[(int64_t)k * s.L] = tau;
     upp = up;
     const RT t = rexp<RT>(-tau * iu);
@@ -344,8 +349,20 @@ clear_fwdk_kernel(DevSensor s, DevBatch bt, ProfWork<RT> w, DevOut o) {
   }
   RT tes;
   const RT Bs = planck(c1, c2, bc1, bc2, tsfc, tes);
-  const double eps_d = bt.emissivity[(size_t)m * bt.Ltot + lg];
-  const RT eps = (RT)eps_d, rho = RT(1) - eps;
+  const int64_t sl = (int64_t)m * s.L + lc;
+  // surface emissivity and reflectivity: the user's (reflectivity 1 - e), or
+  // the computed optics with the reflection correction at the column
+  // transmittance (sfc_optics.h)
+  const bool sfc_comp = o.sp.mode != SFC_USER;
+  double eps_d, rho_d = 0.0, trans = 1.0;
+  SfcLin zl;
+  if (!sfc_comp) {
+    eps_d = bt.emissivity[(size_t)m * bt.Ltot + lg];
+  } else {
+    trans = exp(-odsum);
+    sfc_forward(o.sp, sl, trans, eps_d, rho_d, zl);
+  }
+  const RT eps = (RT)eps_d, rho = sfc_comp ? (RT)rho_d : RT(1) - eps;
   const RT sfc_src = eps * Bs + rho * rd;
   const double R = (double)(uacc + tc * sfc_src);
   // CRTM_Planck_Temperature in FP64 (CRTM_Planck_Functions.f90:470-492)
@@ -354,7 +371,6 @@ clear_fwdk_kernel(DevSensor s, DevBatch bt, ProfWork<RT> w, DevOut o) {
   // profiles with cloudy layers in an all-sky K-matrix call: the scattering
   // path has written the outputs and hands over adjoint seeds
   const bool seeded = DOK && o.ks.mask && ok_prof && o.ks.mask[m] > 0;   // wave-uniform
-  const int64_t sl = (int64_t)m * s.L + lc;
   if (valid && !seeded) {
     o.radiance[ol] = R;
     o.bt[ol] = bt_v;
@@ -362,6 +378,7 @@ clear_fwdk_kernel(DevSensor s, DevBatch bt, ProfWork<RT> w, DevOut o) {
     if (o.down_radiance) o.down_radiance[ol] = (double)rd;
     if (o.sfc_planck) o.sfc_planck[ol] = (double)Bs;
     if (o.sfc_emis) o.sfc_emis[ol] = eps_d;
+    if (o.sfc_refl) o.sfc_refl[ol] = sfc_comp ? rho_d : (double)rho;
   }
   if constexpr (!DOK) return;
 
@@ -375,6 +392,17 @@ clear_fwdk_kernel(DevSensor s, DevBatch bt, ProfWork<RT> w, DevOut o) {
   if (valid && o.rad_k && !seeded) o.rad_k[ol] = rad_ad_d;
   const RT rad_ad = (RT)rad_ad_d;
   const RT upad_sfc = rad_ad * tc;           // adjoint of the surface-leaving radiance
+  // computed optics: CRTM_Compute_SfcOptics_AD from the emissivity and
+  // reflectivity adjoints (sg: water temperature, salinity, wind speed, wind
+  // direction), and CRTM_Compute_Transmittance_AD: every layer's optical
+  // depth adjoint takes -T * T_AD (tau_sfc_ad, added in the sweep below)
+  double sg[4] = {0.0, 0.0, 0.0, 0.0};
+  RT tau_sfc_ad = RT(0);
+  if (sfc_comp && valid) {
+    double tr_ad;
+    sfc_adjoint(o.sp, sl, zl, (double)(upad_sfc * Bs), (double)(upad_sfc * rd), tr_ad, sg);
+    tau_sfc_ad = (RT)(-trans * tr_ad);
+  }
   if constexpr (RED) {   // surface temperature adjoints [M]: channel sums
     if (ok_prof) {
       const double bs_seed = (seeded && valid) ? o.ks.bs[sl] : 0.0;
@@ -383,18 +411,31 @@ clear_fwdk_kernel(DevSensor s, DevBatch bt, ProfWork<RT> w, DevOut o) {
       if (lane == 0)
         for (int c = 0; c < 4; ++c)
           if (o.sfc_t_k[c]) unsafeAtomicAdd(o.sfc_t_k[c] + m, bt.cov[c][m] * sad);
+      if (sfc_comp) {   // wave-uniform
+        const double s0 = wave_sum(sg[0]);
+        double sj[3];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) sj[j] = wave_sum(sg[1 + j]);
+        if (lane == 0) {
+          if (o.sfc_t_k[1]) unsafeAtomicAdd(o.sfc_t_k[1] + m, s0);
+          for (int j = 0; j < 3; ++j)
+            if (o.sfc_k[j]) unsafeAtomicAdd(o.sfc_k[j] + m, sj[j]);
+        }
+      }
     }
     if (valid && o.emis_k) {
       const double e_seed = seeded ? o.ks.emis[sl] : 0.0;
-      o.emis_k[ol] = (double)(upad_sfc * Bs - upad_sfc * rd) + e_seed;
+      o.emis_k[ol] = sfc_comp ? 0.0 : (double)(upad_sfc * Bs - upad_sfc * rd) + e_seed;
     }
   } else if (valid) {
     const double e_seed = seeded ? o.ks.emis[sl] : 0.0, bs_seed = seeded ? o.ks.bs[sl] : 0.0;
-    if (o.emis_k) o.emis_k[ol] = (double)(upad_sfc * Bs - upad_sfc * rd) + e_seed;
+    if (o.emis_k) o.emis_k[ol] = sfc_comp ? 0.0 : (double)(upad_sfc * Bs - upad_sfc * rd) + e_seed;
     const RT dbsdt = c2 * bc2 * Bs * (Bs + c1) * ic1 / (tes * tes);
     const double sad = (double)(dbsdt * (eps * upad_sfc + (RT)bs_seed));
     for (int c = 0; c < 4; ++c)
-      if (o.sfc_t_k[c]) o.sfc_t_k[c][ol] = bt.cov[c][m] * sad;
+      if (o.sfc_t_k[c]) o.sfc_t_k[c][ol] = bt.cov[c][m] * sad + (c == 1 ? sg[0] : 0.0);
+    for (int j = 0; j < 3; ++j)
+      if (o.sfc_k[j]) o.sfc_k[j][ol] = sg[1 + j];
   }
 
   // ------------- adjoint sweep, bottom-up: user levels interleaved with ODPS layers -------------
@@ -454,7 +495,7 @@ clear_fwdk_kernel(DevSensor s, DevBatch bt, ProfWork<RT> w, DevOut o) {
       const RT a_up = t * (ru - B) * upad;
       const RT a_dn = gd - B * dn;
       RT b_ad = (upad + dn) * (RT(1) - t);
-      RT tau_ad = -(a_up + a_dn) * iu;
+      RT tau_ad = -(a_up + a_dn) * iu + tau_sfc_ad;
       RT t_seed = RT(0);
       if (seeded) {
         const int64_t is = (loff + k) * s.L + lc;

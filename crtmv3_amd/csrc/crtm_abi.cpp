@@ -46,6 +46,8 @@ int64_t wide_doubling_tape_elems(int64_t n_items);
 void launch_aerosol_check(const DevSensor&, int M, const DevClouds&, const CloudWork&, const DevCloudCoeff&,
                           hipStream_t);
 void launch_status_merge(int M, const int* prep_status, const int* aer_err, int* call_status, hipStream_t st);
+void launch_sfc_optics(const DevSensor& s, const DevBatch& b, const DevSfcIn& in, int mode, const DevMWwater& mw,
+                       const DevIRwater& ir, double* planes, int64_t stride, int* err, hipStream_t st);
 void launch_zero(void* p, size_t bytes, hipStream_t st);
 }  // namespace crtm
 
@@ -146,8 +148,8 @@ class Cursor {
 struct SpcCoeffH {
   std::string sensor_id;
   int sensor_type = 0, n_channels = 0;
-  std::vector<int> sensor_channel;
-  std::vector<double> frequency, wavenumber, c1, c2, bc1, bc2, cosmic, solar;
+  std::vector<int> sensor_channel, polarization;
+  std::vector<double> pol_angle, frequency, wavenumber, c1, c2, bc1, bc2, cosmic, solar;
 };
 
 // SpcCoeff_Binary_IO.f90:374-470
@@ -171,8 +173,8 @@ SpcCoeffH read_spccoeff(const std::string& path) {
   if (sc.sensor_type > 100) sc.sensor_type -= 100;   // ACTIVE_SENSOR offset
   Cursor r3(rec.next());
   sc.sensor_channel = r3.arr<int32_t>(n);
-  r3.arr<int32_t>(n);                    // polarization
-  if (ver > 2) r3.arr<double>(n);        // PolAngle
+  sc.polarization = r3.arr<int32_t>(n);
+  sc.pol_angle = ver > 2 ? r3.arr<double>(n) : std::vector<double>(n, 0.0);   // PolAngle from version 3
   r3.arr<int32_t>(n);                    // channel flag
   sc.frequency = r3.arr<double>(n);
   sc.wavenumber = r3.arr<double>(n);
@@ -549,6 +551,107 @@ void build_sensor(Sensor& S, int chan_offset) {
   d.frequency = S.put(sc.frequency);
   d.wavenumber = S.put(sc.wavenumber);
   d.sensor_type = sc.sensor_type;
+  d.polarization = S.put(sc.polarization);
+  d.pol_angle = S.put(sc.pol_angle);
+}
+
+// Global attributes of the binary coefficient files: per attribute a length
+// record and, for a non-zero length, the text (ReadGAtts_Binary_File,
+// src/Utility/Binary_File_Utility.f90)
+void skip_gatts(Records& rec) {
+  for (int i = 0; i < 5; ++i) {
+    Cursor c(rec.next());
+    const int32_t n = c.get<int32_t>();
+    c.done("global attribute length");
+    if (n > 0) rec.next();
+  }
+}
+
+// FASTEM coefficient file (MWwaterCoeff_ReadFile, src/Coefficients/EmisCoeff/
+// MW_Water/MWwaterCoeff_Define.f90:699-922): release/version, attributes, the
+// FC, FR, RC, AZ, SSC, LSC FitCoeff blocks (FitCoeff_ReadFile.inc), LUT flag
+struct MWwaterH {
+  int version = 0;
+  std::vector<double> fit[6];
+};
+MWwaterH read_mwwater(const std::string& path) {
+  Records rec(path);
+  MWwaterH h;
+  Cursor r0(rec.next());
+  const int rel = r0.get<int32_t>();
+  h.version = r0.get<int32_t>();
+  r0.done("MWwaterCoeff release");
+  if (rel != 1) throw CrtmError(fmt("MWwaterCoeff Release check failed for %s (release %d)", path.c_str(), rel));
+  skip_gatts(rec);
+  // expected shapes: fc >= 2, fr >= 6, rc (3,7,2), az (6,6,2), ssc >= 8, lsc (3,6,2)
+  static const int rank[6] = {1, 1, 3, 3, 1, 3};
+  static const int need[6][3] = {{2, 0, 0}, {6, 0, 0}, {3, 7, 2}, {6, 6, 2}, {8, 0, 0}, {3, 6, 2}};
+  for (int b = 0; b < 6; ++b) {
+    Cursor c0(rec.next());
+    const int frel = c0.get<int32_t>();
+    c0.get<int32_t>();
+    c0.done("FitCoeff release");
+    if (frel != 1) throw CrtmError(fmt("FitCoeff Release check failed in %s", path.c_str()));
+    Cursor c1(rec.next());
+    const int nd = c1.get<int32_t>();
+    c1.done("FitCoeff rank");
+    if (nd != rank[b]) throw CrtmError(fmt("%s: FitCoeff block %d has rank %d", path.c_str(), b + 1, nd));
+    Cursor c2(rec.next());
+    std::vector<int32_t> dims = c2.arr<int32_t>(nd);
+    c2.done("FitCoeff dimensions");
+    size_t n = 1;
+    for (int k = 0; k < nd; ++k) {
+      n *= (size_t)dims[k];
+      if ((nd == 1 && dims[k] < need[b][0]) || (nd == 3 && dims[k] != need[b][k]))
+        throw CrtmError(fmt("%s: FitCoeff block %d has unexpected dimensions", path.c_str(), b + 1));
+    }
+    skip_gatts(rec);
+    Cursor c3(rec.next());
+    h.fit[b] = c3.arr<double>(n);
+    c3.done("FitCoeff coefficients");
+  }
+  Cursor r9(rec.next());
+  if (r9.get<int32_t>() != 0) throw CrtmError(path + ": FASTEM look-up tables are not supported");
+  if (h.version != 6) throw CrtmError(fmt("%s: MWwaterCoeff version %d; the FASTEM6 model (version 6) is supported",
+                                          path.c_str(), h.version));
+  return h;
+}
+
+// IR sea-surface emissivity table (IRwaterCoeff_ReadFile,
+// src/Coefficients/EmisCoeff/IR_Water/IRwaterCoeff_Define.f90:805-984)
+struct IRwaterH {
+  int na = 0, nf = 0, nv = 0, nt = 0;
+  std::string name;
+  std::vector<double> angle, freq, wind, temp, emis;
+};
+IRwaterH read_irwater(const std::string& path) {
+  Records rec(path);
+  IRwaterH h;
+  Cursor r0(rec.next());
+  const int rel = r0.get<int32_t>();
+  r0.get<int32_t>();
+  r0.done("IRwaterCoeff release");
+  if (rel != 4) throw CrtmError(fmt("IRwaterCoeff Release check failed for %s (release %d)", path.c_str(), rel));
+  Cursor r1(rec.next());
+  h.na = r1.get<int32_t>(); h.nf = r1.get<int32_t>(); h.nv = r1.get<int32_t>(); h.nt = r1.get<int32_t>();
+  r1.done("IRwaterCoeff dimensions");
+  if (h.na < 4 || h.nf < 4 || h.nv < 4 || h.nt < 1)
+    throw CrtmError(path + ": the IR emissivity table needs at least 4 points per interpolated axis");
+  skip_gatts(rec);
+  Cursor r2(rec.next());
+  h.name = r2.str(80);
+  r2.done("IRwaterCoeff classification");
+  if (h.name == "Nalli2" && h.nt < 4) throw CrtmError(path + ": a Nalli2 table needs at least 4 temperatures");
+  Cursor r3(rec.next());
+  h.angle = r3.arr<double>(h.na);
+  h.freq = r3.arr<double>(h.nf);
+  h.wind = r3.arr<double>(h.nv);
+  h.temp = r3.arr<double>(h.nt);
+  r3.done("IRwaterCoeff dimension vectors");
+  Cursor r4(rec.next());
+  h.emis = r4.arr<double>((size_t)h.na * h.nf * h.nv * h.nt);
+  r4.done("IRwaterCoeff emissivity");
+  return h;
 }
 
 // CloudCoeff release 3 (CloudCoeff_Binary_IO.f90:367-446): dimensions, axes,
@@ -761,6 +864,13 @@ struct crtm_context {
   // device copy of per-profile stream counts given in host memory to a device-memory call
   DevBuf nsprof;
   DevBuf skipbuf;      // Skip_Profile flags staged from host memory in device-memory calls
+  // surface emissivity models (loaded at init when their files are present)
+  // and the per-call optics planes / unsupported-surface flags
+  bool mw_loaded = false, ir_loaded = false;
+  DevMWwater mw{};
+  DevIRwater ir{};
+  std::vector<std::unique_ptr<DevBuf>> sfc_bufs;
+  DevBuf sfcplanes, sfcerr;
   // kernel timing (crtm_profile_enable): HIP events on the context stream
   bool profiling = false;
   std::vector<hipEvent_t> ev;      // pairs (start, stop) per recorded launch
@@ -1155,6 +1265,15 @@ void run_sensors(crtm_context* c, const DevBatch& be, DevOut o, const CloudCall&
   c->call_status = static_cast<int*>(take(c, c->cstatus, (size_t)be.M * sizeof(int)));
   launch_zero(c->call_status, (size_t)be.M * sizeof(int), c->stream);
   const int* aer_err = cc.on ? cc.cw.aer_err : nullptr;
+  // computed surface optics (clear-sky calls only): the optics kernels flag
+  // profiles with a surface other than water, which then fail
+  const bool sfc_comp = o.sp.mode != SFC_USER;
+  int* sfc_err = nullptr;
+  if (sfc_comp) {
+    sfc_err = static_cast<int*>(take(c, c->sfcerr, (size_t)be.M * sizeof(int)));
+    launch_zero(sfc_err, (size_t)be.M * sizeof(int), c->stream);
+    aer_err = sfc_err;
+  }
   o.fail = aer_err;
   auto merge = [&] {
     launch_status_merge(be.M, c->prep_status, aer_err, c->call_status, c->stream);
@@ -1174,6 +1293,20 @@ void run_sensors(crtm_context* c, const DevBatch& be, DevOut o, const CloudCall&
       c->ctau.reserve(std::max<size_t>((size_t)be.NT * S->d.L * sizeof(RT), 16));
       o.tau_g = c->ctau.p;
       o.tau_mask = cc.cw.cl_count;
+    }
+    if (sfc_comp) {
+      const int mode = S->d.sensor_type == CRTM_MICROWAVE_SENSOR ? SFC_MW : SFC_IR;
+      const int64_t stride = (int64_t)be.M * S->d.L;
+      double* planes = static_cast<double*>(
+          take(c, c->sfcplanes, (size_t)stride * (mode == SFC_MW ? MP_N : IP_N) * sizeof(double)));
+      if (c->profiling) {
+        c->mark();
+        c->ev_kind.push_back(0);
+      }
+      launch_sfc_optics(S->d, be, be.sfc_in, mode, c->mw, c->ir, planes, stride, sfc_err, c->stream);
+      if (c->profiling) c->mark();
+      HIPCHECK(hipGetLastError());
+      o.sp = SfcPlanes{mode, planes, stride};
     }
     ProfWork<RT> w = run_sensor<RT, KT, DOK, RED>(c, *S, be, o);
     if (cc.on) run_allsky<RT>(c, *S, be, cc.cl, cc.cw, cc.total, w, o, cc.ao);
@@ -1240,8 +1373,33 @@ int do_call(crtm_context* ctx, const crtm_atmosphere* atm, const crtm_surface* s
             const crtm_options* opt, crtm_rtsolution* rts, crtm_k_output* k, bool adj = false) {
   if (!ctx) return fail(nullptr, "null context");
   if (!atm || !sfc || !geo || !opt || !rts) return fail(ctx, "null argument");
-  if (!opt->use_emissivity || !opt->emissivity)
-    return fail(ctx, "surface optics models are not part of this library: set Options%Use_Emissivity");
+  // Options%Use_Emissivity = .FALSE.: the surface optics are computed
+  // (water surfaces, clear-sky calls)
+  const bool sfc_comp = !opt->use_emissivity;
+  if (!sfc_comp && !opt->emissivity) return fail(ctx, "Options%Emissivity required with Use_Emissivity");
+  if (sfc_comp) {
+    if (atm->n_clouds > 0 || atm->n_aerosols > 0)
+      return fail(ctx, "computed surface optics are available for clear-sky calls; give the emissivity "
+                       "(Options%Use_Emissivity) for calls with clouds or aerosols");
+    for (auto& S : ctx->sensors) {
+      const int st = S->sc.sensor_type;
+      if (st == CRTM_MICROWAVE_SENSOR) {
+        if (!ctx->mw_loaded)
+          return fail(ctx, "The MW water emissivity coefficients must be loaded (FASTEM6.MWwater.EmisCoeff.bin in "
+                           "the crtm_init path) to compute microwave surface optics");
+        for (int l = 0; l < S->sc.n_channels; ++l)
+          if (S->sc.polarization[l] < 1 || S->sc.polarization[l] > 14)
+            return fail(ctx, fmt("Unrecognised polarization flag for microwave channel index %d", l + 1));
+      } else if (st == CRTM_INFRARED_SENSOR) {
+        if (!ctx->ir_loaded)
+          return fail(ctx, "The IR water emissivity coefficients must be loaded (Nalli.IRwater.EmisCoeff.bin in "
+                           "the crtm_init path) to compute infrared surface optics");
+      } else {
+        return fail(ctx, fmt("%s: computed surface optics cover microwave and infrared sensors",
+                             S->sc.sensor_id.c_str()));
+      }
+    }
+  }
   const int M = atm->n_profiles;
   if (M <= 0) return CRTM_SUCCESS;
   if (atm->n_absorbers < 1 || atm->n_absorbers > NA_MAX)
@@ -1358,6 +1516,13 @@ int do_call(crtm_context* ctx, const crtm_atmosphere* atm, const crtm_surface* s
           return fail(ctx, fmt("Input data check failed for profile #%d", m + 1));
     }
   }
+  if (sfc_comp && host)
+    for (int m = 0; m < M; ++m) {
+      if (opt->skip_profile && opt->skip_profile[m]) continue;
+      if (sfc->land_coverage[m] > 0.0 || sfc->snow_coverage[m] > 0.0 || sfc->ice_coverage[m] > 0.0)
+        return fail(ctx, fmt("profile #%d: computed surface optics cover water surfaces (land, snow and ice "
+                             "coverage must be zero)", m + 1));
+    }
   const size_t szK = kf32 ? 4 : 8;
   if (rts->status && host)
     for (int m = 0; m < M; ++m) rts->status[m] = CRTM_SUCCESS;
@@ -1370,6 +1535,7 @@ int do_call(crtm_context* ctx, const crtm_atmosphere* atm, const crtm_surface* s
   for (int j = 0; j < NA_MAX; ++j) b.jmap[j] = j < na ? abs_id[j] : -1;
   DevOut o{};
   o.k_f32 = kf32;
+  o.sp.mode = sfc_comp ? SFC_MW : SFC_USER;   // per sensor in run_sensors
   const bool dok = k != nullptr;
   if (host) {
     // stage inputs
@@ -1412,7 +1578,14 @@ int do_call(crtm_context* ctx, const crtm_atmosphere* atm, const crtm_surface* s
     }
     in(geo->sensor_zenith_angle, M * sizeof(double), (const void**)&b.zenith);
     in(geo->surface_altitude, M * sizeof(double), (const void**)&b.altitude);
-    in(opt->emissivity, (size_t)M * L * sizeof(double), (const void**)&b.emissivity);
+    in(sfc_comp ? nullptr : opt->emissivity, (size_t)M * L * sizeof(double), (const void**)&b.emissivity);
+    if (sfc_comp) {
+      in(sfc->salinity, M * sizeof(double), (const void**)&b.sfc_in.salinity);
+      in(sfc->wind_speed, M * sizeof(double), (const void**)&b.sfc_in.wind_speed);
+      in(sfc->wind_direction, M * sizeof(double), (const void**)&b.sfc_in.wind_direction);
+      in(geo->sensor_azimuth_angle, M * sizeof(double), (const void**)&b.sfc_in.azimuth);
+      in(geo->sensor_scan_angle, M * sizeof(double), (const void**)&b.sfc_in.scan);
+    }
     if (dok) {
       in(k->brightness_temperature_k, (size_t)M * L * sizeof(double), (const void**)&o.bt_k);
       in(k->radiance_k, (size_t)M * L * sizeof(double), (const void**)&o.rad_k);
@@ -1432,6 +1605,7 @@ int do_call(crtm_context* ctx, const crtm_atmosphere* atm, const crtm_surface* s
     out(rts->down_radiance, ML, (void**)&o.down_radiance);
     out(rts->surface_planck_radiance, ML, (void**)&o.sfc_planck);
     out(rts->surface_emissivity, ML, (void**)&o.sfc_emis);
+    out(rts->surface_reflectivity, ML, (void**)&o.sfc_refl);
     out(rts->r_clear, ML, (void**)&cc.ao.r_clear);
     out(rts->tb_clear, ML, (void**)&cc.ao.tb_clear);
     out(rts->total_cloud_cover, ML, (void**)&cc.ao.tcc);
@@ -1440,6 +1614,8 @@ int do_call(crtm_context* ctx, const crtm_atmosphere* atm, const crtm_surface* s
       out(k->surface_emissivity_k, ML, (void**)&o.emis_k);
       double* stk[4] = {k->land_temperature_k, k->water_temperature_k, k->snow_temperature_k, k->ice_temperature_k};
       for (int c = 0; c < 4; ++c) out(stk[c], adj ? M * sizeof(double) : ML, (void**)&o.sfc_t_k[c], adj);
+      double* sk3[3] = {k->salinity_k, k->wind_speed_k, k->wind_direction_k};
+      for (int j = 0; j < 3; ++j) out(sk3[j], adj ? M * sizeof(double) : ML, (void**)&o.sfc_k[j], adj);
       out(k->temperature_k, (size_t)NT * rowsL * szO, (void**)&o.t_k, adj);
       out(k->absorber_k, (size_t)na * NT * rowsL * szO, (void**)&o.a_k, adj);
       if (cc.on) {
@@ -1535,14 +1711,25 @@ int do_call(crtm_context* ctx, const crtm_atmosphere* atm, const crtm_surface* s
     if (!b.cov[c] || !b.sfc_t[c]) return fail(ctx, "surface coverage/temperature arrays required");
   b.zenith = geo->sensor_zenith_angle;
   b.altitude = geo->surface_altitude;
-  b.emissivity = opt->emissivity;
+  b.emissivity = sfc_comp ? nullptr : opt->emissivity;
+  if (sfc_comp) {
+    b.sfc_in.salinity = sfc->salinity;
+    b.sfc_in.wind_speed = sfc->wind_speed;
+    b.sfc_in.wind_direction = sfc->wind_direction;
+    b.sfc_in.azimuth = geo->sensor_azimuth_angle;
+    b.sfc_in.scan = geo->sensor_scan_angle;
+  }
   o.radiance = rts->radiance;
   o.bt = rts->brightness_temperature;
   o.up_radiance = rts->up_radiance;
   o.down_radiance = rts->down_radiance;
   o.sfc_planck = rts->surface_planck_radiance;
   o.sfc_emis = rts->surface_emissivity;
+  o.sfc_refl = rts->surface_reflectivity;
   if (dok) {
+    o.sfc_k[0] = k->salinity_k;
+    o.sfc_k[1] = k->wind_speed_k;
+    o.sfc_k[2] = k->wind_direction_k;
     o.bt_k = k->brightness_temperature_k;
     o.rad_k = k->radiance_k;
     o.emis_k = k->surface_emissivity_k;
@@ -1641,6 +1828,31 @@ int crtm_init(int n_sensors, const char* const* sensor_ids, const char* file_pat
       upload(*ctx->cloud_bufs.back(), gauss_table());
       ctx->cc.gauss = ctx->cloud_bufs.back()->as<double>();
     }
+    // surface emissivity models under the reference's default file names
+    // (src/CRTM_LifeCycle.f90:682, 691)
+    auto putd = [&](const std::vector<double>& v) {
+      ctx->sfc_bufs.emplace_back(new DevBuf);
+      upload(*ctx->sfc_bufs.back(), v);
+      return ctx->sfc_bufs.back()->as<double>();
+    };
+    const std::string mwpath = dir + "FASTEM6.MWwater.EmisCoeff.bin";
+    if (std::ifstream(mwpath).good()) {
+      const MWwaterH h = read_mwwater(mwpath);
+      DevMWwater& c = ctx->mw;
+      c.fc = putd(h.fit[0]); c.fr = putd(h.fit[1]); c.rc = putd(h.fit[2]);
+      c.az = putd(h.fit[3]); c.ssc = putd(h.fit[4]); c.lsc = putd(h.fit[5]);
+      ctx->mw_loaded = true;
+    }
+    const std::string irpath = dir + "Nalli.IRwater.EmisCoeff.bin";
+    if (std::ifstream(irpath).good()) {
+      const IRwaterH h = read_irwater(irpath);
+      DevIRwater& c = ctx->ir;
+      c.na = h.na; c.nf = h.nf; c.nv = h.nv; c.nt = h.nt;
+      c.has_t = h.name == "Nalli2";
+      c.angle = putd(h.angle); c.frequency = putd(h.freq); c.wind = putd(h.wind); c.temp = putd(h.temp);
+      c.emis = putd(h.emis);
+      ctx->ir_loaded = true;
+    }
     const std::string acpath = dir + "AerosolCoeff.bin";
     if (std::ifstream(acpath).good()) {
       const AerosolCoeffH h = read_aerosolcoeff(acpath);
@@ -1723,6 +1935,9 @@ int crtm_adjoint(crtm_context* ctx, const crtm_atmosphere* atm, const crtm_surfa
   k.brightness_temperature_k = ad->brightness_temperature_ad;
   k.radiance_k = ad->radiance_ad;
   k.surface_emissivity_k = ad->surface_emissivity_ad;
+  k.salinity_k = ad->salinity_ad;
+  k.wind_speed_k = ad->wind_speed_ad;
+  k.wind_direction_k = ad->wind_direction_ad;
   k.temperature_k = ad->temperature_ad;
   k.absorber_k = ad->absorber_ad;
   k.land_temperature_k = ad->land_temperature_ad;

@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "sfc_optics.h"
+
 namespace crtm {
 
 constexpr int WAVE = 64;           // CDNA wavefront width: one channel per lane
@@ -131,6 +133,8 @@ struct DevSensor {
   const double* frequency;  // GHz (microwave cloud tables)
   const double* wavenumber; // cm^-1 (infrared cloud tables, stream choice)
   int sensor_type;          // 1 microwave, 2 infrared, ...
+  const int* polarization;  // SpcCoeff%Polarization
+  const double* pol_angle;  // SpcCoeff%PolAngle, degrees
 };
 
 // blk_meta layout
@@ -163,6 +167,7 @@ struct DevBatch {
   const int64_t* out_loff;  // [M+1] caller's layer offsets (Jacobian rows)
   int64_t out_NT;           // caller's total layers (absorber Jacobian stride)
   const int* skip;          // [M] or null: Options(m)%Skip_Profile
+  DevSfcIn sfc_in;          // computed surface optics: salinity, wind, azimuths
 };
 
 // Extended profiles written by the AddLayers kernel: profile m occupies
@@ -249,6 +254,12 @@ struct DevOut {
   KSeeds ks;               // all-sky K-matrix seeds (ks.mask null: none)
   const int* fail;         // [M] or null: nonzero fails the profile (scattering-path
                            // failures found before the clear-sky kernels run)
+  // computed surface optics (sp.mode != SFC_USER): the sensor's planes,
+  // RTSolution%Surface_Reflectivity, and the salinity / wind speed / wind
+  // direction Jacobians ([M][Ltot]; adjoint calls: [M] channel sums)
+  SfcPlanes sp;
+  double* sfc_refl;
+  double* sfc_k[3];
 };
 
 // Cloud Jacobians of the K-matrix, caller's layer layout, element type KT:

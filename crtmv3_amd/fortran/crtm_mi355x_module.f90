@@ -21,10 +21,12 @@
 ! src/RTSolution/CRTM_RTSolution_Define.f90:201-245,
 ! src/Options/CRTM_Options_Define.f90:122-179).  As in the reference,
 ! Atmosphere_K / Surface_K are accumulated into (callers zero them first).
-! Components of the reference types that feed models outside this path
-! (surface optics, NLTE, antenna correction, solar geometry) are present so
-! that callers compile; the path needs Options%Use_Emissivity and ignores
-! them.  Helpers a CRTM caller uses around the calls (CRTM_Version,
+! Without Options%Use_Emissivity the surface optics of water surfaces are
+! computed (FASTEM6, IR sea-surface emissivity table; Surface%Wind_Speed,
+! %Wind_Direction, %Salinity, Geometry%Sensor_Azimuth_Angle), and Surface_K
+! carries their sensitivities.  Components of the reference types that feed
+! models outside this path (land, snow and ice surface models, NLTE,
+! antenna correction, solar geometry) are present so that callers compile.  Helpers a CRTM caller uses around the calls (CRTM_Version,
 ! Display_Message, CRTM_Geometry_SetValue, CRTM_ChannelInfo_n_Channels,
 ! CRTM_*_Associated / _Destroy / _Inspect) are provided with the reference's
 ! names and arguments.
@@ -94,8 +96,10 @@ MODULE CRTM_MI355X_Module
   END TYPE
 
   ! src/Surface/CRTM_Surface_Define.f90:204-244: the coverages and
-  ! temperatures enter this path; the type and state components feed the
-  ! surface-optics models, which this path replaces by Options%Emissivity
+  ! temperatures enter every call; Wind_Speed, Wind_Direction and Salinity
+  ! the computed optics of water surfaces (Options%Use_Emissivity = .FALSE.);
+  ! the land, snow and ice type and state components feed surface models
+  ! this path does not have (those surfaces need Options%Emissivity)
   TYPE, PUBLIC :: CRTM_Surface_type
     REAL(fp) :: Land_Coverage = 0, Water_Coverage = 0, Snow_Coverage = 0, Ice_Coverage = 0
     REAL(fp) :: Land_Temperature = 283, Water_Temperature = 283, Snow_Temperature = 263, Ice_Temperature = 263
@@ -123,6 +127,7 @@ MODULE CRTM_MI355X_Module
     INTEGER :: WMO_Satellite_Id = 1023, WMO_Sensor_Id = 2047, Sensor_Channel = 0
     REAL(fp) :: Radiance = 0, Brightness_Temperature = 0
     REAL(fp) :: Up_Radiance = 0, Down_Radiance = 0, Surface_Planck_Radiance = 0, Surface_Emissivity = 0
+    REAL(fp) :: Surface_Reflectivity = 0
     REAL(fp) :: R_Clear = 0, Tb_Clear = 0, Total_Cloud_Cover = 0
   END TYPE
 
@@ -161,10 +166,12 @@ MODULE CRTM_MI355X_Module
   END TYPE
   TYPE, BIND(C) :: c_surface
     TYPE(C_PTR) :: land_coverage, water_coverage, snow_coverage, ice_coverage, &
-                   land_temperature, water_temperature, snow_temperature, ice_temperature
+                   land_temperature, water_temperature, snow_temperature, ice_temperature, &
+                   wind_speed, wind_direction, salinity
   END TYPE
   TYPE, BIND(C) :: c_geometry
-    TYPE(C_PTR) :: sensor_zenith_angle, sensor_scan_angle, source_zenith_angle, surface_altitude
+    TYPE(C_PTR) :: sensor_zenith_angle, sensor_scan_angle, source_zenith_angle, surface_altitude, &
+                   sensor_azimuth_angle
   END TYPE
   TYPE, BIND(C) :: c_options
     INTEGER(C_INT) :: use_emissivity
@@ -178,19 +185,21 @@ MODULE CRTM_MI355X_Module
   TYPE, BIND(C) :: c_rtsolution
     TYPE(C_PTR) :: radiance, brightness_temperature, up_radiance, down_radiance, &
                    surface_planck_radiance, surface_emissivity, status, r_clear, tb_clear, &
-                   total_cloud_cover
+                   total_cloud_cover, surface_reflectivity
   END TYPE
   TYPE, BIND(C) :: c_ad_output
     TYPE(C_PTR) :: brightness_temperature_ad, radiance_ad, surface_emissivity_ad, temperature_ad, absorber_ad, &
                    land_temperature_ad, water_temperature_ad, snow_temperature_ad, ice_temperature_ad, &
                    cloud_water_content_ad, cloud_effective_radius_ad, cloud_fraction_ad, &
-                   aerosol_concentration_ad, aerosol_effective_radius_ad
+                   aerosol_concentration_ad, aerosol_effective_radius_ad, &
+                   salinity_ad, wind_speed_ad, wind_direction_ad
   END TYPE
   TYPE, BIND(C) :: c_k_output
     TYPE(C_PTR) :: brightness_temperature_k, radiance_k, surface_emissivity_k, temperature_k, &
                    absorber_k, land_temperature_k, water_temperature_k, snow_temperature_k, ice_temperature_k, &
                    cloud_water_content_k, cloud_effective_radius_k, cloud_fraction_k, &
-                   aerosol_concentration_k, aerosol_effective_radius_k
+                   aerosol_concentration_k, aerosol_effective_radius_k, &
+                   salinity_k, wind_speed_k, wind_direction_k
   END TYPE
 
   INTERFACE
@@ -256,9 +265,9 @@ MODULE CRTM_MI355X_Module
     INTEGER(C_INT), ALLOCATABLE :: nl(:), ids(:), units(:), clim(:)
     INTEGER(C_INT64_T), ALLOCATABLE :: off(:)
     REAL(C_DOUBLE), ALLOCATABLE :: lp(:), p(:), t(:), a(:), cov(:, :), st(:, :), zen(:), scan(:), src(:), emis(:)
-    REAL(C_DOUBLE), ALLOCATABLE :: alt(:)
+    REAL(C_DOUBLE), ALLOCATABLE :: alt(:), ws(:), wd(:), sal(:), azi(:)
     INTEGER(C_INT), ALLOCATABLE :: skip(:)
-    REAL(C_DOUBLE), ALLOCATABLE :: rad(:), bt(:), up(:), dn(:), sp(:), se(:), rcl(:), tbc(:), tcc(:)
+    REAL(C_DOUBLE), ALLOCATABLE :: rad(:), bt(:), up(:), dn(:), sp(:), se(:), sr(:), rcl(:), tbc(:), tcc(:)
     INTEGER(C_INT), ALLOCATABLE :: status(:), ctype(:)
     REAL(C_DOUBLE), ALLOCATABLE :: cwc(:), creff(:), cfrac(:)
     INTEGER(C_INT), ALLOCATABLE :: atype(:)
@@ -275,7 +284,7 @@ MODULE CRTM_MI355X_Module
   TYPE :: k_buffers
     TYPE(batch_type) :: b
     INTEGER :: na = 0, nc = 0, nae = 0
-    REAL(C_DOUBLE), ALLOCATABLE :: btk(:), radk(:), emk(:), tk(:), ak(:), stk(:, :)
+    REAL(C_DOUBLE), ALLOCATABLE :: btk(:), radk(:), emk(:), tk(:), ak(:), stk(:, :), swk(:, :)
     REAL(C_DOUBLE), ALLOCATABLE :: cwk(:), crk(:), cfk(:), ack(:), ark(:)
   END TYPE
 
@@ -443,14 +452,14 @@ CONTAINS
     INTEGER :: Error_Status
     TYPE(batch_type), TARGET :: b
     TYPE(c_k_output) :: k
-    REAL(C_DOUBLE), ALLOCATABLE, TARGET :: btk(:), radk(:), emk(:), tk(:), ak(:), stk(:, :)
+    REAL(C_DOUBLE), ALLOCATABLE, TARGET :: btk(:), radk(:), emk(:), tk(:), ak(:), stk(:, :), swk(:, :)
     REAL(C_DOUBLE), ALLOCATABLE, TARGET :: cwk(:), crk(:), cfk(:), ack(:), ark(:)
     INTEGER :: L, M, mm, ic, j, kk, NT, na, K0, nc, nae
     Error_Status = pack_batch(Atmosphere, Surface, Geometry, ChannelInfo, RTSolution, Options, b, L, M)
     IF (Error_Status /= SUCCESS) RETURN
     NT = INT(b%off(M + 1))
     na = Atmosphere(1)%n_Absorbers
-    ALLOCATE(btk(L * M), radk(L * M), emk(L * M), tk(NT * L), ak(na * NT * L), stk(L * M, 4))
+    ALLOCATE(btk(L * M), radk(L * M), emk(L * M), tk(NT * L), ak(na * NT * L), stk(L * M, 4), swk(L * M, 3))
     DO mm = 1, M
       DO ic = 1, L
         btk((mm - 1) * L + ic) = RTSolution_K(ic, mm)%Brightness_Temperature
@@ -466,6 +475,9 @@ CONTAINS
     k%water_temperature_k = C_LOC(stk(1, 2))
     k%snow_temperature_k = C_LOC(stk(1, 3))
     k%ice_temperature_k = C_LOC(stk(1, 4))
+    k%salinity_k = C_LOC(swk(1, 1))
+    k%wind_speed_k = C_LOC(swk(1, 2))
+    k%wind_direction_k = C_LOC(swk(1, 3))
     nc = Atmosphere(1)%n_Clouds
     ALLOCATE(cwk(MAX(nc, 1) * NT * L), crk(MAX(nc, 1) * NT * L), cfk(NT * L))
     k%cloud_water_content_k = C_LOC(cwk)
@@ -517,6 +529,9 @@ CONTAINS
         Surface_K(ic, mm)%Water_Temperature = Surface_K(ic, mm)%Water_Temperature + stk((mm - 1) * L + ic, 2)
         Surface_K(ic, mm)%Snow_Temperature = Surface_K(ic, mm)%Snow_Temperature + stk((mm - 1) * L + ic, 3)
         Surface_K(ic, mm)%Ice_Temperature = Surface_K(ic, mm)%Ice_Temperature + stk((mm - 1) * L + ic, 4)
+        Surface_K(ic, mm)%Salinity = Surface_K(ic, mm)%Salinity + swk((mm - 1) * L + ic, 1)
+        Surface_K(ic, mm)%Wind_Speed = Surface_K(ic, mm)%Wind_Speed + swk((mm - 1) * L + ic, 2)
+        Surface_K(ic, mm)%Wind_Direction = Surface_K(ic, mm)%Wind_Direction + swk((mm - 1) * L + ic, 3)
       END DO
     END DO
     CALL label_rts(ChannelInfo, RTSolution)
@@ -541,7 +556,7 @@ CONTAINS
     INTEGER :: Error_Status
     TYPE(batch_type), TARGET :: b
     TYPE(c_ad_output) :: ad
-    REAL(C_DOUBLE), ALLOCATABLE, TARGET :: btad(:), radad(:), emad(:), tad(:), aad(:), stad(:, :)
+    REAL(C_DOUBLE), ALLOCATABLE, TARGET :: btad(:), radad(:), emad(:), tad(:), aad(:), stad(:, :), swad(:, :)
     REAL(C_DOUBLE), ALLOCATABLE, TARGET :: cwad(:), crad(:), cfad(:), acad(:), arad(:)
     INTEGER :: L, M, mm, ic, kk, j, K0, NT, na, nc, nae
     Error_Status = pack_batch(Atmosphere, Surface, Geometry, ChannelInfo, RTSolution, Options, b, L, M)
@@ -550,7 +565,8 @@ CONTAINS
     na = Atmosphere(1)%n_Absorbers
     nc = Atmosphere(1)%n_Clouds
     nae = Atmosphere(1)%n_Aerosols
-    ALLOCATE(btad(L * M), radad(L * M), emad(L * M), tad(NT), aad(na * NT), stad(M, 4))
+    ALLOCATE(btad(L * M), radad(L * M), emad(L * M), tad(NT), aad(na * NT), stad(M, 4), swad(M, 3))
+    swad = 0
     ALLOCATE(cwad(MAX(nc, 1) * NT), crad(MAX(nc, 1) * NT), cfad(NT), acad(MAX(nae, 1) * NT), arad(MAX(nae, 1) * NT))
     tad = 0; aad = 0; stad = 0; emad = 0; cwad = 0; crad = 0; cfad = 0; acad = 0; arad = 0
     DO mm = 1, M
@@ -568,6 +584,9 @@ CONTAINS
     ad%water_temperature_ad = C_LOC(stad(1, 2))
     ad%snow_temperature_ad = C_LOC(stad(1, 3))
     ad%ice_temperature_ad = C_LOC(stad(1, 4))
+    ad%salinity_ad = C_LOC(swad(1, 1))
+    ad%wind_speed_ad = C_LOC(swad(1, 2))
+    ad%wind_direction_ad = C_LOC(swad(1, 3))
     ad%cloud_water_content_ad = C_LOC(cwad)
     ad%cloud_effective_radius_ad = C_LOC(crad)
     ad%cloud_fraction_ad = C_LOC(cfad)
@@ -589,6 +608,9 @@ CONTAINS
       Surface_AD(mm)%Water_Temperature = Surface_AD(mm)%Water_Temperature + stad(mm, 2)
       Surface_AD(mm)%Snow_Temperature = Surface_AD(mm)%Snow_Temperature + stad(mm, 3)
       Surface_AD(mm)%Ice_Temperature = Surface_AD(mm)%Ice_Temperature + stad(mm, 4)
+      Surface_AD(mm)%Salinity = Surface_AD(mm)%Salinity + swad(mm, 1)
+      Surface_AD(mm)%Wind_Speed = Surface_AD(mm)%Wind_Speed + swad(mm, 2)
+      Surface_AD(mm)%Wind_Direction = Surface_AD(mm)%Wind_Direction + swad(mm, 3)
       DO kk = 1, Atmosphere(mm)%n_Layers
         Atmosphere_AD(mm)%Temperature(kk) = Atmosphere_AD(mm)%Temperature(kk) + tad(K0 + kk)
         DO j = 1, na
@@ -634,11 +656,13 @@ CONTAINS
     INTEGER :: Error_Status
     TYPE(k_buffers) :: kb
     TYPE(CRTM_RTSolution_type), ALLOCATABLE :: seed(:, :)
+    TYPE(CRTM_Options_type), ALLOCATABLE :: od(:)
     INTEGER :: M, L, m0, m1, mm, ic, kk, j, K0, NT, nl
     REAL(fp) :: s
-    IF (.NOT. PRESENT(Options)) THEN
-      Error_Status = FAILURE
-      RETURN
+    IF (PRESENT(Options)) THEN
+      od = Options
+    ELSE   ! the reference defaults
+      ALLOCATE(od(SIZE(Atmosphere)))
     END IF
     M = SIZE(Atmosphere)
     ALLOCATE(seed(SIZE(RTSolution, 1), M))
@@ -648,7 +672,7 @@ CONTAINS
     DO WHILE (m0 <= M)
       m1 = chunk_end(Atmosphere, ChannelInfo, m0)
       Error_Status = run_k_chunk(Atmosphere(m0:m1), Surface(m0:m1), Geometry(m0:m1), ChannelInfo, &
-                                 RTSolution(:, m0:m1), Options, m0, m1, seed(:, m0:m1), kb, L, NT)
+                                 RTSolution(:, m0:m1), od, m0, m1, seed(:, m0:m1), kb, L, NT)
       IF (Error_Status /= SUCCESS) RETURN
       DO mm = m0, m1
         K0 = INT(kb%b%off(mm - m0 + 1))
@@ -657,7 +681,10 @@ CONTAINS
           s = kb%stk((mm - m0) * L + ic, 1) * Surface_TL(mm)%Land_Temperature + &
               kb%stk((mm - m0) * L + ic, 2) * Surface_TL(mm)%Water_Temperature + &
               kb%stk((mm - m0) * L + ic, 3) * Surface_TL(mm)%Snow_Temperature + &
-              kb%stk((mm - m0) * L + ic, 4) * Surface_TL(mm)%Ice_Temperature
+              kb%stk((mm - m0) * L + ic, 4) * Surface_TL(mm)%Ice_Temperature + &
+              kb%swk((mm - m0) * L + ic, 1) * Surface_TL(mm)%Salinity + &
+              kb%swk((mm - m0) * L + ic, 2) * Surface_TL(mm)%Wind_Speed + &
+              kb%swk((mm - m0) * L + ic, 3) * Surface_TL(mm)%Wind_Direction
           DO kk = 1, nl
             s = s + kb%tk((K0 + kk - 1) * L + ic) * Atmosphere_TL(mm)%Temperature(kk)
             DO j = 1, kb%na
@@ -730,9 +757,10 @@ CONTAINS
     kb%na = Atm(1)%n_Absorbers
     kb%nc = Atm(1)%n_Clouds
     kb%nae = Atm(1)%n_Aerosols
-    IF (ALLOCATED(kb%btk)) DEALLOCATE(kb%btk, kb%radk, kb%emk, kb%tk, kb%ak, kb%stk, kb%cwk, kb%crk, kb%cfk, &
-                                      kb%ack, kb%ark)
-    ALLOCATE(kb%btk(L * M), kb%radk(L * M), kb%emk(L * M), kb%tk(NT * L), kb%ak(kb%na * NT * L), kb%stk(L * M, 4))
+    IF (ALLOCATED(kb%btk)) DEALLOCATE(kb%btk, kb%radk, kb%emk, kb%tk, kb%ak, kb%stk, kb%swk, kb%cwk, kb%crk, &
+                                      kb%cfk, kb%ack, kb%ark)
+    ALLOCATE(kb%btk(L * M), kb%radk(L * M), kb%emk(L * M), kb%tk(NT * L), kb%ak(kb%na * NT * L), kb%stk(L * M, 4), &
+             kb%swk(L * M, 3))
     ALLOCATE(kb%cwk(MAX(kb%nc, 1) * NT * L), kb%crk(MAX(kb%nc, 1) * NT * L), kb%cfk(NT * L))
     ALLOCATE(kb%ack(MAX(kb%nae, 1) * NT * L), kb%ark(MAX(kb%nae, 1) * NT * L))
     DO mm = 1, M
@@ -750,6 +778,9 @@ CONTAINS
     k%water_temperature_k = C_LOC(kb%stk(1, 2))
     k%snow_temperature_k = C_LOC(kb%stk(1, 3))
     k%ice_temperature_k = C_LOC(kb%stk(1, 4))
+    k%salinity_k = C_LOC(kb%swk(1, 1))
+    k%wind_speed_k = C_LOC(kb%swk(1, 2))
+    k%wind_direction_k = C_LOC(kb%swk(1, 3))
     k%cloud_water_content_k = C_LOC(kb%cwk)
     k%cloud_effective_radius_k = C_LOC(kb%crk)
     k%cloud_fraction_k = C_LOC(kb%cfk)
@@ -912,22 +943,28 @@ CONTAINS
   END SUBROUTINE label_rts
 
   ! ---- helpers ----
-  FUNCTION pack_batch(Atm, Sfc, Geo, ChannelInfo, RTS, Opt, b, L, M) RESULT(Error_Status)
+  FUNCTION pack_batch(Atm, Sfc, Geo, ChannelInfo, RTS, Opt_in, b, L, M) RESULT(Error_Status)
     TYPE(CRTM_Atmosphere_type), INTENT(IN) :: Atm(:)
     TYPE(CRTM_Surface_type), INTENT(IN) :: Sfc(:)
     TYPE(CRTM_Geometry_type), INTENT(IN) :: Geo(:)
     TYPE(CRTM_ChannelInfo_type), INTENT(IN) :: ChannelInfo(:)
     TYPE(CRTM_RTSolution_type), INTENT(IN) :: RTS(:, :)
-    TYPE(CRTM_Options_type), OPTIONAL, INTENT(IN) :: Opt(:)
+    TYPE(CRTM_Options_type), OPTIONAL, INTENT(IN) :: Opt_in(:)
     TYPE(batch_type), TARGET, INTENT(INOUT) :: b
     INTEGER, INTENT(OUT) :: L, M
     INTEGER :: Error_Status, mm, NT, na, K0, K, j, nc, nae
+    TYPE(CRTM_Options_type), ALLOCATABLE :: Opt(:)
     Error_Status = FAILURE
     M = SIZE(Atm)
     L = SUM(ChannelInfo(:)%n_Channels)
     IF (SIZE(Sfc) /= M .OR. SIZE(Geo) /= M .OR. SIZE(RTS, 2) < M .OR. SIZE(RTS, 1) < L) RETURN
-    IF (.NOT. PRESENT(Opt)) RETURN                  ! emissivity is required by this path
-    IF (SIZE(Opt) /= M) RETURN
+    ! no Options: the reference defaults (computed surface optics)
+    IF (PRESENT(Opt_in)) THEN
+      IF (SIZE(Opt_in) /= M) RETURN
+      Opt = Opt_in
+    ELSE
+      ALLOCATE(Opt(M))
+    END IF
     na = Atm(1)%n_Absorbers
     ALLOCATE(b%nl(M), b%off(M + 1), b%clim(M))
     b%off(1) = 0
@@ -940,7 +977,9 @@ CONTAINS
     NT = INT(b%off(M + 1))
     ALLOCATE(b%ids(na), b%units(na), b%lp(NT + M), b%p(NT), b%t(NT), b%a(na * NT), b%cov(M, 4), b%st(M, 4), &
              b%zen(M), b%scan(M), b%src(M), b%alt(M), b%skip(M), b%emis(L * M), b%rad(L * M), b%bt(L * M), b%up(L * M), &
-             b%dn(L * M), b%sp(L * M), b%se(L * M), b%status(M), b%rcl(L * M), b%tbc(L * M), b%tcc(L * M))
+             b%dn(L * M), b%sp(L * M), b%se(L * M), b%status(M), b%rcl(L * M), b%tbc(L * M), b%tcc(L * M), &
+             b%sr(L * M), b%ws(M), b%wd(M), b%sal(M), b%azi(M))
+    b%emis = 0
     nc = Atm(1)%n_Clouds
     ALLOCATE(b%ctype(MAX(nc, 1) * M), b%cwc(MAX(nc, 1) * NT), b%creff(MAX(nc, 1) * NT), b%cfrac(NT))
     b%ctype = 0; b%cwc = 0; b%creff = 0; b%cfrac = 0
@@ -965,6 +1004,10 @@ CONTAINS
       b%scan(mm) = Geo(mm)%Sensor_Scan_Angle
       b%src(mm) = Geo(mm)%Source_Zenith_Angle
       b%alt(mm) = Geo(mm)%Surface_Altitude
+      b%ws(mm) = Sfc(mm)%Wind_Speed
+      b%wd(mm) = Sfc(mm)%Wind_Direction
+      b%sal(mm) = Sfc(mm)%Salinity
+      b%azi(mm) = Geo(mm)%Sensor_Azimuth_Angle
       b%skip(mm) = MERGE(1, 0, Opt(mm)%Skip_Profile)
       IF (Opt(mm)%Skip_Profile) CYCLE                ! no inputs read (src/CRTM_Forward_Module.f90:536)
       IF (Opt(mm)%RT_Algorithm_Id /= RT_ADA) THEN
@@ -975,9 +1018,16 @@ CONTAINS
         CALL Display_Message('CRTM', 'Include_Scattering must be the same for all profiles of a call', FAILURE)
         RETURN
       END IF
-      IF (.NOT. Opt(mm)%Use_Emissivity) RETURN
-      IF (SIZE(Opt(mm)%Emissivity) < L) RETURN
-      b%emis((mm - 1) * L + 1:mm * L) = Opt(mm)%Emissivity(1:L)
+      ! Use_Emissivity is one switch per call here: user emissivities for every
+      ! profile, or the computed (water) surface optics for every profile
+      IF (Opt(mm)%Use_Emissivity .NEQV. Opt(1)%Use_Emissivity) THEN
+        CALL Display_Message('CRTM', 'Use_Emissivity must be the same for all profiles of a call', FAILURE)
+        RETURN
+      END IF
+      IF (Opt(mm)%Use_Emissivity) THEN
+        IF (SIZE(Opt(mm)%Emissivity) < L) RETURN
+        b%emis((mm - 1) * L + 1:mm * L) = Opt(mm)%Emissivity(1:L)
+      END IF
       IF (Atm(mm)%n_Clouds /= nc) RETURN
       IF (nc > 0) THEN
         b%cfrac(K0 + 1:K0 + K) = Atm(mm)%Cloud_Fraction(1:K)
@@ -1028,7 +1078,11 @@ CONTAINS
     b%geo%sensor_scan_angle = C_LOC(b%scan)
     b%geo%source_zenith_angle = C_LOC(b%src)
     b%geo%surface_altitude = C_LOC(b%alt)
-    b%opt%use_emissivity = 1
+    b%geo%sensor_azimuth_angle = C_LOC(b%azi)
+    b%sfc%wind_speed = C_LOC(b%ws)
+    b%sfc%wind_direction = C_LOC(b%wd)
+    b%sfc%salinity = C_LOC(b%sal)
+    b%opt%use_emissivity = MERGE(1, 0, Opt(1)%Use_Emissivity)
     b%opt%emissivity = C_LOC(b%emis)
     b%opt%precision = Opt(1)%Precision
     b%opt%memory = 0
@@ -1057,6 +1111,7 @@ CONTAINS
     b%rts%r_clear = C_LOC(b%rcl)
     b%rts%tb_clear = C_LOC(b%tbc)
     b%rts%total_cloud_cover = C_LOC(b%tcc)
+    b%rts%surface_reflectivity = C_LOC(b%sr)
     Error_Status = SUCCESS
   END FUNCTION pack_batch
 
@@ -1070,6 +1125,7 @@ CONTAINS
     r%Down_Radiance = b%dn(i)
     r%Surface_Planck_Radiance = b%sp(i)
     r%Surface_Emissivity = b%se(i)
+    r%Surface_Reflectivity = b%sr(i)
     r%R_Clear = b%rcl(i)
     r%Tb_Clear = b%tbc(i)
     r%Total_Cloud_Cover = b%tcc(i)

@@ -442,6 +442,12 @@ class ProfileSet:
     aerosol_concentration: Optional[np.ndarray] = None
     aerosol_effective_radius: Optional[np.ndarray] = None
     climatology: Optional[np.ndarray] = None         # int32 [M], 1..6 (None: US standard)
+    # water surface and view azimuth, inputs of the computed surface optics
+    # (None: the reference defaults 5 m/s, 0 degrees, 33 ppt, azimuth off)
+    wind_speed: Optional[np.ndarray] = None          # m/s [M]
+    wind_direction: Optional[np.ndarray] = None      # degrees [M]
+    salinity: Optional[np.ndarray] = None            # ppt [M]
+    sensor_azimuth: Optional[np.ndarray] = None      # degrees [M]
 
     @property
     def n_profiles(self) -> int:
@@ -757,13 +763,15 @@ def make_mwwater_coeff(seed: int = 20261206) -> coeffio.MWwaterCoeff:
     u = lambda *shape: rng.uniform(0.6, 1.4, shape)
     fc = np.array([2.95e-6, 3.52])                         # Monahan-O'Muircheartaigh whitecap law
     fr = np.array([0.93, -2.0e-4, -2.0e-5, 1.0e-8, 1.0, -2.0e-3])
-    # reflection correction: each of the 7 predictors (1, var, var/cos, 1/cos,
-    # (var/cos)^2, 1/cos^2, var^2; slope variance var ~ 0.003 + 0.005 U) adds
-    # at most ~0.04 to the roughness exponent
-    zmag = np.array([1.0, 0.05, 0.07, 1.5, 4.0e-3, 2.2, 2.5e-3])
+    # reflection correction: each of the 21 terms (predictors 1, var, var/cos,
+    # 1/cos, (var/cos)^2, 1/cos^2, var^2 at their largest -- slope variance
+    # var <= 0.2 up to 35 m/s, cos >= 0.5 -- times 1, x, x^2 with |x| =
+    # |ln(od)| <= 7) adds at most 0.02 to the roughness exponent, which so
+    # stays within [0.58, 1.42]
+    zmag = np.array([1.0, 0.2, 0.4, 2.0, 0.16, 4.0, 0.04])
     rc = np.empty((3, 7, 2))
-    for t, odmax in enumerate((1.0, 4.6, 21.0)):
-        rc[t] = 0.04 / (odmax * zmag)[:, None] * u(7, 2) * rng.choice((-1.0, 1.0), (7, 2))
+    for t, xmax in enumerate((1.0, 7.0, 49.0)):
+        rc[t] = 0.02 / (xmax * zmag)[:, None] * u(7, 2) * rng.choice((-1.0, 1.0), (7, 2))
     az = np.zeros((6, 6, 2))
     fscale = u(6)
     az[0, :, 0] = 1.0
@@ -793,19 +801,39 @@ def make_irwater_coeff(with_temperature: bool = True) -> coeffio.IRwaterCoeff:
     return coeffio.IRwaterCoeff("Nalli2" if with_temperature else "Nalli", ang, nu, v, t, e)
 
 
+def _write_if_changed(path: str, writer) -> str:
+    tmp = f"{path}.{os.getpid()}.tmp"
+    writer(tmp)
+    if os.path.exists(path) and open(path, "rb").read() == open(tmp, "rb").read():
+        os.remove(tmp)
+    else:
+        os.replace(tmp, path)
+    return path
+
+
 def write_mwwater_coeff(dirpath: str) -> str:
     os.makedirs(dirpath, exist_ok=True)
-    path = os.path.join(dirpath, MWWATERCOEFF_FILE)
-    if not os.path.exists(path):
-        coeffio.write_mwwatercoeff(path + ".tmp", make_mwwater_coeff())
-        os.replace(path + ".tmp", path)
-    return path
+    return _write_if_changed(os.path.join(dirpath, MWWATERCOEFF_FILE),
+                             lambda p: coeffio.write_mwwatercoeff(p, make_mwwater_coeff()))
 
 
 def write_irwater_coeff(dirpath: str, with_temperature: bool = True, name: str = IRWATERCOEFF_FILE) -> str:
     os.makedirs(dirpath, exist_ok=True)
-    path = os.path.join(dirpath, name)
-    if not os.path.exists(path):
-        coeffio.write_irwatercoeff(path + ".tmp", make_irwater_coeff(with_temperature))
-        os.replace(path + ".tmp", path)
-    return path
+    return _write_if_changed(os.path.join(dirpath, name),
+                             lambda p: coeffio.write_irwatercoeff(p, make_irwater_coeff(with_temperature)))
+
+
+def ocean_surface(ps: ProfileSet, seed: int = 20261207, wind_range=(0.5, 22.0)) -> ProfileSet:
+    """All-water surfaces with random wind, salinity and view azimuth (the
+    inputs of the computed surface optics); scan angles consistent with the
+    zenith angles of an 833 km orbit."""
+    rng = np.random.default_rng(seed)
+    M = ps.n_profiles
+    zen = np.asarray(ps.sensor_zenith, np.float64)
+    # scan angle from the zenith angle: sin(scan) = sin(zenith) R / (R + h)
+    scan = np.degrees(np.arcsin(np.sin(np.radians(zen)) * 6370.0 / (6370.0 + 833.0)))
+    return dataclasses.replace(
+        ps, land_coverage=np.zeros(M), water_coverage=np.ones(M), snow_coverage=np.zeros(M),
+        ice_coverage=np.zeros(M), sensor_scan=scan * rng.choice((-1.0, 1.0), M),
+        wind_speed=rng.uniform(*wind_range, M), wind_direction=rng.uniform(0.0, 360.0, M),
+        salinity=rng.uniform(30.0, 37.0, M), sensor_azimuth=rng.uniform(0.0, 360.0, M))

@@ -103,6 +103,11 @@ typedef struct {
   const double* water_temperature;
   const double* snow_temperature;
   const double* ice_temperature;
+  /* Water surface inputs of the computed emissivity (use_emissivity = 0),
+   * [M] each, NULL = the reference defaults (src/Surface/CRTM_Surface_Define.f90:183-185) */
+  const double* wind_speed;       /* m/s, default 5 */
+  const double* wind_direction;   /* degrees, default 0 */
+  const double* salinity;         /* ppt, default 33 */
 } crtm_surface;
 
 /* ---- Geometry (src/GeometryInfo/CRTM_Geometry_Define.f90:99-121): [M] ---- */
@@ -111,11 +116,19 @@ typedef struct {
   const double* sensor_scan_angle;     /* degrees, may be NULL */
   const double* source_zenith_angle;   /* degrees, may be NULL (100 = no source) */
   const double* surface_altitude;      /* metres, may be NULL */
+  const double* sensor_azimuth_angle;  /* degrees, may be NULL (999.9: no azimuth dependence of the
+                                          FASTEM emissivity; src/GeometryInfo/CRTM_Geometry_Define.f90:111) */
 } crtm_geometry;
 
 /* ---- Options (src/Options/CRTM_Options_Define.f90:122-179) ---- */
 typedef struct {
-  int use_emissivity;            /* Options%Use_Emissivity */
+  int use_emissivity;            /* Options%Use_Emissivity; 0: the surface optics are computed
+                                    (CRTM_Compute_SfcOptics, src/SfcOptics/CRTM_SfcOptics.f90:454):
+                                    FASTEM6 for microwave and the IR sea-surface emissivity table
+                                    for infrared sensors, for water surfaces (land, snow and ice
+                                    coverage 0) in clear-sky calls; needs
+                                    FASTEM6.MWwater.EmisCoeff.bin / Nalli.IRwater.EmisCoeff.bin in
+                                    the crtm_init path */
   const double* emissivity;      /* [M][L] when use_emissivity */
   int precision;                 /* CRTM_PREC_* */
   int memory;                    /* CRTM_MEM_* */
@@ -163,6 +176,7 @@ typedef struct {
   double* r_clear;                 /* RTSolution%R_Clear  [M][L], optional */
   double* tb_clear;                /* RTSolution%Tb_Clear [M][L], optional */
   double* total_cloud_cover;       /* RTSolution%Total_Cloud_Cover [M][L], optional */
+  double* surface_reflectivity;    /* RTSolution%Surface_Reflectivity [M][L], optional */
 } crtm_rtsolution;
 
 /* ---- K-matrix adjoint inputs/outputs ---- */
@@ -197,6 +211,12 @@ typedef struct {
    * required when the atmosphere has aerosols; overwritten. */
   void* aerosol_concentration_k;
   void* aerosol_effective_radius_k;
+  /* Surface_K%Salinity, %Wind_Speed, %Wind_Direction [M][L], optional: the
+   * computed optics' sensitivities (CRTM_Compute_SfcOptics_AD,
+   * src/SfcOptics/CRTM_SfcOptics.f90:1857; zero with user emissivity) */
+  double* salinity_k;
+  double* wind_speed_k;
+  double* wind_direction_k;
 } crtm_k_output;
 
 /* CRTM_Init (src/CRTM_LifeCycle.f90:520): loads <id>.SpcCoeff.bin and
@@ -253,6 +273,10 @@ typedef struct {
   double* cloud_fraction_ad;                /* Atmosphere_AD%Cloud_Fraction [NT] */
   double* aerosol_concentration_ad;         /* Atmosphere_AD%Aerosol(n)%Concentration [n_aerosols][NT] */
   double* aerosol_effective_radius_ad;      /* Atmosphere_AD%Aerosol(n)%Effective_Radius [n_aerosols][NT] */
+  double* salinity_ad;                      /* Surface_AD%Salinity, %Wind_Speed, %Wind_Direction [M],
+                                               optional (computed surface optics) */
+  double* wind_speed_ad;
+  double* wind_direction_ad;
 } crtm_ad_output;
 
 /* CRTM_Adjoint (src/CRTM_Adjoint_Module.f90:299-325): the forward RTSolution
