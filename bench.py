@@ -188,21 +188,24 @@ def main():
     ap.add_argument("--streams", type=int, default=8, help="allsky-ir: ADA stream count (8 or 16)")
     ap.add_argument("--wavefront-angles", type=int, default=0,
                     help="all-sky: angle counts >= this run on the wavefront (matrix-core) kernels (6, 9)")
-    ap.add_argument("--workload", choices=("clear-ir", "allsky-mw", "allsky-ir", "batched"), default="clear-ir",
+    ap.add_argument("--workload", choices=("clear-ir", "allsky-mw", "allsky-ir", "batched", "sfc"), default="clear-ir",
                     help="clear-ir: the headline (CrIS-like forward + K-matrix); allsky-mw: cloudy ATMS-like "
                          "forward + K-matrix with the ADA scattering solver at 4 and 8 streams; allsky-ir: "
                          "IASI-like forward + K-matrix, 60/40 clear/cloudy, 8 streams; batched: three sensors, "
                          "ragged profiles with clouds and aerosols, per-profile stream counts, gradient of a "
-                         "variational cost through torch.autograd")
+                         "variational cost through torch.autograd; sfc: clear-sky K-matrix over an ocean with "
+                         "the surface optics computed (FASTEM6 for ATMS-like, the IR emissivity table for "
+                         "CrIS-like) against the same call with user emissivity")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
         return
-    if args.workload in ("allsky-mw", "allsky-ir", "batched"):
+    if args.workload in ("allsky-mw", "allsky-ir", "batched", "sfc"):
         if args.gpus != 1:
             print("bench.py measures one GPU. Use --gpus 1.", file=sys.stderr)
             sys.exit(2)
-        {"allsky-mw": main_allsky, "allsky-ir": main_allsky_ir, "batched": main_batched}[args.workload](args)
+        {"allsky-mw": main_allsky, "allsky-ir": main_allsky_ir, "batched": main_batched,
+         "sfc": main_sfc}[args.workload](args)
         return
     if args.gpus != 1:
         print("bench.py measures one GPU: the MI355X path runs on a single device (profiles shard "
@@ -327,6 +330,98 @@ def main():
     line.update(result)
     print(json.dumps(line), flush=True)
     ctx.destroy()
+
+
+def main_sfc(args):
+    """Computed surface optics: clear-sky K-matrix over an all-water surface
+    with random wind, salinity and azimuths, ATMS-like (FASTEM6, 16384
+    profiles) and CrIS-like (IR emissivity table, 4096 profiles), FP32-split
+    and FP64; the same call with user emissivity for the overhead; the optics
+    kernels timed with HIP events; BT and Jacobian errors against the oracle
+    on a sample."""
+    import torch
+    from crtmv3_amd import api, coeffio, synthetic
+    from oracle import oracle as orc
+    dev = torch.device("cuda:0")
+    coeff_dir = synthetic.default_coeff_dir()
+    synthetic.write_mwwater_coeff(coeff_dir)
+    synthetic.write_irwater_coeff(coeff_dir)
+    mw = orc.MWwater(coeffio.read_mwwatercoeff(os.path.join(coeff_dir, synthetic.MWWATERCOEFF_FILE)))
+    ir = orc.IRwater(coeffio.read_irwatercoeff(os.path.join(coeff_dir, synthetic.IRWATERCOEFF_FILE)))
+    K = 100
+    results = {}
+    for sensor, M in (("atms_syn", 16384), ("cris_syn", 4096)):
+        synthetic.write_sensor(coeff_dir, sensor)
+        ctx = api.crtm_init([sensor], coeff_dir)
+        L = ctx.n_channels
+        st = ctx.channel_info[0].sensor_type
+        ps = synthetic.ocean_surface(synthetic.make_profiles(M, K, seed=20261203 + 9), seed=20261203 + 10)
+        dps = api.device_profiles(ps, dev)
+        emis = torch.as_tensor(np.tile(synthetic.default_emissivity(st, L), (M, 1)), device=dev)
+        btk = torch.ones((M, L), dtype=torch.float64, device=dev)
+        torch.cuda.synchronize()
+        r = {}
+        for prec, pname, steps in ((api.PREC_FP32_SPLIT, "split", args.steps), (api.PREC_FP64, "fp64", args.fp64_steps)):
+            for mode, e in (("computed", None), ("user", emis)):
+                prev = ctx.k_matrix(dps, e, bt_k=btk, precision=prec, jacobian_f32=prec == api.PREC_FP32_SPLIT)
+                for _ in range(args.warmup):
+                    prev = ctx.k_matrix(dps, e, bt_k=btk, precision=prec, jacobian_f32=prec == api.PREC_FP32_SPLIT,
+                                        out=prev)
+                ctx.synchronize()
+                ctx.profile_enable(True)
+                t0 = time.perf_counter()
+                for _ in range(steps):
+                    prev = ctx.k_matrix(dps, e, bt_k=btk, precision=prec, jacobian_f32=prec == api.PREC_FP32_SPLIT,
+                                        out=prev)
+                ctx.synchronize()
+                torch.cuda.synchronize()
+                t = (time.perf_counter() - t0) / steps
+                ms, _ = ctx.profile_query_kinds()
+                ctx.profile_enable(False)
+                r[f"{pname}_{mode}"] = {"ms_per_step": t * 1e3, "profile_channels_per_s": M * L / t,
+                                        "kernel_ms": {"prep": ms[0] / steps, "clear": ms[1] / steps,
+                                                      "surface_optics": ms[3] / steps}}
+                log(f"  {sensor} {pname} {mode}: {t * 1e3:.2f} ms/step, kernels {r[f'{pname}_{mode}']['kernel_ms']}")
+                if mode == "computed":
+                    n = 8
+                    rts, kd = prev
+                    bt = rts["brightness_temperature"][:n].cpu().numpy()
+                    tk = kd["temperature_k"].double().cpu().numpy()
+                    wk = kd["wind_speed_k"][:n].cpu().numpy()
+                    tau = coeffio.read_odps(os.path.join(coeff_dir, f"{sensor}.TauCoeff.bin"))
+                    spc = coeffio.read_spccoeff(os.path.join(coeff_dir, f"{sensor}.SpcCoeff.bin"))
+                    S = orc.Sensor(tau, spc)
+                    bt_err = tk_err = wk_err = 0.0
+                    for m in range(n):
+                        so = orc.SurfaceOptics(spc, mw=mw if st == 1 else None, ir=ir if st != 1 else None,
+                                               salinity=ps.salinity[m], wind_speed=ps.wind_speed[m],
+                                               wind_direction=ps.wind_direction[m], sensor_azimuth=ps.sensor_azimuth[m])
+                        o = orc.run_profile_sfc(S, ps, m, so, "k", bt_k=np.ones(L))
+                        a, b = int(ps.layer_offset[m]), int(ps.layer_offset[m + 1])
+                        bt_err = max(bt_err, float(np.abs(bt[m] - o["bt"]).max()))
+                        tk_err = max(tk_err, float(np.abs(tk[a:b].T - o["t_k"]).max() / np.abs(o["t_k"]).max()))
+                        sw = np.abs(o["sfc_k"][:, 1]).max()
+                        if sw > 0:
+                            wk_err = max(wk_err, float(np.abs(wk[m] - o["sfc_k"][:, 1]).max() / sw))
+                    r[f"{pname}_vs_oracle"] = {"bt_err_K": bt_err, "temperature_k_rel": tk_err,
+                                               "wind_speed_k_rel": wk_err, "sample_profiles": n}
+                del prev
+                torch.cuda.empty_cache()
+        for pname in ("split", "fp64"):
+            r[f"{pname}_overhead"] = r[f"{pname}_computed"]["ms_per_step"] / r[f"{pname}_user"]["ms_per_step"]
+        results[sensor] = {"n_profiles": M, "n_channels": L, **r}
+        ctx.destroy()
+    a = results["atms_syn"]
+    line = {"metric": "clear-sky K-matrix profile-channels/s with computed water surface optics", "value":
+            a["split_computed"]["profile_channels_per_s"], "unit": "profile-channels/s", "n_gpus": 1,
+            "steps": args.steps, "warmup": args.warmup, "ms_per_step": a["split_computed"]["ms_per_step"],
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32+fp64-split",
+            "data": "synthetic (seeded ocean profiles; synthetic ODPS/SpcCoeff/EmisCoeff files)",
+            "config": {"workload": "clear-sky K-matrix over an ocean, FASTEM6 (ATMS-like 22 ch x 16384 profiles) and "
+                                   "the IR sea-surface emissivity table (CrIS-like 2211 ch x 4096 profiles), 100 layers",
+                       "parallelism": "single GPU"},
+            "by_sensor": results, "roofline": None, "cpu_baseline": None}
+    print(json.dumps(line), flush=True)
 
 
 def main_allsky(args):

@@ -607,10 +607,10 @@ class Context:
 
     def profile_query_kinds(self):
         """Milliseconds and launches per kernel kind since the last enable/query:
-        ([prep, forward/K, scattering], [n_prep, n_main, n_scattering])."""
-        ms = (ctypes.c_double * 3)()
-        n = (ctypes.c_int * 3)()
-        if lib().crtm_profile_query_kinds(self._h, ms, n, 3) != SUCCESS:
+        ([prep, forward/K, scattering, surface optics], [n_prep, n_main, n_scattering, n_surface])."""
+        ms = (ctypes.c_double * 4)()
+        n = (ctypes.c_int * 4)()
+        if lib().crtm_profile_query_kinds(self._h, ms, n, 4) != SUCCESS:
             self._err("crtm_profile_query_kinds")
         return list(ms), list(n)
 

@@ -1301,7 +1301,7 @@ void run_sensors(crtm_context* c, const DevBatch& be, DevOut o, const CloudCall&
           take(c, c->sfcplanes, (size_t)stride * (mode == SFC_MW ? MP_N : IP_N) * sizeof(double)));
       if (c->profiling) {
         c->mark();
-        c->ev_kind.push_back(0);
+        c->ev_kind.push_back(3);
       }
       launch_sfc_optics(S->d, be, be.sfc_in, mode, c->mw, c->ir, planes, stride, sfc_err, c->stream);
       if (c->profiling) c->mark();
@@ -2048,8 +2048,8 @@ int crtm_profile_enable(crtm_context* ctx, int on) {
 int crtm_profile_query_kinds(crtm_context* ctx, double* ms, int* n_launches, int n_kinds) {
   if (!ctx) return CRTM_FAILURE;
   if (hipStreamSynchronize(ctx->stream) != hipSuccess) return fail(ctx, "stream synchronize failed");
-  double t[3] = {0.0, 0.0, 0.0};
-  int n[3] = {0, 0, 0};
+  double t[4] = {0.0, 0.0, 0.0, 0.0};
+  int n[4] = {0, 0, 0, 0};
   for (size_t i = 0; i < ctx->ev_kind.size(); ++i) {
     float e = 0.f;
     if (hipEventElapsedTime(&e, ctx->ev[2 * i], ctx->ev[2 * i + 1]) != hipSuccess)
@@ -2057,7 +2057,7 @@ int crtm_profile_query_kinds(crtm_context* ctx, double* ms, int* n_launches, int
     t[ctx->ev_kind[i]] += e;
     n[ctx->ev_kind[i]] += 1;
   }
-  for (int q = 0; q < n_kinds && q < 3; ++q) {
+  for (int q = 0; q < n_kinds && q < 4; ++q) {
     if (ms) ms[q] = t[q];
     if (n_launches) n_launches[q] = n[q];
   }

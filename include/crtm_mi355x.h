@@ -353,7 +353,8 @@ const char* crtm_last_error(const crtm_context* ctx);
 int crtm_profile_enable(crtm_context* ctx, int on);
 int crtm_profile_query(crtm_context* ctx, double* prep_ms, double* main_ms, int* n_launches);
 /* Per kind: [0] preparation, [1] forward/K-matrix, [2] all-sky scattering
- * kernels (cloud optics, layer R/T, adding sweep); n_kinds <= 3. */
+ * kernels (cloud optics, layer R/T, adding sweep), [3] surface optics;
+ * n_kinds <= 4. */
 int crtm_profile_query_kinds(crtm_context* ctx, double* ms, int* n_launches, int n_kinds);
 
 #ifdef __cplusplus
