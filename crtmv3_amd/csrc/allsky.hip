@@ -174,8 +174,10 @@ __global__ void __launch_bounds__(SCAN_THREADS) cloud_scan_kernel(int M, CloudWo
 }
 
 // Slot numbers of each profile's scattering layers (after the scan); a
-// profile whose layers do not all fit the slot capacity fails.
-__global__ void __launch_bounds__(64) cloud_slot_kernel(DevBatch be, CloudWork cw) {
+// profile whose layers do not all fit the slot capacity fails, and so does
+// one whose per-profile stream count (device memory, not checked on the
+// host in a captured call) is not one the kernels take.
+__global__ void __launch_bounds__(64) cloud_slot_kernel(DevBatch be, DevClouds cl, CloudWork cw) {
   const int m = blockIdx.x * blockDim.x + threadIdx.x;
   if (m >= be.M) return;
   const int K = be.n_layers[m], count = cw.cl_count[m], base = cw.cl_base[m];
@@ -191,7 +193,12 @@ __global__ void __launch_bounds__(64) cloud_slot_kernel(DevBatch be, CloudWork c
     ++j;
     cw.cl_slot[le + kk] = slot;
   }
-  cw.aer_err[m] = (count > 0 && base + count > cw.cap) ? 2 : 0;
+  int err = (count > 0 && base + count > cw.cap) ? 2 : 0;
+  if (cl.ns_prof) {
+    const int n = cl.ns_prof[m];
+    if (!(n == 0 || n == 2 || n == 4 || n == 6 || n == 8 || n == 16)) err = 3;
+  }
+  cw.aer_err[m] = err;
 }
 
 // =========================================================================
@@ -635,7 +642,8 @@ __global__ void status_merge_kernel(int M, const int* prep_status, const int* ae
   int st = call_status[m];
   if (prep_status[m] > st) st = prep_status[m];
   if (aer_err && aer_err[m]) {
-    const int a = aer_err[m] == 2 ? 6 : 5;
+    // 5: aerosol failure, 6: layer capacity exceeded, 8: invalid stream count
+    const int a = aer_err[m] == 2 ? 6 : aer_err[m] == 3 ? 8 : 5;
     if (a > st) st = a;
   }
   call_status[m] = st;
@@ -660,7 +668,7 @@ __global__ void zero_fill_kernel(uint32_t* p, int64_t n_words) {
 void launch_cloud_prep(const DevBatch& be, const DevClouds& cl, const CloudWork& cw, hipStream_t st) {
   hipLaunchKernelGGL(cloud_prep_kernel, dim3((be.M + 63) / 64), dim3(64), 0, st, be, cl, cw);
   hipLaunchKernelGGL(cloud_scan_kernel, dim3(1), dim3(SCAN_THREADS), 0, st, be.M, cw);
-  hipLaunchKernelGGL(cloud_slot_kernel, dim3((be.M + 63) / 64), dim3(64), 0, st, be, cw);
+  hipLaunchKernelGGL(cloud_slot_kernel, dim3((be.M + 63) / 64), dim3(64), 0, st, be, cl, cw);
 }
 
 void launch_aerosol_check(const DevSensor& s, int M, const DevClouds& cl, const CloudWork& cw,

@@ -1418,9 +1418,9 @@ int do_call(crtm_context* ctx, const crtm_atmosphere* atm, const crtm_surface* s
     if (cloudy && ctx->cloud_cap <= 0)
       return fail(ctx, "graph capture: a cloudy call reads its cloud-layer count back unless a layer capacity is "
                        "set (crtm_set_cloudy_layer_capacity)");
-    if (cloudy && opt->n_streams_profile)
-      return fail(ctx, "graph capture: per-profile stream counts are staged through the host; give one n_streams "
-                       "for the batch");
+    if (cloudy && opt->n_streams_profile && host_resident(opt->n_streams_profile))
+      return fail(ctx, "graph capture: per-profile stream counts in host memory are staged through the host; give "
+                       "them in device memory or one n_streams for the batch");
   }
   CloudCall cc;
   cc.on = atm->n_clouds > 0 || atm->n_aerosols > 0;
@@ -1442,7 +1442,14 @@ int do_call(crtm_context* ctx, const crtm_atmosphere* atm, const crtm_surface* s
     // per-profile stream counts (Options(m)%n_Streams): a host copy gives the
     // angle counts to launch for
     cc.cl.nz_mask = angle_mask(opt->n_streams);
-    if (opt->n_streams_profile) {
+    if (opt->n_streams_profile && capture != hipStreamCaptureStatusNone) {
+      // captured call: the counts stay on the device; every angle count a
+      // valid count can give is launched (kernels skip the items of other
+      // counts) and a profile with an invalid count fails in the cloud
+      // preparation
+      cc.cl.nz_mask = 0;
+      for (int n : {0, 2, 4, 6, 8, 16}) cc.cl.nz_mask |= angle_mask(n);
+    } else if (opt->n_streams_profile) {
       std::vector<int> nsp(M);
       if (opt->memory == CRTM_MEM_HOST || host_resident(opt->n_streams_profile)) {
         std::memcpy(nsp.data(), opt->n_streams_profile, M * sizeof(int));

@@ -144,9 +144,10 @@ typedef struct {
                                     n_streams; same values.  Host memory is read
                                     directly in either memory mode; device memory is
                                     copied back with one synchronisation.  Under HIP
-                                    graph capture a cloudy call rejects it (the counts
-                                    choose the launches on the host): give one
-                                    n_streams for the batch there */
+                                    graph capture a cloudy call takes them in device
+                                    memory only, reads nothing back, launches the
+                                    kernels of every stream count and fails a profile
+                                    whose count is not 0, 2, 4, 6, 8 or 16 (status 8) */
   int no_scattering;             /* nonzero: Options%Include_Scattering = .FALSE.: clouds and
                                     aerosols absorb only (extinction ke (1 - w), no phase
                                     function) and the radiance is the emission solution

@@ -168,6 +168,63 @@ def test_captured_cloudy_k_matrix_replays_match_eager(coeff_dir):
         ctx.destroy()
 
 
+def test_captured_cloudy_call_with_per_profile_streams(coeff_dir):
+    """Per-profile stream counts (Options(m)%n_Streams) in device memory
+    inside a captured cloudy K-matrix call: replays equal eager calls with
+    the same counts, and a count the kernels do not take fails its profile
+    (status 8) without failing the others."""
+    import torch
+    print("graph capture of a cloudy K-matrix call with per-profile stream counts", flush=True)
+    dev = torch.device("cuda:0")
+    ps, dps = _cloudy_batch(coeff_dir, dev)
+    ref = api.crtm_init(["atms_syn", "hirs_syn"], coeff_dir)
+    ctx = api.crtm_init(["atms_syn", "hirs_syn"], coeff_dir)
+    try:
+        ctx.set_cloudy_layer_capacity(int(ps.layer_offset[-1]))
+        L = ctx.n_channels
+        M = ps.n_profiles
+        emis = torch.full((M, L), 0.9, dtype=torch.float64, device=dev)
+        btk = torch.ones((M, L), dtype=torch.float64, device=dev)
+        counts = torch.as_tensor(np.resize([4, 8, 16, 0, 6], M), dtype=torch.int32, device=dev)
+        kw = dict(bt_k=btk, precision=api.PREC_FP32_SPLIT, jacobian_f32=True, check_input=False)
+        s = torch.cuda.Stream()
+        with torch.cuda.stream(s):
+            out = ctx.k_matrix(dps, emis, n_streams=counts, **kw)
+            out = ctx.k_matrix(dps, emis, n_streams=counts, out=out, **kw)
+        s.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g, stream=s):
+            captured = ctx.k_matrix(dps, emis, n_streams=counts, out=out, **kw)
+        base = counts.clone()
+        for rep, bad in enumerate((None, 3)):
+            counts.copy_(base)
+            if bad is not None:
+                counts[bad] = 5
+            g.replay()
+            torch.cuda.synchronize()
+            got_st = captured[0]["status"].clone()
+            got_bt = captured[0]["brightness_temperature"].clone()
+            got_k = {k: captured[1][k].clone() for k in _k_keys(captured[1])}
+            if bad is None:
+                r_e, k_e = ref.k_matrix(dps, emis, n_streams=counts, **kw)
+                torch.cuda.synchronize()
+                assert int((r_e["status"] != 0).sum()) == 0
+                assert torch.equal(got_st, r_e["status"])
+                assert torch.equal(got_bt, r_e["brightness_temperature"])
+                for key in got_k:
+                    assert torch.equal(got_k[key], k_e[key]), key
+            else:
+                st = got_st.cpu().numpy()
+                assert st[bad] == 8 and np.all(np.delete(st, bad) == 0), st
+                assert torch.isnan(got_bt[bad]).all()
+                keep = [m for m in range(M) if m != bad]
+                assert torch.equal(got_bt[keep], r_e["brightness_temperature"][keep])
+        counts.copy_(base)
+    finally:
+        ref.destroy()
+        ctx.destroy()
+
+
 def _scattering_layer_counts(ps):
     """Scattering layers per profile as the cloud preparation counts them:
     a typed cloud with water content above 1e-6 kg/m^2 in a layer whose cloud
