@@ -6,8 +6,10 @@
 ! temperature K and emissivity K as text, and the adjoint (seeded with
 ! BT_AD(l) = 0.5 + l/L) and tangent linear (T_TL = 1 K, absorber_TL = 1 %,
 ! water temperature_TL = 0.5 K) to <output.txt>.ad, and a forward call with
-! Options%Skip_Profile on the last profile to <output.txt>.skip (used by the
-! Fortran tests).
+! Options%Skip_Profile on the last profile to <output.txt>.skip, and a
+! K-matrix with the surface optics computed (Options%Use_Emissivity =
+! .FALSE.; wind speed 4 + m m/s, wind direction 30 m degrees, salinity 34,
+! sensor azimuth 100 degrees) to <output.txt>.sfc (used by the Fortran tests).
 !
 !   crtm_mi355x_driver <input.txt> <output.txt>
 PROGRAM crtm_mi355x_driver
@@ -130,6 +132,36 @@ PROGRAM crtm_mi355x_driver
   WRITE(u, '(*(I8))') (rts_tl(l1, 1)%Sensor_Channel, l1 = 1, L)
   DO m1 = 1, M
     WRITE(u, '(*(ES25.16E3))') (rts_tl(l1, m1)%Brightness_Temperature, l1 = 1, L)
+  END DO
+  CLOSE(u)
+
+  ! computed water surface optics: emissivity / reflectivity and Surface_K
+  ! from the wind, salinity and view azimuth
+  DO m1 = 1, M
+    opt(m1)%Skip_Profile = .FALSE.
+    opt(m1)%Use_Emissivity = .FALSE.
+    sfc(m1)%Wind_Speed = 4.0_fp + REAL(m1, fp)
+    sfc(m1)%Wind_Direction = 30.0_fp * REAL(m1, fp)
+    sfc(m1)%Salinity = 34.0_fp
+    geo(m1)%Sensor_Azimuth_Angle = 100.0_fp
+  END DO
+  CALL CRTM_Atmosphere_Zero(atm_k)
+  CALL CRTM_Surface_Zero(sfc_k)
+  rts_k(:, :)%Brightness_Temperature = 1.0_fp
+  rts_k(:, :)%Radiance = 0.0_fp
+  err = CRTM_K_Matrix(atm, sfc, rts_k, geo, chinfo, atm_k, sfc_k, rts, opt)
+  IF (err /= SUCCESS) STOP 'CRTM_K_Matrix with computed surface optics failed'
+  OPEN(NEWUNIT=u, FILE=TRIM(fout)//'.sfc', STATUS='REPLACE', ACTION='WRITE')
+  DO m1 = 1, M
+    K = atm(m1)%n_Layers
+    WRITE(u, '(*(ES25.16E3))') (rts(l1, m1)%Brightness_Temperature, l1 = 1, L)
+    WRITE(u, '(*(ES25.16E3))') (rts(l1, m1)%Surface_Emissivity, l1 = 1, L)
+    WRITE(u, '(*(ES25.16E3))') (rts(l1, m1)%Surface_Reflectivity, l1 = 1, L)
+    WRITE(u, '(*(ES25.16E3))') (sfc_k(l1, m1)%Water_Temperature, l1 = 1, L)
+    WRITE(u, '(*(ES25.16E3))') (sfc_k(l1, m1)%Wind_Speed, l1 = 1, L)
+    DO k1 = 1, K
+      WRITE(u, '(*(ES25.16E3))') (atm_k(l1, m1)%Temperature(k1), l1 = 1, L)
+    END DO
   END DO
   CLOSE(u)
   err = CRTM_Destroy(chinfo)

@@ -403,14 +403,16 @@ CONTAINS
     END DO
   END SUBROUTINE CRTM_Atmosphere_Zero
 
-  ! CRTM_Surface_Zero (src/Surface/CRTM_Surface_Define.f90): zero the
-  ! coverages and temperatures of a Surface_K array before CRTM_K_Matrix
+  ! CRTM_Surface_Zero (src/Surface/CRTM_Surface_Define.f90:461-479, with
+  ! CRTM_WaterSurface_Zero :1904-1911): zero the coverages, temperatures,
+  ! salinity and wind of a Surface_K array before CRTM_K_Matrix
   SUBROUTINE CRTM_Surface_Zero(Sfc)
     TYPE(CRTM_Surface_type), INTENT(INOUT) :: Sfc(:, :)
     Sfc(:, :)%Land_Coverage = 0; Sfc(:, :)%Water_Coverage = 0
     Sfc(:, :)%Snow_Coverage = 0; Sfc(:, :)%Ice_Coverage = 0
     Sfc(:, :)%Land_Temperature = 0; Sfc(:, :)%Water_Temperature = 0
     Sfc(:, :)%Snow_Temperature = 0; Sfc(:, :)%Ice_Temperature = 0
+    Sfc(:, :)%Wind_Speed = 0; Sfc(:, :)%Wind_Direction = 0; Sfc(:, :)%Salinity = 0
   END SUBROUTINE CRTM_Surface_Zero
 
   FUNCTION CRTM_Forward(Atmosphere, Surface, Geometry, ChannelInfo, RTSolution, Options) RESULT(Error_Status)

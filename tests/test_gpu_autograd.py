@@ -102,3 +102,60 @@ def test_gradient_equals_oracle_adjoint(setup, coeff_dir, oracle_mod):
         np.testing.assert_allclose(A.grad[:, a:b].cpu().numpy(), ref["abs_k"].sum(0), rtol=1e-9,
                                    atol=1e-12 * np.abs(ref["abs_k"]).max())
         np.testing.assert_allclose(E.grad[m].cpu().numpy(), ref["emis_k"], rtol=1e-9, atol=1e-14)
+
+
+def test_gradient_with_computed_surface_optics(coeff_dir, oracle_mod):
+    """emissivity=None: the FASTEM6 optics are computed and the cost's
+    gradient reaches wind speed, wind direction and salinity; it equals the
+    channel sum of the oracle's Surface_K and central differences of the
+    GPU forward model."""
+    import torch
+    from crtmv3_amd import api, coeffio, synthetic
+    from crtmv3_amd.autograd import crtm_bt
+    print("autograd with computed surface optics", flush=True)
+    synthetic.write_mwwater_coeff(coeff_dir)
+    synthetic.write_sensor(coeff_dir, "atms_syn")
+    ctx = api.crtm_init(["atms_syn"], coeff_dir)
+    try:
+        dev = torch.device("cuda:0")
+        ps = synthetic.ocean_surface(synthetic.make_profiles(3, [100, 90, 100], seed=33), seed=34)
+        dps = api.device_profiles(ps, dev)
+        L = ctx.n_channels
+        y = torch.full((3, L), 230.0, dtype=torch.float64, device=dev)
+        W = dps.wind_speed.clone().requires_grad_(True)
+        D = dps.wind_direction.clone().requires_grad_(True)
+        S = dps.salinity.clone().requires_grad_(True)
+
+        def cost(w, d, s_, prec=api.PREC_FP64):
+            bt = crtm_bt(ctx, dps, dps.temperature, dps.absorber, dps.water_temperature, None, precision=prec,
+                         wind_speed=w, wind_direction=d, salinity=s_)
+            return (((y - bt) / 0.5) ** 2).sum(), bt
+
+        J, bt = cost(W, D, S)
+        J.backward()
+        torch.cuda.synchronize()
+        tau, spc = load_sensor(coeff_dir, "atms_syn")
+        Sn = oracle_mod.Sensor(tau, spc)
+        mw = oracle_mod.MWwater(coeffio.read_mwwatercoeff(f"{coeff_dir}/{synthetic.MWWATERCOEFF_FILE}"))
+        seed = (-2.0 * (y - bt) / 0.25).detach().cpu().numpy()
+        for m in range(3):
+            so = oracle_mod.SurfaceOptics(spc, mw=mw, salinity=ps.salinity[m], wind_speed=ps.wind_speed[m],
+                                          wind_direction=ps.wind_direction[m], sensor_azimuth=ps.sensor_azimuth[m])
+            r = oracle_mod.run_profile_sfc(Sn, ps, m, so, "k", bt_k=seed[m])
+            want = r["sfc_k"].sum(0)   # salinity, wind speed, wind direction
+            got = np.array([S.grad[m].item(), W.grad[m].item(), D.grad[m].item()])
+            np.testing.assert_allclose(got, want, rtol=1e-8, atol=1e-10 * np.abs(want).max())
+        for which, h in (("wind", 1e-4), ("dir", 1e-3)):
+            with torch.no_grad():
+                wp, wm, dp, dm = W.detach().clone(), W.detach().clone(), D.detach().clone(), D.detach().clone()
+                if which == "wind":
+                    wp[1] += h
+                    wm[1] -= h
+                else:
+                    dp[1] += h
+                    dm[1] -= h
+                fd = (cost(wp, dp, S.detach())[0] - cost(wm, dm, S.detach())[0]) / (2 * h)
+            got = (W.grad if which == "wind" else D.grad)[1].item()
+            assert abs(got - fd.item()) <= 1e-5 * max(abs(fd.item()), 1e-6), (which, got, fd.item())
+    finally:
+        ctx.destroy()

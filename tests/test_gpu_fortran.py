@@ -2,7 +2,9 @@
 CRTM_Init / CRTM_Forward / CRTM_K_Matrix / CRTM_Adjoint /
 CRTM_Tangent_Linear / CRTM_Destroy the way a CRTM application does, through
 the ISO_C_BINDING layer, and its results match the FP64 oracle; a call with
-Options%Skip_Profile set on one profile zeroes that profile only."""
+Options%Skip_Profile set on one profile zeroes that profile only, and a
+K-matrix without Options%Use_Emissivity computes the water surface optics
+(the IR sea-surface emissivity table) as the oracle does."""
 from __future__ import annotations
 
 import os
@@ -52,6 +54,7 @@ def test_fortran_driver_matches_oracle(coeff_dir, oracle_mod, tmp_path, prec):
     name = "cris_small_syn"
     ps = synthetic.make_profiles(3, [100, 77, 120], seed=55)
     tau, spc = load_sensor(coeff_dir, name)
+    synthetic.write_irwater_coeff(coeff_dir)
     L = tau.n_channels
     eps = np.random.default_rng(1).uniform(0.9, 0.99, (3, L))
     fin, fout = str(tmp_path / "in.txt"), str(tmp_path / "out.txt")
@@ -96,3 +99,20 @@ def test_fortran_driver_matches_oracle(coeff_dir, oracle_mod, tmp_path, prec):
     for m in range(2):
         np.testing.assert_array_equal(bt_skip[m], got[m]["bt_fwd"])
 
+    # computed surface optics (the driver's wind / salinity / azimuth)
+    from crtmv3_amd import coeffio
+    ir = oracle_mod.IRwater(coeffio.read_irwatercoeff(os.path.join(coeff_dir, synthetic.IRWATERCOEFF_FILE)))
+    vals = iter(float(v) for v in open(fout + ".sfc").read().split())
+    for m in range(3):
+        K = int(ps.n_layers[m])
+        take = lambda n: np.array([next(vals) for _ in range(n)])
+        bt, emis, refl, tsk, wsk, tk = take(L), take(L), take(L), take(L), take(L), take(K * L).reshape(K, L).T
+        so = oracle_mod.SurfaceOptics(spc, ir=ir, salinity=34.0, wind_speed=4.0 + (m + 1),
+                                      wind_direction=30.0 * (m + 1), sensor_azimuth=100.0)
+        f = oracle_mod.run_profile_sfc(s, ps, m, so, "forward")
+        r = oracle_mod.run_profile_sfc(s, ps, m, so, "k")
+        assert np.abs(bt - f["bt"]).max() <= bt_tol
+        assert np.abs(emis - f["emis"]).max() <= (1e-12 if prec == 0 else 1e-6)
+        assert np.abs(refl - f["refl"]).max() <= (1e-12 if prec == 0 else 1e-6)
+        for got_, want in ((tsk, r["sfc_t_k"][:, 1]), (wsk, r["sfc_k"][:, 1]), (tk, r["t_k"])):
+            assert np.abs(got_ - want).max() <= jac_tol * np.abs(want).max()
