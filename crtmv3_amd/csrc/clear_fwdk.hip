@@ -89,14 +89,15 @@ __device__ inline RT comp_layer_od(const float* ck, const int* npb, const RT* xk
 }
 
 // Workgroup shape per arithmetic type, from measurements at the benchmark
-// shape: FP32 runs best with 16 profiles per workgroup at 4 waves per SIMD
-// (<= 128 VGPRs); FP64 needs more registers and runs best with 8 profiles
-// and no wave-count target.
+// shape: two workgroups per CU (each about 75 KB of LDS), so one workgroup's
+// layer barrier and staged-load wait overlap the other's arithmetic.  FP32:
+// 8 profiles per workgroup at 4 waves per SIMD (<= 128 VGPRs); FP64 needs
+// more registers (2 waves per SIMD) and takes 4 profiles per workgroup.
 //   PW   profiles (waves) per workgroup, all on one 64-channel block
 //   WPE  waves per SIMD the register allocation targets (0: compiler's choice)
 // Each layer is staged in one of two LDS buffers while the other is read.
-template <class RT> struct KCfg { static constexpr int PW = 16, WPE = 4; };
-template <> struct KCfg<double> { static constexpr int PW = 8, WPE = 0; };
+template <class RT> struct KCfg { static constexpr int PW = 8, WPE = 4; };
+template <> struct KCfg<double> { static constexpr int PW = 4, WPE = 0; };
 constexpr int NBUF = 2;
 // LDS row of one wave: predictors X or the adjoint row, followed by the OPTRAN table O
 constexpr int TABW = (NX_MAX > NR_MAX ? NX_MAX : NR_MAX) + NO;
@@ -107,8 +108,8 @@ constexpr int PF = 4;                          // user-layer rows prefetched one
 
 // OPTRAN ln(chi) of one lane at one layer (Add_OPTRAN_wloOD,
 // ODPS_AtmAbsorption.f90:664-727): b_p = sum_q coef(p,q) Ap(q), lnchi = sum_p b_p OX(p)
-template <class RT, class OC>
-__device__ inline RT optran_lnchi(const OC& oc, const int* meta, const RT* ok) {
+template <class RT, class OC, class MP>
+__device__ inline RT optran_lnchi(const OC& oc, MP meta, const RT* ok) {
   const int nrow = meta[BM_ONROW], nq = meta[BM_OMAXO] + 1;
   RT lnchi = RT(0);
   for (int r = 0; r < nrow; ++r) {
@@ -147,7 +148,9 @@ clear_fwdk_kernel(DevSensor s, DevBatch bt, ProfWork<RT> w, DevOut o) {
   if (tix >= total) return;                 // uniform over the workgroup
   const int b = s.blk_order[(int)(tix / ng)];
   const int tid = threadIdx.x;
-  const int wv = tid >> 6;
+  // the wave index is wave-uniform: from an SGPR, every per-profile address
+  // below is uniform and its loads become scalar loads
+  const int wv = __builtin_amdgcn_readfirstlane(tid >> 6);
   const int lane = tid & (WAVE - 1);
   const int m_raw = (int)(tix % ng) * PW + wv;
   const bool in_batch = m_raw < bt.M;
@@ -202,7 +205,7 @@ clear_fwdk_kernel(DevSensor s, DevBatch bt, ProfWork<RT> w, DevOut o) {
   const int K = ok_prof ? bt.n_layers[m] : 0;
   const int64_t loff = bt.loff[m];
   const int Ko = s.Ko;
-  const int* meta = s.blk_meta + b * BM_N;
+  const cst_ptr<int> meta = as_const(s.blk_meta + b * BM_N);
   int npb[Tg::NCOMP];
 #pragma unroll
   for (int j = 0; j < Tg::NCOMP; ++j) npb[j] = meta[BM_NP0 + j];
@@ -267,9 +270,26 @@ clear_fwdk_kernel(DevSensor s, DevBatch bt, ProfWork<RT> w, DevOut o) {
   auto oc_g = [&](int i) -> RT { return (RT)ocg[i * WAVE + lane]; };
 
   // ------------- forward (1): ODPS layer OD and level-to-space path -------------
-  RT path[KO_MAX + 1];
-  uint64_t odclamp[2] = {0, 0}, lcclamp[2] = {0, 0};
-  path[0] = RT(0);
+  // Path at the user levels, interpolated (Interpolate_Profile) as soon as the
+  // two ODPS levels that bracket a level are known; the ODPS-level path itself
+  // is only carried from one layer to the next.
+  RT upth[K_MAX + 1];
+  RT pk = RT(0);
+  const cst_ptr<int> ui = as_const(w.uidx + (size_t)m * (bt.max_k + 1));
+  const cst_ptr<RT> uf = as_const(w.ufrac + (size_t)m * (bt.max_k + 1));
+  const int nup = ok_prof ? bt.n_layers[m] + 1 : 0;
+  int iup = 0;
+  int ui_n = nup > 0 ? ui[0] : Ko + 1;
+  // clamp bitmasks of layers 0..63 and 64..127, kept in registers (a
+  // dynamically indexed array would live in scratch memory)
+  uint64_t odc_lo = 0, odc_hi = 0, lcc_lo = 0, lcc_hi = 0;
+  auto setbit = [](uint64_t& lo, uint64_t& hi, int k) {
+    if (k < 64) lo |= 1ull << k;
+    else hi |= 1ull << (k - 64);
+  };
+  auto getbit = [](uint64_t lo, uint64_t hi, int k) -> bool {
+    return ((k < 64 ? lo : hi) >> (k & 63)) & 1;
+  };
   cload(0, 0);
   rload(X, Tg::NX, 0, 0);
   stage_wait();
@@ -289,10 +309,10 @@ clear_fwdk_kernel(DevSensor s, DevBatch bt, ProfWork<RT> w, DevOut o) {
         RT c;
         if (lnchi > RT(LIMIT_EXP)) {
           c = RT(LIMIT_LOG);
-          lcclamp[k >> 6] |= 1ull << (k & 63);
+          setbit(lcc_lo, lcc_hi, k);
         } else if (lnchi < RT(-LIMIT_EXP)) {
           c = RT(0);
-          lcclamp[k >> 6] |= 1ull << (k & 63);
+          setbit(lcc_lo, lcc_hi, k);
         } else {
           c = rexp<RT>(lnchi);
         }
@@ -302,31 +322,33 @@ clear_fwdk_kernel(DevSensor s, DevBatch bt, ProfWork<RT> w, DevOut o) {
     RT odc = od;
     if (od < RT(0)) {
       odc = RT(0);
-      odclamp[k >> 6] |= 1ull << (k & 63);
+      setbit(odc_lo, odc_hi, k);
     } else if (od > RT(MAX_OD)) {
       odc = RT(MAX_OD);
-      odclamp[k >> 6] |= 1ull << (k & 63);
+      setbit(odc_lo, odc_hi, k);
     }
-    path[k + 1] = path[k] + odc;
+    const RT pk1 = pk + odc;
+    while (ui_n == k) {   // user levels bracketed by ODPS levels k and k+1 (brackets never decrease)
+      const RT f = uf[iup];
+      upth[iup] = f < RT(0) ? pk : pk + (pk1 - pk) * f;
+      ++iup;
+      ui_n = iup < nup ? ui[iup] : Ko + 1;
+    }
+    pk = pk1;
     stage_wait();
     __syncthreads();
   }
 
+  for (; iup < nup; ++iup) upth[iup] = pk;   // below the lowest ODPS level
   // ------------- forward (2): user layers, Planck, Emission (top-down) -------------
-  const double* geo = w.geo + (size_t)m * 4;
+  const cst_ptr<double> geo = as_const(w.geo + (size_t)m * 4);
   const double sec_sfc = geo[0], cosu = geo[1], tsfc = geo[2];
   const RT isec = (RT)(1.0 / sec_sfc);
   const RT iu = (RT)(1.0 / cosu);
-  const int* ui = w.uidx + (size_t)m * (bt.max_k + 1);
-  const RT* uf = w.ufrac + (size_t)m * (bt.max_k + 1);
-  auto upath = [&](int i) -> RT {
-    const int k1 = ui[i];
-    const RT f = uf[i];
-    return f < RT(0) ? path[k1] : path[k1] + (path[k1 + 1] - path[k1]) * f;
-  };
+  auto upath = [&](int i) -> RT { return upth[i]; };
   const RT c1 = (RT)s.c1[lc], c2 = (RT)s.c2[lc], bc1 = (RT)s.bc1[lc], bc2 = (RT)s.bc2[lc];
   const RT ic1 = (RT)(1.0 / s.c1[lc]);
-  const double* tuser = bt.t + loff;
+  const cst_ptr<double> tuser = as_const(bt.t + loff);
   RT rd = (RT)s.cosmic[lc];
   RT uacc = RT(0), tc = RT(1);
   const RT up0 = ok_prof ? upath(0) : RT(0);
@@ -406,7 +428,7 @@ clear_fwdk_kernel(DevSensor s, DevBatch bt, ProfWork<RT> w, DevOut o) {
   if constexpr (RED) {   // surface temperature adjoints [M]: channel sums
     if (ok_prof) {
       const double bs_seed = (seeded && valid) ? o.ks.bs[sl] : 0.0;
-      const RT dbsdt = c2 * bc2 * Bs * (Bs + c1) * ic1 / (tes * tes);
+      const RT dbsdt = c2 * bc2 * Bs * (Bs + c1) * ic1 * rrcp<RT>(tes * tes);
       const double sad = wave_sum(valid ? (double)(dbsdt * (eps * upad_sfc + (RT)bs_seed)) : 0.0);
       if (lane == 0)
         for (int c = 0; c < 4; ++c)
@@ -430,7 +452,7 @@ clear_fwdk_kernel(DevSensor s, DevBatch bt, ProfWork<RT> w, DevOut o) {
   } else if (valid) {
     const double e_seed = seeded ? o.ks.emis[sl] : 0.0, bs_seed = seeded ? o.ks.bs[sl] : 0.0;
     if (o.emis_k) o.emis_k[ol] = sfc_comp ? 0.0 : (double)(upad_sfc * Bs - upad_sfc * rd) + e_seed;
-    const RT dbsdt = c2 * bc2 * Bs * (Bs + c1) * ic1 / (tes * tes);
+    const RT dbsdt = c2 * bc2 * Bs * (Bs + c1) * ic1 * rrcp<RT>(tes * tes);
     const double sad = (double)(dbsdt * (eps * upad_sfc + (RT)bs_seed));
     for (int c = 0; c < 4; ++c)
       if (o.sfc_t_k[c]) o.sfc_t_k[c][ol] = bt.cov[c][m] * sad + (c == 1 ? sg[0] : 0.0);
@@ -506,7 +528,7 @@ clear_fwdk_kernel(DevSensor s, DevBatch bt, ProfWork<RT> w, DevOut o) {
       gd -= B * (RT(1) - t) * dn;
       dn *= t;
       ru = ru * t + B * (RT(1) - t);
-      const RT dbdt = c2 * bc2 * B * (B + c1) * ic1 / (te * te);
+      const RT dbdt = c2 * bc2 * B * (B + c1) * ic1 * rrcp<RT>(te * te);
       if constexpr (RED) {
         if (k >= nx) red_add(tad + k, dbdt * b_ad + t_seed);   // wave-uniform: every lane takes part
       } else if (valid && k >= nx) {
@@ -543,8 +565,8 @@ clear_fwdk_kernel(DevSensor s, DevBatch bt, ProfWork<RT> w, DevOut o) {
     }
   }
   RT pad_next = ok_prof ? pad[Ko] : RT(0);
-  const int* acc_i = w.acc_i + (size_t)m * Ko * 3;
-  const RT* acc_w = w.acc_w + (size_t)m * NZ_MAX;
+  const cst_ptr<int> acc_i = as_const(w.acc_i + (size_t)m * Ko * 3);
+  const cst_ptr<RT> acc_w = as_const(w.acc_w + (size_t)m * NZ_MAX);
   RT tw[MW], aw[MW][NAW];
 #pragma unroll
   for (int q = 0; q < MW; ++q) {
@@ -557,8 +579,8 @@ clear_fwdk_kernel(DevSensor s, DevBatch bt, ProfWork<RT> w, DevOut o) {
   // pf_lo / pf_lo_n), so the read does not stall the synchronized waves.
   RT pfc[PF], pfn[PF];
   int pf_lo = 0, pf_lo_n = 0;
-  const int* frange = w.frange + (size_t)m * Ko * 2;
-  const RT* uwin = w.uwin + (size_t)m * bt.max_k * MW;
+  const cst_ptr<int> frange = as_const(w.frange + (size_t)m * Ko * 2);
+  const cst_ptr<RT> uwin = as_const(w.uwin + (size_t)m * bt.max_k * MW);
   auto final_lo = [&](int k) { return frange[2 * k]; };
   auto prefetch = [&](int k) {   // rows finalized at ODPS layer k, into pfn
     pf_lo_n = final_lo(k);
@@ -571,7 +593,7 @@ clear_fwdk_kernel(DevSensor s, DevBatch bt, ProfWork<RT> w, DevOut o) {
   auto finish_user_layer = [&](int iu) {   // completed at the ODPS layer in window slot 0
     if constexpr (RED) {   // the Planck part was summed in the Emission_AD pass
       if (iu < nx) return;                         // wave-uniform
-      const RT* wr = uwin + (size_t)iu * MW;
+      const cst_ptr<RT> wr = uwin + (size_t)iu * MW;
       RT st = RT(0), sa[NAW];
 #pragma unroll
       for (int ju = 0; ju < NAW; ++ju) sa[ju] = RT(0);
@@ -591,14 +613,14 @@ clear_fwdk_kernel(DevSensor s, DevBatch bt, ProfWork<RT> w, DevOut o) {
     if (!valid || iu < nx) return;
     RT st;                                         // Planck part from the Emission_AD pass
     if ((unsigned)(iu - pf_lo) < (unsigned)PF) {
+      const int d = iu - pf_lo;
       st = pfc[0];
 #pragma unroll
-      for (int q = 1; q < PF; ++q)
-        if (iu - pf_lo == q) st = pfc[q];
+      for (int q = 1; q < PF; ++q) st = d == q ? pfc[q] : st;   // value selects keep pfc in registers
     } else {
       st = (RT)tko[(int64_t)iu * Lt];
     }
-    const RT* wr = uwin + (size_t)iu * MW;         // weights from ODPS layers in slots 0..MW-1
+    const cst_ptr<RT> wr = uwin + (size_t)iu * MW;   // weights from ODPS layers in slots 0..MW-1
     RT sa[NAW];
 #pragma unroll
     for (int ju = 0; ju < NAW; ++ju) sa[ju] = RT(0);
@@ -646,7 +668,7 @@ clear_fwdk_kernel(DevSensor s, DevBatch bt, ProfWork<RT> w, DevOut o) {
     if (ok_prof) {
       S += pad_next;
       pad_next = k > 0 ? pad[k] : RT(0);
-      const RT od_ad = ((odclamp[k >> 6] >> (k & 63)) & 1) ? RT(0) : S;
+      const RT od_ad = getbit(odc_lo, odc_hi, k) ? RT(0) : S;
       const RT* uk = tab[wv][k % NBUF];
       const float* ck = cs[k % NBUF] + lane;
       RT y[Qi::NQ];
@@ -658,7 +680,7 @@ clear_fwdk_kernel(DevSensor s, DevBatch bt, ProfWork<RT> w, DevOut o) {
         // Add_OPTRAN_wloOD_AD (ODPS_AtmAbsorption.f90:911-984) with chi recomputed
         const RT* okr = uk + RR::NR;
         const RT lnchi = oc_lds ? optran_lnchi<RT>(oc_l, meta, okr) : optran_lnchi<RT>(oc_g, meta, okr);
-        const bool lcc = (lcclamp[k >> 6] >> (k & 63)) & 1;
+        const bool lcc = getbit(lcc_lo, lcc_hi, k);
         const RT chi = lane_opt ? (lcc ? (lnchi > RT(0) ? RT(LIMIT_LOG) : RT(0)) : rexp<RT>(lnchi)) : RT(0);
         const RT da_ad = lane_opt ? od_ad * chi : RT(0);
         const RT lc_ad = (lane_opt && !lcc) ? chi * od_ad * okr[O_DA] : RT(0);
@@ -677,8 +699,8 @@ clear_fwdk_kernel(DevSensor s, DevBatch bt, ProfWork<RT> w, DevOut o) {
             ap1_ad += lc_ad * okr[O_X0 + p] * dbp;
             t_ad += xa * okr[O_DX_DT0 + p];
             vap_ad += xa * okr[O_DX_DV0 + p];
-            if (p == 12) e12 += xa;
-            if (p == 13) e13 += xa;
+            e12 += p == 12 ? xa : RT(0);   // selects, not a pointer to one of the two
+            e13 += p == 13 ? xa : RT(0);
           }
         };
         if (oc_lds) rows_ad(oc_l);
@@ -700,7 +722,11 @@ clear_fwdk_kernel(DevSensor s, DevBatch bt, ProfWork<RT> w, DevOut o) {
       RT a_ad[Tg::NABS];
 #pragma unroll
       for (int j = 0; j < Tg::NABS; ++j) a_ad[j] = y[Qi::RA0 + j] * od_ad;
-      if (s.optran) a_ad[h2o_j] += vap_ad;
+      if (s.optran) {   // compile-time indices keep a_ad in registers
+#pragma unroll
+        for (int j = 0; j < Tg::NABS; ++j)
+          if (j == h2o_j) a_ad[j] += vap_ad;
+      }
       // the integrated ratios accumulate from the top, so their adjoints
       // carry down the sweep (ODPS_Predictor.f90:707-759 reversed)
       const RT pdp = uk[RR::PDP];
@@ -739,7 +765,7 @@ clear_fwdk_kernel(DevSensor s, DevBatch bt, ProfWork<RT> w, DevOut o) {
       } else if (RED) {
         // wide supports or more absorbers, adjoint: channel sums per row
         const int i1 = acc_i[3 * k], n = acc_i[3 * k + 1];
-        const RT* awk = acc_w + acc_i[3 * k + 2];
+        const cst_ptr<RT> awk = acc_w + acc_i[3 * k + 2];
         for (int q = 0; q < n; ++q)
           if (i1 + q >= nx) red_add(tad + i1 + q, awk[q] * t_ad);
 #pragma unroll
@@ -755,7 +781,7 @@ clear_fwdk_kernel(DevSensor s, DevBatch bt, ProfWork<RT> w, DevOut o) {
         // returnless atomic add retires without a load round trip (same-lane,
         // same-address operations stay in program order).
         const int i1 = acc_i[3 * k], n = acc_i[3 * k + 1];
-        const RT* awk = acc_w + acc_i[3 * k + 2];
+        const cst_ptr<RT> awk = acc_w + acc_i[3 * k + 2];
         for (int q = 0; q < n; ++q)
           if (i1 + q >= nx) unsafeAtomicAdd(tko + (int64_t)(i1 + q) * Lt, (KT)(awk[q] * t_ad));
 #pragma unroll

@@ -25,16 +25,32 @@ constexpr int OPO = 11;            // OPTRAN polynomial orders 0..10
 // kernel computes it, its outputs are zero and it does not fail the call
 constexpr int ST_SKIP = 7;
 
+// Read-only view of data no kernel writes while it runs, through the
+// constant address space: loads at wave-uniform addresses become scalar
+// loads (scalar cache, counted apart from the vector-memory counter the
+// staged layer loads use), the others stay vector loads.
+template <class T> using cst_ptr = const __attribute__((address_space(4))) T*;
+template <class T> __device__ inline cst_ptr<T> as_const(const T* p) { return (cst_ptr<T>)p; }
+
 template <class RT> __device__ inline RT rexp(RT x);
 template <> __device__ inline float rexp<float>(float x) { return __expf(x); }
 template <> __device__ inline double rexp<double>(double x) { return exp(x); }
 
+// Reciprocal: the hardware's 1-ulp v_rcp_f32 in FP32 (the FP32 path's error
+// budget is far above an ulp), a correctly rounded division in FP64.
+template <class RT> __device__ inline RT rrcp(RT x);
+template <> __device__ inline float rrcp<float>(float x) { return __builtin_amdgcn_rcpf(x); }
+template <> __device__ inline double rrcp<double>(double x) { return 1.0 / x; }
+
 // Planck radiance B = c1 / (exp(c2/Te) - 1), Te = bc1 + bc2*T
-// (CRTM_Planck_Functions.f90:116-136); expm1 keeps the microwave case
-// (c2/Te ~ 0.01) accurate in FP32.
+// (CRTM_Planck_Functions.f90:116-136).  In FP32, expm1 keeps the microwave
+// case (c2/Te ~ 0.01) accurate; where c2/Te > 0.5 (infrared) exp(x) - 1
+// loses at most a factor 2.5 to cancellation and the fast exponential serves.
 __device__ inline float planck(float c1, float c2, float bc1, float bc2, double t, float& te) {
   te = bc1 + bc2 * (float)t;
-  return __fdividef(c1, expm1f(__fdividef(c2, te)));
+  const float x = c2 * __builtin_amdgcn_rcpf(te);
+  const float em1 = x > 0.5f ? __expf(x) - 1.0f : expm1f(x);
+  return c1 * __builtin_amdgcn_rcpf(em1);
 }
 __device__ inline double planck(double c1, double c2, double bc1, double bc2, double t, double& te) {
   te = bc1 + bc2 * t;
