@@ -50,14 +50,38 @@ __device__ inline void sens_q(RT c, const RT* dr, RT* y) {
     sens_q<G, RT, S, E + (on ? 1 : 0), Q + 1>(c, dr, y);
   }
 }
+// Slots are taken in chunks of SCH: the chunk's coefficients are read from
+// LDS first and the scheduling barrier keeps the reads together, so one
+// wait covers several reads instead of one read pair per wait.
+constexpr int SCH = 4;
+template <int G, class RT, int S, int I, int N>
+__device__ inline void sens_load(const float* ck, RT* c) {
+  if constexpr (I < N) {
+    if constexpr (SensPattern<G>::tab.mask[S + I] != 0u) c[I] = (RT)ck[(S + I) * WAVE];
+    sens_load<G, RT, S, I + 1, N>(ck, c);
+  }
+}
+template <int G, class RT, int S, int I, int N>
+__device__ inline void sens_use(const RT* c, const RT* dr, RT* y) {
+  if constexpr (I < N) {
+    if constexpr (SensPattern<G>::tab.mask[S + I] != 0u)
+      sens_q<G, RT, S + I, SensPattern<G>::tab.off[S + I], 0>(c[I], dr, y);
+    sens_use<G, RT, S, I + 1, N>(c, dr, y);
+  }
+}
 template <int G, class RT, int S, int SEND>
 __device__ inline void sens_slots(const float* ck, const RT* dr, RT* y) {
   if constexpr (S < SEND) {
-    if constexpr (SensPattern<G>::tab.mask[S] != 0u) {
-      const RT c = (RT)ck[S * WAVE];
-      sens_q<G, RT, S, SensPattern<G>::tab.off[S], 0>(c, dr, y);
-    }
-    sens_slots<G, RT, S + 1, SEND>(ck, dr, y);
+    constexpr int N = SEND - S < SCH ? SEND - S : SCH;
+    constexpr int E0 = SensPattern<G>::tab.off[S], NE = SensPattern<G>::tab.off[S + N] - E0;
+    RT c[SCH];
+    RT dv[NE > 0 ? NE : 1];
+    sens_load<G, RT, S, 0, N>(ck, c);
+#pragma unroll
+    for (int e = 0; e < NE; ++e) dv[e] = dr[E0 + e];
+    __builtin_amdgcn_sched_barrier(0);
+    sens_use<G, RT, S, 0, N>(c, dv - E0, y);
+    sens_slots<G, RT, S + N, SEND>(ck, dr, y);
   }
 }
 template <int G, class RT, int J>

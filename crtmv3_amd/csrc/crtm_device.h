@@ -45,11 +45,11 @@ template <> __device__ inline double rrcp<double>(double x) { return 1.0 / x; }
 // Planck radiance B = c1 / (exp(c2/Te) - 1), Te = bc1 + bc2*T
 // (CRTM_Planck_Functions.f90:116-136).  In FP32, expm1 keeps the microwave
 // case (c2/Te ~ 0.01) accurate; where c2/Te > 0.5 (infrared) exp(x) - 1
-// loses at most a factor 2.5 to cancellation and the fast exponential serves.
+// loses at most a factor 2.5 to cancellation and the shorter expf serves.
 __device__ inline float planck(float c1, float c2, float bc1, float bc2, double t, float& te) {
   te = bc1 + bc2 * (float)t;
   const float x = c2 * __builtin_amdgcn_rcpf(te);
-  const float em1 = x > 0.5f ? __expf(x) - 1.0f : expm1f(x);
+  const float em1 = x > 0.5f ? expf(x) - 1.0f : expm1f(x);
   return c1 * __builtin_amdgcn_rcpf(em1);
 }
 __device__ inline double planck(double c1, double c2, double bc1, double bc2, double t, double& te) {
