@@ -101,6 +101,22 @@ class CADOutput(ctypes.Structure):
 
 _lib = None
 
+N_TILES = 3   # tile shapes of the fused clear-sky kernel (CRTM_N_TILES)
+
+
+def tile_profiles(precision: int, tile: int) -> int:
+    """Profiles per workgroup of a tile index (crtm_tile_profiles)."""
+    return int(lib().crtm_tile_profiles(int(precision), int(tile)))
+
+
+def _tile_key(sensor_id: str, precision: int, n_profiles: int) -> str:
+    try:
+        import torch
+        dev = torch.cuda.get_device_name(0) if torch.cuda.is_available() else "cpu"
+    except Exception:
+        dev = "unknown"
+    return f"{dev}|{sensor_id}|prec{int(precision)}|M{int(n_profiles)}"
+
 
 def lib():
     """Load the native library; fails loudly if it has not been built."""
@@ -126,6 +142,10 @@ def lib():
         L.crtm_set_stream.argtypes = [_vp, _vp, ctypes.c_int]
         L.crtm_set_wavefront_angles.argtypes = [_vp, ctypes.c_int]
         L.crtm_set_cloudy_layer_capacity.argtypes = [_vp, ctypes.c_int64]
+        L.crtm_tile_profiles.argtypes = [ctypes.c_int, ctypes.c_int]
+        L.crtm_set_tile.argtypes = [_vp, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        L.crtm_get_tile.argtypes = [_vp, ctypes.c_int, ctypes.c_int, _ip]
+        L.crtm_autotune_k_matrix.argtypes = L.crtm_k_matrix.argtypes + [ctypes.c_int, _dp, _ip]
         L.crtm_last_error.argtypes = [_vp]
         L.crtm_last_error.restype = ctypes.c_char_p
         L.crtm_profile_enable.argtypes = [_vp, ctypes.c_int]
@@ -204,6 +224,62 @@ class Context:
         (matrix-core) scattering kernels, smaller ones on the per-lane kernels."""
         if lib().crtm_set_wavefront_angles(self._h, int(min_angles)) != SUCCESS:
             self._err("crtm_set_wavefront_angles")
+
+    def set_tile(self, sensor: int, precision: int, tile: int):
+        """Tile index (0..N_TILES-1) of the fused clear-sky kernel for one
+        sensor and precision: 64 channels x tile_profiles(precision, tile)
+        profiles per workgroup."""
+        if lib().crtm_set_tile(self._h, int(sensor), int(precision), int(tile)) != SUCCESS:
+            self._err("crtm_set_tile")
+
+    def get_tile(self, sensor: int, precision: int) -> int:
+        t = ctypes.c_int()
+        if lib().crtm_get_tile(self._h, int(sensor), int(precision), ctypes.byref(t)) != SUCCESS:
+            self._err("crtm_get_tile")
+        return t.value
+
+    def autotune(self, prof, emissivity, precision=PREC_FP32_SPLIT, repeats=3, jacobian_f32=True, cache=None,
+                 **kw):
+        """Tile-shape autotuner (crtm_autotune_k_matrix): times the fused
+        clear-sky kernel of every sensor at each tile on this batch and keeps
+        the fastest tile per sensor at `precision`.  Returns {"ms": [n_sensors]
+        [N_TILES] mean kernel times, "best": [n_sensors] tile indices,
+        "profiles_per_workgroup": [n_sensors]}.  cache: a JSON file that
+        records the choice under (device, sensor id, precision, profiles,
+        channels); apply_tile_cache() applies it in a later process."""
+        ns = len(self.channel_info)
+        ms = np.zeros((ns, N_TILES), np.float64)
+        best = np.zeros(ns, np.int32)
+        self.k_matrix(prof, emissivity, precision=precision, jacobian_f32=jacobian_f32,
+                      _tune=(int(repeats), ms, best), **kw)
+        rep = {"ms": ms.tolist(), "best": best.tolist(),
+               "profiles_per_workgroup": [tile_profiles(precision, int(b)) for b in best]}
+        if cache is not None:
+            import json
+            try:
+                d = json.load(open(cache))
+            except (OSError, ValueError):
+                d = {}
+            for i, ci in enumerate(self.channel_info):
+                d[_tile_key(ci.sensor_id, precision, prof.n_profiles)] = int(best[i])
+            with open(cache, "w") as f:
+                json.dump(d, f, indent=1, sort_keys=True)
+        return rep
+
+    def apply_tile_cache(self, cache, precision, n_profiles) -> bool:
+        """Set the tiles an earlier autotune() recorded in `cache` for these
+        sensors at this precision and batch size; False if any is missing."""
+        import json
+        try:
+            d = json.load(open(cache))
+        except (OSError, ValueError):
+            return False
+        keys = [_tile_key(ci.sensor_id, precision, n_profiles) for ci in self.channel_info]
+        if not all(k in d for k in keys):
+            return False
+        for i, k in enumerate(keys):
+            self.set_tile(i, precision, int(d[k]))
+        return True
 
     def set_cloudy_layer_capacity(self, max_layers: int):
         """Size the scattering launches for max_layers scattering layers of a
@@ -357,7 +433,7 @@ class Context:
 
     def k_matrix(self, prof, emissivity, bt_k=None, rad_k=None, precision=PREC_FP64,
                  jacobian_f32=False, check_input=True, out=None, n_streams=0, overlap=0, extras=False,
-                 no_alloc=False, include_scattering=True, skip_profile=None):
+                 no_alloc=False, include_scattering=True, skip_profile=None, _tune=None):
         """CRTM_K_Matrix: returns (RTSolution dict, K dict).
 
         K layout: temperature_k [NT, L] (profile m rows layer_offset[m]:...),
@@ -420,8 +496,14 @@ class Context:
             kd["radiance_k"] = rad_k.clone() if _is_torch(rad_k) else np.array(rad_k, np.float64)
         rts = CRTSolution(**{k: _ptr(v) for k, v in rts_d.items()})
         ko = CKOutput(brightness_temperature_k=_ptr(bt_k), **{k: _ptr(v) for k, v in kd.items()})
-        entry = "crtm_k_matrix_noalloc" if no_alloc else "crtm_k_matrix"
-        st = getattr(lib(), entry)(self._h, atm, sfc, geo, opt, rts, ko)
+        if _tune is not None:   # autotune(): the same call, run per tile
+            entry = "crtm_autotune_k_matrix"
+            reps, ms, best = _tune
+            st = lib().crtm_autotune_k_matrix(self._h, atm, sfc, geo, opt, rts, ko, reps,
+                                              ms.ctypes.data_as(_dp), best.ctypes.data_as(_ip))
+        else:
+            entry = "crtm_k_matrix_noalloc" if no_alloc else "crtm_k_matrix"
+            st = getattr(lib(), entry)(self._h, atm, sfc, geo, opt, rts, ko)
         if st != SUCCESS:
             self._err(entry)
         return rts_d, kd

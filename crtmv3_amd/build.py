@@ -18,9 +18,12 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OBJ = os.path.join(HERE, "_obj")
 LIB = os.path.join(HERE, "libcrtm_mi355x.so")
-SOURCES = ["odps_prep.hip", "clear_fwdk.hip", "sfc_optics.hip", "allsky.hip", "allsky_ad.hip", "allsky_wide.hip",
-           "allsky_wide_ad.hip", "crtm_abi.cpp"]
-HEADERS = ["crtm_device.h", "sfc_optics.h", "allsky_common.h", "allsky_wide.h", os.path.join("..", "..", "include", "crtm_mi355x.h")]
+# clear_fwdk_pw0/pw2.hip instantiate the fused clear-sky kernel's other two
+# tile sets from clear_fwdk.hip, so the three compile in parallel
+SOURCES = ["odps_prep.hip", "clear_fwdk.hip", "clear_fwdk_pw0.hip", "clear_fwdk_pw2.hip", "sfc_optics.hip",
+           "allsky.hip", "allsky_ad.hip", "allsky_wide.hip", "allsky_wide_ad.hip", "crtm_abi.cpp"]
+HEADERS = ["crtm_device.h", "sfc_optics.h", "allsky_common.h", "allsky_wide.h", "pred_adjoint.h", "clear_fwdk.hip",
+           os.path.join("..", "..", "include", "crtm_mi355x.h")]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("CRTM_OFFLOAD_ARCH", "gfx950")

@@ -112,16 +112,15 @@ __device__ inline RT comp_layer_od(const float* ck, const int* npb, const RT* xk
   }
 }
 
-// Workgroup shape per arithmetic type, from measurements at the benchmark
-// shape: two workgroups per CU (each about 75 KB of LDS), so one workgroup's
-// layer barrier and staged-load wait overlap the other's arithmetic.  FP32:
-// 8 profiles per workgroup at 4 waves per SIMD (<= 128 VGPRs); FP64 needs
-// more registers (2 waves per SIMD) and takes 4 profiles per workgroup.
-//   PW   profiles (waves) per workgroup, all on one 64-channel block
-//   WPE  waves per SIMD the register allocation targets (0: compiler's choice)
-// Each layer is staged in one of two LDS buffers while the other is read.
-template <class RT> struct KCfg { static constexpr int PW = 8, WPE = 4; };
-template <> struct KCfg<double> { static constexpr int PW = 4, WPE = 0; };
+// Workgroup shape: one 64-channel block x PW profiles (one wave each).  PW is
+// a tile-shape choice per sensor and precision (crtm_set_tile /
+// crtm_autotune pick it at run time from TileSet<RT>); the default,
+// measured at the CrIS-like benchmark shape, runs two workgroups per CU so
+// that one workgroup's layer barrier overlaps the other's arithmetic.
+//   WPE  waves per SIMD the register allocation targets (0: compiler's choice):
+//        FP32 keeps <= 128 VGPRs (4 waves per SIMD); FP64 needs more registers.
+template <class RT> struct KCfg { static constexpr int WPE = 4; };
+template <> struct KCfg<double> { static constexpr int WPE = 0; };
 constexpr int NBUF = 2;
 // LDS row of one wave: predictors X or the adjoint row, followed by the OPTRAN table O
 constexpr int TABW = (NX_MAX > NR_MAX ? NX_MAX : NR_MAX) + NO;
@@ -150,13 +149,13 @@ __device__ inline RT optran_lnchi(const OC& oc, MP meta, const RT* ok) {
 // 1378-1396): each wave sums its 64 channels and adds the sum to the profile's
 // row (FP64 atomics across channel blocks), so no per-channel Jacobian is
 // stored; KT is double.
-template <int G, class RT, class KT, bool DOK, bool RED>
-__global__ void __attribute__((amdgpu_flat_work_group_size(KCfg<RT>::PW * WAVE, KCfg<RT>::PW * WAVE),
+template <int G, class RT, class KT, bool DOK, bool RED, int PW>
+__global__ void __attribute__((amdgpu_flat_work_group_size(PW * WAVE, PW * WAVE),
                                amdgpu_waves_per_eu(KCfg<RT>::WPE > 0 ? KCfg<RT>::WPE : 1)))
 clear_fwdk_kernel(DevSensor s, DevBatch bt, ProfWork<RT> w, DevOut o) {
   static_assert(!RED || (DOK && sizeof(KT) == 8), "the adjoint accumulates FP64 channel sums");
   using Tg = GT<G>;
-  constexpr int PW = KCfg<RT>::PW, TPB = PW * WAVE;
+  constexpr int TPB = PW * WAVE;
   // Workgroup = one 64-channel block x PW profiles (one wave each).  The
   // block's coefficients of the current layer are staged once in LDS and
   // read by all PW waves, so each coefficient leaves L2 once per PW
@@ -822,19 +821,26 @@ clear_fwdk_kernel(DevSensor s, DevBatch bt, ProfWork<RT> w, DevOut o) {
   }
 }
 
-template <int G, class RT, class KT, bool DOK, bool RED>
-void launch_fwdk(const DevSensor& s, const DevBatch& b, const ProfWork<RT>& w, const DevOut& o,
-                 hipStream_t stream) {
-  constexpr int PW = KCfg<RT>::PW;
+template <int G, class RT, class KT, bool DOK, bool RED, int PW>
+void launch_fwdk_pw(const DevSensor& s, const DevBatch& b, const ProfWork<RT>& w, const DevOut& o,
+                    hipStream_t stream) {
   const int64_t ng = (b.M + PW - 1) / PW;
   const int64_t n = 8 * ((s.n_blocks * ng + 7) / 8);
-  hipLaunchKernelGGL((clear_fwdk_kernel<G, RT, KT, DOK, RED>), dim3((unsigned)n), dim3(PW * WAVE), 0, stream, s, b,
-                     w, o);
+  hipLaunchKernelGGL((clear_fwdk_kernel<G, RT, KT, DOK, RED, PW>), dim3((unsigned)n), dim3(PW * WAVE), 0, stream,
+                     s, b, w, o);
 }
 
+// The three tile sets are instantiated in three translation units compiled in
+// parallel: this file (index 1, the default), clear_fwdk_pw0.hip and
+// clear_fwdk_pw2.hip, each defining CRTM_FWDK_TILE before including this one.
+#ifndef CRTM_FWDK_TILE
+#define CRTM_FWDK_TILE 1
+#define CRTM_FWDK_DISPATCH 1
+#endif
+
 #define INST(G, RT, KT, DOK, RED) \
-  template void launch_fwdk<G, RT, KT, DOK, RED>(const DevSensor&, const DevBatch&, const ProfWork<RT>&, \
-                                                 const DevOut&, hipStream_t);
+  template void launch_fwdk_pw<G, RT, KT, DOK, RED, TileSet<RT>::pw[CRTM_FWDK_TILE]>( \
+      const DevSensor&, const DevBatch&, const ProfWork<RT>&, const DevOut&, hipStream_t);
 #define INST_G(G)                                                                                     \
   INST(G, float, float, true, false) INST(G, float, double, true, false)                              \
   INST(G, float, float, false, false) INST(G, double, double, true, false)                            \
@@ -843,5 +849,40 @@ void launch_fwdk(const DevSensor& s, const DevBatch& b, const ProfWork<RT>& w, c
 INST_G(1)
 INST_G(2)
 INST_G(3)
+
+#ifdef CRTM_FWDK_DISPATCH
+// the other two tile sets come from their own translation units
+#define EXT(G, RT, KT, DOK, RED, T) \
+  extern template void launch_fwdk_pw<G, RT, KT, DOK, RED, TileSet<RT>::pw[T]>( \
+      const DevSensor&, const DevBatch&, const ProfWork<RT>&, const DevOut&, hipStream_t);
+#define EXT_G(G, T)                                                                                   \
+  EXT(G, float, float, true, false, T) EXT(G, float, double, true, false, T)                          \
+  EXT(G, float, float, false, false, T) EXT(G, double, double, true, false, T)                        \
+  EXT(G, double, float, true, false, T) EXT(G, double, double, false, false, T)                       \
+  EXT(G, float, double, true, true, T) EXT(G, double, double, true, true, T)
+EXT_G(1, 0) EXT_G(2, 0) EXT_G(3, 0)
+EXT_G(1, 2) EXT_G(2, 2) EXT_G(3, 2)
+// tile index -> instantiation (any other index: the default)
+template <int G, class RT, class KT, bool DOK, bool RED>
+void launch_fwdk(const DevSensor& s, const DevBatch& b, const ProfWork<RT>& w, const DevOut& o, hipStream_t stream,
+                 int tile) {
+  switch (tile) {
+    case 0: launch_fwdk_pw<G, RT, KT, DOK, RED, TileSet<RT>::pw[0]>(s, b, w, o, stream); break;
+    case 2: launch_fwdk_pw<G, RT, KT, DOK, RED, TileSet<RT>::pw[2]>(s, b, w, o, stream); break;
+    default: launch_fwdk_pw<G, RT, KT, DOK, RED, TileSet<RT>::pw[1]>(s, b, w, o, stream); break;
+  }
+}
+#define INSTD(G, RT, KT, DOK, RED) \
+  template void launch_fwdk<G, RT, KT, DOK, RED>(const DevSensor&, const DevBatch&, const ProfWork<RT>&, \
+                                                 const DevOut&, hipStream_t, int);
+#define INSTD_G(G)                                                                                    \
+  INSTD(G, float, float, true, false) INSTD(G, float, double, true, false)                            \
+  INSTD(G, float, float, false, false) INSTD(G, double, double, true, false)                          \
+  INSTD(G, double, float, true, false) INSTD(G, double, double, false, false)                         \
+  INSTD(G, float, double, true, true) INSTD(G, double, double, true, true)
+INSTD_G(1)
+INSTD_G(2)
+INSTD_G(3)
+#endif
 
 }  // namespace crtm

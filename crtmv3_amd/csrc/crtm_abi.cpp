@@ -27,7 +27,7 @@ void launch_add_layers(const DevBatch&, const ExtBatch&, const double* clim, con
 template <int G, class RT>
 void launch_prep(const DevSensor&, const DevBatch&, const ProfWork<RT>&, hipStream_t);
 template <int G, class RT, class KT, bool DOK, bool RED>
-void launch_fwdk(const DevSensor&, const DevBatch&, const ProfWork<RT>&, const DevOut&, hipStream_t);
+void launch_fwdk(const DevSensor&, const DevBatch&, const ProfWork<RT>&, const DevOut&, hipStream_t, int tile);
 void launch_cloud_prep(const DevBatch&, const DevClouds&, const CloudWork&, hipStream_t);
 template <class RT>
 void launch_allsky(const DevSensor&, const DevBatch&, const DevClouds&, const CloudWork&, const DevCloudCoeff&,
@@ -326,6 +326,9 @@ struct Sensor {
   SpcCoeffH sc;
   DevSensor d{};
   int n_blocks = 0;
+  int index = 0;               // position in the context's sensor list
+  int tile[2] = {1, 1};        // tile index of the fused clear-sky kernel: [FP64, FP32-split]
+  template <class RT> int tile_of() const { return tile[sizeof(RT) == 4 ? 1 : 0]; }
   std::vector<std::unique_ptr<DevBuf>> bufs;
   template <class T> const T* put(const std::vector<T>& v) {
     bufs.emplace_back(new DevBuf);
@@ -875,6 +878,7 @@ struct crtm_context {
   bool profiling = false;
   std::vector<hipEvent_t> ev;      // pairs (start, stop) per recorded launch
   std::vector<int> ev_kind;        // 0 = preparation kernel, 1 = forward/K kernel, 2 = scattering kernels
+  std::vector<int> ev_sensor;      // sensor of each recorded launch (-1: not sensor specific)
   size_t ev_used = 0;
   hipEvent_t mark() {
     if (ev_used == ev.size()) {
@@ -968,11 +972,13 @@ void run_fwdk(crtm_context* c, Sensor& S, const DevBatch& b, const ProfWork<RT>&
   if (c->profiling) {
     c->mark();
     c->ev_kind.push_back(1);
+    c->ev_sensor.push_back(S.index);
   }
+  const int tile = S.tile_of<RT>();
   switch (S.d.group) {
-    case 1: launch_fwdk<1, RT, KT, DOK, RED>(S.d, bb, w, o, c->stream); break;
-    case 2: launch_fwdk<2, RT, KT, DOK, RED>(S.d, bb, w, o, c->stream); break;
-    case 3: launch_fwdk<3, RT, KT, DOK, RED>(S.d, bb, w, o, c->stream); break;
+    case 1: launch_fwdk<1, RT, KT, DOK, RED>(S.d, bb, w, o, c->stream, tile); break;
+    case 2: launch_fwdk<2, RT, KT, DOK, RED>(S.d, bb, w, o, c->stream, tile); break;
+    case 3: launch_fwdk<3, RT, KT, DOK, RED>(S.d, bb, w, o, c->stream, tile); break;
   }
   if (c->profiling) c->mark();
   HIPCHECK(hipGetLastError());
@@ -994,22 +1000,24 @@ ProfWork<RT> run_sensor(crtm_context* c, Sensor& S, const DevBatch& b, const Dev
     if (c->profiling) {
       c->mark();
       c->ev_kind.push_back(kind);
+      c->ev_sensor.push_back(S.index);
     }
     fn();
     if (c->profiling) c->mark();
   };
+  const int tile = S.tile_of<RT>();
   switch (S.d.group) {
     case 1:
       timed(0, [&] { launch_prep<1, RT>(S.d, bb, w, c->stream); });
-      timed(1, [&] { launch_fwdk<1, RT, KT, DOK, RED>(S.d, bb, w, o, c->stream); });
+      timed(1, [&] { launch_fwdk<1, RT, KT, DOK, RED>(S.d, bb, w, o, c->stream, tile); });
       break;
     case 2:
       timed(0, [&] { launch_prep<2, RT>(S.d, bb, w, c->stream); });
-      timed(1, [&] { launch_fwdk<2, RT, KT, DOK, RED>(S.d, bb, w, o, c->stream); });
+      timed(1, [&] { launch_fwdk<2, RT, KT, DOK, RED>(S.d, bb, w, o, c->stream, tile); });
       break;
     case 3:
       timed(0, [&] { launch_prep<3, RT>(S.d, bb, w, c->stream); });
-      timed(1, [&] { launch_fwdk<3, RT, KT, DOK, RED>(S.d, bb, w, o, c->stream); });
+      timed(1, [&] { launch_fwdk<3, RT, KT, DOK, RED>(S.d, bb, w, o, c->stream, tile); });
       break;
   }
   HIPCHECK(hipGetLastError());
@@ -1129,6 +1137,7 @@ void run_allsky(crtm_context* c, Sensor& S, const DevBatch& be, const DevClouds&
     if (c->profiling) {
       c->mark();
       c->ev_kind.push_back(2);
+    c->ev_sensor.push_back(-1);
     }
     fn();
     if (c->profiling) c->mark();
@@ -1250,6 +1259,7 @@ void run_sensor_k_allsky(crtm_context* c, Sensor& S, const DevBatch& be, const D
   if (c->profiling) {
     c->mark();
     c->ev_kind.push_back(2);
+    c->ev_sensor.push_back(-1);
   }
   launch_allsky_k<RT, KT>(S.d, be, cc.cl, cw, c->cc, sc, sa, reinterpret_cast<const RT*>(o1.tau_g), w.geo,
                           (int)items, nz.data(), (int)nz.size(), o2, cc.ao, cc.ko, c->ctape.as<RT>(), tape_lanes, te,
@@ -1302,6 +1312,7 @@ void run_sensors(crtm_context* c, const DevBatch& be, DevOut o, const CloudCall&
       if (c->profiling) {
         c->mark();
         c->ev_kind.push_back(3);
+        c->ev_sensor.push_back(-1);
       }
       launch_sfc_optics(S->d, be, be.sfc_in, mode, c->mw, c->ir, planes, stride, sfc_err, c->stream);
       if (c->profiling) c->mark();
@@ -1808,6 +1819,7 @@ int crtm_init(int n_sensors, const char* const* sensor_ids, const char* file_pat
       S->sc = read_spccoeff(dir + sensor_ids[i] + ".SpcCoeff.bin");
       S->tc = read_odps(dir + sensor_ids[i] + ".TauCoeff.bin");
       build_sensor(*S, off);
+      S->index = i;
       off += S->tc.L;
       ctx->sensors.push_back(std::move(S));
     }
@@ -2005,6 +2017,103 @@ int crtm_k_matrix_noalloc(crtm_context* ctx, const crtm_atmosphere* atm, const c
   return st;
 }
 
+int crtm_tile_profiles(int precision, int tile) {
+  if (tile < 0 || tile >= N_TILES) return 0;
+  return precision == CRTM_PREC_FP64 ? TileSet<double>::pw[tile] : TileSet<float>::pw[tile];
+}
+
+int crtm_set_tile(crtm_context* ctx, int sensor, int precision, int tile) {
+  if (!ctx) return CRTM_FAILURE;
+  if (sensor < 0 || sensor >= (int)ctx->sensors.size()) return fail(ctx, fmt("sensor index %d", sensor));
+  if (precision != CRTM_PREC_FP64 && precision != CRTM_PREC_FP32_SPLIT)
+    return fail(ctx, fmt("precision %d", precision));
+  if (tile < 0 || tile >= N_TILES) return fail(ctx, fmt("tile index %d: 0..%d", tile, N_TILES - 1));
+  ctx->sensors[sensor]->tile[precision == CRTM_PREC_FP64 ? 0 : 1] = tile;
+  return CRTM_SUCCESS;
+}
+
+int crtm_get_tile(const crtm_context* ctx, int sensor, int precision, int* tile) {
+  if (!ctx || !tile || sensor < 0 || sensor >= (int)ctx->sensors.size()) return CRTM_FAILURE;
+  if (precision != CRTM_PREC_FP64 && precision != CRTM_PREC_FP32_SPLIT) return CRTM_FAILURE;
+  *tile = ctx->sensors[sensor]->tile[precision == CRTM_PREC_FP64 ? 0 : 1];
+  return CRTM_SUCCESS;
+}
+
+int crtm_autotune_k_matrix(crtm_context* ctx, const crtm_atmosphere* atm, const crtm_surface* sfc,
+                           const crtm_geometry* geo, const crtm_options* opt, crtm_rtsolution* rts,
+                           crtm_k_output* k, int repeats, double* ms, int* best) {
+  if (!ctx) return fail(nullptr, "null context");
+  if (!opt) return fail(ctx, "null options");
+  if (repeats < 1 || repeats > 100) return fail(ctx, fmt("autotune repeats %d: 1..100", repeats));
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(ctx->stream, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone)
+    return fail(ctx, "crtm_autotune_k_matrix cannot run under graph capture (it synchronizes to read timers)");
+  const int ns = (int)ctx->sensors.size();
+  const int pi = opt->precision == CRTM_PREC_FP64 ? 0 : 1;
+  std::vector<int> saved(ns);
+  for (int i = 0; i < ns; ++i) saved[i] = ctx->sensors[i]->tile[pi];
+  const bool was_profiling = ctx->profiling;
+  std::vector<double> t((size_t)ns * N_TILES, 0.0);
+  std::vector<int> cnt((size_t)ns * N_TILES, 0);
+  auto restore = [&] {
+    for (int i = 0; i < ns; ++i) ctx->sensors[i]->tile[pi] = saved[i];
+    ctx->profiling = was_profiling;
+    ctx->ev_used = 0;
+    ctx->ev_kind.clear();
+    ctx->ev_sensor.clear();
+  };
+  for (int tl = 0; tl < N_TILES; ++tl) {
+    for (int i = 0; i < ns; ++i) ctx->sensors[i]->tile[pi] = tl;
+    for (int r = 0; r <= repeats; ++r) {   // the first run of each tile is not timed
+      if (hipStreamSynchronize(ctx->stream) != hipSuccess) {
+        restore();
+        return fail(ctx, "stream synchronize failed");
+      }
+      ctx->profiling = r > 0;
+      ctx->ev_used = 0;
+      ctx->ev_kind.clear();
+      ctx->ev_sensor.clear();
+      const int rc = crtm_k_matrix(ctx, atm, sfc, geo, opt, rts, k);
+      if (rc != CRTM_SUCCESS) {
+        restore();
+        return rc;
+      }
+      if (hipStreamSynchronize(ctx->stream) != hipSuccess) {
+        restore();
+        return fail(ctx, "stream synchronize failed");
+      }
+      for (size_t e = 0; r > 0 && e < ctx->ev_kind.size(); ++e) {
+        if (ctx->ev_kind[e] != 1 || ctx->ev_sensor[e] < 0) continue;
+        float el = 0.f;
+        if (hipEventElapsedTime(&el, ctx->ev[2 * e], ctx->ev[2 * e + 1]) != hipSuccess) {
+          restore();
+          return fail(ctx, "hipEventElapsedTime failed");
+        }
+        t[(size_t)ctx->ev_sensor[e] * N_TILES + tl] += el;
+        cnt[(size_t)ctx->ev_sensor[e] * N_TILES + tl] += 1;
+      }
+    }
+  }
+  restore();
+  for (int i = 0; i < ns; ++i) {
+    int b = saved[i];
+    double bt = -1.0;
+    for (int tl = 0; tl < N_TILES; ++tl) {
+      const size_t q = (size_t)i * N_TILES + tl;
+      const double v = cnt[q] ? t[q] / cnt[q] : -1.0;
+      if (ms) ms[q] = v;
+      if (v >= 0.0 && (bt < 0.0 || v < bt)) {
+        bt = v;
+        b = tl;
+      }
+    }
+    ctx->sensors[i]->tile[pi] = b;
+    if (best) best[i] = b;
+  }
+  // leave the outputs computed with the chosen tiles
+  return crtm_k_matrix(ctx, atm, sfc, geo, opt, rts, k);
+}
+
 int crtm_set_wavefront_angles(crtm_context* ctx, int min_angles) {
   if (!ctx) return CRTM_FAILURE;
   if (min_angles != 6 && min_angles != 9)
@@ -2031,6 +2140,7 @@ int crtm_set_stream(crtm_context* ctx, void* stream, int own_stream) {
   ctx->stream = next;
   ctx->ev_used = 0;
   ctx->ev_kind.clear();
+  ctx->ev_sensor.clear();
   return CRTM_SUCCESS;
 }
 
@@ -2049,6 +2159,7 @@ int crtm_profile_enable(crtm_context* ctx, int on) {
   ctx->profiling = on != 0;
   ctx->ev_used = 0;
   ctx->ev_kind.clear();
+  ctx->ev_sensor.clear();
   return CRTM_SUCCESS;
 }
 
@@ -2070,6 +2181,7 @@ int crtm_profile_query_kinds(crtm_context* ctx, double* ms, int* n_launches, int
   }
   ctx->ev_used = 0;
   ctx->ev_kind.clear();
+  ctx->ev_sensor.clear();
   return CRTM_SUCCESS;
 }
 

@@ -25,6 +25,12 @@ constexpr int OPO = 11;            // OPTRAN polynomial orders 0..10
 // kernel computes it, its outputs are zero and it does not fail the call
 constexpr int ST_SKIP = 7;
 
+// Tile shapes of the fused clear-sky kernel: profiles (waves) per workgroup,
+// each workgroup one 64-channel block.  Index 1 is the default.
+constexpr int N_TILES = 3;
+template <class RT> struct TileSet { static constexpr int pw[N_TILES] = {4, 8, 16}; };
+template <> struct TileSet<double> { static constexpr int pw[N_TILES] = {2, 4, 8}; };
+
 // Read-only view of data no kernel writes while it runs, through the
 // constant address space: loads at wave-uniform addresses become scalar
 // loads (scalar cache, counted apart from the vector-memory counter the

@@ -328,6 +328,30 @@ int crtm_set_stream(crtm_context* ctx, void* stream, int own_stream);
  * serial form (src/RTSolution/ADA/ADA_Module.f90). */
 int crtm_set_wavefront_angles(crtm_context* ctx, int min_angles);
 
+/* ---- Tile shapes of the fused clear-sky forward / K-matrix kernel ----
+ * No counterpart in the reference (its channel loop has no GPU tiles).  A
+ * workgroup of that kernel is one 64-channel block x P profiles (one wave per
+ * profile); CRTM_N_TILES tile indices, index 1 the default, with P =
+ * crtm_tile_profiles(precision, tile) (FP32-split 4 / 8 / 16, FP64 2 / 4 / 8).
+ * Results do not depend on the tile (each profile-channel runs the same
+ * arithmetic in one lane); only the speed does. */
+#define CRTM_N_TILES 3
+int crtm_tile_profiles(int precision, int tile);
+/* tile used for sensor s (index in the crtm_init list) at a precision */
+int crtm_set_tile(crtm_context* ctx, int sensor, int precision, int tile);
+int crtm_get_tile(const crtm_context* ctx, int sensor, int precision, int* tile);
+/* Autotuner: runs crtm_k_matrix with the caller's arguments `repeats` times
+ * per tile (after one untimed run each), times the fused kernel of each
+ * sensor with HIP events on the context stream, and keeps for every sensor
+ * the tile with the shortest mean time at options->precision.  ms
+ * [n_sensors][CRTM_N_TILES] (may be NULL) receives the mean kernel times in
+ * milliseconds, best [n_sensors] (may be NULL) the chosen tile indices.  The
+ * outputs hold a final run with the chosen tiles.  Synchronizes the context
+ * stream (not capturable). */
+int crtm_autotune_k_matrix(crtm_context* ctx, const crtm_atmosphere* atm, const crtm_surface* sfc,
+                           const crtm_geometry* geo, const crtm_options* opt, crtm_rtsolution* rts,
+                           crtm_k_output* k, int repeats, double* ms, int* best);
+
 /* Scattering-layer capacity for calls with clouds or aerosols.  0 (the
  * default): each such call counts its scattering layers on the device and
  * reads the count back once to size the scattering launches.  max_layers > 0:
