@@ -60,14 +60,15 @@ class CSurface(ctypes.Structure):
 
 class CGeometry(ctypes.Structure):
     _fields_ = [(n, _vp) for n in ("sensor_zenith_angle", "sensor_scan_angle", "source_zenith_angle",
-                                   "surface_altitude", "sensor_azimuth_angle")]
+                                   "surface_altitude", "sensor_azimuth_angle", "ifov")]
 
 
 class COptions(ctypes.Structure):
     _fields_ = [("use_emissivity", ctypes.c_int), ("emissivity", _vp), ("precision", ctypes.c_int),
                 ("memory", ctypes.c_int), ("jacobian_f32", ctypes.c_int), ("check_input", ctypes.c_int),
                 ("n_streams", ctypes.c_int), ("overlap_id", ctypes.c_int), ("n_streams_profile", _vp),
-                ("no_scattering", ctypes.c_int), ("skip_profile", _vp)]
+                ("no_scattering", ctypes.c_int), ("skip_profile", _vp),
+                ("use_antenna_correction", ctypes.c_int), ("no_nlte_correction", ctypes.c_int)]
 
 
 class CRTSolution(ctypes.Structure):
@@ -295,7 +296,8 @@ class Context:
 
     # -- argument marshalling ------------------------------------------------
     def _structs(self, prof, emissivity, precision, memory, jacobian_f32, check_input, keep, n_streams=0,
-                 overlap=0, include_scattering=True, skip_profile=None):
+                 overlap=0, include_scattering=True, skip_profile=None, use_antenna_correction=False,
+                 apply_nlte_correction=True):
         def arr(x, dt):
             if x is None:
                 return None
@@ -355,7 +357,11 @@ class Context:
         geo.source_zenith_angle = _ptr(arr(prof.source_zenith, np.float64))
         az = getattr(prof, "sensor_azimuth", None)
         geo.sensor_azimuth_angle = _ptr(arr(az, np.float64)) if az is not None else None
+        ifov = getattr(prof, "ifov", None)
+        geo.ifov = _ptr(arr(ifov, np.int32)) if ifov is not None else None
         opt = COptions()
+        opt.use_antenna_correction = int(bool(use_antenna_correction))
+        opt.no_nlte_correction = 0 if apply_nlte_correction else 1
         # emissivity None: Options%Use_Emissivity = .FALSE., the surface optics are computed
         opt.use_emissivity = 0 if emissivity is None else 1
         opt.emissivity = _ptr(arr(emissivity, np.float64))
@@ -399,7 +405,8 @@ class Context:
         return np.zeros(shape, dtype)
 
     def forward(self, prof, emissivity, precision=PREC_FP64, extras=False, check_input=True, n_streams=0,
-                overlap=0, include_scattering=True, skip_profile=None):
+                overlap=0, include_scattering=True, skip_profile=None, use_antenna_correction=False,
+                apply_nlte_correction=True):
         """CRTM_Forward over a ProfileSet (host numpy) or a device batch (torch).
 
         Profiles carrying clouds take the all-sky path; n_streams is
@@ -413,7 +420,8 @@ class Context:
             self._use_torch_stream()
         keep = []
         atm, sfc, geo, opt = self._structs(prof, emissivity, precision, memory, False, check_input, keep,
-                                           n_streams, overlap, include_scattering, skip_profile)
+                                           n_streams, overlap, include_scattering, skip_profile,
+                                           use_antenna_correction, apply_nlte_correction)
         M, L = prof.n_profiles, self.n_channels
         out = {k: self._alloc((M, L), np.float64, memory, prof.temperature)
                for k in ("radiance", "brightness_temperature")}
@@ -433,7 +441,8 @@ class Context:
 
     def k_matrix(self, prof, emissivity, bt_k=None, rad_k=None, precision=PREC_FP64,
                  jacobian_f32=False, check_input=True, out=None, n_streams=0, overlap=0, extras=False,
-                 no_alloc=False, include_scattering=True, skip_profile=None, _tune=None):
+                 no_alloc=False, include_scattering=True, skip_profile=None, use_antenna_correction=False,
+                 apply_nlte_correction=True, _tune=None):
         """CRTM_K_Matrix: returns (RTSolution dict, K dict).
 
         K layout: temperature_k [NT, L] (profile m rows layer_offset[m]:...),
@@ -455,7 +464,7 @@ class Context:
         keep = []
         atm, sfc, geo, opt = self._structs(prof, emissivity, precision, memory, jacobian_f32,
                                            check_input, keep, n_streams, overlap, include_scattering,
-                                           skip_profile)
+                                           skip_profile, use_antenna_correction, apply_nlte_correction)
         nc, nae = int(atm.n_clouds), int(atm.n_aerosols)
         M, L = prof.n_profiles, self.n_channels
         host_nl = getattr(prof, "n_layers_host", None)
@@ -509,7 +518,8 @@ class Context:
         return rts_d, kd
 
     def adjoint(self, prof, emissivity, bt_ad, precision=PREC_FP64, n_streams=0, overlap=0,
-                include_scattering=True, skip_profile=None, max_work_bytes=16 << 30, progress=None):
+                include_scattering=True, skip_profile=None, max_work_bytes=16 << 30, progress=None,
+                use_antenna_correction=False, apply_nlte_correction=True):
         """CRTM_Adjoint (src/CRTM_Adjoint_Module.f90:299-325) through
         crtm_adjoint: the channel-summed adjoint of the brightness
         temperatures seeded with bt_ad [M, L] (RTSolution_AD%
@@ -575,14 +585,16 @@ class Context:
             part = {k: (v[..., a:b] if k in _LAYER_AD else v[m0:m1]) for k, v in ad.items()}
             self._adjoint_call(sub, emissivity if (whole or emissivity is None) else emissivity[m0:m1],
                                bt_ad if whole else bt_ad[m0:m1],
-                               precision, ns, overlap, include_scattering, sk, bt[m0:m1], part, m0)
+                               precision, ns, overlap, include_scattering, sk, bt[m0:m1], part, m0,
+                               use_antenna_correction, apply_nlte_correction)
             if progress:
                 progress(f"adjoint profiles {m0}-{m1} of {M} done")
             m0 = m1
         return bt, ad
 
     def _adjoint_call(self, prof, emissivity, bt_ad, precision, n_streams, overlap, include_scattering,
-                      skip_profile, bt_out, ad, m0):
+                      skip_profile, bt_out, ad, m0, use_antenna_correction=False,
+                      apply_nlte_correction=True):
         memory = MEM_DEVICE if _is_torch(prof.temperature) else MEM_HOST
         if memory == MEM_DEVICE:
             self._use_torch_stream()
@@ -591,7 +603,8 @@ class Context:
             emissivity = emissivity.contiguous() if memory == MEM_DEVICE else np.ascontiguousarray(emissivity, np.float64)
         bt_ad = bt_ad.contiguous() if memory == MEM_DEVICE else np.ascontiguousarray(bt_ad, np.float64)
         atm, sfc, geo, opt = self._structs(prof, emissivity, precision, memory, False, False, keep, n_streams,
-                                           overlap, include_scattering, skip_profile)
+                                           overlap, include_scattering, skip_profile, use_antenna_correction,
+                                           apply_nlte_correction)
         M, L = prof.n_profiles, self.n_channels
         rts_d = {"radiance": self._alloc((M, L), np.float64, memory, prof.temperature),
                  "brightness_temperature": bt_out, "status": self._alloc((M,), np.int32, memory, prof.temperature)}

@@ -119,6 +119,9 @@ class RecordReader:
             raise ValueError(f"record length mismatch {off} != {len(payload)}")
         return out
 
+    def remaining(self) -> bool:
+        return self._pos < len(self._buf)
+
     def read_ints(self, n):
         payload = self._next()
         return np.frombuffer(payload, "<i4", n)
@@ -145,10 +148,120 @@ class SpcCoeff:
     band_c2: np.ndarray
     cosmic_background_radiance: np.ndarray
     solar_irradiance: np.ndarray
+    ac: Optional["ACCoeff"] = None      # antenna correction block (microwave sensors)
+    nc: Optional["NLTECoeff"] = None    # non-LTE correction block (infrared sensors)
 
     @property
     def n_channels(self) -> int:
         return int(self.sensor_channel.size)
+
+
+ACCOEFF_RELEASE = 1              # src/Coefficients/ACCoeff/ACCoeff_Define.f90:71-72
+ACCOEFF_VERSION = 1
+NLTECOEFF_RELEASE = 2            # src/Coefficients/NLTECoeff/NLTECoeff_Define.f90
+NLTECOEFF_VERSION = 1
+N_NLTE_LAYERS = 2                # src/NLTE/NLTE_Parameters.f90
+
+
+@dataclasses.dataclass
+class ACCoeff:
+    """Antenna correction (ACCoeff_type, src/Coefficients/ACCoeff/ACCoeff_Define.f90:78-96):
+    antenna temperature = (A_earth + A_platform) * BT + A_space * TSPACE per field of view."""
+    sensor_id: str
+    wmo_satellite_id: int
+    wmo_sensor_id: int
+    sensor_channel: np.ndarray          # int32 [L]
+    a_earth: np.ndarray                 # f8 [n_fovs, L]
+    a_space: np.ndarray
+    a_platform: np.ndarray
+
+    @property
+    def n_fovs(self) -> int:
+        return int(self.a_earth.shape[0])
+
+
+@dataclasses.dataclass
+class NLTECoeff:
+    """Non-LTE correction (NLTECoeff_type, src/Coefficients/NLTECoeff/NLTECoeff_Define.f90:87-121)."""
+    sensor_id: str
+    wmo_satellite_id: int
+    wmo_sensor_id: int
+    sensor_channel: np.ndarray          # int32 [L]
+    upper_plevel: np.ndarray            # f8 [2]
+    lower_plevel: np.ndarray            # f8 [2]
+    min_tm: np.ndarray                  # f8 [2]
+    max_tm: np.ndarray
+    mean_tm: np.ndarray
+    nlte_channel: np.ndarray            # int32 [n_nlte]  sensor channel numbers
+    secant_sensor_zenith: np.ndarray    # f8 [n_sen]
+    secant_solar_zenith: np.ndarray     # f8 [n_sol]
+    c_index: np.ndarray                 # int32 [L]: 1-based index into C's last axis, 0 = not NLTE
+    c: np.ndarray                       # f8 [n_pred, n_sen, n_sol, n_nlte]
+
+
+def _write_ac_block(w, ac: ACCoeff) -> None:
+    """Record order of ACCoeff_Binary_ReadFile (src/Coefficients/ACCoeff/ACCoeff_Binary_IO.f90)."""
+    n, L = ac.a_earth.shape
+    w.record(np.int32(ACCOEFF_RELEASE), np.int32(ACCOEFF_VERSION))
+    w.record(np.int32(n), np.int32(L))
+    w.record(ac.sensor_id, np.int32(ac.wmo_satellite_id), np.int32(ac.wmo_sensor_id))
+    w.record(np.asarray(ac.sensor_channel, np.int32))
+    w.record(np.asarray(ac.a_earth, np.float64), np.asarray(ac.a_space, np.float64),
+             np.asarray(ac.a_platform, np.float64))
+
+
+def _read_ac_block(r) -> ACCoeff:
+    rel, ver = r.read([("i4", ()), ("i4", ())])
+    if rel != ACCOEFF_RELEASE:
+        raise ValueError(f"ACCoeff release {rel} unsupported")
+    n, L = r.read([("i4", ()), ("i4", ())])
+    sid, wsat, wsen = r.read([("s", None), ("i4", ()), ("i4", ())])
+    (ch,) = r.read([("i4", (L,))])
+    ae, asp, ap = r.read([("f8", (n, L))] * 3)
+    return ACCoeff(sid, int(wsat), int(wsen), ch.copy(), ae.copy(), asp.copy(), ap.copy())
+
+
+def _write_nc_block(w, nc: NLTECoeff) -> None:
+    """Record order of NLTECoeff_Binary_ReadFile (src/Coefficients/NLTECoeff/NLTECoeff_Binary_IO.f90)."""
+    n1, n2, n3, n4 = nc.c.shape
+    L = int(nc.sensor_channel.size)
+    w.record(np.int32(NLTECOEFF_RELEASE), np.int32(NLTECOEFF_VERSION))
+    w.record(np.int32(n1), np.int32(n2), np.int32(n3), np.int32(n4), np.int32(L))
+    w.record(nc.sensor_id, np.int32(nc.wmo_satellite_id), np.int32(nc.wmo_sensor_id),
+             np.asarray(nc.sensor_channel, np.int32))
+    w.record(np.asarray(nc.upper_plevel, np.float64), np.asarray(nc.lower_plevel, np.float64))
+    w.record(np.asarray(nc.min_tm, np.float64), np.asarray(nc.max_tm, np.float64),
+             np.asarray(nc.mean_tm, np.float64))
+    w.record(np.asarray(nc.nlte_channel, np.int32))
+    w.record(np.asarray(nc.secant_sensor_zenith, np.float64), np.asarray(nc.secant_solar_zenith, np.float64))
+    w.record(np.asarray(nc.c_index, np.int32), np.asarray(nc.c, np.float64))
+
+
+def _read_nc_block(r) -> NLTECoeff:
+    rel, ver = r.read([("i4", ()), ("i4", ())])
+    if rel != NLTECOEFF_RELEASE:
+        raise ValueError(f"NLTECoeff release {rel} unsupported")
+    n1, n2, n3, n4, L = r.read([("i4", ())] * 5)
+    sid, wsat, wsen, ch = r.read([("s", None), ("i4", ()), ("i4", ()), ("i4", (L,))])
+    up, lo = r.read([("f8", (N_NLTE_LAYERS,))] * 2)
+    mn, mx, me = r.read([("f8", (N_NLTE_LAYERS,))] * 3)
+    (nch,) = r.read([("i4", (n4,))])
+    ss, so = r.read([("f8", (n2,)), ("f8", (n3,))])
+    ci, c = r.read([("i4", (L,)), ("f8", (n1, n2, n3, n4))])
+    return NLTECoeff(sid, int(wsat), int(wsen), ch.copy(), up.copy(), lo.copy(), mn.copy(), mx.copy(),
+                     me.copy(), nch.copy(), ss.copy(), so.copy(), ci.copy(), c.copy())
+
+
+def write_accoeff(path: str, ac: ACCoeff) -> None:
+    """A standalone ACCoeff file (magic number, then the block)."""
+    with RecordWriter(path) as w:
+        _write_ac_block(w, ac)
+
+
+def write_nltecoeff(path: str, nc: NLTECoeff) -> None:
+    """A standalone NLTECoeff file (magic number, then the block)."""
+    with RecordWriter(path) as w:
+        _write_nc_block(w, nc)
 
 
 def write_spccoeff(path: str, sc: SpcCoeff) -> None:
@@ -170,8 +283,14 @@ def write_spccoeff(path: str, sc: SpcCoeff) -> None:
                  np.asarray(sc.band_c2, np.float64),
                  np.asarray(sc.cosmic_background_radiance, np.float64),
                  np.asarray(sc.solar_irradiance, np.float64))
-        w.record(np.int32(0))   # antenna-correction data not present
-        w.record(np.int32(0))   # NLTE-correction data not present
+        # antenna- and NLTE-correction blocks, each behind a DATA_PRESENT flag
+        # (SpcCoeff_Binary_IO.f90:466-527)
+        w.record(np.int32(1 if sc.ac is not None else 0))
+        if sc.ac is not None:
+            _write_ac_block(w, sc.ac)
+        w.record(np.int32(1 if sc.nc is not None else 0))
+        if sc.nc is not None:
+            _write_nc_block(w, sc.nc)
 
 
 def read_spccoeff(path: str) -> SpcCoeff:
@@ -190,8 +309,20 @@ def read_spccoeff(path: str) -> SpcCoeff:
         ch, pol, flag = a[:3]
         pang = np.zeros(n)
         rest = a[3:]
+    ac = nc = None
+    if r.remaining():
+        (acp,) = r.read([("i4", ())])
+        if acp == 1:
+            ac = _read_ac_block(r)
+        (ncp,) = r.read([("i4", ())])
+        if ncp == 1:
+            nc = _read_nc_block(r)
+    for blk, what in ((ac, "antenna correction"), (nc, "NLTE correction")):
+        if blk is not None and (blk.sensor_id != sid or blk.wmo_satellite_id != wsat
+                                or blk.wmo_sensor_id != wsen or not np.array_equal(blk.sensor_channel, ch)):
+            raise ValueError(f"{path}: {what} sensor information is inconsistent with SpcCoeff")
     return SpcCoeff(sid, int(stype), int(wsat), int(wsen), ch.copy(), pol.copy(), pang.copy(),
-                    flag.copy(), *[x.copy() for x in rest])
+                    flag.copy(), *[x.copy() for x in rest], ac=ac, nc=nc)
 
 
 # --------------------------------------------------------------------------

@@ -409,10 +409,35 @@ clear_fwdk_kernel(DevSensor s, DevBatch bt, ProfWork<RT> w, DevOut o) {
   }
   const RT eps = (RT)eps_d, rho = sfc_comp ? (RT)rho_d : RT(1) - eps;
   const RT sfc_src = eps * Bs + rho * rd;
-  const double R = (double)(uacc + tc * sfc_src);
-  // CRTM_Planck_Temperature in FP64 (CRTM_Planck_Functions.f90:470-492)
+  // Post_Process_RTSolution (src/CRTM_Forward_Module.f90:1166-1199): the
+  // non-LTE correction of the radiance (Compute_NLTE_Correction,
+  // src/NLTE/NLTE_Correction/CRTM_NLTECorrection.f90:503-547), the FP64
+  // Planck temperature (CRTM_Planck_Functions.f90:470-492), the antenna
+  // correction of the brightness temperature (CRTM_Compute_AntCorr,
+  // src/AntennaCorrection/CRTM_AntennaCorrection.f90:118-140)
+  double R = (double)(uacc + tc * sfc_src);
+  const cst_ptr<double> nlr = (w.nlte && ok_prof) ? as_const(w.nlte + (size_t)m * NLTE_REC) : nullptr;
+  const bool nl_on = nlr && nlr[NL_ACTIVE] != 0.0 && s.nl_cidx[lc] > 0;
+  double nlc[3] = {0.0, 0.0, 0.0};   // the channel's coefficients at the profile's angles
+  if (nl_on) {
+    const int ich = s.nl_cidx[lc] - 1, is = (int)nlr[NL_ISEN], io = (int)nlr[NL_ISOL];
+    const int ns = s.nl_nsen, no = s.nl_nsol;
+    auto C = [&](int i, int a, int b) { return s.nl_c[i + 3 * (a + ns * (b + no * ich))]; };
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+      nlc[i] = nlr[NL_WGT + 0] * C(i, is, io) + nlr[NL_WGT + 1] * C(i, is, io + 1) + nlr[NL_WGT + 2] * C(i, is + 1, io) +
+               nlr[NL_WGT + 3] * C(i, is + 1, io + 1);
+      R += nlc[i] * nlr[NL_PRED + i];
+    }
+  }
   const double c1d = s.c1[lc], c2d = s.c2[lc], bc1d = s.bc1[lc], bc2d = s.bc2[lc];
-  const double bt_v = (c2d / log(c1d / R + 1.0) - bc1d) / bc2d;
+  double bt_v = (c2d / log(c1d / R + 1.0) - bc1d) / bc2d;
+  double a_ep = 1.0;   // d(antenna temperature) / d(brightness temperature)
+  if (bt.ac_on && s.ac_nfov > 0 && bt.ifov && ok_prof && bt.ifov[m] > 0) {
+    const int64_t fa = (int64_t)(bt.ifov[m] - 1) * s.L + lc;
+    a_ep = s.ac_ep[fa];
+    bt_v = a_ep * bt_v + s.ac_sp[fa] * TSPACE;
+  }
   // profiles with cloudy layers in an all-sky K-matrix call: the scattering
   // path has written the outputs and hands over adjoint seeds
   const bool seeded = DOK && o.ks.mask && ok_prof && o.ks.mask[m] > 0;   // wave-uniform
@@ -433,9 +458,25 @@ clear_fwdk_kernel(DevSensor s, DevBatch bt, ProfWork<RT> w, DevOut o) {
   const double F = c1d * c2d / (bc2d * arg * (R * lga) * (R * lga));
   double rad_ad_d = 0.0;
   if (seeded) rad_ad_d = valid ? o.ks.rad[sl] : 0.0;
-  else rad_ad_d = valid ? (o.rad_k ? o.rad_k[ol] : 0.0) + F * (o.bt_k ? o.bt_k[ol] : 0.0) : 0.0;
+  else rad_ad_d = valid ? (o.rad_k ? o.rad_k[ol] : 0.0) + F * a_ep * (o.bt_k ? o.bt_k[ol] : 0.0) : 0.0;
   if (valid && o.rad_k && !seeded) o.rad_k[ol] = rad_ad_d;
   const RT rad_ad = (RT)rad_ad_d;
+  // Compute_NLTE_Correction_AD and Compute_NLTE_Predictor_AD (CRTM_NLTECorrection.f90:396-446,
+  // 704-744): each mean-temperature layer's predictor adjoint spread evenly over its
+  // levels (k1, k2], added to those layers' temperature adjoint in the Emission_AD pass
+  RT nl_g[2] = {RT(0), RT(0)};
+  int nl_k1[2] = {0, 0}, nl_k2[2] = {0, 0};
+  if (nl_on && nlr[NL_CTM] != 0.0) {
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      nl_k1[i] = (int)nlr[NL_K1 + i];
+      nl_k2[i] = (int)nlr[NL_K2 + i];
+      if (nl_k2[i] > nl_k1[i]) nl_g[i] = (RT)(nlc[1 + i] * rad_ad_d / (double)(nl_k2[i] - nl_k1[i]));
+    }
+  }
+  auto nl_t = [&](int k) -> RT {   // the NLTE part of user layer k's temperature adjoint
+    return (k >= nl_k1[0] && k < nl_k2[0] ? nl_g[0] : RT(0)) + (k >= nl_k1[1] && k < nl_k2[1] ? nl_g[1] : RT(0));
+  };
   const RT upad_sfc = rad_ad * tc;           // adjoint of the surface-leaving radiance
   // computed optics: CRTM_Compute_SfcOptics_AD from the emissivity and
   // reflectivity adjoints (sg: water temperature, salinity, wind speed, wind
@@ -553,9 +594,9 @@ clear_fwdk_kernel(DevSensor s, DevBatch bt, ProfWork<RT> w, DevOut o) {
       ru = ru * t + B * (RT(1) - t);
       const RT dbdt = c2 * bc2 * B * (B + c1) * ic1 * rrcp<RT>(te * te);
       if constexpr (RED) {
-        if (k >= nx) red_add(tad + k, dbdt * b_ad + t_seed);   // wave-uniform: every lane takes part
+        if (k >= nx) red_add(tad + k, dbdt * b_ad + t_seed + nl_t(k));   // wave-uniform: every lane takes part
       } else if (valid && k >= nx) {
-        tko[(int64_t)k * Lt] = (KT)(dbdt * b_ad + t_seed);
+        tko[(int64_t)k * Lt] = (KT)(dbdt * b_ad + t_seed + nl_t(k));
         if (!winmap)
           for (int ja = 0; ja < na; ++ja) ako[ja * astr + (int64_t)k * Lt] = KT(0);
       }

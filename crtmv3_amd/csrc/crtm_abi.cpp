@@ -90,6 +90,7 @@ class Records {
     std::memcpy(&magic, r.data(), 4);
     if (magic != 123456789) throw CrtmError("Invalid magic number in " + path);
   }
+  bool more() const { return pos_ < buf_.size(); }
   std::vector<char> next() {
     if (pos_ + 4 > buf_.size()) throw CrtmError("Unexpected end of file " + path_);
     int32_t n;
@@ -147,10 +148,101 @@ class Cursor {
 
 struct SpcCoeffH {
   std::string sensor_id;
-  int sensor_type = 0, n_channels = 0;
+  int sensor_type = 0, n_channels = 0, wmo_sat = 0, wmo_sen = 0;
   std::vector<int> sensor_channel, polarization;
   std::vector<double> pol_angle, frequency, wavenumber, c1, c2, bc1, bc2, cosmic, solar;
+  // antenna correction (ACCoeff, [fov + n_fov * channel]); n_fov = 0: absent
+  int ac_nfov = 0;
+  std::vector<double> ac_e, ac_s, ac_p;
+  // non-LTE correction (NLTECoeff); nl_nsen = 0: absent
+  int nl_npred = 0, nl_nsen = 0, nl_nsol = 0, nl_nch = 0;
+  std::vector<double> nl_up, nl_lo, nl_min, nl_max, nl_mean, nl_ssen, nl_ssol, nl_c;
+  std::vector<int> nl_cidx;
 };
+
+// the sensor identification record of an ACCoeff / NLTECoeff block must
+// repeat the SpcCoeff's (SpcCoeff_Binary_IO.f90:486-495, 516-525)
+void check_block_sensor(const SpcCoeffH& sc, const std::string& id, int wsat, int wsen, const std::vector<int>& ch,
+                        const char* what) {
+  if (id != sc.sensor_id || wsat != sc.wmo_sat || wsen != sc.wmo_sen || ch != sc.sensor_channel)
+    throw CrtmError(fmt("%s sensor information is inconsistent with SpcCoeff", what));
+}
+
+// ACCoeff_Binary_ReadFile (src/Coefficients/ACCoeff/ACCoeff_Binary_IO.f90), as
+// embedded in a SpcCoeff file
+void read_ac_block(Records& rec, SpcCoeffH& sc) {
+  Cursor r0(rec.next());
+  const int rel = r0.get<int32_t>();
+  r0.get<int32_t>();
+  r0.done("ACCoeff release");
+  if (rel != 1) throw CrtmError(fmt("ACCoeff Release check failed (release %d)", rel));
+  Cursor r1(rec.next());
+  const int n = r1.get<int32_t>(), L = r1.get<int32_t>();
+  r1.done("ACCoeff dimensions");
+  if (n < 1 || L != sc.n_channels) throw CrtmError("ACCoeff dimensions inconsistent with SpcCoeff");
+  Cursor r2(rec.next());
+  const std::string id = r2.str(20);
+  const int wsat = r2.get<int32_t>(), wsen = r2.get<int32_t>();
+  r2.done("ACCoeff sensor ids");
+  Cursor r3(rec.next());
+  const std::vector<int> ch = r3.arr<int32_t>(L);
+  r3.done("ACCoeff sensor channels");
+  check_block_sensor(sc, id, wsat, wsen, ch, "Antenna correction");
+  Cursor r4(rec.next());
+  sc.ac_e = r4.arr<double>((size_t)n * L);
+  sc.ac_s = r4.arr<double>((size_t)n * L);
+  sc.ac_p = r4.arr<double>((size_t)n * L);
+  r4.done("ACCoeff coefficients");
+  sc.ac_nfov = n;
+}
+
+// NLTECoeff_Binary_ReadFile (src/Coefficients/NLTECoeff/NLTECoeff_Binary_IO.f90),
+// as embedded in a SpcCoeff file
+void read_nc_block(Records& rec, SpcCoeffH& sc) {
+  Cursor r0(rec.next());
+  const int rel = r0.get<int32_t>();
+  r0.get<int32_t>();
+  r0.done("NLTECoeff release");
+  if (rel != 2) throw CrtmError(fmt("NLTECoeff Release check failed (release %d)", rel));
+  Cursor r1(rec.next());
+  const int n1 = r1.get<int32_t>(), n2 = r1.get<int32_t>(), n3 = r1.get<int32_t>(), n4 = r1.get<int32_t>(),
+            L = r1.get<int32_t>();
+  r1.done("NLTECoeff dimensions");
+  if (n1 != 3 || n2 < 2 || n3 < 2 || n4 < 0 || L != sc.n_channels)
+    throw CrtmError("NLTECoeff dimensions inconsistent (3 predictors, at least two angles per axis)");
+  Cursor r2(rec.next());
+  const std::string id = r2.str(20);
+  const int wsat = r2.get<int32_t>(), wsen = r2.get<int32_t>();
+  const std::vector<int> ch = r2.arr<int32_t>(L);
+  r2.done("NLTECoeff sensor ids");
+  check_block_sensor(sc, id, wsat, wsen, ch, "non-LTE correction");
+  Cursor r3(rec.next());
+  sc.nl_up = r3.arr<double>(2);
+  sc.nl_lo = r3.arr<double>(2);
+  r3.done("NLTECoeff pressure levels");
+  Cursor r4(rec.next());
+  sc.nl_min = r4.arr<double>(2);
+  sc.nl_max = r4.arr<double>(2);
+  sc.nl_mean = r4.arr<double>(2);
+  r4.done("NLTECoeff temperatures");
+  Cursor r5(rec.next());
+  r5.arr<int32_t>(n4);   // NLTE_Channel
+  r5.done("NLTECoeff channels");
+  Cursor r6(rec.next());
+  sc.nl_ssen = r6.arr<double>(n2);
+  sc.nl_ssol = r6.arr<double>(n3);
+  r6.done("NLTECoeff secants");
+  Cursor r7(rec.next());
+  sc.nl_cidx = r7.arr<int32_t>(L);
+  sc.nl_c = r7.arr<double>((size_t)n1 * n2 * n3 * n4);
+  r7.done("NLTECoeff coefficients");
+  for (int c : sc.nl_cidx)
+    if (c < 0 || c > n4) throw CrtmError("NLTECoeff C_Index out of range");
+  sc.nl_npred = n1;
+  sc.nl_nsen = n2;
+  sc.nl_nsol = n3;
+  sc.nl_nch = n4;
+}
 
 // SpcCoeff_Binary_IO.f90:374-470
 SpcCoeffH read_spccoeff(const std::string& path) {
@@ -167,8 +259,8 @@ SpcCoeffH read_spccoeff(const std::string& path) {
   Cursor r2(rec.next());
   sc.sensor_id = r2.str(20);
   sc.sensor_type = r2.get<int32_t>();
-  r2.get<int32_t>();
-  r2.get<int32_t>();
+  sc.wmo_sat = r2.get<int32_t>();
+  sc.wmo_sen = r2.get<int32_t>();
   r2.done("SpcCoeff sensor ids");
   if (sc.sensor_type > 100) sc.sensor_type -= 100;   // ACTIVE_SENSOR offset
   Cursor r3(rec.next());
@@ -185,6 +277,18 @@ SpcCoeffH read_spccoeff(const std::string& path) {
   sc.cosmic = r3.arr<double>(n);
   sc.solar = r3.arr<double>(n);
   r3.done("SpcCoeff channel data");
+  // antenna- and NLTE-correction blocks behind their DATA_PRESENT flags
+  // (SpcCoeff_Binary_IO.f90:466-527)
+  if (rec.more()) {
+    Cursor fa(rec.next());
+    const int ac_present = fa.get<int32_t>();
+    fa.done("antenna correction data indicator");
+    if (ac_present == 1) read_ac_block(rec, sc);
+    Cursor fn(rec.next());
+    const int nc_present = fn.get<int32_t>();
+    fn.done("NLTE correction data indicator");
+    if (nc_present == 1) read_nc_block(rec, sc);
+  }
   return sc;
 }
 
@@ -556,6 +660,36 @@ void build_sensor(Sensor& S, int chan_offset) {
   d.sensor_type = sc.sensor_type;
   d.polarization = S.put(sc.polarization);
   d.pol_angle = S.put(sc.pol_angle);
+  // antenna correction: [fov][channel] A_earth + A_platform and A_space
+  d.ac_nfov = sc.ac_nfov;
+  if (sc.ac_nfov > 0) {
+    const int n = sc.ac_nfov, L = sc.n_channels;
+    std::vector<double> ep((size_t)n * L), sp((size_t)n * L);
+    for (int l = 0; l < L; ++l)
+      for (int f = 0; f < n; ++f) {
+        const size_t src = f + (size_t)n * l, dst = (size_t)f * L + l;
+        ep[dst] = sc.ac_e[src] + sc.ac_p[src];
+        sp[dst] = sc.ac_s[src];
+      }
+    d.ac_ep = S.put(ep);
+    d.ac_sp = S.put(sp);
+  }
+  d.nl_nsen = sc.nl_nsen;
+  d.nl_nsol = sc.nl_nsol;
+  d.nl_nch = sc.nl_nch;
+  if (sc.nl_nsen > 0) {
+    d.nl_ssen = S.put(sc.nl_ssen);
+    d.nl_ssol = S.put(sc.nl_ssol);
+    d.nl_cidx = S.put(sc.nl_cidx);
+    d.nl_c = S.put(sc.nl_c.empty() ? std::vector<double>(1, 0.0) : sc.nl_c);
+    for (int i = 0; i < 2; ++i) {
+      d.nl_up[i] = sc.nl_up[i];
+      d.nl_lo[i] = sc.nl_lo[i];
+      d.nl_min[i] = sc.nl_min[i];
+      d.nl_max[i] = sc.nl_max[i];
+      d.nl_mean[i] = sc.nl_mean[i];
+    }
+  }
 }
 
 // Global attributes of the binary coefficient files: per attribute a length
@@ -836,7 +970,7 @@ struct crtm_context {
   int n_channels = 0;
   std::string last_error;
   // work pool (grown on demand, reused across calls)
-  DevBuf wx, wu, wo, wacci, waccw, wuidx, wufrac, wgeo, wstatus, wuwin, wfrange, waccw64, wlm;
+  DevBuf wx, wu, wo, wacci, waccw, wuidx, wufrac, wgeo, wstatus, wuwin, wfrange, waccw64, wlm, wnlte;
   DevBuf stage;  // host-mode staging of inputs/outputs
   DevBuf clim;   // model climatology table (AddLayers)
   DevBuf xnl, xloff, xnx, xlp, xp, xt, xa;   // profiles extended by AddLayers
@@ -916,7 +1050,7 @@ void* take(crtm_context* c, DevBuf& b, size_t bytes) {
 
 // Bytes of each per-profile work array of one sensor's preparation and fused
 // kernels (ProfWork), in the order make_work takes them.
-constexpr int N_WORK = 13;
+constexpr int N_WORK = 14;
 template <class RT>
 std::array<size_t, N_WORK> work_sizes(int M, int Ko, int nx, int max_k) {
   const size_t m = (size_t)M, ko = (size_t)Ko, mk = (size_t)max_k;
@@ -924,14 +1058,14 @@ std::array<size_t, N_WORK> work_sizes(int M, int Ko, int nx, int max_k) {
           m * ko * 3 * sizeof(int),       m * NZ_MAX * sizeof(RT),         m * (mk + 1) * sizeof(int),
           m * (mk + 1) * sizeof(RT),      m * 4 * sizeof(double),          m * mk * MAP_WIN * sizeof(RT),
           m * ko * 2 * sizeof(int),       m * sizeof(int),                 m * NZ_MAX * sizeof(double),
-          (3 * mk + 1) * m * sizeof(double)};
+          (3 * mk + 1) * m * sizeof(double), m * NLTE_REC * sizeof(double)};
 }
 
 template <class RT>
 ProfWork<RT> make_work(crtm_context* c, int M, int Ko, int nx, int max_k) {
   const std::array<size_t, N_WORK> sz = work_sizes<RT>(M, Ko, nx, max_k);
   DevBuf* bufs[N_WORK] = {&c->wx,    &c->wu,   &c->wo,     &c->wacci,   &c->waccw,   &c->wuidx, &c->wufrac,
-                          &c->wgeo,  &c->wuwin, &c->wfrange, &c->wstatus, &c->waccw64, &c->wlm};
+                          &c->wgeo,  &c->wuwin, &c->wfrange, &c->wstatus, &c->waccw64, &c->wlm, &c->wnlte};
   void* p[N_WORK];
   for (int i = 0; i < N_WORK; ++i) p[i] = take(c, *bufs[i], sz[i]);
   ProfWork<RT> w{};
@@ -948,6 +1082,7 @@ ProfWork<RT> make_work(crtm_context* c, int M, int Ko, int nx, int max_k) {
   w.status = static_cast<int*>(p[10]);
   w.accw64 = static_cast<double*>(p[11]);
   w.lm = static_cast<double*>(p[12]);
+  w.nlte = static_cast<double*>(p[13]);
   c->prep_status = w.status;
   return w;
 }
@@ -1435,6 +1570,18 @@ int do_call(crtm_context* ctx, const crtm_atmosphere* atm, const crtm_surface* s
   }
   CloudCall cc;
   cc.on = atm->n_clouds > 0 || atm->n_aerosols > 0;
+  // the antenna and non-LTE corrections are applied by the clear-sky kernel
+  // only: a cloudy call where either would change a result is refused
+  if (cc.on) {
+    for (const auto& S : ctx->sensors) {
+      if (opt->use_antenna_correction && S->sc.ac_nfov > 0 && geo->ifov)
+        return fail(ctx, "the antenna correction is applied in clear-sky calls only; a call with clouds or aerosols "
+                         "needs use_antenna_correction = 0 or no ifov");
+      if (!opt->no_nlte_correction && S->sc.nl_nsen > 0 && geo->source_zenith_angle)
+        return fail(ctx, "the non-LTE correction is applied in clear-sky calls only; a call with clouds or aerosols "
+                         "needs no_nlte_correction = 1 or no source zenith angles");
+    }
+  }
   if (cc.on) {
     if (atm->n_clouds > 0 && !ctx->cloud_loaded)
       return fail(ctx, "The CloudCoeff data must be loaded (CloudCoeff.bin in the crtm_init path) for cloudy profiles");
@@ -1547,6 +1694,8 @@ int do_call(crtm_context* ctx, const crtm_atmosphere* atm, const crtm_surface* s
   DevBatch b{};
   b.M = M;
   b.na = na;
+  b.ac_on = opt->use_antenna_correction != 0;
+  b.nlte_on = opt->no_nlte_correction == 0;
   b.max_k = max_k;
   b.NT = NT;
   b.Ltot = L;
@@ -1596,6 +1745,8 @@ int do_call(crtm_context* ctx, const crtm_atmosphere* atm, const crtm_surface* s
     }
     in(geo->sensor_zenith_angle, M * sizeof(double), (const void**)&b.zenith);
     in(geo->surface_altitude, M * sizeof(double), (const void**)&b.altitude);
+    in(geo->source_zenith_angle, M * sizeof(double), (const void**)&b.source_zenith);
+    in(geo->ifov, M * sizeof(int), (const void**)&b.ifov);
     in(sfc_comp ? nullptr : opt->emissivity, (size_t)M * L * sizeof(double), (const void**)&b.emissivity);
     if (sfc_comp) {
       in(sfc->salinity, M * sizeof(double), (const void**)&b.sfc_in.salinity);
@@ -1729,6 +1880,8 @@ int do_call(crtm_context* ctx, const crtm_atmosphere* atm, const crtm_surface* s
     if (!b.cov[c] || !b.sfc_t[c]) return fail(ctx, "surface coverage/temperature arrays required");
   b.zenith = geo->sensor_zenith_angle;
   b.altitude = geo->surface_altitude;
+  b.source_zenith = geo->source_zenith_angle;
+  b.ifov = geo->ifov;
   b.emissivity = sfc_comp ? nullptr : opt->emissivity;
   if (sfc_comp) {
     b.sfc_in.salinity = sfc->salinity;

@@ -25,6 +25,16 @@ constexpr int OPO = 11;            // OPTRAN polynomial orders 0..10
 // kernel computes it, its outputs are zero and it does not fail the call
 constexpr int ST_SKIP = 7;
 
+// Per-profile non-LTE correction predictor (NLTE_Predictor_type,
+// src/NLTE/NLTE_Predictor/NLTE_Predictor_Define.f90:72-94) as the
+// preparation kernel leaves it for the fused kernel, FP64:
+//   active, compute_tm, k1[2], k2[2] (0-based level indices of the mean-
+//   temperature layers), predictors [3] (1, Tm1, Tm2), bilinear weights
+//   w11, w12, w21, w22, isen, isol (0-based lower corners)
+enum NLteRec { NL_ACTIVE = 0, NL_CTM, NL_K1, NL_K2 = NL_K1 + 2, NL_PRED = NL_K2 + 2, NL_WGT = NL_PRED + 3,
+               NL_ISEN = NL_WGT + 4, NL_ISOL, NLTE_REC = 16 };
+constexpr double TSPACE = 2.7253;   // K, src/CRTM_Parameters.f90:173
+
 // Tile shapes of the fused clear-sky kernel: profiles (waves) per workgroup,
 // each workgroup one 64-channel block.  Index 1 is the default.
 constexpr int N_TILES = 3;
@@ -157,6 +167,18 @@ struct DevSensor {
   int sensor_type;          // 1 microwave, 2 infrared, ...
   const int* polarization;  // SpcCoeff%Polarization
   const double* pol_angle;  // SpcCoeff%PolAngle, degrees
+  // antenna correction (SpcCoeff%AC, src/Coefficients/ACCoeff/ACCoeff_Define.f90:78-96):
+  // per field of view f and channel l, [f][l]: A_earth + A_platform, and A_space
+  int ac_nfov;              // 0: the sensor has no antenna correction data
+  const double* ac_ep;
+  const double* ac_sp;
+  // non-LTE correction (SpcCoeff%NC, src/Coefficients/NLTECoeff/NLTECoeff_Define.f90:87-121)
+  int nl_nsen, nl_nsol, nl_nch;  // nl_nsen = 0: no NLTE correction data
+  const double* nl_ssen;    // [nsen] secants of the sensor zenith angle
+  const double* nl_ssol;    // [nsol] secants of the solar zenith angle
+  const int* nl_cidx;       // [L] 1-based index of the channel's coefficients, 0: not an NLTE channel
+  const double* nl_c;       // [nch][nsol][nsen][3] (the reference's C(3, nsen, nsol, nch), first index fastest)
+  double nl_up[2], nl_lo[2], nl_min[2], nl_max[2], nl_mean[2];
 };
 
 // blk_meta layout
@@ -190,6 +212,11 @@ struct DevBatch {
   int64_t out_NT;           // caller's total layers (absorber Jacobian stride)
   const int* skip;          // [M] or null: Options(m)%Skip_Profile
   DevSfcIn sfc_in;          // computed surface optics: salinity, wind, azimuths
+  // post-processing (Post_Process_RTSolution, src/CRTM_Forward_Module.f90:1166-1199)
+  const double* source_zenith;  // [M] degrees, or null (100: no source)
+  const int* ifov;              // [M] Geometry%iFOV (0: no antenna correction), or null
+  int ac_on;                    // Options%Use_Antenna_Correction
+  int nlte_on;                  // Options%Apply_NLTE_Correction
 };
 
 // Extended profiles written by the AddLayers kernel: profile m occupies
@@ -226,6 +253,7 @@ template <class RT> struct ProfWork {
   int* frange;    // [M][Ko][2]: user layers whose last Map_Input contribution comes from ODPS layer k
   int* status;    // [M]
   double* accw64; // [M][NZ_MAX]: the LayerAvg weights in FP64 (mapping of T and absorbers)
+  double* nlte;   // [M][NLTE_REC]: the profile's non-LTE correction predictor (NLteRec)
   double* lm;     // [3 * max_k + 1][M]: per-profile columns of the layer-mapping kernel
                   // (user layer lnP, user level lnP, LayerAvg accumulation column)
 };

@@ -14,6 +14,9 @@
 // Reference: src/AtmAbsorption/ODPS/ODPS_CoordinateMapping.f90:151-319,
 // :607-634, :963-1012, :1108-1320; src/AtmAbsorption/ODPS/ODPS_Predictor.f90:
 // 290-357, 645-1095, 2716-2816; src/GeometryInfo/CRTM_GeometryInfo.f90:100-143.
+#include <cfloat>
+#include <climits>
+
 #include "crtm_device.h"
 #include "pred_adjoint.h"
 
@@ -35,7 +38,7 @@ constexpr int PREP_THREADS = 128;
 }  // namespace
 
 // Status codes written per profile
-enum { ST_OK = 0, ST_LAYERAVG = 1, ST_NZ = 2, ST_LAYERS = 3, ST_BRACKET = 4 };
+enum { ST_OK = 0, ST_LAYERAVG = 1, ST_NZ = 2, ST_LAYERS = 3, ST_BRACKET = 4, ST_FOV = 9 };
 
 // ---------------------------------------------------------------------------
 // CRTM_Atmosphere_AddLayers (src/Atmosphere/CRTM_Atmosphere.f90:248-400):
@@ -151,6 +154,11 @@ __global__ void __launch_bounds__(64) layer_map_kernel(DevSensor s, DevBatch bt,
   const int K = bt.n_layers[m];
   if (K < 1 || K > K_MAX || K > bt.max_k || Ko > KO_MAX) {
     w.status[m] = ST_LAYERS;
+    return;
+  }
+  // the antenna correction indexes its coefficients by Geometry%iFOV
+  if (bt.ac_on && s.ac_nfov > 0 && bt.ifov && (bt.ifov[m] < 0 || bt.ifov[m] > s.ac_nfov)) {
+    w.status[m] = ST_FOV;
     return;
   }
   const int64_t M = bt.M, loff = bt.loff[m];
@@ -363,6 +371,84 @@ __device__ inline void sens_slot_row(const double* U, double* R) {
 // geometry, mapped T / absorbers with the Min/Max clamps, secants, integrated
 // ratios, predictors, the adjoint table and the OPTRAN table.
 // ---------------------------------------------------------------------------
+// Compute_NLTE_Predictor (src/NLTE/NLTE_Correction/CRTM_NLTECorrection.f90:95-260)
+// for one profile: mean temperatures of the two NLTE layers (the profile's
+// layers between each pair of pressure levels, clamped to the coefficient
+// file's range), the bilinear weights in the sensor- and solar-zenith secants
+// (clamped to the tables).  The predictor stays inactive without NLTE data,
+// without the correction, or with the source below the horizon (secant < 0).
+// compute_tm is one flag for both layers, left by the last layer, as in the
+// reference.
+__device__ void nlte_predictor(const DevSensor& s, const DevBatch& bt, int m, const double* lp, const double* t,
+                               int K, double sec_sensor, double* rec) {
+  for (int i = 0; i < NLTE_REC; ++i) rec[i] = 0.0;
+  if (!bt.nlte_on || s.nl_nsen == 0) return;
+  const double szen = bt.source_zenith ? bt.source_zenith[m] : 100.0;
+  const double cosv = cos(D2R * szen);
+  const double sec_src = cosv != 0.0 ? 1.0 / cosv : (cosv < 0.0 ? -DBL_MAX : DBL_MAX);
+  if (sec_src < 0.0) return;
+  bool ctm = true;
+  int k1 = INT_MAX, k2 = INT_MAX;
+  for (int i = 0; i < 2; ++i) {
+    double tm;
+    if (s.nl_up[i] < lp[0] || s.nl_lo[i] > lp[K]) {
+      tm = s.nl_mean[i];
+      ctm = false;
+    } else {
+      for (int k = 1; k <= K; ++k)
+        if (lp[k] > s.nl_up[i]) {
+          k1 = k - 1;
+          break;
+        }
+      for (int k = k1; k <= K; ++k)
+        if (lp[k] > s.nl_lo[i]) {
+          k2 = k;
+          break;
+        }
+      tm = 0.0;
+      for (int k = k1 + 1; k <= k2; ++k) tm += t[k - 1];
+      tm /= (double)(k2 - k1);
+      rec[NL_K1 + i] = k1;
+      rec[NL_K2 + i] = k2;
+      ctm = true;
+      if (tm < s.nl_min[i]) {
+        tm = s.nl_min[i];
+        ctm = false;
+      } else if (tm > s.nl_max[i]) {
+        tm = s.nl_max[i];
+        ctm = false;
+      }
+    }
+    rec[NL_PRED + 1 + i] = tm;
+  }
+  rec[NL_PRED] = 1.0;
+  const int ns = s.nl_nsen, no = s.nl_nsol;
+  const double x = fmin(fmax(sec_sensor, s.nl_ssen[0]), s.nl_ssen[ns - 1]);
+  const double y = fmin(fmax(sec_src, s.nl_ssol[0]), s.nl_ssol[no - 1]);
+  int isen = 0, isol = 0;
+  for (int i = 0; i < ns - 1; ++i)
+    if (x >= s.nl_ssen[i] && x <= s.nl_ssen[i + 1]) {
+      isen = i;
+      break;
+    }
+  for (int i = 0; i < no - 1; ++i)
+    if (y >= s.nl_ssol[i] && y <= s.nl_ssol[i + 1]) {
+      isol = i;
+      break;
+    }
+  const double x1 = s.nl_ssen[isen], x2 = s.nl_ssen[isen + 1], y1 = s.nl_ssol[isol], y2 = s.nl_ssol[isol + 1];
+  const double xx1 = (x - x2) / (x1 - x2), xx2 = (x - x1) / (x2 - x1);
+  const double yy1 = (y - y2) / (y1 - y2), yy2 = (y - y1) / (y2 - y1);
+  rec[NL_WGT + 0] = xx1 * yy1;
+  rec[NL_WGT + 1] = xx1 * yy2;
+  rec[NL_WGT + 2] = xx2 * yy1;
+  rec[NL_WGT + 3] = xx2 * yy2;
+  rec[NL_ISEN] = isen;
+  rec[NL_ISOL] = isol;
+  rec[NL_CTM] = ctm ? 1.0 : 0.0;
+  rec[NL_ACTIVE] = 1.0;
+}
+
 template <int G, class RT>
 __global__ void __launch_bounds__(PREP_THREADS) odps_prep_kernel(DevSensor s, DevBatch bt, ProfWork<RT> w) {
   using T = GT<G>;
@@ -412,6 +498,7 @@ __global__ void __launch_bounds__(PREP_THREADS) odps_prep_kernel(DevSensor s, De
     g[0] = geo[0];
     g[1] = geo[1];
     g[2] = geo[2];
+    if (w.nlte) nlte_predictor(s, bt, m, lp_u, t_u, K, sec_sensor, w.nlte + (size_t)m * NLTE_REC);
   }
   __syncthreads();
 

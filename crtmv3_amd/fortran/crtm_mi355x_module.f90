@@ -142,6 +142,9 @@ MODULE CRTM_MI355X_Module
     INTEGER :: Overlap_Id = 4                      ! average overlap (CRTM_CloudCover_Define.f90:79)
     INTEGER :: RT_Algorithm_Id = RT_ADA
     LOGICAL :: Skip_Profile = .FALSE.
+    ! src/Options/CRTM_Options_Define.f90:133,136 (one setting per call here)
+    LOGICAL :: Use_Antenna_Correction = .FALSE.
+    LOGICAL :: Apply_NLTE_Correction = .TRUE.
     INTEGER :: Precision = CRTM_PREC_FP64          ! extension: arithmetic of the GPU path
   END TYPE
 
@@ -171,7 +174,7 @@ MODULE CRTM_MI355X_Module
   END TYPE
   TYPE, BIND(C) :: c_geometry
     TYPE(C_PTR) :: sensor_zenith_angle, sensor_scan_angle, source_zenith_angle, surface_altitude, &
-                   sensor_azimuth_angle
+                   sensor_azimuth_angle, ifov
   END TYPE
   TYPE, BIND(C) :: c_options
     INTEGER(C_INT) :: use_emissivity
@@ -181,6 +184,7 @@ MODULE CRTM_MI355X_Module
     TYPE(C_PTR) :: n_streams_profile
     INTEGER(C_INT) :: no_scattering
     TYPE(C_PTR) :: skip_profile
+    INTEGER(C_INT) :: use_antenna_correction, no_nlte_correction
   END TYPE
   TYPE, BIND(C) :: c_rtsolution
     TYPE(C_PTR) :: radiance, brightness_temperature, up_radiance, down_radiance, &
@@ -266,7 +270,7 @@ MODULE CRTM_MI355X_Module
     INTEGER(C_INT64_T), ALLOCATABLE :: off(:)
     REAL(C_DOUBLE), ALLOCATABLE :: lp(:), p(:), t(:), a(:), cov(:, :), st(:, :), zen(:), scan(:), src(:), emis(:)
     REAL(C_DOUBLE), ALLOCATABLE :: alt(:), ws(:), wd(:), sal(:), azi(:)
-    INTEGER(C_INT), ALLOCATABLE :: skip(:)
+    INTEGER(C_INT), ALLOCATABLE :: skip(:), fov(:)
     REAL(C_DOUBLE), ALLOCATABLE :: rad(:), bt(:), up(:), dn(:), sp(:), se(:), sr(:), rcl(:), tbc(:), tcc(:)
     INTEGER(C_INT), ALLOCATABLE :: status(:), ctype(:)
     REAL(C_DOUBLE), ALLOCATABLE :: cwc(:), creff(:), cfrac(:)
@@ -980,7 +984,7 @@ CONTAINS
     ALLOCATE(b%ids(na), b%units(na), b%lp(NT + M), b%p(NT), b%t(NT), b%a(na * NT), b%cov(M, 4), b%st(M, 4), &
              b%zen(M), b%scan(M), b%src(M), b%alt(M), b%skip(M), b%emis(L * M), b%rad(L * M), b%bt(L * M), b%up(L * M), &
              b%dn(L * M), b%sp(L * M), b%se(L * M), b%status(M), b%rcl(L * M), b%tbc(L * M), b%tcc(L * M), &
-             b%sr(L * M), b%ws(M), b%wd(M), b%sal(M), b%azi(M))
+             b%sr(L * M), b%ws(M), b%wd(M), b%sal(M), b%azi(M), b%fov(M))
     b%emis = 0
     nc = Atm(1)%n_Clouds
     ALLOCATE(b%ctype(MAX(nc, 1) * M), b%cwc(MAX(nc, 1) * NT), b%creff(MAX(nc, 1) * NT), b%cfrac(NT))
@@ -1010,6 +1014,7 @@ CONTAINS
       b%wd(mm) = Sfc(mm)%Wind_Direction
       b%sal(mm) = Sfc(mm)%Salinity
       b%azi(mm) = Geo(mm)%Sensor_Azimuth_Angle
+      b%fov(mm) = Geo(mm)%iFOV
       b%skip(mm) = MERGE(1, 0, Opt(mm)%Skip_Profile)
       IF (Opt(mm)%Skip_Profile) CYCLE                ! no inputs read (src/CRTM_Forward_Module.f90:536)
       IF (Opt(mm)%RT_Algorithm_Id /= RT_ADA) THEN
@@ -1024,6 +1029,12 @@ CONTAINS
       ! profile, or the computed (water) surface optics for every profile
       IF (Opt(mm)%Use_Emissivity .NEQV. Opt(1)%Use_Emissivity) THEN
         CALL Display_Message('CRTM', 'Use_Emissivity must be the same for all profiles of a call', FAILURE)
+        RETURN
+      END IF
+      IF ((Opt(mm)%Use_Antenna_Correction .NEQV. Opt(1)%Use_Antenna_Correction) .OR. &
+          (Opt(mm)%Apply_NLTE_Correction .NEQV. Opt(1)%Apply_NLTE_Correction)) THEN
+        CALL Display_Message('CRTM', 'Use_Antenna_Correction and Apply_NLTE_Correction must be the same for all '// &
+                             'profiles of a call', FAILURE)
         RETURN
       END IF
       IF (Opt(mm)%Use_Emissivity) THEN
@@ -1081,6 +1092,7 @@ CONTAINS
     b%geo%source_zenith_angle = C_LOC(b%src)
     b%geo%surface_altitude = C_LOC(b%alt)
     b%geo%sensor_azimuth_angle = C_LOC(b%azi)
+    b%geo%ifov = C_LOC(b%fov)
     b%sfc%wind_speed = C_LOC(b%ws)
     b%sfc%wind_direction = C_LOC(b%wd)
     b%sfc%salinity = C_LOC(b%sal)
@@ -1096,6 +1108,8 @@ CONTAINS
     b%opt%no_scattering = MERGE(0, 1, Opt(1)%Include_Scattering)
     b%opt%skip_profile = C_NULL_PTR
     IF (ANY(b%skip /= 0)) b%opt%skip_profile = C_LOC(b%skip)
+    b%opt%use_antenna_correction = MERGE(1, 0, Opt(1)%Use_Antenna_Correction)
+    b%opt%no_nlte_correction = MERGE(0, 1, Opt(1)%Apply_NLTE_Correction)
     IF (SIZE(Opt) == M .AND. M > 1) THEN   ! Options(m)%n_Streams per profile
       ALLOCATE(b%nsp(M))
       DO mm = 1, M

@@ -158,8 +158,17 @@ def _ir_strengths(nu, group, rng):
     return A
 
 
+# sensors whose SpcCoeff files carry correction data: the channel set of a
+# base sensor plus antenna-correction (microwave) or non-LTE-correction
+# (infrared) blocks
+_DERIVED = {"amsua_ac_syn": "amsua_syn", "cris_nlte_syn": "cris_small_syn"}
+N_FOVS_AC = 30   # fields of view of the antenna correction (AMSU-A has 30)
+
+
 def sensor_def(name: str, seed: int = 20261203) -> SensorDef:
     rng = np.random.default_rng(seed + sum(map(ord, name)))
+    if name in _DERIVED:
+        return dataclasses.replace(sensor_def(_DERIVED[name], seed), sensor_id=name)
     if name == "amsua_syn":
         f = np.array([23.8, 31.4, 50.3, 52.8, 53.596, 54.4, 54.94, 55.5,
                       57.29, 57.29, 57.29, 57.29, 57.29, 57.29, 89.0])
@@ -202,7 +211,8 @@ def sensor_def(name: str, seed: int = 20261203) -> SensorDef:
     raise KeyError(name)
 
 
-SENSORS = ("amsua_syn", "atms_syn", "cris_syn", "iasi_syn", "hirs_syn", "cris_small_syn")
+SENSORS = ("amsua_syn", "atms_syn", "cris_syn", "iasi_syn", "hirs_syn", "cris_small_syn", "amsua_ac_syn",
+           "cris_nlte_syn")
 
 
 def _layer_shapes(group, lvl_p, lay_p, ref_abs):
@@ -367,12 +377,56 @@ def make_spc_coeff(sd: SensorDef, seed: int = 20261203) -> coeffio.SpcCoeff:
         pol = np.resize(np.asarray(pattern, np.int32), L)
         pol_angle = np.where(np.isin(pol, (CONST_MIXED_POLARIZATION, PRA_POLARIZATION)),
                              20.0 + 3.0 * np.arange(L), 0.0)
-    return coeffio.SpcCoeff(
+    sc = coeffio.SpcCoeff(
         sensor_id=sd.sensor_id, sensor_type=sd.sensor_type, wmo_satellite_id=1023,
         wmo_sensor_id=2047, sensor_channel=np.arange(1, L + 1, dtype=np.int32),
         polarization=pol, pol_angle=pol_angle, channel_flag=np.zeros(L, np.int32),
         frequency=sd.frequency, wavenumber=nu, planck_c1=c1, planck_c2=c2, band_c1=bc1,
         band_c2=bc2, cosmic_background_radiance=cosmic, solar_irradiance=np.zeros(L))
+    if sd.sensor_id == "amsua_ac_syn":
+        sc.ac = make_ac_coeff(sc, N_FOVS_AC, rng)
+    if sd.sensor_id == "cris_nlte_syn":
+        sc.nc = make_nlte_coeff(sc, rng)
+    return sc
+
+
+def make_ac_coeff(sc: coeffio.SpcCoeff, n_fovs: int, rng) -> coeffio.ACCoeff:
+    """Antenna efficiencies of a cross-track scanner: the space and platform
+    fractions grow towards the scan edges, the Earth fraction takes the rest."""
+    L = sc.n_channels
+    edge = (np.abs(np.arange(n_fovs) - (n_fovs - 1) / 2) / ((n_fovs - 1) / 2)) ** 2
+    a_space = (0.004 + 0.012 * edge)[:, None] * rng.uniform(0.6, 1.4, (1, L))
+    a_platform = (0.001 + 0.004 * edge)[:, None] * rng.uniform(0.5, 1.5, (1, L))
+    a_earth = 1.0 - a_space - a_platform + rng.uniform(-5e-4, 5e-4, (n_fovs, L))
+    return coeffio.ACCoeff(sc.sensor_id, sc.wmo_satellite_id, sc.wmo_sensor_id, sc.sensor_channel.copy(),
+                           a_earth, a_space, a_platform)
+
+
+def make_nlte_coeff(sc: coeffio.SpcCoeff, rng) -> coeffio.NLTECoeff:
+    """Non-LTE radiance excess of the 4.3 um band (channels above 2200 cm^-1):
+    daytime excess growing with the sun's height and with the mean
+    temperatures of the 0.005-0.2 hPa and 0.2-52 hPa layers (the pressure
+    levels of src/NLTE/NLTE_Parameters.f90)."""
+    L = sc.n_channels
+    nlte = np.nonzero(sc.wavenumber > 2200.0)[0]
+    n = nlte.size
+    ssen = 1.0 / np.cos(np.deg2rad(np.array([0.0, 20.0, 35.0, 45.0, 55.0, 65.0])))
+    ssol = 1.0 / np.cos(np.deg2rad(np.array([0.0, 30.0, 50.0, 65.0, 75.0, 80.0, 85.0])))
+    c = np.zeros((3, ssen.size, ssol.size, n))
+    sun = 1.0 / ssol                                   # cosine of the solar zenith
+    view = 1.0 / ssen
+    band = np.exp(-((sc.wavenumber[nlte] - 2330.0) / 60.0) ** 2)   # strongest in the band core
+    scale = 0.1 * (0.3 + band) * rng.uniform(0.8, 1.2, n)
+    for i, (t0, w) in enumerate(((210.0, 1.0), (240.0, 0.5))):
+        c[1 + i] = scale * w * 1e-3 * (sun[None, :, None] * (0.7 + 0.3 * view[:, None, None]))
+        c[0] -= c[1 + i] * (t0 - 40.0)                 # excess at the layer temperature t0: 40 K x slope
+    c_index = np.zeros(L, np.int32)
+    c_index[nlte] = np.arange(1, n + 1, dtype=np.int32)
+    return coeffio.NLTECoeff(sc.sensor_id, sc.wmo_satellite_id, sc.wmo_sensor_id, sc.sensor_channel.copy(),
+                             upper_plevel=np.array([0.005, 0.2]), lower_plevel=np.array([0.2, 52.0]),
+                             min_tm=np.array([170.0, 200.0]), max_tm=np.array([300.0, 290.0]),
+                             mean_tm=np.array([215.0, 245.0]), nlte_channel=sc.sensor_channel[nlte].copy(),
+                             secant_sensor_zenith=ssen, secant_solar_zenith=ssol, c_index=c_index, c=c)
 
 
 def write_sensor(dirpath: str, name: str, seed: int = 20261203) -> str:
@@ -448,6 +502,7 @@ class ProfileSet:
     wind_direction: Optional[np.ndarray] = None      # degrees [M]
     salinity: Optional[np.ndarray] = None            # ppt [M]
     sensor_azimuth: Optional[np.ndarray] = None      # degrees [M]
+    ifov: Optional[np.ndarray] = None                # int32 [M] Geometry%iFOV (None: 0, no antenna correction)
 
     @property
     def n_profiles(self) -> int:

@@ -118,6 +118,9 @@ typedef struct {
   const double* surface_altitude;      /* metres, may be NULL */
   const double* sensor_azimuth_angle;  /* degrees, may be NULL (999.9: no azimuth dependence of the
                                           FASTEM emissivity; src/GeometryInfo/CRTM_Geometry_Define.f90:111) */
+  const int* ifov;                     /* Geometry%iFOV, the field-of-view index of the antenna
+                                          correction (1..n_FOVs of the sensor's ACCoeff data; 0: no
+                                          correction), may be NULL (all 0) */
 } crtm_geometry;
 
 /* ---- Options (src/Options/CRTM_Options_Define.f90:122-179) ---- */
@@ -163,6 +166,17 @@ typedef struct {
                                     src/CRTM_K_Matrix_Module.f90:594, 631).  Host memory in
                                     host-memory calls; device memory in device-memory calls
                                     (host memory there is staged, outside graph capture) */
+  int use_antenna_correction;    /* nonzero: Options%Use_Antenna_Correction = .TRUE. (default
+                                    .FALSE.): the brightness temperature of a sensor with antenna
+                                    correction data in its SpcCoeff file becomes the antenna
+                                    temperature (A_earth + A_platform) Tb + A_space TSPACE at the
+                                    profile's Geometry%iFOV (src/AntennaCorrection/
+                                    CRTM_AntennaCorrection.f90:118-140); clear-sky calls */
+  int no_nlte_correction;        /* nonzero: Options%Apply_NLTE_Correction = .FALSE.; 0 keeps the
+                                    reference default .TRUE.: sensors with non-LTE correction data
+                                    correct the radiance of their NLTE channels where the source
+                                    (sun) is above the horizon (src/NLTE/NLTE_Correction/
+                                    CRTM_NLTECorrection.f90); clear-sky calls */
 } crtm_options;
 
 /* ---- RTSolution (src/RTSolution/CRTM_RTSolution_Define.f90:201-245) ---- */

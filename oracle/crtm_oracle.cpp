@@ -33,6 +33,7 @@ namespace {
 
 typedef std::complex<double> cplx;
 
+constexpr double TSPACE = 2.7253;   // K, src/CRTM_Parameters.f90:173
 inline double re(double x) { return x; }
 inline double re(const cplx& z) { return z.real(); }
 inline double im(double) { return 0.0; }
@@ -227,7 +228,83 @@ struct ProfileState {
   std::vector<double> pdp, tz_ref, tzp_ref, gaz_ref, gazp_ref, gatzp_ref;
   std::vector<R> tz, tzp, gazp, gatzp;     // [Ko], [Ko], [Ko*Jm], [Ko*Jm]
   std::vector<R> int_vapor, ave_a, s_t, s_p, inverse, ap1;  // OPTRAN intermediates [Ko]
+  // non-LTE correction predictor (NLTE_Predictor_type): mean layer temperatures
+  // as predictors, their level ranges (k1, k2], bilinear angle weights
+  bool nl_active = false, nl_ctm = true;
+  int nl_k1[2] = {0, 0}, nl_k2[2] = {0, 0}, nl_isen = 0, nl_isol = 0;
+  R nl_pred[3] = {R(0.0), R(0.0), R(0.0)};
+  double nl_w[4] = {0.0, 0.0, 0.0, 0.0};
 };
+
+// Compute_NLTE_Predictor (src/NLTE/NLTE_Correction/CRTM_NLTECorrection.f90:95-260)
+template <class R>
+void nlte_predictor(const or_sensor* s, const or_profile* p, const R* t, double sec_sensor, ProfileState<R>& st) {
+  if (p->no_nlte_correction || s->nl_nsen == 0) return;
+  const double cosv = std::cos(D2R * p->source_zenith);
+  const double sec_src = cosv != 0.0 ? 1.0 / cosv : (cosv < 0.0 ? -1e308 : 1e308);
+  if (sec_src < 0.0) return;
+  const int K = p->n_layers;
+  const double* lp = p->level_pressure;
+  int k1 = 1 << 30, k2 = 1 << 30;
+  for (int i = 0; i < 2; ++i) {
+    R tm;
+    if (s->nl_upper[i] < lp[0] || s->nl_lower[i] > lp[K]) {
+      tm = R(s->nl_mean_tm[i]);
+      st.nl_ctm = false;
+    } else {
+      for (int k = 1; k <= K; ++k)
+        if (lp[k] > s->nl_upper[i]) { k1 = k - 1; break; }
+      for (int k = k1; k <= K; ++k)
+        if (lp[k] > s->nl_lower[i]) { k2 = k; break; }
+      tm = R(0.0);
+      for (int k = k1 + 1; k <= k2; ++k) tm = tm + t[k - 1];
+      tm = tm / double(k2 - k1);
+      st.nl_k1[i] = k1;
+      st.nl_k2[i] = k2;
+      st.nl_ctm = true;
+      if (re(tm) < s->nl_min_tm[i]) { tm = R(s->nl_min_tm[i]); st.nl_ctm = false; }
+      else if (re(tm) > s->nl_max_tm[i]) { tm = R(s->nl_max_tm[i]); st.nl_ctm = false; }
+    }
+    st.nl_pred[1 + i] = tm;
+  }
+  st.nl_pred[0] = R(1.0);
+  const int ns = s->nl_nsen, no = s->nl_nsol;
+  const double* X = s->nl_sec_sensor;
+  const double* Y = s->nl_sec_solar;
+  const double x = std::min(std::max(sec_sensor, X[0]), X[ns - 1]);
+  const double y = std::min(std::max(sec_src, Y[0]), Y[no - 1]);
+  int isen = 0, isol = 0;
+  for (int i = 0; i < ns - 1; ++i) if (x >= X[i] && x <= X[i + 1]) { isen = i; break; }
+  for (int i = 0; i < no - 1; ++i) if (y >= Y[i] && y <= Y[i + 1]) { isol = i; break; }
+  const double xx1 = (x - X[isen + 1]) / (X[isen] - X[isen + 1]), xx2 = (x - X[isen]) / (X[isen + 1] - X[isen]);
+  const double yy1 = (y - Y[isol + 1]) / (Y[isol] - Y[isol + 1]), yy2 = (y - Y[isol]) / (Y[isol + 1] - Y[isol]);
+  st.nl_w[0] = xx1 * yy1; st.nl_w[1] = xx1 * yy2; st.nl_w[2] = xx2 * yy1; st.nl_w[3] = xx2 * yy2;
+  st.nl_isen = isen;
+  st.nl_isol = isol;
+  st.nl_active = true;
+}
+
+// the channel's NLTE coefficients at the profile's angles (Compute_NLTE_Correction :535-540);
+// false when the correction does not apply to channel l
+template <class R>
+bool nlte_coeffs(const or_sensor* s, const ProfileState<R>& st, int l, double c[3]) {
+  if (!st.nl_active || s->nl_c_index[l] <= 0) return false;
+  const int ich = s->nl_c_index[l] - 1, ns = s->nl_nsen, no = s->nl_nsol, a = st.nl_isen, b = st.nl_isol;
+  auto C = [&](int i, int is, int io) { return s->nl_c[i + 3 * (is + ns * (io + no * ich))]; };
+  for (int i = 0; i < 3; ++i)
+    c[i] = st.nl_w[0] * C(i, a, b) + st.nl_w[1] * C(i, a, b + 1) + st.nl_w[2] * C(i, a + 1, b) +
+           st.nl_w[3] * C(i, a + 1, b + 1);
+  return true;
+}
+
+// (A_earth + A_platform, A_space) of channel l at the profile's field of view; false without the correction
+inline bool antenna_coeffs(const or_sensor* s, const or_profile* p, int l, double& aep, double& as) {
+  if (!p->use_antenna_correction || s->ac_nfov == 0 || p->ifov <= 0) return false;
+  const size_t i = (size_t)(p->ifov - 1) + (size_t)s->ac_nfov * l;
+  aep = s->ac_earth[i] + s->ac_platform[i];
+  as = s->ac_space[i];
+  return true;
+}
 
 inline int XI(int k, int j, int i, int J) { return (k * J + j) * MAXP + i; }
 
@@ -248,6 +325,7 @@ void map_and_predict(const or_sensor* s, const or_profile* p, const R* t_user, c
     sec_trans = sec_sensor;
   }
   st.sec_sfc = sec_trans;
+  nlte_predictor(s, p, t_user, sec_sensor, st);
   st.cos_sensor = 1.0 / sec_sensor;   // Common_RTSolution.f90:224
   // Map_Input (ODPS_CoordinateMapping.f90:151-319)
   std::vector<double> ref_lnp(Ko), user_lnp(K);
@@ -626,8 +704,16 @@ void channel_forward(const or_sensor* s, const or_profile* p, const R* t_user, c
     cf.up_radiance = cf.up_radiance * cf.tu[k - 1] + src;
   }
   cf.radiance = cf.ru[0];
-  // CRTM_Planck_Temperature (CRTM_Planck_Functions.f90:470-492)
+  // Post_Process_RTSolution (src/CRTM_Forward_Module.f90:1166-1199): the
+  // non-LTE correction of the radiance, the Planck temperature
+  // (CRTM_Planck_Functions.f90:470-492), the antenna correction
+  // (CRTM_Compute_AntCorr, src/AntennaCorrection/CRTM_AntennaCorrection.f90:118-140)
+  double nc[3];
+  if (nlte_coeffs(s, st, l, nc))
+    for (int i = 0; i < 3; ++i) cf.radiance = cf.radiance + nc[i] * st.nl_pred[i];
   cf.bt = (c2 / std::log(c1 / cf.radiance + 1.0) - bc1) / bc2;
+  double aep, as;
+  if (antenna_coeffs(s, p, l, aep, as)) cf.bt = aep * cf.bt + as * TSPACE;
 }
 
 double sfc_temperature(const or_profile* p) {
@@ -1113,7 +1199,11 @@ void k_matrix_profile(const or_sensor* s, const or_profile* p, const double* emi
     if (o->up_radiance) o->up_radiance[l] = cf.up_radiance;
     if (o->down_radiance) o->down_radiance[l] = cf.rd[K];
     // Pre_Process_RTSolution_K (src/CRTM_K_Matrix_Module.f90:1725-1761)
-    double rad_ad = ext ? sd->rad_ad_clear : rad_k[l] + planck_t_ad_factor(s, l, cf.radiance) * bt_k[l];
+    // CRTM_Compute_AntCorr_AD scales the BT adjoint; the Planck temperature
+    // adjoint is taken at the NLTE-corrected radiance
+    double aep = 1.0, as_ = 0.0;
+    antenna_coeffs(s, p, l, aep, as_);
+    double rad_ad = ext ? sd->rad_ad_clear : rad_k[l] + planck_t_ad_factor(s, l, cf.radiance) * aep * bt_k[l];
     if (o->radiance_k) o->radiance_k[l] = ext ? sd->rad_k : rad_ad;
     // CRTM_Emission_AD (Emission_Module.f90:280-395)
     double u = st.cos_sensor, sec_down = 1.0 / u, eps = cf.eps, rho = cf.rho;
@@ -1170,6 +1260,17 @@ void k_matrix_profile(const or_sensor* s, const or_profile* p, const double* emi
     if (tk)
       for (int k = 0; k < K; ++k)
         tk[k] = planck_r_ad_factor(s, l, p->temperature[k]) * w.b_ad[k] + (ext ? sd->t_ad[k] : 0.0);
+    // Compute_NLTE_Correction_AD / Compute_NLTE_Predictor_AD
+    // (CRTM_NLTECorrection.f90:396-446, 704-744): the mean temperature's
+    // adjoint spread over the levels (k1, k2] of each layer
+    double nc[3];
+    if (tk && !ext && st.nl_ctm && nlte_coeffs(s, st, l, nc))
+      for (int i = 0; i < 2; ++i) {
+        const int k1 = st.nl_k1[i], k2 = st.nl_k2[i];
+        if (k2 <= k1) continue;
+        const double g = nc[1 + i] * rad_ad / double(k2 - k1);
+        for (int k = k1; k < k2; ++k) tk[k] += g;
+      }
     if (ak) for (int i = 0; i < na * K; ++i) ak[i] = 0.0;
     double sfct_ad = planck_r_ad_factor(s, l, tsfc) * bs_ad;
     if (o->sfc_t_k)

@@ -44,6 +44,17 @@ typedef struct {
   const double* cosmic_background;
   const double* frequency;           /* GHz [L] (microwave cloud tables) */
   const double* wavenumber;          /* cm^-1 [L] (infrared tables, stream choice) */
+  /* antenna correction (src/Coefficients/ACCoeff/ACCoeff_Define.f90:78-96), (n_fov, L)
+   * column-major; ac_nfov = 0: absent */
+  int ac_nfov;
+  const double *ac_earth, *ac_space, *ac_platform;
+  /* non-LTE correction (src/Coefficients/NLTECoeff/NLTECoeff_Define.f90:87-121); nl_nsen = 0: absent */
+  int nl_nsen, nl_nsol, nl_nch;
+  const double* nl_sec_sensor;       /* [nsen] */
+  const double* nl_sec_solar;        /* [nsol] */
+  const int* nl_c_index;             /* [L] 1-based, 0 = not an NLTE channel */
+  const double* nl_c;                /* (3, nsen, nsol, nch) column-major */
+  double nl_upper[2], nl_lower[2], nl_min_tm[2], nl_max_tm[2], nl_mean_tm[2];
 } or_sensor;
 
 typedef struct {
@@ -57,6 +68,10 @@ typedef struct {
   double sfc_temperature[4];
   double sensor_zenith, sensor_scan, surface_altitude;
   int climatology;              /* 1..6 (src/Atmosphere/CRTM_Atmosphere_Define.f90:406-411), 0 = US standard */
+  double source_zenith;         /* degrees (non-LTE correction; >= 90: no source) */
+  int ifov;                     /* Geometry%iFOV of the antenna correction (0: none) */
+  int use_antenna_correction;   /* Options%Use_Antenna_Correction */
+  int no_nlte_correction;       /* Options%Apply_NLTE_Correction = .FALSE. */
 } or_profile;
 
 typedef struct {

@@ -32,7 +32,13 @@ class OrSensor(ctypes.Structure):
                 ("alpha_c2", ctypes.c_double), ("ocomponent_index", ctypes.c_int),
                 ("sensor_type", ctypes.c_int), ("planck_c1", _dp), ("planck_c2", _dp),
                 ("band_c1", _dp), ("band_c2", _dp), ("cosmic_background", _dp),
-                ("frequency", _dp), ("wavenumber", _dp)]
+                ("frequency", _dp), ("wavenumber", _dp),
+                ("ac_nfov", ctypes.c_int), ("ac_earth", _dp), ("ac_space", _dp), ("ac_platform", _dp),
+                ("nl_nsen", ctypes.c_int), ("nl_nsol", ctypes.c_int), ("nl_nch", ctypes.c_int),
+                ("nl_sec_sensor", _dp), ("nl_sec_solar", _dp), ("nl_c_index", _ip), ("nl_c", _dp),
+                ("nl_upper", ctypes.c_double * 2), ("nl_lower", ctypes.c_double * 2),
+                ("nl_min_tm", ctypes.c_double * 2), ("nl_max_tm", ctypes.c_double * 2),
+                ("nl_mean_tm", ctypes.c_double * 2)]
 
 
 class OrAerosolCoeff(ctypes.Structure):
@@ -75,7 +81,9 @@ class OrProfile(ctypes.Structure):
                 ("level_pressure", _dp), ("pressure", _dp), ("temperature", _dp), ("absorber", _dp),
                 ("coverage", ctypes.c_double * 4), ("sfc_temperature", ctypes.c_double * 4),
                 ("sensor_zenith", ctypes.c_double), ("sensor_scan", ctypes.c_double),
-                ("surface_altitude", ctypes.c_double), ("climatology", ctypes.c_int)]
+                ("surface_altitude", ctypes.c_double), ("climatology", ctypes.c_int),
+                ("source_zenith", ctypes.c_double), ("ifov", ctypes.c_int),
+                ("use_antenna_correction", ctypes.c_int), ("no_nlte_correction", ctypes.c_int)]
 
 
 class OrMWwater(ctypes.Structure):
@@ -228,11 +236,26 @@ class Sensor:
         s.cosmic_background = _p(keep(spc.cosmic_background_radiance, np.float64), _dp)
         s.frequency = _p(keep(spc.frequency, np.float64), _dp)
         s.wavenumber = _p(keep(spc.wavenumber, np.float64), _dp)
+        ac = getattr(spc, "ac", None)
+        if ac is not None:   # antenna correction, (n_fov, L) column-major
+            s.ac_nfov = ac.n_fovs
+            s.ac_earth = _p(keep(ac.a_earth, np.float64), _dp)
+            s.ac_space = _p(keep(ac.a_space, np.float64), _dp)
+            s.ac_platform = _p(keep(ac.a_platform, np.float64), _dp)
+        nc = getattr(spc, "nc", None)
+        if nc is not None:   # non-LTE correction
+            s.nl_nsen, s.nl_nsol, s.nl_nch = nc.c.shape[1], nc.c.shape[2], nc.c.shape[3]
+            s.nl_sec_sensor = _p(keep(nc.secant_sensor_zenith, np.float64), _dp)
+            s.nl_sec_solar = _p(keep(nc.secant_solar_zenith, np.float64), _dp)
+            s.nl_c_index = _p(keep(nc.c_index, np.int32), _ip)
+            s.nl_c = _p(keep(nc.c if nc.c.size else np.zeros(1), np.float64), _dp)
+            s.nl_upper[:], s.nl_lower[:] = list(nc.upper_plevel), list(nc.lower_plevel)
+            s.nl_min_tm[:], s.nl_max_tm[:], s.nl_mean_tm[:] = list(nc.min_tm), list(nc.max_tm), list(nc.mean_tm)
         self.s = s
         self.n_channels = tau.n_channels
 
 
-def make_profile(ps, m, keep):
+def make_profile(ps, m, keep, use_antenna_correction=False, apply_nlte_correction=True):
     pr = ps.profile(m)
     p = OrProfile()
     K = pr["temperature"].size
@@ -254,13 +277,20 @@ def make_profile(ps, m, keep):
     p.surface_altitude = 0.0
     clim = getattr(ps, "climatology", None)
     p.climatology = int(clim[m]) if clim is not None else 0
+    sz = getattr(ps, "source_zenith", None)
+    p.source_zenith = float(sz[m]) if sz is not None else 100.0
+    ifov = getattr(ps, "ifov", None)
+    p.ifov = int(ifov[m]) if ifov is not None else 0
+    p.use_antenna_correction = int(bool(use_antenna_correction))
+    p.no_nlte_correction = 0 if apply_nlte_correction else 1
     return p
 
 
-def run_profile(sensor: Sensor, ps, m, emissivity, mode="forward", bt_k=None, rad_k=None):
-    """mode: 'forward', 'k' (reverse adjoint) or 'cstep' (complex-step)."""
+def run_profile(sensor: Sensor, ps, m, emissivity, mode="forward", bt_k=None, rad_k=None, **opts):
+    """mode: 'forward', 'k' (reverse adjoint) or 'cstep' (complex-step); opts:
+    use_antenna_correction, apply_nlte_correction (Options)."""
     keep = []
-    p = make_profile(ps, m, keep)
+    p = make_profile(ps, m, keep, **opts)
     L = sensor.n_channels
     K = p.n_layers
     na = p.n_absorbers
@@ -284,12 +314,12 @@ def run_profile(sensor: Sensor, ps, m, emissivity, mode="forward", bt_k=None, ra
     return out
 
 
-def k_matrix_batch(sensor: Sensor, ps, emissivity, n_threads=0, bt_k=None, rad_k=None):
+def k_matrix_batch(sensor: Sensor, ps, emissivity, n_threads=0, bt_k=None, rad_k=None, **opts):
     """All profiles of a ProfileSet; returns dict of outputs (t_k layer-concatenated)."""
     M = ps.n_profiles
     L = sensor.n_channels
     keep = []
-    profs = (OrProfile * M)(*[make_profile(ps, m, keep) for m in range(M)])
+    profs = (OrProfile * M)(*[make_profile(ps, m, keep, **opts) for m in range(M)])
     na = ps.absorber.shape[0]
     ntot = int(ps.layer_offset[-1])
     out = {"radiance": np.zeros((M, L)), "bt": np.zeros((M, L)), "t_k": np.zeros(ntot * L),

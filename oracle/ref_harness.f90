@@ -44,6 +44,8 @@ MODULE ref_harness
   USE AerosolCoeff_Binary_IO, ONLY: AerosolCoeff_Binary_ReadFile
   USE SpcCoeff_Define, ONLY: SpcCoeff_type, SpcCoeff_Destroy
   USE SpcCoeff_Binary_IO, ONLY: SpcCoeff_Binary_ReadFile
+  USE ACCoeff_Define, ONLY: ACCoeff_Associated
+  USE NLTECoeff_Define, ONLY: NLTECoeff_Associated
   USE CRTM_AtmOptics, ONLY: CRTM_Compute_Transmittance, CRTM_Compute_Transmittance_AD
   USE MWwaterCoeff_Define, ONLY: MWwaterCoeff_type, MWwaterCoeff_Destroy, MWwaterCoeff_ReadFile
   USE CRTM_FastemX, ONLY: FastemX_iVar_type => iVar_type, Compute_FastemX, Compute_FastemX_AD
@@ -347,6 +349,44 @@ CONTAINS
     cosmic(1:L) = scf%Cosmic_Background_Radiance
     solar(1:L) = scf%Solar_Irradiance
   END SUBROUTINE ref_spccoeff_fetch
+
+  ! Antenna-correction block of the loaded SpcCoeff (SpcCoeff%AC, ACCoeff_type):
+  ! A_earth, A_space, A_platform (n_FOVs, n_Channels) column-major; returns
+  ! n_FOVs, 0 when the file has no such block
+  INTEGER(C_INT) FUNCTION ref_spccoeff_ac(ae, asp, ap) BIND(C, NAME='ref_spccoeff_ac')
+    REAL(C_DOUBLE), INTENT(OUT) :: ae(*), asp(*), ap(*)
+    INTEGER :: n
+    ref_spccoeff_ac = 0
+    IF (.NOT. ACCoeff_Associated(scf%AC)) RETURN
+    n = SIZE(scf%AC%A_earth)
+    ae(1:n) = RESHAPE(scf%AC%A_earth, (/n/))
+    asp(1:n) = RESHAPE(scf%AC%A_space, (/n/))
+    ap(1:n) = RESHAPE(scf%AC%A_platform, (/n/))
+    ref_spccoeff_ac = scf%AC%n_FOVs
+  END FUNCTION ref_spccoeff_ac
+
+  ! Non-LTE block of the loaded SpcCoeff (SpcCoeff%NC, NLTECoeff_type): dims =
+  ! n_Predictors, n_Sensor_Angles, n_Solar_Angles, n_NLTE_Channels; lev =
+  ! Upper_Plevel(2), Lower_Plevel(2); tm = Min_Tm(2), Max_Tm(2), Mean_Tm(2);
+  ! returns 1, 0 when the file has no such block
+  INTEGER(C_INT) FUNCTION ref_spccoeff_nc(dims, lev, tm, ssen, ssol, cidx, isnlte, c) &
+      BIND(C, NAME='ref_spccoeff_nc')
+    INTEGER(C_INT), INTENT(OUT) :: dims(4), cidx(*), isnlte(*)
+    REAL(C_DOUBLE), INTENT(OUT) :: lev(4), tm(6), ssen(*), ssol(*), c(*)
+    INTEGER :: n
+    ref_spccoeff_nc = 0
+    IF (.NOT. NLTECoeff_Associated(scf%NC)) RETURN
+    dims = (/ scf%NC%n_Predictors, scf%NC%n_Sensor_Angles, scf%NC%n_Solar_Angles, scf%NC%n_NLTE_Channels /)
+    lev = (/ scf%NC%Upper_Plevel, scf%NC%Lower_Plevel /)
+    tm = (/ scf%NC%Min_Tm, scf%NC%Max_Tm, scf%NC%Mean_Tm /)
+    ssen(1:dims(2)) = scf%NC%Secant_Sensor_Zenith
+    ssol(1:dims(3)) = scf%NC%Secant_Solar_Zenith
+    cidx(1:scf%NC%n_Channels) = scf%NC%C_Index
+    isnlte(1:scf%NC%n_Channels) = MERGE(1, 0, scf%NC%Is_NLTE_Channel)
+    n = SIZE(scf%NC%C)
+    c(1:n) = RESHAPE(scf%NC%C, (/n/))
+    ref_spccoeff_nc = 1
+  END FUNCTION ref_spccoeff_nc
 
   ! MWwaterCoeff_ReadFile (FASTEM coefficient file)
   INTEGER(C_INT) FUNCTION ref_mwwater_load(path, n) BIND(C, NAME='ref_mwwater_load')
