@@ -479,4 +479,94 @@ CONTAINS
     od_ad = ao_ad%Optical_Depth
   END SUBROUTINE ref_transmittance
 
+  ! Geometry files through the reference writer and reader
+  ! (CRTM_Geometry_WriteFile / CRTM_Geometry_ReadFile).  Per profile:
+  ! ints(1:4) = iFOV, Year, Month, Day; reals(1:9) = Longitude, Latitude,
+  ! Surface_Altitude, Sensor_Scan_Angle, Sensor_Zenith_Angle,
+  ! Sensor_Azimuth_Angle, Source_Zenith_Angle, Source_Azimuth_Angle,
+  ! Flux_Zenith_Angle.
+  INTEGER(C_INT) FUNCTION ref_geometry_write(path, n, m, ints, reals) BIND(C, NAME='ref_geometry_write')
+    USE CRTM_Geometry_Define, ONLY: CRTM_Geometry_WriteFile
+    CHARACTER(KIND=C_CHAR), INTENT(IN) :: path(*)
+    INTEGER(C_INT), VALUE :: n, m
+    INTEGER(C_INT), INTENT(IN) :: ints(4, m)
+    REAL(C_DOUBLE), INTENT(IN) :: reals(9, m)
+    TYPE(CRTM_Geometry_type) :: g(m)
+    INTEGER :: i
+    DO i = 1, m
+      g(i)%iFOV = ints(1, i); g(i)%Year = ints(2, i); g(i)%Month = ints(3, i); g(i)%Day = ints(4, i)
+      g(i)%Longitude = reals(1, i); g(i)%Latitude = reals(2, i); g(i)%Surface_Altitude = reals(3, i)
+      g(i)%Sensor_Scan_Angle = reals(4, i); g(i)%Sensor_Zenith_Angle = reals(5, i)
+      g(i)%Sensor_Azimuth_Angle = reals(6, i); g(i)%Source_Zenith_Angle = reals(7, i)
+      g(i)%Source_Azimuth_Angle = reals(8, i); g(i)%Flux_Zenith_Angle = reals(9, i)
+    END DO
+    ref_geometry_write = CRTM_Geometry_WriteFile(c2f(path, n), g, Quiet=.TRUE.)
+  END FUNCTION ref_geometry_write
+
+  ! Reads up to mcap profiles; mout = the file's profile count
+  INTEGER(C_INT) FUNCTION ref_geometry_read(path, n, mcap, mout, ints, reals) BIND(C, NAME='ref_geometry_read')
+    USE CRTM_Geometry_Define, ONLY: CRTM_Geometry_ReadFile
+    CHARACTER(KIND=C_CHAR), INTENT(IN) :: path(*)
+    INTEGER(C_INT), VALUE :: n, mcap
+    INTEGER(C_INT), INTENT(OUT) :: mout
+    INTEGER(C_INT), INTENT(OUT) :: ints(4, mcap)
+    REAL(C_DOUBLE), INTENT(OUT) :: reals(9, mcap)
+    TYPE(CRTM_Geometry_type), ALLOCATABLE :: g(:)
+    INTEGER :: i
+    ref_geometry_read = CRTM_Geometry_ReadFile(c2f(path, n), g, Quiet=.TRUE.)
+    mout = 0
+    IF (ref_geometry_read /= 0) RETURN
+    mout = SIZE(g)
+    DO i = 1, MIN(mout, mcap)
+      ints(:, i) = (/ g(i)%iFOV, g(i)%Year, g(i)%Month, g(i)%Day /)
+      reals(:, i) = (/ g(i)%Longitude, g(i)%Latitude, g(i)%Surface_Altitude, g(i)%Sensor_Scan_Angle, &
+                       g(i)%Sensor_Zenith_Angle, g(i)%Sensor_Azimuth_Angle, g(i)%Source_Zenith_Angle, &
+                       g(i)%Source_Azimuth_Angle, g(i)%Flux_Zenith_Angle /)
+    END DO
+  END FUNCTION ref_geometry_read
+
+  ! Cloud files through CRTM_Cloud_WriteFile / CRTM_Cloud_ReadFile: nc clouds
+  ! of k layers each; reff, rvar, wc (k, nc)
+  INTEGER(C_INT) FUNCTION ref_cloud_write(path, n, nc, k, types, reff, rvar, wc) BIND(C, NAME='ref_cloud_write')
+    USE CRTM_Cloud_Define, ONLY: CRTM_Cloud_type, CRTM_Cloud_Create, CRTM_Cloud_WriteFile
+    CHARACTER(KIND=C_CHAR), INTENT(IN) :: path(*)
+    INTEGER(C_INT), VALUE :: n, nc, k
+    INTEGER(C_INT), INTENT(IN) :: types(nc)
+    REAL(C_DOUBLE), INTENT(IN) :: reff(k, nc), rvar(k, nc), wc(k, nc)
+    TYPE(CRTM_Cloud_type) :: c(nc)
+    INTEGER :: i
+    DO i = 1, nc
+      CALL CRTM_Cloud_Create(c(i), k)
+      c(i)%Type = types(i)
+      c(i)%Effective_Radius = reff(:, i)
+      c(i)%Effective_Variance = rvar(:, i)
+      c(i)%Water_Content = wc(:, i)
+    END DO
+    ref_cloud_write = CRTM_Cloud_WriteFile(c2f(path, n), c, Quiet=.TRUE.)
+  END FUNCTION ref_cloud_write
+
+  INTEGER(C_INT) FUNCTION ref_cloud_read(path, n, nccap, kcap, ncout, kout, types, reff, rvar, wc) &
+      BIND(C, NAME='ref_cloud_read')
+    USE CRTM_Cloud_Define, ONLY: CRTM_Cloud_type, CRTM_Cloud_ReadFile
+    CHARACTER(KIND=C_CHAR), INTENT(IN) :: path(*)
+    INTEGER(C_INT), VALUE :: n, nccap, kcap
+    INTEGER(C_INT), INTENT(OUT) :: ncout, kout
+    INTEGER(C_INT), INTENT(OUT) :: types(nccap)
+    REAL(C_DOUBLE), INTENT(OUT) :: reff(kcap, nccap), rvar(kcap, nccap), wc(kcap, nccap)
+    TYPE(CRTM_Cloud_type), ALLOCATABLE :: c(:)
+    INTEGER :: i, kk
+    ref_cloud_read = CRTM_Cloud_ReadFile(c2f(path, n), c, Quiet=.TRUE.)
+    ncout = 0; kout = 0
+    IF (ref_cloud_read /= 0) RETURN
+    ncout = SIZE(c)
+    DO i = 1, MIN(ncout, nccap)
+      kk = MIN(c(i)%n_Layers, kcap)
+      kout = c(i)%n_Layers
+      types(i) = c(i)%Type
+      reff(1:kk, i) = c(i)%Effective_Radius(1:kk)
+      rvar(1:kk, i) = c(i)%Effective_Variance(1:kk)
+      wc(1:kk, i) = c(i)%Water_Content(1:kk)
+    END DO
+  END FUNCTION ref_cloud_read
+
 END MODULE ref_harness
